@@ -1,0 +1,223 @@
+/*
+ * lrt_hip.h -- the C ABI of the MI355X Gaussian-splat tile rasteriser.
+ *
+ * The reference has no C ABI: its rasteriser is the C++ class
+ * lrt::render::TileRasterizer over slang-rhi. Each entry point below names the
+ * reference interface it replaces (path:line in lucabRTrender). All pointers
+ * called "device" are HIP device pointers the caller owns (hipMalloc, or a
+ * torch tensor's data_ptr()); "host" pointers are ordinary memory. No call
+ * takes a torch type. Every call that can fail returns an lrt_status; the
+ * message of the last failure on a context is lrt_last_error(ctx).
+ *
+ * Threading: one context is one caller and one HIP stream, as the reference's
+ * device has one caller (CLAUDE.md:110-112). Calls on one context must not
+ * overlap; different contexts are independent.
+ */
+#ifndef LRT_HIP_H
+#define LRT_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define LRT_ABI_VERSION 1
+
+/* lrt::ErrorCode (modules/core/include/lrt/core/Error.h:14-26), shifted so
+ * that 0 is success. */
+typedef enum {
+    LRT_OK = 0,
+    LRT_UNKNOWN = 1,
+    LRT_NOT_FOUND = 2,
+    LRT_INVALID_ARGUMENT = 3,
+    LRT_UNSUPPORTED = 4,
+    LRT_IO_FAILURE = 5,
+    LRT_DEVICE_FAILURE = 6,
+    LRT_SHADER_FAILURE = 7,
+    LRT_PLUGIN_FAILURE = 8,
+    LRT_OUT_OF_MEMORY = 9,
+    LRT_CANCELLED = 10,
+    LRT_INTERNAL_ERROR = 11
+} lrt_status;
+
+typedef struct lrt_context lrt_context;
+
+/* ---- host math: cameras (no device needed) ------------------------------ */
+
+/* lrt::render::Lens (modules/render/include/lrt/render/Camera.h:73-94), the
+ * fields the rasteriser reads. lrt_lens_default fills the reference's defaults. */
+typedef struct {
+    int32_t orthographic;
+    double focal;            /* mm */
+    double haperture;        /* mm */
+    double vaperture;        /* mm */
+    double near_z;
+    double far_z;
+    double window_translate[2];
+    double window_scale[2];
+    double window_roll;      /* degrees */
+} lrt_lens;
+
+/* lrt::render::Camera (Camera.h:96-103): object-to-world, row-major 4x4. */
+typedef struct {
+    double camera_to_world[16];
+    lrt_lens lens;
+} lrt_camera;
+
+/* lrt::render::Projection (Camera.h:106-115), the fields the rasteriser reads. */
+typedef struct {
+    double world_to_view[16];
+    double focal_x, focal_y, centre_x, centre_y, near_z, far_z;
+    int32_t orthographic;
+    double eye_world[3];
+} lrt_projection;
+
+void lrt_lens_default(lrt_lens* out);
+/* Camera::lookingAt (modules/render/src/Camera.cpp:9-13, aofx Transform.h:376-400). */
+void lrt_camera_looking_at(const double eye[3], const double target[3], const double up[3], lrt_camera* out);
+/* projectionFor (modules/render/src/Camera.cpp:20-48). */
+lrt_status lrt_projection_for(const lrt_camera* camera, uint32_t width, uint32_t height, lrt_projection* out);
+
+/* ---- the scene ------------------------------------------------------------ */
+
+/* scene::GpuSplats (modules/scene/include/lrt/scene/GpuClouds.h:41-76), packed as
+ * shaders/lrt/common/packing.slang:5-11 lays it out. Device pointers.
+ *   positions  float4 a splat: x, y, z, linear opacity
+ *   shape      4 u32 a splat: smallest-three rotation; f16 ln sx | ln sy;
+ *              f16 ln sz | base r; f16 base g | base b
+ *   sh         sh_words u32 a splat: f16 rest coefficients, rgb per basis
+ *              (may be NULL when rest_per_colour is 0)
+ *   filter3d   float a splat: Mip-Splatting 3D filter std (NULL: none); read
+ *              only when lrt_render_settings.filter3d is set. Not in the reference. */
+typedef struct {
+    const float* positions;
+    const uint32_t* shape;
+    const uint32_t* sh;
+    const float* filter3d;
+    uint32_t count;
+    uint32_t rest_per_colour;   /* 0, 3, 8, 15 */
+    uint32_t sh_words;          /* (rest_per_colour * 3 + 1) / 2, 1 at degree 0 */
+} lrt_splats;
+
+/* render::SplatEdit (modules/render/include/lrt/render/TileRasterizer.h:158-179). */
+typedef struct {
+    int32_t active;
+    uint32_t shape;      /* 0 box, 1 sphere */
+    uint32_t mode;       /* 0 keep inside, 1 remove inside, 2 grade inside */
+    int32_t invert;
+    float centre[3];
+    float size[3];
+    float tint[3];
+    float saturation, brightness, opacity, min_opacity, max_scale;
+} lrt_splat_edit;
+
+/* render::SplatInstance (TileRasterizer.h:181-196), without relighting. */
+typedef struct {
+    const lrt_splats* splats;
+    double object_to_world[16];
+    lrt_splat_edit edit;
+} lrt_splat_instance;
+
+/* render::RenderSettings (TileRasterizer.h:234-247) plus two filter settings the
+ * reference fixes: dilation (its +0.3 px^2) and filter3d (off). */
+typedef struct {
+    uint32_t width, height;
+    int32_t antialias;
+    uint32_t max_sh_degree;
+    float background[4];          /* premultiplied, linear */
+    int32_t depth_mode;           /* 0 mean, 1 threshold */
+    float depth_threshold;
+    int32_t linearise;
+    int32_t time_stages;          /* per-stage HIP event timing into lrt_frame_stats */
+    float dilation;               /* 0.3 reproduces the reference */
+    int32_t filter3d;             /* 1: read lrt_splats.filter3d */
+} lrt_render_settings;
+
+/* render::RenderTargets (TileRasterizer.h:257-263): device images, bottom row
+ * first, width * height each. colour: float4 premultiplied linear; depth:
+ * float view z (may be NULL). The caller allocates them at the settings' size. */
+typedef struct {
+    uint32_t width, height;
+    float* colour;
+    float* depth;
+} lrt_render_targets;
+
+/* render::FrameStats (TileRasterizer.h:249-255). depth_sort_ms is 0: one
+ * 64-bit (tile, depth) sort replaces the reference's depth sort + tile sort
+ * and is reported as tile_sort_ms. */
+typedef struct {
+    uint32_t splats, visible, pairs;
+    double project_ms, depth_sort_ms, counts_ms, emit_ms, tile_sort_ms, blend_ms, total_ms;
+} lrt_frame_stats;
+
+void lrt_render_settings_default(lrt_render_settings* out);
+void lrt_splat_edit_default(lrt_splat_edit* out);
+
+/* ---- the rasteriser ---------------------------------------------------------- */
+
+/* TileRasterizer::create (TileRasterizer.h:138, TileRasterizer.cpp:59-103).
+ * device: HIP device ordinal; stream: a hipStream_t to queue on (NULL: the
+ * null stream). Nothing is compiled at run time. */
+lrt_status lrt_context_create(int device, void* stream, lrt_context** out);
+void lrt_context_destroy(lrt_context* ctx);
+lrt_status lrt_context_set_stream(lrt_context* ctx, void* stream);
+/* The message of the context's last failure; with NULL, that of the calling
+ * thread's last failed lrt_context_create. */
+const char* lrt_last_error(const lrt_context* ctx);
+
+/* TileRasterizer::render(const Projection&, ...) (TileRasterizer.h:143-155,
+ * TileRasterizer.cpp:188-484), splat instances only. Synchronous, as the
+ * reference's: returns once the images are written. stats may be NULL. */
+lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
+                             const lrt_splat_instance* instances, uint32_t instance_count,
+                             const lrt_render_settings* settings, lrt_render_targets* targets,
+                             lrt_frame_stats* stats);
+
+/* What the last render left on the device, copied to host, for bit-exact
+ * checks. *needed (may be NULL) receives the byte size; bytes may be 0 to ask. */
+typedef enum {
+    LRT_DEBUG_COUNTS = 0,        /* u32[4]: splats, visible, pairs, tiles */
+    LRT_DEBUG_TOUCHED = 1,       /* u32 a slot: tiles touched (0: culled) */
+    LRT_DEBUG_OFFSETS = 2,       /* u32 a slot: exclusive scan of TOUCHED */
+    LRT_DEBUG_PROJ = 3,          /* 12 floats a slot: screen, conic, colour */
+    LRT_DEBUG_KEYS = 4,          /* u64 a pair, sorted: tile << 32 | bits(view z) */
+    LRT_DEBUG_VALUES = 5,        /* u32 a pair, sorted: slot */
+    LRT_DEBUG_RANGES = 6,        /* u32[2] a tile: [start, end) */
+    LRT_DEBUG_LAST_COUNT = 7,    /* u32 a pixel: pairs walked to the last contributor */
+    LRT_DEBUG_FINAL_T = 8,       /* float a pixel: final transmittance */
+    LRT_DEBUG_RECTS = 9          /* u32[2] a slot: x0 | y0 << 16, x1 | y1 << 16 */
+} lrt_debug_buffer;
+lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed);
+
+/* The kernel constant block one instance would be drawn with (host only):
+ * FrameParams of lucabrtrender_amd/csrc/frame.h, the reference's setFrame +
+ * setObject (modules/render/src/FrameParams.h:32-69, TileRasterizer.cpp:297-314). */
+lrt_status lrt_frame_params(const lrt_projection* projection, const lrt_splat_instance* instance,
+                            const lrt_render_settings* settings, uint32_t base, void* out, size_t bytes,
+                            size_t* needed);
+
+/* ---- the algorithms -------------------------------------------------------- */
+
+/* gpu::PrefixSum::apply (modules/gpu/include/lrt/gpu/algo/PrefixSum.h:43-44):
+ * out[i] = in[0] + ... + in[i-1] in u32, *total = the sum. Device pointers;
+ * in and out must differ. Queued on the context's stream, not waited for. */
+lrt_status lrt_prefix_sum(lrt_context* ctx, const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count);
+
+/* gpu::RadixSort::sort (modules/gpu/include/lrt/gpu/algo/RadixSort.h:56-57):
+ * stable sort of count (key, value) pairs by the key's low key_bits bits
+ * (1..64), eight bits a pass. Keys are u64 here (the reference splits them in
+ * keysLo/keysHi). scratch_* are same-sized buffers the passes alternate with;
+ * the result is left in keys/values, or in scratch_* when *result_in_scratch is
+ * set (an odd pass count; the reference copies back instead). Queued, not waited. */
+lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, uint64_t* scratch_keys,
+                          uint32_t* scratch_values, uint32_t count, uint32_t key_bits, int32_t* result_in_scratch);
+
+/* Waits for everything queued on the context's stream. */
+lrt_status lrt_context_synchronize(lrt_context* ctx);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

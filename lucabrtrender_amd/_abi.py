@@ -1,0 +1,167 @@
+"""ctypes mirror of include/lrt_hip.h and include/lrt_synth.h.
+
+Loads ``liblrt_hip.so`` from this package directory. There is no fallback: a
+missing or stale library raises ImportError with the build command.
+"""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+_HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(_HERE, "liblrt_hip.so")
+
+LRT_OK = 0
+STATUS_NAMES = {
+    0: "Ok", 1: "Unknown", 2: "NotFound", 3: "InvalidArgument", 4: "Unsupported", 5: "IoFailure",
+    6: "DeviceFailure", 7: "ShaderFailure", 8: "PluginFailure", 9: "OutOfMemory", 10: "Cancelled",
+    11: "InternalError",
+}
+
+DEBUG_COUNTS, DEBUG_TOUCHED, DEBUG_OFFSETS, DEBUG_PROJ, DEBUG_KEYS, DEBUG_VALUES, DEBUG_RANGES, \
+    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS = range(10)
+
+
+class Lens(C.Structure):
+    _fields_ = [
+        ("orthographic", C.c_int32),
+        ("focal", C.c_double),
+        ("haperture", C.c_double),
+        ("vaperture", C.c_double),
+        ("near_z", C.c_double),
+        ("far_z", C.c_double),
+        ("window_translate", C.c_double * 2),
+        ("window_scale", C.c_double * 2),
+        ("window_roll", C.c_double),
+    ]
+
+
+class CameraC(C.Structure):
+    _fields_ = [("camera_to_world", C.c_double * 16), ("lens", Lens)]
+
+
+class ProjectionC(C.Structure):
+    _fields_ = [
+        ("world_to_view", C.c_double * 16),
+        ("focal_x", C.c_double), ("focal_y", C.c_double),
+        ("centre_x", C.c_double), ("centre_y", C.c_double),
+        ("near_z", C.c_double), ("far_z", C.c_double),
+        ("orthographic", C.c_int32),
+        ("eye_world", C.c_double * 3),
+    ]
+
+
+class Splats(C.Structure):
+    _fields_ = [
+        ("positions", C.c_void_p),
+        ("shape", C.c_void_p),
+        ("sh", C.c_void_p),
+        ("filter3d", C.c_void_p),
+        ("count", C.c_uint32),
+        ("rest_per_colour", C.c_uint32),
+        ("sh_words", C.c_uint32),
+    ]
+
+
+class SplatEdit(C.Structure):
+    _fields_ = [
+        ("active", C.c_int32), ("shape", C.c_uint32), ("mode", C.c_uint32), ("invert", C.c_int32),
+        ("centre", C.c_float * 3), ("size", C.c_float * 3), ("tint", C.c_float * 3),
+        ("saturation", C.c_float), ("brightness", C.c_float), ("opacity", C.c_float),
+        ("min_opacity", C.c_float), ("max_scale", C.c_float),
+    ]
+
+
+class SplatInstanceC(C.Structure):
+    _fields_ = [("splats", C.POINTER(Splats)), ("object_to_world", C.c_double * 16), ("edit", SplatEdit)]
+
+
+class RenderSettingsC(C.Structure):
+    _fields_ = [
+        ("width", C.c_uint32), ("height", C.c_uint32),
+        ("antialias", C.c_int32), ("max_sh_degree", C.c_uint32),
+        ("background", C.c_float * 4),
+        ("depth_mode", C.c_int32), ("depth_threshold", C.c_float),
+        ("linearise", C.c_int32), ("time_stages", C.c_int32),
+        ("dilation", C.c_float), ("filter3d", C.c_int32),
+    ]
+
+
+class RenderTargetsC(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("colour", C.c_void_p), ("depth", C.c_void_p)]
+
+
+class FrameStatsC(C.Structure):
+    _fields_ = [
+        ("splats", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32),
+        ("project_ms", C.c_double), ("depth_sort_ms", C.c_double), ("counts_ms", C.c_double),
+        ("emit_ms", C.c_double), ("tile_sort_ms", C.c_double), ("blend_ms", C.c_double),
+        ("total_ms", C.c_double),
+    ]
+
+
+class SceneSpecC(C.Structure):
+    _fields_ = [
+        ("kind", C.c_uint32), ("degree", C.c_uint32), ("seed", C.c_uint64),
+        ("offset", C.c_float * 3),
+        ("log_scale_mean", C.c_float), ("log_scale_sd", C.c_float),
+        ("opacity_logit_mean", C.c_float), ("opacity_logit_sd", C.c_float),
+        ("sh_sd", C.c_float), ("clusters", C.c_uint32),
+        ("box_half", C.c_float), ("cluster_sd_lo", C.c_float), ("cluster_sd_hi", C.c_float),
+        ("near_fraction", C.c_float), ("near_log_scale_mean", C.c_float), ("near_log_scale_sd", C.c_float),
+    ]
+
+
+# Every symbol include/lrt_hip.h and include/lrt_synth.h declare, with its signature.
+SIGNATURES = {
+    "lrt_lens_default": (None, [C.POINTER(Lens)]),
+    "lrt_camera_looking_at": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
+                                     C.POINTER(CameraC)]),
+    "lrt_projection_for": (C.c_int, [C.POINTER(CameraC), C.c_uint32, C.c_uint32, C.POINTER(ProjectionC)]),
+    "lrt_render_settings_default": (None, [C.POINTER(RenderSettingsC)]),
+    "lrt_splat_edit_default": (None, [C.POINTER(SplatEdit)]),
+    "lrt_context_create": (C.c_int, [C.c_int, C.c_void_p, C.POINTER(C.c_void_p)]),
+    "lrt_context_destroy": (None, [C.c_void_p]),
+    "lrt_context_set_stream": (C.c_int, [C.c_void_p, C.c_void_p]),
+    "lrt_last_error": (C.c_char_p, [C.c_void_p]),
+    "lrt_raster_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
+                                    C.POINTER(RenderSettingsC), C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
+    "lrt_raster_debug": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "lrt_frame_params": (C.c_int, [C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.POINTER(RenderSettingsC),
+                                   C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
+    "lrt_prefix_sum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "lrt_radix_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                 C.c_uint32, C.POINTER(C.c_int32)]),
+    "lrt_context_synchronize": (C.c_int, [C.c_void_p]),
+    "lrt_synth_scene": (C.c_int, [C.POINTER(SceneSpecC), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+    "lrt_pack_splats": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
+}
+
+
+def _load() -> C.CDLL:
+    # torch ships its own libamdhip64.so.7. Loading it first makes the library's
+    # DT_NEEDED resolve to that same runtime, so torch's streams and allocations
+    # and the library's kernels live in one HIP runtime, not two.
+    import torch  # noqa: F401
+    if not os.path.exists(LIB_PATH):
+        raise ImportError(
+            f"{LIB_PATH} is not built: run `python -c 'import __graft_entry__ as g; g.build()'` "
+            "(or `make -C lucabrtrender_amd/csrc`) first")
+    lib = C.CDLL(LIB_PATH)
+    for name, (res, args) in SIGNATURES.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = _load()
+
+
+class LrtError(RuntimeError):
+    """A failed lrt_* call: `code` is the lrt_status (the reference's ErrorCode)."""
+
+    def __init__(self, code: int, message: str):
+        super().__init__(f"{STATUS_NAMES.get(code, code)}: {message}")
+        self.code = code
+        self.name = STATUS_NAMES.get(code, str(code))

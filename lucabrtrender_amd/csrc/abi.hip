@@ -1,0 +1,303 @@
+// The extern "C" boundary of include/lrt_hip.h over the C++20 rasteriser.
+#include <hip/hip_runtime.h>
+
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/lrt_hip.h"
+#include "rasterizer.h"
+
+struct lrt_context {
+    lrt::TileRasterizer raster;
+    std::string error;
+};
+
+namespace {
+
+// The message of the last failure without a context (lrt_context_create).
+thread_local std::string gNoContextError;
+
+lrt_status finish(lrt_context* ctx, const lrt::Status& s) {
+    if (ctx != nullptr) ctx->error = s.ok() ? std::string() : s.message;
+    return static_cast<lrt_status>(static_cast<int>(s.code));
+}
+
+lrt::Mat4 toMat(const double m[16]) {
+    lrt::Mat4 out;
+    for (int k = 0; k < 16; ++k) out.m[static_cast<size_t>(k)] = m[k];
+    return out;
+}
+
+lrt::Projection toProjection(const lrt_projection& p) {
+    lrt::Projection out;
+    out.worldToView = toMat(p.world_to_view);
+    out.focalX = p.focal_x;
+    out.focalY = p.focal_y;
+    out.centreX = p.centre_x;
+    out.centreY = p.centre_y;
+    out.nearZ = p.near_z;
+    out.farZ = p.far_z;
+    out.orthographic = p.orthographic != 0;
+    out.eyeWorld = {p.eye_world[0], p.eye_world[1], p.eye_world[2]};
+    return out;
+}
+
+lrt::RenderSettings toSettings(const lrt_render_settings& s) {
+    lrt::RenderSettings out;
+    out.width = s.width;
+    out.height = s.height;
+    out.antialias = s.antialias != 0;
+    out.maxShDegree = s.max_sh_degree;
+    for (int k = 0; k < 4; ++k) out.background[static_cast<size_t>(k)] = s.background[k];
+    out.depthMode = s.depth_mode != 0 ? 1u : 0u;
+    out.depthThreshold = s.depth_threshold;
+    out.linearise = s.linearise != 0;
+    out.timeStages = s.time_stages != 0;
+    out.dilation = s.dilation;
+    out.filter3d = s.filter3d != 0;
+    return out;
+}
+
+lrt::SplatEditParams toEdit(const lrt_splat_edit& e) {
+    lrt::SplatEditParams out;
+    out.active = e.active != 0 ? 1u : 0u;
+    out.shape = e.shape;
+    out.mode = e.mode;
+    out.invert = e.invert != 0 ? 1u : 0u;
+    for (int k = 0; k < 3; ++k) {
+        out.centre[k] = e.centre[k];
+        out.size[k] = e.size[k];
+        out.tint[k] = e.tint[k];
+    }
+    out.saturation = e.saturation;
+    out.brightness = e.brightness;
+    out.opacity = e.opacity;
+    out.minOpacity = e.min_opacity;
+    out.maxScale = e.max_scale;
+    return out;
+}
+
+lrt::SplatCloud toCloud(const lrt_splats& c) {
+    lrt::SplatCloud out;
+    out.positions = reinterpret_cast<const float4*>(c.positions);
+    out.shape = reinterpret_cast<const uint4*>(c.shape);
+    out.sh = c.sh;
+    out.filter3d = c.filter3d;
+    out.count = c.count;
+    out.restPerColour = c.rest_per_colour;
+    out.shWords = c.sh_words;
+    return out;
+}
+
+}   // namespace
+
+extern "C" {
+
+void lrt_lens_default(lrt_lens* out) {
+    if (out == nullptr) return;
+    const lrt::Lens l;
+    out->orthographic = l.orthographic ? 1 : 0;
+    out->focal = l.focal;
+    out->haperture = l.haperture;
+    out->vaperture = l.vaperture;
+    out->near_z = l.nearZ;
+    out->far_z = l.farZ;
+    out->window_translate[0] = l.windowTranslate[0];
+    out->window_translate[1] = l.windowTranslate[1];
+    out->window_scale[0] = l.windowScale[0];
+    out->window_scale[1] = l.windowScale[1];
+    out->window_roll = l.windowRoll;
+}
+
+void lrt_camera_looking_at(const double eye[3], const double target[3], const double up[3], lrt_camera* out) {
+    if (out == nullptr || eye == nullptr || target == nullptr) return;
+    const lrt::Vec3 u = up != nullptr ? lrt::Vec3{up[0], up[1], up[2]} : lrt::Vec3{0.0, 1.0, 0.0};
+    const lrt::Camera c = lrt::Camera::lookingAt({eye[0], eye[1], eye[2]}, {target[0], target[1], target[2]}, u);
+    for (int k = 0; k < 16; ++k) out->camera_to_world[k] = c.cameraToWorld.m[static_cast<size_t>(k)];
+    lrt_lens_default(&out->lens);
+}
+
+lrt_status lrt_projection_for(const lrt_camera* camera, uint32_t width, uint32_t height, lrt_projection* out) {
+    if (camera == nullptr || out == nullptr) return LRT_INVALID_ARGUMENT;
+    lrt::Camera c;
+    c.cameraToWorld = toMat(camera->camera_to_world);
+    c.lens.orthographic = camera->lens.orthographic != 0;
+    c.lens.focal = camera->lens.focal;
+    c.lens.haperture = camera->lens.haperture;
+    c.lens.vaperture = camera->lens.vaperture;
+    c.lens.nearZ = camera->lens.near_z;
+    c.lens.farZ = camera->lens.far_z;
+    for (int k = 0; k < 2; ++k) {
+        c.lens.windowTranslate[k] = camera->lens.window_translate[k];
+        c.lens.windowScale[k] = camera->lens.window_scale[k];
+    }
+    c.lens.windowRoll = camera->lens.window_roll;
+    const lrt::Projection p = lrt::projectionFor(c, width, height);
+    for (int k = 0; k < 16; ++k) out->world_to_view[k] = p.worldToView.m[static_cast<size_t>(k)];
+    out->focal_x = p.focalX;
+    out->focal_y = p.focalY;
+    out->centre_x = p.centreX;
+    out->centre_y = p.centreY;
+    out->near_z = p.nearZ;
+    out->far_z = p.farZ;
+    out->orthographic = p.orthographic ? 1 : 0;
+    out->eye_world[0] = p.eyeWorld.x;
+    out->eye_world[1] = p.eyeWorld.y;
+    out->eye_world[2] = p.eyeWorld.z;
+    return LRT_OK;
+}
+
+void lrt_render_settings_default(lrt_render_settings* out) {
+    if (out == nullptr) return;
+    const lrt::RenderSettings s;
+    std::memset(out, 0, sizeof *out);
+    out->width = s.width;
+    out->height = s.height;
+    out->antialias = s.antialias ? 1 : 0;
+    out->max_sh_degree = s.maxShDegree;
+    for (int k = 0; k < 4; ++k) out->background[k] = s.background[static_cast<size_t>(k)];
+    out->depth_mode = 0;
+    out->depth_threshold = s.depthThreshold;
+    out->linearise = s.linearise ? 1 : 0;
+    out->time_stages = 0;
+    out->dilation = s.dilation;
+    out->filter3d = 0;
+}
+
+void lrt_splat_edit_default(lrt_splat_edit* out) {
+    if (out == nullptr) return;
+    const lrt::SplatEditParams e;
+    std::memset(out, 0, sizeof *out);
+    out->active = 0;
+    out->shape = e.shape;
+    out->mode = e.mode;
+    out->invert = 0;
+    for (int k = 0; k < 3; ++k) {
+        out->centre[k] = e.centre[k];
+        out->size[k] = e.size[k];
+        out->tint[k] = e.tint[k];
+    }
+    out->saturation = e.saturation;
+    out->brightness = e.brightness;
+    out->opacity = e.opacity;
+    out->min_opacity = e.minOpacity;
+    out->max_scale = e.maxScale;
+}
+
+lrt_status lrt_context_create(int device, void* stream, lrt_context** out) {
+    if (out == nullptr) return LRT_INVALID_ARGUMENT;
+    *out = nullptr;
+    auto* ctx = new (std::nothrow) lrt_context();
+    if (ctx == nullptr) return LRT_OUT_OF_MEMORY;
+    const lrt::Status s = ctx->raster.create(device, static_cast<hipStream_t>(stream));
+    if (!s.ok()) {
+        gNoContextError = s.message;
+        const lrt_status code = static_cast<lrt_status>(static_cast<int>(s.code));
+        delete ctx;
+        return code;
+    }
+    *out = ctx;
+    return LRT_OK;
+}
+
+void lrt_context_destroy(lrt_context* ctx) { delete ctx; }
+
+lrt_status lrt_context_set_stream(lrt_context* ctx, void* stream) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    ctx->raster.setStream(static_cast<hipStream_t>(stream));
+    return finish(ctx, {});
+}
+
+const char* lrt_last_error(const lrt_context* ctx) {
+    return ctx != nullptr ? ctx->error.c_str() : gNoContextError.c_str();
+}
+
+lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection, const lrt_splat_instance* instances,
+                             uint32_t instance_count, const lrt_render_settings* settings, lrt_render_targets* targets,
+                             lrt_frame_stats* stats) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (projection == nullptr || settings == nullptr || targets == nullptr || (instance_count > 0 && instances == nullptr)) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    std::vector<lrt::SplatCloud> clouds(instance_count);
+    std::vector<lrt::SplatInstance> list(instance_count);
+    for (uint32_t k = 0; k < instance_count; ++k) {
+        if (instances[k].splats != nullptr) {
+            clouds[k] = toCloud(*instances[k].splats);
+            list[k].splats = &clouds[k];
+        }
+        list[k].objectToWorld = toMat(instances[k].object_to_world);
+        list[k].edit = toEdit(instances[k].edit);
+    }
+    lrt::RenderTargets t;
+    t.width = targets->width;
+    t.height = targets->height;
+    t.colour = reinterpret_cast<float4*>(targets->colour);
+    t.depth = targets->depth;
+    lrt::FrameStats fs;
+    const lrt::Status s = ctx->raster.render(toProjection(*projection), list, toSettings(*settings), t, &fs);
+    if (s.ok() && stats != nullptr) {
+        stats->splats = fs.splats;
+        stats->visible = fs.visible;
+        stats->pairs = fs.pairs;
+        stats->project_ms = fs.projectMs;
+        stats->depth_sort_ms = fs.depthSortMs;
+        stats->counts_ms = fs.countsMs;
+        stats->emit_ms = fs.emitMs;
+        stats->tile_sort_ms = fs.tileSortMs;
+        stats->blend_ms = fs.blendMs;
+        stats->total_ms = fs.totalMs;
+    }
+    return finish(ctx, s);
+}
+
+lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    return finish(ctx, ctx->raster.debugCopy(which, host, bytes, needed));
+}
+
+lrt_status lrt_frame_params(const lrt_projection* projection, const lrt_splat_instance* instance,
+                            const lrt_render_settings* settings, uint32_t base, void* out, size_t bytes,
+                            size_t* needed) {
+    if (needed != nullptr) *needed = sizeof(lrt::FrameParams);
+    if (projection == nullptr || instance == nullptr || settings == nullptr) return LRT_INVALID_ARGUMENT;
+    if (out == nullptr || bytes == 0) return LRT_OK;
+    if (bytes < sizeof(lrt::FrameParams)) return LRT_INVALID_ARGUMENT;
+    lrt::SplatCloud cloud;
+    lrt::SplatInstance inst;
+    if (instance->splats != nullptr) {
+        cloud = toCloud(*instance->splats);
+        inst.splats = &cloud;
+    }
+    inst.objectToWorld = toMat(instance->object_to_world);
+    inst.edit = toEdit(instance->edit);
+    const lrt::FrameParams f = lrt::frameParamsFor(toProjection(*projection), inst, toSettings(*settings), base);
+    std::memcpy(out, &f, sizeof f);
+    return LRT_OK;
+}
+
+lrt_status lrt_prefix_sum(lrt_context* ctx, const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    return finish(ctx, ctx->raster.prefixSum(in, out, total, count));
+}
+
+lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, uint64_t* scratch_keys,
+                          uint32_t* scratch_values, uint32_t count, uint32_t key_bits, int32_t* result_in_scratch) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    bool alt = false;
+    const lrt::Status s =
+        ctx->raster.radixSort(keys, values, scratch_keys, scratch_values, count, key_bits, &alt);
+    if (result_in_scratch != nullptr) *result_in_scratch = alt ? 1 : 0;
+    return finish(ctx, s);
+}
+
+lrt_status lrt_context_synchronize(lrt_context* ctx) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    const hipError_t e = hipStreamSynchronize(ctx->raster.stream());
+    return finish(ctx, e == hipSuccess ? lrt::Status{}
+                                       : lrt::Status::error(lrt::ErrorCode::DeviceFailure, hipGetErrorString(e)));
+}
+
+}   // extern "C"

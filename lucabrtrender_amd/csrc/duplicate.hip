@@ -1,0 +1,71 @@
+// Duplication: one (tile, depth) key per tile a splat's footprint touches,
+// written at the splat's offset from the exclusive scan of the touched counts.
+// Repeats the projection's exact tile test, as the reference's splatEmit does
+// (shaders/lrt/splat/splat_emit.slang:17-50), so each splat writes exactly the
+// count it was given.
+//
+// Keys are 64-bit: tile id in the high word, the view depth's float bits in
+// the low word (positive floats order as their bits). Values are slots. The
+// pairs leave this kernel in slot order, which is what makes one stable sort on
+// (tile, depth >> 7) give each tile its splats in the reference's order:
+// depth key first, slot on ties.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "splat_math.h"
+
+namespace lrt::kern {
+namespace {
+
+using namespace lrt::dev;
+
+constexpr uint32_t kBlock = 256;
+
+__global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_t slots,
+                                                          const float4* __restrict__ proj,
+                                                          const uint2* __restrict__ rects,
+                                                          const uint32_t* __restrict__ touched,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ values, uint32_t capacity,
+                                                          uint32_t* __restrict__ overflow) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= slots) return;
+    const uint32_t expected = touched[s];
+    if (expected == 0) return;
+    const uint32_t first = offsets[s];
+    const float4 screen = proj[static_cast<uint64_t>(s) * 3];
+    const float4 conic = proj[static_cast<uint64_t>(s) * 3 + 1];
+    const uint2 rc = rects[s];
+    const uint32_t x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
+    const uint64_t depthBits = floatToBits(screen.z);
+    uint32_t written = 0;
+    for (uint32_t ty = y0; ty <= y1 && written < expected; ++ty) {
+        for (uint32_t tx = x0; tx <= x1 && written < expected; ++tx) {
+            if (!tileTouched(f.width, f.height, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
+                continue;
+            }
+            const uint32_t at = first + written;
+            if (at < capacity) {
+                keys[at] = (static_cast<uint64_t>(ty * f.tilesX + tx) << 32) | depthBits;
+                values[at] = s;
+            } else {
+                atomicAdd(overflow, 1u);
+            }
+            written += 1;
+        }
+    }
+}
+
+}   // namespace
+
+void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
+               const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
+               uint32_t capacity, uint32_t* overflow, hipStream_t stream) {
+    if (slots == 0) return;
+    const uint32_t blocks = (slots + kBlock - 1) / kBlock;
+    duplicateKernel<<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values, capacity,
+                                                   overflow);
+}
+
+}   // namespace lrt::kern

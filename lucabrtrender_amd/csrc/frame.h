@@ -1,0 +1,54 @@
+// The per-launch constant block of the splat kernels and the records they pass
+// between stages. Plain data, shared by the C++20 host and the HIP kernels.
+//
+// FrameParams carries the fields of the reference's FrameParams
+// (shaders/lrt/splat/frame.slang:33-81) that the splat-only path reads, plus
+// two filter parameters that the reference fixes: `dilation` (its +0.3 px^2,
+// splat_project.slang:122-125) and `useFilter3d` (Mip-Splatting's per-splat 3D
+// filter, which the reference does not have).
+#pragma once
+
+#include <cstdint>
+
+namespace lrt {
+
+inline constexpr uint32_t kTile = 16;   // frame.slang:31
+
+struct SplatEditParams {   // shaders/lrt/common/edit.slang:23-36
+    uint32_t active = 0, shape = 0, mode = 2, invert = 0;
+    float centre[3] = {0, 0, 0};
+    float size[3] = {1, 1, 1};
+    float tint[3] = {1, 1, 1};
+    float saturation = 1.0F, brightness = 1.0F, opacity = 1.0F, minOpacity = 0.0F, maxScale = 0.0F;
+};
+
+struct FrameParams {
+    uint32_t width, height, tilesX, tilesY;
+    float focalX, focalY, centreX, centreY, nearZ, farZ;
+    uint32_t orthographic, antialias;
+    float m[12];          // object -> view, rows of a 3x4
+    float eye[3];         // the eye in the cloud's own space
+    uint32_t count;       // splats of this instance
+    uint32_t base;        // its first slot
+    uint32_t restPerColour, shWords, shLimit, tileBits;
+    uint32_t depthMode;
+    float depthThreshold;
+    float bg[4];
+    uint32_t linearise;
+    float dilation;
+    uint32_t useFilter3d;
+    SplatEditParams edit;
+};
+
+// What the projection leaves for the blend, one record per slot: the
+// reference's ProjRecord (frame.slang:85-89), three float4 = 48 bytes.
+//   screen = centre.xy, view z, cutoff power
+//   conic  = a, b, c, alpha
+//   colour = rgb (encoded), 0
+struct ProjRecord {
+    float screen[4];
+    float conic[4];
+    float colour[4];
+};
+
+}   // namespace lrt

@@ -1,0 +1,76 @@
+// Host-side launchers of the HIP kernels. Each takes device pointers and the
+// stream to queue on; none synchronises. Sizes are validated by the caller
+// (rasterizer.hip) before a launch: every launcher assumes its buffers hold
+// what its comment says.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "frame.h"
+
+namespace lrt::kern {
+
+// u32 fill of `count` words (used for counters, flags and the look-back
+// status words, instead of hipMemsetAsync, so that a captured graph replays it).
+void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
+
+// ---- scan ----------------------------------------------------------------
+// Exclusive prefix sum of `n` u32 into `out` (may not alias `in`), the total
+// into *total and the number of non-zero inputs into *nonzero. `status` holds
+// scanPartitions(n) u64 words and `ticket` one u32; both are cleared here.
+uint32_t scanPartitions(uint32_t n);
+void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* ticket,
+                   uint32_t* total, uint32_t* nonzero, uint32_t* error, hipStream_t stream);
+
+// ---- projection -----------------------------------------------------------
+// One instance: f.count splats from index 0 of the cloud arrays, written to
+// slots f.base + i of proj (3 float4 a slot), rects (uint2 a slot: x0 | y0 << 16,
+// x1 | y1 << 16) and touched (u32 a slot).
+void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
+             const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream);
+
+// ---- duplication ------------------------------------------------------------
+// For each of `slots` slots with touched > 0: its pairs at offsets[slot].., keys
+// (tile << 32 | bits(view z)) and values (slot). Writes past `capacity` are
+// dropped and counted in *overflow.
+void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
+               const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
+               uint32_t capacity, uint32_t* overflow, hipStream_t stream);
+
+// ---- sort -------------------------------------------------------------------
+// Which bits of a 64-bit key the digits come from.
+enum class KeyMode : uint32_t {
+    Plain = 0,        // bits [bitLo, bitLo + keyBits)
+    TileDepth = 1,    // (key >> 32) << 24 | (key & 0x7FFFFFFF) >> 7: tile id over the
+                      // top 24 bits of a positive float depth; keyBits = tileBits + 24
+};
+
+struct SortWorkspace {
+    uint32_t* histogram;   // passes * 256
+    uint32_t* offsets;     // passes * 256
+    uint32_t* status;      // passes * partitions * 256
+    uint32_t* tickets;     // passes
+    uint32_t* error;       // 1
+};
+uint32_t sortPartitions(uint32_t n);
+uint32_t sortPasses(uint32_t keyBits);
+uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits);
+// Stable LSD radix sort, eight bits a pass, one onesweep launch a pass. The
+// result is in (keys, values) after an even number of passes and in
+// (keysAlt, valuesAlt) after an odd one; returns true for the latter.
+bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
+               KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream);
+
+// ---- ranges -----------------------------------------------------------------
+// ranges[t] = [start, end) of tile t's run in the sorted keys, (0, 0) when empty.
+void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream);
+
+// ---- blend ------------------------------------------------------------------
+// Per pixel, front to back over its tile's pairs; slots bounds the pair values.
+// lastCount/finalT may be null.
+void blend(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* pairSlots, const float4* proj,
+           float4* colour, float* depth, uint32_t* lastCount, float* finalT, hipStream_t stream);
+
+}   // namespace lrt::kern

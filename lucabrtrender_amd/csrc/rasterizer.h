@@ -1,0 +1,203 @@
+// The C++20 host side of the splat tile rasteriser: the shape of the
+// reference's lrt::render::TileRasterizer (modules/render/include/lrt/render/
+// TileRasterizer.h:265-322) over HIP instead of slang-rhi. It owns every
+// scratch buffer (grown by half again from 1024 and never shrunk, as the
+// reference's reserve* do, TileRasterizer.cpp:26-32,105-177), queues the
+// kernels of kernels.h on one stream and waits where the reference waits.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <array>
+#include <cstdint>
+#include <span>
+#include <string>
+
+#include "frame.h"
+
+namespace lrt {
+
+enum class ErrorCode : int {   // modules/core/include/lrt/core/Error.h:14-26
+    Ok = 0,
+    Unknown,
+    NotFound,
+    InvalidArgument,
+    Unsupported,
+    IoFailure,
+    DeviceFailure,
+    ShaderFailure,
+    PluginFailure,
+    OutOfMemory,
+    Cancelled,
+    InternalError,
+};
+
+struct Status {
+    ErrorCode code = ErrorCode::Ok;
+    std::string message;
+    [[nodiscard]] bool ok() const noexcept { return code == ErrorCode::Ok; }
+    static Status error(ErrorCode c, std::string m) { return Status{c, std::move(m)}; }
+};
+
+// Host math in FP64 (aofx Transform.h): row-major 4x4 acting on column vectors.
+struct Mat4 {
+    std::array<double, 16> m{1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1, 0, 0, 0, 0, 1};
+    [[nodiscard]] double at(int r, int c) const { return m[static_cast<size_t>(r * 4 + c)]; }
+    double& at(int r, int c) { return m[static_cast<size_t>(r * 4 + c)]; }
+};
+struct Vec3 {
+    double x = 0, y = 0, z = 0;
+};
+
+Mat4 operator*(const Mat4& a, const Mat4& b);
+Mat4 inverseAffine(const Mat4& m);
+Mat4 scaling(const Vec3& s);
+Mat4 rotationZ(double degrees);
+Mat4 lookAt(const Vec3& eye, const Vec3& target, const Vec3& up);
+Vec3 transformPoint(const Mat4& m, const Vec3& p);
+
+struct Lens {   // modules/render/include/lrt/render/Camera.h:73-94
+    bool orthographic = false;
+    double focal = 50.0, haperture = 24.576, vaperture = 18.672, nearZ = 0.1, farZ = 10000.0;
+    double windowTranslate[2] = {0.0, 0.0};
+    double windowScale[2] = {1.0, 1.0};
+    double windowRoll = 0.0;
+};
+
+struct Camera {   // Camera.h:96-103
+    Mat4 cameraToWorld;
+    Lens lens;
+    static Camera lookingAt(const Vec3& eye, const Vec3& target, const Vec3& up = {0.0, 1.0, 0.0});
+};
+
+struct Projection {   // Camera.h:106-115
+    Mat4 worldToView;
+    double focalX = 0, focalY = 0, centreX = 0, centreY = 0, nearZ = 0.1, farZ = 10000.0;
+    bool orthographic = false;
+    Vec3 eyeWorld;
+};
+
+Projection projectionFor(const Camera& camera, uint32_t width, uint32_t height);
+uint32_t bitsFor(uint32_t values);
+
+struct SplatCloud {   // scene::GpuSplats, device pointers the caller owns
+    const float4* positions = nullptr;
+    const uint4* shape = nullptr;
+    const uint32_t* sh = nullptr;
+    const float* filter3d = nullptr;
+    uint32_t count = 0, restPerColour = 0, shWords = 1;
+};
+
+struct SplatInstance {   // render::SplatInstance without relighting
+    const SplatCloud* splats = nullptr;
+    Mat4 objectToWorld;
+    SplatEditParams edit;
+};
+
+struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
+    uint32_t width = 1920, height = 1080;
+    bool antialias = true;
+    uint32_t maxShDegree = 3;
+    std::array<float, 4> background{0, 0, 0, 0};
+    uint32_t depthMode = 0;
+    float depthThreshold = 0.5F;
+    bool linearise = true;
+    bool timeStages = false;
+    float dilation = 0.3F;
+    bool filter3d = false;
+};
+
+struct RenderTargets {   // caller-owned device images, bottom row first
+    uint32_t width = 0, height = 0;
+    float4* colour = nullptr;
+    float* depth = nullptr;
+};
+
+struct FrameStats {   // render::FrameStats (TileRasterizer.h:249-255)
+    uint32_t splats = 0, visible = 0, pairs = 0;
+    double projectMs = 0, depthSortMs = 0, countsMs = 0, emitMs = 0, tileSortMs = 0, blendMs = 0, totalMs = 0;
+};
+
+// setFrame + setObject (modules/render/src/FrameParams.h:32-69).
+FrameParams frameParamsFor(const Projection& projection, const SplatInstance& instance,
+                           const RenderSettings& settings, uint32_t base);
+
+// A device buffer that grows by half again from 1024 elements and never shrinks.
+class DeviceBuffer {
+public:
+    DeviceBuffer() = default;
+    DeviceBuffer(const DeviceBuffer&) = delete;
+    DeviceBuffer& operator=(const DeviceBuffer&) = delete;
+    ~DeviceBuffer();
+    Status reserve(uint64_t bytes, const char* label);
+    [[nodiscard]] void* get() const noexcept { return ptr_; }
+    template <typename T>
+    [[nodiscard]] T* as() const noexcept {
+        return static_cast<T*>(ptr_);
+    }
+    [[nodiscard]] uint64_t bytes() const noexcept { return bytes_; }
+
+private:
+    void* ptr_ = nullptr;
+    uint64_t bytes_ = 0;
+};
+
+class TileRasterizer {
+public:
+    TileRasterizer() = default;
+    TileRasterizer(const TileRasterizer&) = delete;
+    TileRasterizer& operator=(const TileRasterizer&) = delete;
+    ~TileRasterizer();
+
+    Status create(int device, hipStream_t stream);
+    void setStream(hipStream_t stream) { stream_ = stream; }
+    [[nodiscard]] hipStream_t stream() const noexcept { return stream_; }
+
+    Status render(const Projection& projection, std::span<const SplatInstance> instances,
+                  const RenderSettings& settings, RenderTargets& targets, FrameStats* stats);
+    Status render(const Camera& camera, std::span<const SplatInstance> instances, const RenderSettings& settings,
+                  RenderTargets& targets, FrameStats* stats) {
+        return render(projectionFor(camera, settings.width, settings.height), instances, settings, targets, stats);
+    }
+
+    // PrefixSum::apply and RadixSort::sort over caller buffers, on this stream.
+    Status prefixSum(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count);
+    Status radixSort(uint64_t* keys, uint32_t* values, uint64_t* scratchKeys, uint32_t* scratchValues,
+                     uint32_t count, uint32_t keyBits, bool* resultInScratch);
+
+    // What the last render left (LRT_DEBUG_* of lrt_hip.h).
+    Status debugCopy(int which, void* host, size_t bytes, size_t* needed);
+
+private:
+    Status reserveSplats(uint32_t count);
+    Status reservePairs(uint32_t count, uint32_t keyBits);
+    Status reserveTargets(uint32_t width, uint32_t height, uint32_t tiles);
+    Status check(hipError_t e, const char* what);
+
+    int device_ = 0;
+    hipStream_t stream_ = nullptr;
+    bool created_ = false;
+
+    // Per slot.
+    uint32_t splatCapacity_ = 0;
+    DeviceBuffer proj_, rects_, touched_, offsets_, scanStatus_;
+    // Per pair, ping-pong.
+    uint32_t pairCapacity_ = 0;
+    DeviceBuffer keys_[2], values_[2], sortWork_;
+    uint64_t sortWorkWords_ = 0;
+    // Per tile and per pixel.
+    uint32_t tileCapacity_ = 0;
+    uint64_t pixelCapacity_ = 0;
+    DeviceBuffer ranges_, lastCount_, finalT_;
+    // Counters: [0] pairs, [1] visible, [2] overflow, [3] scan ticket, [4] sort error, [5] scan error.
+    DeviceBuffer counters_;
+    uint32_t* hostCounters_ = nullptr;   // pinned
+    hipEvent_t events_[7] = {};
+
+    // What the last render left.
+    uint32_t lastSplats_ = 0, lastVisible_ = 0, lastPairs_ = 0, lastTiles_ = 0;
+    uint64_t lastPixels_ = 0;
+    int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
+};
+
+}   // namespace lrt

@@ -1,0 +1,358 @@
+// TileRasterizer::render over HIP: the reference's frame
+// (modules/render/src/TileRasterizer.cpp:188-484) with its two sorts replaced
+// by one stable sort of 64-bit (tile, depth) keys.
+//
+//   project (per instance) -> scan of tiles touched (pairs + visible count)
+//   -> [one wait: the pair count sizes the pair buffers, as the reference's
+//      readback at :385-397 does] -> duplicate -> sort -> ranges -> blend
+//   -> [the final wait, as :476].
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstring>
+#include <string>
+
+#include "kernels.h"
+#include "rasterizer.h"
+
+namespace lrt {
+namespace {
+
+uint64_t grow(uint64_t have, uint64_t need) {
+    uint64_t size = std::max<uint64_t>(have, 1024);
+    while (size < need) size = size * 3 / 2;
+    return size;
+}
+
+// The sort keeps 30-bit counts in its look-back words.
+constexpr uint32_t kMaxPairs = (1u << 30) - 1;
+
+}   // namespace
+
+DeviceBuffer::~DeviceBuffer() {
+    if (ptr_ != nullptr) (void)hipFree(ptr_);
+}
+
+Status DeviceBuffer::reserve(uint64_t bytes, const char* label) {
+    if (bytes <= bytes_) return {};
+    if (ptr_ != nullptr) {
+        (void)hipFree(ptr_);
+        ptr_ = nullptr;
+        bytes_ = 0;
+    }
+    const hipError_t e = hipMalloc(&ptr_, std::max<uint64_t>(bytes, 256));
+    if (e != hipSuccess) {
+        ptr_ = nullptr;
+        return Status::error(e == hipErrorOutOfMemory ? ErrorCode::OutOfMemory : ErrorCode::DeviceFailure,
+                             std::string("allocating ") + label + " (" + std::to_string(bytes) +
+                                 " bytes): " + hipGetErrorString(e));
+    }
+    bytes_ = bytes;
+    return {};
+}
+
+TileRasterizer::~TileRasterizer() {
+    if (!created_) return;
+    (void)hipSetDevice(device_);
+    for (hipEvent_t& e : events_)
+        if (e != nullptr) (void)hipEventDestroy(e);
+    if (hostCounters_ != nullptr) (void)hipHostFree(hostCounters_);
+}
+
+Status TileRasterizer::check(hipError_t e, const char* what) {
+    if (e == hipSuccess) return {};
+    return Status::error(e == hipErrorOutOfMemory ? ErrorCode::OutOfMemory : ErrorCode::DeviceFailure,
+                         std::string(what) + ": " + hipGetErrorString(e));
+}
+
+Status TileRasterizer::create(int device, hipStream_t stream) {
+    int count = 0;
+    if (hipGetDeviceCount(&count) != hipSuccess || count == 0) {
+        return Status::error(ErrorCode::DeviceFailure, "no HIP device");
+    }
+    if (device < 0 || device >= count) {
+        return Status::error(ErrorCode::InvalidArgument, "device " + std::to_string(device) + " of " +
+                                                             std::to_string(count));
+    }
+    device_ = device;
+    stream_ = stream;
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    for (hipEvent_t& e : events_) {
+        s = check(hipEventCreate(&e), "hipEventCreate");
+        if (!s.ok()) return s;
+    }
+    s = check(hipHostMalloc(reinterpret_cast<void**>(&hostCounters_), 64, hipHostMallocDefault), "hipHostMalloc");
+    if (!s.ok()) return s;
+    s = counters_.reserve(64, "counters");
+    if (!s.ok()) return s;
+    kern::fill32(counters_.as<uint32_t>(), 0u, 16, stream_);
+    created_ = true;
+    return check(hipStreamSynchronize(stream_), "create");
+}
+
+Status TileRasterizer::reserveSplats(uint32_t count) {
+    if (count <= splatCapacity_) return {};
+    const uint64_t n = std::min<uint64_t>(grow(splatCapacity_, count), 0xFFFFFFFFull);
+    Status s;
+    if (!(s = proj_.reserve(n * sizeof(ProjRecord), "splat.proj")).ok()) return s;
+    if (!(s = rects_.reserve(n * 8, "splat.rects")).ok()) return s;
+    if (!(s = touched_.reserve(n * 4, "splat.touched")).ok()) return s;
+    if (!(s = offsets_.reserve(n * 4, "splat.offsets")).ok()) return s;
+    if (!(s = scanStatus_.reserve(uint64_t{kern::scanPartitions(static_cast<uint32_t>(n))} * 8 + 8,
+                                  "splat.scanStatus"))
+             .ok())
+        return s;
+    splatCapacity_ = static_cast<uint32_t>(n);
+    return {};
+}
+
+Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits) {
+    Status s;
+    if (count > pairCapacity_) {
+        const uint64_t n = std::min<uint64_t>(grow(pairCapacity_, count), kMaxPairs);
+        for (int k = 0; k < 2; ++k) {
+            if (!(s = keys_[k].reserve(n * 8, "pair.keys")).ok()) return s;
+            if (!(s = values_[k].reserve(n * 4, "pair.values")).ok()) return s;
+        }
+        pairCapacity_ = static_cast<uint32_t>(n);
+    }
+    const uint64_t words = kern::sortWorkspaceWords(pairCapacity_, std::max<uint32_t>(keyBits, 40));
+    if (words > sortWorkWords_) {
+        if (!(s = sortWork_.reserve(words * 4, "pair.sortWorkspace")).ok()) return s;
+        sortWorkWords_ = words;
+    }
+    return {};
+}
+
+Status TileRasterizer::reserveTargets(uint32_t width, uint32_t height, uint32_t tiles) {
+    Status s;
+    if (tiles > tileCapacity_) {
+        if (!(s = ranges_.reserve(uint64_t{tiles} * 8, "tile.ranges")).ok()) return s;
+        tileCapacity_ = tiles;
+    }
+    const uint64_t pixels = uint64_t{width} * height;
+    if (pixels > pixelCapacity_) {
+        if (!(s = lastCount_.reserve(pixels * 4, "pixel.lastCount")).ok()) return s;
+        if (!(s = finalT_.reserve(pixels * 4, "pixel.finalT")).ok()) return s;
+        pixelCapacity_ = pixels;
+    }
+    return {};
+}
+
+Status TileRasterizer::render(const Projection& projection, std::span<const SplatInstance> instances,
+                              const RenderSettings& settings, RenderTargets& targets, FrameStats* statsOut) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (settings.width == 0 || settings.height == 0) {
+        return Status::error(ErrorCode::InvalidArgument, "a frame needs a size");
+    }
+    if (settings.width > 0xFFFFu * kTile || settings.height > 0xFFFFu * kTile) {
+        return Status::error(ErrorCode::Unsupported, "a frame wider or taller than 1048560 pixels");
+    }
+    if (targets.width != settings.width || targets.height != settings.height || targets.colour == nullptr) {
+        return Status::error(ErrorCode::InvalidArgument, "the targets are not the frame's size");
+    }
+    uint64_t total = 0;
+    for (const SplatInstance& instance : instances) {
+        if (instance.splats == nullptr) continue;
+        const SplatCloud& c = *instance.splats;
+        if (c.count > 0 && (c.positions == nullptr || c.shape == nullptr)) {
+            return Status::error(ErrorCode::InvalidArgument, "a cloud without positions or shape");
+        }
+        if (c.count > 0 && c.restPerColour > 0 && c.sh == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a cloud with harmonics but no sh buffer");
+        }
+        if (c.restPerColour != 0 && c.restPerColour != 3 && c.restPerColour != 8 && c.restPerColour != 15) {
+            return Status::error(ErrorCode::InvalidArgument, "restPerColour must be 0, 3, 8 or 15");
+        }
+        if (c.restPerColour > 0 && c.shWords < (c.restPerColour * 3 + 1) / 2) {
+            return Status::error(ErrorCode::InvalidArgument, "shWords too small for restPerColour");
+        }
+        if (c.restPerColour > 0 && c.shWords > 23) {
+            return Status::error(ErrorCode::Unsupported, "more than 23 harmonic words a splat");
+        }
+        if (settings.filter3d && c.count > 0 && c.filter3d == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and a cloud has no filter3d");
+        }
+        total += c.count;
+    }
+    if (total > 0xFFFFFFFFull) return Status::error(ErrorCode::Unsupported, "more than 2^32 splats in one frame");
+    const uint32_t splatCount = static_cast<uint32_t>(total);
+    const uint32_t tilesX = (settings.width + kTile - 1) / kTile;
+    const uint32_t tilesY = (settings.height + kTile - 1) / kTile;
+    const uint32_t tiles = tilesX * tilesY;
+    const uint32_t keyBits = 24 + bitsFor(tiles);
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = reserveSplats(std::max(splatCount, 1u))).ok()) return s;
+    if (!(s = reserveTargets(settings.width, settings.height, tiles)).ok()) return s;
+
+    uint32_t* counters = counters_.as<uint32_t>();
+    const bool timed = settings.timeStages;
+    FrameStats stats;
+    stats.splats = splatCount;
+    if (timed) (void)hipEventRecord(events_[0], stream_);
+
+    // Project every instance into its own slots.
+    uint32_t base = 0;
+    FrameParams first{};
+    bool haveFirst = false;
+    for (const SplatInstance& instance : instances) {
+        const SplatCloud* cloud = instance.splats;
+        FrameParams f = frameParamsFor(projection, instance, settings, base);
+        if (!haveFirst) {
+            first = f;
+            haveFirst = true;
+        }
+        if (cloud == nullptr || cloud->count == 0) continue;
+        kern::project(f, cloud->positions, cloud->shape, cloud->sh, cloud->filter3d, proj_.as<float4>(),
+                      rects_.as<uint2>(), touched_.as<uint32_t>(), stream_);
+        base += cloud->count;
+    }
+    if (!haveFirst) {
+        SplatInstance none;
+        first = frameParamsFor(projection, none, settings, 0);
+    }
+    if (timed) (void)hipEventRecord(events_[1], stream_);
+
+    // How many pairs and how many visible: one scan, one readback.
+    kern::fill32(counters, 0u, 8, stream_);
+    kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), splatCount, scanStatus_.as<uint64_t>(),
+                        counters + 3, counters + 0, counters + 1, counters + 5, stream_);
+    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "counts")).ok())
+        return s;
+    if (!(s = check(hipStreamSynchronize(stream_), "counts")).ok()) return s;
+    if (timed) (void)hipEventRecord(events_[2], stream_);
+    const uint32_t pairs = splatCount > 0 ? hostCounters_[0] : 0u;
+    const uint32_t visible = splatCount > 0 ? hostCounters_[1] : 0u;
+    if (hostCounters_[5] != 0) return Status::error(ErrorCode::InternalError, "the scan's look-back gave up");
+    if (pairs > kMaxPairs) {
+        return Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 (tile, splat) pairs in one frame");
+    }
+    stats.visible = visible;
+    stats.pairs = pairs;
+    if (!(s = reservePairs(std::max(pairs, 1u), keyBits)).ok()) return s;
+
+    // Duplicate into 64-bit keys, in slot order.
+    kern::duplicate(first, splatCount, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
+                    offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), pairCapacity_,
+                    counters + 2, stream_);
+    if (timed) (void)hipEventRecord(events_[3], stream_);
+
+    // One stable sort on (tile, depth >> 7), then the ranges.
+    const uint32_t passes = kern::sortPasses(keyBits);
+    const uint32_t parts = kern::sortPartitions(pairCapacity_);
+    uint32_t* work = sortWork_.as<uint32_t>();
+    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512, work + passes * 512 + uint64_t{passes} * parts * 256,
+                           counters + 4};
+    const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
+                                       values_[1].as<uint32_t>(), pairs, kern::KeyMode::TileDepth, 0, keyBits, ws,
+                                       stream_);
+    const int sorted = inAlt ? 1 : 0;
+    kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_);
+    if (timed) (void)hipEventRecord(events_[4], stream_);
+
+    kern::blend(first, splatCount, ranges_.as<uint2>(), values_[sorted].as<uint32_t>(), proj_.as<float4>(), targets.colour,
+                targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(), stream_);
+    if (timed) (void)hipEventRecord(events_[5], stream_);
+    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "frame")).ok())
+        return s;
+    if (!(s = check(hipStreamSynchronize(stream_), "frame")).ok()) return s;
+    if (!(s = check(hipGetLastError(), "frame")).ok()) return s;
+    if (hostCounters_[2] != 0) return Status::error(ErrorCode::InternalError, "the duplication overran its pairs");
+    if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
+
+    lastSplats_ = splatCount;
+    lastVisible_ = visible;
+    lastPairs_ = pairs;
+    lastTiles_ = tiles;
+    lastPixels_ = uint64_t{settings.width} * settings.height;
+    lastSorted_ = sorted;
+    if (timed) {
+        float ms[5] = {};
+        for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], events_[k], events_[k + 1]);
+        stats.projectMs = ms[0];
+        stats.countsMs = ms[1];
+        stats.emitMs = ms[2];
+        stats.tileSortMs = ms[3];
+        stats.blendMs = ms[4];
+        float all = 0.0F;
+        (void)hipEventElapsedTime(&all, events_[0], events_[5]);
+        stats.totalMs = all;
+    }
+    if (statsOut != nullptr) *statsOut = stats;
+    return {};
+}
+
+Status TileRasterizer::prefixSum(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (in == nullptr || out == nullptr || total == nullptr) return Status::error(ErrorCode::InvalidArgument, "null buffer");
+    if (in == out) return Status::error(ErrorCode::InvalidArgument, "the scan's input and output must differ");
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = reserveSplats(std::max(count, 1u))).ok()) return s;
+    uint32_t* counters = counters_.as<uint32_t>();
+    kern::scanExclusive(in, out, count, scanStatus_.as<uint64_t>(), counters + 3, total, counters + 1, counters + 5,
+                        stream_);
+    return check(hipGetLastError(), "prefix sum");
+}
+
+Status TileRasterizer::radixSort(uint64_t* keys, uint32_t* values, uint64_t* scratchKeys, uint32_t* scratchValues,
+                                 uint32_t count, uint32_t keyBits, bool* resultInScratch) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (keyBits < 1 || keyBits > 64) return Status::error(ErrorCode::InvalidArgument, "keyBits must be 1..64");
+    if (count > kMaxPairs) return Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 pairs");
+    if (count > 0 && (keys == nullptr || values == nullptr || scratchKeys == nullptr || scratchValues == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "null buffer");
+    }
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    const uint32_t passes = kern::sortPasses(keyBits);
+    const uint32_t parts = kern::sortPartitions(count);
+    const uint64_t words = kern::sortWorkspaceWords(count, keyBits);
+    if (words > sortWorkWords_) {
+        if (!(s = sortWork_.reserve(words * 4, "pair.sortWorkspace")).ok()) return s;
+        sortWorkWords_ = words;
+    }
+    uint32_t* work = sortWork_.as<uint32_t>();
+    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512, work + passes * 512 + uint64_t{passes} * parts * 256,
+                           counters_.as<uint32_t>() + 4};
+    const bool alt = kern::sortPairs(keys, values, scratchKeys, scratchValues, count, kern::KeyMode::Plain, 0, keyBits,
+                                     ws, stream_);
+    if (resultInScratch != nullptr) *resultInScratch = alt;
+    return check(hipGetLastError(), "radix sort");
+}
+
+Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* needed) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    const void* src = nullptr;
+    size_t size = 0;
+    uint32_t counts[4] = {lastSplats_, lastVisible_, lastPairs_, lastTiles_};
+    switch (which) {
+    case 0: size = sizeof counts; break;
+    case 1: src = touched_.get(); size = size_t{lastSplats_} * 4; break;
+    case 2: src = offsets_.get(); size = size_t{lastSplats_} * 4; break;
+    case 3: src = proj_.get(); size = size_t{lastSplats_} * sizeof(ProjRecord); break;
+    case 4: src = keys_[lastSorted_].get(); size = size_t{lastPairs_} * 8; break;
+    case 5: src = values_[lastSorted_].get(); size = size_t{lastPairs_} * 4; break;
+    case 6: src = ranges_.get(); size = size_t{lastTiles_} * 8; break;
+    case 7: src = lastCount_.get(); size = size_t{lastPixels_} * 4; break;
+    case 8: src = finalT_.get(); size = size_t{lastPixels_} * 4; break;
+    case 9: src = rects_.get(); size = size_t{lastSplats_} * 8; break;
+    default: return Status::error(ErrorCode::InvalidArgument, "unknown debug buffer " + std::to_string(which));
+    }
+    if (needed != nullptr) *needed = size;
+    if (bytes == 0 || host == nullptr) return {};
+    if (bytes < size) return Status::error(ErrorCode::InvalidArgument, "the host buffer is too small");
+    if (which == 0) {
+        std::memcpy(host, counts, size);
+        return {};
+    }
+    if (size == 0) return {};
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = check(hipMemcpyAsync(host, src, size, hipMemcpyDeviceToHost, stream_), "debug copy")).ok()) return s;
+    return check(hipStreamSynchronize(stream_), "debug copy");
+}
+
+}   // namespace lrt

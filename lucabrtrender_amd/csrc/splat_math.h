@@ -1,0 +1,260 @@
+// Device arithmetic of the splat path: exp/log, fp16 and quaternion decode,
+// the footprint cut and the exact tile test, harmonics and the SplatEdit.
+//
+// Every function here feeds the tile counts, the depth keys or the projected
+// record, all of which must agree bit for bit with the CPU restatement of the
+// reference. So: no FMA contraction (the library is built with
+// -ffp-contract=off), IEEE division and sqrt (-fhip-fp32-correctly-rounded-
+// divide-sqrt), operations in the reference's source order, and exp/log by a
+// fixed published method instead of the device library's, whose last bits
+// differ from any host library's.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstdint>
+
+#include "frame.h"
+
+namespace lrt::dev {
+
+__device__ __forceinline__ float bitsToFloat(uint32_t u) { return __uint_as_float(u); }
+__device__ __forceinline__ uint32_t floatToBits(float f) { return __float_as_uint(f); }
+__device__ __forceinline__ float fminS(float a, float b) { return a < b ? a : b; }
+__device__ __forceinline__ float fmaxS(float a, float b) { return a > b ? a : b; }
+__device__ __forceinline__ float clampS(float v, float lo, float hi) { return fminS(fmaxS(v, lo), hi); }
+
+// exp(x): Cody & Waite reduction x = k ln2 + r with ln2 split in a 15-bit head
+// (so k * head is exact for |k| < 256) and a tail, then the degree-7 Taylor
+// polynomial of e^r in Horner form (|r| <= ln2/2, truncation < 4e-9 relative),
+// then an exact scaling by 2^k (two steps when the result is subnormal).
+__device__ __forceinline__ float expF(float x) {
+    if (x != x) return x;
+    if (x > 88.72283935546875f) return __int_as_float(0x7F800000);
+    if (x < -103.97208404541016f) return 0.0f;
+    const float kf = __builtin_rintf(x * 1.4426950408889634f);
+    int k = static_cast<int>(kf);
+    float r = x - kf * 0.693145751953125f;
+    r = r - kf * 1.428606765330187e-06f;
+    float p = 1.0f / 5040.0f;
+    p = p * r + 1.0f / 720.0f;
+    p = p * r + 1.0f / 120.0f;
+    p = p * r + 1.0f / 24.0f;
+    p = p * r + 1.0f / 6.0f;
+    p = p * r + 0.5f;
+    p = p * r + 1.0f;
+    p = p * r + 1.0f;
+    if (k > 127) {
+        p = p * 2.0f;
+        k -= 1;
+    }
+    if (k >= -126) return p * bitsToFloat(static_cast<uint32_t>(k + 127) << 23);
+    return (p * bitsToFloat(static_cast<uint32_t>(k + 64 + 127) << 23)) * bitsToFloat(static_cast<uint32_t>(63) << 23);
+}
+
+// log(x) = e ln2 + log(m), m in [sqrt(1/2), sqrt(2)]; log(m) = 2 atanh(s) with
+// s = f/(2+f), f = m - 1, written as f - s (f - R(s^2)) where R is the atanh
+// series' tail 2z/3 + 2z^2/5 + ... + 2z^5/11.
+__device__ __forceinline__ float logF(float x) {
+    if (x != x) return x;
+    if (x < 0.0f) return __int_as_float(0x7FC00000);
+    if (x == 0.0f) return __int_as_float(0xFF800000);
+    uint32_t b = floatToBits(x);
+    if (b == 0x7F800000u) return x;
+    int e = 0;
+    if (b < 0x00800000u) {
+        x = x * 33554432.0f;
+        b = floatToBits(x);
+        e = -25;
+    }
+    e += static_cast<int>(b >> 23) - 127;
+    float m = bitsToFloat((b & 0x007FFFFFu) | 0x3F800000u);
+    if (m > 1.41421353816986083984375f) {
+        m = m * 0.5f;
+        e += 1;
+    }
+    const float f = m - 1.0f;
+    const float s = f / (2.0f + f);
+    const float z = s * s;
+    const float r = z * (2.0f / 3.0f + z * (2.0f / 5.0f + z * (2.0f / 7.0f + z * (2.0f / 9.0f + z * (2.0f / 11.0f)))));
+    const float lm = f - s * (f - r);
+    const float ef = static_cast<float>(e);
+    return ef * 0.693145751953125f + (ef * 1.428606765330187e-06f + lm);
+}
+
+// fp16 -> fp32, exact.
+__device__ __forceinline__ float halfToFloat(uint32_t h) {
+    const uint32_t sign = (h & 0x8000u) << 16;
+    const uint32_t e = (h >> 10) & 31u;
+    const uint32_t m = h & 1023u;
+    if (e == 0) return bitsToFloat(floatToBits(static_cast<float>(m) * 5.9604644775390625e-08f) | sign);
+    if (e == 31) return bitsToFloat(sign | 0x7F800000u | (m << 13));
+    return bitsToFloat(sign | ((e + 112u) << 23) | (m << 13));
+}
+__device__ __forceinline__ float lowHalf(uint32_t w) { return halfToFloat(w & 0xFFFFu); }
+__device__ __forceinline__ float highHalf(uint32_t w) { return halfToFloat(w >> 16); }
+
+// Smallest-three quaternion (reference packing.slang:52-69): three 10-bit
+// components in [-sqrt(1/2), sqrt(1/2)], the largest from unit length.
+__device__ __forceinline__ void decodeQuaternion(uint32_t word, float q[4]) {
+    const uint32_t largest = word & 3u;
+    const float kSqrtHalf = 0.70710678f;
+    float s0 = static_cast<float>((word >> 22) & 1023u);
+    float s1 = static_cast<float>((word >> 12) & 1023u);
+    float s2 = static_cast<float>((word >> 2) & 1023u);
+    s0 = (s0 / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    s1 = (s1 / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    s2 = (s2 / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    const float rest = __builtin_sqrtf(fmaxS(0.0f, 1.0f - ((s0 * s0 + s1 * s1) + s2 * s2)));
+    // Place the three stored components around the largest one.
+    q[0] = largest == 0 ? rest : s0;
+    q[1] = largest == 1 ? rest : (largest == 0 ? s0 : s1);
+    q[2] = largest == 2 ? rest : (largest <= 1 ? s1 : s2);
+    q[3] = largest == 3 ? rest : s2;
+}
+
+// The squared Mahalanobis radius a splat is cut at (frame.slang:105-107).
+__device__ __forceinline__ float cutoffPower(float alpha) {
+    return fminS(2.0f * logF(fmaxS(255.0f * alpha, 1.0001f)), 9.0f);
+}
+
+// Exact minimum of a dx^2 + 2b dx dy + c dy^2 over a box (frame.slang:113-130).
+__device__ __forceinline__ float minPowerOverBox(float a, float b, float c, float lox, float loy, float hix, float hiy) {
+    if (lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f) return 0.0f;
+    float best = 3.4e38f;
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const float x = e == 0 ? lox : hix;
+        const float y = c > 1e-12f ? clampS(-b * x / c, loy, hiy) : loy;
+        best = fminS(best, a * x * x + 2.0f * b * x * y + c * y * y);
+    }
+#pragma unroll
+    for (int e = 0; e < 2; ++e) {
+        const float y = e == 0 ? loy : hiy;
+        const float x = a > 1e-12f ? clampS(-b * y / a, lox, hix) : lox;
+        best = fminS(best, a * x * x + 2.0f * b * x * y + c * y * y);
+    }
+    return best;
+}
+
+// Does tile (tx, ty) hold a pixel centre inside the footprint (frame.slang:133-138)?
+__device__ __forceinline__ bool tileTouched(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
+                                            float c, float cutoff, uint32_t tx, uint32_t ty) {
+    const float fx = static_cast<float>(tx * kTile) + 0.5f;
+    const float fy = static_cast<float>(ty * kTile) + 0.5f;
+    const uint32_t ex = (tx + 1u) * kTile < width ? (tx + 1u) * kTile : width;
+    const uint32_t ey = (ty + 1u) * kTile < height ? (ty + 1u) * kTile : height;
+    const float lx = static_cast<float>(ex - 1u) + 0.5f;
+    const float ly = static_cast<float>(ey - 1u) + 0.5f;
+    return minPowerOverBox(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
+}
+
+// The footprint's tile rectangle, inclusive (frame.slang:142-161); empty
+// (x0 > x1) when it is off screen.
+__device__ __forceinline__ void tileRect(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
+                                         float c, float cutoff, uint32_t rect[4]) {
+    const float det = a * c - b * b;
+    const float inv = det > 1e-20f ? 1.0f / det : 0.0f;
+    const float ex = __builtin_sqrtf(fmaxS(cutoff * c * inv, 0.0f));
+    const float ey = __builtin_sqrtf(fmaxS(cutoff * a * inv, 0.0f));
+    const float x1 = cx - ex, y1 = cy - ey, x2 = cx + ex, y2 = cy + ey;
+    if (x2 < 0.0f || y2 < 0.0f || x1 >= static_cast<float>(width) || y1 >= static_cast<float>(height)) {
+        rect[0] = 1; rect[1] = 1; rect[2] = 0; rect[3] = 0;
+        return;
+    }
+    rect[0] = static_cast<uint32_t>(fmaxS(x1, 0.0f)) / kTile;
+    rect[1] = static_cast<uint32_t>(fmaxS(y1, 0.0f)) / kTile;
+    rect[2] = static_cast<uint32_t>(fminS(x2, static_cast<float>(width) - 1.0f)) / kTile;
+    rect[3] = static_cast<uint32_t>(fminS(y2, static_cast<float>(height) - 1.0f)) / kTile;
+}
+
+__device__ __forceinline__ uint32_t degreeOf(uint32_t rest) {
+    return rest >= 15 ? 3 : rest >= 8 ? 2 : rest >= 3 ? 1 : 0;
+}
+
+// The view-dependent part of the colour (reference sh.slang:25-50), the
+// coefficients already unpacked to fp32, rgb per basis: k[basis * 3 + channel].
+__device__ __forceinline__ void evaluateRest(const float* k, uint32_t degree, float x, float y, float z, float out[3]) {
+    const float kC1 = 0.4886025119029199f;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) out[ch] = 0.0f;
+    if (degree == 0) return;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) out[ch] = -kC1 * y * k[0 + ch] + kC1 * z * k[3 + ch] - kC1 * x * k[6 + ch];
+    if (degree < 2) return;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float xy = x * y, yz = y * z, xz = x * z;
+    const float c20 = 1.0925484305920792f, c21 = -1.0925484305920792f, c22 = 0.31539156525252005f,
+                c23 = -1.0925484305920792f, c24 = 0.5462742152960396f;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        out[ch] = out[ch] + (c20 * xy * k[9 + ch] + c21 * yz * k[12 + ch] + c22 * (2.0f * zz - xx - yy) * k[15 + ch] +
+                             c23 * xz * k[18 + ch] + c24 * (xx - yy) * k[21 + ch]);
+    if (degree < 3) return;
+    const float c30 = -0.5900435899266435f, c31 = 2.890611442640554f, c32 = -0.4570457994644658f,
+                c33 = 0.3731763325901154f, c34 = -0.4570457994644658f, c35 = 1.445305721320277f,
+                c36 = -0.5900435899266435f;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch)
+        out[ch] = out[ch] + (c30 * y * (3.0f * xx - yy) * k[24 + ch] + c31 * xy * z * k[27 + ch] +
+                             c32 * y * (4.0f * zz - xx - yy) * k[30 + ch] +
+                             c33 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy) * k[33 + ch] +
+                             c34 * x * (4.0f * zz - xx - yy) * k[36 + ch] + c35 * z * (xx - yy) * k[39 + ch] +
+                             c36 * x * (xx - 3.0f * yy) * k[42 + ch]);
+}
+
+struct EditFate {
+    bool removed;
+    bool graded;
+    float opacity;
+};
+
+// What a SplatEdit does to a splat (reference edit.slang:46-78).
+__device__ __forceinline__ EditFate editFate(const SplatEditParams& e, float px, float py, float pz, float opacity,
+                                             float sx, float sy, float sz) {
+    EditFate fate{false, false, opacity};
+    if (e.active == 0) return fate;
+    if (e.minOpacity > 0.0f && opacity < e.minOpacity) {
+        fate.removed = true;
+        return fate;
+    }
+    if (e.maxScale > 0.0f && fmaxS(fmaxS(__builtin_fabsf(sx), __builtin_fabsf(sy)), __builtin_fabsf(sz)) > e.maxScale) {
+        fate.removed = true;
+        return fate;
+    }
+    const float rx = px - e.centre[0], ry = py - e.centre[1], rz = pz - e.centre[2];
+    bool inside = e.shape == 1 ? ((rx * rx + ry * ry) + rz * rz) <= e.size[0] * e.size[0]
+                               : (__builtin_fabsf(rx) <= e.size[0] && __builtin_fabsf(ry) <= e.size[1] &&
+                                  __builtin_fabsf(rz) <= e.size[2]);
+    if (e.invert != 0) inside = !inside;
+    if ((e.mode == 0 && !inside) || (e.mode == 1 && inside)) {
+        fate.removed = true;
+        return fate;
+    }
+    if (inside) {
+        fate.graded = true;
+        fate.opacity = opacity * e.opacity;
+        fate.removed = !(fate.opacity >= 1.0f / 255.0f);
+    }
+    return fate;
+}
+
+// The grade (reference edit.slang:81-85).
+__device__ __forceinline__ void editGrade(const SplatEditParams& e, float c[3]) {
+    const float r = c[0] * e.tint[0] * e.brightness;
+    const float g = c[1] * e.tint[1] * e.brightness;
+    const float b = c[2] * e.tint[2] * e.brightness;
+    const float luma = (r * 0.2126f + g * 0.7152f) + b * 0.0722f;
+    c[0] = fmaxS(luma + (r - luma) * e.saturation, 0.0f);
+    c[1] = fmaxS(luma + (g - luma) * e.saturation, 0.0f);
+    c[2] = fmaxS(luma + (b - luma) * e.saturation, 0.0f);
+}
+
+// The reference's sRGB curve, extended past 1 (color.slang:7-10), with pow as
+// exp(2.4 log y) by the functions above.
+__device__ __forceinline__ float srgbToLinear(float c) {
+    c = fmaxS(c, 0.0f);
+    return c <= 0.04045f ? c / 12.92f : expF(2.4f * logF((c + 0.055f) / 1.055f));
+}
+
+}   // namespace lrt::dev

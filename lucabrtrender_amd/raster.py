@@ -1,0 +1,301 @@
+"""Python mirror of the reference's rasteriser interface over the C ABI.
+
+Names and argument meanings follow lucabRTrender's lrt::render
+(modules/render/include/lrt/render/{Camera,TileRasterizer}.h): ``Camera.looking_at``,
+``Lens``, ``projection_for``, ``RenderSettings``, ``SplatInstance``,
+``TileRasterizer.render`` returning ``FrameStats``, and errors carrying the
+reference's ErrorCode names. Device memory and streams come from torch; every
+computation runs in liblrt_hip.so.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from dataclasses import dataclass, field
+from typing import Optional, Sequence
+
+import numpy as np
+
+from . import _abi
+from ._abi import LrtError, lib
+
+IDENTITY = tuple(float(v) for v in np.eye(4).reshape(-1))
+
+
+def _mat(m) -> "C.Array":
+    a = np.asarray(m, dtype=np.float64).reshape(16)
+    return (C.c_double * 16)(*a.tolist())
+
+
+@dataclass
+class Lens:
+    """render::Lens (Camera.h:73-94); defaults are the reference's."""
+    orthographic: bool = False
+    focal: float = 50.0
+    haperture: float = 24.576
+    vaperture: float = 18.672
+    near_z: float = 0.1
+    far_z: float = 10000.0
+    window_translate: tuple = (0.0, 0.0)
+    window_scale: tuple = (1.0, 1.0)
+    window_roll: float = 0.0
+
+    def to_c(self) -> _abi.Lens:
+        out = _abi.Lens()
+        out.orthographic = 1 if self.orthographic else 0
+        out.focal, out.haperture, out.vaperture = self.focal, self.haperture, self.vaperture
+        out.near_z, out.far_z = self.near_z, self.far_z
+        out.window_translate[:] = list(self.window_translate)
+        out.window_scale[:] = list(self.window_scale)
+        out.window_roll = self.window_roll
+        return out
+
+
+@dataclass
+class Camera:
+    """render::Camera (Camera.h:96-103)."""
+    camera_to_world: tuple = IDENTITY
+    lens: Lens = field(default_factory=Lens)
+
+    @staticmethod
+    def looking_at(eye, target, up=(0.0, 1.0, 0.0)) -> "Camera":
+        c = _abi.CameraC()
+        lib.lrt_camera_looking_at((C.c_double * 3)(*eye), (C.c_double * 3)(*target), (C.c_double * 3)(*up),
+                                  C.byref(c))
+        return Camera(tuple(c.camera_to_world), Lens())
+
+    def to_c(self) -> _abi.CameraC:
+        c = _abi.CameraC()
+        c.camera_to_world[:] = list(np.asarray(self.camera_to_world, dtype=np.float64).reshape(16))
+        c.lens = self.lens.to_c()
+        return c
+
+
+def projection_for(camera: Camera, width: int, height: int) -> _abi.ProjectionC:
+    """projectionFor (modules/render/src/Camera.cpp:20-48)."""
+    out = _abi.ProjectionC()
+    code = lib.lrt_projection_for(C.byref(camera.to_c()), width, height, C.byref(out))
+    if code != _abi.LRT_OK:
+        raise LrtError(code, "projection_for")
+    return out
+
+
+@dataclass
+class SplatEdit:
+    """render::SplatEdit (TileRasterizer.h:158-179)."""
+    active: bool = False
+    shape: int = 0
+    mode: int = 2
+    centre: tuple = (0.0, 0.0, 0.0)
+    size: tuple = (1.0, 1.0, 1.0)
+    tint: tuple = (1.0, 1.0, 1.0)
+    saturation: float = 1.0
+    brightness: float = 1.0
+    opacity: float = 1.0
+    min_opacity: float = 0.0
+    max_scale: float = 0.0
+    invert: bool = False
+
+    def to_c(self) -> _abi.SplatEdit:
+        e = _abi.SplatEdit()
+        e.active = 1 if self.active else 0
+        e.shape, e.mode, e.invert = self.shape, self.mode, 1 if self.invert else 0
+        e.centre[:], e.size[:], e.tint[:] = list(self.centre), list(self.size), list(self.tint)
+        e.saturation, e.brightness, e.opacity = self.saturation, self.brightness, self.opacity
+        e.min_opacity, e.max_scale = self.min_opacity, self.max_scale
+        return e
+
+
+@dataclass
+class RenderSettings:
+    """render::RenderSettings (TileRasterizer.h:234-247) + dilation / filter3d."""
+    width: int = 1920
+    height: int = 1080
+    antialias: bool = True
+    max_sh_degree: int = 3
+    background: tuple = (0.0, 0.0, 0.0, 0.0)
+    depth_mode: int = 0
+    depth_threshold: float = 0.5
+    linearise: bool = True
+    time_stages: bool = False
+    dilation: float = 0.3
+    filter3d: bool = False
+
+    def to_c(self) -> _abi.RenderSettingsC:
+        s = _abi.RenderSettingsC()
+        s.width, s.height = self.width, self.height
+        s.antialias = 1 if self.antialias else 0
+        s.max_sh_degree = self.max_sh_degree
+        s.background[:] = list(self.background)
+        s.depth_mode, s.depth_threshold = self.depth_mode, self.depth_threshold
+        s.linearise = 1 if self.linearise else 0
+        s.time_stages = 1 if self.time_stages else 0
+        s.dilation = self.dilation
+        s.filter3d = 1 if self.filter3d else 0
+        return s
+
+
+class GpuSplats:
+    """scene::GpuSplats over torch device tensors (GpuClouds.h:41-76).
+
+    positions: float32 [N, 4]; shape: int32/uint32 [N, 4]; sh: [N, sh_words] or
+    None at degree 0; filter3d: float32 [N] or None.
+    """
+
+    def __init__(self, positions, shape, sh=None, rest_per_colour: int = 0, filter3d=None):
+        import torch
+        n = int(positions.shape[0])
+        if positions.dtype != torch.float32 or tuple(positions.shape) != (n, 4) or not positions.is_contiguous():
+            raise LrtError(3, "positions must be a contiguous float32 [N, 4] tensor")
+        if tuple(shape.shape) != (n, 4) or shape.element_size() != 4 or not shape.is_contiguous():
+            raise LrtError(3, "shape must be a contiguous 32-bit [N, 4] tensor")
+        words = 1 if rest_per_colour == 0 else (rest_per_colour * 3 + 1) // 2
+        if rest_per_colour:
+            if sh is None or tuple(sh.shape) != (n, words) or sh.element_size() != 4 or not sh.is_contiguous():
+                raise LrtError(3, f"sh must be a contiguous 32-bit [N, {words}] tensor")
+        if filter3d is not None and (tuple(filter3d.shape) != (n,) or filter3d.dtype != torch.float32):
+            raise LrtError(3, "filter3d must be float32 [N]")
+        for t in (positions, shape, sh, filter3d):
+            if t is not None and not t.is_cuda:
+                raise LrtError(3, "splat tensors must live on the GPU")
+        self.positions, self.shape, self.sh, self.filter3d = positions, shape, sh, filter3d
+        self.count, self.rest_per_colour, self.sh_words = n, rest_per_colour, words
+        self._c = _abi.Splats(positions.data_ptr(), shape.data_ptr(), sh.data_ptr() if sh is not None else None,
+                              filter3d.data_ptr() if filter3d is not None else None, n, rest_per_colour, words)
+
+    @property
+    def c(self) -> _abi.Splats:
+        return self._c
+
+
+@dataclass
+class SplatInstance:
+    """render::SplatInstance (TileRasterizer.h:181-196), without relighting."""
+    splats: Optional[GpuSplats]
+    object_to_world: tuple = IDENTITY
+    edit: SplatEdit = field(default_factory=SplatEdit)
+
+
+@dataclass
+class FrameStats:
+    """render::FrameStats (TileRasterizer.h:249-255)."""
+    splats: int = 0
+    visible: int = 0
+    pairs: int = 0
+    project_ms: float = 0.0
+    depth_sort_ms: float = 0.0
+    counts_ms: float = 0.0
+    emit_ms: float = 0.0
+    tile_sort_ms: float = 0.0
+    blend_ms: float = 0.0
+    total_ms: float = 0.0
+
+
+class RenderTargets:
+    """render::RenderTargets: colour float4 [H, W, 4] and depth [H, W], bottom row first."""
+
+    def __init__(self, width: int, height: int, device="cuda"):
+        import torch
+        self.width, self.height = width, height
+        self.colour = torch.empty((height, width, 4), dtype=torch.float32, device=device)
+        self.depth = torch.empty((height, width), dtype=torch.float32, device=device)
+
+
+def _instances_c(instances: Sequence[SplatInstance]):
+    arr = (_abi.SplatInstanceC * max(1, len(instances)))()
+    for k, inst in enumerate(instances):
+        if inst.splats is not None:
+            arr[k].splats = C.pointer(inst.splats.c)
+        arr[k].object_to_world[:] = list(np.asarray(inst.object_to_world, dtype=np.float64).reshape(16))
+        arr[k].edit = inst.edit.to_c()
+    return arr
+
+
+class TileRasterizer:
+    """render::TileRasterizer (TileRasterizer.h:265-322) on one HIP stream."""
+
+    def __init__(self, device: int = 0, stream=None):
+        import torch
+        self._ctx = C.c_void_p()
+        self.device = device
+        s = stream if stream is not None else torch.cuda.current_stream(device)
+        code = lib.lrt_context_create(device, C.c_void_p(s.cuda_stream), C.byref(self._ctx))
+        if code != _abi.LRT_OK:
+            raise LrtError(code, "lrt_context_create: " + lib.lrt_last_error(None).decode())
+        self._stream = s
+
+    @staticmethod
+    def create(device: int = 0, stream=None) -> "TileRasterizer":
+        return TileRasterizer(device, stream)
+
+    def close(self):
+        if self._ctx:
+            lib.lrt_context_destroy(self._ctx)
+            self._ctx = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def _check(self, code: int):
+        if code != _abi.LRT_OK:
+            raise LrtError(code, lib.lrt_last_error(self._ctx).decode())
+
+    def render(self, camera_or_projection, instances: Sequence[SplatInstance], settings: RenderSettings,
+               targets: RenderTargets) -> FrameStats:
+        """TileRasterizer::render (TileRasterizer.cpp:179-484), splat instances only."""
+        proj = camera_or_projection
+        if isinstance(proj, Camera):
+            proj = projection_for(proj, settings.width, settings.height)
+        arr = _instances_c(instances)
+        t = _abi.RenderTargetsC(targets.width, targets.height, targets.colour.data_ptr(),
+                                targets.depth.data_ptr() if targets.depth is not None else None)
+        st = _abi.FrameStatsC()
+        self._check(lib.lrt_raster_render(self._ctx, C.byref(proj), arr, len(instances), C.byref(settings.to_c()),
+                                          C.byref(t), C.byref(st)))
+        return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
+
+    def debug(self, which: int) -> np.ndarray:
+        """What the last render left on the device (LRT_DEBUG_*), copied to host."""
+        need = C.c_size_t()
+        self._check(lib.lrt_raster_debug(self._ctx, which, None, 0, C.byref(need)))
+        dtype = {_abi.DEBUG_PROJ: np.float32, _abi.DEBUG_KEYS: np.uint64, _abi.DEBUG_FINAL_T: np.float32}.get(
+            which, np.uint32)
+        out = np.empty(need.value // np.dtype(dtype).itemsize, dtype=dtype)
+        if need.value:
+            self._check(lib.lrt_raster_debug(self._ctx, which, out.ctypes.data, need.value, None))
+        return out
+
+    def prefix_sum(self, inp, out, total):
+        """PrefixSum::apply (PrefixSum.h:43-44) on torch uint32/int32 device tensors."""
+        self._check(lib.lrt_prefix_sum(self._ctx, inp.data_ptr(), out.data_ptr(), total.data_ptr(), inp.numel()))
+
+    def radix_sort(self, keys, values, scratch_keys, scratch_values, key_bits: int) -> bool:
+        """RadixSort::sort (RadixSort.h:56-57); returns True when the result is in the scratch pair."""
+        flag = C.c_int32()
+        self._check(lib.lrt_radix_sort(self._ctx, keys.data_ptr(), values.data_ptr(), scratch_keys.data_ptr(),
+                                       scratch_values.data_ptr(), keys.numel(), key_bits, C.byref(flag)))
+        return bool(flag.value)
+
+    def synchronize(self):
+        self._check(lib.lrt_context_synchronize(self._ctx))
+
+
+def frame_params_bytes(projection: _abi.ProjectionC, instance: SplatInstance, settings: RenderSettings,
+                       base: int = 0, count: int = 0, rest_per_colour: int = 0) -> bytes:
+    """The kernel constant block one instance would be drawn with (host only)."""
+    splats = _abi.Splats(None, None, None, None, count, rest_per_colour,
+                         1 if rest_per_colour == 0 else (rest_per_colour * 3 + 1) // 2)
+    inst = _abi.SplatInstanceC()
+    inst.splats = C.pointer(splats)
+    inst.object_to_world[:] = list(np.asarray(instance.object_to_world, dtype=np.float64).reshape(16))
+    inst.edit = instance.edit.to_c()
+    need = C.c_size_t()
+    lib.lrt_frame_params(C.byref(projection), C.byref(inst), C.byref(settings.to_c()), base, None, 0, C.byref(need))
+    buf = (C.c_char * need.value)()
+    code = lib.lrt_frame_params(C.byref(projection), C.byref(inst), C.byref(settings.to_c()), base, buf,
+                                need.value, None)
+    if code != _abi.LRT_OK:
+        raise LrtError(code, "lrt_frame_params")
+    return bytes(buf)

@@ -1,0 +1,80 @@
+"""Runs one frame through the HIP library and through the oracle on the same
+packed bytes, and returns both sides' buffers as host arrays."""
+from __future__ import annotations
+
+import numpy as np
+
+import oracle_lib as O
+from lucabrtrender_amd import _abi, raster
+
+
+def oracle_camera(cam: raster.Camera) -> O.OrCamera:
+    lens = cam.lens
+    return O.camera(np.array(cam.camera_to_world), focal=lens.focal, orthographic=lens.orthographic,
+                    haperture=lens.haperture, near_z=lens.near_z, far_z=lens.far_z,
+                    window_translate=lens.window_translate, window_scale=lens.window_scale,
+                    window_roll=lens.window_roll)
+
+
+def oracle_edit(e: raster.SplatEdit) -> O.OrEdit:
+    return O.edit(e.active, e.shape, e.mode, e.invert, e.centre, e.size, e.tint, e.saturation, e.brightness,
+                  e.opacity, e.min_opacity, e.max_scale)
+
+
+def oracle_settings(s: raster.RenderSettings) -> O.OrSettings:
+    return O.settings(s.width, s.height, s.antialias, s.max_sh_degree, s.background, s.depth_mode,
+                      s.depth_threshold, s.linearise, s.dilation, s.filter3d)
+
+
+def oracle_frames(cam, settings, instances):
+    """instances: list of (HostScene, object_to_world 4x4, raster.SplatEdit)."""
+    proj = O.projection_for(oracle_camera(cam), settings.width, settings.height)
+    return O.frames_for(proj, oracle_settings(settings),
+                        [(sc, np.asarray(o2w, dtype=np.float64), oracle_edit(ed)) for sc, o2w, ed in instances])
+
+
+def run_gpu(cam, settings, instances, rast=None):
+    import torch
+    rast = rast or raster.TileRasterizer()
+    dev = [sc.to_device() for sc, _, _ in instances]
+    inst = [raster.SplatInstance(d, tuple(np.asarray(o2w, dtype=np.float64).reshape(-1)), ed)
+            for d, (_, o2w, ed) in zip(dev, instances)]
+    targets = raster.RenderTargets(settings.width, settings.height)
+    stats = rast.render(cam, inst, settings, targets)
+    torch.cuda.synchronize()
+    out = dict(stats=stats,
+               counts=rast.debug(_abi.DEBUG_COUNTS),
+               touched=rast.debug(_abi.DEBUG_TOUCHED),
+               offsets=rast.debug(_abi.DEBUG_OFFSETS),
+               proj=rast.debug(_abi.DEBUG_PROJ).reshape(-1, 12),
+               keys=rast.debug(_abi.DEBUG_KEYS),
+               values=rast.debug(_abi.DEBUG_VALUES),
+               ranges=rast.debug(_abi.DEBUG_RANGES).reshape(-1, 2),
+               last_count=rast.debug(_abi.DEBUG_LAST_COUNT).reshape(settings.height, settings.width),
+               final_t=rast.debug(_abi.DEBUG_FINAL_T).reshape(settings.height, settings.width),
+               colour=targets.colour.cpu().numpy(),
+               depth=targets.depth.cpu().numpy())
+    return out
+
+
+def run_oracle(cam, settings, instances, threads=0):
+    frames, clouds, n = oracle_frames(cam, settings, instances)
+    out = O.raster(frames, clouds, n, threads=threads)
+    out["frames"], out["clouds"], out["n"] = frames, clouds, n
+    return out
+
+
+def assert_bit_exact(gpu, ora):
+    """Counts, projected records, sorted (tile, splat) pairs and ranges agree bit for bit."""
+    n = ora["touched"].size
+    assert gpu["counts"][0] == ora["splats"]
+    assert gpu["counts"][1] == ora["visible"], (gpu["counts"], ora["visible"])
+    assert gpu["counts"][2] == ora["pairs"], (gpu["counts"], ora["pairs"])
+    np.testing.assert_array_equal(gpu["touched"][:n], ora["touched"])
+    excl = np.concatenate([[0], np.cumsum(ora["touched"].astype(np.uint64))[:-1]]).astype(np.uint32)
+    np.testing.assert_array_equal(gpu["offsets"][:n], excl)
+    vis = ora["touched"] > 0
+    np.testing.assert_array_equal(gpu["proj"][:n][vis].view(np.uint32), ora["proj"][vis].view(np.uint32))
+    np.testing.assert_array_equal((gpu["keys"] >> np.uint64(32)).astype(np.uint32), ora["pair_tiles"])
+    np.testing.assert_array_equal(gpu["values"], ora["pair_splats"])
+    np.testing.assert_array_equal(gpu["ranges"], ora["ranges"])

@@ -1,0 +1,247 @@
+"""ctypes bindings of the oracle (oracle/liblrt_oracle.so): the CPU restatement
+of the reference rasteriser, used only as the checker."""
+from __future__ import annotations
+
+import ctypes as C
+import os
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+ORACLE_PATH = os.path.join(ROOT, "oracle", "liblrt_oracle.so")
+
+
+class OrCamera(C.Structure):
+    _fields_ = [("camera_to_world", C.c_double * 16), ("orthographic", C.c_int32), ("focal", C.c_double),
+                ("haperture", C.c_double), ("vaperture", C.c_double), ("near_z", C.c_double),
+                ("far_z", C.c_double), ("window_translate", C.c_double * 2), ("window_scale", C.c_double * 2),
+                ("window_roll", C.c_double)]
+
+
+class OrProjection(C.Structure):
+    _fields_ = [("world_to_view", C.c_double * 16), ("focal_x", C.c_double), ("focal_y", C.c_double),
+                ("centre_x", C.c_double), ("centre_y", C.c_double), ("near_z", C.c_double),
+                ("far_z", C.c_double), ("orthographic", C.c_int32), ("eye_world", C.c_double * 3)]
+
+
+class OrEdit(C.Structure):
+    _fields_ = [("active", C.c_uint32), ("shape", C.c_uint32), ("mode", C.c_uint32), ("invert", C.c_uint32),
+                ("centre", C.c_float * 3), ("size", C.c_float * 3), ("tint", C.c_float * 3),
+                ("saturation", C.c_float), ("brightness", C.c_float), ("opacity", C.c_float),
+                ("min_opacity", C.c_float), ("max_scale", C.c_float)]
+
+
+class OrFrame(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32),
+                ("focal_x", C.c_float), ("focal_y", C.c_float), ("centre_x", C.c_float), ("centre_y", C.c_float),
+                ("near_z", C.c_float), ("far_z", C.c_float), ("orthographic", C.c_uint32),
+                ("antialias", C.c_uint32), ("m", C.c_float * 12), ("eye", C.c_float * 3), ("count", C.c_uint32),
+                ("base", C.c_uint32), ("rest_per_colour", C.c_uint32), ("sh_words", C.c_uint32),
+                ("sh_limit", C.c_uint32), ("tile_bits", C.c_uint32), ("depth_mode", C.c_uint32),
+                ("depth_threshold", C.c_float), ("bg", C.c_float * 4), ("linearise", C.c_uint32),
+                ("dilation", C.c_float), ("use_filter3d", C.c_uint32), ("edit", OrEdit)]
+
+
+class OrSettings(C.Structure):
+    _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("antialias", C.c_int32),
+                ("max_sh_degree", C.c_uint32), ("background", C.c_float * 4), ("depth_mode", C.c_int32),
+                ("depth_threshold", C.c_float), ("linearise", C.c_int32), ("dilation", C.c_float),
+                ("filter3d", C.c_int32)]
+
+
+class OrEncoding(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ("floats_per_record", "x", "y", "z", "opacity", "scale0", "scale1",
+                                          "scale2", "rot_w", "rot_x", "rot_y", "rot_z", "dc0", "dc1", "dc2",
+                                          "rest_base", "file_per_colour", "rest_colour_outer")]
+
+
+class OrCloud(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("shape", C.c_void_p), ("sh", C.c_void_p), ("filter3d", C.c_void_p)]
+
+
+class OrStats(C.Structure):
+    _fields_ = [("splats", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32)]
+
+
+P = C.c_void_p
+_SIGS = {
+    "or_expf": (C.c_float, [C.c_float]),
+    "or_logf": (C.c_float, [C.c_float]),
+    "or_half_to_float": (C.c_float, [C.c_uint16]),
+    "or_float_to_half": (C.c_uint16, [C.c_float]),
+    "or_encode_quaternion": (C.c_uint32, [C.POINTER(C.c_float)]),
+    "or_decode_quaternion": (None, [C.c_uint32, C.POINTER(C.c_float)]),
+    "or_pack_halves": (C.c_uint32, [C.c_float, C.c_float]),
+    "or_decode_records": (C.c_uint32, [P, C.c_uint32, C.POINTER(OrEncoding), C.c_uint32, P, P, P, C.c_uint32]),
+    "or_look_at": (None, [C.POINTER(C.c_double)] * 4),
+    "or_mat_mul": (None, [C.POINTER(C.c_double)] * 3),
+    "or_inverse_affine": (None, [C.POINTER(C.c_double)] * 2),
+    "or_translation": (None, [C.POINTER(C.c_double)] * 2),
+    "or_scaling": (None, [C.POINTER(C.c_double)] * 2),
+    "or_rotation_y": (None, [C.c_double, C.POINTER(C.c_double)]),
+    "or_rotation_z": (None, [C.c_double, C.POINTER(C.c_double)]),
+    "or_projection_for": (None, [C.POINTER(OrCamera), C.c_uint32, C.c_uint32, C.POINTER(OrProjection)]),
+    "or_bits_for": (C.c_uint32, [C.c_uint32]),
+    "or_frame_for": (None, [C.POINTER(OrProjection), C.POINTER(OrSettings), C.POINTER(C.c_double),
+                            C.POINTER(OrEdit), C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32,
+                            C.POINTER(OrFrame)]),
+    "or_project": (None, [C.POINTER(OrFrame), P, P, P, P, P, P, P, P, P, C.c_int]),
+    "or_scan_exclusive": (C.c_uint64, [P, P, C.c_uint32]),
+    "or_radix_sort": (None, [P, P, C.c_uint32, C.c_uint32]),
+    "or_raster": (C.c_uint32, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, P, P, P, C.c_uint32,
+                               P, P, P, P, P, C.POINTER(OrStats), C.c_int]),
+    "or_blend": (None, [C.POINTER(OrFrame), P, P, P, P, P, P, P, C.c_int]),
+    "or_reference_render_f32": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
+                                       C.c_int]),
+    "or_reference_render_f64": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
+                                       C.c_int]),
+    "or_compare_images": (None, [P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
+                                 C.POINTER(C.c_uint64)]),
+    "or_srgb_to_linear": (C.c_float, [C.c_float]),
+}
+
+
+def load():
+    if not os.path.exists(ORACLE_PATH):
+        raise RuntimeError(f"{ORACLE_PATH} is not built: run `make -C oracle`")
+    lib = C.CDLL(ORACLE_PATH)
+    for name, (res, args) in _SIGS.items():
+        fn = getattr(lib, name)
+        fn.restype = res
+        fn.argtypes = args
+    return lib
+
+
+lib = load()
+
+
+def dbl(values) -> "C.Array":
+    a = [float(v) for v in np.asarray(values, dtype=np.float64).reshape(-1)]
+    return (C.c_double * len(a))(*a)
+
+
+def ptr(a: np.ndarray):
+    return a.ctypes.data if a is not None else None
+
+
+def camera(camera_to_world, focal=50.0, orthographic=False, haperture=24.576, near_z=0.1, far_z=10000.0,
+           window_translate=(0.0, 0.0), window_scale=(1.0, 1.0), window_roll=0.0) -> OrCamera:
+    c = OrCamera()
+    c.camera_to_world[:] = [float(v) for v in np.asarray(camera_to_world, dtype=np.float64).reshape(16)]
+    c.orthographic = 1 if orthographic else 0
+    c.focal, c.haperture, c.vaperture, c.near_z, c.far_z = focal, haperture, 18.672, near_z, far_z
+    c.window_translate[:] = list(window_translate)
+    c.window_scale[:] = list(window_scale)
+    c.window_roll = window_roll
+    return c
+
+
+def look_at(eye, target, up=(0.0, 1.0, 0.0)) -> np.ndarray:
+    out = (C.c_double * 16)()
+    lib.or_look_at(dbl(eye), dbl(target), dbl(up), out)
+    return np.array(out[:], dtype=np.float64)
+
+
+def projection_for(cam: OrCamera, width: int, height: int) -> OrProjection:
+    out = OrProjection()
+    lib.or_projection_for(C.byref(cam), width, height, C.byref(out))
+    return out
+
+
+def settings(width, height, antialias=True, max_sh_degree=3, background=(0, 0, 0, 0), depth_mode=0,
+             depth_threshold=0.5, linearise=True, dilation=0.3, filter3d=False) -> OrSettings:
+    s = OrSettings()
+    s.width, s.height, s.antialias, s.max_sh_degree = width, height, 1 if antialias else 0, max_sh_degree
+    s.background[:] = [float(v) for v in background]
+    s.depth_mode, s.depth_threshold, s.linearise = depth_mode, depth_threshold, 1 if linearise else 0
+    s.dilation, s.filter3d = dilation, 1 if filter3d else 0
+    return s
+
+
+def edit(active=False, shape=0, mode=2, invert=False, centre=(0, 0, 0), size=(1, 1, 1), tint=(1, 1, 1),
+         saturation=1.0, brightness=1.0, opacity=1.0, min_opacity=0.0, max_scale=0.0) -> OrEdit:
+    e = OrEdit()
+    e.active, e.shape, e.mode, e.invert = 1 if active else 0, shape, mode, 1 if invert else 0
+    e.centre[:], e.size[:], e.tint[:] = [float(v) for v in centre], [float(v) for v in size], \
+        [float(v) for v in tint]
+    e.saturation, e.brightness, e.opacity = saturation, brightness, opacity
+    e.min_opacity, e.max_scale = min_opacity, max_scale
+    return e
+
+
+def frames_for(projection: OrProjection, s: OrSettings, instances):
+    """instances: list of (scene, object_to_world, OrEdit or None). Returns OrFrame array and clouds."""
+    n = len(instances)
+    frames = (OrFrame * max(1, n))()
+    clouds = (OrCloud * max(1, n))()
+    base = 0
+    for k, (scene, o2w, ed) in enumerate(instances):
+        lib.or_frame_for(C.byref(projection), C.byref(s), dbl(o2w), C.byref(ed) if ed is not None else None,
+                         scene.count, base, scene.rest_per_colour, scene.sh_words, C.byref(frames[k]))
+        clouds[k].positions = ptr(scene.positions)
+        clouds[k].shape = ptr(scene.shape)
+        clouds[k].sh = ptr(scene.sh) if scene.sh is not None else None
+        clouds[k].filter3d = ptr(scene.filter3d) if scene.filter3d is not None else None
+        base += scene.count
+    return frames, clouds, n
+
+
+def raster(frames, clouds, n, threads=0, want_image=True):
+    """The reference's whole frame restated: dict of host arrays."""
+    total = sum(frames[k].count for k in range(n))
+    f0 = frames[0]
+    tiles = f0.tiles_x * f0.tiles_y
+    px = f0.width * f0.height
+    proj = np.zeros((max(total, 1), 12), np.float32)
+    touched = np.zeros(max(total, 1), np.uint32)
+    keys = np.zeros(max(total, 1), np.uint32)
+    ranges = np.zeros(tiles * 2, np.uint32)
+    colour = np.zeros((f0.height, f0.width, 4), np.float32) if want_image else None
+    depth = np.zeros((f0.height, f0.width), np.float32) if want_image else None
+    last = np.zeros((f0.height, f0.width), np.uint32) if want_image else None
+    final_t = np.zeros((f0.height, f0.width), np.float32) if want_image else None
+    stats = OrStats()
+    # First pass sizes the pairs.
+    pairs = lib.or_raster(frames, clouds, n, ptr(proj), ptr(touched), ptr(keys), None, None, 0, None, None, None,
+                          None, None, C.byref(stats), threads)
+    tiles_out = np.zeros(max(pairs, 1), np.uint32)
+    splats_out = np.zeros(max(pairs, 1), np.uint32)
+    lib.or_raster(frames, clouds, n, ptr(proj), ptr(touched), ptr(keys), ptr(tiles_out), ptr(splats_out), pairs,
+                  ptr(ranges), ptr(colour), ptr(depth), ptr(last), ptr(final_t), C.byref(stats), threads)
+    del px
+    return dict(proj=proj[:total], touched=touched[:total], depth_keys=keys[:total], pair_tiles=tiles_out[:pairs],
+                pair_splats=splats_out[:pairs], ranges=ranges.reshape(-1, 2), colour=colour, depth=depth,
+                last_count=last, final_t=final_t, visible=stats.visible, pairs=pairs, splats=stats.splats)
+
+
+def reference(frames, clouds, n, f64=False, pixels=None, threads=0):
+    """The brute force; pixels: None (whole image) or (xs, ys) uint32 arrays."""
+    f0 = frames[0]
+    dt = np.float64 if f64 else np.float32
+    fn = lib.or_reference_render_f64 if f64 else lib.or_reference_render_f32
+    if pixels is None:
+        colour = np.zeros((f0.height, f0.width, 4), dt)
+        depth = np.zeros((f0.height, f0.width), dt)
+        fn(frames, clouds, n, None, None, 0, ptr(colour), ptr(depth), threads)
+    else:
+        xs = np.ascontiguousarray(pixels[0], dtype=np.uint32)
+        ys = np.ascontiguousarray(pixels[1], dtype=np.uint32)
+        colour = np.zeros((xs.size, 4), dt)
+        depth = np.zeros(xs.size, dt)
+        fn(frames, clouds, n, ptr(xs), ptr(ys), xs.size, ptr(colour), ptr(depth), threads)
+    return colour, depth
+
+
+def compare_images(a: np.ndarray, b: np.ndarray):
+    a = np.ascontiguousarray(a, dtype=np.float32)
+    b = np.ascontiguousarray(b, dtype=np.float32)
+    h, w = a.shape[0], a.shape[1]
+    p99, mx, over2 = C.c_uint32(), C.c_uint32(), C.c_uint64()
+    lib.or_compare_images(ptr(a), ptr(b), w, h, C.byref(p99), C.byref(mx), C.byref(over2))
+    return p99.value, mx.value, over2.value
+
+
+def linear_error(a: np.ndarray, b: np.ndarray):
+    """p99 and max of the absolute per-channel difference in linear units."""
+    d = np.abs(np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64)).reshape(-1)
+    return float(np.quantile(d, 0.99)), float(d.max())
