@@ -1,0 +1,229 @@
+#!/usr/bin/env python3
+"""Forward frames/s of the splat tile rasteriser on one MI355X.
+
+Workload: the 3,000,000-splat clustered scene at 1920x1080 (256 anisotropic
+clusters, SH degree 3 in fp16, Mip-Splatting 3D filter on, 2D filter 0.1,
+camera on an orbit of radius 15), synthetic and seeded, resident in HBM before
+the timed region. A step is one whole frame through the C ABI: preprocess,
+scan, duplication into 64-bit keys, onesweep sort, ranges, blend, including
+the frame's own pair-count readback and final wait.
+
+Prints one JSON line: frames/s, per-kernel GPU times from HIP events on the
+rasteriser's stream with each stage's algorithmic bytes and fraction of HBM
+peak, the dominant kernel's roofline, sort Gkeys/s, and the CPU baseline (the
+oracle's restatement of the reference forward path on the host cores).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0          # MI355X HBM3E
+FP32_PEAK_TFLOPS = 157.3       # MI355X vector FP32
+METRIC = "fwd frames/s @3M splats 1080p; fwd+bwd ms/view; sort Gkeys/s; %HBM peak"
+STAGES = ["project", "scan", "duplicate", "sort_histogram", "sort_passes", "ranges", "blend"]
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes):
+    """Algorithmic HBM bytes of each kernel for one frame."""
+    return {
+        # every splat's packed record, harmonics and 3D filter read; record (48),
+        # rect (8) and count (4) written
+        "project": n * (32 + 4 + 4 * sh_words) + n * 60,
+        # one u32 read and one written per splat
+        "scan": n * 8,
+        # the visible splats' record, rect, count and offset read; key + value written per pair
+        "duplicate": visible * (16 + 16 + 8 + 4 + 4) + pairs * 12,
+        # every key read once
+        "sort_histogram": pairs * 8,
+        # each pass reads and writes a 12-byte pair
+        "sort_passes": pairs * 24 * passes,
+        # keys read, two u32 a tile written
+        "ranges": pairs * 8 + tiles * 8,
+        # each pair's slot and record read once a tile, the pixel out (colour 16 + depth 4 + state 8)
+        "blend": pairs * (4 + 48) + pixels * 28,
+    }
+
+
+def copy_bandwidth_gbs(torch):
+    """Achieved device-to-device copy bandwidth (read + write bytes / time)."""
+    n = 1 << 28   # 1 GiB of float32
+    a = torch.empty(n, dtype=torch.float32, device="cuda")
+    b = torch.empty_like(a)
+    for _ in range(3):
+        b.copy_(a)
+    torch.cuda.synchronize()
+    start, stop = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record()
+    reps = 10
+    for _ in range(reps):
+        b.copy_(a)
+    stop.record()
+    torch.cuda.synchronize()
+    ms = start.elapsed_time(stop) / reps
+    del a, b
+    return 2 * n * 4 / (ms * 1e-3) / 1e9
+
+
+def cpu_baseline(cfg, settings, seconds_budget=15.0):
+    """The oracle's forward frame on the host cores, a bounded number of frames."""
+    import ctypes as C
+
+    import numpy as np
+
+    sys.path.insert(0, os.path.join(ROOT, "tests"))
+    import harness as H
+    import oracle_lib as O
+
+    try:
+        cores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        cores = os.cpu_count() or 1
+    threads = max(1, min(16, cores))
+    from lucabrtrender_amd import raster
+    frames, clouds, n = H.oracle_frames(cfg.camera, settings, [(cfg.scene, np.eye(4), raster.SplatEdit())])
+    f0 = frames[0]
+    total = cfg.scene.count
+    proj = np.zeros((total, 12), np.float32)
+    touched = np.zeros(total, np.uint32)
+    keys = np.zeros(total, np.uint32)
+    tiles = f0.tiles_x * f0.tiles_y
+    ranges = np.zeros(tiles * 2, np.uint32)
+    colour = np.zeros((f0.height, f0.width, 4), np.float32)
+    depth = np.zeros((f0.height, f0.width), np.float32)
+    stats = O.OrStats()
+
+    def frame():
+        return O.lib.or_raster(frames, clouds, n, O.ptr(proj), O.ptr(touched), O.ptr(keys), None, None, 0,
+                               O.ptr(ranges), O.ptr(colour), O.ptr(depth), None, None, C.byref(stats), threads)
+
+    frame()   # warm-up
+    done, start = 0, time.perf_counter()
+    while True:
+        frame()
+        done += 1
+        elapsed = time.perf_counter() - start
+        if elapsed >= seconds_budget or done >= 200:
+            break
+    model = ""
+    try:
+        with open("/proc/cpuinfo") as fh:
+            for line in fh:
+                if line.startswith("model name"):
+                    model = line.split(":", 1)[1].strip()
+                    break
+    except OSError:
+        pass
+    return {"value": done / elapsed, "unit": "frames/s", "cores": threads, "kind": "port",
+            "sample": f"{done} whole frames of the same 3M-splat 1080p scene ({elapsed:.1f} s), the oracle's C "
+                      f"restatement of the reference forward path (project, 24-bit depth sort, emit, tile "
+                      f"sort, ranges, blend) on {threads} OpenMP threads, {model}"}
+
+
+def main():
+    ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=10)
+    ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py measures one GPU: this build runs no multi-GPU path (asked for --gpus {args.gpus}); "
+              "run with --gpus 1", file=sys.stderr)
+        sys.exit(2)
+
+    import numpy as np
+    import torch
+
+    from lucabrtrender_amd import _abi, raster, scenes
+
+    t0 = time.perf_counter()
+    cfg = scenes.config(2, count=args.count)
+    log(f"scene: {cfg.scene.count} splats generated in {time.perf_counter() - t0:.1f} s")
+    settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
+                                     filter3d=cfg.filter3d)
+    dev = cfg.scene.to_device()
+    inst = [raster.SplatInstance(dev)]
+    targets = raster.RenderTargets(cfg.width, cfg.height)
+    rast = raster.TileRasterizer()
+    proj = raster.projection_for(cfg.camera, cfg.width, cfg.height)
+
+    for _ in range(args.warmup):
+        rast.render(proj, inst, settings, targets)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(args.steps):
+        stats = rast.render(proj, inst, settings, targets)
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - start
+    fps = args.steps / elapsed
+    log(f"{stats.splats} splats ({stats.visible} visible), {stats.pairs} pairs: {fps:.1f} frames/s")
+
+    # Per-kernel GPU time: the same frames with HIP events between the kernels.
+    timed = raster.RenderSettings(**{**settings.__dict__, "time_stages": True})
+    reps = max(10, min(args.steps, 50))
+    acc = np.zeros(8)
+    for _ in range(reps):
+        rast.render(proj, inst, timed, targets)
+        acc += rast.debug(_abi.DEBUG_KERNEL_MS)
+    kernel_ms = acc / reps
+    counts = rast.debug(_abi.DEBUG_COUNTS)
+    n, visible, pairs, tiles = (int(v) for v in counts)
+    passes = (24 + max(1, int(np.ceil(np.log2(max(tiles, 2))))) + 7) // 8
+    sbytes = stage_bytes(n, visible, pairs, tiles, cfg.width * cfg.height, cfg.scene.sh_words, passes)
+    stages = {}
+    for k, name in enumerate(STAGES):
+        ms = float(kernel_ms[k])
+        gbs = sbytes[name] / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        stages[name] = {"ms": round(ms, 4), "bytes": int(sbytes[name]), "GB/s": round(gbs, 1),
+                        "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
+    dominant = max(STAGES, key=lambda s: stages[s]["ms"])
+    sort_ms = stages["sort_histogram"]["ms"] + stages["sort_passes"]["ms"]
+    roofline = {"kernel": dominant, "bound": "hbm", "achieved": stages[dominant]["GB/s"], "peak": HBM_PEAK_GBS,
+                "unit": "GB/s", "frac": stages[dominant]["hbm_frac"], "traffic": None}
+    copy_gbs = copy_bandwidth_gbs(torch)
+
+    cpu = None
+    if args.cpu_seconds > 0:
+        log("CPU baseline ...")
+        cpu = cpu_baseline(cfg, settings, args.cpu_seconds)
+
+    out = {
+        "metric": METRIC,
+        "value": round(fps, 2),
+        "unit": "frames/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(1e3 * elapsed / args.steps, 4),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic: seeded 3M-splat clustered scene, packed fp16 shape/harmonics",
+        "config": {"workload": "scene-fwd-3M-1080p", "splats": n, "visible": visible, "pairs": pairs,
+                   "width": cfg.width, "height": cfg.height, "tiles": tiles, "sh_degree": 3, "filter3d": True,
+                   "dilation": cfg.dilation, "sort_passes": passes},
+        "stages": stages,
+        "gpu_ms_per_frame": round(float(kernel_ms[7]), 4),
+        "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3) if sort_ms > 0 else None,
+        "copy_GB/s": round(copy_gbs, 1),
+        "roofline": roofline,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

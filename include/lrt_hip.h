@@ -146,7 +146,9 @@ typedef struct {
 
 /* render::FrameStats (TileRasterizer.h:249-255). depth_sort_ms is 0: one
  * 64-bit (tile, depth) sort replaces the reference's depth sort + tile sort
- * and is reported as tile_sort_ms. */
+ * and is reported as tile_sort_ms. total_ms is the call's wall clock, as the
+ * reference's; the stage figures (with time_stages) are GPU time from HIP
+ * events, counts_ms including the pair-count readback's wait. */
 typedef struct {
     uint32_t splats, visible, pairs;
     double project_ms, depth_sort_ms, counts_ms, emit_ms, tile_sort_ms, blend_ms, total_ms;
@@ -187,7 +189,10 @@ typedef enum {
     LRT_DEBUG_RANGES = 6,        /* u32[2] a tile: [start, end) */
     LRT_DEBUG_LAST_COUNT = 7,    /* u32 a pixel: pairs walked to the last contributor */
     LRT_DEBUG_FINAL_T = 8,       /* float a pixel: final transmittance */
-    LRT_DEBUG_RECTS = 9          /* u32[2] a slot: x0 | y0 << 16, x1 | y1 << 16 */
+    LRT_DEBUG_RECTS = 9,         /* u32[2] a slot: x0 | y0 << 16, x1 | y1 << 16 */
+    LRT_DEBUG_KERNEL_MS = 10     /* float[8], last render with time_stages: GPU ms of project,
+                                    scan, duplicate, sort histogram, sort passes, ranges,
+                                    blend, and their sum (HIP events on the context's stream) */
 } lrt_debug_buffer;
 lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed);
 

@@ -19,7 +19,7 @@ STATUS_NAMES = {
 }
 
 DEBUG_COUNTS, DEBUG_TOUCHED, DEBUG_OFFSETS, DEBUG_PROJ, DEBUG_KEYS, DEBUG_VALUES, DEBUG_RANGES, \
-    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS = range(10)
+    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS, DEBUG_KERNEL_MS = range(11)
 
 
 class Lens(C.Structure):

@@ -60,8 +60,10 @@ uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits);
 // Stable LSD radix sort, eight bits a pass, one onesweep launch a pass. The
 // result is in (keys, values) after an even number of passes and in
 // (keysAlt, valuesAlt) after an odd one; returns true for the latter.
+// afterHistogram (may be null) is recorded between the histogram and the passes.
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
-               KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream);
+               KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
+               hipEvent_t afterHistogram = nullptr);
 
 // ---- ranges -----------------------------------------------------------------
 // ranges[t] = [start, end) of tile t's run in the sorted keys, (0, 0) when empty.

@@ -192,7 +192,10 @@ private:
     // Counters: [0] pairs, [1] visible, [2] overflow, [3] scan ticket, [4] sort error, [5] scan error.
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
-    hipEvent_t events_[7] = {};
+    hipEvent_t events_[9] = {};
+    // GPU milliseconds of the last timed render: project, scan, duplicate, sort
+    // histogram, sort passes, ranges, blend, their sum.
+    float kernelMs_[8] = {};
 
     // What the last render left.
     uint32_t lastSplats_ = 0, lastVisible_ = 0, lastPairs_ = 0, lastTiles_ = 0;

@@ -9,6 +9,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <chrono>
 #include <cstring>
 #include <string>
 
@@ -189,9 +190,14 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
 
     uint32_t* counters = counters_.as<uint32_t>();
     const bool timed = settings.timeStages;
+    const auto wallStart = std::chrono::steady_clock::now();
     FrameStats stats;
     stats.splats = splatCount;
-    if (timed) (void)hipEventRecord(events_[0], stream_);
+    // Event k marks kernel boundary k (see kernelMs_): recorded only when timed.
+    const auto mark = [&](int k) {
+        if (timed) (void)hipEventRecord(events_[k], stream_);
+    };
+    mark(0);
 
     // Project every instance into its own slots.
     uint32_t base = 0;
@@ -213,16 +219,17 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         SplatInstance none;
         first = frameParamsFor(projection, none, settings, 0);
     }
-    if (timed) (void)hipEventRecord(events_[1], stream_);
+    mark(1);
 
     // How many pairs and how many visible: one scan, one readback.
     kern::fill32(counters, 0u, 8, stream_);
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), splatCount, scanStatus_.as<uint64_t>(),
                         counters + 3, counters + 0, counters + 1, counters + 5, stream_);
+    mark(2);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "counts")).ok())
         return s;
     if (!(s = check(hipStreamSynchronize(stream_), "counts")).ok()) return s;
-    if (timed) (void)hipEventRecord(events_[2], stream_);
+    const auto countsDone = std::chrono::steady_clock::now();
     const uint32_t pairs = splatCount > 0 ? hostCounters_[0] : 0u;
     const uint32_t visible = splatCount > 0 ? hostCounters_[1] : 0u;
     if (hostCounters_[5] != 0) return Status::error(ErrorCode::InternalError, "the scan's look-back gave up");
@@ -234,33 +241,36 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     if (!(s = reservePairs(std::max(pairs, 1u), keyBits)).ok()) return s;
 
     // Duplicate into 64-bit keys, in slot order.
+    mark(3);
     kern::duplicate(first, splatCount, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
                     offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), pairCapacity_,
                     counters + 2, stream_);
-    if (timed) (void)hipEventRecord(events_[3], stream_);
+    mark(4);
 
     // One stable sort on (tile, depth >> 7), then the ranges.
     const uint32_t passes = kern::sortPasses(keyBits);
     const uint32_t parts = kern::sortPartitions(pairCapacity_);
     uint32_t* work = sortWork_.as<uint32_t>();
-    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512, work + passes * 512 + uint64_t{passes} * parts * 256,
-                           counters + 4};
+    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512,
+                           work + passes * 512 + uint64_t{passes} * parts * 256, counters + 4};
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairs, kern::KeyMode::TileDepth, 0, keyBits, ws,
-                                       stream_);
+                                       stream_, timed ? events_[5] : nullptr);
+    mark(6);
     const int sorted = inAlt ? 1 : 0;
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_);
-    if (timed) (void)hipEventRecord(events_[4], stream_);
+    mark(7);
 
-    kern::blend(first, splatCount, ranges_.as<uint2>(), values_[sorted].as<uint32_t>(), proj_.as<float4>(), targets.colour,
-                targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(), stream_);
-    if (timed) (void)hipEventRecord(events_[5], stream_);
+    kern::blend(first, splatCount, ranges_.as<uint2>(), values_[sorted].as<uint32_t>(), proj_.as<float4>(),
+                targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(), stream_);
+    mark(8);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "frame")).ok())
         return s;
     if (!(s = check(hipStreamSynchronize(stream_), "frame")).ok()) return s;
     if (!(s = check(hipGetLastError(), "frame")).ok()) return s;
     if (hostCounters_[2] != 0) return Status::error(ErrorCode::InternalError, "the duplication overran its pairs");
     if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
+    const auto wallEnd = std::chrono::steady_clock::now();
 
     lastSplats_ = splatCount;
     lastVisible_ = visible;
@@ -268,17 +278,23 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     lastTiles_ = tiles;
     lastPixels_ = uint64_t{settings.width} * settings.height;
     lastSorted_ = sorted;
+    stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
     if (timed) {
-        float ms[5] = {};
-        for (int k = 0; k < 5; ++k) (void)hipEventElapsedTime(&ms[k], events_[k], events_[k + 1]);
-        stats.projectMs = ms[0];
-        stats.countsMs = ms[1];
-        stats.emitMs = ms[2];
-        stats.tileSortMs = ms[3];
-        stats.blendMs = ms[4];
-        float all = 0.0F;
-        (void)hipEventElapsedTime(&all, events_[0], events_[5]);
-        stats.totalMs = all;
+        // project, scan, duplicate, sort histogram, sort passes, ranges, blend
+        static constexpr int kFrom[7] = {0, 1, 3, 4, 5, 6, 7};
+        for (int k = 0; k < 7; ++k) {
+            float ms = 0.0F;
+            (void)hipEventElapsedTime(&ms, events_[kFrom[k]], events_[kFrom[k] + 1]);
+            kernelMs_[k] = ms;
+        }
+        kernelMs_[7] = kernelMs_[0] + kernelMs_[1] + kernelMs_[2] + kernelMs_[3] + kernelMs_[4] + kernelMs_[5] +
+                       kernelMs_[6];
+        stats.projectMs = kernelMs_[0];
+        stats.countsMs = kernelMs_[1] + std::chrono::duration<double, std::milli>(countsDone - wallStart).count() -
+                         kernelMs_[0] - kernelMs_[1];
+        stats.emitMs = kernelMs_[2];
+        stats.tileSortMs = kernelMs_[3] + kernelMs_[4] + kernelMs_[5];
+        stats.blendMs = kernelMs_[6];
     }
     if (statsOut != nullptr) *statsOut = stats;
     return {};
@@ -339,6 +355,7 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     case 7: src = lastCount_.get(); size = size_t{lastPixels_} * 4; break;
     case 8: src = finalT_.get(); size = size_t{lastPixels_} * 4; break;
     case 9: src = rects_.get(); size = size_t{lastSplats_} * 8; break;
+    case 10: size = sizeof kernelMs_; break;
     default: return Status::error(ErrorCode::InvalidArgument, "unknown debug buffer " + std::to_string(which));
     }
     if (needed != nullptr) *needed = size;
@@ -346,6 +363,10 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     if (bytes < size) return Status::error(ErrorCode::InvalidArgument, "the host buffer is too small");
     if (which == 0) {
         std::memcpy(host, counts, size);
+        return {};
+    }
+    if (which == 10) {
+        std::memcpy(host, kernelMs_, size);
         return {};
     }
     if (size == 0) return {};

@@ -225,7 +225,8 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
 
 template <KeyMode MODE>
 void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
-                  uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream) {
+                  uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
+                  hipEvent_t afterHistogram) {
     const uint32_t passes = sortPasses(keyBits);
     const uint32_t parts = sortPartitions(n);
     const uint64_t mask = keyBits >= 64 ? ~0ull : ((1ull << keyBits) - 1ull);
@@ -235,6 +236,7 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     const uint32_t histBlocks = parts < 1024 ? parts : 1024;
     histogramKernel<MODE><<<histBlocks, kBlock, 0, stream>>>(keys, n, bitLo, mask, passes, ws.histogram);
     digitOffsetsKernel<<<passes, kRadix, 0, stream>>>(ws.histogram, ws.offsets);
+    if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
     const uint64_t* kin = keys;
     const uint32_t* vin = values;
     uint64_t* kout = keysAlt;
@@ -262,12 +264,18 @@ uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits) {
 }
 
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n, KeyMode mode,
-               uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream) {
-    if (n < 2 || keyBits == 0) return false;
+               uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
+               hipEvent_t afterHistogram) {
+    if (n < 2 || keyBits == 0) {
+        if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
+        return false;
+    }
     if (mode == KeyMode::TileDepth) {
-        launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, bitLo, keyBits, ws, stream);
+        launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, bitLo, keyBits, ws, stream,
+                                         afterHistogram);
     } else {
-        launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, bitLo, keyBits, ws, stream);
+        launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, bitLo, keyBits, ws, stream,
+                                     afterHistogram);
     }
     return (sortPasses(keyBits) & 1u) != 0u;
 }

@@ -260,8 +260,8 @@ class TileRasterizer:
         """What the last render left on the device (LRT_DEBUG_*), copied to host."""
         need = C.c_size_t()
         self._check(lib.lrt_raster_debug(self._ctx, which, None, 0, C.byref(need)))
-        dtype = {_abi.DEBUG_PROJ: np.float32, _abi.DEBUG_KEYS: np.uint64, _abi.DEBUG_FINAL_T: np.float32}.get(
-            which, np.uint32)
+        dtype = {_abi.DEBUG_PROJ: np.float32, _abi.DEBUG_KEYS: np.uint64, _abi.DEBUG_FINAL_T: np.float32,
+                 _abi.DEBUG_KERNEL_MS: np.float32}.get(which, np.uint32)
         out = np.empty(need.value // np.dtype(dtype).itemsize, dtype=dtype)
         if need.value:
             self._check(lib.lrt_raster_debug(self._ctx, which, out.ctypes.data, need.value, None))
