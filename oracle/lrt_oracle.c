@@ -1198,6 +1198,12 @@ static void ref_project_f64(const or_frame* f, const or_cloud* cl, uint32_t i, r
 
 typedef struct { double z; uint32_t index; } depth_entry;
 
+/* The rasteriser's own tie rule at its own resolution: the 24-bit key
+ * bits(float z) >> 7 (splat_project.slang:216-220), then the index. */
+static double keyed_depth(double z) {
+    return (double)(f2u((float)z) >> 7);
+}
+
 static int by_depth(const void* a, const void* b) {
     const depth_entry* x = (const depth_entry*)a;
     const depth_entry* y = (const depth_entry*)b;
@@ -1210,7 +1216,7 @@ static int by_depth(const void* a, const void* b) {
  * order: the same sequence its per-pixel insertion sort builds. */
 void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                              const uint32_t* px, const uint32_t* py, uint32_t npix, float* colour,
-                             float* depth, int threads) {
+                             float* depth, int keyed, int threads) {
     (void)threads;
     uint32_t total = 0;
     for (uint32_t k = 0; k < instances; ++k) total += frames[k].count;
@@ -1226,7 +1232,7 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
     uint32_t live = 0;
     for (uint32_t s = 0; s < total; ++s)
         if (rec[s].screen[2] >= 0.0f) {
-            order[live].z = rec[s].screen[2];
+            order[live].z = keyed ? keyed_depth(rec[s].screen[2]) : rec[s].screen[2];
             order[live].index = s;
             live++;
         }
@@ -1278,7 +1284,7 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
 
 void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                              const uint32_t* px, const uint32_t* py, uint32_t npix, double* colour,
-                             double* depth, int threads) {
+                             double* depth, int keyed, int threads) {
     (void)threads;
     uint32_t total = 0;
     for (uint32_t k = 0; k < instances; ++k) total += frames[k].count;
@@ -1294,7 +1300,7 @@ void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uin
     uint32_t live = 0;
     for (uint32_t s = 0; s < total; ++s)
         if (rec[s].screen[2] >= 0.0) {
-            order[live].z = rec[s].screen[2];
+            order[live].z = keyed ? keyed_depth(rec[s].screen[2]) : rec[s].screen[2];
             order[live].index = s;
             live++;
         }

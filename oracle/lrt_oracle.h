@@ -179,14 +179,15 @@ void or_blend(const or_frame* f, const uint32_t* ranges, const uint32_t* pair_sp
  * reference_blend.slang:24-119) in FP32, without the 1024-hit cap: every splat
  * in (view z, index) order is walked per pixel. px/py select a pixel subset
  * (NULL: every pixel, colour/depth full images); otherwise colour/depth hold
- * one entry per listed pixel. */
+ * one entry per listed pixel. keyed = 1 orders by the rasteriser's 24-bit
+ * depth key instead of the exact view z (ties by index either way). */
 void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                              const uint32_t* px, const uint32_t* py, uint32_t npix, float* colour,
-                             float* depth, int threads);
+                             float* depth, int keyed, int threads);
 /* The same in FP64 throughout (the error oracle). */
 void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                              const uint32_t* px, const uint32_t* py, uint32_t npix, double* colour,
-                             double* depth, int threads);
+                             double* depth, int keyed, int threads);
 
 /* compareImages (modules/render/src/ReferenceRenderer.cpp:248-297 with
  * shaders/lrt/reference/image_compare.slang:27-51): 8-bit sRGB code-value

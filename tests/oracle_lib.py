@@ -92,9 +92,9 @@ _SIGS = {
                                P, P, P, P, P, C.POINTER(OrStats), C.c_int]),
     "or_blend": (None, [C.POINTER(OrFrame), P, P, P, P, P, P, P, C.c_int]),
     "or_reference_render_f32": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
-                                       C.c_int]),
+                                       C.c_int, C.c_int]),
     "or_reference_render_f64": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
-                                       C.c_int]),
+                                       C.c_int, C.c_int]),
     "or_compare_images": (None, [P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint64)]),
     "or_srgb_to_linear": (C.c_float, [C.c_float]),
@@ -214,21 +214,22 @@ def raster(frames, clouds, n, threads=0, want_image=True):
                 last_count=last, final_t=final_t, visible=stats.visible, pairs=pairs, splats=stats.splats)
 
 
-def reference(frames, clouds, n, f64=False, pixels=None, threads=0):
-    """The brute force; pixels: None (whole image) or (xs, ys) uint32 arrays."""
+def reference(frames, clouds, n, f64=False, pixels=None, threads=0, keyed=False):
+    """The brute force; pixels: None (whole image) or (xs, ys) uint32 arrays; keyed:
+    order by the rasteriser's 24-bit depth key instead of exact view z."""
     f0 = frames[0]
     dt = np.float64 if f64 else np.float32
     fn = lib.or_reference_render_f64 if f64 else lib.or_reference_render_f32
     if pixels is None:
         colour = np.zeros((f0.height, f0.width, 4), dt)
         depth = np.zeros((f0.height, f0.width), dt)
-        fn(frames, clouds, n, None, None, 0, ptr(colour), ptr(depth), threads)
+        fn(frames, clouds, n, None, None, 0, ptr(colour), ptr(depth), 1 if keyed else 0, threads)
     else:
         xs = np.ascontiguousarray(pixels[0], dtype=np.uint32)
         ys = np.ascontiguousarray(pixels[1], dtype=np.uint32)
         colour = np.zeros((xs.size, 4), dt)
         depth = np.zeros(xs.size, dt)
-        fn(frames, clouds, n, ptr(xs), ptr(ys), xs.size, ptr(colour), ptr(depth), threads)
+        fn(frames, clouds, n, ptr(xs), ptr(ys), xs.size, ptr(colour), ptr(depth), 1 if keyed else 0, threads)
     return colour, depth
 
 
