@@ -137,6 +137,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
+    ap.add_argument("--blend-batch", type=int, default=256, help="pairs the blend stages in LDS: 64, 128, 256")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -152,7 +153,7 @@ def main():
     cfg = scenes.config(2, count=args.count)
     log(f"scene: {cfg.scene.count} splats generated in {time.perf_counter() - t0:.1f} s")
     settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
-                                     filter3d=cfg.filter3d)
+                                     filter3d=cfg.filter3d, blend_batch=args.blend_batch)
     dev = cfg.scene.to_device()
     inst = [raster.SplatInstance(dev)]
     targets = raster.RenderTargets(cfg.width, cfg.height)
@@ -193,6 +194,16 @@ def main():
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": stages[dominant]["GB/s"], "peak": HBM_PEAK_GBS,
                 "unit": "GB/s", "frac": stages[dominant]["hbm_frac"], "traffic": None}
     copy_gbs = copy_bandwidth_gbs(torch)
+    # The blend at each batch depth and launch order, for the record.
+    variants = {}
+    for batch in (64, 128, 256):
+        for order in (0, 1):
+            vs = raster.RenderSettings(**{**timed.__dict__, "blend_batch": batch, "tile_order": order})
+            acc_v = np.zeros(8)
+            for _ in range(10):
+                rast.render(proj, inst, vs, targets)
+                acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
+            variants[f"batch{batch}_{'heavy_first' if order == 0 else 'index_order'}"] = round(float(acc_v[6] / 10), 4)
 
     cpu = None
     if args.cpu_seconds > 0:
@@ -217,6 +228,7 @@ def main():
                    "dilation": cfg.dilation, "sort_passes": passes},
         "stages": stages,
         "gpu_ms_per_frame": round(float(kernel_ms[7]), 4),
+        "blend_ms_by_setting": variants,
         "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3) if sort_ms > 0 else None,
         "copy_GB/s": round(copy_gbs, 1),
         "roofline": roofline,

@@ -133,6 +133,8 @@ typedef struct {
     int32_t time_stages;          /* per-stage HIP event timing into lrt_frame_stats */
     float dilation;               /* 0.3 reproduces the reference */
     int32_t filter3d;             /* 1: read lrt_splats.filter3d */
+    uint32_t blend_batch;         /* pairs the blend stages in LDS at a time: 64, 128, 256 (0: 256) */
+    int32_t tile_order;           /* 0: heaviest tiles launched first; 1: tile index order */
 } lrt_render_settings;
 
 /* render::RenderTargets (TileRasterizer.h:257-263): device images, bottom row
@@ -190,9 +192,11 @@ typedef enum {
     LRT_DEBUG_LAST_COUNT = 7,    /* u32 a pixel: pairs walked to the last contributor */
     LRT_DEBUG_FINAL_T = 8,       /* float a pixel: final transmittance */
     LRT_DEBUG_RECTS = 9,         /* u32[2] a slot: x0 | y0 << 16, x1 | y1 << 16 */
-    LRT_DEBUG_KERNEL_MS = 10     /* float[8], last render with time_stages: GPU ms of project,
+    LRT_DEBUG_KERNEL_MS = 10,    /* float[8], last render with time_stages: GPU ms of project,
                                     scan, duplicate, sort histogram, sort passes, ranges,
                                     blend, and their sum (HIP events on the context's stream) */
+    LRT_DEBUG_TILE_ORDER = 11    /* u32 a tile: the order the blend took the tiles in (empty
+                                    when the last render used index order) */
 } lrt_debug_buffer;
 lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed);
 

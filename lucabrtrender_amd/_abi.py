@@ -19,7 +19,7 @@ STATUS_NAMES = {
 }
 
 DEBUG_COUNTS, DEBUG_TOUCHED, DEBUG_OFFSETS, DEBUG_PROJ, DEBUG_KEYS, DEBUG_VALUES, DEBUG_RANGES, \
-    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS, DEBUG_KERNEL_MS = range(11)
+    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS, DEBUG_KERNEL_MS, DEBUG_TILE_ORDER = range(12)
 
 
 class Lens(C.Structure):
@@ -83,7 +83,7 @@ class RenderSettingsC(C.Structure):
         ("background", C.c_float * 4),
         ("depth_mode", C.c_int32), ("depth_threshold", C.c_float),
         ("linearise", C.c_int32), ("time_stages", C.c_int32),
-        ("dilation", C.c_float), ("filter3d", C.c_int32),
+        ("dilation", C.c_float), ("filter3d", C.c_int32), ("blend_batch", C.c_uint32), ("tile_order", C.c_int32),
     ]
 
 

@@ -57,6 +57,8 @@ lrt::RenderSettings toSettings(const lrt_render_settings& s) {
     out.timeStages = s.time_stages != 0;
     out.dilation = s.dilation;
     out.filter3d = s.filter3d != 0;
+    out.blendBatch = s.blend_batch == 0 ? 256u : s.blend_batch;
+    out.heaviestTilesFirst = s.tile_order == 0;
     return out;
 }
 
@@ -164,6 +166,8 @@ void lrt_render_settings_default(lrt_render_settings* out) {
     out->time_stages = 0;
     out->dilation = s.dilation;
     out->filter3d = 0;
+    out->blend_batch = 256;
+    out->tile_order = 0;
 }
 
 void lrt_splat_edit_default(lrt_splat_edit* out) {

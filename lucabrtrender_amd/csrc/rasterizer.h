@@ -105,6 +105,8 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
     bool timeStages = false;
     float dilation = 0.3F;
     bool filter3d = false;
+    uint32_t blendBatch = 256;          // pairs staged in LDS at a time: 64, 128 or 256
+    bool heaviestTilesFirst = true;     // launch the tiles with the most pairs first
 };
 
 struct RenderTargets {   // caller-owned device images, bottom row first
@@ -188,7 +190,7 @@ private:
     // Per tile and per pixel.
     uint32_t tileCapacity_ = 0;
     uint64_t pixelCapacity_ = 0;
-    DeviceBuffer ranges_, lastCount_, finalT_;
+    DeviceBuffer ranges_, tileOrder_, lastCount_, finalT_;
     // Counters: [0] pairs, [1] visible, [2] overflow, [3] scan ticket, [4] sort error, [5] scan error.
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
@@ -199,6 +201,7 @@ private:
 
     // What the last render left.
     uint32_t lastSplats_ = 0, lastVisible_ = 0, lastPairs_ = 0, lastTiles_ = 0;
+    bool lastOrdered_ = false;
     uint64_t lastPixels_ = 0;
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
 };

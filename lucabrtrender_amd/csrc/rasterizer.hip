@@ -130,6 +130,7 @@ Status TileRasterizer::reserveTargets(uint32_t width, uint32_t height, uint32_t 
     Status s;
     if (tiles > tileCapacity_) {
         if (!(s = ranges_.reserve(uint64_t{tiles} * 8, "tile.ranges")).ok()) return s;
+        if (!(s = tileOrder_.reserve(uint64_t{tiles} * 4, "tile.order")).ok()) return s;
         tileCapacity_ = tiles;
     }
     const uint64_t pixels = uint64_t{width} * height;
@@ -261,8 +262,14 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_);
     mark(7);
 
-    kern::blend(first, splatCount, ranges_.as<uint2>(), values_[sorted].as<uint32_t>(), proj_.as<float4>(),
-                targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(), stream_);
+    const uint32_t* order = nullptr;
+    if (settings.heaviestTilesFirst) {
+        kern::tileOrder(ranges_.as<uint2>(), tiles, tileOrder_.as<uint32_t>(), stream_);
+        order = tileOrder_.as<uint32_t>();
+    }
+    kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), values_[sorted].as<uint32_t>(),
+                proj_.as<float4>(), targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(),
+                stream_);
     mark(8);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "frame")).ok())
         return s;
@@ -276,6 +283,7 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     lastVisible_ = visible;
     lastPairs_ = pairs;
     lastTiles_ = tiles;
+    lastOrdered_ = settings.heaviestTilesFirst;
     lastPixels_ = uint64_t{settings.width} * settings.height;
     lastSorted_ = sorted;
     stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
@@ -356,6 +364,7 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     case 8: src = finalT_.get(); size = size_t{lastPixels_} * 4; break;
     case 9: src = rects_.get(); size = size_t{lastSplats_} * 8; break;
     case 10: size = sizeof kernelMs_; break;
+    case 11: src = tileOrder_.get(); size = lastOrdered_ ? size_t{lastTiles_} * 4 : 0; break;
     default: return Status::error(ErrorCode::InvalidArgument, "unknown debug buffer " + std::to_string(which));
     }
     if (needed != nullptr) *needed = size;

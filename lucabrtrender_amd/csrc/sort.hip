@@ -144,10 +144,14 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
         }
         const uint32_t before = static_cast<uint32_t>(__popcll(peers & below));
         const uint32_t count = static_cast<uint32_t>(__popcll(peers));
+        // The count a leader stores is read by other lanes of the wave at the
+        // next key: relaxed atomics, so the compiler keeps no per-lane copy.
         uint32_t prev = 0;
-        if (valid) prev = sWave[wave][d[j]];
+        if (valid) prev = __hip_atomic_load(&sWave[wave][d[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         r[j] = prev + before;
-        if (valid && before == 0) sWave[wave][d[j]] = prev + count;
+        if (valid && before == 0) {
+            __hip_atomic_store(&sWave[wave][d[j]], prev + count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        }
     }
     __syncthreads();
 

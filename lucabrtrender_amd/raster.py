@@ -119,6 +119,8 @@ class RenderSettings:
     time_stages: bool = False
     dilation: float = 0.3
     filter3d: bool = False
+    blend_batch: int = 256          # pairs the blend stages in LDS at a time: 64, 128 or 256
+    tile_order: int = 0             # 0 heaviest tiles first, 1 tile index order
 
     def to_c(self) -> _abi.RenderSettingsC:
         s = _abi.RenderSettingsC()
@@ -131,6 +133,8 @@ class RenderSettings:
         s.time_stages = 1 if self.time_stages else 0
         s.dilation = self.dilation
         s.filter3d = 1 if self.filter3d else 0
+        s.blend_batch = self.blend_batch
+        s.tile_order = self.tile_order
         return s
 
 

@@ -217,3 +217,46 @@ def test_errors_carry_the_reference_codes(rast, cloud):
     gpu = H.run_gpu(CAMERAS["perspective"][0], s, [(cloud, np.eye(4), raster.SplatEdit())], rast)
     assert gpu["counts"][2] > 0
     assert rast.debug(_abi.DEBUG_COUNTS)[2] == gpu["counts"][2]
+
+
+@pytest.mark.parametrize("batch,order", [(64, 0), (128, 1), (256, 0), (256, 1)])
+def test_every_batch_depth_and_tile_order_draws_the_same_frame(rast, cloud, batch, order):
+    """Batch depth and launch order are performance parameters only."""
+    cam, _ = CAMERAS["perspective"]
+    s = raster.RenderSettings(width=W, height=HGT, blend_batch=batch, tile_order=order)
+    inst = [(cloud, np.eye(4), raster.SplatEdit())]
+    gpu = H.run_gpu(cam, s, inst, rast)
+    ora = H.run_oracle(cam, s, inst)
+    H.assert_bit_exact(gpu, ora)
+    _image_checks(gpu, ora)
+    cfg = scenes.config(1, count=20000)
+    s1 = raster.RenderSettings(width=cfg.width, height=cfg.height, blend_batch=batch, tile_order=order, depth_mode=1)
+    inst1 = [(cfg.scene, np.eye(4), raster.SplatEdit())]
+    gpu = H.run_gpu(cfg.camera, s1, inst1, rast)
+    ora = H.run_oracle(cfg.camera, s1, inst1)
+    H.assert_bit_exact(gpu, ora)
+    _image_checks(gpu, ora)
+
+
+def _heaviest_first(ranges):
+    """Tiles by the bit length of their pair count, descending, index order within one length."""
+    counts = (ranges[:, 1].astype(np.int64) - ranges[:, 0].astype(np.int64))
+    lengths = np.array([int(c).bit_length() for c in counts])
+    return np.array(sorted(range(len(counts)), key=lambda t: (-lengths[t], t)), dtype=np.uint32)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("size", [(W, HGT), (640, 480), (1920, 1080)])
+def test_tile_order_is_a_stable_heaviest_first_permutation(rast, cloud, size):
+    """Every tile exactly once, buckets by pair-count bit length descending, stable inside a bucket."""
+    cam, _ = CAMERAS["perspective"]
+    s = raster.RenderSettings(width=size[0], height=size[1], tile_order=0)
+    inst = [(cloud, np.eye(4), raster.SplatEdit())]
+    H.run_gpu(cam, s, inst, rast)
+    ranges = rast.debug(_abi.DEBUG_RANGES).reshape(-1, 2)
+    order = rast.debug(_abi.DEBUG_TILE_ORDER)
+    assert len(order) == len(ranges)
+    np.testing.assert_array_equal(order, _heaviest_first(ranges))
+    s.tile_order = 1
+    H.run_gpu(cam, s, inst, rast)
+    assert len(rast.debug(_abi.DEBUG_TILE_ORDER)) == 0
