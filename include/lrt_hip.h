@@ -133,8 +133,8 @@ typedef struct {
     int32_t time_stages;          /* per-stage HIP event timing into lrt_frame_stats */
     float dilation;               /* 0.3 reproduces the reference */
     int32_t filter3d;             /* 1: read lrt_splats.filter3d */
-    uint32_t blend_batch;         /* pairs the blend stages in LDS at a time: 64, 128, 256 (0: 256) */
-    int32_t tile_order;           /* 0: heaviest tiles launched first; 1: tile index order */
+    uint32_t blend_batch;         /* records each 8x8 quadrant's LDS list holds: 128 or 256 (0: 128) */
+    int32_t tile_order;           /* 0: tile index order; 1: heaviest tiles launched first */
 } lrt_render_settings;
 
 /* render::RenderTargets (TileRasterizer.h:257-263): device images, bottom row

@@ -1,18 +1,20 @@
-// Blend: one 256-thread workgroup per 16x16 tile, one thread per pixel, each
-// wave64 owning one 8x8 quadrant of the tile. The tile's pairs are walked front
-// to back in batches staged in LDS:
-//   1. each thread stages one pair's record (one coalesced gather per pair, not
-//      one per pixel) and tests it against the four quadrants with the exact
-//      minimum of the Gaussian's power over each quadrant's pixel centres;
-//   2. the batch is compacted into one index list per quadrant (ballot counts,
-//      a four-wave prefix), so each wave walks only the pairs that can reach one
-//      of its 64 pixels -- about 1.6 of a pair's 4 quadrants on the 3M-splat
-//      scene, 2.5x fewer pixel-pair steps than walking the whole tile;
-//   3. each wave walks its list with a branch-free body (a skipped pair
-//      contributes alpha 0, which changes nothing) and leaves the walk once its
-//      64 pixels are opaque; the workgroup stops between batches when all are.
-// Tiles are taken in the order the host gives (heaviest first), so the longest
-// sequential walks start at once instead of trailing the launch.
+// Blend: one wave64 workgroup per 8x8 quadrant of a 16x16 tile, one lane per
+// pixel. The wave walks its tile's sorted pairs front to back in rounds of 128:
+//   1. each lane takes two pairs of the round (their records gathered one round
+//      ahead, so the loads are in flight while the previous round is worked)
+//      and tests each against the quadrant's pixel-centre box with a
+//      conservative bound on the Gaussian's power (quadrantReach);
+//   2. the pairs that pass are compacted (ballot + popcount) into the wave's
+//      own LDS list, records and all, in pair order, until the list is full or
+//      the pairs run out;
+//   3. the wave walks the list kGroup records at a time with a branch-free body
+//      (a skipped pair contributes alpha 0, which changes nothing) and leaves
+//      once its 64 pixels are opaque.
+// No wave ever waits for another: quadrants finish as soon as their own pixels
+// do, and a quadrant with few pairs frees its slot for the next. The four
+// quadrants of one tile are blocks b, b + 8, b + 16, b + 24, which the
+// dispatcher deals to one XCD, so their gathers of the tile's records share
+// that XCD's L2.
 //
 // Per pixel it is the reference's splatBlend walk (shaders/lrt/splat/
 // splat_blend.slang:23-108, splat-only entry :110-114): skip past the cutoff,
@@ -20,9 +22,6 @@
 // with the transmittance, stop below 1e-4, then linearise rgb / coverage. The
 // Gaussian's exponent runs on the hardware exp2 and the quadratic form on
 // FMAs; both move the result by a few ulps, which the image tolerances cover.
-// Skipping a pair for a whole quadrant needs a test that never drops a pixel
-// the per-pixel test would keep: the quadrant minimum is compared with the
-// cutoff widened by a relative 1e-5, far above the float rounding of either.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -33,9 +32,36 @@ namespace {
 
 using namespace lrt::dev;
 
-constexpr uint32_t kBlock = kTile * kTile;
-constexpr uint32_t kWaves = kBlock / 64;
+constexpr uint32_t kLanes = 64;          // one wave: an 8x8 quadrant
+constexpr uint32_t kPerLane = 2;         // pairs a lane stages per round
+constexpr uint32_t kRound = kLanes * kPerLane;
+constexpr uint32_t kGroup = 4;           // list records walked per step
 constexpr float kExpScale = -0.72134752044448170368f;   // -log2(e) / 2
+
+// Can the Gaussian (power a x^2 + 2 b x y + c y^2 about its centre) reach
+// cutoff at some point of the box [lox, hix] x [loy, hiy] (coordinates about
+// the centre)? The power is convex, so its minimum over the box is 0 when the
+// centre is inside and otherwise lies on an edge, at the clamped minimiser of
+// the edge's 1-D quadratic. The answer may be yes for a box that no pixel
+// reaches, never no for one that a pixel reaches: the slack bounds the float
+// rounding of this evaluation and of the per-pixel one, both below a few ulps
+// of (a + 2|b| + c) * r2 with r2 the largest squared coordinate in the box,
+// ten times over; the edge minimisers use the hardware reciprocal, whose error
+// moves the value only to second order. A degenerate form is always kept.
+__device__ __forceinline__ bool quadrantReach(float a, float b, float c, float cutoff, float lox, float loy,
+                                              float hix, float hiy) {
+    const bool centreInside = lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f;
+    const bool degenerate = !(a > 1e-12f && c > 1e-12f);
+    const float ra = __builtin_amdgcn_rcpf(a), rc = __builtin_amdgcn_rcpf(c);
+    const auto form = [&](float x, float y) { return __builtin_fmaf(a * x, x, __builtin_fmaf(2.0f * b * x, y, c * y * y)); };
+    float best = form(lox, fminS(fmaxS(-b * lox * rc, loy), hiy));
+    best = fminS(best, form(hix, fminS(fmaxS(-b * hix * rc, loy), hiy)));
+    best = fminS(best, form(fminS(fmaxS(-b * loy * ra, lox), hix), loy));
+    best = fminS(best, form(fminS(fmaxS(-b * hiy * ra, lox), hix), hiy));
+    const float r2 = fmaxS(fmaxS(lox * lox, hix * hix), fmaxS(loy * loy, hiy * hiy));
+    const float slack = 4e-6f * (a + 2.0f * __builtin_fabsf(b) + c) * r2 + 1e-5f * cutoff + 1e-6f;
+    return centreInside || degenerate || best <= cutoff + slack;
+}
 
 // The finished pixel: linearise rgb / coverage, the background behind, the
 // depth (reference splat_blend.slang:89-107, splat-only).
@@ -58,126 +84,147 @@ __device__ __forceinline__ void finishPixel(const FrameParams& f, uint32_t px, u
     if (finalT != nullptr) finalT[at] = t;
 }
 
-template <uint32_t DEPTH_MODE, uint32_t BATCH>
-__global__ __launch_bounds__(kBlock) void blendKernel(FrameParams f, uint32_t slots, const uint32_t* __restrict__ order,
+// One pair's projected record, or an empty one (cutoff -1) past the range.
+struct Staged {
+    float4 screen;   // centre.xy, view z, cutoff
+    float4 conic;    // a, b, c, alpha
+    float4 col;      // rgb, 0
+};
+
+__device__ __forceinline__ Staged gather(const float4* __restrict__ proj, uint32_t slot, uint32_t slots) {
+    Staged s{make_float4(0.0f, 0.0f, 0.0f, -1.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f),
+             make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+    if (slot < slots) {
+        const uint64_t s3 = static_cast<uint64_t>(slot) * 3;
+        s.screen = proj[s3];
+        s.conic = proj[s3 + 1];
+        s.col = proj[s3 + 2];
+    }
+    return s;
+}
+
+template <uint32_t DEPTH_MODE, uint32_t LIST>
+__global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t slots, const uint32_t* __restrict__ order,
                                                       const uint2* __restrict__ ranges,
                                                       const uint32_t* __restrict__ pairSlots,
                                                       const float4* __restrict__ proj,
                                                       float4* __restrict__ colour, float* __restrict__ depth,
                                                       uint32_t* __restrict__ lastCount,
                                                       float* __restrict__ finalT) {
-    static_assert(BATCH == 64 || BATCH == 128 || BATCH == 256, "batch depth");
-    __shared__ float4 sGeom[BATCH];     // centre.xy, cutoff, alpha
-    __shared__ float4 sConic[BATCH];    // a, 2b, c, view z
-    __shared__ float4 sColour[BATCH];   // rgb, 0
-    __shared__ uint16_t sList[kWaves][BATCH];
-    __shared__ uint32_t sCount[kWaves][kWaves];   // [quadrant][wave that found them]
+    static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
+    // The list: records that reach this quadrant, in pair order, padded to a
+    // multiple of kGroup with empty records (cutoff -1: never pass the test).
+    __shared__ float4 sGeom[LIST + kGroup];     // centre.xy, cutoff, alpha
+    __shared__ float4 sConic[LIST + kGroup];    // a, 2b, c, view z
+    __shared__ float4 sColour[LIST + kGroup];   // rgb, bits of the pair's place in the tile's range + 1
 
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & 63u;
-    const uint32_t wave = tid >> 6;
-    const uint32_t tile = order != nullptr ? order[blockIdx.x] : blockIdx.x;
-    if (tile >= f.tilesX * f.tilesY) return;   // an order entry that is not a tile: draw nothing
-    const uint32_t tileX = tile % f.tilesX;
-    const uint32_t tileY = tile / f.tilesX;
-    // wave w: quadrant (w & 1, w >> 1); lane: 8x8 inside it
-    const uint32_t px = tileX * kTile + (wave & 1u) * 8u + (lane & 7u);
-    const uint32_t py = tileY * kTile + (wave >> 1) * 8u + (lane >> 3);
-    const bool inside = px < f.width && py < f.height;
-    const uint2 range = ranges[tile];
+    const uint32_t lane = threadIdx.x;
+    const uint32_t tiles = f.tilesX * f.tilesY;
+    const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
+    const uint32_t place = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
+    if (place >= tiles) return;
+    const uint32_t tile = order != nullptr ? order[place] : place;
+    if (tile >= tiles) return;   // an order entry that is not a tile: draw nothing
+    const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
+    const uint32_t y0 = (tile / f.tilesX) * kTile + (quadrant >> 1) * 8u;
+    if (x0 >= f.width || y0 >= f.height) return;   // a quadrant past the frame's edge has no pixels
+    const uint32_t x1 = x0 + 8u < f.width ? x0 + 8u : f.width;
+    const uint32_t y1 = y0 + 8u < f.height ? y0 + 8u : f.height;
+    const uint32_t px = x0 + (lane & 7u);
+    const uint32_t py = y0 + (lane >> 3);
+    const bool inside = px < x1 && py < y1;
     const float pxf = static_cast<float>(px) + 0.5f;
     const float pyf = static_cast<float>(py) + 0.5f;
-    // Each quadrant's pixel-centre box, for the staging threads' tests.
-    float boxLo[kWaves][2], boxHi[kWaves][2];
+    const float boxLoX = static_cast<float>(x0) + 0.5f, boxLoY = static_cast<float>(y0) + 0.5f;
+    const float boxHiX = static_cast<float>(x1 - 1u) + 0.5f, boxHiY = static_cast<float>(y1 - 1u) + 0.5f;
+    const uint2 range = ranges[tile];
+    const uint64_t below = (1ull << lane) - 1ull;
+
+    // Round k's records sit in `staged` while round k + 1's are gathered.
+    auto slotAt = [&](uint32_t at) -> uint32_t { return at < range.y ? pairSlots[at] : 0xFFFFFFFFu; };
+    Staged staged[kPerLane];
 #pragma unroll
-    for (uint32_t q = 0; q < kWaves; ++q) {
-        const uint32_t x0 = tileX * kTile + (q & 1u) * 8u, y0 = tileY * kTile + (q >> 1) * 8u;
-        const uint32_t x1 = x0 + 8u < f.width ? x0 + 8u : f.width;
-        const uint32_t y1 = y0 + 8u < f.height ? y0 + 8u : f.height;
-        boxLo[q][0] = static_cast<float>(x0) + 0.5f;
-        boxLo[q][1] = static_cast<float>(y0) + 0.5f;
-        // An empty quadrant (past the frame's edge) gets an inverted box: never reached.
-        boxHi[q][0] = x1 > x0 ? static_cast<float>(x1 - 1u) + 0.5f : -1e30f;
-        boxHi[q][1] = y1 > y0 ? static_cast<float>(y1 - 1u) + 0.5f : -1e30f;
-    }
+    for (uint32_t u = 0; u < kPerLane; ++u) staged[u] = gather(proj, slotAt(range.x + u * kLanes + lane), slots);
 
     float r = 0.0f, g = 0.0f, b = 0.0f, t = 1.0f, depthSum = 0.0f, depthAt = -1.0f;
     uint32_t last = 0;
     bool active = inside;
-    for (uint32_t batch = range.x; batch < range.y; batch += BATCH) {
-        if (__syncthreads_count(active ? 1 : 0) == 0) break;
-        // 1. stage and test
-        uint32_t mask = 0;
-        if (tid < BATCH && batch + tid < range.y) {
-            const uint32_t slot = pairSlots[batch + tid];
-            float4 screen = make_float4(0.0f, 0.0f, 0.0f, -1.0f), conic = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            float4 col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            if (slot < slots) {
-                const uint64_t s3 = static_cast<uint64_t>(slot) * 3;
-                screen = proj[s3];
-                conic = proj[s3 + 1];
-                col = proj[s3 + 2];
-            }
-            sGeom[tid] = make_float4(screen.x, screen.y, screen.w, conic.w);
-            sConic[tid] = make_float4(conic.x, conic.y + conic.y, conic.z, screen.z);
-            sColour[tid] = col;
-            const float reach = screen.w * 1.00001f + 1e-6f;
+    uint32_t next = range.x;   // the first pair of the round in `staged`
+    while (next < range.y) {
+        // 1-2. stage rounds into the list while a whole round still fits
+        uint32_t fill = 0;
+        while (next < range.y && fill + kRound <= LIST) {
+            Staged ahead[kPerLane];
 #pragma unroll
-            for (uint32_t q = 0; q < kWaves; ++q) {
-                if (boxHi[q][0] < 0.0f || boxHi[q][1] < 0.0f) continue;
-                const float m = minPowerOverBox(conic.x, conic.y, conic.z, boxLo[q][0] - screen.x,
-                                                boxLo[q][1] - screen.y, boxHi[q][0] - screen.x,
-                                                boxHi[q][1] - screen.y);
-                if (m <= reach) mask |= 1u << q;
+            for (uint32_t u = 0; u < kPerLane; ++u) {
+                ahead[u] = gather(proj, slotAt(next + kRound + u * kLanes + lane), slots);
             }
+#pragma unroll
+            for (uint32_t u = 0; u < kPerLane; ++u) {
+                const Staged& s = staged[u];
+                const uint32_t at = next + u * kLanes + lane;
+                const bool keep = at < range.y &&
+                                  quadrantReach(s.conic.x, s.conic.y, s.conic.z, s.screen.w, boxLoX - s.screen.x,
+                                                boxLoY - s.screen.y, boxHiX - s.screen.x, boxHiY - s.screen.y);
+                const uint64_t vote = __ballot(keep);
+                if (keep) {
+                    const uint32_t to = fill + static_cast<uint32_t>(__popcll(vote & below));
+                    sGeom[to] = make_float4(s.screen.x, s.screen.y, s.screen.w, s.conic.w);
+                    sConic[to] = make_float4(s.conic.x, s.conic.y + s.conic.y, s.conic.z, s.screen.z);
+                    sColour[to] = make_float4(s.col.x, s.col.y, s.col.z, __uint_as_float(at - range.x + 1u));
+                }
+                fill += static_cast<uint32_t>(__popcll(vote));
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kPerLane; ++u) staged[u] = ahead[u];
+            next += kRound;
         }
-        // 2. compact: list q gets, in pair order, the batch entries reaching quadrant q
-        uint32_t before[kWaves];
-        const uint64_t below = (1ull << lane) - 1ull;
-#pragma unroll
-        for (uint32_t q = 0; q < kWaves; ++q) {
-            const uint64_t vote = __ballot((mask >> q) & 1u);
-            before[q] = static_cast<uint32_t>(__popcll(vote & below));
-            if (lane == 0) sCount[q][wave] = static_cast<uint32_t>(__popcll(vote));
-        }
-        __syncthreads();
-#pragma unroll
-        for (uint32_t q = 0; q < kWaves; ++q) {
-            if ((mask >> q) & 1u) {
-                uint32_t at = before[q];
-                for (uint32_t w = 0; w < wave; ++w) at += sCount[q][w];
-                sList[q][at] = static_cast<uint16_t>(tid);
-            }
+        const uint32_t padded = (fill + kGroup - 1u) & ~(kGroup - 1u);
+        if (lane < padded - fill) {
+            sGeom[fill + lane] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
+            sConic[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sColour[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         __syncthreads();
-        // 3. walk this wave's list
-        const uint32_t n = sCount[wave][0] + sCount[wave][1] + sCount[wave][2] + sCount[wave][3];
-        if (__ballot(active) != 0) {
-            for (uint32_t j = 0; j < n; ++j) {
-                const uint32_t e = sList[wave][j];
-                const float4 r0 = sGeom[e];
-                const float4 r1 = sConic[e];
-                const float dx = pxf - r0.x;
-                const float dy = pyf - r0.y;
-                const float power = __builtin_fmaf(dx, __builtin_fmaf(r1.x, dx, r1.y * dy), (r1.z * dy) * dy);
-                const float alpha = fminS(0.99f, r0.w * __builtin_amdgcn_exp2f(kExpScale * power));
-                const bool ok = active && power <= r0.z && alpha >= 1.0f / 255.0f;
+        // 3. walk the list: all kGroup records' reads issued together, then the
+        // pixels blend them in order
+        for (uint32_t j = 0; j < padded; j += kGroup) {
+            float4 g0[kGroup], g1[kGroup], g2[kGroup];
+#pragma unroll
+            for (uint32_t u = 0; u < kGroup; ++u) {
+                g0[u] = sGeom[j + u];
+                g1[u] = sConic[j + u];
+                g2[u] = sColour[j + u];
+                // the place is needed only when the pair contributes; without
+                // this the compiler loads it behind a branch in the walk
+                asm volatile("" ::"v"(g2[u].w));
+            }
+#pragma unroll
+            for (uint32_t u = 0; u < kGroup; ++u) {
+                const float dx = pxf - g0[u].x;
+                const float dy = pyf - g0[u].y;
+                const float power = __builtin_fmaf(dx, __builtin_fmaf(g1[u].x, dx, g1[u].y * dy),
+                                                   (g1[u].z * dy) * dy);
+                const float alpha = __builtin_fminf(0.99f, g0[u].w * __builtin_amdgcn_exp2f(kExpScale * power));
+                // bitwise, not short-circuit: no branch inside the walk
+                const bool ok = active & (power <= g0[u].z) & (alpha >= 1.0f / 255.0f);
                 const float a = ok ? alpha : 0.0f;
-                const float4 r2 = sColour[e];
                 const float weight = t * a;
-                r = __builtin_fmaf(weight, r2.x, r);
-                g = __builtin_fmaf(weight, r2.y, g);
-                b = __builtin_fmaf(weight, r2.z, b);
-                depthSum = __builtin_fmaf(weight, r1.w, depthSum);
+                r = __builtin_fmaf(weight, g2[u].x, r);
+                g = __builtin_fmaf(weight, g2[u].y, g);
+                b = __builtin_fmaf(weight, g2[u].z, b);
+                depthSum = __builtin_fmaf(weight, g1[u].w, depthSum);
                 t = t * (1.0f - a);
-                last = ok ? batch + e - range.x + 1 : last;
+                last = ok ? __float_as_uint(g2[u].w) : last;
                 if constexpr (DEPTH_MODE == 1) {
-                    depthAt = (ok && depthAt < 0.0f && 1.0f - t >= f.depthThreshold) ? r1.w : depthAt;
+                    depthAt = (ok && depthAt < 0.0f && 1.0f - t >= f.depthThreshold) ? g1[u].w : depthAt;
                 }
                 active = active && !(t < 1e-4f);
-                if (__ballot(active) == 0) break;
             }
+            if (__ballot(active) == 0) break;
         }
+        if (__ballot(active) == 0) break;
+        __syncthreads();   // the list is refilled next
     }
     if (!inside) return;
     finishPixel(f, px, py, r, g, b, t, depthSum, depthAt, last, colour, depth, lastCount, finalT);
@@ -233,19 +280,18 @@ __global__ __launch_bounds__(1024) void tileOrderKernel(const uint2* __restrict_
 }
 
 template <uint32_t DEPTH_MODE>
-void launchBlend(uint32_t batch, const FrameParams& f, uint32_t slots, const uint32_t* order, const uint2* ranges,
+void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint32_t* order, const uint2* ranges,
                  const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
                  float* finalT, hipStream_t stream) {
+    // Four blocks a tile, the grid rounded up to whole groups of eight tiles.
     const uint32_t tiles = f.tilesX * f.tilesY;
-    if (batch <= 64) {
-        blendKernel<DEPTH_MODE, 64><<<tiles, kBlock, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                  depth, lastCount, finalT);
-    } else if (batch <= 128) {
-        blendKernel<DEPTH_MODE, 128><<<tiles, kBlock, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                   depth, lastCount, finalT);
+    const uint32_t blocks = (tiles + 7u) / 8u * 32u;
+    if (list <= 128) {
+        blendKernel<DEPTH_MODE, 128><<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
+                                                                    depth, lastCount, finalT);
     } else {
-        blendKernel<DEPTH_MODE, 256><<<tiles, kBlock, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                   depth, lastCount, finalT);
+        blendKernel<DEPTH_MODE, 256><<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
+                                                                    depth, lastCount, finalT);
     }
 }
 
@@ -256,14 +302,14 @@ void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t
     tileOrderKernel<<<1, 1024, 0, stream>>>(ranges, tiles, order);
 }
 
-void blend(const FrameParams& f, uint32_t batch, uint32_t slots, const uint32_t* order, const uint2* ranges,
+void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
            float* finalT, hipStream_t stream) {
     if (f.tilesX * f.tilesY == 0) return;
     if (f.depthMode == 0) {
-        launchBlend<0>(batch, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, stream);
+        launchBlend<0>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, stream);
     } else {
-        launchBlend<1>(batch, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, stream);
+        launchBlend<1>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, stream);
     }
 }
 

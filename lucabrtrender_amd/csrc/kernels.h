@@ -72,10 +72,10 @@ void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ran
 // ---- blend ------------------------------------------------------------------
 // Tile indices, heaviest tile (most pairs) first, for the blend's launch order.
 void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t stream);
-// Per pixel, front to back over its tile's pairs, batch (64, 128 or 256) pairs
-// staged in LDS at a time; tiles in `order` (null: index order); slots bounds
-// the pair values. lastCount/finalT may be null.
-void blend(const FrameParams& f, uint32_t batch, uint32_t slots, const uint32_t* order, const uint2* ranges,
+// Per pixel, front to back over its tile's pairs; one wave per 8x8 quadrant,
+// whose LDS list holds `list` (128 or 256) records; tiles in `order` (null:
+// index order); slots bounds the pair values. lastCount/finalT may be null.
+void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
            float* finalT, hipStream_t stream);
 

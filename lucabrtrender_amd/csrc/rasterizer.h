@@ -105,8 +105,8 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
     bool timeStages = false;
     float dilation = 0.3F;
     bool filter3d = false;
-    uint32_t blendBatch = 256;          // pairs staged in LDS at a time: 64, 128 or 256
-    bool heaviestTilesFirst = true;     // launch the tiles with the most pairs first
+    uint32_t blendBatch = 128;          // records each quadrant's LDS list holds: 128 or 256
+    bool heaviestTilesFirst = false;    // launch the tiles with the most pairs first
 };
 
 struct RenderTargets {   // caller-owned device images, bottom row first

@@ -119,8 +119,8 @@ class RenderSettings:
     time_stages: bool = False
     dilation: float = 0.3
     filter3d: bool = False
-    blend_batch: int = 256          # pairs the blend stages in LDS at a time: 64, 128 or 256
-    tile_order: int = 0             # 0 heaviest tiles first, 1 tile index order
+    blend_batch: int = 128          # records each 8x8 quadrant's LDS list holds: 128 or 256
+    tile_order: int = 0             # 0 tile index order, 1 heaviest tiles first
 
     def to_c(self) -> _abi.RenderSettingsC:
         s = _abi.RenderSettingsC()
