@@ -179,6 +179,44 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
                              const lrt_render_settings* settings, lrt_render_targets* targets,
                              lrt_frame_stats* stats);
 
+/* ---- the differentiable frame ---------------------------------------------------
+ * No counterpart in the reference, which has no backward: the entries that the
+ * boundary's lrt_raster_backward stands for. One cloud of FP32 parameters in
+ * device memory, drawn by the same stages as lrt_raster_render; the context
+ * keeps what the backward needs until its next render. */
+typedef struct {
+    const float* means;           /* N x 3, the cloud's own space */
+    const float* log_scales;      /* N x 3: scale = exp(log scale) */
+    const float* quats;           /* N x 4: x, y, z, w (w real), normalised on use */
+    const float* opacity_logits;  /* N: opacity = sigmoid(logit) */
+    const float* sh;              /* N x (1 + rest_per_colour) x 3: dc, then the rest coefficients,
+                                     rgb per basis; colour = max(0.5 + C0 dc + rest(dir), 0) */
+    const float* filter3d;        /* N, or NULL; read when settings.filter3d */
+    uint32_t count;
+    uint32_t rest_per_colour;     /* 0, 3, 8, 15 */
+} lrt_diff_splats;
+
+/* Device gradient buffers of the same shapes, overwritten by the backward. */
+typedef struct {
+    float* means;
+    float* log_scales;
+    float* quats;
+    float* opacity_logits;
+    float* sh;
+} lrt_diff_grads;
+
+/* A frame of one FP32-parameter cloud; as lrt_raster_render otherwise. */
+lrt_status lrt_raster_render_diff(lrt_context* ctx, const lrt_projection* projection,
+                                  const lrt_diff_splats* splats, const double object_to_world[16],
+                                  const lrt_render_settings* settings, lrt_render_targets* targets,
+                                  lrt_frame_stats* stats);
+/* Gradients of the last lrt_raster_render_diff's parameters from dL/d(colour)
+ * (device, one float4 a pixel, bottom row first). The depth image receives no
+ * gradient. millis (host, 2 floats, may be NULL): GPU ms of the blend's and
+ * the preprocess's backward. Synchronous. */
+lrt_status lrt_raster_backward(lrt_context* ctx, const float* grad_colour, const lrt_diff_grads* grads,
+                               float* millis);
+
 /* What the last render left on the device, copied to host, for bit-exact
  * checks. *needed (may be NULL) receives the byte size; bytes may be 0 to ask. */
 typedef enum {

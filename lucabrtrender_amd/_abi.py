@@ -100,6 +100,21 @@ class FrameStatsC(C.Structure):
     ]
 
 
+class DiffSplatsC(C.Structure):
+    _fields_ = [
+        ("means", C.c_void_p), ("log_scales", C.c_void_p), ("quats", C.c_void_p),
+        ("opacity_logits", C.c_void_p), ("sh", C.c_void_p), ("filter3d", C.c_void_p),
+        ("count", C.c_uint32), ("rest_per_colour", C.c_uint32),
+    ]
+
+
+class DiffGradsC(C.Structure):
+    _fields_ = [
+        ("means", C.c_void_p), ("log_scales", C.c_void_p), ("quats", C.c_void_p),
+        ("opacity_logits", C.c_void_p), ("sh", C.c_void_p),
+    ]
+
+
 class SceneSpecC(C.Structure):
     _fields_ = [
         ("kind", C.c_uint32), ("degree", C.c_uint32), ("seed", C.c_uint64),
@@ -126,6 +141,10 @@ SIGNATURES = {
     "lrt_last_error": (C.c_char_p, [C.c_void_p]),
     "lrt_raster_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
                                     C.POINTER(RenderSettingsC), C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
+    "lrt_raster_render_diff": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(DiffSplatsC),
+                                         C.POINTER(C.c_double), C.POINTER(RenderSettingsC),
+                                         C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
+    "lrt_raster_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DiffGradsC), C.POINTER(C.c_float)]),
     "lrt_raster_debug": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "lrt_frame_params": (C.c_int, [C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.POINTER(RenderSettingsC),
                                    C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),

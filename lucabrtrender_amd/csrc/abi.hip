@@ -257,6 +257,48 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
     return finish(ctx, s);
 }
 
+lrt_status lrt_raster_render_diff(lrt_context* ctx, const lrt_projection* projection,
+                                  const lrt_diff_splats* splats, const double object_to_world[16],
+                                  const lrt_render_settings* settings, lrt_render_targets* targets,
+                                  lrt_frame_stats* stats) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (projection == nullptr || splats == nullptr || settings == nullptr || targets == nullptr) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    lrt::kern::DiffCloud cloud{splats->means, splats->log_scales, splats->quats, splats->opacity_logits, splats->sh,
+                               splats->filter3d, splats->count, splats->rest_per_colour};
+    lrt::Mat4 o2w;
+    if (object_to_world != nullptr) o2w = toMat(object_to_world);
+    lrt::RenderTargets t;
+    t.width = targets->width;
+    t.height = targets->height;
+    t.colour = reinterpret_cast<float4*>(targets->colour);
+    t.depth = targets->depth;
+    lrt::FrameStats fs;
+    const lrt::Status s = ctx->raster.renderDiff(toProjection(*projection), cloud, o2w, toSettings(*settings), t, &fs);
+    if (s.ok() && stats != nullptr) {
+        stats->splats = fs.splats;
+        stats->visible = fs.visible;
+        stats->pairs = fs.pairs;
+        stats->project_ms = fs.projectMs;
+        stats->depth_sort_ms = fs.depthSortMs;
+        stats->counts_ms = fs.countsMs;
+        stats->emit_ms = fs.emitMs;
+        stats->tile_sort_ms = fs.tileSortMs;
+        stats->blend_ms = fs.blendMs;
+        stats->total_ms = fs.totalMs;
+    }
+    return finish(ctx, s);
+}
+
+lrt_status lrt_raster_backward(lrt_context* ctx, const float* grad_colour, const lrt_diff_grads* grads,
+                               float* millis) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (grads == nullptr) return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    const lrt::kern::DiffGrads g{grads->means, grads->log_scales, grads->quats, grads->opacity_logits, grads->sh};
+    return finish(ctx, ctx->raster.backward(reinterpret_cast<const float4*>(grad_colour), g, millis));
+}
+
 lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed) {
     if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
     return finish(ctx, ctx->raster.debugCopy(which, host, bytes, needed));

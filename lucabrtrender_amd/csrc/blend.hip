@@ -38,38 +38,15 @@ constexpr uint32_t kRound = kLanes * kPerLane;
 constexpr uint32_t kGroup = 4;           // list records walked per step
 constexpr float kExpScale = -0.72134752044448170368f;   // -log2(e) / 2
 
-// Can the Gaussian (power a x^2 + 2 b x y + c y^2 about its centre) reach
-// cutoff at some point of the box [lox, hix] x [loy, hiy] (coordinates about
-// the centre)? The power is convex, so its minimum over the box is 0 when the
-// centre is inside and otherwise lies on an edge, at the clamped minimiser of
-// the edge's 1-D quadratic. The answer may be yes for a box that no pixel
-// reaches, never no for one that a pixel reaches: the slack bounds the float
-// rounding of this evaluation and of the per-pixel one, both below a few ulps
-// of (a + 2|b| + c) * r2 with r2 the largest squared coordinate in the box,
-// ten times over; the edge minimisers use the hardware reciprocal, whose error
-// moves the value only to second order. A degenerate form is always kept.
-__device__ __forceinline__ bool quadrantReach(float a, float b, float c, float cutoff, float lox, float loy,
-                                              float hix, float hiy) {
-    const bool centreInside = lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f;
-    const bool degenerate = !(a > 1e-12f && c > 1e-12f);
-    const float ra = __builtin_amdgcn_rcpf(a), rc = __builtin_amdgcn_rcpf(c);
-    const auto form = [&](float x, float y) { return __builtin_fmaf(a * x, x, __builtin_fmaf(2.0f * b * x, y, c * y * y)); };
-    float best = form(lox, fminS(fmaxS(-b * lox * rc, loy), hiy));
-    best = fminS(best, form(hix, fminS(fmaxS(-b * hix * rc, loy), hiy)));
-    best = fminS(best, form(fminS(fmaxS(-b * loy * ra, lox), hix), loy));
-    best = fminS(best, form(fminS(fmaxS(-b * hiy * ra, lox), hix), hiy));
-    const float r2 = fmaxS(fmaxS(lox * lox, hix * hix), fmaxS(loy * loy, hiy * hiy));
-    const float slack = 4e-6f * (a + 2.0f * __builtin_fabsf(b) + c) * r2 + 1e-5f * cutoff + 1e-6f;
-    return centreInside || degenerate || best <= cutoff + slack;
-}
-
 // The finished pixel: linearise rgb / coverage, the background behind, the
 // depth (reference splat_blend.slang:89-107, splat-only).
 __device__ __forceinline__ void finishPixel(const FrameParams& f, uint32_t px, uint32_t py, float r, float g, float b,
                                             float t, float depthSum, float depthAt, uint32_t last,
                                             float4* __restrict__ colour, float* __restrict__ depth,
-                                            uint32_t* __restrict__ lastCount, float* __restrict__ finalT) {
+                                            uint32_t* __restrict__ lastCount, float* __restrict__ finalT,
+                                            float4* __restrict__ accum) {
     const float coverage = 1.0f - t;
+    if (accum != nullptr) accum[py * f.width + px] = make_float4(r, g, b, depthSum);
     if (f.linearise && coverage > 1e-6f) {
         r = srgbToLinear(r / coverage) * coverage;
         g = srgbToLinear(g / coverage) * coverage;
@@ -110,7 +87,7 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
                                                       const float4* __restrict__ proj,
                                                       float4* __restrict__ colour, float* __restrict__ depth,
                                                       uint32_t* __restrict__ lastCount,
-                                                      float* __restrict__ finalT) {
+                                                      float* __restrict__ finalT, float4* __restrict__ accum) {
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
     // The list: records that reach this quadrant, in pair order, padded to a
     // multiple of kGroup with empty records (cutoff -1: never pass the test).
@@ -227,7 +204,7 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
         __syncthreads();   // the list is refilled next
     }
     if (!inside) return;
-    finishPixel(f, px, py, r, g, b, t, depthSum, depthAt, last, colour, depth, lastCount, finalT);
+    finishPixel(f, px, py, r, g, b, t, depthSum, depthAt, last, colour, depth, lastCount, finalT, accum);
 }
 
 // Heaviest tiles first: tiles bucketed by the bit length of their pair count,
@@ -282,16 +259,16 @@ __global__ __launch_bounds__(1024) void tileOrderKernel(const uint2* __restrict_
 template <uint32_t DEPTH_MODE>
 void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint32_t* order, const uint2* ranges,
                  const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-                 float* finalT, hipStream_t stream) {
+                 float* finalT, float4* accum, hipStream_t stream) {
     // Four blocks a tile, the grid rounded up to whole groups of eight tiles.
     const uint32_t tiles = f.tilesX * f.tilesY;
     const uint32_t blocks = (tiles + 7u) / 8u * 32u;
     if (list <= 128) {
         blendKernel<DEPTH_MODE, 128><<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                    depth, lastCount, finalT);
+                                                                    depth, lastCount, finalT, accum);
     } else {
         blendKernel<DEPTH_MODE, 256><<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                    depth, lastCount, finalT);
+                                                                    depth, lastCount, finalT, accum);
     }
 }
 
@@ -304,12 +281,14 @@ void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t
 
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-           float* finalT, hipStream_t stream) {
+           float* finalT, float4* accum, hipStream_t stream) {
     if (f.tilesX * f.tilesY == 0) return;
     if (f.depthMode == 0) {
-        launchBlend<0>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, stream);
+        launchBlend<0>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, accum,
+                       stream);
     } else {
-        launchBlend<1>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, stream);
+        launchBlend<1>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, accum,
+                       stream);
     }
 }
 

@@ -1,0 +1,684 @@
+// The differentiable path: the preprocess from FP32 parameters, and the
+// backward of the blend and of the preprocess (EWA projection, scales,
+// rotation, opacity, spherical harmonics). The reference has no backward; the
+// forward it differentiates is the reference's frame (splat_project.slang:
+// 52-221, splat_blend.slang:23-114), drawn by the same kernels as every other
+// frame.
+//
+// Per-pixel state is not stored per splat: the blend leaves each pixel's final
+// transmittance, the place of its last contributor in its tile's pair list
+// and its raw accumulated rgb. The backward walks each tile's pairs back to
+// front from there, recovering T before each contributor as T / (1 - alpha).
+//
+// Gradients reach the splats without atomics. The duplication gave each pair
+// its index in the slot-major pair order as its sort value; a tile's four
+// quadrant waves reduce each pair's per-pixel gradients across their lanes
+// (DPP), the workgroup adds the four quadrants in LDS and writes one record per
+// pair at that index; each splat then sums its own contiguous run of records.
+#include <hip/hip_runtime.h>
+
+#include "kernels.h"
+#include "projection.h"
+#include "splat_math.h"
+
+namespace lrt::kern {
+namespace {
+
+using namespace lrt::dev;
+
+constexpr uint32_t kBlock = 256;
+constexpr float kShC0 = 0.28209479177387814f;
+constexpr float kExpScale = -0.72134752044448170368f;   // -log2(e) / 2
+
+__device__ __forceinline__ uint32_t shDegree(const FrameParams& f) {
+    const uint32_t d = degreeOf(f.restPerColour);
+    return d < f.shLimit ? d : f.shLimit;
+}
+
+// Basis functions used at each degree: 3, 8, 15 rest coefficients per colour.
+__device__ __forceinline__ uint32_t restCount(uint32_t degree) {
+    return degree == 0 ? 0u : degree == 1 ? 3u : degree == 2 ? 8u : 15u;
+}
+
+// A splat of the FP32 cloud, decoded as the packed path decodes its own:
+// scales exp(log scale), opacity sigmoid(logit), the quaternion normalised
+// (the zero quaternion is the identity).
+struct Decoded {
+    SplatShape shape;
+    float sigmoid;     // opacity before the 3D filter
+    float qLen;        // |q| before normalising
+    float s[3];        // exp(log scale), before the 3D filter
+};
+
+__device__ __forceinline__ Decoded decode(const DiffCloud& c, uint32_t i, bool useFilter3d) {
+    Decoded d;
+    const float* mean = c.means + static_cast<uint64_t>(i) * 3;
+    const float* ls = c.logScales + static_cast<uint64_t>(i) * 3;
+    const float* qr = c.quats + static_cast<uint64_t>(i) * 4;
+    d.s[0] = expF(ls[0]);
+    d.s[1] = expF(ls[1]);
+    d.s[2] = expF(ls[2]);
+    d.sigmoid = 1.0f / (1.0f + expF(-c.logits[i]));
+    float q[4] = {qr[0], qr[1], qr[2], qr[3]};
+    d.qLen = __builtin_sqrtf((q[0] * q[0] + q[1] * q[1]) + (q[2] * q[2] + q[3] * q[3]));
+    if (d.qLen > 0.0f) {
+        for (float& v : q) v = v / d.qLen;
+    } else {
+        q[0] = q[1] = q[2] = 0.0f;
+        q[3] = 1.0f;
+    }
+    d.shape = SplatShape{mean[0], mean[1], mean[2], d.sigmoid, d.s[0], d.s[1], d.s[2], {q[0], q[1], q[2], q[3]}};
+    if (useFilter3d) applyFilter3d(c.filter3d[i], d.shape.sx, d.shape.sy, d.shape.sz, d.shape.opacity);
+    return d;
+}
+
+__global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffCloud c, float4* __restrict__ proj,
+                                                            uint2* __restrict__ rects,
+                                                            uint32_t* __restrict__ touchedOut) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.count) return;
+    const uint32_t slot = f.base + i;
+    const Decoded d = decode(c, i, f.useFilter3d != 0);
+    const uint32_t degree = shDegree(f);
+    const uint32_t stride = (1u + c.restPerColour) * 3u;
+    const float* row = c.sh + static_cast<uint64_t>(i) * stride;
+    const auto colourOf = [&](float dx, float dy, float dz, float colour[3]) {
+        float coeff[45];
+        const uint32_t used = restCount(degree) * 3u;
+#pragma unroll
+        for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
+        float rest[3];
+        evaluateRest(coeff, degree, dx, dy, dz, rest);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) colour[ch] = fmaxS((0.5f + kShC0 * row[ch]) + rest[ch], 0.0f);
+    };
+    const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
+    uint2 rect = emptyRect;
+    const uint32_t touched = projectOne(f, d.shape, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
+    rects[slot] = touched > 0 ? rect : emptyRect;
+    touchedOut[slot] = touched;
+}
+
+__global__ __launch_bounds__(kBlock) void gatherSlotsKernel(const uint32_t* __restrict__ pairIndex,
+                                                            const uint32_t* __restrict__ pairSlot, uint32_t pairs,
+                                                            uint32_t* __restrict__ sortedSlots) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i < pairs) sortedSlots[i] = pairSlot[pairIndex[i]];
+}
+
+// ---- blend backward ---------------------------------------------------------
+
+// DPP lane moves for the wave sum; lanes outside row_mask read 0.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dppMove(float v) {
+    return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over the 64 lanes, landing in lane 63: pairs, quads, half rows and rows
+// by quad_perm and the row mirrors, then row 0 into row 1 and row 2 into row 3
+// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31). Every lane must
+// be active.
+__device__ __forceinline__ float waveSum(float v) {
+    v += dppMove<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
+    v += dppMove<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
+    v += dppMove<0x141, 0xF>(v);   // row_half_mirror
+    v += dppMove<0x140, 0xF>(v);   // row_mirror
+    v += dppMove<0x142, 0xA>(v);   // row_bcast:15
+    v += dppMove<0x143, 0xC>(v);   // row_bcast:31
+    return v;
+}
+
+// sRGB decode and its derivative, for the blend's linearisation
+// (reference color.slang:7-10).
+__device__ __forceinline__ void srgbToLinearGrad(float x, float& value, float& slope) {
+    x = fmaxS(x, 0.0f);
+    if (x <= 0.04045f) {
+        value = x / 12.92f;
+        slope = 1.0f / 12.92f;
+    } else {
+        const float u = (x + 0.055f) / 1.055f;
+        value = expF(2.4f * logF(u));
+        slope = 2.4f / 1.055f * (value / u);
+    }
+}
+
+constexpr uint32_t kWaves = 4;
+constexpr uint32_t kBatch = 128;   // pairs staged per step, by the first 128 threads
+
+__global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uint32_t slots,
+                                                              const uint2* __restrict__ ranges,
+                                                              const uint32_t* __restrict__ sortedSlots,
+                                                              const uint32_t* __restrict__ pairIndex,
+                                                              const float4* __restrict__ proj,
+                                                              const float4* __restrict__ accum,
+                                                              const uint32_t* __restrict__ lastCount,
+                                                              const float* __restrict__ finalT,
+                                                              const float4* __restrict__ gradColour,
+                                                              float4* __restrict__ pairGrad) {
+    __shared__ float4 sGeom[kBatch];     // centre.xy, cutoff, alpha
+    __shared__ float4 sConic[kBatch];    // a, 2b, c, (unused)
+    __shared__ float4 sColour[kBatch];   // rgb, bits of the pair's place in the tile's range
+    __shared__ uint32_t sMask[kBatch];   // quadrants the pair can reach
+    __shared__ uint16_t sList[kWaves][kBatch];
+    __shared__ uint32_t sCount[kWaves][kWaves];
+    __shared__ float4 sPart[kWaves][kBatch][3];
+    __shared__ uint32_t sMaxLast;
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u;
+    const uint32_t wave = tid >> 6;
+    const uint32_t tile = blockIdx.x;
+    const uint32_t tileX = tile % f.tilesX;
+    const uint32_t tileY = tile / f.tilesX;
+    const uint32_t px = tileX * kTile + (wave & 1u) * 8u + (lane & 7u);
+    const uint32_t py = tileY * kTile + (wave >> 1) * 8u + (lane >> 3);
+    const bool inside = px < f.width && py < f.height;
+    const uint2 range = ranges[tile];
+    const float pxf = static_cast<float>(px) + 0.5f;
+    const float pyf = static_cast<float>(py) + 0.5f;
+    float boxLo[kWaves][2], boxHi[kWaves][2];
+#pragma unroll
+    for (uint32_t q = 0; q < kWaves; ++q) {
+        const uint32_t x0 = tileX * kTile + (q & 1u) * 8u, y0 = tileY * kTile + (q >> 1) * 8u;
+        const uint32_t x1 = x0 + 8u < f.width ? x0 + 8u : f.width;
+        const uint32_t y1 = y0 + 8u < f.height ? y0 + 8u : f.height;
+        boxLo[q][0] = static_cast<float>(x0) + 0.5f;
+        boxLo[q][1] = static_cast<float>(y0) + 0.5f;
+        boxHi[q][0] = x1 > x0 ? static_cast<float>(x1 - 1u) + 0.5f : -1e30f;
+        boxHi[q][1] = y1 > y0 ? static_cast<float>(y1 - 1u) + 0.5f : -1e30f;
+    }
+
+    // The pixel's seeds: dL/d(raw rgb) and dL/d(final T) through the
+    // linearisation and the background (splat_blend.slang:89-107).
+    uint32_t last = 0;
+    float tFinal = 1.0f, dR[3] = {0.0f, 0.0f, 0.0f}, dT = 0.0f;
+    if (inside) {
+        const uint32_t at = py * f.width + px;
+        last = lastCount[at];
+        tFinal = finalT[at];
+        const float4 go = gradColour[at];
+        const float4 raw = accum[at];
+        const float coverage = 1.0f - tFinal;
+        const float g[3] = {go.x, go.y, go.z};
+        const float rgb[3] = {raw.x, raw.y, raw.z};
+        dT = go.w * (f.bg[3] - 1.0f);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            dT += g[ch] * f.bg[ch];
+            if (f.linearise && coverage > 1e-6f) {
+                const float x = rgb[ch] / coverage;
+                float value, slope;
+                srgbToLinearGrad(x, value, slope);
+                dR[ch] = g[ch] * slope;
+                dT -= g[ch] * (value - slope * x);   // d coverage / d T = -1
+            } else {
+                dR[ch] = g[ch];
+            }
+        }
+    }
+    if (tid == 0) sMaxLast = 0;
+    __syncthreads();
+    if (last > 0) atomicMax(&sMaxLast, last);
+    __syncthreads();
+    // A last contributor lies inside its tile's range; the bound keeps the walk there.
+    const uint32_t maxLast = sMaxLast < range.y - range.x ? sMaxLast : range.y - range.x;
+    // Pairs past every pixel's last contributor get zero gradients.
+    for (uint32_t pos = range.x + maxLast + tid; pos < range.y; pos += kBlock) {
+        float4* out = pairGrad + static_cast<uint64_t>(pairIndex[pos]) * 3;
+        out[0] = out[1] = out[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    }
+
+    float t = tFinal, behind[3] = {0.0f, 0.0f, 0.0f};
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t top = range.x + maxLast; top > range.x;) {
+        const uint32_t lo = top - range.x > kBatch ? top - kBatch : range.x;
+        const uint32_t n = top - lo;
+        // 1. stage entry k = the pair at top - 1 - k, with its quadrant mask
+        uint32_t mask = 0;
+        if (tid < kBatch) {
+            float4 screen = make_float4(0.0f, 0.0f, 0.0f, -1.0f), conic = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            float4 col = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            uint32_t place = 0;
+            if (tid < n) {
+                const uint32_t pos = top - 1u - tid;
+                place = pos - range.x;
+                const uint32_t slot = sortedSlots[pos];
+                if (slot < slots) {
+                    const uint64_t s3 = static_cast<uint64_t>(slot) * 3;
+                    screen = proj[s3];
+                    conic = proj[s3 + 1];
+                    col = proj[s3 + 2];
+#pragma unroll
+                    for (uint32_t q = 0; q < kWaves; ++q) {
+                        if (boxHi[q][0] < 0.0f || boxHi[q][1] < 0.0f) continue;
+                        if (quadrantReach(conic.x, conic.y, conic.z, screen.w, boxLo[q][0] - screen.x,
+                                             boxLo[q][1] - screen.y, boxHi[q][0] - screen.x,
+                                             boxHi[q][1] - screen.y)) {
+                            mask |= 1u << q;
+                        }
+                    }
+                }
+            }
+            sGeom[tid] = make_float4(screen.x, screen.y, screen.w, conic.w);
+            sConic[tid] = make_float4(conic.x, conic.y + conic.y, conic.z, 0.0f);
+            sColour[tid] = make_float4(col.x, col.y, col.z, __uint_as_float(place));
+            sMask[tid] = mask;
+        }
+        // 2. one list per quadrant, back to front
+        uint32_t before[kWaves];
+#pragma unroll
+        for (uint32_t q = 0; q < kWaves; ++q) {
+            const uint64_t vote = __ballot((mask >> q) & 1u);
+            before[q] = static_cast<uint32_t>(__popcll(vote & below));
+            if (lane == 0) sCount[q][wave] = static_cast<uint32_t>(__popcll(vote));
+        }
+        __syncthreads();
+#pragma unroll
+        for (uint32_t q = 0; q < kWaves; ++q) {
+            if ((mask >> q) & 1u) {
+                uint32_t at = before[q];
+#pragma unroll
+                for (uint32_t w = 0; w + 1 < kWaves; ++w) at += w < wave ? sCount[q][w] : 0u;
+                sList[q][at] = static_cast<uint16_t>(tid);
+            }
+        }
+        const uint32_t count = static_cast<uint32_t>(__builtin_amdgcn_readfirstlane(
+            static_cast<int>(sCount[wave][0] + sCount[wave][1] + sCount[wave][2] + sCount[wave][3])));
+        __syncthreads();
+        // 3. this quadrant's pixels, pair by pair, back to front
+        for (uint32_t j = 0; j < count; ++j) {
+            const uint32_t k = sList[wave][j];
+            const float4 g0 = sGeom[k];
+            const float4 g1 = sConic[k];
+            const float4 g2 = sColour[k];
+            const uint32_t place = __float_as_uint(g2.w);
+            // The forward's arithmetic, so the same pairs pass the same tests.
+            const float dx = pxf - g0.x;
+            const float dy = pyf - g0.y;
+            const float power = __builtin_fmaf(dx, __builtin_fmaf(g1.x, dx, g1.y * dy), (g1.z * dy) * dy);
+            const float gauss = __builtin_amdgcn_exp2f(kExpScale * power);
+            const float raw = g0.w * gauss;
+            const float alpha = __builtin_fminf(0.99f, raw);
+            const bool ok = (place < last) & (power <= g0.z) & (alpha >= 1.0f / 255.0f);
+            float v[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+            if (ok) {
+                const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
+                const float tBefore = t * inv;
+                const float c[3] = {g2.x, g2.y, g2.z};
+                float dAlpha = -dT * tFinal * inv;
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) dAlpha += dR[ch] * (tBefore * c[ch] - behind[ch] * inv);
+                const float weight = tBefore * alpha;
+#pragma unroll
+                for (int ch = 0; ch < 3; ++ch) {
+                    v[6 + ch] = dR[ch] * weight;
+                    behind[ch] += weight * c[ch];
+                }
+                t = tBefore;
+                if (raw < 0.99f) {
+                    // alpha = opacity * exp(-power / 2)
+                    const float dPower = -0.5f * alpha * dAlpha;
+                    v[5] = dAlpha * gauss;
+                    v[0] = -dPower * (2.0f * g1.x * dx + g1.y * dy);
+                    v[1] = -dPower * (g1.y * dx + 2.0f * g1.z * dy);
+                    v[2] = dPower * dx * dx;
+                    v[3] = dPower * 2.0f * dx * dy;
+                    v[4] = dPower * dy * dy;
+                }
+            }
+            if (__ballot(ok) != 0) {
+#pragma unroll
+                for (int e = 0; e < 9; ++e) v[e] = waveSum(v[e]);
+            }
+            if (lane == 63) {
+                sPart[wave][k][0] = make_float4(v[0], v[1], v[2], v[3]);
+                sPart[wave][k][1] = make_float4(v[4], v[5], v[6], v[7]);
+                sPart[wave][k][2] = make_float4(v[8], 0.0f, 0.0f, 0.0f);
+            }
+        }
+        __syncthreads();
+        // 4. the four quadrants' sums, one record per pair
+        if (tid < n) {
+            const uint32_t m = sMask[tid];
+            float4 acc[3] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f),
+                             make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+#pragma unroll
+            for (uint32_t q = 0; q < kWaves; ++q) {
+                if ((m >> q) & 1u) {
+#pragma unroll
+                    for (int r = 0; r < 3; ++r) {
+                        const float4 p = sPart[q][tid][r];
+                        acc[r].x += p.x;
+                        acc[r].y += p.y;
+                        acc[r].z += p.z;
+                        acc[r].w += p.w;
+                    }
+                }
+            }
+            float4* out = pairGrad + static_cast<uint64_t>(pairIndex[top - 1u - tid]) * 3;
+            out[0] = acc[0];
+            out[1] = acc[1];
+            out[2] = acc[2];
+        }
+        __syncthreads();
+        top = lo;
+    }
+}
+
+// ---- preprocess backward ------------------------------------------------------
+
+// The rest of the colour and its gradients: given dL/d(rest) per channel,
+// dL/d(coefficient) and dL/d(direction) (sh.slang:25-50 differentiated).
+__device__ __forceinline__ void evaluateRestGrad(const float* k, uint32_t degree, float x, float y, float z,
+                                                 const float dRest[3], float* dCoeff, float dDir[3]) {
+    float basis[15];
+    float grad[15][3];
+    const float kC1 = 0.4886025119029199f;
+    basis[0] = -kC1 * y; grad[0][0] = 0.0f; grad[0][1] = -kC1; grad[0][2] = 0.0f;
+    basis[1] = kC1 * z;  grad[1][0] = 0.0f; grad[1][1] = 0.0f; grad[1][2] = kC1;
+    basis[2] = -kC1 * x; grad[2][0] = -kC1; grad[2][1] = 0.0f; grad[2][2] = 0.0f;
+    const float xx = x * x, yy = y * y, zz = z * z;
+    const float c20 = 1.0925484305920792f, c21 = -1.0925484305920792f, c22 = 0.31539156525252005f,
+                c23 = -1.0925484305920792f, c24 = 0.5462742152960396f;
+    basis[3] = c20 * x * y;               grad[3][0] = c20 * y;  grad[3][1] = c20 * x;   grad[3][2] = 0.0f;
+    basis[4] = c21 * y * z;               grad[4][0] = 0.0f;     grad[4][1] = c21 * z;   grad[4][2] = c21 * y;
+    basis[5] = c22 * (2.0f * zz - xx - yy);
+    grad[5][0] = -2.0f * c22 * x; grad[5][1] = -2.0f * c22 * y; grad[5][2] = 4.0f * c22 * z;
+    basis[6] = c23 * x * z;               grad[6][0] = c23 * z;  grad[6][1] = 0.0f;      grad[6][2] = c23 * x;
+    basis[7] = c24 * (xx - yy);           grad[7][0] = 2.0f * c24 * x; grad[7][1] = -2.0f * c24 * y; grad[7][2] = 0.0f;
+    const float c30 = -0.5900435899266435f, c31 = 2.890611442640554f, c32 = -0.4570457994644658f,
+                c33 = 0.3731763325901154f, c34 = -0.4570457994644658f, c35 = 1.445305721320277f,
+                c36 = -0.5900435899266435f;
+    basis[8] = c30 * y * (3.0f * xx - yy);
+    grad[8][0] = c30 * 6.0f * x * y; grad[8][1] = c30 * (3.0f * xx - 3.0f * yy); grad[8][2] = 0.0f;
+    basis[9] = c31 * x * y * z;
+    grad[9][0] = c31 * y * z; grad[9][1] = c31 * x * z; grad[9][2] = c31 * x * y;
+    basis[10] = c32 * y * (4.0f * zz - xx - yy);
+    grad[10][0] = c32 * -2.0f * x * y; grad[10][1] = c32 * (4.0f * zz - xx - 3.0f * yy); grad[10][2] = c32 * 8.0f * y * z;
+    basis[11] = c33 * z * (2.0f * zz - 3.0f * xx - 3.0f * yy);
+    grad[11][0] = c33 * -6.0f * x * z; grad[11][1] = c33 * -6.0f * y * z;
+    grad[11][2] = c33 * (6.0f * zz - 3.0f * xx - 3.0f * yy);
+    basis[12] = c34 * x * (4.0f * zz - xx - yy);
+    grad[12][0] = c34 * (4.0f * zz - 3.0f * xx - yy); grad[12][1] = c34 * -2.0f * x * y; grad[12][2] = c34 * 8.0f * x * z;
+    basis[13] = c35 * z * (xx - yy);
+    grad[13][0] = c35 * 2.0f * x * z; grad[13][1] = c35 * -2.0f * y * z; grad[13][2] = c35 * (xx - yy);
+    basis[14] = c36 * x * (xx - 3.0f * yy);
+    grad[14][0] = c36 * (3.0f * xx - 3.0f * yy); grad[14][1] = c36 * -6.0f * x * y; grad[14][2] = 0.0f;
+    const uint32_t used = restCount(degree);
+    dDir[0] = dDir[1] = dDir[2] = 0.0f;
+#pragma unroll
+    for (uint32_t bI = 0; bI < 15; ++bI) {
+        const bool on = bI < used;
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) {
+            dCoeff[bI * 3 + ch] = on ? basis[bI] * dRest[ch] : 0.0f;
+            const float w = on ? k[bI * 3 + ch] * dRest[ch] : 0.0f;
+            dDir[0] += w * grad[bI][0];
+            dDir[1] += w * grad[bI][1];
+            dDir[2] += w * grad[bI][2];
+        }
+    }
+}
+
+__global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, DiffCloud c,
+                                                                const uint32_t* __restrict__ touched,
+                                                                const uint32_t* __restrict__ offsets,
+                                                                const float4* __restrict__ pairGrad,
+                                                                DiffGrads out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.count) return;
+    const uint32_t slot = f.base + i;
+    const uint32_t stride = (1u + c.restPerColour) * 3u;
+    float* gMean = out.means + static_cast<uint64_t>(i) * 3;
+    float* gScale = out.logScales + static_cast<uint64_t>(i) * 3;
+    float* gQuat = out.quats + static_cast<uint64_t>(i) * 4;
+    float* gSh = out.sh + static_cast<uint64_t>(i) * stride;
+    const uint32_t n = touched[slot];
+    // The splat's screen-space gradient: its run of pair records.
+    float g[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
+    if (n > 0) {
+        const float4* run = pairGrad + static_cast<uint64_t>(offsets[slot]) * 3;
+        for (uint32_t e = 0; e < n; ++e) {
+            const float4 a = run[e * 3], b = run[e * 3 + 1], d = run[e * 3 + 2];
+            g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
+            g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
+            g[8] += d.x;
+        }
+    }
+    if (n == 0) {
+        for (int k = 0; k < 3; ++k) gMean[k] = gScale[k] = 0.0f;
+        for (int k = 0; k < 4; ++k) gQuat[k] = 0.0f;
+        out.logits[i] = 0.0f;
+        for (uint32_t k = 0; k < stride; ++k) gSh[k] = 0.0f;
+        return;
+    }
+    const float dCx = g[0], dCy = g[1], dCa = g[2], dCb = g[3], dCc = g[4], dAlpha = g[5];
+    const float dCol[3] = {g[6], g[7], g[8]};
+
+    // Recompute the forward (projectOne) and walk it back.
+    const Decoded dec = decode(c, i, f.useFilter3d != 0);
+    const SplatShape& s = dec.shape;
+    const float* m = f.m;
+    const float tx = m[0] * s.px + m[1] * s.py + m[2] * s.pz + m[3];
+    const float ty = m[4] * s.px + m[5] * s.py + m[6] * s.pz + m[7];
+    const float tz = m[8] * s.px + m[9] * s.py + m[10] * s.pz + m[11];
+    float r[9];
+    rotationOf(s.q, r);
+    const float sc[3] = {s.sx, s.sy, s.sz};
+    float mm[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int b = 0; b < 3; ++b) mm[a * 3 + b] = r[a * 3 + b] * sc[b];
+    float jx0, jx2, jy1, jy2;
+    bool clampX = false, clampY = false;
+    float limX = 0.0f, limY = 0.0f;
+    const float invZ = 1.0f / tz;
+    if (f.orthographic) {
+        jx0 = f.focalX; jx2 = 0.0f;
+        jy1 = f.focalY; jy2 = 0.0f;
+    } else {
+        const float limLoX = (-0.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
+        const float limHiX = (1.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
+        const float limLoY = (-0.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
+        const float limHiY = (1.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
+        const float ux = tx * invZ, uy = ty * invZ;
+        clampX = ux < limLoX || ux > limHiX;
+        clampY = uy < limLoY || uy > limHiY;
+        limX = ux < limLoX ? limLoX : limHiX;
+        limY = uy < limLoY ? limLoY : limHiY;
+        const float ctx = clampS(ux, limLoX, limHiX) * tz;
+        const float cty = clampS(uy, limLoY, limHiY) * tz;
+        jx0 = f.focalX * invZ;
+        jx2 = -f.focalX * ctx * invZ * invZ;
+        jy1 = f.focalY * invZ;
+        jy2 = -f.focalY * cty * invZ * invZ;
+    }
+    const float t3x[3] = {jx0 * m[0] + jx2 * m[8], jx0 * m[1] + jx2 * m[9], jx0 * m[2] + jx2 * m[10]};
+    const float t3y[3] = {jy1 * m[4] + jy2 * m[8], jy1 * m[5] + jy2 * m[9], jy1 * m[6] + jy2 * m[10]};
+    float ax[3], ay[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        ax[j] = t3x[0] * mm[0 * 3 + j] + t3x[1] * mm[1 * 3 + j] + t3x[2] * mm[2 * 3 + j];
+        ay[j] = t3y[0] * mm[0 * 3 + j] + t3y[1] * mm[1 * 3 + j] + t3y[2] * mm[2 * 3 + j];
+    }
+    const float a0 = ax[0] * ax[0] + ax[1] * ax[1] + ax[2] * ax[2];
+    const float b = ax[0] * ay[0] + ax[1] * ay[1] + ax[2] * ay[2];
+    const float c0 = ay[0] * ay[0] + ay[1] * ay[1] + ay[2] * ay[2];
+    const float detB = a0 * c0 - b * b;
+    const float a1 = a0 + f.dilation, c1 = c0 + f.dilation;
+    const float det = a1 * c1 - b * b;
+    const float invDet = 1.0f / det;
+
+    // alpha = opacity * sqrt(max(detB, 0) / det) with antialiasing
+    float dOpacity = dAlpha, dDet = 0.0f, dDetB = 0.0f;
+    if (f.antialias) {
+        const float ratio = fmaxS(detB, 0.0f) * invDet;
+        const float root = __builtin_sqrtf(ratio);
+        dOpacity = dAlpha * root;
+        if (detB > 0.0f && root > 0.0f) {
+            const float alpha = s.opacity * root;
+            dDetB = dAlpha * alpha * 0.5f / detB;
+            dDet = -dAlpha * alpha * 0.5f * invDet;
+        }
+    }
+    // conic = (c1, -b, a1) / det
+    dDet += -(dCa * c1 - dCb * b + dCc * a1) * invDet * invDet;
+    float dA = dCc * invDet + dDet * c1 + dDetB * c0;
+    float dC = dCa * invDet + dDet * a1 + dDetB * a0;
+    const float dB = -dCb * invDet - 2.0f * b * (dDet + dDetB);
+    // (a, b, c) = (ax.ax, ax.ay, ay.ay)
+    float dAx[3], dAy[3];
+#pragma unroll
+    for (int j = 0; j < 3; ++j) {
+        dAx[j] = 2.0f * dA * ax[j] + dB * ay[j];
+        dAy[j] = dB * ax[j] + 2.0f * dC * ay[j];
+    }
+    // A = T3 M
+    float dT3x[3], dT3y[3], dM[9];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        dT3x[a] = dAx[0] * mm[a * 3 + 0] + dAx[1] * mm[a * 3 + 1] + dAx[2] * mm[a * 3 + 2];
+        dT3y[a] = dAy[0] * mm[a * 3 + 0] + dAy[1] * mm[a * 3 + 1] + dAy[2] * mm[a * 3 + 2];
+#pragma unroll
+        for (int j = 0; j < 3; ++j) dM[a * 3 + j] = t3x[a] * dAx[j] + t3y[a] * dAy[j];
+    }
+    // M = R diag(s)
+    float dR[9], dS[3] = {0.0f, 0.0f, 0.0f};
+#pragma unroll
+    for (int a = 0; a < 3; ++a)
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            dR[a * 3 + j] = dM[a * 3 + j] * sc[j];
+            dS[j] += dM[a * 3 + j] * r[a * 3 + j];
+        }
+    // the 3D filter: s' = sqrt(s^2 + f^2), opacity * prod(s / s')
+    float dSigmoid = dOpacity;
+    float dRaw[3] = {dS[0], dS[1], dS[2]};
+    if (f.useFilter3d) {
+        const float fl = c.filter3d[i];
+        const float factor = s.opacity / dec.sigmoid;
+        dSigmoid = dOpacity * factor;
+#pragma unroll
+        for (int j = 0; j < 3; ++j) {
+            const float sj = dec.s[j], sp = sc[j];
+            dRaw[j] = dS[j] * sj / sp + dOpacity * s.opacity * (fl * fl) / (sj * sp * sp);
+        }
+    }
+#pragma unroll
+    for (int j = 0; j < 3; ++j) gScale[j] = dRaw[j] * dec.s[j];   // s = exp(log s)
+    out.logits[i] = dSigmoid * dec.sigmoid * (1.0f - dec.sigmoid);
+
+    // R(q), q normalised
+    {
+        const float x = s.q[0], y = s.q[1], z = s.q[2], w = s.q[3];
+        const float dx = 2.0f * (dR[1] * y + dR[2] * z + dR[3] * y - 2.0f * dR[4] * x - dR[5] * w + dR[6] * z +
+                                 dR[7] * w - 2.0f * dR[8] * x);
+        const float dy = 2.0f * (-2.0f * dR[0] * y + dR[1] * x + dR[2] * w + dR[3] * x + dR[5] * z - dR[6] * w +
+                                 dR[7] * z - 2.0f * dR[8] * y);
+        const float dz = 2.0f * (-2.0f * dR[0] * z - dR[1] * w + dR[2] * x + dR[3] * w - 2.0f * dR[4] * z +
+                                 dR[5] * y + dR[6] * x + dR[7] * y);
+        const float dw = 2.0f * (-dR[1] * z + dR[2] * y + dR[3] * z - dR[5] * x - dR[6] * y + dR[7] * x);
+        const float dq[4] = {dx, dy, dz, dw};
+        if (dec.qLen > 0.0f) {
+            const float proj1 = dq[0] * s.q[0] + dq[1] * s.q[1] + dq[2] * s.q[2] + dq[3] * s.q[3];
+#pragma unroll
+            for (int k = 0; k < 4; ++k) gQuat[k] = (dq[k] - s.q[k] * proj1) / dec.qLen;
+        } else {
+#pragma unroll
+            for (int k = 0; k < 4; ++k) gQuat[k] = 0.0f;
+        }
+    }
+
+    // J and the centre, from the view position t = W p + w
+    float dT[3] = {0.0f, 0.0f, 0.0f};
+    if (f.orthographic) {
+        dT[0] = dCx * f.focalX;
+        dT[1] = dCy * f.focalY;
+    } else {
+        const float dJx0 = dT3x[0] * m[0] + dT3x[1] * m[1] + dT3x[2] * m[2];
+        const float dJx2 = dT3x[0] * m[8] + dT3x[1] * m[9] + dT3x[2] * m[10];
+        const float dJy1 = dT3y[0] * m[4] + dT3y[1] * m[5] + dT3y[2] * m[6];
+        const float dJy2 = dT3y[0] * m[8] + dT3y[1] * m[9] + dT3y[2] * m[10];
+        const float iz2 = invZ * invZ;
+        dT[2] += -(dJx0 * f.focalX + dJy1 * f.focalY) * iz2;
+        if (clampX) {
+            dT[2] += dJx2 * f.focalX * limX * iz2;
+        } else {
+            dT[0] += -dJx2 * f.focalX * iz2;
+            dT[2] += dJx2 * 2.0f * f.focalX * tx * iz2 * invZ;
+        }
+        if (clampY) {
+            dT[2] += dJy2 * f.focalY * limY * iz2;
+        } else {
+            dT[1] += -dJy2 * f.focalY * iz2;
+            dT[2] += dJy2 * 2.0f * f.focalY * ty * iz2 * invZ;
+        }
+        dT[0] += dCx * f.focalX * invZ;
+        dT[1] += dCy * f.focalY * invZ;
+        dT[2] += -(dCx * f.focalX * tx + dCy * f.focalY * ty) * iz2;
+    }
+    float dP[3];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) dP[k] = m[0 + k] * dT[0] + m[4 + k] * dT[1] + m[8 + k] * dT[2];
+
+    // colour = max(0.5 + C0 dc + rest(dir), 0), dir = (p - eye) / |p - eye|
+    const uint32_t degree = shDegree(f);
+    const float* row = c.sh + static_cast<uint64_t>(i) * stride;
+    const float dxo = s.px - f.eye[0], dyo = s.py - f.eye[1], dzo = s.pz - f.eye[2];
+    const float len = __builtin_sqrtf((dxo * dxo + dyo * dyo) + dzo * dzo);
+    const float dir[3] = {dxo / len, dyo / len, dzo / len};
+    float coeff[45];
+    const uint32_t used = restCount(degree) * 3u;
+#pragma unroll
+    for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
+    float rest[3];
+    evaluateRest(coeff, degree, dir[0], dir[1], dir[2], rest);
+    float dPre[3];
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) {
+        const float pre = (0.5f + kShC0 * row[ch]) + rest[ch];
+        dPre[ch] = pre > 0.0f ? dCol[ch] : 0.0f;
+        gSh[ch] = kShC0 * dPre[ch];
+    }
+    float dCoeff[45], dDir[3];
+    evaluateRestGrad(coeff, degree, dir[0], dir[1], dir[2], dPre, dCoeff, dDir);
+#pragma unroll
+    for (uint32_t k = 0; k < 45; ++k)
+        if (k + 3u < stride) gSh[3 + k] = dCoeff[k];
+    const float along = dDir[0] * dir[0] + dDir[1] * dir[1] + dDir[2] * dir[2];
+#pragma unroll
+    for (int k = 0; k < 3; ++k) gMean[k] = dP[k] + (dDir[k] - dir[k] * along) / len;
+}
+
+}   // namespace
+
+void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uint2* rects, uint32_t* touched,
+                 hipStream_t stream) {
+    if (f.count == 0) return;
+    projectDiffKernel<<<(f.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(f, cloud, proj, rects, touched);
+}
+
+void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
+                 hipStream_t stream) {
+    if (pairs == 0) return;
+    gatherSlotsKernel<<<(pairs + kBlock - 1) / kBlock, kBlock, 0, stream>>>(pairIndex, pairSlot, pairs, sortedSlots);
+}
+
+void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
+                   const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
+                   const float* finalT, const float4* gradColour, float4* pairGrad, hipStream_t stream) {
+    const uint32_t tiles = f.tilesX * f.tilesY;
+    if (tiles == 0) return;
+    blendBackwardKernel<<<tiles, kBlock, 0, stream>>>(f, slots, ranges, sortedSlots, pairIndex, proj, accum,
+                                                      lastCount, finalT, gradColour, pairGrad);
+}
+
+void projectBackward(const FrameParams& f, const DiffCloud& cloud, const uint32_t* touched, const uint32_t* offsets,
+                     const float4* pairGrad, const DiffGrads& grads, hipStream_t stream) {
+    if (f.count == 0) return;
+    projectBackwardKernel<<<(f.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(f, cloud, touched, offsets,
+                                                                                  pairGrad, grads);
+}
+
+}   // namespace lrt::kern

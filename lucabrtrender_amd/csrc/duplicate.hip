@@ -9,6 +9,11 @@
 // pairs leave this kernel in slot order, which is what makes one stable sort on
 // (tile, depth >> 7) give each tile its splats in the reference's order:
 // depth key first, slot on ties.
+//
+// For a render whose gradients will be asked for, the value is instead the
+// pair's own index in this slot-major order (and its slot goes to pairSlot):
+// after the sort each pair then knows where its splat's run of pairs lies, so
+// per-pair gradients written there sum per splat over contiguous runs.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -21,13 +26,15 @@ using namespace lrt::dev;
 
 constexpr uint32_t kBlock = 256;
 
+template <bool EMIT_INDEX>
 __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_t slots,
                                                           const float4* __restrict__ proj,
                                                           const uint2* __restrict__ rects,
                                                           const uint32_t* __restrict__ touched,
                                                           const uint32_t* __restrict__ offsets,
                                                           uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ values, uint32_t capacity,
+                                                          uint32_t* __restrict__ values,
+                                                          uint32_t* __restrict__ pairSlot, uint32_t capacity,
                                                           uint32_t* __restrict__ overflow) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     if (s >= slots) return;
@@ -48,7 +55,12 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
             const uint32_t at = first + written;
             if (at < capacity) {
                 keys[at] = (static_cast<uint64_t>(ty * f.tilesX + tx) << 32) | depthBits;
-                values[at] = s;
+                if constexpr (EMIT_INDEX) {
+                    values[at] = at;
+                    pairSlot[at] = s;
+                } else {
+                    values[at] = s;
+                }
             } else {
                 atomicAdd(overflow, 1u);
             }
@@ -61,11 +73,16 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
 
 void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
                const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
-               uint32_t capacity, uint32_t* overflow, hipStream_t stream) {
+               uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream) {
     if (slots == 0) return;
     const uint32_t blocks = (slots + kBlock - 1) / kBlock;
-    duplicateKernel<<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values, capacity,
-                                                   overflow);
+    if (pairSlot != nullptr) {
+        duplicateKernel<true><<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values,
+                                                             pairSlot, capacity, overflow);
+    } else {
+        duplicateKernel<false><<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values,
+                                                              nullptr, capacity, overflow);
+    }
 }
 
 }   // namespace lrt::kern

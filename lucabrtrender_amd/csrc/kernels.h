@@ -33,11 +33,12 @@ void project(const FrameParams& f, const float4* positions, const uint4* shape, 
 
 // ---- duplication ------------------------------------------------------------
 // For each of `slots` slots with touched > 0: its pairs at offsets[slot].., keys
-// (tile << 32 | bits(view z)) and values (slot). Writes past `capacity` are
-// dropped and counted in *overflow.
+// (tile << 32 | bits(view z)) and values (slot). With pairSlot non-null the
+// value is instead the pair's own index and pairSlot[index] its slot. Writes
+// past `capacity` are dropped and counted in *overflow.
 void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
                const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
-               uint32_t capacity, uint32_t* overflow, hipStream_t stream);
+               uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream);
 
 // ---- sort -------------------------------------------------------------------
 // Which bits of a 64-bit key the digits come from.
@@ -74,9 +75,47 @@ void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ran
 void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t stream);
 // Per pixel, front to back over its tile's pairs; one wave per 8x8 quadrant,
 // whose LDS list holds `list` (128 or 256) records; tiles in `order` (null:
-// index order); slots bounds the pair values. lastCount/finalT may be null.
+// index order); slots bounds the pair values. lastCount/finalT/accum may be
+// null; accum receives each pixel's raw rgb and depth sum before the
+// linearisation and the background.
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-           float* finalT, hipStream_t stream);
+           float* finalT, float4* accum, hipStream_t stream);
+
+// ---- differentiable path -------------------------------------------------------
+// An FP32-parameter cloud (device pointers): means N x 3, log scales N x 3,
+// quaternions N x 4 (x, y, z, w; any length), opacity logits N, harmonics
+// N x (1 + restPerColour) x 3 (dc first, rgb per basis), filter3d N or null.
+struct DiffCloud {
+    const float* means;
+    const float* logScales;
+    const float* quats;
+    const float* logits;
+    const float* sh;
+    const float* filter3d;
+    uint32_t count;
+    uint32_t restPerColour;
+};
+// Gradients of the same shapes, overwritten.
+struct DiffGrads {
+    float* means;
+    float* logScales;
+    float* quats;
+    float* logits;
+    float* sh;
+};
+void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uint2* rects, uint32_t* touched,
+                 hipStream_t stream);
+// sortedSlots[i] = pairSlot[pairIndex[i]] for i < pairs.
+void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
+                 hipStream_t stream);
+// dL/d(colour) per pixel -> one screen-space gradient record per pair (3 float4:
+// d centre.xy, d conic a b; d conic c, d alpha, d r g; d b) at pairGrad[index].
+void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
+                   const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
+                   const float* finalT, const float4* gradColour, float4* pairGrad, hipStream_t stream);
+// Each splat's run of pair records -> gradients of its parameters.
+void projectBackward(const FrameParams& f, const DiffCloud& cloud, const uint32_t* touched, const uint32_t* offsets,
+                     const float4* pairGrad, const DiffGrads& grads, hipStream_t stream);
 
 }   // namespace lrt::kern

@@ -14,6 +14,7 @@
 #include <string>
 
 #include "frame.h"
+#include "kernels.h"
 
 namespace lrt {
 
@@ -162,6 +163,15 @@ public:
         return render(projectionFor(camera, settings.width, settings.height), instances, settings, targets, stats);
     }
 
+    // The differentiable frame: one FP32-parameter cloud, drawn by the same
+    // stages as render(); it keeps what backward() needs.
+    Status renderDiff(const Projection& projection, const kern::DiffCloud& cloud, const Mat4& objectToWorld,
+                      const RenderSettings& settings, RenderTargets& targets, FrameStats* stats);
+    // Gradients of the last renderDiff's parameters given dL/d(colour) (one
+    // float4 a pixel, bottom row first). millis (may be null) receives the GPU
+    // ms of the blend backward and of the preprocess backward.
+    Status backward(const float4* gradColour, const kern::DiffGrads& grads, float* millis);
+
     // PrefixSum::apply and RadixSort::sort over caller buffers, on this stream.
     Status prefixSum(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count);
     Status radixSort(uint64_t* keys, uint32_t* values, uint64_t* scratchKeys, uint32_t* scratchValues,
@@ -172,8 +182,12 @@ public:
 
 private:
     Status reserveSplats(uint32_t count);
-    Status reservePairs(uint32_t count, uint32_t keyBits);
-    Status reserveTargets(uint32_t width, uint32_t height, uint32_t tiles);
+    Status reservePairs(uint32_t count, uint32_t keyBits, bool training = false);
+    Status reserveTargets(uint32_t width, uint32_t height, uint32_t tiles, bool training);
+    Status checkFrame(const RenderSettings& settings, const RenderTargets& targets) const;
+    // Everything after the projection: scan, duplicate, sort, ranges, blend.
+    Status finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
+                       RenderTargets& targets, FrameStats* stats, bool training);
     Status check(hipError_t e, const char* what);
 
     int device_ = 0;
@@ -186,11 +200,14 @@ private:
     // Per pair, ping-pong.
     uint32_t pairCapacity_ = 0;
     DeviceBuffer keys_[2], values_[2], sortWork_;
+    // Training frames: each pair's slot in slot-major order, the slots in sorted
+    // order, and the backward's per-pair gradient records.
+    DeviceBuffer pairSlot_, sortedSlots_, pairGrad_;
     uint64_t sortWorkWords_ = 0;
     // Per tile and per pixel.
     uint32_t tileCapacity_ = 0;
     uint64_t pixelCapacity_ = 0;
-    DeviceBuffer ranges_, tileOrder_, lastCount_, finalT_;
+    DeviceBuffer ranges_, tileOrder_, lastCount_, finalT_, accum_;
     // Counters: [0] pairs, [1] visible, [2] overflow, [3] scan ticket, [4] sort error, [5] scan error.
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
@@ -204,6 +221,10 @@ private:
     bool lastOrdered_ = false;
     uint64_t lastPixels_ = 0;
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
+    // The last differentiable frame.
+    bool diffReady_ = false;
+    kern::DiffCloud diffCloud_{};
+    FrameParams diffFrame_{};
 };
 
 }   // namespace lrt

@@ -108,7 +108,7 @@ Status TileRasterizer::reserveSplats(uint32_t count) {
     return {};
 }
 
-Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits) {
+Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits, bool training) {
     Status s;
     if (count > pairCapacity_) {
         const uint64_t n = std::min<uint64_t>(grow(pairCapacity_, count), kMaxPairs);
@@ -118,6 +118,10 @@ Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits) {
         }
         pairCapacity_ = static_cast<uint32_t>(n);
     }
+    if (training) {
+        if (!(s = pairSlot_.reserve(uint64_t{pairCapacity_} * 4, "pair.slot")).ok()) return s;
+        if (!(s = sortedSlots_.reserve(uint64_t{pairCapacity_} * 4, "pair.sortedSlots")).ok()) return s;
+    }
     const uint64_t words = kern::sortWorkspaceWords(pairCapacity_, std::max<uint32_t>(keyBits, 40));
     if (words > sortWorkWords_) {
         if (!(s = sortWork_.reserve(words * 4, "pair.sortWorkspace")).ok()) return s;
@@ -126,7 +130,7 @@ Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits) {
     return {};
 }
 
-Status TileRasterizer::reserveTargets(uint32_t width, uint32_t height, uint32_t tiles) {
+Status TileRasterizer::reserveTargets(uint32_t width, uint32_t height, uint32_t tiles, bool training) {
     Status s;
     if (tiles > tileCapacity_) {
         if (!(s = ranges_.reserve(uint64_t{tiles} * 8, "tile.ranges")).ok()) return s;
@@ -139,11 +143,11 @@ Status TileRasterizer::reserveTargets(uint32_t width, uint32_t height, uint32_t 
         if (!(s = finalT_.reserve(pixels * 4, "pixel.finalT")).ok()) return s;
         pixelCapacity_ = pixels;
     }
+    if (training && !(s = accum_.reserve(pixels * 16, "pixel.accum")).ok()) return s;
     return {};
 }
 
-Status TileRasterizer::render(const Projection& projection, std::span<const SplatInstance> instances,
-                              const RenderSettings& settings, RenderTargets& targets, FrameStats* statsOut) {
+Status TileRasterizer::checkFrame(const RenderSettings& settings, const RenderTargets& targets) const {
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
     if (settings.width == 0 || settings.height == 0) {
         return Status::error(ErrorCode::InvalidArgument, "a frame needs a size");
@@ -154,6 +158,13 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     if (targets.width != settings.width || targets.height != settings.height || targets.colour == nullptr) {
         return Status::error(ErrorCode::InvalidArgument, "the targets are not the frame's size");
     }
+    return {};
+}
+
+Status TileRasterizer::render(const Projection& projection, std::span<const SplatInstance> instances,
+                              const RenderSettings& settings, RenderTargets& targets, FrameStats* statsOut) {
+    Status s = checkFrame(settings, targets);
+    if (!s.ok()) return s;
     uint64_t total = 0;
     for (const SplatInstance& instance : instances) {
         if (instance.splats == nullptr) continue;
@@ -180,26 +191,12 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     }
     if (total > 0xFFFFFFFFull) return Status::error(ErrorCode::Unsupported, "more than 2^32 splats in one frame");
     const uint32_t splatCount = static_cast<uint32_t>(total);
-    const uint32_t tilesX = (settings.width + kTile - 1) / kTile;
-    const uint32_t tilesY = (settings.height + kTile - 1) / kTile;
-    const uint32_t tiles = tilesX * tilesY;
-    const uint32_t keyBits = 24 + bitsFor(tiles);
-    Status s = check(hipSetDevice(device_), "hipSetDevice");
-    if (!s.ok()) return s;
+    const uint32_t tiles = ((settings.width + kTile - 1) / kTile) * ((settings.height + kTile - 1) / kTile);
+    if (!(s = check(hipSetDevice(device_), "hipSetDevice")).ok()) return s;
     if (!(s = reserveSplats(std::max(splatCount, 1u))).ok()) return s;
-    if (!(s = reserveTargets(settings.width, settings.height, tiles)).ok()) return s;
+    if (!(s = reserveTargets(settings.width, settings.height, tiles, false)).ok()) return s;
 
-    uint32_t* counters = counters_.as<uint32_t>();
-    const bool timed = settings.timeStages;
-    const auto wallStart = std::chrono::steady_clock::now();
-    FrameStats stats;
-    stats.splats = splatCount;
-    // Event k marks kernel boundary k (see kernelMs_): recorded only when timed.
-    const auto mark = [&](int k) {
-        if (timed) (void)hipEventRecord(events_[k], stream_);
-    };
-    mark(0);
-
+    if (settings.timeStages) (void)hipEventRecord(events_[0], stream_);
     // Project every instance into its own slots.
     uint32_t base = 0;
     FrameParams first{};
@@ -220,7 +217,64 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         SplatInstance none;
         first = frameParamsFor(projection, none, settings, 0);
     }
+    diffReady_ = false;
+    return finishFrame(first, splatCount, settings, targets, statsOut, false);
+}
+
+Status TileRasterizer::renderDiff(const Projection& projection, const kern::DiffCloud& cloud,
+                                  const Mat4& objectToWorld, const RenderSettings& settings, RenderTargets& targets,
+                                  FrameStats* statsOut) {
+    Status s = checkFrame(settings, targets);
+    if (!s.ok()) return s;
+    if (cloud.count > 0 && (cloud.means == nullptr || cloud.logScales == nullptr || cloud.quats == nullptr ||
+                            cloud.logits == nullptr || cloud.sh == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "a differentiable cloud with a missing parameter");
+    }
+    if (cloud.restPerColour != 0 && cloud.restPerColour != 3 && cloud.restPerColour != 8 &&
+        cloud.restPerColour != 15) {
+        return Status::error(ErrorCode::InvalidArgument, "restPerColour must be 0, 3, 8 or 15");
+    }
+    if (settings.filter3d && cloud.count > 0 && cloud.filter3d == nullptr) {
+        return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and the cloud has no filter3d");
+    }
+    const uint32_t tiles = ((settings.width + kTile - 1) / kTile) * ((settings.height + kTile - 1) / kTile);
+    if (!(s = check(hipSetDevice(device_), "hipSetDevice")).ok()) return s;
+    if (!(s = reserveSplats(std::max(cloud.count, 1u))).ok()) return s;
+    if (!(s = reserveTargets(settings.width, settings.height, tiles, true)).ok()) return s;
+
+    SplatCloud shapeOnly;
+    shapeOnly.count = cloud.count;
+    shapeOnly.restPerColour = cloud.restPerColour;
+    SplatInstance instance;
+    instance.splats = &shapeOnly;
+    instance.objectToWorld = objectToWorld;
+    const FrameParams f = frameParamsFor(projection, instance, settings, 0);
+    if (settings.timeStages) (void)hipEventRecord(events_[0], stream_);
+    kern::projectDiff(f, cloud, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(), stream_);
+    diffReady_ = false;
+    if (!(s = finishFrame(f, cloud.count, settings, targets, statsOut, true)).ok()) return s;
+    diffCloud_ = cloud;
+    diffFrame_ = f;
+    diffReady_ = true;
+    return {};
+}
+
+Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
+                                   RenderTargets& targets, FrameStats* statsOut, bool training) {
+    const auto wallStart = std::chrono::steady_clock::now();
+    const uint32_t tiles = first.tilesX * first.tilesY;
+    const uint32_t keyBits = 24 + bitsFor(tiles);
+    uint32_t* counters = counters_.as<uint32_t>();
+    const bool timed = settings.timeStages;
+    FrameStats stats;
+    stats.splats = splatCount;
+    // Event k marks kernel boundary k (see kernelMs_): recorded only when timed;
+    // event 0 was recorded before the projection.
+    const auto mark = [&](int k) {
+        if (timed) (void)hipEventRecord(events_[k], stream_);
+    };
     mark(1);
+    Status s;
 
     // How many pairs and how many visible: one scan, one readback.
     kern::fill32(counters, 0u, 8, stream_);
@@ -239,13 +293,13 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     }
     stats.visible = visible;
     stats.pairs = pairs;
-    if (!(s = reservePairs(std::max(pairs, 1u), keyBits)).ok()) return s;
+    if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) return s;
 
     // Duplicate into 64-bit keys, in slot order.
     mark(3);
     kern::duplicate(first, splatCount, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
-                    offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), pairCapacity_,
-                    counters + 2, stream_);
+                    offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(),
+                    training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_, counters + 2, stream_);
     mark(4);
 
     // One stable sort on (tile, depth >> 7), then the ranges.
@@ -260,6 +314,13 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     mark(6);
     const int sorted = inAlt ? 1 : 0;
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_);
+    // The blend reads slots; a training frame's sort carried pair indices.
+    const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
+    if (training) {
+        kern::gatherSlots(values_[sorted].as<uint32_t>(), pairSlot_.as<uint32_t>(), pairs,
+                          sortedSlots_.as<uint32_t>(), stream_);
+        pairSlots = sortedSlots_.as<uint32_t>();
+    }
     mark(7);
 
     const uint32_t* order = nullptr;
@@ -267,9 +328,9 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         kern::tileOrder(ranges_.as<uint2>(), tiles, tileOrder_.as<uint32_t>(), stream_);
         order = tileOrder_.as<uint32_t>();
     }
-    kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), values_[sorted].as<uint32_t>(),
-                proj_.as<float4>(), targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(),
-                stream_);
+    kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots, proj_.as<float4>(),
+                targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(),
+                training ? accum_.as<float4>() : nullptr, stream_);
     mark(8);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "frame")).ok())
         return s;
@@ -308,8 +369,39 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     return {};
 }
 
+Status TileRasterizer::backward(const float4* gradColour, const kern::DiffGrads& grads, float* millis) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (!diffReady_) {
+        return Status::error(ErrorCode::InvalidArgument, "no differentiable render to take the backward of");
+    }
+    if (gradColour == nullptr) return Status::error(ErrorCode::InvalidArgument, "no colour gradient");
+    if (diffCloud_.count > 0 && (grads.means == nullptr || grads.logScales == nullptr || grads.quats == nullptr ||
+                                 grads.logits == nullptr || grads.sh == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "a missing gradient buffer");
+    }
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = pairGrad_.reserve(uint64_t{std::max(pairCapacity_, 1u)} * 48, "pair.grad")).ok()) return s;
+    if (millis != nullptr) (void)hipEventRecord(events_[0], stream_);
+    kern::blendBackward(diffFrame_, lastSplats_, ranges_.as<uint2>(), sortedSlots_.as<uint32_t>(),
+                        values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(), accum_.as<float4>(),
+                        lastCount_.as<uint32_t>(), finalT_.as<float>(), gradColour, pairGrad_.as<float4>(), stream_);
+    if (millis != nullptr) (void)hipEventRecord(events_[1], stream_);
+    kern::projectBackward(diffFrame_, diffCloud_, touched_.as<uint32_t>(), offsets_.as<uint32_t>(),
+                          pairGrad_.as<float4>(), grads, stream_);
+    if (millis != nullptr) (void)hipEventRecord(events_[2], stream_);
+    if (!(s = check(hipStreamSynchronize(stream_), "backward")).ok()) return s;
+    if (!(s = check(hipGetLastError(), "backward")).ok()) return s;
+    if (millis != nullptr) {
+        (void)hipEventElapsedTime(&millis[0], events_[0], events_[1]);
+        (void)hipEventElapsedTime(&millis[1], events_[1], events_[2]);
+    }
+    return {};
+}
+
 Status TileRasterizer::prefixSum(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count) {
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    diffReady_ = false;   // the scratch it may regrow holds the last frame
     if (in == nullptr || out == nullptr || total == nullptr) return Status::error(ErrorCode::InvalidArgument, "null buffer");
     if (in == out) return Status::error(ErrorCode::InvalidArgument, "the scan's input and output must differ");
     Status s = check(hipSetDevice(device_), "hipSetDevice");

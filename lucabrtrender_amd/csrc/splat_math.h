@@ -137,6 +137,31 @@ __device__ __forceinline__ float minPowerOverBox(float a, float b, float c, floa
     return best;
 }
 
+// Can the Gaussian (power a x^2 + 2 b x y + c y^2 about its centre) reach
+// cutoff at some point of the box [lox, hix] x [loy, hiy] (coordinates about
+// the centre)? The power is convex, so its minimum over the box is 0 when the
+// centre is inside and otherwise lies on an edge, at the clamped minimiser of
+// the edge's 1-D quadratic. The answer may be yes for a box that no pixel
+// reaches, never no for one that a pixel reaches: the slack bounds the float
+// rounding of this evaluation and of the per-pixel one, both below a few ulps
+// of (a + 2|b| + c) * r2 with r2 the largest squared coordinate in the box,
+// ten times over; the edge minimisers use the hardware reciprocal, whose error
+// moves the value only to second order. A degenerate form is always kept.
+__device__ __forceinline__ bool quadrantReach(float a, float b, float c, float cutoff, float lox, float loy,
+                                              float hix, float hiy) {
+    const bool centreInside = lox <= 0.0f && hix >= 0.0f && loy <= 0.0f && hiy >= 0.0f;
+    const bool degenerate = !(a > 1e-12f && c > 1e-12f);
+    const float ra = __builtin_amdgcn_rcpf(a), rc = __builtin_amdgcn_rcpf(c);
+    const auto form = [&](float x, float y) { return __builtin_fmaf(a * x, x, __builtin_fmaf(2.0f * b * x, y, c * y * y)); };
+    float best = form(lox, fminS(fmaxS(-b * lox * rc, loy), hiy));
+    best = fminS(best, form(hix, fminS(fmaxS(-b * hix * rc, loy), hiy)));
+    best = fminS(best, form(fminS(fmaxS(-b * loy * ra, lox), hix), loy));
+    best = fminS(best, form(fminS(fmaxS(-b * hiy * ra, lox), hix), hiy));
+    const float r2 = fmaxS(fmaxS(lox * lox, hix * hix), fmaxS(loy * loy, hiy * hiy));
+    const float slack = 4e-6f * (a + 2.0f * __builtin_fabsf(b) + c) * r2 + 1e-5f * cutoff + 1e-6f;
+    return centreInside || degenerate || best <= cutoff + slack;
+}
+
 // Does tile (tx, ty) hold a pixel centre inside the footprint (frame.slang:133-138)?
 __device__ __forceinline__ bool tileTouched(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
                                             float c, float cutoff, uint32_t tx, uint32_t ty) {

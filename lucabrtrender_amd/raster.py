@@ -171,6 +171,41 @@ class GpuSplats:
         return self._c
 
 
+class DiffSplats:
+    """A cloud of FP32 parameters for the differentiable frame, over torch device tensors.
+
+    means [N, 3] (the cloud's own space), log_scales [N, 3], quats [N, 4] (x, y, z, w;
+    normalised on use), opacity_logits [N], sh [N, 1 + rest, 3] (dc first, rgb per
+    basis; rest 0, 3, 8 or 15), filter3d [N] or None. Drawn as the packed cloud
+    would be: scale exp(log s), opacity sigmoid(logit), colour max(0.5 + C0 dc +
+    rest(dir), 0).
+    """
+
+    def __init__(self, means, log_scales, quats, opacity_logits, sh, filter3d=None):
+        import torch
+        n = int(means.shape[0])
+        rest = int(sh.shape[1]) - 1 if sh.dim() == 3 else -1
+        want = {"means": (means, (n, 3)), "log_scales": (log_scales, (n, 3)), "quats": (quats, (n, 4)),
+                "opacity_logits": (opacity_logits, (n,)), "sh": (sh, (n, rest + 1, 3))}
+        if filter3d is not None:
+            want["filter3d"] = (filter3d, (n,))
+        for name, (t, shape) in want.items():
+            if t.dtype != torch.float32 or tuple(t.shape) != shape or not t.is_contiguous() or not t.is_cuda:
+                raise LrtError(3, f"{name} must be a contiguous float32 CUDA tensor of shape {shape}")
+        if rest not in (0, 3, 8, 15):
+            raise LrtError(3, "sh must hold 1, 4, 9 or 16 coefficients per colour")
+        self.means, self.log_scales, self.quats = means, log_scales, quats
+        self.opacity_logits, self.sh, self.filter3d = opacity_logits, sh, filter3d
+        self.count, self.rest_per_colour = n, rest
+        self._c = _abi.DiffSplatsC(means.data_ptr(), log_scales.data_ptr(), quats.data_ptr(),
+                                   opacity_logits.data_ptr(), sh.data_ptr(),
+                                   filter3d.data_ptr() if filter3d is not None else None, n, rest)
+
+    @property
+    def c(self) -> _abi.DiffSplatsC:
+        return self._c
+
+
 @dataclass
 class SplatInstance:
     """render::SplatInstance (TileRasterizer.h:181-196), without relighting."""
@@ -259,6 +294,36 @@ class TileRasterizer:
         self._check(lib.lrt_raster_render(self._ctx, C.byref(proj), arr, len(instances), C.byref(settings.to_c()),
                                           C.byref(t), C.byref(st)))
         return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
+
+    def render_diff(self, camera_or_projection, splats: DiffSplats, settings: RenderSettings,
+                    targets: RenderTargets, object_to_world=IDENTITY) -> FrameStats:
+        """A frame of one FP32-parameter cloud, kept for backward()."""
+        proj = camera_or_projection
+        if isinstance(proj, Camera):
+            proj = projection_for(proj, settings.width, settings.height)
+        t = _abi.RenderTargetsC(targets.width, targets.height, targets.colour.data_ptr(),
+                                targets.depth.data_ptr() if targets.depth is not None else None)
+        st = _abi.FrameStatsC()
+        self._diff_serial = getattr(self, "_diff_serial", 0) + 1
+        self._check(lib.lrt_raster_render_diff(self._ctx, C.byref(proj), C.byref(splats.c), _mat(object_to_world),
+                                               C.byref(settings.to_c()), C.byref(t), C.byref(st)))
+        return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
+
+    def backward(self, grad_colour, grads: dict) -> tuple:
+        """Gradients of the last render_diff's parameters from dL/d(colour) [H, W, 4].
+
+        grads: float32 device tensors keyed means, log_scales, quats, opacity_logits, sh,
+        shaped as the parameters; overwritten. Returns the GPU ms of the blend's and the
+        preprocess's backward.
+        """
+        import torch
+        if grad_colour.dtype != torch.float32 or not grad_colour.is_contiguous() or not grad_colour.is_cuda:
+            raise LrtError(3, "grad_colour must be a contiguous float32 CUDA tensor")
+        g = _abi.DiffGradsC(grads["means"].data_ptr(), grads["log_scales"].data_ptr(), grads["quats"].data_ptr(),
+                            grads["opacity_logits"].data_ptr(), grads["sh"].data_ptr())
+        ms = (C.c_float * 2)()
+        self._check(lib.lrt_raster_backward(self._ctx, C.c_void_p(grad_colour.data_ptr()), C.byref(g), ms))
+        return float(ms[0]), float(ms[1])
 
     def debug(self, which: int) -> np.ndarray:
         """What the last render left on the device (LRT_DEBUG_*), copied to host."""

@@ -134,6 +134,39 @@ def filter3d_for(positions: np.ndarray, cameras, width: int, height: int, scale:
     return (best * math.sqrt(scale)).astype(np.float32)
 
 
+SH_C0 = 0.28209479177387814
+
+
+def diff_params(count: int, seed: int = 0, degree: int = 3, offset=(0.0, 0.0, -4.0), log_scale=(-4.0, 0.5),
+                opacity_logit=(0.0, 1.0), sh_sd: float = 0.3) -> dict:
+    """FP32 parameters of a cloud drawn as config 1 (the differentiable frame's
+    input): means U[-1,1]^3 + offset, log-scales N(mean, sd), quaternions of a
+    4-D normal (uniform rotations), opacity logits N(mean, sd), dc from a base
+    colour U[0,1] (base = 0.5 + C0 dc), rest band l ~ N(0, sh_sd / l)."""
+    rng = np.random.default_rng(seed)
+    rest = REST[degree]
+    means = (rng.uniform(-1.0, 1.0, (count, 3)) + np.asarray(offset)).astype(np.float32)
+    log_scales = rng.normal(log_scale[0], log_scale[1], (count, 3)).astype(np.float32)
+    quats = rng.normal(0.0, 1.0, (count, 4)).astype(np.float32)
+    logits = rng.normal(opacity_logit[0], opacity_logit[1], count).astype(np.float32)
+    sh = np.zeros((count, 1 + rest, 3), np.float32)
+    sh[:, 0, :] = (rng.uniform(0.0, 1.0, (count, 3)) - 0.5) / SH_C0
+    band = [1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3]
+    for k in range(rest):
+        sh[:, 1 + k, :] = rng.normal(0.0, sh_sd / band[k], (count, 3))
+    return dict(means=means, log_scales=log_scales, quats=quats, opacity_logits=logits, sh=sh)
+
+
+def perturbed(params: dict, seed: int = 1, means_sd=0.01, logit_sd=0.2, sh_sd=0.05) -> dict:
+    """The target scene of the fwd+bwd benchmark: means + N(0, 0.01), logits + N(0, 0.2), SH + N(0, 0.05)."""
+    rng = np.random.default_rng(seed)
+    out = {k: v.copy() for k, v in params.items()}
+    out["means"] += rng.normal(0.0, means_sd, out["means"].shape).astype(np.float32)
+    out["opacity_logits"] += rng.normal(0.0, logit_sd, out["opacity_logits"].shape).astype(np.float32)
+    out["sh"] += rng.normal(0.0, sh_sd, out["sh"].shape).astype(np.float32)
+    return out
+
+
 @dataclass
 class Config:
     name: str

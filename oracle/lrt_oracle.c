@@ -1064,40 +1064,30 @@ static double srgb_to_linear_d(double c) {
 
 static double half_d(uint32_t w, int high) { return (double)(high ? high_half(w) : low_half(w)); }
 
-/* reference_project.slang:31-105 in FP64 (exp, log, sqrt from libm in double). */
-static void ref_project_f64(const or_frame* f, const or_cloud* cl, uint32_t i, ref_f64* out) {
+/* reference_project.slang:31-105 in FP64 (exp, log, sqrt from libm in double),
+ * from decoded inputs: centre p, opacity, scales s (after any 3D filter), the
+ * quaternion q (normalised here), the base colour and the rest coefficients
+ * (rgb per basis). Returns 0 for a culled splat; the colour before any edit
+ * grade is left in out->colour. */
+static int ref_project_core_f64(const or_frame* f, const double p[3], double opacity, const double s[3],
+                                const double qin[4], const double base[3], const double* coeff, uint32_t degree,
+                                ref_f64* out) {
     out->screen[2] = -1.0;
-    const float* pf = cl->positions + (size_t)i * 4;
-    const double p[4] = {pf[0], pf[1], pf[2], pf[3]};
-    const uint32_t* row = cl->shape + (size_t)i * 4;
-    double s[3] = {exp(half_d(row[1], 0)), exp(half_d(row[1], 1)), exp(half_d(row[2], 0))};
-    const edit_fate fate = edit_fate_of(&f->edit, pf[0], pf[1], pf[2], pf[3], (float)s[0], (float)s[1], (float)s[2]);
-    if (fate.removed) return;
-    double opacity = fate.graded ? p[3] * (double)f->edit.opacity : p[3];
-    if (f->use_filter3d) {
-        const double fl = cl->filter3d[i], f2 = fl * fl;
-        double det1 = 1.0, det2 = 1.0;
-        for (int k = 0; k < 3; ++k) {
-            det1 *= s[k] * s[k];
-            det2 *= s[k] * s[k] + f2;
-            s[k] = sqrt(s[k] * s[k] + f2);
-        }
-        opacity *= sqrt(det1 / det2);
-    }
     const float* mf = f->m;
     double m[12];
     for (int k = 0; k < 12; ++k) m[k] = mf[k];
     const double v[3] = {m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3],
                          m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7],
                          m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11]};
-    if (!(v[2] > (double)f->near_z) || v[2] > (double)f->far_z) return;
-    float qf[4];
-    or_decode_quaternion(row[0], qf);
-    /* The stored quaternion's decode is part of the input; the FP64 path
-     * normalises it in double. */
-    double q[4] = {qf[0], qf[1], qf[2], qf[3]};
+    if (!(v[2] > (double)f->near_z) || v[2] > (double)f->far_z) return 0;
+    double q[4] = {qin[0], qin[1], qin[2], qin[3]};
     const double ql = sqrt(q[0] * q[0] + q[1] * q[1] + q[2] * q[2] + q[3] * q[3]);
-    for (int k = 0; k < 4; ++k) q[k] /= ql;
+    if (ql > 0.0) {
+        for (int k = 0; k < 4; ++k) q[k] /= ql;
+    } else {
+        q[0] = q[1] = q[2] = 0.0;
+        q[3] = 1.0;
+    }
     const double x = q[0], y = q[1], z = q[2], w = q[3];
     const double r[3][3] = {
         {1.0 - 2.0 * (y * y + z * z), 2.0 * (x * y - w * z), 2.0 * (x * z + w * y)},
@@ -1144,22 +1134,18 @@ static void ref_project_f64(const or_frame* f, const or_cloud* cl, uint32_t i, r
     const double before = c2[0][0] * c2[1][1] - c2[0][1] * c2[0][1];
     const double a = c2[0][0] + dil, b = c2[0][1], c = c2[1][1] + dil;
     const double det = a * c - b * b;
-    if (!(det > 1e-12)) return;
+    if (!(det > 1e-12)) return 0;
     double alpha = opacity;
     if (f->antialias) alpha *= sqrt((before > 0.0 ? before : 0.0) / det);
-    if (!(alpha >= 1.0 / 255.0)) return;
+    if (!(alpha >= 1.0 / 255.0)) return 0;
     const double cut = 2.0 * log(255.0 * alpha > 1.0001 ? 255.0 * alpha : 1.0001);
-    const uint32_t degree = degree_of(f->rest_per_colour) < f->sh_limit ? degree_of(f->rest_per_colour) : f->sh_limit;
     const double dxo = p[0] - f->eye[0], dyo = p[1] - f->eye[1], dzo = p[2] - f->eye[2];
     const double len = sqrt(dxo * dxo + dyo * dyo + dzo * dzo);
     const double dx = dxo / len, dy = dyo / len, dz = dzo / len;
-    const double base[3] = {half_d(row[2], 1), half_d(row[3], 0), half_d(row[3], 1)};
-    const uint32_t* shrow = cl->sh + (size_t)i * f->sh_words;
-    double colour[3];
     for (int ch = 0; ch < 3; ++ch) {
         double rest = 0.0;
         if (degree >= 1) {
-#define BD(k) (double)half_at(shrow, (k) * 3u + (uint32_t)ch)
+#define BD(k) coeff[(k) * 3 + ch]
             rest = -0.4886025119029199 * dy * BD(0) + 0.4886025119029199 * dz * BD(1) - 0.4886025119029199 * dx * BD(2);
             if (degree >= 2) {
                 const double xx = dx * dx, yy = dy * dy, zz = dz * dz;
@@ -1178,9 +1164,49 @@ static void ref_project_f64(const or_frame* f, const or_cloud* cl, uint32_t i, r
 #undef BD
         }
         const double v0 = base[ch] + rest;
-        colour[ch] = v0 > 0.0 ? v0 : 0.0;
+        out->colour[ch] = v0 > 0.0 ? v0 : 0.0;
     }
+    out->screen[0] = cx; out->screen[1] = cy; out->screen[2] = v[2]; out->screen[3] = cut < 9.0 ? cut : 9.0;
+    out->conic[0] = c / det; out->conic[1] = -b / det; out->conic[2] = a / det; out->conic[3] = alpha;
+    return 1;
+}
+
+static void filter3d_f64(double fl, double s[3], double* opacity) {
+    const double f2 = fl * fl;
+    double det1 = 1.0, det2 = 1.0;
+    for (int k = 0; k < 3; ++k) {
+        det1 *= s[k] * s[k];
+        det2 *= s[k] * s[k] + f2;
+        s[k] = sqrt(s[k] * s[k] + f2);
+    }
+    *opacity *= sqrt(det1 / det2);
+}
+
+/* The packed cloud's splat i in FP64: its decoded fp16/10-bit inputs, then the core. */
+static void ref_project_f64(const or_frame* f, const or_cloud* cl, uint32_t i, ref_f64* out) {
+    out->screen[2] = -1.0;
+    const float* pf = cl->positions + (size_t)i * 4;
+    const double p[3] = {pf[0], pf[1], pf[2]};
+    const uint32_t* row = cl->shape + (size_t)i * 4;
+    double s[3] = {exp(half_d(row[1], 0)), exp(half_d(row[1], 1)), exp(half_d(row[2], 0))};
+    const edit_fate fate = edit_fate_of(&f->edit, pf[0], pf[1], pf[2], pf[3], (float)s[0], (float)s[1], (float)s[2]);
+    if (fate.removed) return;
+    double opacity = fate.graded ? (double)pf[3] * (double)f->edit.opacity : (double)pf[3];
+    if (f->use_filter3d) filter3d_f64(cl->filter3d[i], s, &opacity);
+    float qf[4];
+    or_decode_quaternion(row[0], qf);
+    /* The stored quaternion's decode is part of the input; the core normalises
+     * it in double. */
+    const double q[4] = {qf[0], qf[1], qf[2], qf[3]};
+    const uint32_t degree = degree_of(f->rest_per_colour) < f->sh_limit ? degree_of(f->rest_per_colour) : f->sh_limit;
+    const double base[3] = {half_d(row[2], 1), half_d(row[3], 0), half_d(row[3], 1)};
+    const uint32_t* shrow = cl->sh + (size_t)i * f->sh_words;
+    double coeff[45] = {0};
+    const uint32_t used = degree == 0 ? 0u : degree == 1 ? 9u : degree == 2 ? 24u : 45u;
+    for (uint32_t k = 0; k < used; ++k) coeff[k] = (double)half_at(shrow, k);
+    if (!ref_project_core_f64(f, p, opacity, s, q, base, coeff, degree, out)) return;
     if (fate.graded) {
+        double* colour = out->colour;
         const double tr = colour[0] * f->edit.tint[0] * f->edit.brightness;
         const double tg = colour[1] * f->edit.tint[1] * f->edit.brightness;
         const double tb = colour[2] * f->edit.tint[2] * f->edit.brightness;
@@ -1191,9 +1217,27 @@ static void ref_project_f64(const or_frame* f, const or_cloud* cl, uint32_t i, r
             colour[k] = g > 0.0 ? g : 0.0;
         }
     }
-    out->screen[0] = cx; out->screen[1] = cy; out->screen[2] = v[2]; out->screen[3] = cut < 9.0 ? cut : 9.0;
-    out->conic[0] = c / det; out->conic[1] = -b / det; out->conic[2] = a / det; out->conic[3] = alpha;
-    memcpy(out->colour, colour, sizeof colour);
+}
+
+/* The FP32-parameter cloud's splat i in FP64: scales exp(log s), opacity
+ * sigmoid(logit), base colour 0.5 + C0 dc, then the core. */
+static void ref_project_diff_f64(const or_frame* f, const or_diff_cloud* c, uint32_t i, ref_f64* out) {
+    out->screen[2] = -1.0;
+    const double p[3] = {c->means[i * 3 + 0], c->means[i * 3 + 1], c->means[i * 3 + 2]};
+    double s[3] = {exp((double)c->log_scales[i * 3 + 0]), exp((double)c->log_scales[i * 3 + 1]),
+                   exp((double)c->log_scales[i * 3 + 2])};
+    double opacity = 1.0 / (1.0 + exp(-(double)c->logits[i]));
+    if (f->use_filter3d) filter3d_f64(c->filter3d[i], s, &opacity);
+    const double q[4] = {c->quats[i * 4 + 0], c->quats[i * 4 + 1], c->quats[i * 4 + 2], c->quats[i * 4 + 3]};
+    const uint32_t degree = degree_of(c->rest_per_colour) < f->sh_limit ? degree_of(c->rest_per_colour) : f->sh_limit;
+    const size_t stride = (size_t)(1 + c->rest_per_colour) * 3;
+    const float* row = c->sh + (size_t)i * stride;
+    const double base[3] = {0.5 + 0.28209479177387814 * row[0], 0.5 + 0.28209479177387814 * row[1],
+                            0.5 + 0.28209479177387814 * row[2]};
+    double coeff[45] = {0};
+    const uint32_t used = degree == 0 ? 0u : degree == 1 ? 9u : degree == 2 ? 24u : 45u;
+    for (uint32_t k = 0; k < used; ++k) coeff[k] = row[3 + k];
+    (void)ref_project_core_f64(f, p, opacity, s, q, base, coeff, degree, out);
 }
 
 typedef struct { double z; uint32_t index; } depth_entry;
@@ -1282,20 +1326,11 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
     free(rec);
 }
 
-void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
-                             const uint32_t* px, const uint32_t* py, uint32_t npix, double* colour,
-                             double* depth, int keyed, int threads) {
+/* The FP64 walk over every projected splat in (z, index) order (or the keyed
+ * order), as or_reference_render_f32's. */
+static void blend_f64(const or_frame* frames, ref_f64* rec, uint32_t total, const uint32_t* px, const uint32_t* py,
+                      uint32_t npix, double* colour, double* depth, int keyed, int threads) {
     (void)threads;
-    uint32_t total = 0;
-    for (uint32_t k = 0; k < instances; ++k) total += frames[k].count;
-    ref_f64* rec = (ref_f64*)calloc(total > 0 ? total : 1, sizeof(ref_f64));
-    for (uint32_t k = 0; k < instances; ++k) {
-#ifdef _OPENMP
-#pragma omp parallel for schedule(static) OR_THREADS(threads)
-#endif
-        for (int64_t i = 0; i < (int64_t)frames[k].count; ++i)
-            ref_project_f64(&frames[k], &clouds[k], (uint32_t)i, rec + frames[k].base + i);
-    }
     depth_entry* order = (depth_entry*)malloc((total > 0 ? total : 1) * sizeof(depth_entry));
     uint32_t live = 0;
     for (uint32_t s = 0; s < total; ++s)
@@ -1348,6 +1383,37 @@ void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uin
     }
     free(sorted);
     free(order);
+}
+
+
+void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                             const uint32_t* px, const uint32_t* py, uint32_t npix, double* colour,
+                             double* depth, int keyed, int threads) {
+    (void)threads;
+    uint32_t total = 0;
+    for (uint32_t k = 0; k < instances; ++k) total += frames[k].count;
+    ref_f64* rec = (ref_f64*)calloc(total > 0 ? total : 1, sizeof(ref_f64));
+    for (uint32_t k = 0; k < instances; ++k) {
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) OR_THREADS(threads)
+#endif
+        for (int64_t i = 0; i < (int64_t)frames[k].count; ++i)
+            ref_project_f64(&frames[k], &clouds[k], (uint32_t)i, rec + frames[k].base + i);
+    }
+    blend_f64(frames, rec, total, px, py, npix, colour, depth, keyed, threads);
+    free(rec);
+}
+
+void or_diff_render_f64(const or_frame* frame, const or_diff_cloud* cloud, const uint32_t* px, const uint32_t* py,
+                        uint32_t npix, double* colour, double* depth, int keyed, int threads) {
+    (void)threads;
+    const uint32_t total = cloud->count;
+    ref_f64* rec = (ref_f64*)calloc(total > 0 ? total : 1, sizeof(ref_f64));
+#ifdef _OPENMP
+#pragma omp parallel for schedule(static) OR_THREADS(threads)
+#endif
+    for (int64_t i = 0; i < (int64_t)total; ++i) ref_project_diff_f64(frame, cloud, (uint32_t)i, rec + i);
+    blend_f64(frame, rec, total, px, py, npix, colour, depth, keyed, threads);
     free(rec);
 }
 

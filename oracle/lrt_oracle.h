@@ -189,6 +189,20 @@ void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uin
                              const uint32_t* px, const uint32_t* py, uint32_t npix, double* colour,
                              double* depth, int keyed, int threads);
 
+/* A cloud of FP32 parameters (the differentiable frame's input): means N x 3,
+ * log scales N x 3, quaternions N x 4 (x, y, z, w; normalised on use), opacity
+ * logits N, harmonics N x (1 + rest_per_colour) x 3 (dc first), filter3d N or
+ * NULL. */
+typedef struct {
+    const float *means, *log_scales, *quats, *logits, *sh, *filter3d;
+    uint32_t count, rest_per_colour;
+} or_diff_cloud;
+/* The brute force of such a cloud in FP64 (frame.base must be 0): scales
+ * exp(log s), opacity sigmoid(logit), base colour 0.5 + C0 dc, then
+ * reference_project.slang:31-105 and the walk of reference_blend.slang:129-223. */
+void or_diff_render_f64(const or_frame* frame, const or_diff_cloud* cloud, const uint32_t* px, const uint32_t* py,
+                        uint32_t npix, double* colour, double* depth, int keyed, int threads);
+
 /* compareImages (modules/render/src/ReferenceRenderer.cpp:248-297 with
  * shaders/lrt/reference/image_compare.slang:27-51): 8-bit sRGB code-value
  * histogram of the worst channel; p99, max, pixels over 2. */

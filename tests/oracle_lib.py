@@ -59,6 +59,11 @@ class OrCloud(C.Structure):
     _fields_ = [("positions", C.c_void_p), ("shape", C.c_void_p), ("sh", C.c_void_p), ("filter3d", C.c_void_p)]
 
 
+class OrDiffCloud(C.Structure):
+    _fields_ = [("means", C.c_void_p), ("log_scales", C.c_void_p), ("quats", C.c_void_p), ("logits", C.c_void_p),
+                ("sh", C.c_void_p), ("filter3d", C.c_void_p), ("count", C.c_uint32), ("rest_per_colour", C.c_uint32)]
+
+
 class OrStats(C.Structure):
     _fields_ = [("splats", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32)]
 
@@ -98,6 +103,8 @@ _SIGS = {
     "or_compare_images": (None, [P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint64)]),
     "or_srgb_to_linear": (C.c_float, [C.c_float]),
+    "or_diff_render_f64": (None, [C.POINTER(OrFrame), C.POINTER(OrDiffCloud), C.c_void_p, C.c_void_p, C.c_uint32,
+                                  C.c_void_p, C.c_void_p, C.c_int, C.c_int]),
 }
 
 
@@ -246,3 +253,25 @@ def linear_error(a: np.ndarray, b: np.ndarray):
     """p99 and max of the absolute per-channel difference in linear units."""
     d = np.abs(np.asarray(a, dtype=np.float64) - np.asarray(b, dtype=np.float64)).reshape(-1)
     return float(np.quantile(d, 0.99)), float(d.max())
+
+
+def diff_frame(projection: OrProjection, s: OrSettings, count: int, rest_per_colour: int,
+               object_to_world=np.eye(4)) -> OrFrame:
+    """The frame one FP32-parameter cloud is drawn with."""
+    f = OrFrame()
+    words = 1 if rest_per_colour == 0 else (rest_per_colour * 3 + 1) // 2
+    lib.or_frame_for(C.byref(projection), C.byref(s), dbl(object_to_world), None, count, 0, rest_per_colour, words,
+                     C.byref(f))
+    return f
+
+
+def diff_render_f64(frame: OrFrame, params: dict, filter3d=None, threads=0) -> np.ndarray:
+    """or_diff_render_f64 of float32 parameter arrays: the FP64 image [H, W, 4]."""
+    arrs = {k: np.ascontiguousarray(params[k], dtype=np.float32) for k in
+            ("means", "log_scales", "quats", "opacity_logits", "sh")}
+    f3 = np.ascontiguousarray(filter3d, dtype=np.float32) if filter3d is not None else None
+    cloud = OrDiffCloud(ptr(arrs["means"]), ptr(arrs["log_scales"]), ptr(arrs["quats"]), ptr(arrs["opacity_logits"]),
+                        ptr(arrs["sh"]), ptr(f3), arrs["means"].shape[0], arrs["sh"].shape[1] - 1)
+    colour = np.zeros((frame.height, frame.width, 4), np.float64)
+    lib.or_diff_render_f64(C.byref(frame), C.byref(cloud), None, None, 0, ptr(colour), None, 1, threads)
+    return colour
