@@ -130,6 +130,140 @@ def cpu_baseline(cfg, settings, seconds_budget=15.0):
                       f"sort, ranges, blend) on {threads} OpenMP threads, {model}"}
 
 
+def fwd_bwd_leg(torch, raster, scenes, steps):
+    """Config 3: 1,000,000 FP32-parameter splats drawn as config 1 with SH degree 3,
+    1280x720; loss = per-pixel L2 against the frame of the perturbed scene. A view
+    is one differentiable frame, the loss gradient and the backward."""
+    params = scenes.diff_params(1_000_000, 0, degree=3)
+    target_p = scenes.perturbed(params, seed=1)
+    cam = raster.Camera.looking_at((0.0, 0.0, 0.0), (0.0, 0.0, -1.0))
+    cam.lens = raster.Lens(focal=scenes.focal_for_fov(60.0))
+    settings = raster.RenderSettings(width=1280, height=720)
+    rast = raster.TileRasterizer()
+
+    def device(p):
+        d = {k: torch.from_numpy(v).cuda() for k, v in p.items()}
+        return d, raster.DiffSplats(d["means"], d["log_scales"], d["quats"], d["opacity_logits"], d["sh"])
+
+    _, tsplats = device(target_p)
+    targets = raster.RenderTargets(1280, 720)
+    rast.render_diff(cam, tsplats, settings, targets)
+    target = targets.colour.clone()
+    dev, splats = device(params)
+    grads = {k: torch.empty_like(v) for k, v in dev.items()}
+
+    def view():
+        stats = rast.render_diff(cam, splats, settings, targets)
+        grad = 2.0 * (targets.colour - target)
+        return stats, rast.backward(grad, grads)
+
+    for _ in range(3):
+        view()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    bwd = [0.0, 0.0]
+    for _ in range(steps):
+        stats, ms = view()
+        bwd[0] += ms[0]
+        bwd[1] += ms[1]
+    torch.cuda.synchronize()
+    per_view = (time.perf_counter() - start) / steps
+    start = time.perf_counter()
+    for _ in range(steps):
+        rast.render_diff(cam, splats, settings, targets)
+    torch.cuda.synchronize()
+    fwd = (time.perf_counter() - start) / steps
+    return {"workload": "fwd-bwd-1M-720p", "splats": 1_000_000, "visible": stats.visible, "pairs": stats.pairs,
+            "ms_per_view": round(per_view * 1e3, 4), "forward_ms": round(fwd * 1e3, 4),
+            "blend_backward_ms": round(bwd[0] / steps, 4), "project_backward_ms": round(bwd[1] / steps, 4),
+            "fwd_bwd_over_fwd": round(per_view / fwd, 3),
+            "data": "synthetic: seeded FP32 parameters, target from means+N(0,0.01), logits+N(0,0.2), SH+N(0,0.05)"}
+
+
+def views_leg(torch, raster, scenes, reps):
+    """Config 4: one 2,000,000-splat clustered scene, 64 cameras on a sphere of radius
+    12 looking at its centre, each view's size drawn from {256x256, 512x384, 960x540,
+    1280x720, 1920x1080}. The batch (one launch sequence, autotuned) against the plain
+    sequence of 64 single-view frames."""
+    import math
+
+    import numpy as np
+    cfg = scenes.config(2, count=2_000_000)
+    dev = cfg.scene.to_device()
+    inst = [raster.SplatInstance(dev)]
+    rng = np.random.default_rng(4)
+    sizes = [(256, 256), (512, 384), (960, 540), (1280, 720), (1920, 1080)]
+    views = []
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    for k in range(64):
+        y = 1.0 - 2.0 * (k + 0.5) / 64.0
+        r = math.sqrt(1.0 - y * y)
+        eye = (12.0 * r * math.cos(golden * k), 12.0 * y, 12.0 * r * math.sin(golden * k))
+        up = (0.0, 0.0, 1.0) if abs(y) > 0.95 else (0.0, 1.0, 0.0)
+        cam = raster.Camera.looking_at(eye, (0.0, 0.0, 0.0), up)
+        cam.lens = raster.Lens(focal=scenes.focal_for_fov(60.0))
+        w, h = sizes[int(rng.integers(0, len(sizes)))]
+        views.append(raster.View(cam, raster.RenderTargets(w, h)))
+    settings = raster.RenderSettings(dilation=cfg.dilation, filter3d=cfg.filter3d)
+    rast = raster.TileRasterizer()
+    tuned, _ = rast.autotune_views(views, inst, settings)
+    rast.render_views(views, inst, tuned)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(reps):
+        stats = rast.render_views(views, inst, tuned)
+    torch.cuda.synchronize()
+    batch_s = (time.perf_counter() - start) / reps
+    single = [raster.RenderSettings(**{**settings.__dict__, "width": v.targets.width, "height": v.targets.height})
+              for v in views]
+    for v, s in zip(views, single):
+        rast.render(v.camera, inst, s, v.targets)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(reps):
+        for v, s in zip(views, single):
+            rast.render(v.camera, inst, s, v.targets)
+    torch.cuda.synchronize()
+    seq_s = (time.perf_counter() - start) / reps
+    pixels = sum(v.targets.width * v.targets.height for v in views)
+    return {"workload": "views-fwd-2M-64", "splats": 2_000_000, "views": 64, "pixels": pixels, "pairs": stats.pairs,
+            "batch_ms": round(batch_s * 1e3, 3), "views_per_s_batched": round(64 / batch_s, 1),
+            "sequence_ms": round(seq_s * 1e3, 3), "views_per_s_single": round(64 / seq_s, 1),
+            "speedup": round(seq_s / batch_s, 3),
+            "tuned": {"blend_batch": tuned.blend_batch, "persistent_blend": tuned.persistent_blend,
+                      "use_graph": tuned.use_graph}}
+
+
+def sort_stress_leg(torch, raster, scenes, _abi, reps):
+    """Config 5: 6,000,000 splats at 3840x2160, 80% as config 1 and 20% large splats
+    near the camera; the frame's stages timed, the sort's keys/s and HBM fraction."""
+    import numpy as np
+    cfg = scenes.config(5)
+    dev = cfg.scene.to_device()
+    settings = raster.RenderSettings(width=cfg.width, height=cfg.height, time_stages=True)
+    targets = raster.RenderTargets(cfg.width, cfg.height)
+    rast = raster.TileRasterizer()
+    inst = [raster.SplatInstance(dev)]
+    rast.render(cfg.camera, inst, settings, targets)
+    acc = np.zeros(8)
+    start = time.perf_counter()
+    for _ in range(reps):
+        rast.render(cfg.camera, inst, settings, targets)
+        acc += rast.debug(_abi.DEBUG_KERNEL_MS)
+    wall = (time.perf_counter() - start) / reps
+    ms = acc / reps
+    n, visible, pairs, tiles = (int(v) for v in rast.debug(_abi.DEBUG_COUNTS))
+    passes = (24 + max(1, int(np.ceil(np.log2(max(tiles, 2))))) + 7) // 8
+    sort_ms = float(ms[3] + ms[4])
+    sort_bytes = pairs * (8 + 24 * passes)
+    return {"workload": "sort-stress-6M-2160p", "splats": n, "visible": visible, "pairs": pairs,
+            "sort_passes": passes, "sort_ms": round(sort_ms, 4),
+            "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3),
+            "sort_hbm_frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "duplicate_ms": round(float(ms[2]), 4), "blend_ms": round(float(ms[6]), 4),
+            "gpu_ms_per_frame": round(float(ms[7]), 4), "wall_ms_per_frame": round(wall * 1e3, 4)}
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -138,6 +272,9 @@ def main():
     ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
     ap.add_argument("--blend-batch", type=int, default=128, help="records in each quadrant's LDS list: 128 or 256")
+    ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="config-3 fwd+bwd views timed; 0 skips it")
+    ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
+    ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -205,6 +342,19 @@ def main():
                 acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
             variants[f"batch{batch}_{'index_order' if order == 0 else 'heavy_first'}"] = round(float(acc_v[6] / 10), 4)
 
+    fwd_bwd = None
+    if args.fwd_bwd_steps > 0:
+        log("fwd+bwd (1M splats, 720p) ...")
+        fwd_bwd = fwd_bwd_leg(torch, raster, scenes, args.fwd_bwd_steps)
+    batched = None
+    if args.views_reps > 0:
+        log("batched views (2M splats, 64 views) ...")
+        batched = views_leg(torch, raster, scenes, args.views_reps)
+    stress = None
+    if args.stress_reps > 0:
+        log("sort stress (6M splats, 2160p) ...")
+        stress = sort_stress_leg(torch, raster, scenes, _abi, args.stress_reps)
+
     cpu = None
     if args.cpu_seconds > 0:
         log("CPU baseline ...")
@@ -231,6 +381,9 @@ def main():
         "blend_ms_by_setting": variants,
         "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3) if sort_ms > 0 else None,
         "copy_GB/s": round(copy_gbs, 1),
+        "fwd_bwd": fwd_bwd,
+        "views": batched,
+        "sort_stress": stress,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

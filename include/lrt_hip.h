@@ -135,6 +135,8 @@ typedef struct {
     int32_t filter3d;             /* 1: read lrt_splats.filter3d */
     uint32_t blend_batch;         /* records each 8x8 quadrant's LDS list holds: 128 or 256 (0: 128) */
     int32_t tile_order;           /* 0: tile index order; 1: heaviest tiles launched first */
+    int32_t persistent_blend;     /* batches: 1 = a persistent blend taking quadrants from a queue */
+    int32_t use_graph;            /* batches: 1 = capture the launch sequence once and replay it */
 } lrt_render_settings;
 
 /* render::RenderTargets (TileRasterizer.h:257-263): device images, bottom row
@@ -178,6 +180,48 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
                              const lrt_splat_instance* instances, uint32_t instance_count,
                              const lrt_render_settings* settings, lrt_render_targets* targets,
                              lrt_frame_stats* stats);
+
+/* ---- the brute force in FP64 -------------------------------------------------------
+ * ReferenceRenderer (modules/render/src/ReferenceRenderer.cpp:54-160;
+ * reference_project.slang:31-105, reference_blend.slang:129-223) on the GPU in
+ * double, without its per-pixel hit cap: every splat of every instance in
+ * (view z, index) order, or with keyed in the rasteriser's (24-bit depth key,
+ * slot) order, walked per pixel. colour: 4 doubles a pixel, depth: 1 (may be
+ * NULL), device memory of the settings' size, bottom row first. Synchronous.
+ * It clobbers nothing the rasteriser's debug buffers hold. */
+lrt_status lrt_reference_render(lrt_context* ctx, const lrt_projection* projection,
+                                const lrt_splat_instance* instances, uint32_t instance_count,
+                                const lrt_render_settings* settings, double* colour, double* depth,
+                                int32_t keyed);
+
+/* ---- batches of views -------------------------------------------------------------
+ * No single counterpart in the reference, which draws one camera a render()
+ * call (TileRasterizer.h:143-155): the boundary's lrt_raster_render_views. */
+
+/* One camera of a batch: its projection and its images (targets.width/height
+ * are the view's frame size). */
+typedef struct {
+    lrt_projection projection;
+    lrt_render_targets targets;
+} lrt_view;
+
+/* Every view of the same splat instances in one launch sequence with one wait:
+ * each view's images are those lrt_raster_render would draw for it alone, bit
+ * for bit. settings.width/height are ignored. stats: the batch's totals
+ * (splats counts every view's slots). With settings.use_graph the sequence is
+ * captured as a hipGraph once per batch shape and replayed. */
+lrt_status lrt_raster_render_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
+                                   const lrt_splat_instance* instances, uint32_t instance_count,
+                                   const lrt_render_settings* settings, lrt_frame_stats* stats);
+
+/* The autotuner: times the batch under each launch choice this build offers
+ * (list length 128/256, grid or persistent blend, graph replay or not) and
+ * writes the fastest into *settings (its other fields are kept). ms (host, may
+ * be NULL) receives the winner's batch time. Tile size (16) and radix width (8
+ * bits) are fixed in this build. */
+lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
+                              const lrt_splat_instance* instances, uint32_t instance_count,
+                              lrt_render_settings* settings, float* ms);
 
 /* ---- the differentiable frame ---------------------------------------------------
  * No counterpart in the reference, which has no backward: the entries that the

@@ -84,11 +84,16 @@ class RenderSettingsC(C.Structure):
         ("depth_mode", C.c_int32), ("depth_threshold", C.c_float),
         ("linearise", C.c_int32), ("time_stages", C.c_int32),
         ("dilation", C.c_float), ("filter3d", C.c_int32), ("blend_batch", C.c_uint32), ("tile_order", C.c_int32),
+        ("persistent_blend", C.c_int32), ("use_graph", C.c_int32),
     ]
 
 
 class RenderTargetsC(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("colour", C.c_void_p), ("depth", C.c_void_p)]
+
+
+class ViewC(C.Structure):
+    _fields_ = [("projection", ProjectionC), ("targets", RenderTargetsC)]
 
 
 class FrameStatsC(C.Structure):
@@ -141,6 +146,12 @@ SIGNATURES = {
     "lrt_last_error": (C.c_char_p, [C.c_void_p]),
     "lrt_raster_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
                                     C.POINTER(RenderSettingsC), C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
+    "lrt_reference_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
+                                       C.POINTER(RenderSettingsC), C.c_void_p, C.c_void_p, C.c_int32]),
+    "lrt_raster_render_views": (C.c_int, [C.c_void_p, C.POINTER(ViewC), C.c_uint32, C.POINTER(SplatInstanceC),
+                                          C.c_uint32, C.POINTER(RenderSettingsC), C.POINTER(FrameStatsC)]),
+    "lrt_autotune_views": (C.c_int, [C.c_void_p, C.POINTER(ViewC), C.c_uint32, C.POINTER(SplatInstanceC), C.c_uint32,
+                                     C.POINTER(RenderSettingsC), C.POINTER(C.c_float)]),
     "lrt_raster_render_diff": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(DiffSplatsC),
                                          C.POINTER(C.c_double), C.POINTER(RenderSettingsC),
                                          C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),

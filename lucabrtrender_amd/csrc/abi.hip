@@ -59,6 +59,8 @@ lrt::RenderSettings toSettings(const lrt_render_settings& s) {
     out.filter3d = s.filter3d != 0;
     out.blendBatch = s.blend_batch == 0 ? 128u : s.blend_batch;
     out.heaviestTilesFirst = s.tile_order == 1;
+    out.persistentBlend = s.persistent_blend != 0;
+    out.useGraph = s.use_graph != 0;
     return out;
 }
 
@@ -91,6 +93,51 @@ lrt::SplatCloud toCloud(const lrt_splats& c) {
     out.restPerColour = c.rest_per_colour;
     out.shWords = c.sh_words;
     return out;
+}
+
+}   // namespace
+
+namespace {
+
+lrt::Status toBatch(const lrt_view* views, uint32_t view_count, const lrt_splat_instance* instances,
+                    uint32_t instance_count, std::vector<lrt::View>& outViews, std::vector<lrt::SplatCloud>& clouds,
+                    std::vector<lrt::SplatInstance>& list) {
+    if ((view_count > 0 && views == nullptr) || (instance_count > 0 && instances == nullptr)) {
+        return lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument");
+    }
+    outViews.resize(view_count);
+    for (uint32_t v = 0; v < view_count; ++v) {
+        outViews[v].projection = toProjection(views[v].projection);
+        outViews[v].targets.width = views[v].targets.width;
+        outViews[v].targets.height = views[v].targets.height;
+        outViews[v].targets.colour = reinterpret_cast<float4*>(views[v].targets.colour);
+        outViews[v].targets.depth = views[v].targets.depth;
+    }
+    clouds.resize(instance_count);
+    list.resize(instance_count);
+    for (uint32_t k = 0; k < instance_count; ++k) {
+        if (instances[k].splats != nullptr) {
+            clouds[k] = toCloud(*instances[k].splats);
+            list[k].splats = &clouds[k];
+        }
+        list[k].objectToWorld = toMat(instances[k].object_to_world);
+        list[k].edit = toEdit(instances[k].edit);
+    }
+    return {};
+}
+
+void toStats(const lrt::FrameStats& fs, lrt_frame_stats* stats) {
+    if (stats == nullptr) return;
+    stats->splats = fs.splats;
+    stats->visible = fs.visible;
+    stats->pairs = fs.pairs;
+    stats->project_ms = fs.projectMs;
+    stats->depth_sort_ms = fs.depthSortMs;
+    stats->counts_ms = fs.countsMs;
+    stats->emit_ms = fs.emitMs;
+    stats->tile_sort_ms = fs.tileSortMs;
+    stats->blend_ms = fs.blendMs;
+    stats->total_ms = fs.totalMs;
 }
 
 }   // namespace
@@ -168,6 +215,8 @@ void lrt_render_settings_default(lrt_render_settings* out) {
     out->filter3d = 0;
     out->blend_batch = 128;
     out->tile_order = 0;
+    out->persistent_blend = 0;
+    out->use_graph = 0;
 }
 
 void lrt_splat_edit_default(lrt_splat_edit* out) {
@@ -254,6 +303,61 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
         stats->blend_ms = fs.blendMs;
         stats->total_ms = fs.totalMs;
     }
+    return finish(ctx, s);
+}
+
+
+lrt_status lrt_reference_render(lrt_context* ctx, const lrt_projection* projection,
+                                const lrt_splat_instance* instances, uint32_t instance_count,
+                                const lrt_render_settings* settings, double* colour, double* depth,
+                                int32_t keyed) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (projection == nullptr || settings == nullptr || (instance_count > 0 && instances == nullptr)) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    std::vector<lrt::View> none;
+    std::vector<lrt::SplatCloud> clouds;
+    std::vector<lrt::SplatInstance> inst;
+    lrt::Status s = toBatch(nullptr, 0, instances, instance_count, none, clouds, inst);
+    if (!s.ok()) return finish(ctx, s);
+    return finish(ctx, ctx->raster.referenceRender(toProjection(*projection), inst, toSettings(*settings), colour,
+                                                   depth, keyed != 0));
+}
+
+lrt_status lrt_raster_render_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
+                                   const lrt_splat_instance* instances, uint32_t instance_count,
+                                   const lrt_render_settings* settings, lrt_frame_stats* stats) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (settings == nullptr) return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    std::vector<lrt::View> list;
+    std::vector<lrt::SplatCloud> clouds;
+    std::vector<lrt::SplatInstance> inst;
+    lrt::Status s = toBatch(views, view_count, instances, instance_count, list, clouds, inst);
+    if (!s.ok()) return finish(ctx, s);
+    lrt::FrameStats fs;
+    s = ctx->raster.renderViews(list, inst, toSettings(*settings), &fs);
+    if (s.ok()) toStats(fs, stats);
+    return finish(ctx, s);
+}
+
+lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
+                              const lrt_splat_instance* instances, uint32_t instance_count,
+                              lrt_render_settings* settings, float* ms) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (settings == nullptr) return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    std::vector<lrt::View> list;
+    std::vector<lrt::SplatCloud> clouds;
+    std::vector<lrt::SplatInstance> inst;
+    lrt::Status s = toBatch(views, view_count, instances, instance_count, list, clouds, inst);
+    if (!s.ok()) return finish(ctx, s);
+    lrt::RenderSettings best = toSettings(*settings);
+    float bestMs = 0.0F;
+    s = ctx->raster.autotuneViews(list, inst, best, &bestMs);
+    if (!s.ok()) return finish(ctx, s);
+    settings->blend_batch = best.blendBatch;
+    settings->persistent_blend = best.persistentBlend ? 1 : 0;
+    settings->use_graph = best.useGraph ? 1 : 0;
+    if (ms != nullptr) *ms = bestMs;
     return finish(ctx, s);
 }
 

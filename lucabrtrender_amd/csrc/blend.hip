@@ -80,14 +80,15 @@ __device__ __forceinline__ Staged gather(const float4* __restrict__ proj, uint32
     return s;
 }
 
+// Quadrant `quadrant` of tile `tile` of the frame f, whose pairs are
+// [range.x, range.y) of pairSlots; colour and depth are the frame's images,
+// lastCount/finalT/accum its per-pixel state (each may be null).
 template <uint32_t DEPTH_MODE, uint32_t LIST>
-__global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t slots, const uint32_t* __restrict__ order,
-                                                      const uint2* __restrict__ ranges,
-                                                      const uint32_t* __restrict__ pairSlots,
-                                                      const float4* __restrict__ proj,
-                                                      float4* __restrict__ colour, float* __restrict__ depth,
-                                                      uint32_t* __restrict__ lastCount,
-                                                      float* __restrict__ finalT, float4* __restrict__ accum) {
+__device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t tile, uint32_t quadrant, uint2 range,
+                                              uint32_t slots, const uint32_t* __restrict__ pairSlots,
+                                              const float4* __restrict__ proj, float4* __restrict__ colour,
+                                              float* __restrict__ depth, uint32_t* __restrict__ lastCount,
+                                              float* __restrict__ finalT, float4* __restrict__ accum) {
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
     // The list: records that reach this quadrant, in pair order, padded to a
     // multiple of kGroup with empty records (cutoff -1: never pass the test).
@@ -96,12 +97,6 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     __shared__ float4 sColour[LIST + kGroup];   // rgb, bits of the pair's place in the tile's range + 1
 
     const uint32_t lane = threadIdx.x;
-    const uint32_t tiles = f.tilesX * f.tilesY;
-    const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
-    const uint32_t place = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
-    if (place >= tiles) return;
-    const uint32_t tile = order != nullptr ? order[place] : place;
-    if (tile >= tiles) return;   // an order entry that is not a tile: draw nothing
     const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
     const uint32_t y0 = (tile / f.tilesX) * kTile + (quadrant >> 1) * 8u;
     if (x0 >= f.width || y0 >= f.height) return;   // a quadrant past the frame's edge has no pixels
@@ -114,7 +109,6 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     const float pyf = static_cast<float>(py) + 0.5f;
     const float boxLoX = static_cast<float>(x0) + 0.5f, boxLoY = static_cast<float>(y0) + 0.5f;
     const float boxHiX = static_cast<float>(x1 - 1u) + 0.5f, boxHiY = static_cast<float>(y1 - 1u) + 0.5f;
-    const uint2 range = ranges[tile];
     const uint64_t below = (1ull << lane) - 1ull;
 
     // Round k's records sit in `staged` while round k + 1's are gathered.
@@ -207,6 +201,73 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     finishPixel(f, px, py, r, g, b, t, depthSum, depthAt, last, colour, depth, lastCount, finalT, accum);
 }
 
+template <uint32_t DEPTH_MODE, uint32_t LIST>
+__global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t slots, const uint32_t* __restrict__ order,
+                                                      const uint2* __restrict__ ranges,
+                                                      const uint32_t* __restrict__ pairSlots,
+                                                      const float4* __restrict__ proj,
+                                                      float4* __restrict__ colour, float* __restrict__ depth,
+                                                      uint32_t* __restrict__ lastCount,
+                                                      float* __restrict__ finalT, float4* __restrict__ accum) {
+    const uint32_t tiles = f.tilesX * f.tilesY;
+    const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
+    const uint32_t place = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
+    if (place >= tiles) return;
+    const uint32_t tile = order != nullptr ? order[place] : place;
+    if (tile >= tiles) return;   // an order entry that is not a tile: draw nothing
+    blendQuadrant<DEPTH_MODE, LIST>(f, tile, quadrant, ranges[tile], slots, pairSlots, proj, colour, depth,
+                                    lastCount, finalT, accum);
+}
+
+// The view holding batch tile g: the last view whose tiles start at or before it.
+__device__ __forceinline__ uint32_t viewOfTile(const ViewInfo* __restrict__ views, uint32_t viewCount, uint32_t g) {
+    uint32_t lo = 0, hi = viewCount;   // views[lo].tileBase <= g < views[hi].tileBase
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].tileBase <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
+// Every quadrant of every view of a batch. As a grid (PERSISTENT false) block b
+// is task b, the four quadrants of a tile going to blocks b, b + 8, b + 16,
+// b + 24 of one XCD; persistent, each block takes tasks from a shared counter
+// until none are left (task t: tile t / 4, quadrant t % 4), so a block that
+// drew a light quadrant takes the next instead of leaving its slot idle.
+template <uint32_t DEPTH_MODE, uint32_t LIST, bool PERSISTENT>
+__global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __restrict__ params,
+                                                           const ViewInfo* __restrict__ views, uint32_t viewCount,
+                                                           uint32_t tiles, uint32_t tasks, uint32_t slots,
+                                                           const uint2* __restrict__ ranges,
+                                                           const uint32_t* __restrict__ pairSlots,
+                                                           const float4* __restrict__ proj,
+                                                           uint32_t* __restrict__ lastCount,
+                                                           float* __restrict__ finalT, uint32_t* __restrict__ queue) {
+    for (uint32_t round = 0;; ++round) {
+        uint32_t task, g, quadrant;
+        if constexpr (PERSISTENT) {
+            uint32_t taken = 0;
+            if (threadIdx.x == 0) taken = atomicAdd(queue, 1u);
+            task = static_cast<uint32_t>(__shfl(static_cast<int>(taken), 0, 64));
+            if (task >= tasks) return;
+            g = task >> 2;
+            quadrant = task & 3u;
+            if (round > 0) __syncthreads();   // the last task's LDS list is done with
+        } else {
+            task = blockIdx.x;
+            g = (task >> 5) * 8u + (task & 7u);
+            quadrant = (task >> 3) & 3u;
+        }
+        if (g < tiles) {
+            const ViewInfo& v = views[viewOfTile(views, viewCount, g)];
+            blendQuadrant<DEPTH_MODE, LIST>(params[v.params], g - v.tileBase, quadrant, ranges[g], slots, pairSlots,
+                                            proj, reinterpret_cast<float4*>(v.colour), v.depth,
+                                            lastCount + v.pixelBase, finalT + v.pixelBase, nullptr);
+        }
+        if constexpr (!PERSISTENT) return;
+    }
+}
+
 // Heaviest tiles first: tiles bucketed by the bit length of their pair count,
 // buckets in descending order, tiles in index order inside a bucket. One
 // workgroup; bucket counts in LDS.
@@ -272,7 +333,48 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
     }
 }
 
+template <uint32_t DEPTH_MODE, uint32_t LIST>
+void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* params, const ViewInfo* views,
+                      uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges, const uint32_t* pairSlots,
+                      const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue, hipStream_t stream) {
+    if (persistent) {
+        const uint32_t tasks = tiles * 4u;
+        const uint32_t grid = tasks < gridCap ? tasks : gridCap;
+        fill32(queue, 0u, 1, stream);
+        blendViewsKernel<DEPTH_MODE, LIST, true><<<grid, kLanes, 0, stream>>>(
+            params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
+    } else {
+        const uint32_t tasks = (tiles + 7u) / 8u * 32u;
+        blendViewsKernel<DEPTH_MODE, LIST, false><<<tasks, kLanes, 0, stream>>>(
+            params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
+    }
+}
+
 }   // namespace
+
+void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gridCap, const FrameParams* params,
+                const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges,
+                const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue,
+                hipStream_t stream) {
+    if (tiles == 0 || viewCount == 0) return;
+    if (depthMode == 0) {
+        if (list <= 128) {
+            launchBlendViews<0, 128>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+                                     proj, lastCount, finalT, queue, stream);
+        } else {
+            launchBlendViews<0, 256>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+                                     proj, lastCount, finalT, queue, stream);
+        }
+    } else {
+        if (list <= 128) {
+            launchBlendViews<1, 128>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+                                     proj, lastCount, finalT, queue, stream);
+        } else {
+            launchBlendViews<1, 256>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+                                     proj, lastCount, finalT, queue, stream);
+        }
+    }
+}
 
 void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t stream) {
     if (tiles == 0) return;

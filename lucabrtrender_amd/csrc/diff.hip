@@ -100,8 +100,11 @@ __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffC
 }
 
 __global__ __launch_bounds__(kBlock) void gatherSlotsKernel(const uint32_t* __restrict__ pairIndex,
-                                                            const uint32_t* __restrict__ pairSlot, uint32_t pairs,
+                                                            const uint32_t* __restrict__ pairSlot, uint32_t cap,
+                                                            const uint32_t* __restrict__ nDevice,
                                                             uint32_t* __restrict__ sortedSlots) {
+    uint32_t pairs = cap;
+    if (nDevice != nullptr && *nDevice < cap) pairs = *nDevice;
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i < pairs) sortedSlots[i] = pairSlot[pairIndex[i]];
 }
@@ -660,9 +663,10 @@ void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uin
 }
 
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
-                 hipStream_t stream) {
+                 hipStream_t stream, const uint32_t* nDevice) {
     if (pairs == 0) return;
-    gatherSlotsKernel<<<(pairs + kBlock - 1) / kBlock, kBlock, 0, stream>>>(pairIndex, pairSlot, pairs, sortedSlots);
+    gatherSlotsKernel<<<(pairs + kBlock - 1) / kBlock, kBlock, 0, stream>>>(pairIndex, pairSlot, pairs, nDevice,
+                                                                            sortedSlots);
 }
 
 void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,

@@ -26,18 +26,15 @@ using namespace lrt::dev;
 
 constexpr uint32_t kBlock = 256;
 
+// Slot s's pairs: keys (tileBase + tile) << 32 | bits(view z) in the frame of
+// width x height with tilesX tiles a row.
 template <bool EMIT_INDEX>
-__global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_t slots,
-                                                          const float4* __restrict__ proj,
-                                                          const uint2* __restrict__ rects,
-                                                          const uint32_t* __restrict__ touched,
-                                                          const uint32_t* __restrict__ offsets,
-                                                          uint64_t* __restrict__ keys,
-                                                          uint32_t* __restrict__ values,
-                                                          uint32_t* __restrict__ pairSlot, uint32_t capacity,
-                                                          uint32_t* __restrict__ overflow) {
-    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= slots) return;
+__device__ __forceinline__ void duplicateSlot(uint32_t s, uint32_t width, uint32_t height, uint32_t tilesX,
+                                              uint32_t tileBase, const float4* __restrict__ proj,
+                                              const uint2* __restrict__ rects, const uint32_t* __restrict__ touched,
+                                              const uint32_t* __restrict__ offsets, uint64_t* __restrict__ keys,
+                                              uint32_t* __restrict__ values, uint32_t* __restrict__ pairSlot,
+                                              uint32_t capacity, uint32_t* __restrict__ overflow) {
     const uint32_t expected = touched[s];
     if (expected == 0) return;
     const uint32_t first = offsets[s];
@@ -49,12 +46,12 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
     uint32_t written = 0;
     for (uint32_t ty = y0; ty <= y1 && written < expected; ++ty) {
         for (uint32_t tx = x0; tx <= x1 && written < expected; ++tx) {
-            if (!tileTouched(f.width, f.height, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
+            if (!tileTouched(width, height, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
                 continue;
             }
             const uint32_t at = first + written;
             if (at < capacity) {
-                keys[at] = (static_cast<uint64_t>(ty * f.tilesX + tx) << 32) | depthBits;
+                keys[at] = (static_cast<uint64_t>(tileBase + ty * tilesX + tx) << 32) | depthBits;
                 if constexpr (EMIT_INDEX) {
                     values[at] = at;
                     pairSlot[at] = s;
@@ -67,6 +64,39 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
             written += 1;
         }
     }
+}
+
+template <bool EMIT_INDEX>
+__global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_t slots,
+                                                          const float4* __restrict__ proj,
+                                                          const uint2* __restrict__ rects,
+                                                          const uint32_t* __restrict__ touched,
+                                                          const uint32_t* __restrict__ offsets,
+                                                          uint64_t* __restrict__ keys,
+                                                          uint32_t* __restrict__ values,
+                                                          uint32_t* __restrict__ pairSlot, uint32_t capacity,
+                                                          uint32_t* __restrict__ overflow) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= slots) return;
+    duplicateSlot<EMIT_INDEX>(s, f.width, f.height, f.tilesX, 0u, proj, rects, touched, offsets, keys, values,
+                              pairSlot, capacity, overflow);
+}
+
+// A batch of views: slot s belongs to view s / splatsPerView.
+__global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* __restrict__ views,
+                                                               uint32_t splatsPerView, uint32_t slots,
+                                                               const float4* __restrict__ proj,
+                                                               const uint2* __restrict__ rects,
+                                                               const uint32_t* __restrict__ touched,
+                                                               const uint32_t* __restrict__ offsets,
+                                                               uint64_t* __restrict__ keys,
+                                                               uint32_t* __restrict__ values, uint32_t capacity,
+                                                               uint32_t* __restrict__ overflow) {
+    const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
+    if (s >= slots) return;
+    const ViewInfo& v = views[s / splatsPerView];
+    duplicateSlot<false>(s, v.width, v.height, v.tilesX, v.tileBase, proj, rects, touched, offsets, keys, values,
+                         nullptr, capacity, overflow);
 }
 
 }   // namespace
@@ -83,6 +113,14 @@ void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const u
         duplicateKernel<false><<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values,
                                                               nullptr, capacity, overflow);
     }
+}
+
+void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
+                    const uint2* rects, const uint32_t* touched, const uint32_t* offsets, uint64_t* keys,
+                    uint32_t* values, uint32_t capacity, uint32_t* overflow, hipStream_t stream) {
+    if (slots == 0 || splatsPerView == 0) return;
+    duplicateViewsKernel<<<(slots + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
+        views, splatsPerView, slots, proj, rects, touched, offsets, keys, values, capacity, overflow);
 }
 
 }   // namespace lrt::kern

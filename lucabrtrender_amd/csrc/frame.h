@@ -40,6 +40,19 @@ struct FrameParams {
     SplatEditParams edit;
 };
 
+// One camera of a batched frame as the duplication and the blend see it. A
+// batch's views share one slot space (view v's splats at v * splatsPerView)
+// and one tile space (view v's tiles from tileBase); pixelBase places the
+// view's per-pixel blend state in the batch's arrays.
+struct ViewInfo {
+    uint32_t width, height, tilesX, tilesY;
+    uint32_t tileBase;     // first tile of this view in the batch
+    uint32_t params;       // index of its (first instance's) FrameParams
+    uint64_t pixelBase;
+    float* colour;         // float4 a pixel, the caller's
+    float* depth;          // may be null
+};
+
 // What the projection leaves for the blend, one record per slot: the
 // reference's ProjRecord (frame.slang:85-89), three float4 = 48 bytes.
 //   screen = centre.xy, view z, cutoff power

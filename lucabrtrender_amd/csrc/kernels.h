@@ -30,6 +30,11 @@ void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* stat
 // x1 | y1 << 16) and touched (u32 a slot).
 void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
              const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream);
+// The same instance (count splats) for `views` views at once: view v's
+// constants are params[v] (device), its slots start at params[v].base.
+void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,
+                  const uint4* shape, const uint32_t* sh, const float* filter3d, float4* proj, uint2* rects,
+                  uint32_t* touched, hipStream_t stream);
 
 // ---- duplication ------------------------------------------------------------
 // For each of `slots` slots with touched > 0: its pairs at offsets[slot].., keys
@@ -39,6 +44,11 @@ void project(const FrameParams& f, const float4* positions, const uint4* shape, 
 void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
                const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
                uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream);
+// A batch of views (device ViewInfo array): slot s is view s / splatsPerView's,
+// its keys carry the view's tileBase.
+void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
+                    const uint2* rects, const uint32_t* touched, const uint32_t* offsets, uint64_t* keys,
+                    uint32_t* values, uint32_t capacity, uint32_t* overflow, hipStream_t stream);
 
 // ---- sort -------------------------------------------------------------------
 // Which bits of a 64-bit key the digits come from.
@@ -62,13 +72,17 @@ uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits);
 // result is in (keys, values) after an even number of passes and in
 // (keysAlt, valuesAlt) after an odd one; returns true for the latter.
 // afterHistogram (may be null) is recorded between the histogram and the passes.
+// With nDevice non-null the count is *nDevice, read by the kernels, and n is
+// the capacity that bounds it (the launch is sized for n; nothing waits on the host).
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
                KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
-               hipEvent_t afterHistogram = nullptr);
+               hipEvent_t afterHistogram = nullptr, const uint32_t* nDevice = nullptr);
 
 // ---- ranges -----------------------------------------------------------------
 // ranges[t] = [start, end) of tile t's run in the sorted keys, (0, 0) when empty.
-void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream);
+// With nDevice non-null the count is *nDevice and `pairs` its capacity.
+void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream,
+                const uint32_t* nDevice = nullptr);
 
 // ---- blend ------------------------------------------------------------------
 // Tile indices, heaviest tile (most pairs) first, for the blend's launch order.
@@ -81,6 +95,29 @@ void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
            float* finalT, float4* accum, hipStream_t stream);
+
+// A batch of views in one launch (device FrameParams and ViewInfo arrays;
+// views sorted by tileBase, `tiles` in all): as a grid of quadrant tasks, or
+// persistent (at most gridCap workgroups taking tasks from *queue, which is
+// cleared here). lastCount/finalT hold the batch's pixels (ViewInfo.pixelBase).
+void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gridCap, const FrameParams* params,
+                const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges,
+                const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue,
+                hipStream_t stream);
+
+// ---- brute force in FP64 -----------------------------------------------------------
+// Bytes of one FP64 brute-force record.
+uint64_t referenceRecordBytes();
+// Instance f's splats in FP64 into records[slot]; keys[slot] = the depth order
+// key (exact z bits, or with keyed the rasteriser's 24-bit key in the high
+// word), all ones for a culled splat; values[slot] = slot; *live += visible.
+void referenceProject(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
+                      const float* filter3d, void* records, uint64_t* keys, uint32_t* values, uint32_t* live,
+                      bool keyed, hipStream_t stream);
+// Every pixel over the first *live slots of `order`, in FP64: colour 4 doubles
+// a pixel, depth one (may be null).
+void referenceBlend(const FrameParams& f, const void* records, const uint32_t* order, const uint32_t* live,
+                    double* colour, double* depth, hipStream_t stream);
 
 // ---- differentiable path -------------------------------------------------------
 // An FP32-parameter cloud (device pointers): means N x 3, log scales N x 3,
@@ -106,9 +143,10 @@ struct DiffGrads {
 };
 void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uint2* rects, uint32_t* touched,
                  hipStream_t stream);
-// sortedSlots[i] = pairSlot[pairIndex[i]] for i < pairs.
+// sortedSlots[i] = pairSlot[pairIndex[i]] for i < pairs (or < *nDevice, pairs
+// then being the capacity).
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
-                 hipStream_t stream);
+                 hipStream_t stream, const uint32_t* nDevice = nullptr);
 // dL/d(colour) per pixel -> one screen-space gradient record per pair (3 float4:
 // d centre.xy, d conic a b; d conic c, d alpha, d r g; d b) at pairGrad[index].
 void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,

@@ -23,12 +23,11 @@ using namespace lrt::dev;
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kMaxShWords = 23;   // 45 halves at degree 3, rgb per basis
 
-__global__ __launch_bounds__(kBlock) void projectKernel(FrameParams f, const float4* __restrict__ positions,
-                                                        const uint4* __restrict__ shape,
-                                                        const uint32_t* __restrict__ sh,
-                                                        const float* __restrict__ filter3d,
-                                                        float4* __restrict__ proj, uint2* __restrict__ rects,
-                                                        uint32_t* __restrict__ touchedOut) {
+// One workgroup's 256 splats of one instance, from the frame constants f.
+__device__ __forceinline__ void projectBlock(const FrameParams& f, const float4* __restrict__ positions,
+                                             const uint4* __restrict__ shape, const uint32_t* __restrict__ sh,
+                                             const float* __restrict__ filter3d, float4* __restrict__ proj,
+                                             uint2* __restrict__ rects, uint32_t* __restrict__ touchedOut) {
     __shared__ uint32_t sSh[kBlock * kMaxShWords];
     const uint32_t tid = threadIdx.x;
     const uint32_t first = blockIdx.x * kBlock;
@@ -83,6 +82,26 @@ __global__ __launch_bounds__(kBlock) void projectKernel(FrameParams f, const flo
     touchedOut[slot] = touched;
 }
 
+__global__ __launch_bounds__(kBlock) void projectKernel(FrameParams f, const float4* __restrict__ positions,
+                                                        const uint4* __restrict__ shape,
+                                                        const uint32_t* __restrict__ sh,
+                                                        const float* __restrict__ filter3d,
+                                                        float4* __restrict__ proj, uint2* __restrict__ rects,
+                                                        uint32_t* __restrict__ touchedOut) {
+    projectBlock(f, positions, shape, sh, filter3d, proj, rects, touchedOut);
+}
+
+// One instance seen by many views: view blockIdx.y's constants from params.
+__global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* __restrict__ params,
+                                                             const float4* __restrict__ positions,
+                                                             const uint4* __restrict__ shape,
+                                                             const uint32_t* __restrict__ sh,
+                                                             const float* __restrict__ filter3d,
+                                                             float4* __restrict__ proj, uint2* __restrict__ rects,
+                                                             uint32_t* __restrict__ touchedOut) {
+    projectBlock(params[blockIdx.y], positions, shape, sh, filter3d, proj, rects, touchedOut);
+}
+
 }   // namespace
 
 void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
@@ -90,6 +109,14 @@ void project(const FrameParams& f, const float4* positions, const uint4* shape, 
     if (f.count == 0) return;
     const uint32_t blocks = (f.count + kBlock - 1) / kBlock;
     projectKernel<<<blocks, kBlock, 0, stream>>>(f, positions, shape, sh, filter3d, proj, rects, touched);
+}
+
+void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,
+                  const uint4* shape, const uint32_t* sh, const float* filter3d, float4* proj, uint2* rects,
+                  uint32_t* touched, hipStream_t stream) {
+    if (count == 0 || views == 0) return;
+    const dim3 grid((count + kBlock - 1) / kBlock, views);
+    projectViewsKernel<<<grid, kBlock, 0, stream>>>(params, positions, shape, sh, filter3d, proj, rects, touched);
 }
 
 }   // namespace lrt::kern

@@ -108,6 +108,8 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
     bool filter3d = false;
     uint32_t blendBatch = 128;          // records each quadrant's LDS list holds: 128 or 256
     bool heaviestTilesFirst = false;    // launch the tiles with the most pairs first
+    bool persistentBlend = false;       // batches: a persistent blend taking quadrants from a queue
+    bool useGraph = false;              // batches: replay the launch sequence as a captured hipGraph
 };
 
 struct RenderTargets {   // caller-owned device images, bottom row first
@@ -119,6 +121,13 @@ struct RenderTargets {   // caller-owned device images, bottom row first
 struct FrameStats {   // render::FrameStats (TileRasterizer.h:249-255)
     uint32_t splats = 0, visible = 0, pairs = 0;
     double projectMs = 0, depthSortMs = 0, countsMs = 0, emitMs = 0, tileSortMs = 0, blendMs = 0, totalMs = 0;
+};
+
+// One camera of a batch: its projection and its images (the targets' size is
+// the view's frame size).
+struct View {
+    Projection projection;
+    RenderTargets targets;
 };
 
 // setFrame + setObject (modules/render/src/FrameParams.h:32-69).
@@ -163,10 +172,32 @@ public:
         return render(projectionFor(camera, settings.width, settings.height), instances, settings, targets, stats);
     }
 
+    // Many views of the same instances in one launch sequence: one projection
+    // launch per instance for all views, one scan, one duplication, one sort
+    // over every view's pairs, one blend over every view's tiles. No host wait
+    // until the end (the pair count stays on the device); with
+    // settings.useGraph the sequence is captured once per batch shape and
+    // replayed. settings.width/height are ignored (each view's targets give
+    // its size). stats: the batch's totals.
+    Status renderViews(std::span<const View> views, std::span<const SplatInstance> instances,
+                       const RenderSettings& settings, FrameStats* stats);
+    // Times the batch under each launch choice (list length 128/256, grid or
+    // persistent blend, graph replay or not; the best of three runs each) and
+    // leaves the fastest in `settings`; *ms gets its time.
+    Status autotuneViews(std::span<const View> views, std::span<const SplatInstance> instances,
+                         RenderSettings& settings, float* ms);
+
     // The differentiable frame: one FP32-parameter cloud, drawn by the same
     // stages as render(); it keeps what backward() needs.
     Status renderDiff(const Projection& projection, const kern::DiffCloud& cloud, const Mat4& objectToWorld,
                       const RenderSettings& settings, RenderTargets& targets, FrameStats* stats);
+    // The brute force in FP64 (the error oracle of whole frames): every splat of
+    // every instance in (view z, index) order, or with keyed in the
+    // rasteriser's (24-bit depth key, slot) order, per pixel, no cap. colour: 4
+    // doubles a pixel, depth 1 (may be null), device, settings' size.
+    Status referenceRender(const Projection& projection, std::span<const SplatInstance> instances,
+                           const RenderSettings& settings, double* colour, double* depth, bool keyed);
+
     // Gradients of the last renderDiff's parameters given dL/d(colour) (one
     // float4 a pixel, bottom row first). millis (may be null) receives the GPU
     // ms of the blend backward and of the preprocess backward.
@@ -185,12 +216,16 @@ private:
     Status reservePairs(uint32_t count, uint32_t keyBits, bool training = false);
     Status reserveTargets(uint32_t width, uint32_t height, uint32_t tiles, bool training);
     Status checkFrame(const RenderSettings& settings, const RenderTargets& targets) const;
+    // The launch sequence of a batch (no waits), on `stream`.
+    void enqueueViews(std::span<const SplatInstance> instances, uint32_t viewCount, uint32_t splatsPerView,
+                      uint32_t tiles, const RenderSettings& settings, hipStream_t stream);
     // Everything after the projection: scan, duplicate, sort, ranges, blend.
     Status finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
                        RenderTargets& targets, FrameStats* stats, bool training);
     Status check(hipError_t e, const char* what);
 
     int device_ = 0;
+    uint32_t computeUnits_ = 1;
     hipStream_t stream_ = nullptr;
     bool created_ = false;
 
@@ -203,6 +238,8 @@ private:
     // Training frames: each pair's slot in slot-major order, the slots in sorted
     // order, and the backward's per-pair gradient records.
     DeviceBuffer pairSlot_, sortedSlots_, pairGrad_;
+    // The FP64 brute force's records, keys and values (ping-pong) and sort workspace.
+    DeviceBuffer refRecords_, refKeys_[2], refValues_[2], refWork_;
     uint64_t sortWorkWords_ = 0;
     // Per tile and per pixel.
     uint32_t tileCapacity_ = 0;
@@ -221,6 +258,16 @@ private:
     bool lastOrdered_ = false;
     uint64_t lastPixels_ = 0;
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
+    // Batches: the frame constants and view records (pinned host copies that
+    // a launch uploads, and their device arrays), the capture stream and the
+    // graph of the last batch shape.
+    FrameParams* hostParams_ = nullptr;
+    ViewInfo* hostViews_ = nullptr;
+    uint32_t batchCapacity_ = 0;   // FrameParams entries the four buffers hold
+    DeviceBuffer viewParams_, viewInfo_;
+    hipStream_t captureStream_ = nullptr;
+    hipGraphExec_t graph_ = nullptr;
+    std::string graphKey_;
     // The last differentiable frame.
     bool diffReady_ = false;
     kern::DiffCloud diffCloud_{};

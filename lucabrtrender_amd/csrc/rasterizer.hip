@@ -10,6 +10,7 @@
 
 #include <algorithm>
 #include <chrono>
+#include <cstdint>
 #include <cstring>
 #include <string>
 
@@ -58,6 +59,10 @@ TileRasterizer::~TileRasterizer() {
     for (hipEvent_t& e : events_)
         if (e != nullptr) (void)hipEventDestroy(e);
     if (hostCounters_ != nullptr) (void)hipHostFree(hostCounters_);
+    if (hostParams_ != nullptr) (void)hipHostFree(hostParams_);
+    if (hostViews_ != nullptr) (void)hipHostFree(hostViews_);
+    if (graph_ != nullptr) (void)hipGraphExecDestroy(graph_);
+    if (captureStream_ != nullptr) (void)hipStreamDestroy(captureStream_);
 }
 
 Status TileRasterizer::check(hipError_t e, const char* what) {
@@ -83,6 +88,10 @@ Status TileRasterizer::create(int device, hipStream_t stream) {
         s = check(hipEventCreate(&e), "hipEventCreate");
         if (!s.ok()) return s;
     }
+    int cus = 0;
+    s = check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "device attribute");
+    if (!s.ok()) return s;
+    computeUnits_ = static_cast<uint32_t>(cus > 0 ? cus : 1);
     s = check(hipHostMalloc(reinterpret_cast<void**>(&hostCounters_), 64, hipHostMallocDefault), "hipHostMalloc");
     if (!s.ok()) return s;
     s = counters_.reserve(64, "counters");
@@ -366,6 +375,344 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
         stats.blendMs = kernelMs_[6];
     }
     if (statsOut != nullptr) *statsOut = stats;
+    return {};
+}
+
+void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint32_t viewCount,
+                                  uint32_t splatsPerView, uint32_t tiles, const RenderSettings& settings,
+                                  hipStream_t stream) {
+    const uint32_t entries = viewCount * static_cast<uint32_t>(std::max<size_t>(instances.size(), 1));
+    uint32_t* counters = counters_.as<uint32_t>();
+    auto* params = viewParams_.as<FrameParams>();
+    auto* views = viewInfo_.as<ViewInfo>();
+    (void)hipMemcpyAsync(params, hostParams_, sizeof(FrameParams) * entries, hipMemcpyHostToDevice, stream);
+    (void)hipMemcpyAsync(views, hostViews_, sizeof(ViewInfo) * viewCount, hipMemcpyHostToDevice, stream);
+    // Project: one launch per instance covers every view.
+    for (size_t k = 0; k < instances.size(); ++k) {
+        const SplatCloud* cloud = instances[k].splats;
+        if (cloud == nullptr || cloud->count == 0) continue;
+        kern::projectViews(params + k * viewCount, viewCount, cloud->count, cloud->positions, cloud->shape,
+                           cloud->sh, cloud->filter3d, proj_.as<float4>(), rects_.as<uint2>(),
+                           touched_.as<uint32_t>(), stream);
+    }
+    const uint32_t slots = viewCount * splatsPerView;
+    kern::fill32(counters, 0u, 8, stream);
+    kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), slots, scanStatus_.as<uint64_t>(),
+                        counters + 3, counters + 0, counters + 1, counters + 5, stream);
+    kern::duplicateViews(views, splatsPerView, slots, proj_.as<float4>(), rects_.as<uint2>(),
+                         touched_.as<uint32_t>(), offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(),
+                         values_[0].as<uint32_t>(), pairCapacity_, counters + 2, stream);
+    // The pair count (counters[0], from the scan) stays on the device: the
+    // sort and the ranges read it, launched for the capacity.
+    const uint32_t keyBits = 24 + bitsFor(tiles);
+    const uint32_t passes = kern::sortPasses(keyBits);
+    const uint32_t parts = kern::sortPartitions(pairCapacity_);
+    uint32_t* work = sortWork_.as<uint32_t>();
+    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512,
+                           work + passes * 512 + uint64_t{passes} * parts * 256, counters + 4};
+    const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
+                                       values_[1].as<uint32_t>(), pairCapacity_, kern::KeyMode::TileDepth, 0,
+                                       keyBits, ws, stream, nullptr, counters + 0);
+    const int sorted = inAlt ? 1 : 0;
+    kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream, counters + 0);
+    // A persistent blend keeps about 24 one-wave workgroups on each CU.
+    kern::blendViews(settings.depthMode, settings.blendBatch, settings.persistentBlend, computeUnits_ * 24u, params, views, viewCount, tiles, slots, ranges_.as<uint2>(),
+                     values_[sorted].as<uint32_t>(), proj_.as<float4>(), lastCount_.as<uint32_t>(),
+                     finalT_.as<float>(), counters + 6, stream);
+    (void)hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream);
+    lastSorted_ = sorted;
+}
+
+Status TileRasterizer::renderViews(std::span<const View> views, std::span<const SplatInstance> instances,
+                                   const RenderSettings& settings, FrameStats* statsOut) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (views.empty()) return Status::error(ErrorCode::InvalidArgument, "a batch needs a view");
+    uint64_t pixels = 0, tiles = 0;
+    for (const View& v : views) {
+        const RenderTargets& t = v.targets;
+        if (t.width == 0 || t.height == 0 || t.colour == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a view without a size or a colour target");
+        }
+        if (t.width > 0xFFFFu * kTile || t.height > 0xFFFFu * kTile) {
+            return Status::error(ErrorCode::Unsupported, "a frame wider or taller than 1048560 pixels");
+        }
+        pixels += uint64_t{t.width} * t.height;
+        tiles += uint64_t{(t.width + kTile - 1) / kTile} * ((t.height + kTile - 1) / kTile);
+    }
+    uint64_t splats = 0;
+    for (const SplatInstance& instance : instances) {
+        if (instance.splats == nullptr) continue;
+        const SplatCloud& c = *instance.splats;
+        if (c.count > 0 && (c.positions == nullptr || c.shape == nullptr)) {
+            return Status::error(ErrorCode::InvalidArgument, "a cloud without positions or shape");
+        }
+        if (c.count > 0 && c.restPerColour > 0 && c.sh == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a cloud with harmonics but no sh buffer");
+        }
+        if (c.restPerColour != 0 && c.restPerColour != 3 && c.restPerColour != 8 && c.restPerColour != 15) {
+            return Status::error(ErrorCode::InvalidArgument, "restPerColour must be 0, 3, 8 or 15");
+        }
+        if (c.restPerColour > 0 && (c.shWords < (c.restPerColour * 3 + 1) / 2 || c.shWords > 23)) {
+            return Status::error(ErrorCode::InvalidArgument, "shWords does not fit restPerColour");
+        }
+        if (settings.filter3d && c.count > 0 && c.filter3d == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and a cloud has no filter3d");
+        }
+        splats += c.count;
+    }
+    const uint64_t slots = splats * views.size();
+    if (slots > 0xFFFFFFFFull || tiles > (1ull << 24) || views.size() > 0xFFFFu) {
+        return Status::error(ErrorCode::Unsupported, "a batch over 2^32 slots, 2^24 tiles or 65535 views");
+    }
+    const uint32_t viewCount = static_cast<uint32_t>(views.size());
+    const uint32_t splatsPerView = static_cast<uint32_t>(splats);
+    const uint32_t entries = viewCount * static_cast<uint32_t>(std::max<size_t>(instances.size(), 1));
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = reserveSplats(static_cast<uint32_t>(std::max<uint64_t>(slots, 1)))).ok()) return s;
+    if (tiles > tileCapacity_) {
+        if (!(s = ranges_.reserve(tiles * 8, "tile.ranges")).ok()) return s;
+        if (!(s = tileOrder_.reserve(tiles * 4, "tile.order")).ok()) return s;
+        tileCapacity_ = static_cast<uint32_t>(tiles);
+    }
+    if (pixels > pixelCapacity_) {
+        if (!(s = lastCount_.reserve(pixels * 4, "pixel.lastCount")).ok()) return s;
+        if (!(s = finalT_.reserve(pixels * 4, "pixel.finalT")).ok()) return s;
+        pixelCapacity_ = pixels;
+    }
+    if (entries > batchCapacity_) {
+        if (hostParams_ != nullptr) (void)hipHostFree(hostParams_);
+        if (hostViews_ != nullptr) (void)hipHostFree(hostViews_);
+        hostParams_ = nullptr;
+        hostViews_ = nullptr;
+        batchCapacity_ = 0;
+        if (!(s = check(hipHostMalloc(reinterpret_cast<void**>(&hostParams_), sizeof(FrameParams) * entries,
+                                      hipHostMallocDefault), "hipHostMalloc")).ok())
+            return s;
+        if (!(s = check(hipHostMalloc(reinterpret_cast<void**>(&hostViews_), sizeof(ViewInfo) * entries,
+                                      hipHostMallocDefault), "hipHostMalloc")).ok())
+            return s;
+        if (!(s = viewParams_.reserve(sizeof(FrameParams) * entries, "batch.params")).ok()) return s;
+        if (!(s = viewInfo_.reserve(sizeof(ViewInfo) * entries, "batch.views")).ok()) return s;
+        batchCapacity_ = entries;
+    }
+    // The frame constants: entry k * views + v is instance k seen by view v,
+    // its slots from v * splatsPerView + the instance's base.
+    uint32_t tileBase = 0;
+    uint64_t pixelBase = 0;
+    for (uint32_t v = 0; v < viewCount; ++v) {
+        const View& view = views[v];
+        RenderSettings vs = settings;
+        vs.width = view.targets.width;
+        vs.height = view.targets.height;
+        uint32_t base = v * splatsPerView;
+        if (instances.empty()) {
+            hostParams_[v] = frameParamsFor(view.projection, SplatInstance{}, vs, base);
+        }
+        for (size_t k = 0; k < instances.size(); ++k) {
+            hostParams_[k * viewCount + v] = frameParamsFor(view.projection, instances[k], vs, base);
+            if (instances[k].splats != nullptr) base += instances[k].splats->count;
+        }
+        ViewInfo& info = hostViews_[v];
+        info.width = view.targets.width;
+        info.height = view.targets.height;
+        info.tilesX = (info.width + kTile - 1) / kTile;
+        info.tilesY = (info.height + kTile - 1) / kTile;
+        info.tileBase = tileBase;
+        info.params = v;
+        info.pixelBase = pixelBase;
+        info.colour = reinterpret_cast<float*>(view.targets.colour);
+        info.depth = view.targets.depth;
+        tileBase += info.tilesX * info.tilesY;
+        pixelBase += uint64_t{info.width} * info.height;
+    }
+    const uint32_t keyBits = 24 + bitsFor(static_cast<uint32_t>(tiles));
+    const auto wallStart = std::chrono::steady_clock::now();
+    diffReady_ = false;
+    // The pair capacity is the last batch's; a batch with more pairs overflows
+    // the duplication, which the host sees after the wait: then the capacity
+    // grows to the count and the batch runs again.
+    for (int attempt = 0; attempt < 2; ++attempt) {
+        if (!(s = reservePairs(std::max(pairCapacity_, 1u), keyBits)).ok()) return s;
+        if (settings.useGraph) {
+            // A captured graph holds its launches' sizes and every buffer
+            // address: any change of either needs a new capture.
+            std::string key = std::to_string(viewCount) + "/" + std::to_string(splatsPerView) + "/" +
+                              std::to_string(tiles) + "/" + std::to_string(pairCapacity_) + "/" +
+                              std::to_string(settings.blendBatch) + "/" + std::to_string(settings.persistentBlend) +
+                              "/" + std::to_string(settings.depthMode);
+            const auto addr = [&key](const void* p) { key += "/" + std::to_string(reinterpret_cast<uintptr_t>(p)); };
+            for (const SplatInstance& instance : instances) {
+                const SplatCloud* c = instance.splats;
+                addr(c);
+                if (c == nullptr) continue;
+                addr(c->positions);
+                addr(c->shape);
+                addr(c->sh);
+                addr(c->filter3d);
+                key += ":" + std::to_string(c->count);
+            }
+            for (const DeviceBuffer* b : {&proj_, &rects_, &touched_, &offsets_, &scanStatus_, &keys_[0], &keys_[1],
+                                          &values_[0], &values_[1], &sortWork_, &ranges_, &lastCount_, &finalT_,
+                                          &counters_, &viewParams_, &viewInfo_})
+                addr(b->get());
+            addr(hostParams_);
+            addr(hostViews_);
+            addr(hostCounters_);
+            if (graph_ == nullptr || key != graphKey_) {
+                if (graph_ != nullptr) (void)hipGraphExecDestroy(graph_);
+                graph_ = nullptr;
+                if (captureStream_ == nullptr &&
+                    !(s = check(hipStreamCreateWithFlags(&captureStream_, hipStreamNonBlocking), "capture stream"))
+                         .ok())
+                    return s;
+                if (!(s = check(hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal),
+                                "begin capture")).ok())
+                    return s;
+                enqueueViews(instances, viewCount, splatsPerView, static_cast<uint32_t>(tiles), settings,
+                             captureStream_);
+                hipGraph_t graph = nullptr;
+                if (!(s = check(hipStreamEndCapture(captureStream_, &graph), "end capture")).ok()) return s;
+                s = check(hipGraphInstantiate(&graph_, graph, nullptr, nullptr, 0), "graph instantiate");
+                (void)hipGraphDestroy(graph);
+                if (!s.ok()) {
+                    graph_ = nullptr;
+                    return s;
+                }
+                graphKey_ = key;
+            }
+            if (!(s = check(hipGraphLaunch(graph_, stream_), "graph launch")).ok()) return s;
+        } else {
+            enqueueViews(instances, viewCount, splatsPerView, static_cast<uint32_t>(tiles), settings, stream_);
+        }
+        if (!(s = check(hipStreamSynchronize(stream_), "batch")).ok()) return s;
+        if (!(s = check(hipGetLastError(), "batch")).ok()) return s;
+        if (hostCounters_[5] != 0) return Status::error(ErrorCode::InternalError, "the scan's look-back gave up");
+        const uint32_t pairs = hostCounters_[0];
+        if (pairs > kMaxPairs) {
+            return Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 (tile, splat) pairs in one batch");
+        }
+        if (hostCounters_[2] == 0) break;
+        if (attempt == 1) return Status::error(ErrorCode::InternalError, "the duplication overran its pairs");
+        if (!(s = reservePairs(pairs, keyBits)).ok()) return s;
+    }
+    if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
+    const auto wallEnd = std::chrono::steady_clock::now();
+    lastSplats_ = static_cast<uint32_t>(slots);
+    lastVisible_ = hostCounters_[1];
+    lastPairs_ = hostCounters_[0];
+    lastTiles_ = static_cast<uint32_t>(tiles);
+    lastOrdered_ = false;
+    lastPixels_ = pixels;
+    if (statsOut != nullptr) {
+        FrameStats stats;
+        stats.splats = static_cast<uint32_t>(slots);
+        stats.visible = lastVisible_;
+        stats.pairs = lastPairs_;
+        stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
+        *statsOut = stats;
+    }
+    return {};
+}
+
+Status TileRasterizer::referenceRender(const Projection& projection, std::span<const SplatInstance> instances,
+                                       const RenderSettings& settings, double* colour, double* depth, bool keyed) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (settings.width == 0 || settings.height == 0 || colour == nullptr) {
+        return Status::error(ErrorCode::InvalidArgument, "a frame needs a size and a colour target");
+    }
+    uint64_t total = 0;
+    for (const SplatInstance& instance : instances) {
+        if (instance.splats == nullptr) continue;
+        const SplatCloud& c = *instance.splats;
+        if (c.count > 0 && (c.positions == nullptr || c.shape == nullptr)) {
+            return Status::error(ErrorCode::InvalidArgument, "a cloud without positions or shape");
+        }
+        if (c.count > 0 && c.restPerColour > 0 && c.sh == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a cloud with harmonics but no sh buffer");
+        }
+        if (settings.filter3d && c.count > 0 && c.filter3d == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and a cloud has no filter3d");
+        }
+        total += c.count;
+    }
+    if (total > kMaxPairs) return Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 splats");
+    const uint32_t slots = static_cast<uint32_t>(std::max<uint64_t>(total, 1));
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = refRecords_.reserve(uint64_t{slots} * kern::referenceRecordBytes(), "reference.records")).ok()) return s;
+    for (int k = 0; k < 2; ++k) {
+        if (!(s = refKeys_[k].reserve(uint64_t{slots} * 8, "reference.keys")).ok()) return s;
+        if (!(s = refValues_[k].reserve(uint64_t{slots} * 4, "reference.values")).ok()) return s;
+    }
+    const uint64_t words = kern::sortWorkspaceWords(slots, 64);
+    if (!(s = refWork_.reserve(words * 4, "reference.sortWorkspace")).ok()) return s;
+    uint32_t* counters = counters_.as<uint32_t>();
+    kern::fill32(counters, 0u, 8, stream_);
+    uint32_t base = 0;
+    FrameParams first{};
+    bool haveFirst = false;
+    for (const SplatInstance& instance : instances) {
+        const SplatCloud* cloud = instance.splats;
+        const FrameParams f = frameParamsFor(projection, instance, settings, base);
+        if (!haveFirst) {
+            first = f;
+            haveFirst = true;
+        }
+        if (cloud == nullptr || cloud->count == 0) continue;
+        kern::referenceProject(f, cloud->positions, cloud->shape, cloud->sh, cloud->filter3d, refRecords_.get(),
+                               refKeys_[0].as<uint64_t>(), refValues_[0].as<uint32_t>(), counters + 7, keyed, stream_);
+        base += cloud->count;
+    }
+    if (!haveFirst) first = frameParamsFor(projection, SplatInstance{}, settings, 0);
+    const uint32_t passes = kern::sortPasses(64);
+    const uint32_t parts = kern::sortPartitions(static_cast<uint32_t>(total));
+    uint32_t* work = refWork_.as<uint32_t>();
+    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512,
+                           work + passes * 512 + uint64_t{passes} * parts * 256, counters + 4};
+    const bool alt = kern::sortPairs(refKeys_[0].as<uint64_t>(), refValues_[0].as<uint32_t>(),
+                                     refKeys_[1].as<uint64_t>(), refValues_[1].as<uint32_t>(),
+                                     static_cast<uint32_t>(total), kern::KeyMode::Plain, 0, 64, ws, stream_);
+    kern::referenceBlend(first, refRecords_.get(), refValues_[alt ? 1 : 0].as<uint32_t>(), counters + 7, colour,
+                         depth, stream_);
+    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "reference")).ok())
+        return s;
+    if (!(s = check(hipStreamSynchronize(stream_), "reference")).ok()) return s;
+    if (!(s = check(hipGetLastError(), "reference")).ok()) return s;
+    if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
+    return {};
+}
+
+Status TileRasterizer::autotuneViews(std::span<const View> views, std::span<const SplatInstance> instances,
+                                     RenderSettings& settings, float* ms) {
+    RenderSettings best = settings;
+    double bestMs = -1.0;
+    for (const uint32_t list : {128u, 256u}) {
+        for (const bool persistent : {false, true}) {
+            for (const bool graph : {false, true}) {
+                RenderSettings trial = settings;
+                trial.blendBatch = list;
+                trial.persistentBlend = persistent;
+                trial.useGraph = graph;
+                Status s = renderViews(views, instances, trial, nullptr);   // sizes buffers, captures
+                if (!s.ok()) return s;
+                double fastest = -1.0;
+                for (int rep = 0; rep < 3; ++rep) {
+                    const auto t0 = std::chrono::steady_clock::now();
+                    if (!(s = renderViews(views, instances, trial, nullptr)).ok()) return s;
+                    const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0)
+                                         .count();
+                    if (fastest < 0.0 || t < fastest) fastest = t;
+                }
+                if (bestMs < 0.0 || fastest < bestMs) {
+                    bestMs = fastest;
+                    best = trial;
+                }
+            }
+        }
+    }
+    settings = best;
+    if (ms != nullptr) *ms = static_cast<float>(bestMs);
     return {};
 }
 

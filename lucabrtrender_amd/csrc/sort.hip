@@ -19,6 +19,8 @@
 // a stable sort on the chosen key bits.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace lrt::kern {
@@ -35,6 +37,14 @@ constexpr uint32_t kInclusive = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
 constexpr uint32_t kSpinLimit = 1u << 22;
 
+// The pairs to sort: the host's count, or the one a kernel left in device
+// memory (then the host's is the capacity and bounds it).
+__device__ __forceinline__ uint32_t pairCount(uint32_t n, const uint32_t* __restrict__ nDevice) {
+    if (nDevice == nullptr) return n;
+    const uint32_t d = *nDevice;
+    return d < n ? d : n;
+}
+
 template <KeyMode MODE>
 __device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64_t mask) {
     if constexpr (MODE == KeyMode::TileDepth) {
@@ -45,10 +55,12 @@ __device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64
 }
 
 template <KeyMode MODE>
-__global__ __launch_bounds__(kBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t n,
-                                                          uint32_t bitLo, uint64_t mask, uint32_t passes,
+__global__ __launch_bounds__(kBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
+                                                          const uint32_t* __restrict__ nDevice, uint32_t bitLo,
+                                                          uint64_t mask, uint32_t passes,
                                                           uint32_t* __restrict__ hist) {
     __shared__ uint32_t sHist[kMaxPasses * kRadix];
+    const uint32_t n = pairCount(cap, nDevice);
     for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kBlock) sHist[j] = 0;
     __syncthreads();
     const uint32_t stride = gridDim.x * kBlock;
@@ -62,6 +74,20 @@ __global__ __launch_bounds__(kBlock) void histogramKernel(const uint64_t* __rest
     for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kBlock) {
         const uint32_t c = sHist[j];
         if (c != 0) atomicAdd(&hist[j], c);
+    }
+}
+
+// Clears the look-back words of the partitions the count needs, every pass.
+__global__ __launch_bounds__(kBlock) void clearStatusKernel(uint32_t* __restrict__ status, uint32_t passes,
+                                                            uint32_t capParts, uint32_t cap,
+                                                            const uint32_t* __restrict__ nDevice) {
+    const uint32_t n = pairCount(cap, nDevice);
+    const uint64_t used = static_cast<uint64_t>((n + kPart - 1) / kPart) * kRadix;
+    const uint64_t total = used * passes;
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * kBlock;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * kBlock + threadIdx.x; i < total; i += stride) {
+        const uint64_t pass = i / used, at = i - pass * used;
+        status[pass * capParts * kRadix + at] = 0u;
     }
 }
 
@@ -92,7 +118,8 @@ template <KeyMode MODE>
 __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restrict__ keysIn,
                                                          const uint32_t* __restrict__ valsIn,
                                                          uint64_t* __restrict__ keysOut,
-                                                         uint32_t* __restrict__ valsOut, uint32_t n,
+                                                         uint32_t* __restrict__ valsOut, uint32_t cap,
+                                                         const uint32_t* __restrict__ nDevice,
                                                          uint32_t bitLo, uint64_t mask, uint32_t shift,
                                                          const uint32_t* __restrict__ digitOffsets,
                                                          uint32_t* __restrict__ status,
@@ -113,8 +140,12 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
 #pragma unroll
     for (uint32_t w = 0; w < kWaves; ++w) sWave[w][tid] = 0;
     __syncthreads();
+    const uint32_t n = pairCount(cap, nDevice);
     const uint32_t part = sPart;
     const uint32_t base = part * kPart;
+    // Launched for the capacity: partitions past the count have nothing to
+    // do, and no partition before them waits on them.
+    if (base >= n) return;
     const uint32_t waveFirst = base + wave * (kKpt * 64);
 
     uint64_t k[kKpt];
@@ -229,16 +260,22 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
 
 template <KeyMode MODE>
 void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
-                  uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
-                  hipEvent_t afterHistogram) {
+                  const uint32_t* nDevice, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws,
+                  hipStream_t stream, hipEvent_t afterHistogram) {
     const uint32_t passes = sortPasses(keyBits);
     const uint32_t parts = sortPartitions(n);
     const uint64_t mask = keyBits >= 64 ? ~0ull : ((1ull << keyBits) - 1ull);
     fill32(ws.histogram, 0u, static_cast<uint64_t>(passes) * kRadix, stream);
-    fill32(ws.status, 0u, static_cast<uint64_t>(passes) * parts * kRadix, stream);
+    if (nDevice == nullptr) {
+        fill32(ws.status, 0u, static_cast<uint64_t>(passes) * parts * kRadix, stream);
+    } else {
+        const uint64_t words = static_cast<uint64_t>(passes) * parts * kRadix;
+        const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((words + kBlock - 1) / kBlock, 4096));
+        clearStatusKernel<<<grid, kBlock, 0, stream>>>(ws.status, passes, parts, n, nDevice);
+    }
     fill32(ws.tickets, 0u, passes, stream);
     const uint32_t histBlocks = parts < 1024 ? parts : 1024;
-    histogramKernel<MODE><<<histBlocks, kBlock, 0, stream>>>(keys, n, bitLo, mask, passes, ws.histogram);
+    histogramKernel<MODE><<<histBlocks, kBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, passes, ws.histogram);
     digitOffsetsKernel<<<passes, kRadix, 0, stream>>>(ws.histogram, ws.offsets);
     if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
     const uint64_t* kin = keys;
@@ -247,7 +284,7 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     uint32_t* vout = valuesAlt;
     for (uint32_t p = 0; p < passes; ++p) {
         onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(
-            kin, vin, kout, vout, n, bitLo, mask, 8 * p, ws.offsets + p * kRadix,
+            kin, vin, kout, vout, n, nDevice, bitLo, mask, 8 * p, ws.offsets + p * kRadix,
             ws.status + static_cast<uint64_t>(p) * parts * kRadix, ws.tickets + p, ws.error);
         const uint64_t* nk = kout;
         const uint32_t* nv = vout;
@@ -269,16 +306,16 @@ uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits) {
 
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n, KeyMode mode,
                uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
-               hipEvent_t afterHistogram) {
-    if (n < 2 || keyBits == 0) {
+               hipEvent_t afterHistogram, const uint32_t* nDevice) {
+    if ((nDevice == nullptr && n < 2) || n == 0 || keyBits == 0) {
         if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
         return false;
     }
     if (mode == KeyMode::TileDepth) {
-        launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, bitLo, keyBits, ws, stream,
+        launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, ws, stream,
                                          afterHistogram);
     } else {
-        launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, bitLo, keyBits, ws, stream,
+        launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, ws, stream,
                                      afterHistogram);
     }
     return (sortPasses(keyBits) & 1u) != 0u;

@@ -121,6 +121,8 @@ class RenderSettings:
     filter3d: bool = False
     blend_batch: int = 128          # records each 8x8 quadrant's LDS list holds: 128 or 256
     tile_order: int = 0             # 0 tile index order, 1 heaviest tiles first
+    persistent_blend: bool = False  # batches: a persistent blend taking quadrants from a queue
+    use_graph: bool = False         # batches: capture the launch sequence once and replay it
 
     def to_c(self) -> _abi.RenderSettingsC:
         s = _abi.RenderSettingsC()
@@ -135,6 +137,8 @@ class RenderSettings:
         s.filter3d = 1 if self.filter3d else 0
         s.blend_batch = self.blend_batch
         s.tile_order = self.tile_order
+        s.persistent_blend = 1 if self.persistent_blend else 0
+        s.use_graph = 1 if self.use_graph else 0
         return s
 
 
@@ -239,6 +243,13 @@ class RenderTargets:
         self.depth = torch.empty((height, width), dtype=torch.float32, device=device)
 
 
+@dataclass
+class View:
+    """One camera of a batch: a Camera (or a projection) and its images."""
+    camera: object
+    targets: RenderTargets
+
+
 def _instances_c(instances: Sequence[SplatInstance]):
     arr = (_abi.SplatInstanceC * max(1, len(instances)))()
     for k, inst in enumerate(instances):
@@ -294,6 +305,49 @@ class TileRasterizer:
         self._check(lib.lrt_raster_render(self._ctx, C.byref(proj), arr, len(instances), C.byref(settings.to_c()),
                                           C.byref(t), C.byref(st)))
         return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
+
+    def reference_render(self, camera_or_projection, instances: Sequence[SplatInstance], settings: RenderSettings,
+                         keyed: bool = False):
+        """The brute force in FP64 on the GPU: (colour [H, W, 4], depth [H, W]) float64 CUDA tensors."""
+        import torch
+        proj = camera_or_projection
+        if isinstance(proj, Camera):
+            proj = projection_for(proj, settings.width, settings.height)
+        colour = torch.empty((settings.height, settings.width, 4), dtype=torch.float64, device="cuda")
+        depth = torch.empty((settings.height, settings.width), dtype=torch.float64, device="cuda")
+        self._check(lib.lrt_reference_render(self._ctx, C.byref(proj), _instances_c(instances), len(instances),
+                                             C.byref(settings.to_c()), colour.data_ptr(), depth.data_ptr(),
+                                             1 if keyed else 0))
+        return colour, depth
+
+    def _views_c(self, views: Sequence[View]):
+        arr = (_abi.ViewC * max(1, len(views)))()
+        for k, v in enumerate(views):
+            t = v.targets
+            proj = v.camera if isinstance(v.camera, _abi.ProjectionC) else projection_for(v.camera, t.width, t.height)
+            arr[k].projection = proj
+            arr[k].targets = _abi.RenderTargetsC(t.width, t.height, t.colour.data_ptr(),
+                                                 t.depth.data_ptr() if t.depth is not None else None)
+        return arr
+
+    def render_views(self, views: Sequence[View], instances: Sequence[SplatInstance],
+                     settings: RenderSettings) -> FrameStats:
+        """Every view of the instances in one launch sequence (each view's images as render() draws them)."""
+        st = _abi.FrameStatsC()
+        self._check(lib.lrt_raster_render_views(self._ctx, self._views_c(views), len(views), _instances_c(instances),
+                                                len(instances), C.byref(settings.to_c()), C.byref(st)))
+        return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
+
+    def autotune_views(self, views: Sequence[View], instances: Sequence[SplatInstance],
+                       settings: RenderSettings) -> tuple:
+        """The fastest launch choices for this batch: (settings with them, its batch ms)."""
+        import dataclasses
+        c = settings.to_c()
+        ms = C.c_float()
+        self._check(lib.lrt_autotune_views(self._ctx, self._views_c(views), len(views), _instances_c(instances),
+                                           len(instances), C.byref(c), C.byref(ms)))
+        return dataclasses.replace(settings, blend_batch=c.blend_batch, persistent_blend=bool(c.persistent_blend),
+                                   use_graph=bool(c.use_graph)), float(ms.value)
 
     def render_diff(self, camera_or_projection, splats: DiffSplats, settings: RenderSettings,
                     targets: RenderTargets, object_to_world=IDENTITY) -> FrameStats:

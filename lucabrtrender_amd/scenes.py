@@ -13,7 +13,10 @@ so a scene of N splats is the first N splats of any larger one.
             about the origin, 60 degrees; Mip-Splatting 3D filter on, 2D filter 0.1.
   config 3  1,000,000 splats drawn as config 1, degree 3, 1280x720.
   config 5  6,000,000 splats at 3840x2160: 80% as config 1, 20% pulled toward
-            the camera with log-scales N(-1, 0.5); opacity sigmoid(N(1,1)), degree 3.
+            the camera (into 0.2-1 of their distance) with log-scales N(-3.6, 0.3),
+            which puts about 7e8 (tile, splat) pairs in the frame (the config-1
+            part alone gives 51 a splat at this size); opacity sigmoid(N(1,1)),
+            degree 3.
 """
 from __future__ import annotations
 
@@ -198,7 +201,7 @@ def config(n: int, count: Optional[int] = None, seed: int = 0, view: float = 0.0
         return Config("fwd-1M-720p", scene, cam, 1280, 720)
     if n == 5:
         scene = synth(2, count or 6_000_000, seed, degree=3, offset=(0.0, 0.0, -4.0), log_scale=(-4.0, 0.5),
-                      opacity_logit=(1.0, 1.0), near=(0.2, -1.0, 0.5))
+                      opacity_logit=(1.0, 1.0), near=(0.2, -3.6, 0.3))
         cam = Camera.looking_at((0.0, 0.0, 0.0), (0.0, 0.0, -1.0))
         cam.lens = Lens(focal=focal_for_fov(60.0))
         return Config("sort-stress-6M-2160p", scene, cam, 3840, 2160)

@@ -1,0 +1,72 @@
+"""The FP64 brute force on the GPU (lrt_reference_render), the error oracle of
+whole frames.
+
+It agrees with the C restatement of the reference's brute force in FP64
+(or_reference_render_f64, pinned on the CPU by tests/test_oracle_raster.py) on
+the reference's own scene and cameras (randomCloud(3000, 7) at 250x190,
+tests/render/test_splat_render.cpp:70-121), in both orders; then it judges the
+tile rasteriser's full-resolution frames: p99 <= 1e-3 and max <= 1e-2 absolute
+per channel in linear FP32 (the figures are printed).
+"""
+import numpy as np
+import pytest
+import torch
+
+import fixtures as F
+import harness as H
+import oracle_lib as O
+from lucabrtrender_amd import raster, scenes
+
+pytestmark = pytest.mark.gpu
+
+W, HGT = 250, 190
+
+
+@pytest.fixture(scope="module")
+def rast():
+    return raster.TileRasterizer()
+
+
+@pytest.fixture(scope="module")
+def cloud():
+    return F.random_cloud(3000, 7).decode(3)
+
+
+def _cam(eye, target, **lens):
+    c = raster.Camera.looking_at(eye, target)
+    c.lens = raster.Lens(**lens)
+    return c
+
+
+CAMERAS = [_cam((1.5, 1.0, 6.0), (0, 0, 0), focal=30.0), _cam((0.3, -0.2, 2.2), (0, 0, 0), focal=18.0),
+           _cam((0.0, 3.0, 5.0), (0, 0, 0), orthographic=True, focal=5.0)]
+
+
+@pytest.mark.parametrize("keyed", [False, True])
+@pytest.mark.parametrize("cam_index", range(len(CAMERAS)))
+def test_gpu_fp64_brute_force_is_the_c_one(rast, cloud, cam_index, keyed):
+    cam = CAMERAS[cam_index]
+    s = raster.RenderSettings(width=W, height=HGT, background=(0.05, 0.1, 0.0, 0.25))
+    colour, depth = rast.reference_render(cam, [raster.SplatInstance(cloud.to_device())], s, keyed=keyed)
+    frames, clouds, n = H.oracle_frames(cam, s, [(cloud, np.eye(4), raster.SplatEdit())])
+    c_colour, c_depth = O.reference(frames, clouds, n, f64=True, keyed=keyed)
+    d = np.abs(colour.cpu().numpy() - c_colour).reshape(-1)
+    print(f"camera {cam_index} keyed {keyed}: GPU vs C FP64 brute force p99 {np.quantile(d, 0.99):.1e} "
+          f"max {d.max():.1e}")
+    assert np.quantile(d, 0.99) <= 1e-10 and d.max() <= 1e-2
+    dd = np.abs(depth.cpu().numpy() - c_depth).reshape(-1)
+    assert np.quantile(dd, 0.99) <= 1e-9
+
+
+def test_rasteriser_frame_against_the_fp64_brute_force_at_1080p(rast):
+    cfg = scenes.config(2, count=200_000)
+    s = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation, filter3d=cfg.filter3d)
+    inst = [raster.SplatInstance(cfg.scene.to_device())]
+    targets = raster.RenderTargets(cfg.width, cfg.height)
+    rast.render(cfg.camera, inst, s, targets)
+    colour, _ = rast.reference_render(cfg.camera, inst, s)
+    torch.cuda.synchronize()
+    d = np.abs(targets.colour.cpu().numpy().astype(np.float64) - colour.cpu().numpy()).reshape(-1)
+    p99, mx = float(np.quantile(d, 0.99)), float(d.max())
+    print(f"config 2 (200k splats) 1920x1080 vs FP64 brute force: p99 {p99:.2e} max {mx:.2e}")
+    assert p99 <= 1e-3 and mx <= 1e-2
