@@ -275,6 +275,9 @@ def main():
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="config-3 fwd+bwd views timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")
+    ap.add_argument("--profile-frames", type=int, default=0,
+                    help="only draw this many config-2 frames and one 256 MiB device copy (the counter "
+                         "calibration), for rocprofv3 --pmc; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -285,6 +288,22 @@ def main():
     import torch
 
     from lucabrtrender_amd import _abi, raster, scenes
+
+    if args.profile_frames > 0:
+        cfg = scenes.config(2, count=args.count)
+        settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
+                                         filter3d=cfg.filter3d, blend_batch=args.blend_batch)
+        inst = [raster.SplatInstance(cfg.scene.to_device())]
+        targets = raster.RenderTargets(cfg.width, cfg.height)
+        rast = raster.TileRasterizer()
+        for _ in range(args.profile_frames):
+            rast.render(cfg.camera, inst, settings, targets)
+        a = torch.empty(1 << 26, dtype=torch.float32, device="cuda")   # 256 MiB read, 256 MiB written
+        b = torch.empty_like(a)
+        b.copy_(a)
+        torch.cuda.synchronize()
+        log(f"{args.profile_frames} frames and one 256 MiB copy drawn")
+        return
 
     t0 = time.perf_counter()
     cfg = scenes.config(2, count=args.count)

@@ -66,6 +66,12 @@ def test_calls_without_a_context_are_invalid_arguments():
     assert lib.lrt_radix_sort(None, None, None, None, None, 0, 8, None) == 3
     assert lib.lrt_context_synchronize(None) == 3
     assert lib.lrt_projection_for(None, 1, 1, C.byref(p)) == 3
+    assert lib.lrt_raster_render_views(None, None, 0, None, 0, C.byref(s), None) == 3
+    assert lib.lrt_autotune_views(None, None, 0, None, 0, C.byref(s), None) == 3
+    assert lib.lrt_reference_render(None, C.byref(p), None, 0, C.byref(s), None, None, 0) == 3
+    d = _abi.DiffSplatsC()
+    assert lib.lrt_raster_render_diff(None, C.byref(p), C.byref(d), None, C.byref(s), C.byref(t), None) == 3
+    assert lib.lrt_raster_backward(None, None, C.byref(_abi.DiffGradsC()), None) == 3
 
 
 def test_frame_params_need_no_device():
