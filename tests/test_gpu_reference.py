@@ -59,14 +59,24 @@ def test_gpu_fp64_brute_force_is_the_c_one(rast, cloud, cam_index, keyed):
 
 
 def test_rasteriser_frame_against_the_fp64_brute_force_at_1080p(rast):
+    """Against the brute force in its own order, (view z, index), p99 <= 1e-3;
+    against it in the rasteriser's order (24-bit depth key, index), which the
+    reference's tile path shares, also max <= 1e-2. Splats whose depths tie
+    under the 24-bit key but not exactly are blended in another order by the
+    two, which is where the first comparison's largest differences sit."""
     cfg = scenes.config(2, count=200_000)
     s = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation, filter3d=cfg.filter3d)
     inst = [raster.SplatInstance(cfg.scene.to_device())]
     targets = raster.RenderTargets(cfg.width, cfg.height)
     rast.render(cfg.camera, inst, s, targets)
-    colour, _ = rast.reference_render(cfg.camera, inst, s)
-    torch.cuda.synchronize()
-    d = np.abs(targets.colour.cpu().numpy().astype(np.float64) - colour.cpu().numpy()).reshape(-1)
-    p99, mx = float(np.quantile(d, 0.99)), float(d.max())
-    print(f"config 2 (200k splats) 1920x1080 vs FP64 brute force: p99 {p99:.2e} max {mx:.2e}")
-    assert p99 <= 1e-3 and mx <= 1e-2
+    ours = targets.colour.cpu().numpy().astype(np.float64)
+    for keyed in (False, True):
+        colour, _ = rast.reference_render(cfg.camera, inst, s, keyed=keyed)
+        torch.cuda.synchronize()
+        d = np.abs(ours - colour.cpu().numpy()).reshape(-1)
+        p99, mx = float(np.quantile(d, 0.99)), float(d.max())
+        print(f"config 2 (200k splats) 1920x1080 vs FP64 brute force, keyed {keyed}: p99 {p99:.2e} max {mx:.2e} "
+              f"over 1e-3: {np.mean(d > 1e-3):.2e}")
+        assert p99 <= 1e-3
+        if keyed:
+            assert mx <= 1e-2
