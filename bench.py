@@ -55,6 +55,28 @@ def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes):
     }
 
 
+KERNEL_OF_STAGE = {"project": "projectKernel", "scan": "scanKernel", "duplicate": "duplicateKernel",
+                   "sort_histogram": "histogramKernel", "sort_passes": "onesweepKernel", "ranges": "rangesKernel",
+                   "blend": "blendKernel"}
+
+
+def measured_traffic_bytes(stage, passes):
+    """HBM bytes one launch of the stage's kernel moved, from the committed
+    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a device copy
+    of known size), or None without one."""
+    path = os.path.join(ROOT, "profiles", "traffic_scene-fwd-3M-1080p.txt")
+    try:
+        text = open(path).read()
+    except OSError:
+        return None
+    for line in text.splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) == 5 and cells[0].startswith(KERNEL_OF_STAGE[stage]):
+            mb = float(cells[4].replace(",", ""))
+            return int(mb * 1e6 * (passes if stage == "sort_passes" else 1))
+    return None
+
+
 def copy_bandwidth_gbs(torch):
     """Achieved device-to-device copy bandwidth (read + write bytes / time)."""
     n = 1 << 28   # 1 GiB of float32
@@ -348,7 +370,8 @@ def main():
     dominant = max(STAGES, key=lambda s: stages[s]["ms"])
     sort_ms = stages["sort_histogram"]["ms"] + stages["sort_passes"]["ms"]
     roofline = {"kernel": dominant, "bound": "hbm", "achieved": stages[dominant]["GB/s"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": stages[dominant]["hbm_frac"], "traffic": None}
+                "unit": "GB/s", "frac": stages[dominant]["hbm_frac"],
+                "traffic": measured_traffic_bytes(dominant, passes)}
     copy_gbs = copy_bandwidth_gbs(torch)
     # The blend at each batch depth and launch order, for the record.
     variants = {}
