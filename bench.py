@@ -248,10 +248,17 @@ def views_leg(torch, raster, scenes, reps):
     torch.cuda.synchronize()
     seq_s = (time.perf_counter() - start) / reps
     pixels = sum(v.targets.width * v.targets.height for v in views)
+    from lucabrtrender_amd import _abi
+    staged = raster.RenderSettings(**{**tuned.__dict__, "use_graph": False, "time_stages": True})
+    acc = np.zeros(8)
+    for _ in range(reps):
+        rast.render_views(views, inst, staged)
+        acc += rast.debug(_abi.DEBUG_KERNEL_MS)
+    stage_ms = {name: round(float(acc[k] / reps), 3) for k, name in enumerate(STAGES)}
     return {"workload": "views-fwd-2M-64", "splats": 2_000_000, "views": 64, "pixels": pixels, "pairs": stats.pairs,
             "batch_ms": round(batch_s * 1e3, 3), "views_per_s_batched": round(64 / batch_s, 1),
             "sequence_ms": round(seq_s * 1e3, 3), "views_per_s_single": round(64 / seq_s, 1),
-            "speedup": round(seq_s / batch_s, 3),
+            "speedup": round(seq_s / batch_s, 3), "stage_ms_eager": stage_ms,
             "tuned": {"blend_batch": tuned.blend_batch, "persistent_blend": tuned.persistent_blend,
                       "use_graph": tuned.use_graph}}
 

@@ -18,11 +18,12 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
 
 // ---- scan ----------------------------------------------------------------
 // Exclusive prefix sum of `n` u32 into `out` (may not alias `in`), the total
-// into *total and the number of non-zero inputs into *nonzero. `status` holds
-// scanPartitions(n) u64 words and `ticket` one u32; both are cleared here.
+// into *total (saturating at 2^32 - 1) and the number of non-zero inputs into
+// *nonzero. `status` holds scanStatusWords(n) u64 words, cleared here.
 uint32_t scanPartitions(uint32_t n);
-void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* ticket,
-                   uint32_t* total, uint32_t* nonzero, uint32_t* error, hipStream_t stream);
+inline uint64_t scanStatusWords(uint32_t n) { return uint64_t{scanPartitions(n)} * 3 / 2 + 2; }
+void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
+                   uint32_t* nonzero, uint32_t* error, hipStream_t stream);
 
 // ---- projection -----------------------------------------------------------
 // One instance: f.count splats from index 0 of the cloud arrays, written to
@@ -59,15 +60,17 @@ enum class KeyMode : uint32_t {
 };
 
 struct SortWorkspace {
-    uint32_t* histogram;   // passes * 256
+    uint32_t* histogram;   // one row of passes * 256 digit counts per histogram workgroup
     uint32_t* offsets;     // passes * 256
     uint32_t* status;      // passes * partitions * 256
-    uint32_t* tickets;     // passes
     uint32_t* error;       // 1
 };
 uint32_t sortPartitions(uint32_t n);
 uint32_t sortPasses(uint32_t keyBits);
+// Workspace words for a sort of capacity n on keyBits bits, and its layout in
+// `work` (which holds at least that many words).
 uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits);
+SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32_t* error);
 // Stable LSD radix sort, eight bits a pass, one onesweep launch a pass. The
 // result is in (keys, values) after an even number of passes and in
 // (keysAlt, valuesAlt) after an odd one; returns true for the latter.

@@ -91,15 +91,92 @@ __global__ __launch_bounds__(kBlock) void projectKernel(FrameParams f, const flo
     projectBlock(f, positions, shape, sh, filter3d, proj, rects, touchedOut);
 }
 
-// One instance seen by many views: view blockIdx.y's constants from params.
-__global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* __restrict__ params,
+constexpr uint32_t kViewsStaged = 32;   // view constants held in LDS at a time
+static_assert(sizeof(FrameParams) % 4 == 0, "FrameParams is staged as words");
+
+// One instance seen by many views, splat-major: each thread decodes its splat
+// once (scales, edit, 3D filter, rotation, the 45 harmonic coefficients) and
+// projects it for every view, the views' constants staged in LDS. The splat's
+// bytes are read once for the whole batch instead of once a view.
+__global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* __restrict__ params, uint32_t views,
                                                              const float4* __restrict__ positions,
                                                              const uint4* __restrict__ shape,
                                                              const uint32_t* __restrict__ sh,
                                                              const float* __restrict__ filter3d,
                                                              float4* __restrict__ proj, uint2* __restrict__ rects,
                                                              uint32_t* __restrict__ touchedOut) {
-    projectBlock(params[blockIdx.y], positions, shape, sh, filter3d, proj, rects, touchedOut);
+    constexpr uint32_t kWords = sizeof(FrameParams) / 4;
+    __shared__ uint32_t sSh[kBlock * kMaxShWords];
+    __shared__ uint32_t sFWords[kViewsStaged * kWords];   // FrameParams as words (no constructor in LDS)
+    const FrameParams* sF = reinterpret_cast<const FrameParams*>(sFWords);
+    const FrameParams& f0 = params[0];   // what every view shares: count, harmonics, edit, filter
+    const uint32_t tid = threadIdx.x;
+    const uint32_t first = blockIdx.x * kBlock;
+    const uint32_t i = first + tid;
+    const uint32_t count = f0.count;
+    const uint32_t degree = degreeOf(f0.restPerColour) < f0.shLimit ? degreeOf(f0.restPerColour) : f0.shLimit;
+    const uint32_t words = degree == 0 ? 0u : f0.shWords;
+    if (words > 0) {
+        const uint32_t inBlock = count - first < kBlock ? count - first : kBlock;
+        const uint32_t total = inBlock * words;
+        const uint32_t* src = sh + static_cast<uint64_t>(first) * words;
+        for (uint32_t w = tid; w < total; w += kBlock) sSh[w] = src[w];
+    }
+    const bool live = i < count;
+    const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
+    float4 p = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+    uint4 row = make_uint4(0u, 0u, 0u, 0u);
+    EditFate fate{true, false, 0.0f};
+    SplatShape shapeOf{0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f, {0.0f, 0.0f, 0.0f, 1.0f}};
+    if (live) {
+        p = positions[i];
+        row = shape[i];
+        const float sx = expF(lowHalf(row.y)), sy = expF(highHalf(row.y)), sz = expF(lowHalf(row.z));
+        fate = editFate(f0.edit, p.x, p.y, p.z, p.w, sx, sy, sz);
+        shapeOf = SplatShape{p.x, p.y, p.z, fate.opacity, sx, sy, sz, {0.0f, 0.0f, 0.0f, 0.0f}};
+        if (f0.useFilter3d) applyFilter3d(filter3d[i], shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
+        decodeQuaternion(row.x, shapeOf.q);
+    }
+    __syncthreads();
+    float coeff[45];
+    {
+        const uint32_t halves = degree == 0 ? 0u : (degree == 1 ? 9u : degree == 2 ? 24u : 45u);
+        const uint32_t* shRow = sSh + tid * words;
+#pragma unroll
+        for (uint32_t k = 0; k < 45; ++k) {
+            const uint32_t wd = live && k < halves ? shRow[k >> 1] : 0u;
+            coeff[k] = (k & 1u) == 0u ? lowHalf(wd) : highHalf(wd);
+        }
+    }
+    const auto colourOf = [&](float dx, float dy, float dz, float colour[3]) {
+        float rest[3];
+        evaluateRest(coeff, degree, dx, dy, dz, rest);
+        colour[0] = fmaxS(highHalf(row.z) + rest[0], 0.0f);
+        colour[1] = fmaxS(lowHalf(row.w) + rest[1], 0.0f);
+        colour[2] = fmaxS(highHalf(row.w) + rest[2], 0.0f);
+        if (fate.graded) editGrade(f0.edit, colour);
+    };
+    for (uint32_t v0 = 0; v0 < views; v0 += kViewsStaged) {
+        const uint32_t n = views - v0 < kViewsStaged ? views - v0 : kViewsStaged;
+        __syncthreads();   // the last group's constants are done with
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(params + v0);
+        for (uint32_t w = tid; w < n * kWords; w += kBlock) sFWords[w] = src[w];
+        __syncthreads();
+        if (!live) continue;
+        for (uint32_t v = 0; v < n; ++v) {
+            const FrameParams& f = sF[v];
+            const uint32_t slot = f.base + i;
+            if (fate.removed) {
+                touchedOut[slot] = 0;
+                rects[slot] = emptyRect;
+                continue;
+            }
+            uint2 rect = emptyRect;
+            const uint32_t touched = projectOne(f, shapeOf, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
+            rects[slot] = touched > 0 ? rect : emptyRect;
+            touchedOut[slot] = touched;
+        }
+    }
 }
 
 }   // namespace
@@ -115,8 +192,8 @@ void projectViews(const FrameParams* params, uint32_t views, uint32_t count, con
                   const uint4* shape, const uint32_t* sh, const float* filter3d, float4* proj, uint2* rects,
                   uint32_t* touched, hipStream_t stream) {
     if (count == 0 || views == 0) return;
-    const dim3 grid((count + kBlock - 1) / kBlock, views);
-    projectViewsKernel<<<grid, kBlock, 0, stream>>>(params, positions, shape, sh, filter3d, proj, rects, touched);
+    projectViewsKernel<<<(count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(params, views, positions, shape, sh,
+                                                                            filter3d, proj, rects, touched);
 }
 
 }   // namespace lrt::kern

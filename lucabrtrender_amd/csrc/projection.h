@@ -116,10 +116,7 @@ __device__ __forceinline__ uint32_t projectOne(const FrameParams& f, const Splat
     uint32_t rect[4];
     tileRect(f.width, f.height, cx, cy, ca, cb, cc, cutoff, rect);
     if (rect[0] > rect[2] || rect[1] > rect[3]) return 0;
-    uint32_t touched = 0;
-    for (uint32_t tyy = rect[1]; tyy <= rect[3]; ++tyy)
-        for (uint32_t txx = rect[0]; txx <= rect[2]; ++txx)
-            touched += tileTouched(f.width, f.height, cx, cy, ca, cb, cc, cutoff, txx, tyy) ? 1u : 0u;
+    const uint32_t touched = countTouchedTiles(f.width, f.height, cx, cy, ca, cb, cc, cutoff, rect);
     if (touched == 0) return 0;
 
     // Colour from this eye, in the cloud's own space.

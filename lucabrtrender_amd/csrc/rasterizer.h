@@ -245,7 +245,7 @@ private:
     uint32_t tileCapacity_ = 0;
     uint64_t pixelCapacity_ = 0;
     DeviceBuffer ranges_, tileOrder_, lastCount_, finalT_, accum_;
-    // Counters: [0] pairs, [1] visible, [2] overflow, [3] scan ticket, [4] sort error, [5] scan error.
+    // Counters: [0] pairs, [1] visible, [2] overflow, [3] unused, [4] sort error, [5] scan error.
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
     hipEvent_t events_[9] = {};

@@ -109,8 +109,7 @@ Status TileRasterizer::reserveSplats(uint32_t count) {
     if (!(s = rects_.reserve(n * 8, "splat.rects")).ok()) return s;
     if (!(s = touched_.reserve(n * 4, "splat.touched")).ok()) return s;
     if (!(s = offsets_.reserve(n * 4, "splat.offsets")).ok()) return s;
-    if (!(s = scanStatus_.reserve(uint64_t{kern::scanPartitions(static_cast<uint32_t>(n))} * 8 + 8,
-                                  "splat.scanStatus"))
+    if (!(s = scanStatus_.reserve(kern::scanStatusWords(static_cast<uint32_t>(n)) * 8, "splat.scanStatus"))
              .ok())
         return s;
     splatCapacity_ = static_cast<uint32_t>(n);
@@ -288,7 +287,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     // How many pairs and how many visible: one scan, one readback.
     kern::fill32(counters, 0u, 8, stream_);
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), splatCount, scanStatus_.as<uint64_t>(),
-                        counters + 3, counters + 0, counters + 1, counters + 5, stream_);
+                        counters + 0, counters + 1, counters + 5, stream_);
     mark(2);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "counts")).ok())
         return s;
@@ -312,11 +311,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     mark(4);
 
     // One stable sort on (tile, depth >> 7), then the ranges.
-    const uint32_t passes = kern::sortPasses(keyBits);
-    const uint32_t parts = kern::sortPartitions(pairCapacity_);
-    uint32_t* work = sortWork_.as<uint32_t>();
-    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512,
-                           work + passes * 512 + uint64_t{passes} * parts * 256, counters + 4};
+    const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairs, kern::KeyMode::TileDepth, 0, keyBits, ws,
                                        stream_, timed ? events_[5] : nullptr);
@@ -385,8 +380,14 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint
     uint32_t* counters = counters_.as<uint32_t>();
     auto* params = viewParams_.as<FrameParams>();
     auto* views = viewInfo_.as<ViewInfo>();
+    // Stage events as render()'s (kernelMs_), only when not capturing.
+    const bool timed = settings.timeStages && stream == stream_;
+    const auto mark = [&](int k) {
+        if (timed) (void)hipEventRecord(events_[k], stream);
+    };
     (void)hipMemcpyAsync(params, hostParams_, sizeof(FrameParams) * entries, hipMemcpyHostToDevice, stream);
     (void)hipMemcpyAsync(views, hostViews_, sizeof(ViewInfo) * viewCount, hipMemcpyHostToDevice, stream);
+    mark(0);
     // Project: one launch per instance covers every view.
     for (size_t k = 0; k < instances.size(); ++k) {
         const SplatCloud* cloud = instances[k].splats;
@@ -396,29 +397,32 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint
                            touched_.as<uint32_t>(), stream);
     }
     const uint32_t slots = viewCount * splatsPerView;
+    mark(1);
     kern::fill32(counters, 0u, 8, stream);
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), slots, scanStatus_.as<uint64_t>(),
-                        counters + 3, counters + 0, counters + 1, counters + 5, stream);
+                        counters + 0, counters + 1, counters + 5, stream);
+    mark(2);
+    mark(3);
     kern::duplicateViews(views, splatsPerView, slots, proj_.as<float4>(), rects_.as<uint2>(),
                          touched_.as<uint32_t>(), offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(),
                          values_[0].as<uint32_t>(), pairCapacity_, counters + 2, stream);
+    mark(4);
     // The pair count (counters[0], from the scan) stays on the device: the
     // sort and the ranges read it, launched for the capacity.
     const uint32_t keyBits = 24 + bitsFor(tiles);
-    const uint32_t passes = kern::sortPasses(keyBits);
-    const uint32_t parts = kern::sortPartitions(pairCapacity_);
-    uint32_t* work = sortWork_.as<uint32_t>();
-    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512,
-                           work + passes * 512 + uint64_t{passes} * parts * 256, counters + 4};
+    const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairCapacity_, kern::KeyMode::TileDepth, 0,
-                                       keyBits, ws, stream, nullptr, counters + 0);
+                                       keyBits, ws, stream, timed ? events_[5] : nullptr, counters + 0);
+    mark(6);
     const int sorted = inAlt ? 1 : 0;
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream, counters + 0);
+    mark(7);
     // A persistent blend keeps about 24 one-wave workgroups on each CU.
     kern::blendViews(settings.depthMode, settings.blendBatch, settings.persistentBlend, computeUnits_ * 24u, params, views, viewCount, tiles, slots, ranges_.as<uint2>(),
                      values_[sorted].as<uint32_t>(), proj_.as<float4>(), lastCount_.as<uint32_t>(),
                      finalT_.as<float>(), counters + 6, stream);
+    mark(8);
     (void)hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream);
     lastSorted_ = sorted;
 }
@@ -598,6 +602,17 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
     }
     if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
     const auto wallEnd = std::chrono::steady_clock::now();
+    if (settings.timeStages && !settings.useGraph) {
+        static constexpr int kFrom[7] = {0, 1, 3, 4, 5, 6, 7};
+        float sum = 0.0F;
+        for (int k = 0; k < 7; ++k) {
+            float ms = 0.0F;
+            (void)hipEventElapsedTime(&ms, events_[kFrom[k]], events_[kFrom[k] + 1]);
+            kernelMs_[k] = ms;
+            sum += ms;
+        }
+        kernelMs_[7] = sum;
+    }
     lastSplats_ = static_cast<uint32_t>(slots);
     lastVisible_ = hostCounters_[1];
     lastPairs_ = hostCounters_[0];
@@ -665,11 +680,8 @@ Status TileRasterizer::referenceRender(const Projection& projection, std::span<c
         base += cloud->count;
     }
     if (!haveFirst) first = frameParamsFor(projection, SplatInstance{}, settings, 0);
-    const uint32_t passes = kern::sortPasses(64);
-    const uint32_t parts = kern::sortPartitions(static_cast<uint32_t>(total));
-    uint32_t* work = refWork_.as<uint32_t>();
-    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512,
-                           work + passes * 512 + uint64_t{passes} * parts * 256, counters + 4};
+    const kern::SortWorkspace ws =
+        kern::sortWorkspace(refWork_.as<uint32_t>(), static_cast<uint32_t>(total), 64, counters + 4);
     const bool alt = kern::sortPairs(refKeys_[0].as<uint64_t>(), refValues_[0].as<uint32_t>(),
                                      refKeys_[1].as<uint64_t>(), refValues_[1].as<uint32_t>(),
                                      static_cast<uint32_t>(total), kern::KeyMode::Plain, 0, 64, ws, stream_);
@@ -755,7 +767,7 @@ Status TileRasterizer::prefixSum(const uint32_t* in, uint32_t* out, uint32_t* to
     if (!s.ok()) return s;
     if (!(s = reserveSplats(std::max(count, 1u))).ok()) return s;
     uint32_t* counters = counters_.as<uint32_t>();
-    kern::scanExclusive(in, out, count, scanStatus_.as<uint64_t>(), counters + 3, total, counters + 1, counters + 5,
+    kern::scanExclusive(in, out, count, scanStatus_.as<uint64_t>(), total, counters + 1, counters + 5,
                         stream_);
     return check(hipGetLastError(), "prefix sum");
 }
@@ -770,16 +782,12 @@ Status TileRasterizer::radixSort(uint64_t* keys, uint32_t* values, uint64_t* scr
     }
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
-    const uint32_t passes = kern::sortPasses(keyBits);
-    const uint32_t parts = kern::sortPartitions(count);
     const uint64_t words = kern::sortWorkspaceWords(count, keyBits);
     if (words > sortWorkWords_) {
         if (!(s = sortWork_.reserve(words * 4, "pair.sortWorkspace")).ok()) return s;
         sortWorkWords_ = words;
     }
-    uint32_t* work = sortWork_.as<uint32_t>();
-    kern::SortWorkspace ws{work, work + passes * 256, work + passes * 512, work + passes * 512 + uint64_t{passes} * parts * 256,
-                           counters_.as<uint32_t>() + 4};
+    const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), count, keyBits, counters_.as<uint32_t>() + 4);
     const bool alt = kern::sortPairs(keys, values, scratchKeys, scratchValues, count, kern::KeyMode::Plain, 0, keyBits,
                                      ws, stream_);
     if (resultInScratch != nullptr) *resultInScratch = alt;

@@ -175,7 +175,11 @@ __global__ __launch_bounds__(kBlock) void refProjectKernel(FrameParams f, const 
                           colour[0], colour[1], colour[2], 0.0};
     keys[slot] = keyed ? static_cast<uint64_t>(floatToBits(static_cast<float>(v[2])) >> 7) << 32
                        : static_cast<uint64_t>(__double_as_longlong(v[2]));
-    atomicAdd(live, 1u);
+    // One add a wave for its live lanes (the ones that reach here).
+    const uint64_t liveLanes = __ballot(1);
+    if (__lane_id() == static_cast<uint32_t>(__builtin_ctzll(liveLanes))) {
+        atomicAdd(live, static_cast<uint32_t>(__popcll(liveLanes)));
+    }
 }
 
 __device__ __forceinline__ double srgbToLinearF64(double c) {

@@ -1,8 +1,10 @@
 // Exclusive u32 prefix sum in one pass: each workgroup scans a 4096-element
 // partition in registers and LDS, publishes its aggregate, and looks back over
 // its predecessors' published aggregates/prefixes (decoupled look-back) for its
-// starting offset. Partitions are numbered by an atomic ticket, so a
-// workgroup only ever waits on partitions that were already running.
+// starting offset. Partitions are numbered by workgroup id: workgroups reach
+// each XCD in ascending id order, so the lowest unfinished partition is always
+// resident and every look-back ends (an atomic ticket would cost a
+// same-address device atomic, tens of nanoseconds across XCDs, a partition).
 //
 // Replaces the reference's three-dispatch chunked scan (modules/gpu/src/algo/
 // PrefixSum.cpp:58-88, shaders/lrt/algo/prefix_*.slang), whose chunk starts
@@ -21,7 +23,7 @@ constexpr uint32_t kPart = kBlock * kItems;
 constexpr uint64_t kAggregate = 1ull << 62;
 constexpr uint64_t kInclusive = 2ull << 62;
 constexpr uint64_t kValueMask = (1ull << 62) - 1;
-constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint32_t kSpinLimit = 1u << 18;
 
 __global__ void fill32Kernel(uint32_t* dst, uint32_t value, uint64_t count) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -41,18 +43,16 @@ __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t x, uint32_t lane)
 
 __global__ __launch_bounds__(kBlock) void scanKernel(const uint32_t* __restrict__ in, uint32_t* __restrict__ out,
                                                      uint32_t n, uint64_t* __restrict__ status,
-                                                     uint32_t* __restrict__ ticket, uint32_t* __restrict__ total,
-                                                     uint32_t* __restrict__ nonzero, uint32_t* __restrict__ error,
+                                                     uint32_t* __restrict__ total,
+                                                     uint32_t* __restrict__ nonzeroParts, uint32_t* __restrict__ error,
                                                      bool vectorised) {
-    __shared__ uint32_t sPart;
     __shared__ uint32_t sWave[kBlock / 64];
+    __shared__ uint32_t sNonzero[kBlock / 64];
     __shared__ uint32_t sPrefix;
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    if (tid == 0) sPart = atomicAdd(ticket, 1u);
-    __syncthreads();
-    const uint32_t part = sPart;
+    const uint32_t part = blockIdx.x;
     const uint64_t first = static_cast<uint64_t>(part) * kPart + static_cast<uint64_t>(tid) * kItems;
 
     uint32_t v[kItems];
@@ -78,7 +78,10 @@ __global__ __launch_bounds__(kBlock) void scanKernel(const uint32_t* __restrict_
     }
     const uint32_t inclusive = waveInclusiveScan(sum, lane);
     const uint32_t nzWave = waveInclusiveScan(nz, lane);
-    if (lane == 63) sWave[wave] = inclusive;
+    if (lane == 63) {
+        sWave[wave] = inclusive;
+        sNonzero[wave] = nzWave;
+    }
     __syncthreads();
     uint32_t waveBase = 0, blockTotal = 0;
 #pragma unroll
@@ -87,39 +90,59 @@ __global__ __launch_bounds__(kBlock) void scanKernel(const uint32_t* __restrict_
         if (w < wave) waveBase += t;
         blockTotal += t;
     }
-    if (lane == 63) atomicAdd(nonzero, nzWave);
+    if (tid == 0) nonzeroParts[part] = sNonzero[0] + sNonzero[1] + sNonzero[2] + sNonzero[3];
 
-    if (tid == 0) {
+    // Decoupled look-back by the first wave: publish this partition's
+    // aggregate, then read 64 predecessors' words at once (lane 0 the nearest;
+    // before partition 0 reads as an inclusive 0), wait until every word up to
+    // the nearest inclusive one is published, and add them; with no inclusive
+    // word among the 64, move the window back by 64.
+    if (wave == 0) {
         uint64_t* mine = status + part;
-        uint32_t prefix = 0;
+        uint64_t prefix = 0;   // 64-bit, so that a sum past 2^32 shows as such
         if (part == 0) {
-            __hip_atomic_store(mine, kInclusive | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) __hip_atomic_store(mine, kInclusive | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
         } else {
-            __hip_atomic_store(mine, kAggregate | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            int64_t p = static_cast<int64_t>(part) - 1;
+            if (lane == 0) __hip_atomic_store(mine, kAggregate | blockTotal, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            int64_t end = static_cast<int64_t>(part);   // the window is [end - 64, end)
             uint32_t spins = 0;
-            while (p >= 0) {
-                const uint64_t s = __hip_atomic_load(status + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            while (true) {
+                const int64_t p = end - 1 - static_cast<int64_t>(lane);
+                const uint64_t s = p >= 0 ? __hip_atomic_load(status + p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                          : kInclusive;
                 const uint64_t flag = s & ~kValueMask;
-                if (flag == 0) {
-                    // A predecessor that has its ticket is running; it publishes
-                    // within microseconds. The bound only guards a broken launch.
+                const uint64_t inclusive = __ballot(flag == kInclusive);
+                const uint64_t pending = __ballot(flag == 0);
+                const uint32_t nearest = inclusive != 0 ? static_cast<uint32_t>(__builtin_ctzll(inclusive)) : 64u;
+                const uint64_t needed = nearest >= 63 ? ~0ull : ((2ull << nearest) - 1ull);
+                if ((pending & needed) != 0) {
+                    // A predecessor (a lower workgroup id) is running or next on
+                    // its XCD. The bound only guards a broken launch.
                     if (++spins > kSpinLimit) {
-                        atomicOr(error, 1u);
+                        if (lane == 0) atomicOr(error, 1u);
                         break;
                     }
                     __builtin_amdgcn_s_sleep(1);
                     continue;
                 }
-                prefix += static_cast<uint32_t>(s & kValueMask);
-                if (flag == kInclusive) break;
-                --p;
+                uint64_t take = lane <= nearest ? (s & kValueMask) : 0ull;
+#pragma unroll
+                for (uint32_t d = 1; d < 64; d <<= 1) take += __shfl_xor(take, static_cast<int>(d), 64);
+                prefix += take;
+                if (nearest < 64) break;
+                end -= 64;
             }
-            __hip_atomic_store(mine, kInclusive | static_cast<uint32_t>(prefix + blockTotal), __ATOMIC_RELAXED,
-                               __HIP_MEMORY_SCOPE_AGENT);
+            if (lane == 0) {
+                __hip_atomic_store(mine, kInclusive | ((prefix + blockTotal) & kValueMask), __ATOMIC_RELAXED,
+                                   __HIP_MEMORY_SCOPE_AGENT);
+            }
         }
-        sPrefix = prefix;
-        if (static_cast<uint64_t>(part + 1) * kPart >= n) *total = prefix + blockTotal;
+        if (lane == 0) {
+            sPrefix = static_cast<uint32_t>(prefix);
+            // The total saturates at 2^32 - 1 rather than wrap.
+            const uint64_t all = prefix + blockTotal;
+            if (static_cast<uint64_t>(part + 1) * kPart >= n) *total = all > 0xFFFFFFFFull ? 0xFFFFFFFFu : static_cast<uint32_t>(all);
+        }
     }
     __syncthreads();
     uint32_t run = sPrefix + waveBase + inclusive - sum;
@@ -143,6 +166,23 @@ __global__ __launch_bounds__(kBlock) void scanKernel(const uint32_t* __restrict_
     }
 }
 
+// One workgroup: the partitions' non-zero counts added up.
+__global__ __launch_bounds__(1024) void sumPartsKernel(const uint32_t* __restrict__ parts, uint32_t count,
+                                                       uint32_t* __restrict__ out) {
+    __shared__ uint32_t sWave[1024 / 64];
+    uint32_t sum = 0;
+    for (uint32_t i = threadIdx.x; i < count; i += 1024) sum += parts[i];
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) sum += static_cast<uint32_t>(__shfl_xor(static_cast<int>(sum), static_cast<int>(d), 64));
+    if ((threadIdx.x & 63u) == 0) sWave[threadIdx.x >> 6] = sum;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        uint32_t all = 0;
+        for (uint32_t w = 0; w < 1024 / 64; ++w) all += sWave[w];
+        *out = all;
+    }
+}
+
 }   // namespace
 
 void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream) {
@@ -154,19 +194,21 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream) {
 
 uint32_t scanPartitions(uint32_t n) { return (n + kPart - 1) / kPart; }
 
-void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* ticket,
-                   uint32_t* total, uint32_t* nonzero, uint32_t* error, hipStream_t stream) {
+void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
+                   uint32_t* nonzero, uint32_t* error, hipStream_t stream) {
     const uint32_t parts = scanPartitions(n);
     fill32(reinterpret_cast<uint32_t*>(status), 0u, static_cast<uint64_t>(parts) * 2, stream);
-    fill32(ticket, 0u, 1, stream);
-    fill32(nonzero, 0u, 1, stream);
     if (n == 0) {
         fill32(total, 0u, 1, stream);
+        fill32(nonzero, 0u, 1, stream);
         return;
     }
+    // Each partition's non-zero count after its look-back words.
+    uint32_t* nonzeroParts = reinterpret_cast<uint32_t*>(status + parts);
     // 16-byte loads and stores only where both arrays allow them.
     const bool vectorised = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
-    scanKernel<<<parts, kBlock, 0, stream>>>(in, out, n, status, ticket, total, nonzero, error, vectorised);
+    scanKernel<<<parts, kBlock, 0, stream>>>(in, out, n, status, total, nonzeroParts, error, vectorised);
+    sumPartsKernel<<<1, 1024, 0, stream>>>(nonzeroParts, parts, nonzero);
 }
 
 }   // namespace lrt::kern

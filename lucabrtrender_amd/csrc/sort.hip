@@ -1,8 +1,8 @@
 // Stable LSD radix sort of (u64 key, u32 value) pairs: one histogram pass over
 // the keys for every digit at once, then one onesweep launch per 8-bit digit.
 //
-// A onesweep workgroup (256 threads, 4 wave64s) takes the next partition of
-// 3840 pairs by atomic ticket and
+// A onesweep workgroup (256 threads, 4 wave64s) takes partition blockIdx.x of
+// 3840 pairs and
 //   1. ranks its keys by digit within each wave with ballot-built match masks
 //      (eight __ballot per 64 keys), keeping a per-wave digit histogram in LDS;
 //   2. turns the four wave histograms into per-wave offsets and a block digit
@@ -12,7 +12,16 @@
 //   3. scatters its pairs into LDS in local digit order and writes each digit's
 //      run to global memory contiguously.
 // Ranking in (wave, key slot, lane) order, which is input order, and
-// partitions taken in input order make every pass stable, so the sort is.
+// partitions numbered in input order make every pass stable, so the sort is.
+//
+// Partitions are numbered by workgroup id, not by an atomic ticket: one
+// same-address device atomic costs tens of nanoseconds across the eight XCDs,
+// and a ticket per partition made that the floor of a pass. Workgroups are
+// dispatched to each XCD in ascending id order, so the lowest unfinished
+// partition is always resident and every look-back ends; the spin bound only
+// guards a broken launch. The digit histogram likewise writes one row per
+// workgroup, summed by the offsets kernel, rather than adding into one global
+// row with atomics.
 //
 // Replaces the reference's chunk-serial LSD radix (modules/gpu/src/algo/
 // RadixSort.cpp:80-158, shaders/lrt/algo/radix_*.slang) with the same result:
@@ -35,7 +44,11 @@ constexpr uint32_t kMaxPasses = 8;
 constexpr uint32_t kAggregate = 1u << 30;
 constexpr uint32_t kInclusive = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
-constexpr uint32_t kSpinLimit = 1u << 22;
+constexpr uint32_t kSpinLimit = 1u << 18;
+constexpr uint32_t kLookBack = 16;   // predecessor partitions read a look-back step
+constexpr uint32_t kHistBlock = 1024;
+constexpr uint32_t kHistRows = 256;     // histogram workgroups at most (one row of counts each)
+constexpr uint32_t kHistKeys = 8192;    // keys a histogram workgroup below that cap
 
 // The pairs to sort: the host's count, or the one a kernel left in device
 // memory (then the host's is the capacity and bounds it).
@@ -54,27 +67,39 @@ __device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64
     }
 }
 
+// Digit counts of every pass at once: workgroup b writes its row
+// hist[b * passes * 256 + pass * 256 + digit].
 template <KeyMode MODE>
-__global__ __launch_bounds__(kBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
-                                                          const uint32_t* __restrict__ nDevice, uint32_t bitLo,
-                                                          uint64_t mask, uint32_t passes,
-                                                          uint32_t* __restrict__ hist) {
+__global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
+                                                              const uint32_t* __restrict__ nDevice, uint32_t bitLo,
+                                                              uint64_t mask, uint32_t passes,
+                                                              uint32_t* __restrict__ hist) {
     __shared__ uint32_t sHist[kMaxPasses * kRadix];
     const uint32_t n = pairCount(cap, nDevice);
-    for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kBlock) sHist[j] = 0;
+    for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kHistBlock) sHist[j] = 0;
     __syncthreads();
-    const uint32_t stride = gridDim.x * kBlock;
-    for (uint32_t i = blockIdx.x * kBlock + threadIdx.x; i < n; i += stride) {
-        const uint64_t k = sortKey<MODE>(keys[i], bitLo, mask);
-        for (uint32_t p = 0; p < passes; ++p) {
-            atomicAdd(&sHist[p * kRadix + static_cast<uint32_t>((k >> (8 * p)) & 255u)], 1u);
+    const uint32_t stride = gridDim.x * kHistBlock;
+    // Four keys in flight a thread.
+    for (uint32_t i0 = blockIdx.x * kHistBlock + threadIdx.x; i0 < n; i0 += 4 * stride) {
+        uint64_t k[4];
+        bool ok[4];
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            const uint32_t i = i0 + u * stride;   // n < 2^30: no wrap
+            ok[u] = i < n;
+            k[u] = ok[u] ? sortKey<MODE>(keys[i], bitLo, mask) : 0ull;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 4; ++u) {
+            if (!ok[u]) continue;
+            for (uint32_t p = 0; p < passes; ++p) {
+                atomicAdd(&sHist[p * kRadix + static_cast<uint32_t>((k[u] >> (8 * p)) & 255u)], 1u);
+            }
         }
     }
     __syncthreads();
-    for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kBlock) {
-        const uint32_t c = sHist[j];
-        if (c != 0) atomicAdd(&hist[j], c);
-    }
+    uint32_t* row = hist + static_cast<uint64_t>(blockIdx.x) * passes * kRadix;
+    for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kHistBlock) row[j] = sHist[j];
 }
 
 // Clears the look-back words of the partitions the count needs, every pass.
@@ -100,18 +125,43 @@ __device__ __forceinline__ uint32_t waveInclusiveScan(uint32_t x, uint32_t lane)
     return x;
 }
 
-// One workgroup per pass: exclusive scan of the 256 digit counts.
-__global__ __launch_bounds__(kRadix) void digitOffsetsKernel(const uint32_t* __restrict__ hist,
-                                                             uint32_t* __restrict__ offsets) {
+// One workgroup per pass: the histogram rows summed (four row groups of 256
+// threads, eight rows in flight a thread), then the exclusive scan of the 256
+// digit counts.
+__global__ __launch_bounds__(1024) void digitOffsetsKernel(const uint32_t* __restrict__ hist, uint32_t rows,
+                                                           uint32_t passes, uint32_t* __restrict__ offsets) {
+    __shared__ uint32_t sGroup[4][kRadix];
     __shared__ uint32_t sWave[kRadix / 64];
-    const uint32_t t = threadIdx.x, lane = t & 63u, wave = t >> 6;
-    const uint32_t c = hist[blockIdx.x * kRadix + t];
-    const uint32_t incl = waveInclusiveScan(c, lane);
-    if (lane == 63) sWave[wave] = incl;
+    const uint32_t t = threadIdx.x, digit = t & 255u, group = t >> 8;
+    const uint64_t rowWords = static_cast<uint64_t>(passes) * kRadix;
+    const uint32_t* col = hist + static_cast<uint64_t>(blockIdx.x) * kRadix + digit;
+    uint32_t sum = 0;
+    for (uint32_t r0 = group; r0 < rows; r0 += 32) {
+        uint32_t part[8];
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) {
+            const uint32_t r = r0 + 4 * u;
+            part[u] = r < rows ? col[r * rowWords] : 0u;
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < 8; ++u) sum += part[u];
+    }
+    sGroup[group][digit] = sum;
     __syncthreads();
-    uint32_t base = 0;
-    for (uint32_t w = 0; w < wave; ++w) base += sWave[w];
-    offsets[blockIdx.x * kRadix + t] = base + incl - c;
+    // Threads 0-255 (whole waves 0-3) scan; every thread reaches each barrier.
+    const uint32_t lane = t & 63u, wave = t >> 6;
+    uint32_t c = 0, incl = 0;
+    if (t < kRadix) {
+        c = sGroup[0][t] + sGroup[1][t] + sGroup[2][t] + sGroup[3][t];
+        incl = waveInclusiveScan(c, lane);
+        if (lane == 63) sWave[wave] = incl;
+    }
+    __syncthreads();
+    if (t < kRadix) {
+        uint32_t base = 0;
+        for (uint32_t w = 0; w < wave; ++w) base += sWave[w];
+        offsets[blockIdx.x * kRadix + t] = base + incl - c;
+    }
 }
 
 template <KeyMode MODE>
@@ -123,7 +173,6 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
                                                          uint32_t bitLo, uint64_t mask, uint32_t shift,
                                                          const uint32_t* __restrict__ digitOffsets,
                                                          uint32_t* __restrict__ status,
-                                                         uint32_t* __restrict__ ticket,
                                                          uint32_t* __restrict__ error) {
     __shared__ uint64_t sKeys[kPart];
     __shared__ uint32_t sVals[kPart];
@@ -131,17 +180,15 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
     __shared__ uint32_t sLocal[kRadix];
     __shared__ uint32_t sGlobal[kRadix];
     __shared__ uint32_t sScan[kWaves];
-    __shared__ uint32_t sPart;
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    if (tid == 0) sPart = atomicAdd(ticket, 1u);
 #pragma unroll
     for (uint32_t w = 0; w < kWaves; ++w) sWave[w][tid] = 0;
     __syncthreads();
     const uint32_t n = pairCount(cap, nDevice);
-    const uint32_t part = sPart;
+    const uint32_t part = blockIdx.x;
     const uint32_t base = part * kPart;
     // Launched for the capacity: partitions past the count have nothing to
     // do, and no partition before them waits on them.
@@ -200,25 +247,46 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
     if (lane == 63) sScan[wave] = incl;
     uint32_t prefix = 0;
     if (part > 0) {
+        // Look back kLookBack partitions a step: the window's loads are in
+        // flight together, so a walk of L partitions costs about L / kLookBack
+        // round trips to L2 instead of L. Partition 0 is inclusive from the
+        // start, so no walk reaches below it.
         int64_t p = static_cast<int64_t>(part) - 1;
         uint32_t spins = 0;
-        while (p >= 0) {
-            const uint32_t s = __hip_atomic_load(status + static_cast<uint64_t>(p) * kRadix + tid, __ATOMIC_RELAXED,
-                                                 __HIP_MEMORY_SCOPE_AGENT);
-            const uint32_t flag = s & ~kValueMask;
-            if (flag == 0) {
-                // The partition waited on took its ticket earlier and is
-                // running; the bound only guards a broken launch.
+        while (true) {
+            uint32_t s[kLookBack];
+#pragma unroll
+            for (uint32_t j = 0; j < kLookBack; ++j) {
+                s[j] = p - j >= 0 ? __hip_atomic_load(status + static_cast<uint64_t>(p - j) * kRadix + tid,
+                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
+                                  : kInclusive;
+            }
+            uint32_t taken = 0;
+            bool open = true, done = false;
+#pragma unroll
+            for (uint32_t j = 0; j < kLookBack; ++j) {
+                const uint32_t flag = s[j] & ~kValueMask;
+                if (open && flag == 0) open = false;   // not yet published: wait for it
+                if (open) {
+                    prefix += s[j] & kValueMask;
+                    taken += 1;
+                    if (flag == kInclusive) {
+                        done = true;
+                        open = false;
+                    }
+                }
+            }
+            if (done) break;
+            p -= taken;
+            if (taken == 0) {
+                // The partition waited on has a lower workgroup id, so it is
+                // running or next on its XCD; the bound only guards a broken launch.
                 if (++spins > kSpinLimit) {
                     atomicOr(error, 1u);
                     break;
                 }
                 __builtin_amdgcn_s_sleep(1);
-                continue;
             }
-            prefix += s & kValueMask;
-            if (flag == kInclusive) break;
-            --p;
         }
         __hip_atomic_store(mine, kInclusive | (prefix + total), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
     }
@@ -258,6 +326,12 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
     }
 }
 
+// Histogram workgroups for capacity n: one per kHistKeys keys, at most kHistRows.
+uint32_t histogramRows(uint32_t n) {
+    const uint32_t rows = (n + kHistKeys - 1) / kHistKeys;
+    return std::clamp(rows, 1u, kHistRows);
+}
+
 template <KeyMode MODE>
 void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
                   const uint32_t* nDevice, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws,
@@ -265,7 +339,6 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     const uint32_t passes = sortPasses(keyBits);
     const uint32_t parts = sortPartitions(n);
     const uint64_t mask = keyBits >= 64 ? ~0ull : ((1ull << keyBits) - 1ull);
-    fill32(ws.histogram, 0u, static_cast<uint64_t>(passes) * kRadix, stream);
     if (nDevice == nullptr) {
         fill32(ws.status, 0u, static_cast<uint64_t>(passes) * parts * kRadix, stream);
     } else {
@@ -273,19 +346,19 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
         const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((words + kBlock - 1) / kBlock, 4096));
         clearStatusKernel<<<grid, kBlock, 0, stream>>>(ws.status, passes, parts, n, nDevice);
     }
-    fill32(ws.tickets, 0u, passes, stream);
-    const uint32_t histBlocks = parts < 1024 ? parts : 1024;
-    histogramKernel<MODE><<<histBlocks, kBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, passes, ws.histogram);
-    digitOffsetsKernel<<<passes, kRadix, 0, stream>>>(ws.histogram, ws.offsets);
+    const uint32_t rows = histogramRows(n);
+    histogramKernel<MODE><<<rows, kHistBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, passes, ws.histogram);
+    digitOffsetsKernel<<<passes, 1024, 0, stream>>>(ws.histogram, rows, passes, ws.offsets);
     if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
     const uint64_t* kin = keys;
     const uint32_t* vin = values;
     uint64_t* kout = keysAlt;
     uint32_t* vout = valuesAlt;
     for (uint32_t p = 0; p < passes; ++p) {
-        onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(
-            kin, vin, kout, vout, n, nDevice, bitLo, mask, 8 * p, ws.offsets + p * kRadix,
-            ws.status + static_cast<uint64_t>(p) * parts * kRadix, ws.tickets + p, ws.error);
+        onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(kin, vin, kout, vout, n, nDevice, bitLo, mask, 8 * p,
+                                                           ws.offsets + p * kRadix,
+                                                           ws.status + static_cast<uint64_t>(p) * parts * kRadix,
+                                                           ws.error);
         const uint64_t* nk = kout;
         const uint32_t* nv = vout;
         kout = const_cast<uint64_t*>(kin);
@@ -301,7 +374,13 @@ uint32_t sortPartitions(uint32_t n) { return (n + kPart - 1) / kPart; }
 uint32_t sortPasses(uint32_t keyBits) { return (keyBits + 7) / 8; }
 uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits) {
     const uint64_t passes = sortPasses(keyBits);
-    return passes * kRadix * 2 + passes * sortPartitions(n) * kRadix + passes + 1;
+    return uint64_t{histogramRows(n)} * passes * kRadix + passes * kRadix + passes * sortPartitions(n) * kRadix;
+}
+
+SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32_t* error) {
+    const uint64_t passes = sortPasses(keyBits);
+    const uint64_t histWords = uint64_t{histogramRows(n)} * passes * kRadix;
+    return SortWorkspace{work, work + histWords, work + histWords + passes * kRadix, error};
 }
 
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n, KeyMode mode,
