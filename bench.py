@@ -293,6 +293,62 @@ def sort_stress_leg(torch, raster, scenes, _abi, reps):
             "gpu_ms_per_frame": round(float(ms[7]), 4), "wall_ms_per_frame": round(wall * 1e3, 4)}
 
 
+def load_leg(torch, raster, reps, cpu_sample):
+    """The load path: 3,000,000 3DGS-PLY records at degree 3 (62 floats, 248 B a
+    record) decoded into packed splats on the GPU. The decode's kernels are
+    timed with HIP events (records already on the device); the PCIe upload is
+    reported beside it. cpu_sample records go through the oracle's decode on one
+    host core (0 skips it)."""
+    import ctypes as C
+
+    import numpy as np
+    n, floats = 3_000_000, 62
+    rng = np.random.default_rng(11)
+    rec = np.empty((n, floats), np.float32)
+    rec[:, 0:3] = rng.uniform(-10, 10, (n, 3))
+    rec[:, 3:51] = rng.normal(0.0, 0.3, (n, 48))              # dc + 45 rest, colour-outer
+    rec[:, 51] = rng.normal(0.0, 2.0, n)                      # opacity logits: some below 1/255
+    rec[:, 52:55] = rng.normal(-5.0, 0.8, (n, 3))             # log scales
+    rec[:, 55:59] = rng.normal(0.0, 1.0, (n, 4))              # rotation w x y z
+    rec[:, 59:62] = 0.0                                       # normals, ignored
+    enc = dict(floats_per_record=floats, x=0, y=1, z=2, dc0=3, dc1=4, dc2=5, rest_base=6, rest_per_colour=15,
+               file_per_colour=15, rest_colour_outer=1, opacity=51, scale0=52, scale1=53, scale2=54, rot_w=55,
+               rot_x=56, rot_y=57, rot_z=58)
+    raw = raster.raw_splats(rec, enc)
+    rast = raster.TileRasterizer()
+    rast.decode_cloud(raw, 3)   # warm-up: buffers sized
+    up = dec = 0.0
+    for _ in range(reps):
+        _, info = rast.decode_cloud(raw, 3)
+        up += info.upload_ms
+        dec += info.decode_ms
+    up, dec = up / reps, dec / reps
+    kept = info.count
+    # Algorithmic bytes: the validate pass's eight fields, the decode's whole
+    # record, the valid flags and places (scan), the packed splat out.
+    algo = n * (8 * 4 + floats * 4 + 4 * 4) + kept * (32 + 4 * info.sh_words)
+    out = {"workload": "decode-3M-ply-sh3", "records": n, "kept": kept, "record_bytes": floats * 4,
+           "decode_ms": round(dec, 4), "upload_ms": round(up, 4), "bytes": int(algo),
+           "GB/s": round(algo / (dec * 1e-3) / 1e9, 1), "hbm_frac": round(algo / (dec * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+           "records_per_s": round(n / (dec * 1e-3), 1),
+           "upload_GB/s": round(n * floats * 4 / (up * 1e-3) / 1e9, 1)}
+    if cpu_sample > 0:
+        sys.path.insert(0, os.path.join(ROOT, "tests"))
+        import oracle_lib as O
+        k = min(cpu_sample, n)
+        e = O.OrEncoding(**{f: v for f, v in enc.items() if f != "rest_per_colour"})
+        sample = np.ascontiguousarray(rec[:k])
+        pos = np.zeros((k, 4), np.float32)
+        shape = np.zeros((k, 4), np.uint32)
+        sh = np.zeros((k, 23), np.uint32)
+        start = time.perf_counter()
+        O.lib.or_decode_records(O.ptr(sample), k, C.byref(e), 15, O.ptr(pos), O.ptr(shape), O.ptr(sh), 23)
+        sec = time.perf_counter() - start
+        out["cpu_baseline"] = {"value": round(k / sec, 1), "unit": "records/s", "cores": 1, "kind": "port",
+                               "sample": f"the oracle's validate + decode of the first {k} records"}
+    return out
+
+
 def main():
     ap = argparse.ArgumentParser(description=__doc__.split("\n\n")[0])
     ap.add_argument("--gpus", type=int, default=1)
@@ -304,6 +360,7 @@ def main():
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="config-3 fwd+bwd views timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")
+    ap.add_argument("--load-reps", type=int, default=3, help="3M-record decodes timed; 0 skips it")
     ap.add_argument("--profile-frames", type=int, default=0,
                     help="only draw this many config-2 frames and one 256 MiB device copy (the counter "
                          "calibration), for rocprofv3 --pmc; prints no JSON")
@@ -404,6 +461,11 @@ def main():
         log("sort stress (6M splats, 2160p) ...")
         stress = sort_stress_leg(torch, raster, scenes, _abi, args.stress_reps)
 
+    load = None
+    if args.load_reps > 0:
+        log("load path (3M PLY records) ...")
+        load = load_leg(torch, raster, args.load_reps, 300_000 if args.cpu_seconds > 0 else 0)
+
     cpu = None
     if args.cpu_seconds > 0:
         log("CPU baseline ...")
@@ -433,6 +495,7 @@ def main():
         "fwd_bwd": fwd_bwd,
         "views": batched,
         "sort_stress": stress,
+        "load": load,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

@@ -170,7 +170,8 @@ lrt_status lrt_context_create(int device, void* stream, lrt_context** out);
 void lrt_context_destroy(lrt_context* ctx);
 lrt_status lrt_context_set_stream(lrt_context* ctx, void* stream);
 /* The message of the context's last failure; with NULL, that of the calling
- * thread's last failed lrt_context_create. */
+ * thread's last call that takes no context (lrt_context_create,
+ * lrt_splats_read, lrt_raw_splats_create), empty when it succeeded. */
 const char* lrt_last_error(const lrt_context* ctx);
 
 /* TileRasterizer::render(const Projection&, ...) (TileRasterizer.h:143-155,
@@ -307,6 +308,67 @@ lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, ui
 
 /* Waits for everything queued on the context's stream. */
 lrt_status lrt_context_synchronize(lrt_context* ctx);
+
+/* ---- splat files --------------------------------------------------------------
+ * io::RawSplats and io::SplatEncoding (modules/io/include/lrt/io/RawSplats.h:
+ * 15-80): a file as one float a property a record, in the file's order, and
+ * where each field sits. Modes are numbered as RawSplats.h:42-58 numbers them:
+ *   opacity_mode   0 logit, 1 linear, 2 byte (v / 255)
+ *   scale_mode     0 log, 1 linear (log(max(v, 1e-30)))
+ *   colour_mode    0 SH DC (0.5 + C0 v), 1 linear, 2 byte (v / 255)
+ *   rotation_mode  0 float (w, x, y, z), 1 byte ((v - 128) / 128 each)
+ *   rest_mode      0 float, 1 byte ((v - 128) / 128)
+ * rest_colour_outer 1: rest coefficients rrr.. ggg.. bbb.. with
+ * file_per_colour a colour (a 3DGS PLY); 0: rgb per basis. */
+typedef struct lrt_raw_splats lrt_raw_splats;
+typedef struct {
+    uint32_t floats_per_record;
+    uint32_t x, y, z, opacity, scale0, scale1, scale2, rot_w, rot_x, rot_y, rot_z, dc0, dc1, dc2;
+    uint32_t rest_base, rest_per_colour, file_per_colour, rest_colour_outer;
+    uint32_t opacity_mode, scale_mode, colour_mode, rotation_mode, rest_mode;
+} lrt_splat_encoding;
+
+/* io::readSplats (modules/io/src/SplatReaders.cpp:32-195) by extension: a
+ * binary little-endian 3DGS PLY or a .splat file, read on the host. .spz and
+ * .sog are refused (LRT_UNSUPPORTED): this build has neither zstd nor an image
+ * decoder. The reference's codes and messages otherwise: not a PLY, no
+ * vertices, a missing 3DGS property -> LRT_INVALID_ARGUMENT; ASCII or
+ * big-endian, a list in the vertex element, harmonics not contiguous or not a
+ * whole degree -> LRT_UNSUPPORTED; a file shorter than its vertices ->
+ * LRT_IO_FAILURE. The message: lrt_last_error(NULL). */
+lrt_status lrt_splats_read(const char* path, lrt_raw_splats** out);
+/* Raw records from memory (another reader's output): count * floats_per_record floats, copied. */
+lrt_status lrt_raw_splats_create(const float* records, uint32_t count, const lrt_splat_encoding* encoding,
+                                 lrt_raw_splats** out);
+void lrt_raw_splats_destroy(lrt_raw_splats* raw);
+uint32_t lrt_raw_splats_count(const lrt_raw_splats* raw);
+lrt_status lrt_raw_splats_encoding(const lrt_raw_splats* raw, lrt_splat_encoding* out);
+/* The records, host memory owned by raw: count * floats_per_record floats. */
+const float* lrt_raw_splats_records(const lrt_raw_splats* raw);
+
+/* What a decode made: records declared and kept, the rest basis functions kept
+ * a colour, the packed row's words, and the kept positions' box
+ * (scene::GpuSplats count/declared/bounds, GpuClouds.h:41-76); and the GPU
+ * milliseconds of the records' copies to the device and of the decode's kernels. */
+typedef struct {
+    uint32_t declared, count, rest_per_colour, sh_words;
+    float bounds_min[3], bounds_max[3];
+    float upload_ms, decode_ms;
+} lrt_cloud_info;
+
+/* The sh_words a decode of raw at max_degree packs a splat. */
+uint32_t lrt_cloud_sh_words(const lrt_raw_splats* raw, uint32_t max_degree);
+
+/* scene::CloudLoader::upload (modules/scene/src/GpuClouds.cpp:151-227) with
+ * splat_validate.slang and splat_decode.slang: records kept when the position
+ * and log scales are finite, |log scale| < 60 and opacity >= 1/255, packed in
+ * file order into the lrt_splats layout at caller device buffers sized for the
+ * declared count (positions 16 B, shape 16 B, sh 4 * sh_words B a splat; sh
+ * may be NULL when no rest coefficient is kept). Rest coefficients are kept up
+ * to min(the file's degree, max_degree). LRT_INVALID_ARGUMENT when no record
+ * survives. Synchronous; the context's last frame's debug buffers are lost. */
+lrt_status lrt_cloud_decode(lrt_context* ctx, const lrt_raw_splats* raw, uint32_t max_degree, float* positions,
+                            uint32_t* shape, uint32_t* sh, lrt_cloud_info* info);
 
 #ifdef __cplusplus
 }

@@ -133,6 +133,21 @@ class SceneSpecC(C.Structure):
 
 
 # Every symbol include/lrt_hip.h and include/lrt_synth.h declare, with its signature.
+ENCODING_FIELDS = ("floats_per_record", "x", "y", "z", "opacity", "scale0", "scale1", "scale2", "rot_w", "rot_x",
+                   "rot_y", "rot_z", "dc0", "dc1", "dc2", "rest_base", "rest_per_colour", "file_per_colour",
+                   "rest_colour_outer", "opacity_mode", "scale_mode", "colour_mode", "rotation_mode", "rest_mode")
+
+
+class SplatEncodingC(C.Structure):
+    _fields_ = [(n, C.c_uint32) for n in ENCODING_FIELDS]
+
+
+class CloudInfoC(C.Structure):
+    _fields_ = [("declared", C.c_uint32), ("count", C.c_uint32), ("rest_per_colour", C.c_uint32),
+                ("sh_words", C.c_uint32), ("bounds_min", C.c_float * 3), ("bounds_max", C.c_float * 3),
+                ("upload_ms", C.c_float), ("decode_ms", C.c_float)]
+
+
 SIGNATURES = {
     "lrt_lens_default": (None, [C.POINTER(Lens)]),
     "lrt_camera_looking_at": (None, [C.POINTER(C.c_double), C.POINTER(C.c_double), C.POINTER(C.c_double),
@@ -163,6 +178,15 @@ SIGNATURES = {
     "lrt_radix_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.POINTER(C.c_int32)]),
     "lrt_context_synchronize": (C.c_int, [C.c_void_p]),
+    "lrt_splats_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
+    "lrt_raw_splats_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(SplatEncodingC), C.POINTER(C.c_void_p)]),
+    "lrt_raw_splats_destroy": (None, [C.c_void_p]),
+    "lrt_raw_splats_count": (C.c_uint32, [C.c_void_p]),
+    "lrt_raw_splats_encoding": (C.c_int, [C.c_void_p, C.POINTER(SplatEncodingC)]),
+    "lrt_raw_splats_records": (C.POINTER(C.c_float), [C.c_void_p]),
+    "lrt_cloud_sh_words": (C.c_uint32, [C.c_void_p, C.c_uint32]),
+    "lrt_cloud_decode": (C.c_int, [C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p,
+                                   C.POINTER(CloudInfoC)]),
     "lrt_synth_scene": (C.c_int, [C.POINTER(SceneSpecC), C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
     "lrt_pack_splats": (C.c_int, [C.c_void_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_void_p]),
 }

@@ -14,6 +14,10 @@ struct lrt_context {
     std::string error;
 };
 
+struct lrt_raw_splats {
+    lrt::io::RawSplats raw;
+};
+
 namespace {
 
 // The message of the last failure without a context (lrt_context_create).
@@ -140,6 +144,54 @@ void toStats(const lrt::FrameStats& fs, lrt_frame_stats* stats) {
     stats->total_ms = fs.totalMs;
 }
 
+lrt_splat_encoding toEncodingC(const lrt::io::SplatEncoding& e) {
+    lrt_splat_encoding c{};
+    c.floats_per_record = e.floatsPerRecord;
+    c.x = e.x; c.y = e.y; c.z = e.z; c.opacity = e.opacity;
+    c.scale0 = e.scale0; c.scale1 = e.scale1; c.scale2 = e.scale2;
+    c.rot_w = e.rotW; c.rot_x = e.rotX; c.rot_y = e.rotY; c.rot_z = e.rotZ;
+    c.dc0 = e.dc0; c.dc1 = e.dc1; c.dc2 = e.dc2;
+    c.rest_base = e.restBase; c.rest_per_colour = e.restPerColour; c.file_per_colour = e.filePerColour;
+    c.rest_colour_outer = e.restColourOuter;
+    c.opacity_mode = static_cast<uint32_t>(e.opacityMode);
+    c.scale_mode = static_cast<uint32_t>(e.scaleMode);
+    c.colour_mode = static_cast<uint32_t>(e.colourMode);
+    c.rotation_mode = static_cast<uint32_t>(e.rotationMode);
+    c.rest_mode = static_cast<uint32_t>(e.restMode);
+    return c;
+}
+
+// The C encoding checked and converted: modes in range, rest counts one of 0, 3, 8, 15.
+lrt::Status fromEncodingC(const lrt_splat_encoding& c, lrt::io::SplatEncoding& e) {
+    using lrt::ErrorCode;
+    using lrt::Status;
+    using E = lrt::io::SplatEncoding;
+    if (c.opacity_mode > 2 || c.scale_mode > 1 || c.colour_mode > 2 || c.rotation_mode > 1 || c.rest_mode > 1) {
+        return Status::error(ErrorCode::InvalidArgument, "an encoding mode this build does not decode");
+    }
+    if (c.rest_per_colour != 0 && c.rest_per_colour != 3 && c.rest_per_colour != 8 && c.rest_per_colour != 15) {
+        return Status::error(ErrorCode::InvalidArgument, "rest_per_colour must be 0, 3, 8 or 15");
+    }
+    e.floatsPerRecord = c.floats_per_record;
+    e.x = c.x; e.y = c.y; e.z = c.z; e.opacity = c.opacity;
+    e.scale0 = c.scale0; e.scale1 = c.scale1; e.scale2 = c.scale2;
+    e.rotW = c.rot_w; e.rotX = c.rot_x; e.rotY = c.rot_y; e.rotZ = c.rot_z;
+    e.dc0 = c.dc0; e.dc1 = c.dc1; e.dc2 = c.dc2;
+    e.restBase = c.rest_base; e.restPerColour = c.rest_per_colour; e.filePerColour = c.file_per_colour;
+    e.restColourOuter = c.rest_colour_outer != 0 ? 1u : 0u;
+    e.opacityMode = static_cast<E::Opacity>(c.opacity_mode);
+    e.scaleMode = static_cast<E::Scale>(c.scale_mode);
+    e.colourMode = static_cast<E::Colour>(c.colour_mode);
+    e.rotationMode = static_cast<E::Rotation>(c.rotation_mode);
+    e.restMode = static_cast<E::Rest>(c.rest_mode);
+    return {};
+}
+
+lrt_status noContext(const lrt::Status& s) {
+    gNoContextError = s.ok() ? std::string() : s.message;
+    return static_cast<lrt_status>(static_cast<int>(s.code));
+}
+
 }   // namespace
 
 extern "C" {
@@ -246,13 +298,11 @@ lrt_status lrt_context_create(int device, void* stream, lrt_context** out) {
     if (ctx == nullptr) return LRT_OUT_OF_MEMORY;
     const lrt::Status s = ctx->raster.create(device, static_cast<hipStream_t>(stream));
     if (!s.ok()) {
-        gNoContextError = s.message;
-        const lrt_status code = static_cast<lrt_status>(static_cast<int>(s.code));
         delete ctx;
-        return code;
+        return noContext(s);
     }
     *out = ctx;
-    return LRT_OK;
+    return noContext({});
 }
 
 void lrt_context_destroy(lrt_context* ctx) { delete ctx; }
@@ -448,6 +498,86 @@ lrt_status lrt_context_synchronize(lrt_context* ctx) {
     const hipError_t e = hipStreamSynchronize(ctx->raster.stream());
     return finish(ctx, e == hipSuccess ? lrt::Status{}
                                        : lrt::Status::error(lrt::ErrorCode::DeviceFailure, hipGetErrorString(e)));
+}
+
+
+lrt_status lrt_splats_read(const char* path, lrt_raw_splats** out) {
+    if (out == nullptr) return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null output"));
+    *out = nullptr;
+    if (path == nullptr) return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null path"));
+    auto* raw = new (std::nothrow) lrt_raw_splats;
+    if (raw == nullptr) return noContext(lrt::Status::error(lrt::ErrorCode::OutOfMemory, "raw splats"));
+    const lrt::Status s = lrt::io::readSplats(path, raw->raw);
+    if (!s.ok()) {
+        delete raw;
+        return noContext(s);
+    }
+    *out = raw;
+    return noContext({});
+}
+
+lrt_status lrt_raw_splats_create(const float* records, uint32_t count, const lrt_splat_encoding* encoding,
+                                 lrt_raw_splats** out) {
+    using lrt::ErrorCode;
+    using lrt::Status;
+    if (out == nullptr) return noContext(Status::error(ErrorCode::InvalidArgument, "null output"));
+    *out = nullptr;
+    if (encoding == nullptr || (count > 0 && records == nullptr)) {
+        return noContext(Status::error(ErrorCode::InvalidArgument, "null records or encoding"));
+    }
+    auto* raw = new (std::nothrow) lrt_raw_splats;
+    if (raw == nullptr) return noContext(Status::error(ErrorCode::OutOfMemory, "raw splats"));
+    const Status s = fromEncodingC(*encoding, raw->raw.encoding);
+    if (!s.ok()) {
+        delete raw;
+        return noContext(s);
+    }
+    raw->raw.source = "records";
+    raw->raw.count = count;
+    raw->raw.records.assign(records, records + uint64_t{count} * encoding->floats_per_record);
+    *out = raw;
+    return noContext({});
+}
+
+void lrt_raw_splats_destroy(lrt_raw_splats* raw) { delete raw; }
+
+uint32_t lrt_raw_splats_count(const lrt_raw_splats* raw) { return raw != nullptr ? raw->raw.count : 0u; }
+
+lrt_status lrt_raw_splats_encoding(const lrt_raw_splats* raw, lrt_splat_encoding* out) {
+    if (raw == nullptr || out == nullptr) return LRT_INVALID_ARGUMENT;
+    *out = toEncodingC(raw->raw.encoding);
+    return LRT_OK;
+}
+
+const float* lrt_raw_splats_records(const lrt_raw_splats* raw) {
+    return raw != nullptr ? raw->raw.records.data() : nullptr;
+}
+
+uint32_t lrt_cloud_sh_words(const lrt_raw_splats* raw, uint32_t max_degree) {
+    if (raw == nullptr) return 1u;
+    const uint32_t keep = lrt::TileRasterizer::keptPerColour(raw->raw.encoding, max_degree);
+    return keep == 0 ? 1u : (keep * 3 + 1) / 2;
+}
+
+lrt_status lrt_cloud_decode(lrt_context* ctx, const lrt_raw_splats* raw, uint32_t max_degree, float* positions,
+                            uint32_t* shape, uint32_t* sh, lrt_cloud_info* info) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (raw == nullptr) return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null raw splats"));
+    lrt::TileRasterizer::CloudInfo out;
+    const lrt::Status s = ctx->raster.decodeCloud(raw->raw, max_degree, positions, shape, sh, &out);
+    if (s.ok() && info != nullptr) {
+        info->declared = out.declared;
+        info->count = out.count;
+        info->rest_per_colour = out.restPerColour;
+        info->sh_words = out.shWords;
+        for (int k = 0; k < 3; ++k) {
+            info->bounds_min[k] = out.boundsMin[k];
+            info->bounds_max[k] = out.boundsMax[k];
+        }
+        info->upload_ms = out.uploadMs;
+        info->decode_ms = out.decodeMs;
+    }
+    return finish(ctx, s);
 }
 
 }   // extern "C"

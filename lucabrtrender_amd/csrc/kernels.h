@@ -81,6 +81,27 @@ bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* va
                KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
                hipEvent_t afterHistogram = nullptr, const uint32_t* nDevice = nullptr);
 
+// ---- load: raw splat records into packed splats --------------------------------
+// The decode's view of an io::SplatEncoding: field offsets in floats, modes
+// numbered as the encoding numbers them, the rest basis functions kept and the
+// packed row's words.
+struct DecodeParams {
+    uint32_t count, stride, keep, shWords;
+    uint32_t x, y, z, opacity, scale0, scale1, scale2, rotW, rotX, rotY, rotZ, dc0, dc1, dc2;
+    uint32_t restBase, filePerColour, restColourOuter;
+    uint32_t opacityMode, scaleMode, colourMode, rotationMode, restMode;
+};
+// Validates records [0, p.count) of `raw` (device), scans the flags into
+// `dest` (kept count into *kept) and packs each kept record to base + its
+// place. valid/dest hold p.count u32, scanStatus scanStatusWords(p.count) u64.
+void decodeSplats(const float* raw, const DecodeParams& p, uint32_t* valid, uint32_t* dest, uint64_t* scanStatus,
+                  uint32_t* kept, uint32_t* nonzero, uint32_t* scanError, uint32_t base, float4* positions,
+                  uint4* shape, uint32_t* sh, hipStream_t stream);
+// box[0..2] = min x, y, z and box[3..5] = max over positions [0, count);
+// partial holds boundsPartials(count) * 6 floats.
+uint32_t boundsPartials(uint32_t count);
+void splatBounds(const float4* positions, uint32_t count, float* partial, float* box, hipStream_t stream);
+
 // ---- ranges -----------------------------------------------------------------
 // ranges[t] = [start, end) of tile t's run in the sorted keys, (0, 0) when empty.
 // With nDevice non-null the count is *nDevice and `pairs` its capacity.

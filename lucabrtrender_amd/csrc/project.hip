@@ -5,10 +5,12 @@
 // (shaders/lrt/splat/splat_project.slang:52-221; relighting, which is off in
 // every frame this path draws, is not carried).
 //
-// Bandwidth: reads 32 B of packed splat (+ 4 B filter, + 4 * shWords B of
-// harmonics at degree > 0) and writes 48 + 8 + 4 B per splat. A workgroup
-// first stages its 256 splats' harmonic words through LDS with coalesced
-// loads, so the per-thread 92-byte rows cost whole cache lines once.
+// Bandwidth: reads 32 B of packed splat (+ 4 B filter) per splat, and
+// 4 * shWords B of harmonics (degree > 0) per drawn splat; writes 8 + 4 B per
+// splat and 48 B per drawn one. A workgroup first finds which of its 256 splats
+// are drawn (the footprint needs no harmonics), then stages only those splats'
+// harmonic rows through LDS with coalesced loads, so the per-thread 92-byte
+// rows cost whole cache lines once and culled splats cost none.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -23,62 +25,106 @@ using namespace lrt::dev;
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kMaxShWords = 23;   // 45 halves at degree 3, rgb per basis
 
+// The coefficient words of the drawn splats, staged: row r of sSh holds the
+// words of the workgroup's r-th drawn splat (in splat order).
+__device__ __forceinline__ void stageDrawnRows(const uint32_t* __restrict__ sh, uint32_t first, uint32_t words,
+                                               uint32_t rows, const uint32_t* sRowOf, uint32_t* sSh) {
+    if (words == 0) return;
+    const uint32_t total = rows * words;
+    // Word w = tid + k * kBlock is word w % words of row w / words; both are
+    // stepped, not divided, per k.
+    uint32_t row = threadIdx.x / words, col = threadIdx.x - row * words;
+    const uint32_t rowStep = kBlock / words, colStep = kBlock - rowStep * words;
+    for (uint32_t w = threadIdx.x; w < total; w += kBlock) {
+        sSh[w] = sh[static_cast<uint64_t>(first + sRowOf[row]) * words + col];
+        row += rowStep;
+        col += colStep;
+        if (col >= words) {
+            col -= words;
+            row += 1;
+        }
+    }
+}
+
 // One workgroup's 256 splats of one instance, from the frame constants f.
 __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4* __restrict__ positions,
                                              const uint4* __restrict__ shape, const uint32_t* __restrict__ sh,
                                              const float* __restrict__ filter3d, float4* __restrict__ proj,
                                              uint2* __restrict__ rects, uint32_t* __restrict__ touchedOut) {
     __shared__ uint32_t sSh[kBlock * kMaxShWords];
+    __shared__ uint32_t sRowOf[kBlock];
+    __shared__ uint32_t sDrawn[kBlock / 64];
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & 63u, wave = tid >> 6;
     const uint32_t first = blockIdx.x * kBlock;
     const uint32_t i = first + tid;
+    const bool live = i < f.count;
+    const uint32_t slot = f.base + i;
     const uint32_t degree = degreeOf(f.restPerColour) < f.shLimit ? degreeOf(f.restPerColour) : f.shLimit;
     const uint32_t words = degree == 0 ? 0u : f.shWords;
-    if (words > 0) {
-        // Coalesced copy of this workgroup's harmonic rows into LDS.
-        const uint32_t inBlock = f.count - first < kBlock ? f.count - first : kBlock;
-        const uint32_t total = inBlock * words;
-        const uint32_t* src = sh + static_cast<uint64_t>(first) * words;
-        for (uint32_t w = tid; w < total; w += kBlock) sSh[w] = src[w];
-    }
-    __syncthreads();
-    if (i >= f.count) return;
-    const uint32_t slot = f.base + i;
-    const float4 p = positions[i];
-    const uint4 row = shape[i];
-    float sx = expF(lowHalf(row.y));
-    float sy = expF(highHalf(row.y));
-    float sz = expF(lowHalf(row.z));
-    const EditFate fate = editFate(f.edit, p.x, p.y, p.z, p.w, sx, sy, sz);
     const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
-    if (fate.removed) {
+
+    // The footprint: position and shape only.
+    uint4 row = make_uint4(0u, 0u, 0u, 0u);
+    EditFate fate{true, false, 0.0f};
+    SplatShape shapeOf{0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f, {0.0f, 0.0f, 0.0f, 1.0f}};
+    Footprint fp;
+    uint32_t touched = 0;
+    if (live) {
+        const float4 p = positions[i];
+        row = shape[i];
+        const float sx = expF(lowHalf(row.y)), sy = expF(highHalf(row.y)), sz = expF(lowHalf(row.z));
+        fate = editFate(f.edit, p.x, p.y, p.z, p.w, sx, sy, sz);
+        if (!fate.removed) {
+            shapeOf = SplatShape{p.x, p.y, p.z, fate.opacity, sx, sy, sz, {0.0f, 0.0f, 0.0f, 0.0f}};
+            if (f.useFilter3d) applyFilter3d(filter3d[i], shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
+            decodeQuaternion(row.x, shapeOf.q);
+            touched = footprintOf(f, shapeOf, fp);
+        }
+    }
+
+    // The drawn splats in splat order: row `at` of the staged harmonics.
+    const bool drawn = touched > 0;
+    const uint64_t drawnLanes = __ballot(drawn);
+    if (lane == 0) sDrawn[wave] = static_cast<uint32_t>(__popcll(drawnLanes));
+    __syncthreads();
+    uint32_t at = static_cast<uint32_t>(__popcll(drawnLanes & ((1ull << lane) - 1ull)));
+    uint32_t rows = 0;
+#pragma unroll
+    for (uint32_t w = 0; w < kBlock / 64; ++w) {
+        if (w < wave) at += sDrawn[w];
+        rows += sDrawn[w];
+    }
+    if (drawn) sRowOf[at] = tid;
+    __syncthreads();
+    stageDrawnRows(sh + static_cast<uint64_t>(first) * words, 0u, words, rows, sRowOf, sSh);
+    __syncthreads();
+
+    if (!live) return;
+    if (!drawn) {
         touchedOut[slot] = 0;
         rects[slot] = emptyRect;
         return;
     }
-    SplatShape shapeOf{p.x, p.y, p.z, fate.opacity, sx, sy, sz, {0.0f, 0.0f, 0.0f, 0.0f}};
-    if (f.useFilter3d) applyFilter3d(filter3d[i], shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
-    decodeQuaternion(row.x, shapeOf.q);
-    const uint32_t* shRow = sSh + tid * words;
-    const auto colourOf = [&](float dx, float dy, float dz, float colour[3]) {
-        // Unpacked with constant indices so the 45 coefficients stay in registers.
-        float coeff[45];
-        const uint32_t halves = degree == 0 ? 0u : (degree == 1 ? 9u : degree == 2 ? 24u : 45u);
+    // Unpacked with constant indices so the 45 coefficients stay in registers.
+    const uint32_t* shRow = sSh + at * words;
+    float coeff[45];
+    const uint32_t halves = degree == 0 ? 0u : (degree == 1 ? 9u : degree == 2 ? 24u : 45u);
 #pragma unroll
-        for (uint32_t k = 0; k < 45; ++k) {
-            const uint32_t wd = k < halves ? shRow[k >> 1] : 0u;
-            coeff[k] = (k & 1u) == 0u ? lowHalf(wd) : highHalf(wd);
-        }
-        float rest[3];
-        evaluateRest(coeff, degree, dx, dy, dz, rest);
-        colour[0] = fmaxS(highHalf(row.z) + rest[0], 0.0f);
-        colour[1] = fmaxS(lowHalf(row.w) + rest[1], 0.0f);
-        colour[2] = fmaxS(highHalf(row.w) + rest[2], 0.0f);
-        if (fate.graded) editGrade(f.edit, colour);
-    };
-    uint2 rect = emptyRect;
-    const uint32_t touched = projectOne(f, shapeOf, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
-    rects[slot] = touched > 0 ? rect : emptyRect;
+    for (uint32_t k = 0; k < 45; ++k) {
+        const uint32_t wd = k < halves ? shRow[k >> 1] : 0u;
+        coeff[k] = (k & 1u) == 0u ? lowHalf(wd) : highHalf(wd);
+    }
+    float dx, dy, dz;
+    viewDirection(f, shapeOf, dx, dy, dz);
+    float rest[3];
+    evaluateRest(coeff, degree, dx, dy, dz, rest);
+    float colour[3];
+    colour[0] = fmaxS(highHalf(row.z) + rest[0], 0.0f);
+    colour[1] = fmaxS(lowHalf(row.w) + rest[1], 0.0f);
+    colour[2] = fmaxS(highHalf(row.w) + rest[2], 0.0f);
+    if (fate.graded) editGrade(f.edit, colour);
+    writeProjection(fp, colour, proj + static_cast<uint64_t>(slot) * 3, rects + slot);
     touchedOut[slot] = touched;
 }
 

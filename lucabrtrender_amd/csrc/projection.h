@@ -45,17 +45,23 @@ __device__ __forceinline__ void rotationOf(const float q[4], float r[9]) {
     r[6] = 2.0f * (x * z - w * y); r[7] = 2.0f * (y * z + w * x); r[8] = 1.0f - 2.0f * (x * x + y * y);
 }
 
-// Projects splat `slot`; returns the tiles it touches (0: culled, and then the
-// record is left alone). `colourOf(dx, dy, dz, rgb)` gives the colour for the
-// unit direction from the eye, called only for a splat that is drawn.
-template <class ColourOf>
-__device__ __forceinline__ uint32_t projectOne(const FrameParams& f, const SplatShape& s, ColourOf colourOf,
-                                               float4* __restrict__ out, uint2* __restrict__ rectOut) {
+// What the projection keeps of a drawn splat before its colour.
+struct Footprint {
+    float cx, cy, tz, cutoff;   // screen centre, view depth, power cutoff
+    float ca, cb, cc, alpha;    // conic and opacity
+    uint32_t rect[4];           // inclusive tile rectangle x0, y0, x1, y1
+};
+
+// The splat's footprint from frame f; returns the tiles it touches (0:
+// culled, and then fp is partly written). Must be reached by every active
+// lane of the wave (the count of a large footprint is shared among them).
+__device__ __forceinline__ uint32_t footprintOf(const FrameParams& f, const SplatShape& s, Footprint& fp) {
     const float* m = f.m;
     const float tx = m[0] * s.px + m[1] * s.py + m[2] * s.pz + m[3];
     const float ty = m[4] * s.px + m[5] * s.py + m[6] * s.pz + m[7];
     const float tz = m[8] * s.px + m[9] * s.py + m[10] * s.pz + m[11];
     if (!(tz > f.nearZ) || tz > f.farZ) return 0;
+    fp.tz = tz;
     float r[9];
     rotationOf(s.q, r);
     // M = R diag(s)
@@ -113,22 +119,53 @@ __device__ __forceinline__ uint32_t projectOne(const FrameParams& f, const Splat
     if (!(alpha >= 1.0f / 255.0f)) return 0;
     const float ca = c / det, cb = -b / det, cc = a / det;
     const float cutoff = cutoffPower(alpha);
-    uint32_t rect[4];
-    tileRect(f.width, f.height, cx, cy, ca, cb, cc, cutoff, rect);
-    if (rect[0] > rect[2] || rect[1] > rect[3]) return 0;
-    const uint32_t touched = countTouchedTiles(f.width, f.height, cx, cy, ca, cb, cc, cutoff, rect);
-    if (touched == 0) return 0;
+    tileRect(f.width, f.height, cx, cy, ca, cb, cc, cutoff, fp.rect);
+    if (fp.rect[0] > fp.rect[2] || fp.rect[1] > fp.rect[3]) return 0;
+    const uint32_t touched = countTouchedTiles(f.width, f.height, cx, cy, ca, cb, cc, cutoff, fp.rect);
+    fp.cx = cx;
+    fp.cy = cy;
+    fp.cutoff = cutoff;
+    fp.ca = ca;
+    fp.cb = cb;
+    fp.cc = cc;
+    fp.alpha = alpha;
+    return touched;
+}
 
-    // Colour from this eye, in the cloud's own space.
+// The unit direction from the eye to the splat's centre, in the cloud's own
+// space: the argument of the colour's harmonics.
+__device__ __forceinline__ void viewDirection(const FrameParams& f, const SplatShape& s, float& dx, float& dy,
+                                              float& dz) {
     const float dxo = s.px - f.eye[0], dyo = s.py - f.eye[1], dzo = s.pz - f.eye[2];
     const float len = __builtin_sqrtf((dxo * dxo + dyo * dyo) + dzo * dzo);
-    float colour[3];
-    colourOf(dxo / len, dyo / len, dzo / len, colour);
+    dx = dxo / len;
+    dy = dyo / len;
+    dz = dzo / len;
+}
 
-    out[0] = make_float4(cx, cy, tz, cutoff);
-    out[1] = make_float4(ca, cb, cc, alpha);
+// The three float4 of a drawn splat's record and its packed tile rectangle.
+__device__ __forceinline__ void writeProjection(const Footprint& fp, const float colour[3], float4* __restrict__ out,
+                                                uint2* __restrict__ rectOut) {
+    out[0] = make_float4(fp.cx, fp.cy, fp.tz, fp.cutoff);
+    out[1] = make_float4(fp.ca, fp.cb, fp.cc, fp.alpha);
     out[2] = make_float4(colour[0], colour[1], colour[2], 0.0f);
-    *rectOut = make_uint2(rect[0] | (rect[1] << 16), rect[2] | (rect[3] << 16));
+    *rectOut = make_uint2(fp.rect[0] | (fp.rect[1] << 16), fp.rect[2] | (fp.rect[3] << 16));
+}
+
+// Projects splat `slot`; returns the tiles it touches (0: culled, and then the
+// record is left alone). `colourOf(dx, dy, dz, rgb)` gives the colour for the
+// unit direction from the eye, called only for a splat that is drawn.
+template <class ColourOf>
+__device__ __forceinline__ uint32_t projectOne(const FrameParams& f, const SplatShape& s, ColourOf colourOf,
+                                               float4* __restrict__ out, uint2* __restrict__ rectOut) {
+    Footprint fp;
+    const uint32_t touched = footprintOf(f, s, fp);
+    if (touched == 0) return 0;
+    float dx, dy, dz;
+    viewDirection(f, s, dx, dy, dz);
+    float colour[3];
+    colourOf(dx, dy, dz, colour);
+    writeProjection(fp, colour, out, rectOut);
     return touched;
 }
 

@@ -11,18 +11,53 @@ namespace lrt::kern {
 namespace {
 
 constexpr uint32_t kBlock = 256;
+constexpr uint32_t kPer = 4;                     // consecutive pairs a thread
+constexpr uint32_t kNone = 0xFFFFFFFFu;          // no tile: before the first pair, after the last
 
+template <int DPP_CTRL>
+__device__ __forceinline__ uint32_t fromLane(uint32_t v, uint32_t missing) {
+    return static_cast<uint32_t>(__builtin_amdgcn_update_dpp(static_cast<int>(missing), static_cast<int>(v), DPP_CTRL,
+                                                             0xF, 0xF, false));
+}
+
+// Thread g takes pairs [4g, 4g + 4) with two 16-byte loads; the tile before
+// its first pair and after its last come from the neighbouring lanes (DPP
+// wave shifts), the wave's two end lanes loading theirs. Keys must be 16-byte
+// aligned (the rasteriser's sort buffers are).
 __global__ __launch_bounds__(kBlock) void rangesKernel(const uint64_t* __restrict__ keys, uint32_t cap,
                                                        const uint32_t* __restrict__ nDevice, uint32_t tiles,
                                                        uint2* __restrict__ ranges) {
     uint32_t pairs = cap;
     if (nDevice != nullptr && *nDevice < cap) pairs = *nDevice;
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= pairs) return;
-    const uint32_t t = static_cast<uint32_t>(keys[i] >> 32);
-    if (t >= tiles) return;
-    if (i == 0 || static_cast<uint32_t>(keys[i - 1] >> 32) != t) ranges[t].x = i;
-    if (i + 1 == pairs || static_cast<uint32_t>(keys[i + 1] >> 32) != t) ranges[t].y = i + 1;
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint32_t i0 = (blockIdx.x * kBlock + threadIdx.x) * kPer;
+    // Launched for the capacity: a wave wholly past the count leaves at once.
+    if (i0 - lane * kPer >= pairs) return;
+    uint32_t t[kPer];
+    if (i0 + kPer <= pairs) {
+        const uint4* k4 = reinterpret_cast<const uint4*>(keys + i0);
+        const uint4 a = k4[0], b = k4[1];
+        t[0] = a.y;
+        t[1] = a.w;
+        t[2] = b.y;
+        t[3] = b.w;
+    } else {
+#pragma unroll
+        for (uint32_t j = 0; j < kPer; ++j) t[j] = i0 + j < pairs ? static_cast<uint32_t>(keys[i0 + j] >> 32) : kNone;
+    }
+    uint32_t before = fromLane<0x138>(t[kPer - 1], kNone);   // wave_shr:1, lane - 1's last
+    uint32_t after = fromLane<0x130>(t[0], kNone);           // wave_shl:1, lane + 1's first
+    if (lane == 0) before = i0 > 0 && i0 - 1 < pairs ? static_cast<uint32_t>(keys[i0 - 1] >> 32) : kNone;
+    if (lane == 63) after = i0 + kPer < pairs ? static_cast<uint32_t>(keys[i0 + kPer] >> 32) : kNone;
+#pragma unroll
+    for (uint32_t j = 0; j < kPer; ++j) {
+        const uint32_t tile = t[j];
+        const uint32_t prev = j == 0 ? before : t[j - 1];
+        const uint32_t next = j + 1 == kPer ? after : t[j + 1];
+        if (tile >= tiles) continue;   // also kNone
+        if (prev != tile) ranges[tile].x = i0 + j;
+        if (next != tile) ranges[tile].y = i0 + j + 1;
+    }
 }
 
 }   // namespace
@@ -31,7 +66,9 @@ void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ran
                 const uint32_t* nDevice) {
     fill32(reinterpret_cast<uint32_t*>(ranges), 0u, static_cast<uint64_t>(tiles) * 2, stream);
     if (pairs == 0) return;
-    rangesKernel<<<(pairs + kBlock - 1) / kBlock, kBlock, 0, stream>>>(keys, pairs, nDevice, tiles, ranges);
+    const uint64_t threads = (uint64_t{pairs} + kPer - 1) / kPer;
+    rangesKernel<<<static_cast<uint32_t>((threads + kBlock - 1) / kBlock), kBlock, 0, stream>>>(keys, pairs, nDevice,
+                                                                                               tiles, ranges);
 }
 
 }   // namespace lrt::kern

@@ -15,30 +15,10 @@
 
 #include "frame.h"
 #include "kernels.h"
+#include "readers.h"
+#include "status.h"
 
 namespace lrt {
-
-enum class ErrorCode : int {   // modules/core/include/lrt/core/Error.h:14-26
-    Ok = 0,
-    Unknown,
-    NotFound,
-    InvalidArgument,
-    Unsupported,
-    IoFailure,
-    DeviceFailure,
-    ShaderFailure,
-    PluginFailure,
-    OutOfMemory,
-    Cancelled,
-    InternalError,
-};
-
-struct Status {
-    ErrorCode code = ErrorCode::Ok;
-    std::string message;
-    [[nodiscard]] bool ok() const noexcept { return code == ErrorCode::Ok; }
-    static Status error(ErrorCode c, std::string m) { return Status{c, std::move(m)}; }
-};
 
 // Host math in FP64 (aofx Transform.h): row-major 4x4 acting on column vectors.
 struct Mat4 {
@@ -211,6 +191,21 @@ public:
     // What the last render left (LRT_DEBUG_* of lrt_hip.h).
     Status debugCopy(int which, void* host, size_t bytes, size_t* needed);
 
+    // CloudLoader::upload (modules/scene/src/GpuClouds.cpp:199-227): raw
+    // records decoded into packed splats at caller device buffers sized for
+    // raw.count (positions 16 B, shape 16 B, sh shWords u32 a splat; sh may be
+    // null when no rest coefficient is kept), in slices of at most 256 MiB of
+    // records, compacted in file order. Synchronous. Reuses the frame's
+    // per-splat scratch, so the last frame's debug buffers are gone after it.
+    struct CloudInfo {
+        uint32_t declared = 0, count = 0, restPerColour = 0, shWords = 1;
+        float boundsMin[3] = {0, 0, 0}, boundsMax[3] = {0, 0, 0};
+        float uploadMs = 0, decodeMs = 0;   // GPU time: the records' copies in, and the kernels
+    };
+    static uint32_t keptPerColour(const io::SplatEncoding& e, uint32_t maxDegree);
+    Status decodeCloud(const io::RawSplats& raw, uint32_t maxDegree, float* positions, uint32_t* shape, uint32_t* sh,
+                       CloudInfo* info);
+
 private:
     Status reserveSplats(uint32_t count);
     Status reservePairs(uint32_t count, uint32_t keyBits, bool training = false);
@@ -240,6 +235,7 @@ private:
     DeviceBuffer pairSlot_, sortedSlots_, pairGrad_;
     // The FP64 brute force's records, keys and values (ping-pong) and sort workspace.
     DeviceBuffer refRecords_, refKeys_[2], refValues_[2], refWork_;
+    DeviceBuffer loadRaw_, loadBox_;   // a slice of raw records; the bounds' partials and box
     uint64_t sortWorkWords_ = 0;
     // Per tile and per pixel.
     uint32_t tileCapacity_ = 0;

@@ -772,6 +772,108 @@ Status TileRasterizer::prefixSum(const uint32_t* in, uint32_t* out, uint32_t* to
     return check(hipGetLastError(), "prefix sum");
 }
 
+uint32_t TileRasterizer::keptPerColour(const io::SplatEncoding& e, uint32_t maxDegree) {
+    static constexpr uint32_t kPerDegree[4] = {0, 3, 8, 15};
+    return std::min(e.restPerColour, kPerDegree[std::min(maxDegree, 3u)]);
+}
+
+Status TileRasterizer::decodeCloud(const io::RawSplats& raw, uint32_t maxDegree, float* positions, uint32_t* shape,
+                                   uint32_t* sh, CloudInfo* info) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    const io::SplatEncoding& e = raw.encoding;
+    if (raw.count == 0 || e.floatsPerRecord == 0 || raw.records.size() < uint64_t{raw.count} * e.floatsPerRecord) {
+        return Status::error(ErrorCode::InvalidArgument, "'" + raw.source + "': no splat records");
+    }
+    const uint32_t fields[] = {e.x,    e.y,    e.z,    e.opacity, e.scale0, e.scale1, e.scale2,
+                               e.rotW, e.rotX, e.rotY, e.rotZ,    e.dc0,    e.dc1,    e.dc2};
+    for (uint32_t f : fields) {
+        if (f >= e.floatsPerRecord) return Status::error(ErrorCode::InvalidArgument, "an encoding field past the record");
+    }
+    const uint32_t keep = keptPerColour(e, maxDegree);
+    const uint32_t perColourInFile = e.restColourOuter != 0 ? e.filePerColour : keep;
+    if (keep > 0) {
+        // The last rest field the decode reads: blue's last kept basis.
+        const uint64_t last = e.restColourOuter != 0 ? uint64_t{e.restBase} + 2ull * perColourInFile + (keep - 1)
+                                                     : uint64_t{e.restBase} + 3ull * (keep - 1) + 2;
+        if (last >= e.floatsPerRecord || perColourInFile < keep) {
+            return Status::error(ErrorCode::InvalidArgument, "the rest coefficients run past the record");
+        }
+    }
+    if (positions == nullptr || shape == nullptr || (keep > 0 && sh == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "null output buffer");
+    }
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    diffReady_ = false;   // the per-splat scratch it reuses holds the last frame
+    lastSplats_ = lastVisible_ = lastPairs_ = lastTiles_ = 0;
+    const uint64_t recordBytes = uint64_t{e.floatsPerRecord} * 4;
+    constexpr uint64_t kSliceBytes = 256ull << 20;
+    const auto perSlice = static_cast<uint32_t>(std::max<uint64_t>(1, std::min<uint64_t>(kSliceBytes / recordBytes, raw.count)));
+    if (!(s = reserveSplats(perSlice)).ok()) return s;
+    if (!(s = loadRaw_.reserve(uint64_t{perSlice} * recordBytes, "load.raw")).ok()) return s;
+    if (!(s = loadBox_.reserve((uint64_t{kern::boundsPartials(raw.count)} * 6 + 8) * 4, "load.box")).ok()) return s;
+    kern::DecodeParams p{};
+    p.stride = e.floatsPerRecord;
+    p.keep = keep;
+    p.shWords = keep == 0 ? 1u : (keep * 3 + 1) / 2;
+    p.x = e.x; p.y = e.y; p.z = e.z; p.opacity = e.opacity;
+    p.scale0 = e.scale0; p.scale1 = e.scale1; p.scale2 = e.scale2;
+    p.rotW = e.rotW; p.rotX = e.rotX; p.rotY = e.rotY; p.rotZ = e.rotZ;
+    p.dc0 = e.dc0; p.dc1 = e.dc1; p.dc2 = e.dc2;
+    p.restBase = e.restBase;
+    p.filePerColour = perColourInFile;
+    p.restColourOuter = e.restColourOuter;
+    p.opacityMode = static_cast<uint32_t>(e.opacityMode);
+    p.scaleMode = static_cast<uint32_t>(e.scaleMode);
+    p.colourMode = static_cast<uint32_t>(e.colourMode);
+    p.rotationMode = static_cast<uint32_t>(e.rotationMode);
+    p.restMode = static_cast<uint32_t>(e.restMode);
+    uint32_t* counters = counters_.as<uint32_t>();
+    uint32_t written = 0;
+    float uploadMs = 0.0F, decodeMs = 0.0F;
+    for (uint32_t first = 0; first < raw.count; first += perSlice) {
+        p.count = std::min(perSlice, raw.count - first);
+        (void)hipEventRecord(events_[0], stream_);
+        if (!(s = check(hipMemcpyAsync(loadRaw_.get(), raw.records.data() + uint64_t{first} * e.floatsPerRecord,
+                                       uint64_t{p.count} * recordBytes, hipMemcpyHostToDevice, stream_),
+                        "upload raw splats"))
+                 .ok())
+            return s;
+        (void)hipEventRecord(events_[1], stream_);
+        kern::fill32(counters, 0u, 8, stream_);
+        kern::decodeSplats(loadRaw_.as<float>(), p, touched_.as<uint32_t>(), offsets_.as<uint32_t>(),
+                           scanStatus_.as<uint64_t>(), counters + 0, counters + 1, counters + 5, written,
+                           reinterpret_cast<float4*>(positions), reinterpret_cast<uint4*>(shape), sh, stream_);
+        (void)hipEventRecord(events_[2], stream_);
+        (void)hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_);
+        if (!(s = check(hipStreamSynchronize(stream_), "decode splats")).ok()) return s;
+        if (hostCounters_[5] != 0) return Status::error(ErrorCode::InternalError, "the scan's look-back gave up");
+        written += hostCounters_[0];
+        float ms = 0.0F;
+        if (hipEventElapsedTime(&ms, events_[0], events_[1]) == hipSuccess) uploadMs += ms;
+        if (hipEventElapsedTime(&ms, events_[1], events_[2]) == hipSuccess) decodeMs += ms;
+    }
+    CloudInfo out;
+    out.uploadMs = uploadMs;
+    out.decodeMs = decodeMs;
+    out.declared = raw.count;
+    out.count = written;
+    out.restPerColour = keep;
+    out.shWords = p.shWords;
+    if (written == 0) return Status::error(ErrorCode::InvalidArgument, "'" + raw.source + "': no splat survived validation");
+    float* box = loadBox_.as<float>();
+    kern::splatBounds(reinterpret_cast<const float4*>(positions), written, box + 8, box, stream_);
+    float host[6];
+    if (!(s = check(hipMemcpyAsync(host, box, sizeof host, hipMemcpyDeviceToHost, stream_), "read bounds")).ok()) return s;
+    if (!(s = check(hipStreamSynchronize(stream_), "bounds")).ok()) return s;
+    for (int k = 0; k < 3; ++k) {
+        out.boundsMin[k] = host[k];
+        out.boundsMax[k] = host[3 + k];
+    }
+    if (info != nullptr) *info = out;
+    return {};
+}
+
 Status TileRasterizer::radixSort(uint64_t* keys, uint32_t* values, uint64_t* scratchKeys, uint32_t* scratchValues,
                                  uint32_t count, uint32_t keyBits, bool* resultInScratch) {
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");

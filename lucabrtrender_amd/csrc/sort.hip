@@ -68,7 +68,11 @@ __device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64
 }
 
 // Digit counts of every pass at once: workgroup b writes its row
-// hist[b * passes * 256 + pass * 256 + digit].
+// hist[b * passes * 256 + pass * 256 + digit]. The 64 lanes of a wave hold 64
+// consecutive keys, and consecutive keys often share a digit (a splat's pairs
+// share its depth; neighbouring tiles share the tile id's high bits), so each
+// run of equal digits among neighbouring lanes is added once, by its first
+// lane, with its length: no same-address LDS atomics within a wave.
 template <KeyMode MODE>
 __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
                                                               const uint32_t* __restrict__ nDevice, uint32_t bitLo,
@@ -78,22 +82,33 @@ __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __
     const uint32_t n = pairCount(cap, nDevice);
     for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kHistBlock) sHist[j] = 0;
     __syncthreads();
+    const uint32_t lane = threadIdx.x & 63u;
+    const uint64_t after = lane == 63 ? 0ull : ~0ull << (lane + 1);   // lanes above this one
     const uint32_t stride = gridDim.x * kHistBlock;
-    // Four keys in flight a thread.
-    for (uint32_t i0 = blockIdx.x * kHistBlock + threadIdx.x; i0 < n; i0 += 4 * stride) {
+    // The loop bound is the wave's first index, so whole waves iterate together.
+    const uint32_t waveStart = blockIdx.x * kHistBlock + (threadIdx.x & ~63u);
+    for (uint32_t w0 = waveStart; w0 < n; w0 += 4 * stride) {
         uint64_t k[4];
-        bool ok[4];
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
-            const uint32_t i = i0 + u * stride;   // n < 2^30: no wrap
-            ok[u] = i < n;
-            k[u] = ok[u] ? sortKey<MODE>(keys[i], bitLo, mask) : 0ull;
+            const uint32_t i = w0 + u * stride + lane;   // n < 2^30: no wrap
+            k[u] = i < n ? sortKey<MODE>(keys[i], bitLo, mask) : 0ull;
         }
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
-            if (!ok[u]) continue;
+            const uint32_t first = w0 + u * stride;
+            if (first >= n) break;   // uniform across the wave
+            const uint32_t live = n - first < 64u ? n - first : 64u;   // lanes [0, live) hold keys
             for (uint32_t p = 0; p < passes; ++p) {
-                atomicAdd(&sHist[p * kRadix + static_cast<uint32_t>((k[u] >> (8 * p)) & 255u)], 1u);
+                const uint32_t dg = static_cast<uint32_t>((k[u] >> (8 * p)) & 255u);
+                // The previous lane's digit (lane 0 reads a value unequal to its own).
+                const uint32_t prev = static_cast<uint32_t>(
+                    __builtin_amdgcn_update_dpp(static_cast<int>(dg ^ 1u), static_cast<int>(dg), 0x138, 0xF, 0xF, false));
+                const bool head = lane < live && (lane == 0 || prev != dg);
+                const uint64_t heads = __ballot(head);
+                const uint64_t next = heads & after;
+                const uint32_t end = next != 0 ? static_cast<uint32_t>(__builtin_ctzll(next)) : live;
+                if (head) atomicAdd(&sHist[p * kRadix + dg], end - lane);
             }
         }
     }
@@ -164,22 +179,54 @@ __global__ __launch_bounds__(1024) void digitOffsetsKernel(const uint32_t* __res
     }
 }
 
+// Where one pass's 8-bit digit sits in a 64-bit key. TileDepth: the digit of
+// the virtual key at bit `shift` is the low word's bits [shift + 7, +8) below
+// bit 24 of the virtual key (the depth's top 24 bits) and the high word's
+// bits [shift - 24, +8) above it (the tile id). Plain: bits [bitLo + shift, +8).
+// `mask` keeps the digit's bits below keyBits.
+struct PassDigit {
+    uint32_t hiWord;   // TileDepth: the digit is in the high word
+    uint32_t shift;    // TileDepth: within the chosen word; Plain: within the key
+    uint32_t mask;
+};
+
+template <KeyMode MODE>
+__device__ __forceinline__ uint32_t digitOf(uint64_t key, const PassDigit& pd) {
+    if constexpr (MODE == KeyMode::TileDepth) {
+        const uint32_t w = pd.hiWord != 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
+        return (w >> pd.shift) & pd.mask;
+    } else {
+        return static_cast<uint32_t>(key >> pd.shift) & pd.mask;
+    }
+}
+
+PassDigit passDigit(KeyMode mode, uint32_t bitLo, uint64_t mask, uint32_t shift) {
+    const uint32_t m = static_cast<uint32_t>((mask >> shift) & 255u);
+    if (mode == KeyMode::TileDepth) {
+        return shift >= 24 ? PassDigit{1u, shift - 24, m} : PassDigit{0u, shift + 7, m};
+    }
+    return PassDigit{0u, bitLo + shift, m};
+}
+
 template <KeyMode MODE>
 __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restrict__ keysIn,
                                                          const uint32_t* __restrict__ valsIn,
                                                          uint64_t* __restrict__ keysOut,
                                                          uint32_t* __restrict__ valsOut, uint32_t cap,
-                                                         const uint32_t* __restrict__ nDevice,
-                                                         uint32_t bitLo, uint64_t mask, uint32_t shift,
+                                                         const uint32_t* __restrict__ nDevice, PassDigit pd,
                                                          const uint32_t* __restrict__ digitOffsets,
                                                          uint32_t* __restrict__ status,
                                                          uint32_t* __restrict__ error) {
     __shared__ uint64_t sKeys[kPart];
     __shared__ uint32_t sVals[kPart];
+    // Per-wave digit counts; once the scatter positions are in registers, the
+    // digits of sKeys (bytes), so the write-out need not recompute them.
     __shared__ uint32_t sWave[kWaves][kRadix];
     __shared__ uint32_t sLocal[kRadix];
     __shared__ uint32_t sGlobal[kRadix];
     __shared__ uint32_t sScan[kWaves];
+    static_assert(sizeof(sWave) >= kPart, "the digit bytes fit where the wave counts were");
+    uint8_t* sDigit = reinterpret_cast<uint8_t*>(&sWave[0][0]);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -199,29 +246,36 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
     uint32_t v[kKpt];
     uint32_t d[kKpt];
     uint32_t r[kKpt];
+    // Loads clamped to the last pair (no branches); lanes past n are masked
+    // out of every vote below and write nothing.
+    const uint32_t last = n - 1;
 #pragma unroll
     for (uint32_t j = 0; j < kKpt; ++j) {
-        const uint32_t idx = waveFirst + j * 64 + lane;
-        const bool valid = idx < n;
-        k[j] = valid ? keysIn[idx] : 0ull;
-        v[j] = valid ? valsIn[idx] : 0u;
-        d[j] = static_cast<uint32_t>((sortKey<MODE>(k[j], bitLo, mask) >> shift) & 255u);
+        const uint32_t idx = min(waveFirst + j * 64 + lane, last);
+        k[j] = keysIn[idx];
+        v[j] = valsIn[idx];
     }
+#pragma unroll
+    for (uint32_t j = 0; j < kKpt; ++j) d[j] = digitOf<MODE>(k[j], pd);
 
-    // Rank within the wave: peers = lanes holding the same digit.
-    const uint64_t below = (1ull << lane) - 1ull;
+    // Rank within the wave: peers = lanes holding the same digit, the AND over
+    // the digit's bits of the bit's vote (bit set) or its complement (clear);
+    // m = all ones where the bit is clear, so vote ^ m is that term.
 #pragma unroll
     for (uint32_t j = 0; j < kKpt; ++j) {
         const bool valid = waveFirst + j * 64 + lane < n;
-        uint64_t peers = __ballot(valid);
+        const uint64_t live = __ballot(valid);
+        uint32_t lo = static_cast<uint32_t>(live), hi = static_cast<uint32_t>(live >> 32);
+        const int nd = static_cast<int>(~d[j]);
 #pragma unroll
         for (uint32_t b = 0; b < 8; ++b) {
-            const bool bit = ((d[j] >> b) & 1u) != 0u;
-            const uint64_t vote = __ballot(bit);
-            peers &= bit ? vote : ~vote;
+            const uint32_t m = static_cast<uint32_t>(__builtin_amdgcn_sbfe(nd, b, 1));
+            const uint64_t vote = __ballot(m == 0u);
+            lo &= static_cast<uint32_t>(vote) ^ m;
+            hi &= static_cast<uint32_t>(vote >> 32) ^ m;
         }
-        const uint32_t before = static_cast<uint32_t>(__popcll(peers & below));
-        const uint32_t count = static_cast<uint32_t>(__popcll(peers));
+        const uint32_t before = __builtin_amdgcn_mbcnt_hi(hi, __builtin_amdgcn_mbcnt_lo(lo, 0u));
+        const uint32_t count = static_cast<uint32_t>(__popc(lo) + __popc(hi));
         // The count a leader stores is read by other lanes of the wave at the
         // next key: relaxed atomics, so the compiler keeps no per-lane copy.
         uint32_t prev = 0;
@@ -298,32 +352,45 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
     sGlobal[tid] = digitOffsets[tid] + prefix - localStart;
     __syncthreads();
 
-    // Local scatter into digit order, then contiguous runs out.
+    // Local scatter into digit order: positions first (the wave counts are
+    // read), then the pairs and their digits into LDS, then contiguous runs out.
+#pragma unroll
+    for (uint32_t j = 0; j < kKpt; ++j) r[j] += sLocal[d[j]] + sWave[wave][d[j]];
+    __syncthreads();
+    bool misplaced = false;
 #pragma unroll
     for (uint32_t j = 0; j < kKpt; ++j) {
         if (waveFirst + j * 64 + lane < n) {
-            const uint32_t at = sLocal[d[j]] + sWave[wave][d[j]] + r[j];
+            const uint32_t at = r[j];
             if (at < kPart) {
                 sKeys[at] = k[j];
                 sVals[at] = v[j];
-            } else {
-                atomicOr(error, 2u);
+                sDigit[at] = static_cast<uint8_t>(d[j]);
             }
+            misplaced |= at >= kPart;
         }
     }
+    if (misplaced) atomicOr(error, 2u);
     __syncthreads();
+    // Unrolled, stores predicated, a bad position flagged once after the loop:
+    // the LDS reads of all kKpt pairs can be in flight together.
     const uint32_t count = n - base < kPart ? n - base : kPart;
-    for (uint32_t i = tid; i < count; i += kBlock) {
-        const uint64_t key = sKeys[i];
-        const uint32_t dd = static_cast<uint32_t>((sortKey<MODE>(key, bitLo, mask) >> shift) & 255u);
-        const uint32_t out = sGlobal[dd] + i;
-        if (out < n) {
-            keysOut[out] = key;
-            valsOut[out] = sVals[i];
-        } else {
-            atomicOr(error, 4u);
+    bool bad = false;
+#pragma unroll
+    for (uint32_t j = 0; j < kKpt; ++j) {
+        const uint32_t i = tid + j * kBlock;
+        if (i < count) {
+            const uint32_t out = sGlobal[sDigit[i]] + i;
+            const uint64_t key = sKeys[i];
+            const uint32_t val = sVals[i];
+            if (out < n) {
+                keysOut[out] = key;
+                valsOut[out] = val;
+            }
+            bad |= out >= n;
         }
     }
+    if (bad) atomicOr(error, 4u);
 }
 
 // Histogram workgroups for capacity n: one per kHistKeys keys, at most kHistRows.
@@ -355,7 +422,8 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     uint64_t* kout = keysAlt;
     uint32_t* vout = valuesAlt;
     for (uint32_t p = 0; p < passes; ++p) {
-        onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(kin, vin, kout, vout, n, nDevice, bitLo, mask, 8 * p,
+        onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(kin, vin, kout, vout, n, nDevice,
+                                                           passDigit(MODE, bitLo, mask, 8 * p),
                                                            ws.offsets + p * kRadix,
                                                            ws.status + static_cast<uint64_t>(p) * parts * kRadix,
                                                            ws.error);

@@ -36,9 +36,6 @@ struct SplatFields {
     float rest[45];
 };
 
-uint16_t floatToHalf(float f);
-uint32_t packHalves(float lo, float hi);
-uint32_t encodeQuaternion(const float q[4]);
 void packSplat(const SplatFields& s, uint32_t restPerColour, uint32_t shWords, float* position, uint32_t* shape,
                uint32_t* sh);
 void generate(const SceneSpec& spec, uint32_t count, float* positions, uint32_t* shape, uint32_t* sh);

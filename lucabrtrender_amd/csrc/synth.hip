@@ -2,9 +2,7 @@
 // synthetic scenes the rasteriser is measured and checked on.
 //
 // Packing follows the reference's splat_decode (shaders/lrt/scene/
-// splat_decode.slang:17-87) and packing.slang:26-50,104-113: fp16 halves
-// rounded to nearest even after a +-65000 clamp, rotation normalised then
-// stored smallest-three in 10 bits (rounded to nearest, ties to even).
+// splat_decode.slang:17-87) through packing.h, the decode kernel's own.
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
@@ -15,68 +13,10 @@
 #include <vector>
 
 #include "../../include/lrt_synth.h"
+#include "packing.h"
 #include "synth.h"
 
 namespace lrt::synth {
-namespace {
-
-uint32_t f2u(float f) {
-    uint32_t u;
-    std::memcpy(&u, &f, 4);
-    return u;
-}
-float u2f(uint32_t u) {
-    float f;
-    std::memcpy(&f, &u, 4);
-    return f;
-}
-
-}   // namespace
-
-uint16_t floatToHalf(float f) {
-    const uint32_t x = f2u(f);
-    const uint32_t sign = (x >> 16) & 0x8000u;
-    const uint32_t a = x & 0x7FFFFFFFu;
-    if (a >= 0x7F800000u) return static_cast<uint16_t>(sign | 0x7C00u | (a > 0x7F800000u ? (0x200u | ((a >> 13) & 0x3FFu)) : 0u));
-    if (a >= 0x477FF000u) return static_cast<uint16_t>(sign | 0x7C00u);   // rounds past 65504
-    if (a < 0x38800000u) {
-        // Subnormal half: units of 2^-24, rounded to nearest even by rint.
-        return static_cast<uint16_t>(sign | static_cast<uint32_t>(std::rint(u2f(a) * 16777216.0f)));
-    }
-    uint32_t h = (((a >> 23) - 112u) << 10) | ((a & 0x7FFFFFu) >> 13);
-    const uint32_t rest = a & 0x1FFFu;
-    if (rest > 0x1000u || (rest == 0x1000u && (h & 1u) != 0u)) h += 1u;
-    return static_cast<uint16_t>(sign | h);
-}
-
-uint32_t packHalves(float lo, float hi) {
-    const auto safe = [](float v) { return std::min(std::max(v, -65000.0f), 65000.0f); };
-    return static_cast<uint32_t>(floatToHalf(safe(lo))) | (static_cast<uint32_t>(floatToHalf(safe(hi))) << 16);
-}
-
-uint32_t encodeQuaternion(const float in[4]) {
-    const float len = std::sqrt(((in[0] * in[0] + in[1] * in[1]) + in[2] * in[2]) + in[3] * in[3]);
-    float q[4] = {in[0] / len, in[1] / len, in[2] / len, in[3] / len};
-    uint32_t largest = 0;
-    float best = std::fabs(q[0]);
-    for (uint32_t k = 1; k < 4; ++k) {
-        if (std::fabs(q[k]) > best) {
-            best = std::fabs(q[k]);
-            largest = k;
-        }
-    }
-    if (q[largest] < 0.0f)
-        for (float& v : q) v = -v;
-    uint32_t word = largest;
-    uint32_t shift = 22;
-    for (uint32_t k = 0; k < 4; ++k) {
-        if (k == largest) continue;
-        const float unit = std::min(std::max(q[k] / 0.70710678f * 0.5f + 0.5f, 0.0f), 1.0f);
-        word |= static_cast<uint32_t>(std::rint(unit * 1023.0f)) << shift;
-        shift -= 10;
-    }
-    return word;
-}
 
 void packSplat(const SplatFields& s, uint32_t restPerColour, uint32_t shWords, float* position, uint32_t* shape,
                uint32_t* sh) {
@@ -89,16 +29,15 @@ void packSplat(const SplatFields& s, uint32_t restPerColour, uint32_t shWords, f
     if (!(ql > 1e-8f)) {
         q[0] = 0.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 1.0f;
     }
-    shape[0] = encodeQuaternion(q);
-    shape[1] = packHalves(s.logScale[0], s.logScale[1]);
-    shape[2] = packHalves(s.logScale[2], s.base[0]);
-    shape[3] = packHalves(s.base[1], s.base[2]);
+    shape[0] = pack::encodeQuaternion(q);
+    shape[1] = pack::packHalves(s.logScale[0], s.logScale[1]);
+    shape[2] = pack::packHalves(s.logScale[2], s.base[0]);
+    shape[3] = pack::packHalves(s.base[1], s.base[2]);
     if (sh == nullptr) return;
     for (uint32_t w = 0; w < shWords; ++w) sh[w] = 0;
     const uint32_t halves = restPerColour * 3;
     for (uint32_t k = 0; k < halves; ++k) {
-        const auto safe = std::min(std::max(s.rest[k], -65000.0f), 65000.0f);
-        sh[k / 2] |= static_cast<uint32_t>(floatToHalf(safe)) << ((k & 1u) * 16);
+        sh[k / 2] |= static_cast<uint32_t>(pack::floatToHalf(pack::halfSafe(s.rest[k]))) << ((k & 1u) * 16);
     }
 }
 

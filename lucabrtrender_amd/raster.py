@@ -10,6 +10,7 @@ computation runs in liblrt_hip.so.
 from __future__ import annotations
 
 import ctypes as C
+import os
 from dataclasses import dataclass, field
 from typing import Optional, Sequence
 
@@ -250,6 +251,86 @@ class View:
     targets: RenderTargets
 
 
+class RawSplats:
+    """io::RawSplats (modules/io/include/lrt/io/RawSplats.h:15-80): a splat file's
+    records, one float a property in the file's order, held by the library on the
+    host, and their encoding (a dict of lrt_splat_encoding's fields)."""
+
+    def __init__(self, handle: C.c_void_p):
+        self._h = handle
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    @property
+    def count(self) -> int:
+        return int(lib.lrt_raw_splats_count(self._h))
+
+    @property
+    def encoding(self) -> dict:
+        e = _abi.SplatEncodingC()
+        lib.lrt_raw_splats_encoding(self._h, C.byref(e))
+        return {name: int(getattr(e, name)) for name in _abi.ENCODING_FIELDS}
+
+    @property
+    def records(self):
+        """A copy of the records as a float32 [count, floats_per_record] array."""
+        import numpy as np
+        n, w = self.count, self.encoding["floats_per_record"]
+        ptr = lib.lrt_raw_splats_records(self._h)
+        return np.ctypeslib.as_array(ptr, shape=(n * w,)).reshape(n, w).copy() if n * w else np.zeros((n, w), np.float32)
+
+    def close(self):
+        if self._h:
+            lib.lrt_raw_splats_destroy(self._h)
+            self._h = C.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def read_splats(path: str) -> RawSplats:
+    """io::readSplats (modules/io/src/SplatReaders.cpp:32-195): a 3DGS PLY or a
+    .splat file read on the host; raises LrtError with the reference's codes."""
+    h = C.c_void_p()
+    code = lib.lrt_splats_read(os.fsencode(path), C.byref(h))
+    if code != _abi.LRT_OK:
+        raise LrtError(code, lib.lrt_last_error(None).decode())
+    return RawSplats(h)
+
+
+def raw_splats(records, encoding: dict) -> RawSplats:
+    """Raw records from memory: records float32 [count, floats_per_record]; encoding
+    the lrt_splat_encoding fields (missing ones 0)."""
+    import numpy as np
+    rec = np.ascontiguousarray(records, dtype=np.float32)
+    e = _abi.SplatEncodingC(**{k: int(v) for k, v in encoding.items()})
+    if rec.ndim != 2 or rec.shape[1] != e.floats_per_record:
+        raise LrtError(3, "records must be [count, floats_per_record]")
+    h = C.c_void_p()
+    code = lib.lrt_raw_splats_create(rec.ctypes.data_as(C.c_void_p), rec.shape[0], C.byref(e), C.byref(h))
+    if code != _abi.LRT_OK:
+        raise LrtError(code, lib.lrt_last_error(None).decode())
+    return RawSplats(h)
+
+
+@dataclass
+class CloudInfo:
+    """What a decode made (scene::GpuSplats count/declared/bounds, GpuClouds.h:41-76)."""
+    declared: int
+    count: int
+    rest_per_colour: int
+    sh_words: int
+    bounds_min: tuple
+    bounds_max: tuple
+    upload_ms: float = 0.0   # GPU time of the records' copies to the device
+    decode_ms: float = 0.0   # GPU time of validate + scan + decode
+
+
 def _instances_c(instances: Sequence[SplatInstance]):
     arr = (_abi.SplatInstanceC * max(1, len(instances)))()
     for k, inst in enumerate(instances):
@@ -291,6 +372,28 @@ class TileRasterizer:
     def _check(self, code: int):
         if code != _abi.LRT_OK:
             raise LrtError(code, lib.lrt_last_error(self._ctx).decode())
+
+    def decode_cloud(self, raw: RawSplats, max_degree: int = 3):
+        """CloudLoader::upload (GpuClouds.cpp:151-227): the records validated and
+        packed on the GPU, kept ones in file order. Returns (GpuSplats, CloudInfo);
+        the splats' tensors are views of buffers sized for the declared count."""
+        import torch
+        n = raw.count
+        words = int(lib.lrt_cloud_sh_words(raw.handle, max_degree))
+        dev = torch.device("cuda", self.device)
+        positions = torch.empty((n, 4), dtype=torch.float32, device=dev)
+        shape = torch.empty((n, 4), dtype=torch.int32, device=dev)
+        # one word a splat means no rest coefficient is kept: no harmonics buffer
+        sh = torch.empty((n, words), dtype=torch.int32, device=dev) if words > 1 else None
+        info = _abi.CloudInfoC()
+        self._check(lib.lrt_cloud_decode(self._ctx, raw.handle, max_degree, C.c_void_p(positions.data_ptr()),
+                                         C.c_void_p(shape.data_ptr()),
+                                         C.c_void_p(sh.data_ptr()) if sh is not None else None, C.byref(info)))
+        k = int(info.count)
+        rest = int(info.rest_per_colour)
+        cloud = GpuSplats(positions[:k], shape[:k], sh[:k] if rest else None, rest)
+        return cloud, CloudInfo(int(info.declared), k, rest, int(info.sh_words), tuple(info.bounds_min),
+                                tuple(info.bounds_max), float(info.upload_ms), float(info.decode_ms))
 
     def render(self, camera_or_projection, instances: Sequence[SplatInstance], settings: RenderSettings,
                targets: RenderTargets) -> FrameStats:

@@ -194,9 +194,27 @@ uint32_t or_pack_halves(float lo, float hi) {
 
 static inline float half_safe(float v) { return clampf_(v, -65000.0f, 65000.0f); }
 
-/* splat_validate.slang:16-29, splat_decode.slang:17-87 (opacity mode 0 = logit,
- * scale mode 0 = log, colour mode 0 = SH DC, rotation (w,x,y,z) floats, no flip,
- * position scale 1), compacted in record order (GpuClouds.cpp:151-185). */
+/* splat_encoding.slang:28-96, the modes or_encoding names. */
+static float decode_opacity(float v, uint32_t mode) {
+    if (mode == 0) return 1.0f / (1.0f + or_expf(-v));
+    if (mode == 2) return v / 255.0f;
+    return v;
+}
+static float decode_log_scale(float v, uint32_t mode) {
+    if (mode == 2) return v / 16.0f - 10.0f;
+    return mode == 0 ? v : or_logf(fmax_(v, 1e-30f));
+}
+static float decode_colour(float v, uint32_t mode) {
+    if (mode == 0) return 0.5f + OR_SH0 * v;
+    if (mode == 2) return v / 255.0f;
+    if (mode == 3) return 0.5f + OR_SH0 * ((v / 255.0f - 0.5f) / 0.15f);
+    return v;
+}
+static float decode_byte_unit(float v, uint32_t mode) { return mode == 1 ? (v - 128.0f) / 128.0f : v; }
+
+/* splat_validate.slang:16-29, splat_decode.slang:17-87: kept when the
+ * position and log scales are finite, |log scale| < 60 and opacity >= 1/255;
+ * compacted in record order (GpuClouds.cpp:151-185). */
 uint32_t or_decode_records(const float* records, uint32_t count, const or_encoding* e,
                            uint32_t keep, float* positions, uint32_t* shape, uint32_t* sh,
                            uint32_t sh_words) {
@@ -204,8 +222,10 @@ uint32_t or_decode_records(const float* records, uint32_t count, const or_encodi
     for (uint32_t i = 0; i < count; ++i) {
         const float* r = records + (size_t)i * e->floats_per_record;
         const float px = r[e->x], py = r[e->y], pz = r[e->z];
-        const float a = 1.0f / (1.0f + or_expf(-r[e->opacity]));
-        const float l0 = r[e->scale0], l1 = r[e->scale1], l2 = r[e->scale2];
+        const float a = decode_opacity(r[e->opacity], e->opacity_mode);
+        const float l0 = decode_log_scale(r[e->scale0], e->scale_mode);
+        const float l1 = decode_log_scale(r[e->scale1], e->scale_mode);
+        const float l2 = decode_log_scale(r[e->scale2], e->scale_mode);
         const int finite = isfinite(px) && isfinite(py) && isfinite(pz) && isfinite(l0) && isfinite(l1) &&
                            isfinite(l2) && fabsf(l0) < 60.0f && fabsf(l1) < 60.0f && fabsf(l2) < 60.0f;
         if (!(finite && a >= 1.0f / 255.0f)) continue;
@@ -214,14 +234,15 @@ uint32_t or_decode_records(const float* records, uint32_t count, const or_encodi
         positions[j * 4 + 1] = py;
         positions[j * 4 + 2] = pz;
         positions[j * 4 + 3] = clampf_(a, 0.0f, 1.0f);
-        float q[4] = {r[e->rot_x], r[e->rot_y], r[e->rot_z], r[e->rot_w]};
+        float q[4] = {decode_byte_unit(r[e->rot_x], e->rotation_mode), decode_byte_unit(r[e->rot_y], e->rotation_mode),
+                      decode_byte_unit(r[e->rot_z], e->rotation_mode), decode_byte_unit(r[e->rot_w], e->rotation_mode)};
         const float qlen = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
         if (!(qlen > 1e-8f)) {
             q[0] = 0.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 1.0f;
         }
-        const float b0 = 0.5f + OR_SH0 * r[e->dc0];
-        const float b1 = 0.5f + OR_SH0 * r[e->dc1];
-        const float b2 = 0.5f + OR_SH0 * r[e->dc2];
+        const float b0 = decode_colour(r[e->dc0], e->colour_mode);
+        const float b1 = decode_colour(r[e->dc1], e->colour_mode);
+        const float b2 = decode_colour(r[e->dc2], e->colour_mode);
         shape[j * 4 + 0] = or_encode_quaternion(q);
         shape[j * 4 + 1] = or_pack_halves(half_safe(l0), half_safe(l1));
         shape[j * 4 + 2] = or_pack_halves(half_safe(l2), half_safe(b0));
@@ -232,7 +253,8 @@ uint32_t or_decode_records(const float* records, uint32_t count, const or_encodi
             for (uint32_t colour = 0; colour < 3; ++colour) {
                 const uint32_t field = e->rest_colour_outer ? e->rest_base + colour * e->file_per_colour + basis
                                                             : e->rest_base + basis * 3 + colour;
-                word |= (uint32_t)or_float_to_half(half_safe(r[field])) << (filled * 16);
+                const float value = decode_byte_unit(r[field], e->rest_mode);
+                word |= (uint32_t)or_float_to_half(half_safe(value)) << (filled * 16);
                 if (++filled == 2) {
                     sh[(size_t)j * sh_words + at++] = word;
                     word = 0;

@@ -50,18 +50,29 @@ uint32_t or_encode_quaternion(const float q_xyzw[4]);
 void or_decode_quaternion(uint32_t word, float q_xyzw[4]);
 uint32_t or_pack_halves(float lo, float hi);
 
-/* A 3DGS-encoded record set (tests/render/SplatFixtures.h CloudBuilder layout,
- * modules/io RawSplats): positions, logit opacity, log scales, (w,x,y,z),
- * dc[3], rest coefficients stored colour-outer (rrr.. ggg.. bbb..). */
+/* A raw record set and how its fields are encoded (modules/io/include/lrt/io/
+ * RawSplats.h:15-65): field offsets in floats, rest coefficients colour-outer
+ * (rrr.. ggg.. bbb.., a 3DGS PLY) or rgb per basis, and the decode modes,
+ * numbered as RawSplats.h:42-58 numbers them:
+ *   opacity_mode   0 logit, 1 linear, 2 byte (v / 255)
+ *   scale_mode     0 log, 1 linear (log(max(v, 1e-30))), 2 SPZ byte (v / 16 - 10)
+ *   colour_mode    0 SH DC (0.5 + C0 v), 1 linear, 2 byte (v / 255),
+ *                  3 SPZ byte (0.5 + C0 ((v / 255 - 0.5) / 0.15))
+ *   rotation_mode  0 float (w, x, y, z), 1 byte ((v - 128) / 128 each)
+ *   rest_mode      0 float, 1 byte ((v - 128) / 128)
+ * All zero is a 3DGS PLY (tests/render/SplatFixtures.h CloudBuilder). */
 typedef struct {
     uint32_t floats_per_record;
     uint32_t x, y, z, opacity, scale0, scale1, scale2, rot_w, rot_x, rot_y, rot_z;
     uint32_t dc0, dc1, dc2, rest_base, file_per_colour, rest_colour_outer;
+    uint32_t opacity_mode, scale_mode, colour_mode, rotation_mode, rest_mode;
 } or_encoding;
 
 /* splat_validate + scan + splat_decode for one slice (shaders/lrt/scene/
- * splat_validate.slang:16-29, splat_decode.slang:17-87, GpuClouds.cpp:199-227).
- * Writes compacted packed splats; returns the number kept. */
+ * splat_validate.slang:16-29, splat_decode.slang:17-87, splat_encoding.slang:
+ * 28-96, GpuClouds.cpp:151-227). Writes compacted packed splats; returns the
+ * number kept. The SPZ rotation forms and the y/z flip are not restated: no
+ * reader here produces them. */
 uint32_t or_decode_records(const float* records, uint32_t count, const or_encoding* e,
                            uint32_t keep_per_colour, float* positions, uint32_t* shape,
                            uint32_t* sh, uint32_t sh_words);

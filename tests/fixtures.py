@@ -89,6 +89,35 @@ def decode_records(records: np.ndarray, keep: int = 3) -> HostScene:
     return HostScene(pos[:kept].copy(), shape[:kept].copy(), sh[:kept].copy() if keep else None, keep)
 
 
+def decode_raw(raw, keep: int = 3) -> HostScene:
+    """The oracle's decode of a library RawSplats (a file the readers read), with
+    the rest coefficients kept up to `keep` a colour, as the GPU decode keeps them."""
+    e = raw.encoding
+    keep = min(keep, e["rest_per_colour"])
+    enc = O.OrEncoding(**{k: v for k, v in e.items() if k != "rest_per_colour"})
+    if not e["rest_colour_outer"]:
+        enc.file_per_colour = keep
+    records = np.ascontiguousarray(raw.records, dtype=np.float32)
+    n = records.shape[0]
+    words = 1 if keep == 0 else (keep * 3 + 1) // 2
+    pos = np.zeros((n, 4), np.float32)
+    shape = np.zeros((n, 4), np.uint32)
+    sh = np.zeros((n, words), np.uint32)
+    kept = O.lib.or_decode_records(O.ptr(records), n, C.byref(enc), keep, O.ptr(pos), O.ptr(shape), O.ptr(sh),
+                                   words)
+    return HostScene(pos[:kept].copy(), shape[:kept].copy(), sh[:kept].copy() if keep else None, keep)
+
+
+def ply_encoding(rest_per_colour=0, file_per_colour=0, floats=14, rest_base=0, opacity=3, scale0=4, rot_w=7,
+                 dc0=11, x=0):
+    """A 3DGS-PLY lrt_splat_encoding: fields consecutive from each given start
+    (position x y z, scales, rotation w x y z, dc), modes all 0."""
+    return dict(floats_per_record=floats, x=x, y=x + 1, z=x + 2, opacity=opacity, scale0=scale0, scale1=scale0 + 1,
+                scale2=scale0 + 2, rot_w=rot_w, rot_x=rot_w + 1, rot_y=rot_w + 2, rot_z=rot_w + 3, dc0=dc0,
+                dc1=dc0 + 1, dc2=dc0 + 2, rest_base=rest_base, rest_per_colour=rest_per_colour,
+                file_per_colour=file_per_colour, rest_colour_outer=1)
+
+
 def random_cloud(count: int, seed: int, min_size=0.004, max_size=0.6, min_opacity=0.02,
                  max_opacity=0.99) -> CloudBuilder:
     """SplatFixtures.h:57-85, draw for draw."""
