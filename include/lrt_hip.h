@@ -171,7 +171,8 @@ void lrt_context_destroy(lrt_context* ctx);
 lrt_status lrt_context_set_stream(lrt_context* ctx, void* stream);
 /* The message of the context's last failure; with NULL, that of the calling
  * thread's last call that takes no context (lrt_context_create,
- * lrt_splats_read, lrt_raw_splats_create), empty when it succeeded. */
+ * lrt_splats_read, lrt_raw_splats_create, lrt_write_exr), empty when it
+ * succeeded. */
 const char* lrt_last_error(const lrt_context* ctx);
 
 /* TileRasterizer::render(const Projection&, ...) (TileRasterizer.h:143-155,
@@ -308,6 +309,16 @@ lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, ui
 
 /* Waits for everything queued on the context's stream. */
 lrt_status lrt_context_synchronize(lrt_context* ctx);
+
+/* ---- images ------------------------------------------------------------------
+ * io::writeExr (modules/io/src/Exr.cpp:130-157): an OpenEXR scanline file,
+ * ZIP-compressed, channels A B G R (half when half != 0, else float) and Z
+ * (float, when depth is not NULL), rows top first from images stored bottom
+ * row first as the rasteriser stores them. rgba (width * height * 4) and depth
+ * (width * height) are HOST memory. LRT_IO_FAILURE when the file cannot be
+ * written; the message: lrt_last_error(NULL). */
+lrt_status lrt_write_exr(const char* path, uint32_t width, uint32_t height, const float* rgba, const float* depth,
+                         int32_t half);
 
 /* ---- splat files --------------------------------------------------------------
  * io::RawSplats and io::SplatEncoding (modules/io/include/lrt/io/RawSplats.h:

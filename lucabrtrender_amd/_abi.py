@@ -178,6 +178,7 @@ SIGNATURES = {
     "lrt_radix_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.POINTER(C.c_int32)]),
     "lrt_context_synchronize": (C.c_int, [C.c_void_p]),
+    "lrt_write_exr": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32]),
     "lrt_splats_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "lrt_raw_splats_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(SplatEncodingC), C.POINTER(C.c_void_p)]),
     "lrt_raw_splats_destroy": (None, [C.c_void_p]),

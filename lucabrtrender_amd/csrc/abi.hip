@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/lrt_hip.h"
+#include "exr.h"
 #include "rasterizer.h"
 
 struct lrt_context {
@@ -500,6 +501,12 @@ lrt_status lrt_context_synchronize(lrt_context* ctx) {
                                        : lrt::Status::error(lrt::ErrorCode::DeviceFailure, hipGetErrorString(e)));
 }
 
+
+lrt_status lrt_write_exr(const char* path, uint32_t width, uint32_t height, const float* rgba, const float* depth,
+                         int32_t half) {
+    if (path == nullptr) return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null path"));
+    return noContext(lrt::io::writeExr(path, width, height, rgba, depth, half != 0));
+}
 
 lrt_status lrt_splats_read(const char* path, lrt_raw_splats** out) {
     if (out == nullptr) return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null output"));

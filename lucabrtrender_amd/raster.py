@@ -318,6 +318,25 @@ def raw_splats(records, encoding: dict) -> RawSplats:
     return RawSplats(h)
 
 
+def write_exr(path: str, colour, depth=None, half: bool = True):
+    """io::writeExr (modules/io/src/Exr.cpp:130-157): colour float32 [H, W, 4] and
+    depth float32 [H, W] (numpy or torch, bottom row first, as RenderTargets hold
+    them) into a ZIP-compressed scanline EXR, R G B A half (or float) and Z float."""
+    c = np.ascontiguousarray(colour.cpu().numpy() if hasattr(colour, "cpu") else colour, dtype=np.float32)
+    if c.ndim != 3 or c.shape[2] != 4:
+        raise LrtError(3, "colour must be [height, width, 4]")
+    h, w = c.shape[:2]
+    d = None
+    if depth is not None:
+        d = np.ascontiguousarray(depth.cpu().numpy() if hasattr(depth, "cpu") else depth, dtype=np.float32)
+        if d.shape != (h, w):
+            raise LrtError(3, "depth must be [height, width]")
+    code = lib.lrt_write_exr(os.fsencode(path), w, h, c.ctypes.data_as(C.c_void_p),
+                             d.ctypes.data_as(C.c_void_p) if d is not None else None, 1 if half else 0)
+    if code != _abi.LRT_OK:
+        raise LrtError(code, lib.lrt_last_error(None).decode())
+
+
 @dataclass
 class CloudInfo:
     """What a decode made (scene::GpuSplats count/declared/bounds, GpuClouds.h:41-76)."""
