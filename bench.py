@@ -37,9 +37,9 @@ def log(*a):
 def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes):
     """Algorithmic HBM bytes of each kernel for one frame."""
     return {
-        # every splat's packed record, harmonics and 3D filter read; record (48),
-        # rect (8) and count (4) written
-        "project": n * (32 + 4 + 4 * sh_words) + n * 60,
+        # every splat's packed record and 3D filter read, rect (8) and count (4)
+        # written; the drawn ones' harmonics read and record (48) written
+        "project": n * (32 + 4 + 12) + visible * (4 * sh_words + 48),
         # one u32 read and one written per splat
         "scan": n * 8,
         # the visible splats' record, rect, count and offset read; key + value written per pair
