@@ -356,7 +356,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
-    ap.add_argument("--blend-batch", type=int, default=128, help="records in each quadrant's LDS list: 128 or 256")
+    ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 128 or 256")
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="config-3 fwd+bwd views timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")

@@ -133,8 +133,8 @@ typedef struct {
     int32_t time_stages;          /* per-stage HIP event timing into lrt_frame_stats */
     float dilation;               /* 0.3 reproduces the reference */
     int32_t filter3d;             /* 1: read lrt_splats.filter3d */
-    uint32_t blend_batch;         /* records each 8x8 quadrant's LDS list holds: 128 or 256 (0: 128) */
-    int32_t tile_order;           /* 0: tile index order; 1: heaviest tiles launched first */
+    uint32_t blend_batch;         /* records each 8x8 quadrant's LDS list holds: 128 or 256 (0, the default: 256) */
+    int32_t tile_order;           /* 0: tile index order; 1 (default): heaviest tiles launched first */
     int32_t persistent_blend;     /* batches: 1 = a persistent blend taking quadrants from a queue */
     int32_t use_graph;            /* batches: 1 = capture the launch sequence once and replay it */
 } lrt_render_settings;

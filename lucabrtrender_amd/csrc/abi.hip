@@ -62,7 +62,7 @@ lrt::RenderSettings toSettings(const lrt_render_settings& s) {
     out.timeStages = s.time_stages != 0;
     out.dilation = s.dilation;
     out.filter3d = s.filter3d != 0;
-    out.blendBatch = s.blend_batch == 0 ? 128u : s.blend_batch;
+    out.blendBatch = s.blend_batch == 0 ? 256u : s.blend_batch;
     out.heaviestTilesFirst = s.tile_order == 1;
     out.persistentBlend = s.persistent_blend != 0;
     out.useGraph = s.use_graph != 0;
@@ -266,8 +266,8 @@ void lrt_render_settings_default(lrt_render_settings* out) {
     out->time_stages = 0;
     out->dilation = s.dilation;
     out->filter3d = 0;
-    out->blend_batch = 128;
-    out->tile_order = 0;
+    out->blend_batch = 256;
+    out->tile_order = 1;
     out->persistent_blend = 0;
     out->use_graph = 0;
 }

@@ -10,6 +10,14 @@
 //   3. the wave walks the list kGroup records at a time with a branch-free body
 //      (a skipped pair contributes alpha 0, which changes nothing) and leaves
 //      once its 64 pixels are opaque.
+// A staged record carries its Gaussian's form expanded about the quadrant's
+// centre and scaled to the exponent's base 2, the cutoff folded in: a pixel at
+// (u, v) from the centre evaluates q = A u^2 + B u v + C v^2 + D u + E v + F
+// with five FMAs, passes the cutoff test when q >= 0, and has alpha =
+// min(0.99, o' 2^q), o' = opacity * 2^(cutoff scaled). q differs from the
+// direct form's scaled power by float rounding only (below 2e-4 on footprints
+// from a quarter pixel to 300 pixels wide, centres up to 3.5 sigma outside the
+// quadrant).
 // No wave ever waits for another: quadrants finish as soon as their own pixels
 // do, and a quadrant with few pairs frees its slot for the next. The four
 // quadrants of one tile are blocks b, b + 8, b + 16, b + 24, which the
@@ -92,8 +100,8 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
     // The list: records that reach this quadrant, in pair order, padded to a
     // multiple of kGroup with empty records (cutoff -1: never pass the test).
-    __shared__ float4 sGeom[LIST + kGroup];     // centre.xy, cutoff, alpha
-    __shared__ float4 sConic[LIST + kGroup];    // a, 2b, c, view z
+    __shared__ float4 sGeom[LIST + kGroup];     // A, B, C, D of the expanded form
+    __shared__ float4 sConic[LIST + kGroup];    // E, F less the cutoff, o', view z
     __shared__ float4 sColour[LIST + kGroup];   // rgb, bits of the pair's place in the tile's range + 1
 
     const uint32_t lane = threadIdx.x;
@@ -105,8 +113,10 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
     const uint32_t px = x0 + (lane & 7u);
     const uint32_t py = y0 + (lane >> 3);
     const bool inside = px < x1 && py < y1;
-    const float pxf = static_cast<float>(px) + 0.5f;
-    const float pyf = static_cast<float>(py) + 0.5f;
+    // The pixel centre about the quadrant's centre, and that centre.
+    const float pu = static_cast<float>(lane & 7u) - 3.5f;
+    const float pv = static_cast<float>(lane >> 3) - 3.5f;
+    const float xm = static_cast<float>(x0) + 4.0f, ym = static_cast<float>(y0) + 4.0f;
     const float boxLoX = static_cast<float>(x0) + 0.5f, boxLoY = static_cast<float>(y0) + 0.5f;
     const float boxHiX = static_cast<float>(x1 - 1u) + 0.5f, boxHiY = static_cast<float>(y1 - 1u) + 0.5f;
     const uint64_t below = (1ull << lane) - 1ull;
@@ -140,8 +150,14 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                 const uint64_t vote = __ballot(keep);
                 if (keep) {
                     const uint32_t to = fill + static_cast<uint32_t>(__popcll(vote & below));
-                    sGeom[to] = make_float4(s.screen.x, s.screen.y, s.screen.w, s.conic.w);
-                    sConic[to] = make_float4(s.conic.x, s.conic.y + s.conic.y, s.conic.z, s.screen.z);
+                    const float ux = s.screen.x - xm, vy = s.screen.y - ym;
+                    const float fa = kExpScale * s.conic.x, fb = kExpScale * (s.conic.y + s.conic.y);
+                    const float fc = kExpScale * s.conic.z;
+                    const float scaledCut = kExpScale * s.screen.w;
+                    const float fd = -(2.0f * fa * ux + fb * vy), fe = -(2.0f * fc * vy + fb * ux);
+                    const float ff = (ux * (fa * ux + fb * vy) + fc * vy * vy) - scaledCut;
+                    sGeom[to] = make_float4(fa, fb, fc, fd);
+                    sConic[to] = make_float4(fe, ff, s.conic.w * __builtin_amdgcn_exp2f(scaledCut), s.screen.z);
                     sColour[to] = make_float4(s.col.x, s.col.y, s.col.z, __uint_as_float(at - range.x + 1u));
                 }
                 fill += static_cast<uint32_t>(__popcll(vote));
@@ -152,8 +168,8 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
         }
         const uint32_t padded = (fill + kGroup - 1u) & ~(kGroup - 1u);
         if (lane < padded - fill) {
-            sGeom[fill + lane] = make_float4(0.0f, 0.0f, -1.0f, 0.0f);
-            sConic[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sGeom[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sConic[fill + lane] = make_float4(0.0f, -1e30f, 0.0f, 0.0f);   // q far below 0: never passes
             sColour[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
         }
         __syncthreads();
@@ -162,33 +178,31 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
         for (uint32_t j = 0; j < padded; j += kGroup) {
             float4 g0[kGroup], g1[kGroup], g2[kGroup];
 #pragma unroll
-            for (uint32_t u = 0; u < kGroup; ++u) {
-                g0[u] = sGeom[j + u];
-                g1[u] = sConic[j + u];
-                g2[u] = sColour[j + u];
+            for (uint32_t k = 0; k < kGroup; ++k) {
+                g0[k] = sGeom[j + k];
+                g1[k] = sConic[j + k];
+                g2[k] = sColour[j + k];
                 // the place is needed only when the pair contributes; without
                 // this the compiler loads it behind a branch in the walk
-                asm volatile("" ::"v"(g2[u].w));
+                asm volatile("" ::"v"(g2[k].w));
             }
 #pragma unroll
-            for (uint32_t u = 0; u < kGroup; ++u) {
-                const float dx = pxf - g0[u].x;
-                const float dy = pyf - g0[u].y;
-                const float power = __builtin_fmaf(dx, __builtin_fmaf(g1[u].x, dx, g1[u].y * dy),
-                                                   (g1[u].z * dy) * dy);
-                const float alpha = __builtin_fminf(0.99f, g0[u].w * __builtin_amdgcn_exp2f(kExpScale * power));
+            for (uint32_t k = 0; k < kGroup; ++k) {
+                const float q = __builtin_fmaf(pu, __builtin_fmaf(g0[k].x, pu, __builtin_fmaf(g0[k].y, pv, g0[k].w)),
+                                               __builtin_fmaf(pv, __builtin_fmaf(g0[k].z, pv, g1[k].x), g1[k].y));
+                const float alpha = __builtin_fminf(0.99f, g1[k].z * __builtin_amdgcn_exp2f(q));
                 // bitwise, not short-circuit: no branch inside the walk
-                const bool ok = active & (power <= g0[u].z) & (alpha >= 1.0f / 255.0f);
+                const bool ok = active & (q >= 0.0f) & (alpha >= 1.0f / 255.0f);
                 const float a = ok ? alpha : 0.0f;
                 const float weight = t * a;
-                r = __builtin_fmaf(weight, g2[u].x, r);
-                g = __builtin_fmaf(weight, g2[u].y, g);
-                b = __builtin_fmaf(weight, g2[u].z, b);
-                depthSum = __builtin_fmaf(weight, g1[u].w, depthSum);
-                t = t * (1.0f - a);
-                last = ok ? __float_as_uint(g2[u].w) : last;
+                r = __builtin_fmaf(weight, g2[k].x, r);
+                g = __builtin_fmaf(weight, g2[k].y, g);
+                b = __builtin_fmaf(weight, g2[k].z, b);
+                depthSum = __builtin_fmaf(weight, g1[k].w, depthSum);
+                t = t - weight;   // T (1 - alpha)
+                last = ok ? __float_as_uint(g2[k].w) : last;
                 if constexpr (DEPTH_MODE == 1) {
-                    depthAt = (ok && depthAt < 0.0f && 1.0f - t >= f.depthThreshold) ? g1[u].w : depthAt;
+                    depthAt = (ok && depthAt < 0.0f && 1.0f - t >= f.depthThreshold) ? g1[k].w : depthAt;
                 }
                 active = active && !(t < 1e-4f);
             }
@@ -270,49 +284,63 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
 
 // Heaviest tiles first: tiles bucketed by the bit length of their pair count,
 // buckets in descending order, tiles in index order inside a bucket. One
-// workgroup; bucket counts in LDS.
+// workgroup of 16 waves, wave w owning the w-th sixteenth of the tiles: each
+// wave counts its segment's buckets, the counts' prefix over (bucket
+// descending, wave ascending) gives each wave its first place in every bucket,
+// and each wave walks its segment again handing out places.
 constexpr uint32_t kBuckets = 33;
+constexpr uint32_t kOrderWaves = 16;
+constexpr uint32_t kOrderCached = 32768;   // tiles whose bucket the first walk leaves in LDS for the second
 
-__global__ __launch_bounds__(1024) void tileOrderKernel(const uint2* __restrict__ ranges, uint32_t tiles,
-                                                        uint32_t* __restrict__ order) {
-    __shared__ uint32_t sCount[kBuckets];
-    const uint32_t tid = threadIdx.x;
-    if (tid < kBuckets) sCount[tid] = 0;
+__device__ __forceinline__ uint32_t bucketOf(uint2 rg) {
+    return 32u - static_cast<uint32_t>(__clz(rg.y - rg.x));   // 0 for an empty tile
+}
+
+__global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2* __restrict__ ranges, uint32_t tiles,
+                                                                    uint32_t* __restrict__ order) {
+    __shared__ uint32_t sCount[kOrderWaves][kBuckets];
+    __shared__ uint32_t sStart[kOrderWaves][kBuckets];
+    __shared__ uint8_t sBucket[kOrderCached];
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wave = tid >> 6;
+    for (uint32_t k = tid; k < kOrderWaves * kBuckets; k += blockDim.x) (&sCount[0][0])[k] = 0;
     __syncthreads();
-    for (uint32_t t = tid; t < tiles; t += blockDim.x) {
-        const uint2 rg = ranges[t];
-        const uint32_t bucket = 32u - static_cast<uint32_t>(__clz(rg.y - rg.x));   // 0 for an empty tile
-        atomicAdd(&sCount[bucket], 1u);
+    const uint32_t segment = (tiles + kOrderWaves * 64 - 1) / (kOrderWaves * 64) * 64;   // whole chunks of 64
+    const uint32_t begin = wave * segment;
+    const uint32_t end = begin + segment < tiles ? begin + segment : tiles;
+    for (uint32_t t = begin + lane; t < end; t += 64) {
+        const uint32_t bucket = bucketOf(ranges[t]);
+        if (t < kOrderCached) sBucket[t] = static_cast<uint8_t>(bucket);
+        atomicAdd(&sCount[wave][bucket], 1u);
     }
     __syncthreads();
-    // Stable within a bucket: one wave walks the tiles in order. Lane k owns the
-    // next free position of bucket k in a register and hands it out by shuffle:
-    // positions never pass between lanes through memory, which the compiler
-    // would be free to cache per lane.
-    if (tid < 64) {
-        uint32_t next = 0;
-        if (tid < kBuckets) {
-            for (uint32_t k = tid + 1; k < kBuckets; ++k) next += sCount[k];
-        }
-        const uint64_t below = (1ull << tid) - 1ull;
-        for (uint32_t base = 0; base < tiles; base += 64) {
-            const uint32_t t = base + tid;
-            uint32_t bucket = kBuckets;
-            if (t < tiles) {
-                const uint2 rg = ranges[t];
-                bucket = 32u - static_cast<uint32_t>(__clz(rg.y - rg.x));
-            }
-            // One round per bucket present among these 64 tiles.
-            uint64_t todo = __ballot(t < tiles);
-            while (todo != 0) {
-                const int first = static_cast<int>(__builtin_ctzll(todo));
-                const uint32_t k = static_cast<uint32_t>(__shfl(static_cast<int>(bucket), first, 64));
-                const uint64_t vote = __ballot(t < tiles && bucket == k);
-                const uint32_t start = static_cast<uint32_t>(__shfl(static_cast<int>(next), static_cast<int>(k), 64));
-                if (t < tiles && bucket == k) order[start + static_cast<uint32_t>(__popcll(vote & below))] = t;
-                if (tid == k) next += static_cast<uint32_t>(__popcll(vote));
-                todo &= ~vote;
-            }
+    for (uint32_t e = tid; e < kOrderWaves * kBuckets; e += blockDim.x) {
+        const uint32_t w = e / kBuckets, k = e % kBuckets;
+        uint32_t start = 0;
+        for (uint32_t kk = k + 1; kk < kBuckets; ++kk)
+            for (uint32_t ww = 0; ww < kOrderWaves; ++ww) start += sCount[ww][kk];
+        for (uint32_t ww = 0; ww < w; ++ww) start += sCount[ww][k];
+        sStart[w][k] = start;
+    }
+    __syncthreads();
+    // Lane k owns the next free place of bucket k in a register and hands it
+    // out by shuffle: places never pass between lanes through memory, which the
+    // compiler would be free to cache per lane.
+    uint32_t next = lane < kBuckets ? sStart[wave][lane] : 0u;
+    const uint64_t below = (1ull << lane) - 1ull;
+    for (uint32_t base = begin; base < end; base += 64) {
+        const uint32_t t = base + lane;
+        const bool mine = t < end;
+        const uint32_t bucket = !mine ? kBuckets : t < kOrderCached ? sBucket[t] : bucketOf(ranges[t]);
+        // One round per bucket present among these 64 tiles.
+        uint64_t todo = __ballot(mine);
+        while (todo != 0) {
+            const int first = static_cast<int>(__builtin_ctzll(todo));
+            const uint32_t k = static_cast<uint32_t>(__shfl(static_cast<int>(bucket), first, 64));
+            const uint64_t vote = __ballot(mine && bucket == k);
+            const uint32_t start = static_cast<uint32_t>(__shfl(static_cast<int>(next), static_cast<int>(k), 64));
+            if (mine && bucket == k) order[start + static_cast<uint32_t>(__popcll(vote & below))] = t;
+            if (lane == k) next += static_cast<uint32_t>(__popcll(vote));
+            todo &= ~vote;
         }
     }
 }
@@ -378,7 +406,7 @@ void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gri
 
 void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t stream) {
     if (tiles == 0) return;
-    tileOrderKernel<<<1, 1024, 0, stream>>>(ranges, tiles, order);
+    tileOrderKernel<<<1, kOrderWaves * 64, 0, stream>>>(ranges, tiles, order);
 }
 
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,

@@ -86,8 +86,8 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
     bool timeStages = false;
     float dilation = 0.3F;
     bool filter3d = false;
-    uint32_t blendBatch = 128;          // records each quadrant's LDS list holds: 128 or 256
-    bool heaviestTilesFirst = false;    // launch the tiles with the most pairs first
+    uint32_t blendBatch = 256;          // records each quadrant's LDS list holds: 128 or 256
+    bool heaviestTilesFirst = true;     // launch the tiles with the most pairs first
     bool persistentBlend = false;       // batches: a persistent blend taking quadrants from a queue
     bool useGraph = false;              // batches: replay the launch sequence as a captured hipGraph
 };
