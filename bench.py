@@ -202,32 +202,11 @@ def fwd_bwd_leg(torch, raster, scenes, steps):
             "data": "synthetic: seeded FP32 parameters, target from means+N(0,0.01), logits+N(0,0.2), SH+N(0,0.05)"}
 
 
-def views_leg(torch, raster, scenes, reps):
-    """Config 4: one 2,000,000-splat clustered scene, 64 cameras on a sphere of radius
-    12 looking at its centre, each view's size drawn from {256x256, 512x384, 960x540,
-    1280x720, 1920x1080}. The batch (one launch sequence, autotuned) against the plain
-    sequence of 64 single-view frames."""
-    import math
-
+def _batch_against_sequence(torch, raster, rast, views, inst, settings, reps):
+    """One autotuned batch of the views against the plain sequence of single-view
+    frames of the same views; the batch's eager stage times."""
     import numpy as np
-    cfg = scenes.config(2, count=2_000_000)
-    dev = cfg.scene.to_device()
-    inst = [raster.SplatInstance(dev)]
-    rng = np.random.default_rng(4)
-    sizes = [(256, 256), (512, 384), (960, 540), (1280, 720), (1920, 1080)]
-    views = []
-    golden = math.pi * (3.0 - math.sqrt(5.0))
-    for k in range(64):
-        y = 1.0 - 2.0 * (k + 0.5) / 64.0
-        r = math.sqrt(1.0 - y * y)
-        eye = (12.0 * r * math.cos(golden * k), 12.0 * y, 12.0 * r * math.sin(golden * k))
-        up = (0.0, 0.0, 1.0) if abs(y) > 0.95 else (0.0, 1.0, 0.0)
-        cam = raster.Camera.looking_at(eye, (0.0, 0.0, 0.0), up)
-        cam.lens = raster.Lens(focal=scenes.focal_for_fov(60.0))
-        w, h = sizes[int(rng.integers(0, len(sizes)))]
-        views.append(raster.View(cam, raster.RenderTargets(w, h)))
-    settings = raster.RenderSettings(dilation=cfg.dilation, filter3d=cfg.filter3d)
-    rast = raster.TileRasterizer()
+    from lucabrtrender_amd import _abi
     tuned, _ = rast.autotune_views(views, inst, settings)
     rast.render_views(views, inst, tuned)
     torch.cuda.synchronize()
@@ -247,20 +226,59 @@ def views_leg(torch, raster, scenes, reps):
             rast.render(v.camera, inst, s, v.targets)
     torch.cuda.synchronize()
     seq_s = (time.perf_counter() - start) / reps
-    pixels = sum(v.targets.width * v.targets.height for v in views)
-    from lucabrtrender_amd import _abi
     staged = raster.RenderSettings(**{**tuned.__dict__, "use_graph": False, "time_stages": True})
     acc = np.zeros(8)
     for _ in range(reps):
         rast.render_views(views, inst, staged)
         acc += rast.debug(_abi.DEBUG_KERNEL_MS)
-    stage_ms = {name: round(float(acc[k] / reps), 3) for k, name in enumerate(STAGES)}
-    return {"workload": "views-fwd-2M-64", "splats": 2_000_000, "views": 64, "pixels": pixels, "pairs": stats.pairs,
-            "batch_ms": round(batch_s * 1e3, 3), "views_per_s_batched": round(64 / batch_s, 1),
-            "sequence_ms": round(seq_s * 1e3, 3), "views_per_s_single": round(64 / seq_s, 1),
-            "speedup": round(seq_s / batch_s, 3), "stage_ms_eager": stage_ms,
+    n = len(views)
+    return {"views": n, "pixels": sum(v.targets.width * v.targets.height for v in views), "pairs": stats.pairs,
+            "batch_ms": round(batch_s * 1e3, 3), "views_per_s_batched": round(n / batch_s, 1),
+            "sequence_ms": round(seq_s * 1e3, 3), "views_per_s_single": round(n / seq_s, 1),
+            "speedup": round(seq_s / batch_s, 3),
+            "stage_ms_eager": {name: round(float(acc[k] / reps), 3) for k, name in enumerate(STAGES)},
             "tuned": {"blend_batch": tuned.blend_batch, "persistent_blend": tuned.persistent_blend,
                       "use_graph": tuned.use_graph}}
+
+
+def views_leg(torch, raster, scenes, reps):
+    """Config 4: one 2,000,000-splat clustered scene, 64 cameras on a sphere of radius
+    12 looking at its centre, each view's size drawn from {256x256, 512x384, 960x540,
+    1280x720, 1920x1080}. The batch (one launch sequence, autotuned) against the plain
+    sequence of 64 single-view frames."""
+    import math
+
+    import numpy as np
+    cfg = scenes.config(2, count=2_000_000)
+    inst = [raster.SplatInstance(cfg.scene.to_device())]
+    rng = np.random.default_rng(4)
+    sizes = [(256, 256), (512, 384), (960, 540), (1280, 720), (1920, 1080)]
+    views = []
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    for k in range(64):
+        y = 1.0 - 2.0 * (k + 0.5) / 64.0
+        r = math.sqrt(1.0 - y * y)
+        eye = (12.0 * r * math.cos(golden * k), 12.0 * y, 12.0 * r * math.sin(golden * k))
+        up = (0.0, 0.0, 1.0) if abs(y) > 0.95 else (0.0, 1.0, 0.0)
+        cam = raster.Camera.looking_at(eye, (0.0, 0.0, 0.0), up)
+        cam.lens = raster.Lens(focal=scenes.focal_for_fov(60.0))
+        w, h = sizes[int(rng.integers(0, len(sizes)))]
+        views.append(raster.View(cam, raster.RenderTargets(w, h)))
+    settings = raster.RenderSettings(dilation=cfg.dilation, filter3d=cfg.filter3d)
+    out = _batch_against_sequence(torch, raster, raster.TileRasterizer(), views, inst, settings, reps)
+    return {"workload": "views-fwd-2M-64", "splats": 2_000_000, **out}
+
+
+def views_1080p_leg(torch, raster, scenes, cfg, dev, reps):
+    """The batching target's own workload: the benchmark's 3,000,000-splat scene seen
+    by 64 cameras at 1920x1080 on its radius-15 orbit, every 5.625 degrees, as one
+    batch against the plain sequence of 64 single-view frames."""
+    inst = [raster.SplatInstance(dev)]
+    views = [raster.View(scenes.orbit_camera(15.0, 360.0 * k / 64.0), raster.RenderTargets(cfg.width, cfg.height))
+             for k in range(64)]
+    settings = raster.RenderSettings(dilation=cfg.dilation, filter3d=cfg.filter3d)
+    out = _batch_against_sequence(torch, raster, raster.TileRasterizer(), views, inst, settings, reps)
+    return {"workload": "views-fwd-3M-1080p-64", "splats": cfg.scene.count, **out}
 
 
 def sort_stress_leg(torch, raster, scenes, _abi, reps):
@@ -456,6 +474,10 @@ def main():
     if args.views_reps > 0:
         log("batched views (2M splats, 64 views) ...")
         batched = views_leg(torch, raster, scenes, args.views_reps)
+    batched_1080p = None
+    if args.views_reps > 0:
+        log("batched views (3M splats, 64 views at 1080p) ...")
+        batched_1080p = views_1080p_leg(torch, raster, scenes, cfg, dev, max(2, args.views_reps // 2))
     stress = None
     if args.stress_reps > 0:
         log("sort stress (6M splats, 2160p) ...")
@@ -494,6 +516,7 @@ def main():
         "copy_GB/s": round(copy_gbs, 1),
         "fwd_bwd": fwd_bwd,
         "views": batched,
+        "views_3m_1080p": batched_1080p,
         "sort_stress": stress,
         "load": load,
         "roofline": roofline,
