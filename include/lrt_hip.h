@@ -183,6 +183,58 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
                              const lrt_render_settings* settings, lrt_render_targets* targets,
                              lrt_frame_stats* stats);
 
+/* ---- points as opaque discs, and the under-layer ------------------------------------ */
+
+/* scene::GpuPoints (modules/scene/include/lrt/scene/GpuClouds.h:78-85). Device
+ * pointers.
+ *   positions  float4 a point: x, y, z, 1
+ *   colours    2 u32 a point: f16 r | g << 16, b | a << 16 (linear), 8-byte
+ *              aligned; may be NULL when the style's constant_colour is set */
+typedef struct {
+    const float* positions;
+    const uint32_t* colours;
+    uint32_t count;
+} lrt_points;
+
+/* render::PointStyle (modules/render/include/lrt/render/Points.h:21-33), the
+ * fields the disc route reads; the eye-dome and surface-splatting fields
+ * belong to the reference's raster route (PointRasterizer), which this
+ * library does not have. lrt_point_style_default fills the reference's
+ * defaults: world units, 0.01, the cloud's own colours, white. */
+typedef struct {
+    uint32_t size_mode;          /* 0 world units, 1 pixels */
+    float size;                  /* diameter */
+    int32_t constant_colour;     /* 1: colour for every point */
+    float colour[3];
+} lrt_point_style;
+
+/* render::PointInstance (Points.h:35-39). */
+typedef struct {
+    const lrt_points* points;
+    double object_to_world[16];
+    lrt_point_style style;
+} lrt_point_instance;
+
+void lrt_point_style_default(lrt_point_style* out);
+
+/* TileRasterizer::render(projection, instances, settings, targets, points,
+ * under) (TileRasterizer.h:143-155, TileRasterizer.cpp:188-484 with :355-377
+ * and :464-475): point instances drawn as opaque discs (points_project.slang:
+ * 33-86) in the same depth order as the splats, their slots after the
+ * splats'; `under` (may be NULL; the frame's size, colour and depth both set,
+ * device memory as the targets') is what lies under the splats -- each
+ * pixel's walk ends at the first pair at or behind its under depth (when that
+ * is above 0) and the under colour lies behind. Points or an under-layer
+ * select the composite blend (splat_blend.slang:23-120); with neither this is
+ * lrt_raster_render. A composite frame keeps no per-pixel walk state
+ * (LRT_DEBUG_LAST_COUNT and LRT_DEBUG_FINAL_T are empty after it).
+ * Synchronous; stats may be NULL. */
+lrt_status lrt_raster_render_composite(lrt_context* ctx, const lrt_projection* projection,
+                                       const lrt_splat_instance* instances, uint32_t instance_count,
+                                       const lrt_point_instance* points, uint32_t point_count,
+                                       const lrt_render_targets* under, const lrt_render_settings* settings,
+                                       lrt_render_targets* targets, lrt_frame_stats* stats);
+
 /* ---- the brute force in FP64 -------------------------------------------------------
  * ReferenceRenderer (modules/render/src/ReferenceRenderer.cpp:54-160;
  * reference_project.slang:31-105, reference_blend.slang:129-223) on the GPU in

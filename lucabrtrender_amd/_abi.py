@@ -76,6 +76,19 @@ class SplatInstanceC(C.Structure):
     _fields_ = [("splats", C.POINTER(Splats)), ("object_to_world", C.c_double * 16), ("edit", SplatEdit)]
 
 
+class PointsC(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("colours", C.c_void_p), ("count", C.c_uint32)]
+
+
+class PointStyleC(C.Structure):
+    _fields_ = [("size_mode", C.c_uint32), ("size", C.c_float), ("constant_colour", C.c_int32),
+                ("colour", C.c_float * 3)]
+
+
+class PointInstanceC(C.Structure):
+    _fields_ = [("points", C.POINTER(PointsC)), ("object_to_world", C.c_double * 16), ("style", PointStyleC)]
+
+
 class RenderSettingsC(C.Structure):
     _fields_ = [
         ("width", C.c_uint32), ("height", C.c_uint32),
@@ -161,6 +174,11 @@ SIGNATURES = {
     "lrt_last_error": (C.c_char_p, [C.c_void_p]),
     "lrt_raster_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
                                     C.POINTER(RenderSettingsC), C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
+    "lrt_point_style_default": (None, [C.POINTER(PointStyleC)]),
+    "lrt_raster_render_composite": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC),
+                                              C.c_uint32, C.POINTER(PointInstanceC), C.c_uint32,
+                                              C.POINTER(RenderTargetsC), C.POINTER(RenderSettingsC),
+                                              C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
     "lrt_reference_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
                                        C.POINTER(RenderSettingsC), C.c_void_p, C.c_void_p, C.c_int32]),
     "lrt_raster_render_views": (C.c_int, [C.c_void_p, C.POINTER(ViewC), C.c_uint32, C.POINTER(SplatInstanceC),

@@ -318,11 +318,15 @@ const char* lrt_last_error(const lrt_context* ctx) {
     return ctx != nullptr ? ctx->error.c_str() : gNoContextError.c_str();
 }
 
-lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection, const lrt_splat_instance* instances,
-                             uint32_t instance_count, const lrt_render_settings* settings, lrt_render_targets* targets,
-                             lrt_frame_stats* stats) {
+namespace {
+
+lrt_status renderFrame(lrt_context* ctx, const lrt_projection* projection, const lrt_splat_instance* instances,
+                       uint32_t instance_count, const lrt_point_instance* points, uint32_t point_count,
+                       const lrt_render_targets* under, const lrt_render_settings* settings,
+                       lrt_render_targets* targets, lrt_frame_stats* stats) {
     if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
-    if (projection == nullptr || settings == nullptr || targets == nullptr || (instance_count > 0 && instances == nullptr)) {
+    if (projection == nullptr || settings == nullptr || targets == nullptr || (instance_count > 0 && instances == nullptr) ||
+        (point_count > 0 && points == nullptr)) {
         return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
     }
     std::vector<lrt::SplatCloud> clouds(instance_count);
@@ -335,13 +339,41 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
         list[k].objectToWorld = toMat(instances[k].object_to_world);
         list[k].edit = toEdit(instances[k].edit);
     }
+    std::vector<lrt::PointCloud> pointClouds(point_count);
+    std::vector<lrt::PointInstance> pointList(point_count);
+    for (uint32_t k = 0; k < point_count; ++k) {
+        const lrt_point_instance& in = points[k];
+        if (in.points != nullptr) {
+            pointClouds[k].positions = reinterpret_cast<const float4*>(in.points->positions);
+            pointClouds[k].colours = in.points->colours;
+            pointClouds[k].count = in.points->count;
+            pointList[k].points = &pointClouds[k];
+        }
+        pointList[k].objectToWorld = toMat(in.object_to_world);
+        if (in.style.size_mode > 1) {
+            return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "size_mode must be 0 or 1"));
+        }
+        pointList[k].style.sizeMode = in.style.size_mode == 1 ? lrt::PointStyle::Size::Pixels
+                                                              : lrt::PointStyle::Size::World;
+        pointList[k].style.size = in.style.size;
+        pointList[k].style.constantColour = in.style.constant_colour != 0;
+        for (int c = 0; c < 3; ++c) pointList[k].style.colour[static_cast<size_t>(c)] = in.style.colour[c];
+    }
     lrt::RenderTargets t;
     t.width = targets->width;
     t.height = targets->height;
     t.colour = reinterpret_cast<float4*>(targets->colour);
     t.depth = targets->depth;
+    lrt::RenderTargets u;
+    if (under != nullptr) {
+        u.width = under->width;
+        u.height = under->height;
+        u.colour = reinterpret_cast<float4*>(under->colour);
+        u.depth = under->depth;
+    }
     lrt::FrameStats fs;
-    const lrt::Status s = ctx->raster.render(toProjection(*projection), list, toSettings(*settings), t, &fs);
+    const lrt::Status s = ctx->raster.render(toProjection(*projection), list, toSettings(*settings), t, &fs, pointList,
+                                             under != nullptr ? &u : nullptr);
     if (s.ok() && stats != nullptr) {
         stats->splats = fs.splats;
         stats->visible = fs.visible;
@@ -355,6 +387,33 @@ lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection,
         stats->total_ms = fs.totalMs;
     }
     return finish(ctx, s);
+}
+
+}   // namespace
+
+lrt_status lrt_raster_render(lrt_context* ctx, const lrt_projection* projection, const lrt_splat_instance* instances,
+                             uint32_t instance_count, const lrt_render_settings* settings, lrt_render_targets* targets,
+                             lrt_frame_stats* stats) {
+    return renderFrame(ctx, projection, instances, instance_count, nullptr, 0, nullptr, settings, targets, stats);
+}
+
+lrt_status lrt_raster_render_composite(lrt_context* ctx, const lrt_projection* projection,
+                                       const lrt_splat_instance* instances, uint32_t instance_count,
+                                       const lrt_point_instance* points, uint32_t point_count,
+                                       const lrt_render_targets* under, const lrt_render_settings* settings,
+                                       lrt_render_targets* targets, lrt_frame_stats* stats) {
+    return renderFrame(ctx, projection, instances, instance_count, points, point_count, under, settings, targets,
+                       stats);
+}
+
+void lrt_point_style_default(lrt_point_style* out) {
+    if (out == nullptr) return;
+    const lrt::PointStyle st;
+    std::memset(out, 0, sizeof *out);
+    out->size_mode = st.sizeMode == lrt::PointStyle::Size::Pixels ? 1u : 0u;
+    out->size = st.size;
+    out->constant_colour = st.constantColour ? 1 : 0;
+    for (int c = 0; c < 3; ++c) out->colour[c] = st.colour[static_cast<size_t>(c)];
 }
 
 

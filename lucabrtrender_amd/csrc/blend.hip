@@ -30,6 +30,14 @@
 // with the transmittance, stop below 1e-4, then linearise rgb / coverage. The
 // Gaussian's exponent runs on the hardware exp2 and the quadratic form on
 // FMAs; both move the result by a few ulps, which the image tolerances cover.
+//
+// The composite walk (splatBlendComposite, :116-120) is the same kernel
+// instantiated with COMPOSITE: a disc record (colour.w 1) that reaches a pixel
+// ends its walk and lies behind what it blended; a pixel with an under-layer
+// depth above 0 ends its walk at the first pair of its tile at or behind that
+// depth. That first pair may be one the quadrant's list dropped, so each staged
+// record carries the largest view z of every pair up to it (a wave max-scan at
+// staging), and the walk tests that instead of the record's own z.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -69,6 +77,38 @@ __device__ __forceinline__ void finishPixel(const FrameParams& f, uint32_t px, u
     if (finalT != nullptr) finalT[at] = t;
 }
 
+// The finished pixel of the composite walk (reference splat_blend.slang:83-107):
+// without a disc, the under-layer lies behind; whatever lies behind is
+// composited under the linearised splats and weighs in the depth with the
+// transmittance left.
+__device__ __forceinline__ void finishComposite(const FrameParams& f, uint32_t at, float r, float g, float b, float t,
+                                                float depthSum, float depthAt, bool opaqueHit, float4 opaque,
+                                                float opaqueZ, bool underHit, const float4* __restrict__ underColour,
+                                                float underZ, float4* __restrict__ colour, float* __restrict__ depth) {
+    if (!opaqueHit && underHit) {
+        opaqueHit = true;
+        opaque = underColour[at];
+        opaqueZ = underZ;
+    }
+    const float coverage = 1.0f - t;
+    if (f.linearise && coverage > 1e-6f) {
+        r = srgbToLinear(r / coverage) * coverage;
+        g = srgbToLinear(g / coverage) * coverage;
+        b = srgbToLinear(b / coverage) * coverage;
+    }
+    float4 behind = make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3]);
+    if (opaqueHit) {
+        behind = opaque;
+        depthSum = depthSum + t * opaqueZ;
+        if (depthAt < 0.0f) depthAt = opaqueZ;
+    }
+    const float total = opaqueHit ? 1.0f : coverage;
+    colour[at] = make_float4(r + t * behind.x, g + t * behind.y, b + t * behind.z, coverage + t * behind.w);
+    if (depth != nullptr) {
+        depth[at] = f.depthMode == 0 ? (total > 1e-6f ? depthSum / total : 0.0f) : fmaxS(depthAt, 0.0f);
+    }
+}
+
 // One pair's projected record, or an empty one (cutoff -1) past the range.
 struct Staged {
     float4 screen;   // centre.xy, view z, cutoff
@@ -91,10 +131,12 @@ __device__ __forceinline__ Staged gather(const float4* __restrict__ proj, uint32
 // Quadrant `quadrant` of tile `tile` of the frame f, whose pairs are
 // [range.x, range.y) of pairSlots; colour and depth are the frame's images,
 // lastCount/finalT/accum its per-pixel state (each may be null).
-template <uint32_t DEPTH_MODE, uint32_t LIST>
+template <uint32_t DEPTH_MODE, uint32_t LIST, bool COMPOSITE>
 __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t tile, uint32_t quadrant, uint2 range,
                                               uint32_t slots, const uint32_t* __restrict__ pairSlots,
-                                              const float4* __restrict__ proj, float4* __restrict__ colour,
+                                              const float4* __restrict__ proj,
+                                              const float4* __restrict__ underColour,
+                                              const float* __restrict__ underDepth, float4* __restrict__ colour,
                                               float* __restrict__ depth, uint32_t* __restrict__ lastCount,
                                               float* __restrict__ finalT, float4* __restrict__ accum) {
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
@@ -102,7 +144,9 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
     // multiple of kGroup with empty records (cutoff -1: never pass the test).
     __shared__ float4 sGeom[LIST + kGroup];     // A, B, C, D of the expanded form
     __shared__ float4 sConic[LIST + kGroup];    // E, F less the cutoff, o', view z
-    __shared__ float4 sColour[LIST + kGroup];   // rgb, bits of the pair's place in the tile's range + 1
+    // rgb, then bits of the pair's place in the tile's range + 1; composite:
+    // the largest view z of the tile's pairs up to this one, kept or not
+    __shared__ float4 sColour[LIST + kGroup];
 
     const uint32_t lane = threadIdx.x;
     const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
@@ -130,6 +174,19 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
     float r = 0.0f, g = 0.0f, b = 0.0f, t = 1.0f, depthSum = 0.0f, depthAt = -1.0f;
     uint32_t last = 0;
     bool active = inside;
+    // Composite: what ends the walk from behind. A pixel with an under depth
+    // above 0 stops at the first pair at or behind it (whether or not that pair
+    // reaches the pixel: the reference tests the depth first), a disc that
+    // reaches the pixel stops it with its colour.
+    const uint32_t at0 = py * f.width + px;
+    float underZ = 0.0f;
+    if constexpr (COMPOSITE) {
+        if (underDepth != nullptr && inside) underZ = underDepth[at0];
+    }
+    const bool underHit = COMPOSITE && underZ > 0.0f;
+    bool opaqueHit = false;
+    float oR = 0.0f, oG = 0.0f, oB = 0.0f, oZ = 0.0f;
+    float zCarry = -3.4e38f;   // the largest view z of the pairs staged so far
     uint32_t next = range.x;   // the first pair of the round in `staged`
     while (next < range.y) {
         // 1-2. stage rounds into the list while a whole round still fits
@@ -148,6 +205,18 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                                   quadrantReach(s.conic.x, s.conic.y, s.conic.z, s.screen.w, boxLoX - s.screen.x,
                                                 boxLoY - s.screen.y, boxHiX - s.screen.x, boxHiY - s.screen.y);
                 const uint64_t vote = __ballot(keep);
+                float zReach = 0.0f;
+                if constexpr (COMPOSITE) {
+                    // inclusive max-scan of the round's view z in pair order
+                    float zr = at < range.y ? s.screen.z : -3.4e38f;
+#pragma unroll
+                    for (uint32_t d = 1; d < kLanes; d <<= 1) {
+                        const float o = __shfl_up(zr, d, 64);
+                        zr = lane >= d ? fmaxS(zr, o) : zr;
+                    }
+                    zReach = fmaxS(zr, zCarry);
+                    zCarry = shflF(zReach, kLanes - 1);
+                }
                 if (keep) {
                     const uint32_t to = fill + static_cast<uint32_t>(__popcll(vote & below));
                     const float ux = s.screen.x - xm, vy = s.screen.y - ym;
@@ -157,8 +226,12 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                     const float fd = -(2.0f * fa * ux + fb * vy), fe = -(2.0f * fc * vy + fb * ux);
                     const float ff = (ux * (fa * ux + fb * vy) + fc * vy * vy) - scaledCut;
                     sGeom[to] = make_float4(fa, fb, fc, fd);
-                    sConic[to] = make_float4(fe, ff, s.conic.w * __builtin_amdgcn_exp2f(scaledCut), s.screen.z);
-                    sColour[to] = make_float4(s.col.x, s.col.y, s.col.z, __uint_as_float(at - range.x + 1u));
+                    // a disc (colour.w 1, composite only) is marked by o' < 0: its alpha never passes
+                    const bool disc = COMPOSITE && s.col.w > 0.5f;
+                    const float scaledOpacity = disc ? -1.0f : s.conic.w * __builtin_amdgcn_exp2f(scaledCut);
+                    sConic[to] = make_float4(fe, ff, scaledOpacity, s.screen.z);
+                    sColour[to] = make_float4(s.col.x, s.col.y, s.col.z,
+                                              COMPOSITE ? zReach : __uint_as_float(at - range.x + 1u));
                 }
                 fill += static_cast<uint32_t>(__popcll(vote));
             }
@@ -170,7 +243,7 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
         if (lane < padded - fill) {
             sGeom[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
             sConic[fill + lane] = make_float4(0.0f, -1e30f, 0.0f, 0.0f);   // q far below 0: never passes
-            sColour[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+            sColour[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, COMPOSITE ? -3.4e38f : 0.0f);
         }
         __syncthreads();
         // 3. walk the list: all kGroup records' reads issued together, then the
@@ -188,6 +261,7 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
             }
 #pragma unroll
             for (uint32_t k = 0; k < kGroup; ++k) {
+                if constexpr (COMPOSITE) active = active & !(underHit & (g2[k].w >= underZ));
                 const float q = __builtin_fmaf(pu, __builtin_fmaf(g0[k].x, pu, __builtin_fmaf(g0[k].y, pv, g0[k].w)),
                                                __builtin_fmaf(pv, __builtin_fmaf(g0[k].z, pv, g1[k].x), g1[k].y));
                 const float alpha = __builtin_fminf(0.99f, g1[k].z * __builtin_amdgcn_exp2f(q));
@@ -200,9 +274,18 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                 b = __builtin_fmaf(weight, g2[k].z, b);
                 depthSum = __builtin_fmaf(weight, g1[k].w, depthSum);
                 t = t - weight;   // T (1 - alpha)
-                last = ok ? __float_as_uint(g2[k].w) : last;
+                if constexpr (!COMPOSITE) last = ok ? __float_as_uint(g2[k].w) : last;
                 if constexpr (DEPTH_MODE == 1) {
                     depthAt = (ok && depthAt < 0.0f && 1.0f - t >= f.depthThreshold) ? g1[k].w : depthAt;
+                }
+                if constexpr (COMPOSITE) {
+                    const bool hit = active & (q >= 0.0f) & (g1[k].z < 0.0f);
+                    oR = hit ? g2[k].x : oR;
+                    oG = hit ? g2[k].y : oG;
+                    oB = hit ? g2[k].z : oB;
+                    oZ = hit ? g1[k].w : oZ;
+                    opaqueHit = opaqueHit | hit;
+                    active = active & !hit;
                 }
                 active = active && !(t < 1e-4f);
             }
@@ -212,14 +295,21 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
         __syncthreads();   // the list is refilled next
     }
     if (!inside) return;
-    finishPixel(f, px, py, r, g, b, t, depthSum, depthAt, last, colour, depth, lastCount, finalT, accum);
+    if constexpr (COMPOSITE) {
+        finishComposite(f, at0, r, g, b, t, depthSum, depthAt, opaqueHit, make_float4(oR, oG, oB, 1.0f), oZ,
+                        underHit, underColour, underZ, colour, depth);
+    } else {
+        finishPixel(f, px, py, r, g, b, t, depthSum, depthAt, last, colour, depth, lastCount, finalT, accum);
+    }
 }
 
-template <uint32_t DEPTH_MODE, uint32_t LIST>
+template <uint32_t DEPTH_MODE, uint32_t LIST, bool COMPOSITE>
 __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t slots, const uint32_t* __restrict__ order,
                                                       const uint2* __restrict__ ranges,
                                                       const uint32_t* __restrict__ pairSlots,
                                                       const float4* __restrict__ proj,
+                                                      const float4* __restrict__ underColour,
+                                                      const float* __restrict__ underDepth,
                                                       float4* __restrict__ colour, float* __restrict__ depth,
                                                       uint32_t* __restrict__ lastCount,
                                                       float* __restrict__ finalT, float4* __restrict__ accum) {
@@ -229,8 +319,8 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     if (place >= tiles) return;
     const uint32_t tile = order != nullptr ? order[place] : place;
     if (tile >= tiles) return;   // an order entry that is not a tile: draw nothing
-    blendQuadrant<DEPTH_MODE, LIST>(f, tile, quadrant, ranges[tile], slots, pairSlots, proj, colour, depth,
-                                    lastCount, finalT, accum);
+    blendQuadrant<DEPTH_MODE, LIST, COMPOSITE>(f, tile, quadrant, ranges[tile], slots, pairSlots, proj, underColour,
+                                               underDepth, colour, depth, lastCount, finalT, accum);
 }
 
 // The view holding batch tile g: the last view whose tiles start at or before it.
@@ -274,8 +364,9 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
         }
         if (g < tiles) {
             const ViewInfo& v = views[viewOfTile(views, viewCount, g)];
-            blendQuadrant<DEPTH_MODE, LIST>(params[v.params], g - v.tileBase, quadrant, ranges[g], slots, pairSlots,
-                                            proj, reinterpret_cast<float4*>(v.colour), v.depth,
+            blendQuadrant<DEPTH_MODE, LIST, false>(params[v.params], g - v.tileBase, quadrant, ranges[g], slots,
+                                                   pairSlots, proj, nullptr, nullptr,
+                                                   reinterpret_cast<float4*>(v.colour), v.depth,
                                             lastCount + v.pixelBase, finalT + v.pixelBase, nullptr);
         }
         if constexpr (!PERSISTENT) return;
@@ -345,19 +436,22 @@ __global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2*
     }
 }
 
-template <uint32_t DEPTH_MODE>
+template <uint32_t DEPTH_MODE, bool COMPOSITE>
 void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint32_t* order, const uint2* ranges,
-                 const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-                 float* finalT, float4* accum, hipStream_t stream) {
+                 const uint32_t* pairSlots, const float4* proj, const float4* underColour, const float* underDepth,
+                 float4* colour, float* depth, uint32_t* lastCount, float* finalT, float4* accum,
+                 hipStream_t stream) {
     // Four blocks a tile, the grid rounded up to whole groups of eight tiles.
     const uint32_t tiles = f.tilesX * f.tilesY;
     const uint32_t blocks = (tiles + 7u) / 8u * 32u;
     if (list <= 128) {
-        blendKernel<DEPTH_MODE, 128><<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                    depth, lastCount, finalT, accum);
+        blendKernel<DEPTH_MODE, 128, COMPOSITE><<<blocks, kLanes, 0, stream>>>(
+            f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour, depth, lastCount, finalT,
+            accum);
     } else {
-        blendKernel<DEPTH_MODE, 256><<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, colour,
-                                                                    depth, lastCount, finalT, accum);
+        blendKernel<DEPTH_MODE, 256, COMPOSITE><<<blocks, kLanes, 0, stream>>>(
+            f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour, depth, lastCount, finalT,
+            accum);
     }
 }
 
@@ -414,11 +508,24 @@ void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* 
            float* finalT, float4* accum, hipStream_t stream) {
     if (f.tilesX * f.tilesY == 0) return;
     if (f.depthMode == 0) {
-        launchBlend<0>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, accum,
-                       stream);
+        launchBlend<0, false>(list, f, slots, order, ranges, pairSlots, proj, nullptr, nullptr, colour, depth,
+                              lastCount, finalT, accum, stream);
     } else {
-        launchBlend<1>(list, f, slots, order, ranges, pairSlots, proj, colour, depth, lastCount, finalT, accum,
-                       stream);
+        launchBlend<1, false>(list, f, slots, order, ranges, pairSlots, proj, nullptr, nullptr, colour, depth,
+                              lastCount, finalT, accum, stream);
+    }
+}
+
+void blendComposite(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
+                    const uint32_t* pairSlots, const float4* proj, const float4* underColour,
+                    const float* underDepth, float4* colour, float* depth, hipStream_t stream) {
+    if (f.tilesX * f.tilesY == 0) return;
+    if (f.depthMode == 0) {
+        launchBlend<0, true>(list, f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour, depth,
+                             nullptr, nullptr, nullptr, stream);
+    } else {
+        launchBlend<1, true>(list, f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour, depth,
+                             nullptr, nullptr, nullptr, stream);
     }
 }
 

@@ -40,6 +40,16 @@ struct FrameParams {
     SplatEditParams edit;
 };
 
+// How a point cloud's discs are sized and coloured: the reference's
+// PointStyle as its disc route reads it (modules/render/include/lrt/render/
+// Points.h:21-33; shaders/lrt/points/point_frame.slang:7-44).
+struct PointParams {
+    uint32_t sizeMode;         // 0: size in world units, 1: in pixels
+    float size;                // diameter
+    uint32_t constantColour;   // 1: colour below for every point; 0: the cloud's own
+    float colour[3];
+};
+
 // One camera of a batched frame as the duplication and the blend see it. A
 // batch's views share one slot space (view v's splats at v * splatsPerView)
 // and one tile space (view v's tiles from tileBase); pixelBase places the

@@ -36,6 +36,12 @@ void project(const FrameParams& f, const float4* positions, const uint4* shape, 
 void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,
                   const uint4* shape, const uint32_t* sh, const float* filter3d, float4* proj, uint2* rects,
                   uint32_t* touched, hipStream_t stream);
+// One point instance as opaque discs: f.count points (positions float4 xyz 1;
+// colours two words a point, f16 r | g, b | a, 8-byte aligned, read only
+// without style.constantColour) into slots f.base + i, in the same record,
+// rectangle and count layout as a splat's, colour.w = 1.
+void projectPoints(const FrameParams& f, const PointParams& style, const float4* positions, const uint32_t* colours,
+                   float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream);
 
 // ---- duplication ------------------------------------------------------------
 // For each of `slots` slots with touched > 0: its pairs at offsets[slot].., keys
@@ -119,6 +125,14 @@ void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
            float* finalT, float4* accum, hipStream_t stream);
+// The composite walk: a record with colour.w = 1 (a disc) that reaches a pixel
+// ends its walk with its colour behind the splats; with underDepth non-null a
+// pixel whose under depth is above 0 ends its walk at the first record at or
+// behind that depth, and the under-layer's colour (float4 a pixel) lies behind
+// what it blended. Writes colour and depth (may be null) only.
+void blendComposite(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
+                    const uint32_t* pairSlots, const float4* proj, const float4* underColour,
+                    const float* underDepth, float4* colour, float* depth, hipStream_t stream);
 
 // A batch of views in one launch (device FrameParams and ViewInfo arrays;
 // views sorted by tileBase, `tiles` in all): as a grid of quadrant tasks, or

@@ -75,6 +75,26 @@ struct SplatInstance {   // render::SplatInstance without relighting
     SplatEditParams edit;
 };
 
+struct PointCloud {   // scene::GpuPoints (GpuClouds.h:78-85), device pointers the caller owns
+    const float4* positions = nullptr;   // x, y, z, 1
+    const uint32_t* colours = nullptr;   // two words a point: f16 r | g, b | a (linear); 8-byte aligned
+    uint32_t count = 0;
+};
+
+struct PointStyle {   // render::PointStyle (Points.h:21-33), the fields the disc route reads
+    enum class Size { World, Pixels };
+    Size sizeMode = Size::World;
+    float size = 0.01F;                 // diameter, world units or pixels
+    bool constantColour = false;
+    std::array<float, 3> colour{1.0F, 1.0F, 1.0F};
+};
+
+struct PointInstance {   // render::PointInstance (Points.h:35-39)
+    const PointCloud* points = nullptr;
+    Mat4 objectToWorld;
+    PointStyle style;
+};
+
 struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
     uint32_t width = 1920, height = 1080;
     bool antialias = true;
@@ -145,11 +165,19 @@ public:
     void setStream(hipStream_t stream) { stream_ = stream; }
     [[nodiscard]] hipStream_t stream() const noexcept { return stream_; }
 
+    // TileRasterizer::render (TileRasterizer.cpp:188-484). Point instances are
+    // drawn as opaque discs sorted with the splats (their slots after the
+    // splats'); `under` (the frame's size, colour and depth both set) is what
+    // lies under the splats and ends each pixel's walk at its depth. Either
+    // one selects the composite blend.
     Status render(const Projection& projection, std::span<const SplatInstance> instances,
-                  const RenderSettings& settings, RenderTargets& targets, FrameStats* stats);
+                  const RenderSettings& settings, RenderTargets& targets, FrameStats* stats,
+                  std::span<const PointInstance> points = {}, const RenderTargets* under = nullptr);
     Status render(const Camera& camera, std::span<const SplatInstance> instances, const RenderSettings& settings,
-                  RenderTargets& targets, FrameStats* stats) {
-        return render(projectionFor(camera, settings.width, settings.height), instances, settings, targets, stats);
+                  RenderTargets& targets, FrameStats* stats, std::span<const PointInstance> points = {},
+                  const RenderTargets* under = nullptr) {
+        return render(projectionFor(camera, settings.width, settings.height), instances, settings, targets, stats,
+                      points, under);
     }
 
     // Many views of the same instances in one launch sequence: one projection
@@ -214,9 +242,11 @@ private:
     // The launch sequence of a batch (no waits), on `stream`.
     void enqueueViews(std::span<const SplatInstance> instances, uint32_t viewCount, uint32_t splatsPerView,
                       uint32_t tiles, const RenderSettings& settings, hipStream_t stream);
-    // Everything after the projection: scan, duplicate, sort, ranges, blend.
+    // Everything after the projection: scan, duplicate, sort, ranges, blend
+    // (the composite blend when `composite`, over `under` when non-null).
     Status finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
-                       RenderTargets& targets, FrameStats* stats, bool training);
+                       RenderTargets& targets, FrameStats* stats, bool training, bool composite = false,
+                       const RenderTargets* under = nullptr);
     Status check(hipError_t e, const char* what);
 
     int device_ = 0;
@@ -252,6 +282,7 @@ private:
     // What the last render left.
     uint32_t lastSplats_ = 0, lastVisible_ = 0, lastPairs_ = 0, lastTiles_ = 0;
     bool lastOrdered_ = false;
+    bool lastComposite_ = false;   // the composite blend keeps no per-pixel walk state
     uint64_t lastPixels_ = 0;
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
     // Batches: the frame constants and view records (pinned host copies that

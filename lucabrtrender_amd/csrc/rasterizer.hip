@@ -170,9 +170,16 @@ Status TileRasterizer::checkFrame(const RenderSettings& settings, const RenderTa
 }
 
 Status TileRasterizer::render(const Projection& projection, std::span<const SplatInstance> instances,
-                              const RenderSettings& settings, RenderTargets& targets, FrameStats* statsOut) {
+                              const RenderSettings& settings, RenderTargets& targets, FrameStats* statsOut,
+                              std::span<const PointInstance> points, const RenderTargets* under) {
     Status s = checkFrame(settings, targets);
     if (!s.ok()) return s;
+    if (under != nullptr && (under->width != settings.width || under->height != settings.height)) {
+        return Status::error(ErrorCode::InvalidArgument, "the under-layer is not the frame's size");
+    }
+    if (under != nullptr && (under->colour == nullptr || under->depth == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "an under-layer needs both colour and depth");
+    }
     uint64_t total = 0;
     for (const SplatInstance& instance : instances) {
         if (instance.splats == nullptr) continue;
@@ -194,6 +201,20 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         }
         if (settings.filter3d && c.count > 0 && c.filter3d == nullptr) {
             return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and a cloud has no filter3d");
+        }
+        total += c.count;
+    }
+    for (const PointInstance& instance : points) {
+        if (instance.points == nullptr) continue;
+        const PointCloud& c = *instance.points;
+        if (c.count > 0 && c.positions == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a point cloud without positions");
+        }
+        if (c.count > 0 && !instance.style.constantColour && c.colours == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a point cloud without colours and no constant colour");
+        }
+        if (!(instance.style.size >= 0.0F)) {
+            return Status::error(ErrorCode::InvalidArgument, "a point size below 0");
         }
         total += c.count;
     }
@@ -221,12 +242,30 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
                       rects_.as<uint2>(), touched_.as<uint32_t>(), stream_);
         base += cloud->count;
     }
+    // Then the points, as discs, in the slots after the splats'.
+    for (const PointInstance& instance : points) {
+        const PointCloud* cloud = instance.points;
+        if (cloud == nullptr || cloud->count == 0) continue;
+        SplatInstance placed;
+        placed.objectToWorld = instance.objectToWorld;
+        FrameParams f = frameParamsFor(projection, placed, settings, base);
+        f.count = cloud->count;
+        PointParams style{};
+        style.sizeMode = instance.style.sizeMode == PointStyle::Size::Pixels ? 1u : 0u;
+        style.size = instance.style.size;
+        style.constantColour = instance.style.constantColour ? 1u : 0u;
+        for (int k = 0; k < 3; ++k) style.colour[k] = instance.style.colour[static_cast<size_t>(k)];
+        kern::projectPoints(f, style, cloud->positions, cloud->colours, proj_.as<float4>(), rects_.as<uint2>(),
+                            touched_.as<uint32_t>(), stream_);
+        base += cloud->count;
+    }
     if (!haveFirst) {
         SplatInstance none;
         first = frameParamsFor(projection, none, settings, 0);
     }
     diffReady_ = false;
-    return finishFrame(first, splatCount, settings, targets, statsOut, false);
+    const bool composite = under != nullptr || !points.empty();
+    return finishFrame(first, splatCount, settings, targets, statsOut, false, composite, under);
 }
 
 Status TileRasterizer::renderDiff(const Projection& projection, const kern::DiffCloud& cloud,
@@ -268,7 +307,8 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
 }
 
 Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
-                                   RenderTargets& targets, FrameStats* statsOut, bool training) {
+                                   RenderTargets& targets, FrameStats* statsOut, bool training, bool composite,
+                                   const RenderTargets* under) {
     const auto wallStart = std::chrono::steady_clock::now();
     const uint32_t tiles = first.tilesX * first.tilesY;
     const uint32_t keyBits = 24 + bitsFor(tiles);
@@ -332,9 +372,15 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
         kern::tileOrder(ranges_.as<uint2>(), tiles, tileOrder_.as<uint32_t>(), stream_);
         order = tileOrder_.as<uint32_t>();
     }
-    kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots, proj_.as<float4>(),
-                targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(),
-                training ? accum_.as<float4>() : nullptr, stream_);
+    if (composite) {
+        kern::blendComposite(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots,
+                             proj_.as<float4>(), under != nullptr ? under->colour : nullptr,
+                             under != nullptr ? under->depth : nullptr, targets.colour, targets.depth, stream_);
+    } else {
+        kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots,
+                    proj_.as<float4>(), targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(),
+                    training ? accum_.as<float4>() : nullptr, stream_);
+    }
     mark(8);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "frame")).ok())
         return s;
@@ -349,6 +395,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     lastPairs_ = pairs;
     lastTiles_ = tiles;
     lastOrdered_ = settings.heaviestTilesFirst;
+    lastComposite_ = composite;
     lastPixels_ = uint64_t{settings.width} * settings.height;
     lastSorted_ = sorted;
     stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
@@ -618,6 +665,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
     lastPairs_ = hostCounters_[0];
     lastTiles_ = static_cast<uint32_t>(tiles);
     lastOrdered_ = false;
+    lastComposite_ = false;
     lastPixels_ = pixels;
     if (statsOut != nullptr) {
         FrameStats stats;
@@ -909,8 +957,8 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     case 4: src = keys_[lastSorted_].get(); size = size_t{lastPairs_} * 8; break;
     case 5: src = values_[lastSorted_].get(); size = size_t{lastPairs_} * 4; break;
     case 6: src = ranges_.get(); size = size_t{lastTiles_} * 8; break;
-    case 7: src = lastCount_.get(); size = size_t{lastPixels_} * 4; break;
-    case 8: src = finalT_.get(); size = size_t{lastPixels_} * 4; break;
+    case 7: src = lastCount_.get(); size = lastComposite_ ? 0 : size_t{lastPixels_} * 4; break;
+    case 8: src = finalT_.get(); size = lastComposite_ ? 0 : size_t{lastPixels_} * 4; break;
     case 9: src = rects_.get(); size = size_t{lastSplats_} * 8; break;
     case 10: size = sizeof kernelMs_; break;
     case 11: src = tileOrder_.get(); size = lastOrdered_ ? size_t{lastTiles_} * 4 : 0; break;

@@ -174,14 +174,10 @@ __device__ __forceinline__ bool tileTouched(uint32_t width, uint32_t height, flo
     return minPowerOverBox(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
 }
 
-// The footprint's tile rectangle, inclusive (frame.slang:142-161); empty
-// (x0 > x1) when it is off screen.
-__device__ __forceinline__ void tileRect(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
-                                         float c, float cutoff, uint32_t rect[4]) {
-    const float det = a * c - b * b;
-    const float inv = det > 1e-20f ? 1.0f / det : 0.0f;
-    const float ex = __builtin_sqrtf(fmaxS(cutoff * c * inv, 0.0f));
-    const float ey = __builtin_sqrtf(fmaxS(cutoff * a * inv, 0.0f));
+// The tile rectangle, inclusive, of a footprint with half-widths ex, ey about
+// (cx, cy) (frame.slang:149-161); empty (x0 > x1) when it is off screen.
+__device__ __forceinline__ void tileRectOfExtent(uint32_t width, uint32_t height, float cx, float cy, float ex,
+                                                 float ey, uint32_t rect[4]) {
     const float x1 = cx - ex, y1 = cy - ey, x2 = cx + ex, y2 = cy + ey;
     if (x2 < 0.0f || y2 < 0.0f || x1 >= static_cast<float>(width) || y1 >= static_cast<float>(height)) {
         rect[0] = 1; rect[1] = 1; rect[2] = 0; rect[3] = 0;
@@ -191,6 +187,17 @@ __device__ __forceinline__ void tileRect(uint32_t width, uint32_t height, float 
     rect[1] = static_cast<uint32_t>(fmaxS(y1, 0.0f)) / kTile;
     rect[2] = static_cast<uint32_t>(fminS(x2, static_cast<float>(width) - 1.0f)) / kTile;
     rect[3] = static_cast<uint32_t>(fminS(y2, static_cast<float>(height) - 1.0f)) / kTile;
+}
+
+// The tile rectangle of a Gaussian's footprint: its extent from the 2D
+// covariance, sqrt(cutoff sigma_xx) and sqrt(cutoff sigma_yy) (frame.slang:142-146).
+__device__ __forceinline__ void tileRect(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
+                                         float c, float cutoff, uint32_t rect[4]) {
+    const float det = a * c - b * b;
+    const float inv = det > 1e-20f ? 1.0f / det : 0.0f;
+    const float ex = __builtin_sqrtf(fmaxS(cutoff * c * inv, 0.0f));
+    const float ey = __builtin_sqrtf(fmaxS(cutoff * a * inv, 0.0f));
+    tileRectOfExtent(width, height, cx, cy, ex, ey, rect);
 }
 
 // Tile rectangles of more than this many tiles are walked by the whole wave;

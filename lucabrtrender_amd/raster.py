@@ -211,6 +211,76 @@ class DiffSplats:
         return self._c
 
 
+class GpuPoints:
+    """scene::GpuPoints over torch device tensors (GpuClouds.h:78-85).
+
+    positions: float32 [N, 4] (x, y, z, 1); colours: 32-bit [N, 2] (f16 r | g << 16,
+    b | a << 16, linear) or None when every instance's style gives a constant colour.
+    """
+
+    def __init__(self, positions, colours=None):
+        import torch
+        n = int(positions.shape[0])
+        if positions.dtype != torch.float32 or tuple(positions.shape) != (n, 4) or not positions.is_contiguous():
+            raise LrtError(3, "positions must be a contiguous float32 [N, 4] tensor")
+        if colours is not None and (tuple(colours.shape) != (n, 2) or colours.element_size() != 4
+                                    or not colours.is_contiguous()):
+            raise LrtError(3, "colours must be a contiguous 32-bit [N, 2] tensor")
+        for t in (positions, colours):
+            if t is not None and not t.is_cuda:
+                raise LrtError(3, "point tensors must live on the GPU")
+        self.positions, self.colours, self.count = positions, colours, n
+        self._c = _abi.PointsC(positions.data_ptr(), colours.data_ptr() if colours is not None else None, n)
+
+    @property
+    def c(self) -> _abi.PointsC:
+        return self._c
+
+
+def pack_points(xyz, rgb=None):
+    """Host arrays of a point cloud as points_decode.slang:32-41 packs them:
+    positions float32 [N, 4] (x, y, z, 1) and colours uint32 [N, 2] (f16 r | g,
+    b | 1.0), from xyz [N, 3] and linear rgb [N, 3] (None: white)."""
+    xyz = np.asarray(xyz, dtype=np.float32).reshape(-1, 3)
+    n = xyz.shape[0]
+    positions = np.concatenate([xyz, np.ones((n, 1), np.float32)], axis=1)
+    rgb = np.ones((n, 3), np.float32) if rgb is None else np.asarray(rgb, dtype=np.float32).reshape(n, 3)
+    with np.errstate(over="ignore"):
+        halves = np.concatenate([rgb, np.ones((n, 1), np.float32)], axis=1).astype(np.float16).view(np.uint16)
+    h = halves.astype(np.uint32)
+    colours = np.stack([h[:, 0] | (h[:, 1] << 16), h[:, 2] | (h[:, 3] << 16)], axis=1)
+    return positions, colours
+
+
+@dataclass
+class PointStyle:
+    """render::PointStyle (modules/render/include/lrt/render/Points.h:21-33), the
+    fields the disc route reads: size_mode WORLD (0) or PIXELS (1), size (a
+    diameter), constant_colour and colour."""
+    WORLD = 0
+    PIXELS = 1
+    size_mode: int = 0
+    size: float = 0.01
+    constant_colour: bool = False
+    colour: tuple = (1.0, 1.0, 1.0)
+
+    def to_c(self) -> _abi.PointStyleC:
+        c = _abi.PointStyleC()
+        c.size_mode = self.size_mode
+        c.size = self.size
+        c.constant_colour = 1 if self.constant_colour else 0
+        c.colour[:] = [float(v) for v in self.colour]
+        return c
+
+
+@dataclass
+class PointInstance:
+    """render::PointInstance (Points.h:35-39): points drawn as opaque discs."""
+    points: Optional[GpuPoints]
+    object_to_world: tuple = IDENTITY
+    style: PointStyle = field(default_factory=PointStyle)
+
+
 @dataclass
 class SplatInstance:
     """render::SplatInstance (TileRasterizer.h:181-196), without relighting."""
@@ -415,8 +485,12 @@ class TileRasterizer:
                                 tuple(info.bounds_max), float(info.upload_ms), float(info.decode_ms))
 
     def render(self, camera_or_projection, instances: Sequence[SplatInstance], settings: RenderSettings,
-               targets: RenderTargets) -> FrameStats:
-        """TileRasterizer::render (TileRasterizer.cpp:179-484), splat instances only."""
+               targets: RenderTargets, points: Sequence[PointInstance] = (),
+               under: Optional[RenderTargets] = None) -> FrameStats:
+        """TileRasterizer::render (TileRasterizer.cpp:188-484). Point instances are
+        drawn as opaque discs in the splats' depth order; `under` (the frame's
+        size) lies under the splats and ends each pixel's walk at its depth.
+        Either selects the composite blend (splat_blend.slang:116-120)."""
         proj = camera_or_projection
         if isinstance(proj, Camera):
             proj = projection_for(proj, settings.width, settings.height)
@@ -424,8 +498,24 @@ class TileRasterizer:
         t = _abi.RenderTargetsC(targets.width, targets.height, targets.colour.data_ptr(),
                                 targets.depth.data_ptr() if targets.depth is not None else None)
         st = _abi.FrameStatsC()
-        self._check(lib.lrt_raster_render(self._ctx, C.byref(proj), arr, len(instances), C.byref(settings.to_c()),
-                                          C.byref(t), C.byref(st)))
+        if not points and under is None:
+            self._check(lib.lrt_raster_render(self._ctx, C.byref(proj), arr, len(instances),
+                                              C.byref(settings.to_c()), C.byref(t), C.byref(st)))
+        else:
+            parr = (_abi.PointInstanceC * max(1, len(points)))()
+            for k, inst in enumerate(points):
+                if inst.points is not None:
+                    parr[k].points = C.pointer(inst.points.c)
+                parr[k].object_to_world[:] = list(np.asarray(inst.object_to_world, dtype=np.float64).reshape(16))
+                parr[k].style = inst.style.to_c()
+            u = None
+            if under is not None:
+                if under.depth is None:
+                    raise LrtError(3, "an under-layer needs both colour and depth")
+                u = _abi.RenderTargetsC(under.width, under.height, under.colour.data_ptr(), under.depth.data_ptr())
+            self._check(lib.lrt_raster_render_composite(self._ctx, C.byref(proj), arr, len(instances), parr,
+                                                        len(points), C.byref(u) if u is not None else None,
+                                                        C.byref(settings.to_c()), C.byref(t), C.byref(st)))
         return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
 
     def reference_render(self, camera_or_projection, instances: Sequence[SplatInstance], settings: RenderSettings,

@@ -783,6 +783,78 @@ void or_project(const or_frame* f, const float* positions, const uint32_t* shape
         project_one(f, (uint32_t)i, positions, shape, sh, filter3d, proj, rects, touched, visible, depth_keys);
 }
 
+/* pointRadius (shaders/lrt/points/point_frame.slang:39-44): the disc's radius
+ * in pixels, never below half a pixel. */
+static float point_radius(const or_frame* f, const or_point_style* st, float view_z) {
+    const float r = st->size_mode == 1 ? st->size * 0.5f
+                                       : (f->orthographic ? st->size * 0.5f * f->focal_x
+                                                          : st->size * 0.5f * f->focal_x / view_z);
+    return fmax_(r, 0.5f);
+}
+
+/* pointsProject (shaders/lrt/splat/points_project.slang:33-86): a disc record
+ * with the conic of its radius, cutoff 1, alpha 1 and colour.w = 1; the tile
+ * rectangle of the radius itself (:52-53) and the same exact tile count. */
+static void points_project_one(const or_frame* f, const or_point_style* st, uint32_t i, const float* positions,
+                               const uint32_t* colours, or_proj_record* proj, uint32_t* rects,
+                               uint32_t* touched_out, uint32_t* visible, uint32_t* keys) {
+    const uint32_t slot = f->base + i;
+    const float* p = positions + (size_t)i * 4;
+    const float* m = f->m;
+    const float tx_ = m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3];
+    const float ty_ = m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7];
+    const float tz_ = m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11];
+    if (!(tz_ > f->near_z) || tz_ > f->far_z) {
+        cull(slot, rects, touched_out, visible, keys);
+        return;
+    }
+    const float radius = point_radius(f, st, tz_);
+    const float invz = f->orthographic ? 1.0f : 1.0f / tz_;
+    const float cx = f->focal_x * tx_ * invz + f->centre_x;
+    const float cy = f->focal_y * ty_ * invz + f->centre_y;
+    const float k = 1.0f / (radius * radius);
+    const float cutoff = 1.0f;
+    uint32_t rect[4];
+    tile_rect(f, cx, cy, radius, radius, rect);
+    if (rect[0] > rect[2] || rect[1] > rect[3]) {
+        cull(slot, rects, touched_out, visible, keys);
+        return;
+    }
+    uint32_t touched = 0;
+    for (uint32_t tyy = rect[1]; tyy <= rect[3]; ++tyy)
+        for (uint32_t txx = rect[0]; txx <= rect[2]; ++txx)
+            touched += (uint32_t)tile_touched(f, cx, cy, k, 0.0f, k, cutoff, txx, tyy);
+    if (touched == 0) {
+        cull(slot, rects, touched_out, visible, keys);
+        return;
+    }
+    float rgb[3] = {st->colour[0], st->colour[1], st->colour[2]};
+    if (!st->constant_colour) {
+        rgb[0] = low_half(colours[(size_t)i * 2]);
+        rgb[1] = high_half(colours[(size_t)i * 2]);
+        rgb[2] = low_half(colours[(size_t)i * 2 + 1]);
+    }
+    or_proj_record* out = proj + slot;
+    out->screen[0] = cx; out->screen[1] = cy; out->screen[2] = tz_; out->screen[3] = cutoff;
+    out->conic[0] = k; out->conic[1] = 0.0f; out->conic[2] = k; out->conic[3] = 1.0f;
+    out->colour[0] = rgb[0]; out->colour[1] = rgb[1]; out->colour[2] = rgb[2]; out->colour[3] = 1.0f;
+    if (rects) memcpy(rects + (size_t)slot * 4, rect, sizeof rect);
+    if (touched_out) touched_out[slot] = touched;
+    if (visible) visible[slot] = 1;
+    if (keys) keys[slot] = f2u(tz_) >> 7;
+}
+
+void or_points_project(const or_frame* f, const or_point_style* style, const float* positions,
+                       const uint32_t* colours, or_proj_record* proj, uint32_t* rects, uint32_t* touched,
+                       uint32_t* visible, uint32_t* depth_keys, int threads) {
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 4096) OR_THREADS(threads)
+#endif
+    for (int64_t i = 0; i < (int64_t)f->count; ++i)
+        points_project_one(f, style, (uint32_t)i, positions, colours, proj, rects, touched, visible, depth_keys);
+}
+
 /* ====================================================================== */
 /* scan, sort, emit, ranges                                                */
 /* ====================================================================== */
@@ -833,19 +905,39 @@ void or_radix_sort(uint64_t* keys, uint32_t* values, uint32_t n, uint32_t key_bi
 /* splatBlend                                                              */
 /* ====================================================================== */
 
+/* blendPixel (shaders/lrt/splat/splat_blend.slang:23-108). With composite set
+ * (splatBlendComposite, :116-120): the walk ends at the first record at or
+ * behind the under-layer's depth (:52-54), and an opaque disc that reaches the
+ * pixel ends it with its colour behind the splats (:63-68); without a disc the
+ * under-layer is what lies behind (:83-87). under_colour/under_depth may be
+ * NULL (no under-layer, params.hasUnder == 0). */
 static void blend_pixel(const or_frame* f, uint32_t px, uint32_t py, const uint32_t* ranges,
-                        const uint32_t* pair_splats, const or_proj_record* proj, float* colour, float* depth,
+                        const uint32_t* pair_splats, const or_proj_record* proj, const float* under_colour,
+                        const float* under_depth, int composite, float* colour, float* depth,
                         uint32_t* last_count, float* final_t) {
     const uint32_t tile = (py / 16u) * f->tiles_x + px / 16u;
     const uint32_t start = ranges[tile * 2], end = ranges[tile * 2 + 1];
     const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
+    const size_t at = (size_t)py * f->width + px;
+    const int under = composite && under_depth != NULL;
+    const float under_z = under ? under_depth[at] : 0.0f;
+    const int under_hit = under && under_z > 0.0f;
     float r = 0.0f, g = 0.0f, b = 0.0f, t = 1.0f, depth_sum = 0.0f, depth_at = -1.0f;
+    int opaque_hit = 0;
+    float opaque[4] = {0.0f, 0.0f, 0.0f, 0.0f}, opaque_z = 0.0f;
     uint32_t last = 0;
     for (uint32_t k = start; k < end; ++k) {
         const or_proj_record* rec = proj + pair_splats[k];
+        if (under_hit && rec->screen[2] >= under_z) break;
         const float dx = pxf - rec->screen[0], dy = pyf - rec->screen[1];
         const float power = rec->conic[0] * dx * dx + 2.0f * rec->conic[1] * dx * dy + rec->conic[2] * dy * dy;
         if (power > rec->screen[3]) continue;
+        if (composite && rec->colour[3] > 0.5f) {
+            opaque_hit = 1;
+            opaque[0] = rec->colour[0]; opaque[1] = rec->colour[1]; opaque[2] = rec->colour[2]; opaque[3] = 1.0f;
+            opaque_z = rec->screen[2];
+            break;
+        }
         const float alpha = fmin_(0.99f, rec->conic[3] * or_expf(-0.5f * power));
         if (alpha < 1.0f / 255.0f) continue;
         const float weight = t * alpha;
@@ -858,19 +950,30 @@ static void blend_pixel(const or_frame* f, uint32_t px, uint32_t py, const uint3
         if (depth_at < 0.0f && 1.0f - t >= f->depth_threshold) depth_at = rec->screen[2];
         if (t < 1e-4f) break;
     }
+    if (!opaque_hit && under_hit) {
+        opaque_hit = 1;
+        memcpy(opaque, under_colour + at * 4, sizeof opaque);
+        opaque_z = under_z;
+    }
     const float coverage = 1.0f - t;
     if (f->linearise && coverage > 1e-6f) {
         r = or_srgb_to_linear(r / coverage) * coverage;
         g = or_srgb_to_linear(g / coverage) * coverage;
         b = or_srgb_to_linear(b / coverage) * coverage;
     }
-    const size_t at = (size_t)py * f->width + px;
-    colour[at * 4 + 0] = r + t * f->bg[0];
-    colour[at * 4 + 1] = g + t * f->bg[1];
-    colour[at * 4 + 2] = b + t * f->bg[2];
-    colour[at * 4 + 3] = coverage + t * f->bg[3];
+    const float* behind = f->bg;
+    if (opaque_hit) {
+        behind = opaque;
+        depth_sum = depth_sum + t * opaque_z;
+        if (depth_at < 0.0f) depth_at = opaque_z;
+    }
+    const float total = opaque_hit ? 1.0f : coverage;
+    colour[at * 4 + 0] = r + t * behind[0];
+    colour[at * 4 + 1] = g + t * behind[1];
+    colour[at * 4 + 2] = b + t * behind[2];
+    colour[at * 4 + 3] = coverage + t * behind[3];
     if (depth) {
-        depth[at] = f->depth_mode == 0 ? (coverage > 1e-6f ? depth_sum / coverage : 0.0f) : fmax_(depth_at, 0.0f);
+        depth[at] = f->depth_mode == 0 ? (total > 1e-6f ? depth_sum / total : 0.0f) : fmax_(depth_at, 0.0f);
     }
     if (last_count) last_count[at] = last;
     if (final_t) final_t[at] = t;
@@ -885,22 +988,42 @@ void or_blend(const or_frame* f, const uint32_t* ranges, const uint32_t* pair_sp
 #endif
     for (int64_t py = 0; py < (int64_t)f->height; ++py)
         for (uint32_t px = 0; px < f->width; ++px)
-            blend_pixel(f, px, (uint32_t)py, ranges, pair_splats, proj, colour, depth, last_count, final_t);
+            blend_pixel(f, px, (uint32_t)py, ranges, pair_splats, proj, NULL, NULL, 0, colour, depth, last_count,
+                        final_t);
+}
+
+void or_blend_composite(const or_frame* f, const uint32_t* ranges, const uint32_t* pair_splats,
+                        const or_proj_record* proj, const float* under_colour, const float* under_depth,
+                        float* colour, float* depth, int threads) {
+    (void)threads;
+#ifdef _OPENMP
+#pragma omp parallel for schedule(dynamic, 1) OR_THREADS(threads)
+#endif
+    for (int64_t py = 0; py < (int64_t)f->height; ++py)
+        for (uint32_t px = 0; px < f->width; ++px)
+            blend_pixel(f, px, (uint32_t)py, ranges, pair_splats, proj, under_colour, under_depth, 1, colour, depth,
+                        NULL, NULL);
 }
 
 /* ====================================================================== */
 /* TileRasterizer::render (TileRasterizer.cpp:188-484), splat-only          */
 /* ====================================================================== */
 
-uint32_t or_raster(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
-                   or_proj_record* proj, uint32_t* touched, uint32_t* depth_keys,
-                   uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_capacity,
-                   uint32_t* ranges, float* colour, float* depth, uint32_t* last_count,
-                   float* final_t, or_stats* stats, int threads) {
+/* Splat instances' slots first, then the point instances' (TileRasterizer.cpp:
+ * 296-379); the composite blend when there are points or an under-layer
+ * (:464). */
+static uint32_t raster_frame(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                             const or_frame* point_frames, const or_points* points, uint32_t point_instances,
+                             const float* under_colour, const float* under_depth, or_proj_record* proj,
+                             uint32_t* touched, uint32_t* depth_keys, uint32_t* pair_tiles, uint32_t* pair_splats,
+                             uint32_t pair_capacity, uint32_t* ranges, float* colour, float* depth,
+                             uint32_t* last_count, float* final_t, or_stats* stats, int threads) {
     uint32_t total = 0;
     for (uint32_t k = 0; k < instances; ++k) total += frames[k].count;
+    for (uint32_t k = 0; k < point_instances; ++k) total += point_frames[k].count;
+    const int composite = point_instances > 0 || under_depth != NULL;
     const uint32_t all = total > 0 ? total : 1;
-    const or_frame* f0 = &frames[0];
+    const or_frame* f0 = instances > 0 || point_instances == 0 ? &frames[0] : &point_frames[0];   /* shared fields */
     const uint32_t tiles = f0->tiles_x * f0->tiles_y;
     uint32_t* rects = (uint32_t*)calloc((size_t)all * 4, sizeof(uint32_t));
     uint32_t* visible = (uint32_t*)calloc(all, sizeof(uint32_t));
@@ -914,6 +1037,9 @@ uint32_t or_raster(const or_frame* frames, const or_cloud* clouds, uint32_t inst
     for (uint32_t k = 0; k < instances; ++k)
         or_project(&frames[k], clouds[k].positions, clouds[k].shape, clouds[k].sh, clouds[k].filter3d, proj,
                    rects, touched, visible, depth_keys, threads);
+    for (uint32_t k = 0; k < point_instances; ++k)
+        or_points_project(&point_frames[k], &points[k].style, points[k].positions, points[k].colours, proj, rects,
+                          touched, visible, depth_keys, threads);
 
     const uint32_t visible_count = (uint32_t)or_scan_exclusive(visible, vis_off, total);
     const uint32_t pairs = (uint32_t)or_scan_exclusive(touched, touch_off, total);
@@ -968,7 +1094,11 @@ uint32_t or_raster(const or_frame* frames, const or_cloud* clouds, uint32_t inst
         if (i == 0 || (uint32_t)ptiles[i - 1] != t) rng[t * 2] = i;
         if (i + 1 == pairs || (uint32_t)ptiles[i + 1] != t) rng[t * 2 + 1] = i + 1;
     }
-    if (colour) or_blend(f0, rng, psplats, proj, colour, depth, last_count, final_t, threads);
+    if (colour && composite) {
+        or_blend_composite(f0, rng, psplats, proj, under_colour, under_depth, colour, depth, threads);
+    } else if (colour) {
+        or_blend(f0, rng, psplats, proj, colour, depth, last_count, final_t, threads);
+    }
 
     const uint32_t keep = pairs < pair_capacity ? pairs : pair_capacity;
     for (uint32_t i = 0; i < keep; ++i) {
@@ -989,11 +1119,31 @@ uint32_t or_raster(const or_frame* frames, const or_cloud* clouds, uint32_t inst
     return pairs;
 }
 
+uint32_t or_raster(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                   or_proj_record* proj, uint32_t* touched, uint32_t* depth_keys,
+                   uint32_t* pair_tiles, uint32_t* pair_splats, uint32_t pair_capacity,
+                   uint32_t* ranges, float* colour, float* depth, uint32_t* last_count,
+                   float* final_t, or_stats* stats, int threads) {
+    return raster_frame(frames, clouds, instances, NULL, NULL, 0, NULL, NULL, proj, touched, depth_keys, pair_tiles,
+                        pair_splats, pair_capacity, ranges, colour, depth, last_count, final_t, stats, threads);
+}
+
+uint32_t or_raster_composite(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                             const or_frame* point_frames, const or_points* points, uint32_t point_instances,
+                             const float* under_colour, const float* under_depth, or_proj_record* proj,
+                             uint32_t* touched, uint32_t* depth_keys, uint32_t* pair_tiles, uint32_t* pair_splats,
+                             uint32_t pair_capacity, uint32_t* ranges, float* colour, float* depth,
+                             or_stats* stats, int threads) {
+    return raster_frame(frames, clouds, instances, point_frames, points, point_instances, under_colour, under_depth,
+                        proj, touched, depth_keys, pair_tiles, pair_splats, pair_capacity, ranges, colour, depth,
+                        NULL, NULL, stats, threads);
+}
+
 /* ====================================================================== */
 /* The brute-force ground truth                                            */
 /* ====================================================================== */
 
-typedef struct { float screen[4], conic[4], colour[3]; } ref_f32;
+typedef struct { float screen[4], conic[4], colour[3]; int disc; } ref_f32;
 typedef struct { double screen[4], conic[4], colour[3]; } ref_f64;
 
 /* reference_project.slang:31-105 in FP32. screen[2] < 0 marks a culled splat. */
@@ -1278,14 +1428,45 @@ static int by_depth(const void* a, const void* b) {
     return x->index < y->index ? -1 : x->index > y->index ? 1 : 0;
 }
 
+/* referencePointsProject (shaders/lrt/reference/reference_points_project.slang:
+ * 20-42): the disc without tiles, its centre by division. */
+static void ref_points_project_f32(const or_frame* f, const or_points* pts, uint32_t i, ref_f32* out) {
+    out->screen[2] = -1.0f;
+    out->disc = 1;
+    const float* p = pts->positions + (size_t)i * 4;
+    const float* m = f->m;
+    const float v[3] = {m[0] * p[0] + m[1] * p[1] + m[2] * p[2] + m[3],
+                        m[4] * p[0] + m[5] * p[1] + m[6] * p[2] + m[7],
+                        m[8] * p[0] + m[9] * p[1] + m[10] * p[2] + m[11]};
+    if (!(v[2] > f->near_z) || v[2] > f->far_z) return;
+    const or_point_style* st = &pts->style;
+    const float r = st->size_mode == 1 ? fmax_(st->size * 0.5f, 0.5f)
+                                       : fmax_(f->orthographic ? st->size * 0.5f * f->focal_x
+                                                               : st->size * 0.5f * f->focal_x / v[2], 0.5f);
+    out->screen[0] = f->orthographic ? f->focal_x * v[0] + f->centre_x : f->focal_x * v[0] / v[2] + f->centre_x;
+    out->screen[1] = f->orthographic ? f->focal_y * v[1] + f->centre_y : f->focal_y * v[1] / v[2] + f->centre_y;
+    out->screen[2] = v[2];
+    out->screen[3] = 1.0f;
+    out->conic[0] = 1.0f / (r * r); out->conic[1] = 0.0f; out->conic[2] = 1.0f / (r * r); out->conic[3] = 1.0f;
+    if (st->constant_colour) {
+        memcpy(out->colour, st->colour, sizeof out->colour);
+    } else {
+        out->colour[0] = low_half(pts->colours[(size_t)i * 2]);
+        out->colour[1] = high_half(pts->colours[(size_t)i * 2]);
+        out->colour[2] = low_half(pts->colours[(size_t)i * 2 + 1]);
+    }
+}
+
 /* The walk of reference_blend.slang:129-223, over every splat in (z, index)
  * order: the same sequence its per-pixel insertion sort builds. */
-void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
-                             const uint32_t* px, const uint32_t* py, uint32_t npix, float* colour,
-                             float* depth, int keyed, int threads) {
+static void reference_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                          const or_frame* point_frames, const or_points* points, uint32_t point_instances,
+                          const uint32_t* px, const uint32_t* py, uint32_t npix, float* colour, float* depth,
+                          int keyed, int threads) {
     (void)threads;
     uint32_t total = 0;
     for (uint32_t k = 0; k < instances; ++k) total += frames[k].count;
+    for (uint32_t k = 0; k < point_instances; ++k) total += point_frames[k].count;
     ref_f32* rec = (ref_f32*)calloc(total > 0 ? total : 1, sizeof(ref_f32));
     for (uint32_t k = 0; k < instances; ++k) {
 #ifdef _OPENMP
@@ -1293,6 +1474,10 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
 #endif
         for (int64_t i = 0; i < (int64_t)frames[k].count; ++i)
             ref_project_f32(&frames[k], &clouds[k], (uint32_t)i, rec + frames[k].base + i);
+    }
+    for (uint32_t k = 0; k < point_instances; ++k) {
+        for (int64_t i = 0; i < (int64_t)point_frames[k].count; ++i)
+            ref_points_project_f32(&point_frames[k], &points[k], (uint32_t)i, rec + point_frames[k].base + i);
     }
     depth_entry* order = (depth_entry*)malloc((total > 0 ? total : 1) * sizeof(depth_entry));
     uint32_t live = 0;
@@ -1305,7 +1490,7 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
     qsort(order, live, sizeof *order, by_depth);
     ref_f32* sorted = (ref_f32*)malloc((live > 0 ? live : 1) * sizeof(ref_f32));
     for (uint32_t k = 0; k < live; ++k) sorted[k] = rec[order[k].index];
-    const or_frame* f = &frames[0];
+    const or_frame* f = instances > 0 || point_instances == 0 ? &frames[0] : &point_frames[0];
     const uint32_t count = px ? npix : f->width * f->height;
 #ifdef _OPENMP
 #pragma omp parallel for schedule(dynamic, 64) OR_THREADS(threads)
@@ -1315,11 +1500,16 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
         const uint32_t y = px ? py[n] : (uint32_t)(n / f->width);
         const float pxf = (float)x + 0.5f, pyf = (float)y + 0.5f;
         float r = 0.0f, g = 0.0f, b = 0.0f, t = 1.0f, dsum = 0.0f, dat = -1.0f;
+        const ref_f32* disc = NULL;
         for (uint32_t k = 0; k < live; ++k) {
             const ref_f32* s = sorted + k;
             const float dx = pxf - s->screen[0], dy = pyf - s->screen[1];
             const float power = s->conic[0] * dx * dx + 2.0f * s->conic[1] * dx * dy + s->conic[2] * dy * dy;
             if (power > s->screen[3]) continue;
+            if (s->disc) {   /* alpha 1, and the walk ends there (reference_blend.slang:50-51,83-88) */
+                disc = s;
+                break;
+            }
             const float alpha = fmin_(0.99f, s->conic[3] * or_expf(-0.5f * power));
             if (alpha < 1.0f / 255.0f) continue;
             const float weight = t * alpha;
@@ -1337,15 +1527,35 @@ void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uin
             g = or_srgb_to_linear(g / coverage) * coverage;
             b = or_srgb_to_linear(b / coverage) * coverage;
         }
-        colour[n * 4 + 0] = r + t * f->bg[0];
-        colour[n * 4 + 1] = g + t * f->bg[1];
-        colour[n * 4 + 2] = b + t * f->bg[2];
-        colour[n * 4 + 3] = coverage + t * f->bg[3];
-        if (depth) depth[n] = f->depth_mode == 0 ? (coverage > 1e-6f ? dsum / coverage : 0.0f) : fmax_(dat, 0.0f);
+        const float behind[4] = {disc ? disc->colour[0] : f->bg[0], disc ? disc->colour[1] : f->bg[1],
+                                 disc ? disc->colour[2] : f->bg[2], disc ? 1.0f : f->bg[3]};
+        if (disc) {
+            dsum = dsum + t * disc->screen[2];
+            if (dat < 0.0f) dat = disc->screen[2];
+        }
+        const float total_w = disc ? 1.0f : coverage;
+        colour[n * 4 + 0] = r + t * behind[0];
+        colour[n * 4 + 1] = g + t * behind[1];
+        colour[n * 4 + 2] = b + t * behind[2];
+        colour[n * 4 + 3] = coverage + t * behind[3];
+        if (depth) depth[n] = f->depth_mode == 0 ? (total_w > 1e-6f ? dsum / total_w : 0.0f) : fmax_(dat, 0.0f);
     }
     free(sorted);
     free(order);
     free(rec);
+}
+
+void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                             const uint32_t* px, const uint32_t* py, uint32_t npix, float* colour,
+                             float* depth, int keyed, int threads) {
+    reference_f32(frames, clouds, instances, NULL, NULL, 0, px, py, npix, colour, depth, keyed, threads);
+}
+
+void or_reference_render_points_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                                    const or_frame* point_frames, const or_points* points,
+                                    uint32_t point_instances, float* colour, float* depth, int threads) {
+    reference_f32(frames, clouds, instances, point_frames, points, point_instances, NULL, NULL, 0, colour, depth, 0,
+                  threads);
 }
 
 /* The FP64 walk over every projected splat in (z, index) order (or the keyed

@@ -180,6 +180,46 @@ uint32_t or_raster(const or_frame* frames, const or_cloud* clouds, uint32_t inst
                    uint32_t* ranges, float* colour, float* depth, uint32_t* last_count,
                    float* final_t, or_stats* stats, int threads);
 
+/* ---- points as opaque discs ------------------------------------------- */
+/* render::PointStyle (modules/render/include/lrt/render/Points.h:21-33), the
+ * fields the disc route reads (point_frame.slang:7-29). */
+typedef struct {
+    uint32_t size_mode;          /* 0 world units, 1 pixels */
+    float size;                  /* diameter */
+    int32_t constant_colour;     /* 1: colour below; 0: the cloud's own */
+    float colour[3];
+} or_point_style;
+
+/* A point instance (render::PointInstance, Points.h:35-39) whose frame is set
+ * apart: positions float4 xyz 1, colours 2 u32 a point (f16 r|g, b|a). */
+typedef struct {
+    const float* positions;
+    const uint32_t* colours;
+    or_point_style style;
+} or_points;
+
+/* pointsProject (shaders/lrt/splat/points_project.slang:33-86), frame.count
+ * points from index 0, written to slots frame.base + i. */
+void or_points_project(const or_frame* f, const or_point_style* style, const float* positions,
+                       const uint32_t* colours, or_proj_record* proj, uint32_t* rects, uint32_t* touched,
+                       uint32_t* visible, uint32_t* depth_keys, int threads);
+
+/* TileRasterizer::render with point instances and an under-layer
+ * (TileRasterizer.cpp:188-484): the splat instances' slots first, then the
+ * points'; the composite blend. under_colour (4 floats a pixel) and
+ * under_depth may be NULL (no under-layer). */
+uint32_t or_raster_composite(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                             const or_frame* point_frames, const or_points* points, uint32_t point_instances,
+                             const float* under_colour, const float* under_depth, or_proj_record* proj,
+                             uint32_t* touched, uint32_t* depth_keys, uint32_t* pair_tiles, uint32_t* pair_splats,
+                             uint32_t pair_capacity, uint32_t* ranges, float* colour, float* depth,
+                             or_stats* stats, int threads);
+
+/* splatBlendComposite (shaders/lrt/splat/splat_blend.slang:23-108,116-120). */
+void or_blend_composite(const or_frame* f, const uint32_t* ranges, const uint32_t* pair_splats,
+                        const or_proj_record* proj, const float* under_colour, const float* under_depth,
+                        float* colour, float* depth, int threads);
+
 /* splatBlend (shaders/lrt/splat/splat_blend.slang:23-114), splat-only entry. */
 void or_blend(const or_frame* f, const uint32_t* ranges, const uint32_t* pair_splats,
               const or_proj_record* proj, float* colour, float* depth, uint32_t* last_count,
@@ -195,6 +235,12 @@ void or_blend(const or_frame* f, const uint32_t* ranges, const uint32_t* pair_sp
 void or_reference_render_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                              const uint32_t* px, const uint32_t* py, uint32_t npix, float* colour,
                              float* depth, int keyed, int threads);
+/* The brute force with point instances as discs (reference_points_project.slang:
+ * 20-42; reference_blend.slang:50-51,80-115): alpha 1, and the nearest disc
+ * reaching a pixel ends its walk. Whole images, (view z, index) order. */
+void or_reference_render_points_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
+                                    const or_frame* point_frames, const or_points* points,
+                                    uint32_t point_instances, float* colour, float* depth, int threads);
 /* The same in FP64 throughout (the error oracle). */
 void or_reference_render_f64(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                              const uint32_t* px, const uint32_t* py, uint32_t npix, double* colour,

@@ -137,3 +137,73 @@ def random_cloud(count: int, seed: int, min_size=0.004, max_size=0.6, min_opacit
         dc = [rng.range(0.0, 1.2) for _ in range(3)]
         b.add(x, y, z, opacity, size, size * stretch, size * thin, turn, dc, rest)
     return b
+
+
+class PointLcg:
+    """tests/render/test_points.cpp:22-29 (its own starting state)."""
+
+    def __init__(self, seed: int = 0):
+        self.state = 0x2545F4914F6CDD1D ^ seed
+
+    def next(self) -> np.float32:
+        self.state = (self.state * 6364136223846793005 + 1442695040888963407) & 0xFFFFFFFFFFFFFFFF
+        return f32((self.state >> 40) & 0xFFFFFF) / f32(16777216.0)
+
+    def range(self, lo, hi) -> np.float32:
+        return f32(lo) + (f32(hi) - f32(lo)) * self.next()
+
+
+def pack_point_colours(rgb: np.ndarray) -> np.ndarray:
+    """points_decode.slang:39-40: f16 r | g << 16, b | 1.0 << 16 (numpy's
+    float16 conversion rounds to nearest even, as the reference's f32tof16)."""
+    rgb = np.asarray(rgb, np.float32).reshape(-1, 3)
+    h = np.concatenate([rgb, np.ones((rgb.shape[0], 1), np.float32)], axis=1).astype(np.float16).view(np.uint16)
+    h = h.astype(np.uint32)
+    return np.ascontiguousarray(np.stack([h[:, 0] | (h[:, 1] << 16), h[:, 2] | (h[:, 3] << 16)], axis=1))
+
+
+def random_points(count: int, seed: int):
+    """randomPoints (tests/render/test_points.cpp:31-45), draw for draw: positions
+    float32 [N, 4] (x, y, z, 1) and packed linear colours uint32 [N, 2]."""
+    rng = PointLcg(seed)
+    xyz, rgb = [], []
+    for _ in range(count):
+        x, y, z = rng.range(-2, 2), rng.range(-1.5, 1.5), rng.range(-2, 2)
+        r, g, b = rng.range(0, 1), rng.range(0, 1), rng.range(0, 1)
+        xyz.append((x, y, z))
+        rgb.append((r, g, b))
+    positions = np.concatenate([np.array(xyz, np.float32), np.ones((count, 1), np.float32)], axis=1)
+    return np.ascontiguousarray(positions), pack_point_colours(np.array(rgb, np.float32))
+
+
+# The fog's encoding (tests/render/test_points.cpp:150-157): 14 floats, linear
+# opacity, scales and colour.
+FOG_ENCODING = dict(floats_per_record=14, x=0, y=1, z=2, opacity=3, scale0=4, scale1=5, scale2=6, rot_w=7, rot_x=8,
+                    rot_y=9, rot_z=10, dc0=11, dc1=12, dc2=13, rest_base=0, file_per_colour=0, rest_colour_outer=1,
+                    opacity_mode=1, scale_mode=1, colour_mode=1)
+
+
+def fog_records() -> np.ndarray:
+    """The 40 large translucent splats of test_points.cpp:158-165, draw for draw."""
+    rng = PointLcg(0)
+    rows = []
+    for _ in range(40):
+        x, y, z = rng.range(-2, 2), rng.range(-1, 1), rng.range(-2, 3)
+        a = rng.range(0.1, 0.6)
+        s0, s1, s2 = rng.range(0.1, 0.5), rng.range(0.1, 0.5), rng.range(0.1, 0.5)
+        r, g, b = rng.range(0, 1), rng.range(0, 1), rng.range(0, 1)
+        rows.append([x, y, z, a, s0, s1, s2, 1, 0, 0, 0, r, g, b])
+    return np.array(rows, np.float32)
+
+
+def decode_encoded(records: np.ndarray, encoding: dict, keep: int = 0) -> HostScene:
+    """The oracle's decode of records in any encoding (OrEncoding's fields)."""
+    records = np.ascontiguousarray(records, dtype=np.float32)
+    n = records.shape[0]
+    enc = O.OrEncoding(**encoding)
+    words = 1 if keep == 0 else (keep * 3 + 1) // 2
+    pos = np.zeros((n, 4), np.float32)
+    shape = np.zeros((n, 4), np.uint32)
+    sh = np.zeros((n, words), np.uint32)
+    kept = O.lib.or_decode_records(O.ptr(records), n, C.byref(enc), keep, O.ptr(pos), O.ptr(shape), O.ptr(sh), words)
+    return HostScene(pos[:kept].copy(), shape[:kept].copy(), sh[:kept].copy() if keep else None, keep)

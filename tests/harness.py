@@ -78,3 +78,65 @@ def assert_bit_exact(gpu, ora):
     np.testing.assert_array_equal((gpu["keys"] >> np.uint64(32)).astype(np.uint32), ora["pair_tiles"])
     np.testing.assert_array_equal(gpu["values"], ora["pair_splats"])
     np.testing.assert_array_equal(gpu["ranges"], ora["ranges"])
+
+
+def oracle_style(st: raster.PointStyle) -> O.OrPointStyle:
+    return O.point_style(st.size_mode, st.size, st.constant_colour, st.colour)
+
+
+def oracle_composite_frames(cam, settings, instances, points):
+    """instances as oracle_frames'; points: list of (positions float32 [N, 4],
+    colours uint32 [N, 2] or None, object_to_world 4x4, raster.PointStyle), their
+    slots after the splats'."""
+    proj = O.projection_for(oracle_camera(cam), settings.width, settings.height)
+    s = oracle_settings(settings)
+    frames, clouds, n = O.frames_for(proj, s, [(sc, np.asarray(o2w, dtype=np.float64), oracle_edit(ed))
+                                               for sc, o2w, ed in instances])
+    base = sum(sc.count for sc, _, _ in instances)
+    pframes, pts, npts = O.point_frames_for(proj, s, [(p, c, np.asarray(o2w, dtype=np.float64), oracle_style(st))
+                                                      for p, c, o2w, st in points], base)
+    return frames, clouds, n, pframes, pts, npts
+
+
+def run_oracle_composite(cam, settings, instances, points, under_colour=None, under_depth=None, threads=0):
+    frames, clouds, n, pframes, pts, npts = oracle_composite_frames(cam, settings, instances, points)
+    return O.raster_composite(frames, clouds, n, pframes, pts, npts, under_colour, under_depth, threads=threads)
+
+
+def run_reference_points(cam, settings, instances, points, threads=0):
+    frames, clouds, n, pframes, pts, npts = oracle_composite_frames(cam, settings, instances, points)
+    return O.reference_points(frames, clouds, n, pframes, pts, npts, threads=threads)
+
+
+def run_gpu_composite(cam, settings, instances, points, under=None, rast=None):
+    """The HIP composite frame: under is None or (colour [H, W, 4], depth [H, W])
+    host arrays. Returns the images and the debug buffers."""
+    import torch
+    rast = rast or raster.TileRasterizer()
+    dev = [sc.to_device() for sc, _, _ in instances]
+    inst = [raster.SplatInstance(d, tuple(np.asarray(o2w, dtype=np.float64).reshape(-1)), ed)
+            for d, (_, o2w, ed) in zip(dev, instances)]
+    pinst = []
+    for positions, colours, o2w, st in points:
+        gp = raster.GpuPoints(torch.from_numpy(np.ascontiguousarray(positions)).cuda(),
+                              torch.from_numpy(np.ascontiguousarray(colours).view(np.int32)).cuda()
+                              if colours is not None else None)
+        pinst.append(raster.PointInstance(gp, tuple(np.asarray(o2w, dtype=np.float64).reshape(-1)), st))
+    targets = raster.RenderTargets(settings.width, settings.height)
+    u = None
+    if under is not None:
+        u = raster.RenderTargets(settings.width, settings.height)
+        u.colour.copy_(torch.from_numpy(np.ascontiguousarray(under[0], dtype=np.float32)))
+        u.depth.copy_(torch.from_numpy(np.ascontiguousarray(under[1], dtype=np.float32)))
+    stats = rast.render(cam, inst, settings, targets, points=pinst, under=u)
+    torch.cuda.synchronize()
+    return dict(stats=stats,
+                counts=rast.debug(_abi.DEBUG_COUNTS),
+                touched=rast.debug(_abi.DEBUG_TOUCHED),
+                offsets=rast.debug(_abi.DEBUG_OFFSETS),
+                proj=rast.debug(_abi.DEBUG_PROJ).reshape(-1, 12),
+                keys=rast.debug(_abi.DEBUG_KEYS),
+                values=rast.debug(_abi.DEBUG_VALUES),
+                ranges=rast.debug(_abi.DEBUG_RANGES).reshape(-1, 2),
+                colour=targets.colour.cpu().numpy(),
+                depth=targets.depth.cpu().numpy())

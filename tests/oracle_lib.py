@@ -65,6 +65,15 @@ class OrDiffCloud(C.Structure):
                 ("sh", C.c_void_p), ("filter3d", C.c_void_p), ("count", C.c_uint32), ("rest_per_colour", C.c_uint32)]
 
 
+class OrPointStyle(C.Structure):
+    _fields_ = [("size_mode", C.c_uint32), ("size", C.c_float), ("constant_colour", C.c_int32),
+                ("colour", C.c_float * 3)]
+
+
+class OrPoints(C.Structure):
+    _fields_ = [("positions", C.c_void_p), ("colours", C.c_void_p), ("style", OrPointStyle)]
+
+
 class OrStats(C.Structure):
     _fields_ = [("splats", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32)]
 
@@ -97,6 +106,13 @@ _SIGS = {
     "or_raster": (C.c_uint32, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, P, P, P, C.c_uint32,
                                P, P, P, P, P, C.POINTER(OrStats), C.c_int]),
     "or_blend": (None, [C.POINTER(OrFrame), P, P, P, P, P, P, P, C.c_int]),
+    "or_points_project": (None, [C.POINTER(OrFrame), C.POINTER(OrPointStyle), P, P, P, P, P, P, P, C.c_int]),
+    "or_raster_composite": (C.c_uint32, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, C.POINTER(OrFrame),
+                                         C.POINTER(OrPoints), C.c_uint32, P, P, P, P, P, P, P, C.c_uint32, P, P, P,
+                                         C.POINTER(OrStats), C.c_int]),
+    "or_blend_composite": (None, [C.POINTER(OrFrame), P, P, P, P, P, P, P, C.c_int]),
+    "or_reference_render_points_f32": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32,
+                                              C.POINTER(OrFrame), C.POINTER(OrPoints), C.c_uint32, P, P, C.c_int]),
     "or_reference_render_f32": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
                                        C.c_int, C.c_int]),
     "or_reference_render_f64": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
@@ -192,6 +208,65 @@ def frames_for(projection: OrProjection, s: OrSettings, instances):
         clouds[k].filter3d = ptr(scene.filter3d) if scene.filter3d is not None else None
         base += scene.count
     return frames, clouds, n
+
+
+def point_style(size_mode=0, size=0.01, constant_colour=False, colour=(1.0, 1.0, 1.0)) -> OrPointStyle:
+    st = OrPointStyle()
+    st.size_mode, st.size, st.constant_colour = size_mode, size, 1 if constant_colour else 0
+    st.colour[:] = list(colour)
+    return st
+
+
+def point_frames_for(projection: OrProjection, s: OrSettings, instances, base: int):
+    """instances: list of (positions float32 [N, 4], colours uint32 [N, 2] or None,
+    object_to_world, OrPointStyle); their slots from `base`. Returns OrFrame and
+    OrPoints arrays and the count."""
+    n = len(instances)
+    frames = (OrFrame * max(1, n))()
+    points = (OrPoints * max(1, n))()
+    for k, (positions, colours, o2w, style) in enumerate(instances):
+        count = int(positions.shape[0])
+        lib.or_frame_for(C.byref(projection), C.byref(s), dbl(o2w), None, count, base, 0, 1, C.byref(frames[k]))
+        points[k].positions = ptr(positions)
+        points[k].colours = ptr(colours) if colours is not None else None
+        points[k].style = style
+        base += count
+    return frames, points, n
+
+
+def raster_composite(frames, clouds, n, pframes, points, npts, under_colour=None, under_depth=None, threads=0):
+    """The whole frame with point instances and an under-layer, restated."""
+    total = sum(frames[k].count for k in range(n)) + sum(pframes[k].count for k in range(npts))
+    f0 = frames[0] if n > 0 or npts == 0 else pframes[0]
+    tiles = f0.tiles_x * f0.tiles_y
+    proj = np.zeros((max(total, 1), 12), np.float32)
+    touched = np.zeros(max(total, 1), np.uint32)
+    keys = np.zeros(max(total, 1), np.uint32)
+    ranges = np.zeros(tiles * 2, np.uint32)
+    colour = np.zeros((f0.height, f0.width, 4), np.float32)
+    depth = np.zeros((f0.height, f0.width), np.float32)
+    stats = OrStats()
+    uc = np.ascontiguousarray(under_colour, dtype=np.float32) if under_colour is not None else None
+    ud = np.ascontiguousarray(under_depth, dtype=np.float32) if under_depth is not None else None
+    pairs = lib.or_raster_composite(frames, clouds, n, pframes, points, npts, ptr(uc), ptr(ud), ptr(proj),
+                                    ptr(touched), ptr(keys), None, None, 0, None, None, None, C.byref(stats), threads)
+    tiles_out = np.zeros(max(pairs, 1), np.uint32)
+    splats_out = np.zeros(max(pairs, 1), np.uint32)
+    lib.or_raster_composite(frames, clouds, n, pframes, points, npts, ptr(uc), ptr(ud), ptr(proj), ptr(touched),
+                            ptr(keys), ptr(tiles_out), ptr(splats_out), pairs, ptr(ranges), ptr(colour), ptr(depth),
+                            C.byref(stats), threads)
+    return dict(proj=proj[:total], touched=touched[:total], depth_keys=keys[:total], pair_tiles=tiles_out[:pairs],
+                pair_splats=splats_out[:pairs], ranges=ranges.reshape(-1, 2), colour=colour, depth=depth,
+                visible=stats.visible, pairs=pairs, splats=stats.splats)
+
+
+def reference_points(frames, clouds, n, pframes, points, npts, threads=0):
+    """The FP32 brute force with point instances as discs: whole images."""
+    f0 = frames[0] if n > 0 or npts == 0 else pframes[0]
+    colour = np.zeros((f0.height, f0.width, 4), np.float32)
+    depth = np.zeros((f0.height, f0.width), np.float32)
+    lib.or_reference_render_points_f32(frames, clouds, n, pframes, points, npts, ptr(colour), ptr(depth), threads)
+    return colour, depth
 
 
 def raster(frames, clouds, n, threads=0, want_image=True):

@@ -161,7 +161,7 @@ def test_a_descent_step_lowers_the_l2_loss(rast):
         for k in GROUPS:
             leaves[k] -= 1e-4 * leaves[k].grad
     after = loss()
-    assert float(after) < float(before)
+    assert after.item() < before.item()
 
 
 def test_backward_needs_a_differentiable_frame(rast):
