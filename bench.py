@@ -281,6 +281,59 @@ def views_1080p_leg(torch, raster, scenes, cfg, dev, reps):
     return {"workload": "views-fwd-3M-1080p-64", "splats": cfg.scene.count, **out}
 
 
+def composite_leg(torch, raster, _abi, cfg, dev, settings, reps):
+    """Composite frames on the benchmark scene (3,000,000 splats at 1920x1080):
+    1,000,000 points near its splat centres (every third centre moved by a
+    seeded N(0, 0.05) offset a coordinate) drawn as opaque 2-pixel discs with
+    seeded colours, (a) in one sort with the splats, (b) alone, as a layer, and
+    (c) the splats over that layer given as the under-layer. Frames/s, pairs
+    and the blend's GPU time of each, against the splat-only frame; and the
+    share of pixels where (c) and (a) differ by more than 1/255, which the two
+    routes' equal arithmetic leaves to pairs whose 24-bit depth keys tie."""
+    import numpy as np
+    n = 1_000_000
+    rng = np.random.default_rng(5)
+    xyz = dev.positions[::3][:n, :3].cpu().numpy() + rng.normal(0.0, 0.05, (n, 3)).astype(np.float32)
+    rgb = rng.random((n, 3), dtype=np.float32)
+    positions, colours = raster.pack_points(xyz, rgb)
+    points = raster.GpuPoints(torch.from_numpy(positions).cuda(), torch.from_numpy(colours.view(np.int32)).cuda())
+    pinst = [raster.PointInstance(points, style=raster.PointStyle(size_mode=raster.PointStyle.PIXELS, size=2.0))]
+    inst = [raster.SplatInstance(dev)]
+    rast = raster.TileRasterizer()
+    proj = raster.projection_for(cfg.camera, cfg.width, cfg.height)
+    mixed_t = raster.RenderTargets(cfg.width, cfg.height)
+    layer_t = raster.RenderTargets(cfg.width, cfg.height)
+    over_t = raster.RenderTargets(cfg.width, cfg.height)
+    plain_t = raster.RenderTargets(cfg.width, cfg.height)
+    runs = {"splats_only": lambda s: rast.render(proj, inst, s, plain_t),
+            "mixed": lambda s: rast.render(proj, inst, s, mixed_t, points=pinst),
+            "discs_layer": lambda s: rast.render(proj, [], s, layer_t, points=pinst),
+            "over_layer": lambda s: rast.render(proj, inst, s, over_t, under=layer_t)}
+    timed = raster.RenderSettings(**{**settings.__dict__, "time_stages": True})
+    out = {"workload": "scene-fwd-3M-1080p+1M-discs", "discs": n}
+    for name, run in runs.items():
+        for _ in range(2):
+            stats = run(settings)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(reps):
+            stats = run(settings)
+        torch.cuda.synchronize()
+        fps = reps / (time.perf_counter() - t0)
+        blend = 0.0
+        for _ in range(reps):
+            run(timed)
+            blend += float(rast.debug(_abi.DEBUG_KERNEL_MS)[6])
+        out[name] = {"fps": round(fps, 1), "pairs": stats.pairs, "visible": stats.visible,
+                     "blend_ms": round(blend / reps, 4)}
+    runs["mixed"](settings)
+    runs["over_layer"](settings)
+    torch.cuda.synchronize()
+    diff = (mixed_t.colour - over_t.colour).abs().amax(dim=2)
+    out["over_vs_mixed_pixels_over_1_255"] = round(float((diff > 1.0 / 255.0).float().mean()), 6)
+    return out
+
+
 def sort_stress_leg(torch, raster, scenes, _abi, reps):
     """Config 5: 6,000,000 splats at 3840x2160, 80% as config 1 and 20% large splats
     near the camera; the frame's stages timed, the sort's keys/s and HBM fraction."""
@@ -379,6 +432,8 @@ def main():
     ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")
     ap.add_argument("--load-reps", type=int, default=3, help="3M-record decodes timed; 0 skips it")
+    ap.add_argument("--composite-reps", type=int, default=5,
+                    help="composite frames (1M discs with the 3M splats) timed; 0 skips it")
     ap.add_argument("--profile-frames", type=int, default=0,
                     help="only draw this many config-2 frames and one 256 MiB device copy (the counter "
                          "calibration), for rocprofv3 --pmc; prints no JSON")
@@ -483,6 +538,11 @@ def main():
         log("sort stress (6M splats, 2160p) ...")
         stress = sort_stress_leg(torch, raster, scenes, _abi, args.stress_reps)
 
+    composite = None
+    if args.composite_reps > 0:
+        log("composite frames (3M splats + 1M discs, 1080p) ...")
+        composite = composite_leg(torch, raster, _abi, cfg, dev, settings, args.composite_reps)
+
     load = None
     if args.load_reps > 0:
         log("load path (3M PLY records) ...")
@@ -519,6 +579,7 @@ def main():
         "views_3m_1080p": batched_1080p,
         "sort_stress": stress,
         "load": load,
+        "composite": composite,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }
