@@ -235,6 +235,76 @@ lrt_status lrt_raster_render_composite(lrt_context* ctx, const lrt_projection* p
                                        const lrt_render_targets* under, const lrt_render_settings* settings,
                                        lrt_render_targets* targets, lrt_frame_stats* stats);
 
+/* ---- levels of detail ----------------------------------------------------------------
+ * lrt::lod (modules/lod/include/lrt/lod/Lod.h:36-150; shaders/lrt/lod/ kernels)
+ * for a cloud held whole in device memory: the reference's streamed store
+ * (chunks, residency, .lrtc) is not carried. The build sorts the splats by the
+ * Morton code of their position in the bounding cube and keeps, for each
+ * stored octree level (coarsest .. finest, consecutive), one merged Gaussian a
+ * non-empty cell from the splats' additive moments. The cut draws a merged
+ * Gaussian when its cell projects to at most `threshold` pixels and its
+ * parent's does not, and a splat when its finest-level cell does not:
+ * threshold 0 draws every splat and nothing merged. */
+typedef struct lrt_lod_cloud lrt_lod_cloud;
+
+/* LodBuildSettings (Lod.h:83-93) without the streaming chunk size. */
+typedef struct {
+    float max_group_fraction;    /* the finest stored level's cells hold >= 1 / this splats on average */
+    uint32_t coarsest_level;     /* 1 .. 10 */
+} lrt_lod_build_settings;
+
+void lrt_lod_build_settings_default(lrt_lod_build_settings* out);
+
+/* LodBuilder::build (Lod.h:95-106, Lod.cpp:79-263) on the context's stream.
+ * Synchronous. The cloud (device pointers, as for lrt_raster_render) is read
+ * only; the levels own their device memory until lrt_lod_destroy. settings
+ * may be NULL (the defaults). An empty cloud is LRT_INVALID_ARGUMENT. */
+lrt_status lrt_lod_build(lrt_context* ctx, const lrt_splats* cloud, const lrt_lod_build_settings* settings,
+                         lrt_lod_cloud** out);
+void lrt_lod_destroy(lrt_lod_cloud* lod);
+
+/* What a built cloud holds (LodCloud, Lod.h:48-81): per stored level, coarsest
+ * first, its octree level, its first merged Gaussian and its group count. */
+typedef struct {
+    uint32_t count, rest_per_colour, sh_words;
+    uint32_t levels, merged;
+    uint32_t level[10], first[10], groups[10];
+    float bounds_lo[3];
+    float extent;                /* the bounds' largest edge x 1.0001: cells are cubes */
+} lrt_lod_info;
+lrt_status lrt_lod_get_info(const lrt_lod_cloud* lod, lrt_lod_info* out);
+/* The cloud's splats in Morton order and (groups, may be NULL) each one's
+ * finest-level group; level `level_index`'s merged Gaussians and (cells, may
+ * be NULL) their cell codes. Device pointers the cloud owns. */
+lrt_status lrt_lod_splats(const lrt_lod_cloud* lod, lrt_splats* out, const uint32_t** groups);
+lrt_status lrt_lod_gaussians(const lrt_lod_cloud* lod, uint32_t level_index, lrt_splats* out,
+                             const uint32_t** cells);
+
+/* LodInstance (Lod.h:108-113): has_threshold 1 uses `threshold` over the call's. */
+typedef struct {
+    const lrt_lod_cloud* cloud;
+    double object_to_world[16];
+    lrt_splat_edit edit;
+    int32_t has_threshold;
+    float threshold;
+} lrt_lod_instance;
+
+/* CutStats (Lod.h:115-126) without the streaming needs. */
+typedef struct {
+    uint32_t splats, merged, available;
+} lrt_cut_stats;
+
+/* CutSelector::select (Lod.h:135-141, Lod.cpp:346-546): each instance's cut
+ * for this view as splat instances for lrt_raster_render, in order, an
+ * instance that draws nothing left out. out has room for `count` entries and
+ * *out_count gets how many were written; stats (may be NULL) gets one entry an
+ * instance. The instances' clouds are device memory the context owns, valid
+ * until its next lrt_lod_select; their gathers are queued on the context's
+ * stream after one wait for the drawn counts. */
+lrt_status lrt_lod_select(lrt_context* ctx, const lrt_projection* projection, const lrt_lod_instance* instances,
+                          uint32_t count, float threshold, lrt_splat_instance* out, uint32_t* out_count,
+                          lrt_cut_stats* stats);
+
 /* ---- the brute force in FP64 -------------------------------------------------------
  * ReferenceRenderer (modules/render/src/ReferenceRenderer.cpp:54-160;
  * reference_project.slang:31-105, reference_blend.slang:129-223) on the GPU in
@@ -361,6 +431,10 @@ lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, ui
 
 /* Waits for everything queued on the context's stream. */
 lrt_status lrt_context_synchronize(lrt_context* ctx);
+/* Copies `bytes` of device memory to the host after everything queued on the
+ * context's stream: how a caller reads buffers the library owns (the levels of
+ * an lrt_lod_cloud, the clouds of a cut). */
+lrt_status lrt_copy_to_host(lrt_context* ctx, const void* device, void* host, size_t bytes);
 
 /* ---- images ------------------------------------------------------------------
  * io::writeExr (modules/io/src/Exr.cpp:130-157): an OpenEXR scanline file,

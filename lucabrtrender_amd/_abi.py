@@ -133,6 +133,26 @@ class DiffGradsC(C.Structure):
     ]
 
 
+class LodBuildSettingsC(C.Structure):
+    _fields_ = [("max_group_fraction", C.c_float), ("coarsest_level", C.c_uint32)]
+
+
+class LodInfoC(C.Structure):
+    _fields_ = [("count", C.c_uint32), ("rest_per_colour", C.c_uint32), ("sh_words", C.c_uint32),
+                ("levels", C.c_uint32), ("merged", C.c_uint32), ("level", C.c_uint32 * 10),
+                ("first", C.c_uint32 * 10), ("groups", C.c_uint32 * 10), ("bounds_lo", C.c_float * 3),
+                ("extent", C.c_float)]
+
+
+class LodInstanceC(C.Structure):
+    _fields_ = [("cloud", C.c_void_p), ("object_to_world", C.c_double * 16), ("edit", SplatEdit),
+                ("has_threshold", C.c_int32), ("threshold", C.c_float)]
+
+
+class CutStatsC(C.Structure):
+    _fields_ = [("splats", C.c_uint32), ("merged", C.c_uint32), ("available", C.c_uint32)]
+
+
 class SceneSpecC(C.Structure):
     _fields_ = [
         ("kind", C.c_uint32), ("degree", C.c_uint32), ("seed", C.c_uint64),
@@ -196,6 +216,15 @@ SIGNATURES = {
     "lrt_radix_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.POINTER(C.c_int32)]),
     "lrt_context_synchronize": (C.c_int, [C.c_void_p]),
+    "lrt_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
+    "lrt_lod_build_settings_default": (None, [C.POINTER(LodBuildSettingsC)]),
+    "lrt_lod_build": (C.c_int, [C.c_void_p, C.POINTER(Splats), C.POINTER(LodBuildSettingsC), C.POINTER(C.c_void_p)]),
+    "lrt_lod_destroy": (None, [C.c_void_p]),
+    "lrt_lod_get_info": (C.c_int, [C.c_void_p, C.POINTER(LodInfoC)]),
+    "lrt_lod_splats": (C.c_int, [C.c_void_p, C.POINTER(Splats), C.POINTER(C.c_void_p)]),
+    "lrt_lod_gaussians": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(Splats), C.POINTER(C.c_void_p)]),
+    "lrt_lod_select": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(LodInstanceC), C.c_uint32, C.c_float,
+                                 C.POINTER(SplatInstanceC), C.POINTER(C.c_uint32), C.POINTER(CutStatsC)]),
     "lrt_write_exr": (C.c_int, [C.c_char_p, C.c_uint32, C.c_uint32, C.c_void_p, C.c_void_p, C.c_int32]),
     "lrt_splats_read": (C.c_int, [C.c_char_p, C.POINTER(C.c_void_p)]),
     "lrt_raw_splats_create": (C.c_int, [C.c_void_p, C.c_uint32, C.POINTER(SplatEncodingC), C.POINTER(C.c_void_p)]),

@@ -8,11 +8,18 @@
 
 #include "../../include/lrt_hip.h"
 #include "exr.h"
+#include "lod.h"
 #include "rasterizer.h"
 
 struct lrt_context {
     lrt::TileRasterizer raster;
+    lrt::lod::CutSelector cut;
+    std::vector<lrt_splats> cutClouds;   // what the last lrt_lod_select's instances point at
     std::string error;
+};
+
+struct lrt_lod_cloud {
+    lrt::lod::LodCloud lod;
 };
 
 struct lrt_raw_splats {
@@ -560,6 +567,18 @@ lrt_status lrt_context_synchronize(lrt_context* ctx) {
                                        : lrt::Status::error(lrt::ErrorCode::DeviceFailure, hipGetErrorString(e)));
 }
 
+lrt_status lrt_copy_to_host(lrt_context* ctx, const void* device, void* host, size_t bytes) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (bytes == 0) return finish(ctx, {});
+    if (device == nullptr || host == nullptr) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null pointer"));
+    }
+    hipError_t e = hipMemcpyAsync(host, device, bytes, hipMemcpyDeviceToHost, ctx->raster.stream());
+    if (e == hipSuccess) e = hipStreamSynchronize(ctx->raster.stream());
+    return finish(ctx, e == hipSuccess ? lrt::Status{}
+                                       : lrt::Status::error(lrt::ErrorCode::DeviceFailure, hipGetErrorString(e)));
+}
+
 
 lrt_status lrt_write_exr(const char* path, uint32_t width, uint32_t height, const float* rgba, const float* depth,
                          int32_t half) {
@@ -643,6 +662,129 @@ lrt_status lrt_cloud_decode(lrt_context* ctx, const lrt_raw_splats* raw, uint32_
         info->upload_ms = out.uploadMs;
         info->decode_ms = out.decodeMs;
     }
+    return finish(ctx, s);
+}
+
+void lrt_lod_build_settings_default(lrt_lod_build_settings* out) {
+    if (out == nullptr) return;
+    const lrt::lod::LodBuildSettings s;
+    std::memset(out, 0, sizeof *out);
+    out->max_group_fraction = s.maxGroupFraction;
+    out->coarsest_level = s.coarsestLevel;
+}
+
+lrt_status lrt_lod_build(lrt_context* ctx, const lrt_splats* cloud, const lrt_lod_build_settings* settings,
+                         lrt_lod_cloud** out) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (out == nullptr || cloud == nullptr) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    *out = nullptr;
+    lrt::lod::LodBuildSettings s;
+    if (settings != nullptr) {
+        s.maxGroupFraction = settings->max_group_fraction;
+        s.coarsestLevel = settings->coarsest_level;
+    }
+    auto* made = new (std::nothrow) lrt_lod_cloud();
+    if (made == nullptr) return finish(ctx, lrt::Status::error(lrt::ErrorCode::OutOfMemory, "lrt_lod_cloud"));
+    const lrt::Status st = lrt::lod::build(ctx->raster.device(), ctx->raster.stream(), toCloud(*cloud), s, made->lod);
+    if (!st.ok()) {
+        delete made;
+        return finish(ctx, st);
+    }
+    *out = made;
+    return finish(ctx, st);
+}
+
+void lrt_lod_destroy(lrt_lod_cloud* lod) { delete lod; }
+
+lrt_status lrt_lod_get_info(const lrt_lod_cloud* lod, lrt_lod_info* out) {
+    if (lod == nullptr || out == nullptr) return LRT_INVALID_ARGUMENT;
+    std::memset(out, 0, sizeof *out);
+    const lrt::lod::LodCloud& l = lod->lod;
+    out->count = l.count;
+    out->rest_per_colour = l.restPerColour;
+    out->sh_words = l.shWords;
+    out->levels = static_cast<uint32_t>(l.levels.size());
+    out->merged = l.merged;
+    for (size_t k = 0; k < l.levels.size() && k < 10; ++k) {
+        out->level[k] = l.levels[k].level;
+        out->first[k] = l.levels[k].first;
+        out->groups[k] = l.levels[k].count;
+    }
+    for (int k = 0; k < 3; ++k) out->bounds_lo[k] = l.boundsLo[k];
+    out->extent = l.extent;
+    return LRT_OK;
+}
+
+namespace {
+lrt_splats toSplatsC(const lrt::SplatCloud& c) {
+    lrt_splats out{};
+    out.positions = reinterpret_cast<const float*>(c.positions);
+    out.shape = reinterpret_cast<const uint32_t*>(c.shape);
+    out.sh = c.sh;
+    out.filter3d = nullptr;
+    out.count = c.count;
+    out.rest_per_colour = c.restPerColour;
+    out.sh_words = c.shWords;
+    return out;
+}
+}   // namespace
+
+lrt_status lrt_lod_splats(const lrt_lod_cloud* lod, lrt_splats* out, const uint32_t** groups) {
+    if (lod == nullptr || out == nullptr) return LRT_INVALID_ARGUMENT;
+    *out = toSplatsC(lod->lod.splats());
+    if (groups != nullptr) *groups = lod->lod.groups.as<uint32_t>();
+    return LRT_OK;
+}
+
+lrt_status lrt_lod_gaussians(const lrt_lod_cloud* lod, uint32_t level_index, lrt_splats* out,
+                             const uint32_t** cells) {
+    if (lod == nullptr || out == nullptr || level_index >= lod->lod.levels.size()) return LRT_INVALID_ARGUMENT;
+    *out = toSplatsC(lod->lod.gaussians(level_index));
+    if (cells != nullptr) *cells = lod->lod.cells.as<uint32_t>() + lod->lod.levels[level_index].first;
+    return LRT_OK;
+}
+
+lrt_status lrt_lod_select(lrt_context* ctx, const lrt_projection* projection, const lrt_lod_instance* instances,
+                          uint32_t count, float threshold, lrt_splat_instance* out, uint32_t* out_count,
+                          lrt_cut_stats* stats) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (projection == nullptr || out_count == nullptr || (count > 0 && (instances == nullptr || out == nullptr))) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    *out_count = 0;
+    std::vector<lrt::lod::LodInstance> list(count);
+    for (uint32_t k = 0; k < count; ++k) {
+        list[k].cloud = instances[k].cloud != nullptr ? &instances[k].cloud->lod : nullptr;
+        list[k].objectToWorld = toMat(instances[k].object_to_world);
+        list[k].edit = toEdit(instances[k].edit);
+        if (instances[k].has_threshold != 0) list[k].threshold = instances[k].threshold;
+    }
+    std::vector<lrt::SplatInstance> drawn;
+    std::vector<lrt::lod::CutStats> cs;
+    const lrt::Status s = ctx->cut.select(ctx->raster.device(), ctx->raster.stream(), toProjection(*projection), list,
+                                          threshold, drawn, &cs);
+    if (!s.ok()) return finish(ctx, s);
+    ctx->cutClouds.resize(drawn.size());
+    for (size_t k = 0; k < drawn.size(); ++k) {
+        ctx->cutClouds[k] = toSplatsC(*drawn[k].splats);
+        out[k] = lrt_splat_instance{};
+        out[k].splats = &ctx->cutClouds[k];
+        for (int e = 0; e < 16; ++e) out[k].object_to_world[e] = drawn[k].objectToWorld.m[static_cast<size_t>(e)];
+    }
+    // a drawn instance keeps its own edit: the k-th drawn is the k-th instance that drew something
+    size_t next = 0;
+    for (uint32_t k = 0; k < count && next < drawn.size(); ++k) {
+        if (k < cs.size() && cs[k].splats + cs[k].merged > 0) out[next++].edit = instances[k].edit;
+    }
+    if (stats != nullptr) {
+        for (uint32_t k = 0; k < count; ++k) {
+            stats[k] = lrt_cut_stats{};
+            if (k < cs.size()) stats[k] = lrt_cut_stats{cs[k].splats, cs[k].merged, cs[k].available};
+        }
+    }
+    *out_count = static_cast<uint32_t>(drawn.size());
     return finish(ctx, s);
 }
 

@@ -164,6 +164,7 @@ public:
     Status create(int device, hipStream_t stream);
     void setStream(hipStream_t stream) { stream_ = stream; }
     [[nodiscard]] hipStream_t stream() const noexcept { return stream_; }
+    [[nodiscard]] int device() const noexcept { return device_; }
 
     // TileRasterizer::render (TileRasterizer.cpp:188-484). Point instances are
     // drawn as opaque discs sorted with the splats (their slots after the

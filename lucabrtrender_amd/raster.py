@@ -616,6 +616,149 @@ class TileRasterizer:
     def synchronize(self):
         self._check(lib.lrt_context_synchronize(self._ctx))
 
+    def copy_to_host(self, device_ptr: int, nbytes: int, dtype) -> np.ndarray:
+        """Device memory the library owns, read after the stream's queued work."""
+        out = np.empty(nbytes // np.dtype(dtype).itemsize, dtype)
+        if nbytes:
+            self._check(lib.lrt_copy_to_host(self._ctx, C.c_void_p(device_ptr), out.ctypes.data, nbytes))
+        return out
+
+    def lod_build(self, splats, settings: Optional["LodBuildSettings"] = None) -> "LodCloud":
+        """LodBuilder::build (Lod.cpp:79-263): the cloud's levels of detail."""
+        c = (settings or LodBuildSettings()).to_c()
+        h = C.c_void_p()
+        self._check(lib.lrt_lod_build(self._ctx, C.byref(splats.c), C.byref(c), C.byref(h)))
+        return LodCloud(h, self)
+
+    def lod_select(self, camera_or_projection, instances: Sequence["LodInstance"], threshold: float,
+                   settings: RenderSettings):
+        """CutSelector::select (Lod.cpp:346-546): (splat instances to render, one
+        CutStats an instance). The instances' clouds are the context's and are
+        rewritten by the next call."""
+        proj = camera_or_projection
+        if isinstance(proj, Camera):
+            proj = projection_for(proj, settings.width, settings.height)
+        n = len(instances)
+        arr = (_abi.LodInstanceC * max(1, n))()
+        for k, inst in enumerate(instances):
+            arr[k].cloud = inst.cloud.handle if inst.cloud is not None else None
+            arr[k].object_to_world[:] = list(np.asarray(inst.object_to_world, dtype=np.float64).reshape(16))
+            arr[k].edit = inst.edit.to_c()
+            arr[k].has_threshold = 1 if inst.threshold is not None else 0
+            arr[k].threshold = inst.threshold if inst.threshold is not None else 0.0
+        out = (_abi.SplatInstanceC * max(1, n))()
+        count = C.c_uint32()
+        stats = (_abi.CutStatsC * max(1, n))()
+        self._check(lib.lrt_lod_select(self._ctx, C.byref(proj), arr, n, threshold, out, C.byref(count), stats))
+        drawn = []
+        for k in range(count.value):
+            view = SplatsView(out[k].splats.contents)
+            drawn.append(SplatInstance(view, tuple(out[k].object_to_world), _edit_from_c(out[k].edit)))
+        return drawn, [CutStats(stats[k].splats, stats[k].merged, stats[k].available) for k in range(n)]
+
+
+@dataclass
+class LodBuildSettings:
+    """LodBuildSettings (modules/lod/include/lrt/lod/Lod.h:83-93) without chunks."""
+    max_group_fraction: float = 0.5
+    coarsest_level: int = 1
+
+    def to_c(self) -> _abi.LodBuildSettingsC:
+        return _abi.LodBuildSettingsC(self.max_group_fraction, self.coarsest_level)
+
+
+@dataclass
+class CutStats:
+    """CutStats (Lod.h:115-126) without the streaming needs."""
+    splats: int = 0
+    merged: int = 0
+    available: int = 0
+
+
+class SplatsView:
+    """A packed cloud the library owns (an LOD level, a cut), as a SplatInstance's splats."""
+
+    def __init__(self, c: _abi.Splats):
+        self._c = _abi.Splats(c.positions, c.shape, c.sh, c.filter3d, c.count, c.rest_per_colour, c.sh_words)
+        self.count, self.rest_per_colour, self.sh_words = c.count, c.rest_per_colour, c.sh_words
+
+    @property
+    def c(self) -> _abi.Splats:
+        return self._c
+
+    def read(self, rast: "TileRasterizer") -> dict:
+        """positions float32 [N, 4], shape uint32 [N, 4], sh uint32 [N, sh_words] or None."""
+        n, c = self.count, self._c
+        out = dict(positions=rast.copy_to_host(c.positions or 0, n * 16, np.float32).reshape(n, 4),
+                   shape=rast.copy_to_host(c.shape or 0, n * 16, np.uint32).reshape(n, 4), sh=None)
+        if self.rest_per_colour and c.sh:
+            out["sh"] = rast.copy_to_host(c.sh, n * self.sh_words * 4, np.uint32).reshape(n, self.sh_words)
+        return out
+
+
+class LodCloud:
+    """A cloud with its levels of detail (LodCloud, Lod.h:48-81), owned by the library."""
+
+    def __init__(self, handle: C.c_void_p, rast: "TileRasterizer"):
+        self._h = handle
+        self._rast = rast
+        info = _abi.LodInfoC()
+        if lib.lrt_lod_get_info(handle, C.byref(info)) != _abi.LRT_OK:
+            raise LrtError(3, "lrt_lod_get_info")
+        k = int(info.levels)
+        self.count, self.rest_per_colour, self.sh_words = int(info.count), int(info.rest_per_colour), int(info.sh_words)
+        self.levels = [dict(level=int(info.level[i]), first=int(info.first[i]), groups=int(info.groups[i]))
+                       for i in range(k)]
+        self.merged = int(info.merged)
+        self.bounds_lo = tuple(float(v) for v in info.bounds_lo)
+        self.extent = float(info.extent)
+
+    @property
+    def handle(self) -> C.c_void_p:
+        return self._h
+
+    def splats(self) -> tuple:
+        """(the Morton-ordered splats as a SplatsView, each one's finest-level group as numpy)."""
+        c = _abi.Splats()
+        groups = C.c_void_p()
+        if lib.lrt_lod_splats(self._h, C.byref(c), C.byref(groups)) != _abi.LRT_OK:
+            raise LrtError(3, "lrt_lod_splats")
+        return SplatsView(c), self._rast.copy_to_host(groups.value or 0, self.count * 4, np.uint32)
+
+    def gaussians(self, index: int) -> tuple:
+        """(stored level `index`'s merged Gaussians as a SplatsView, their cell codes as numpy)."""
+        c = _abi.Splats()
+        cells = C.c_void_p()
+        if lib.lrt_lod_gaussians(self._h, index, C.byref(c), C.byref(cells)) != _abi.LRT_OK:
+            raise LrtError(3, "lrt_lod_gaussians")
+        return SplatsView(c), self._rast.copy_to_host(cells.value or 0, int(c.count) * 4, np.uint32)
+
+    def close(self):
+        if self._h:
+            lib.lrt_lod_destroy(self._h)
+            self._h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+@dataclass
+class LodInstance:
+    """LodInstance (Lod.h:108-113): threshold None takes the call's."""
+    cloud: Optional[LodCloud]
+    object_to_world: tuple = IDENTITY
+    edit: SplatEdit = field(default_factory=SplatEdit)
+    threshold: Optional[float] = None
+
+
+def _edit_from_c(e: _abi.SplatEdit) -> SplatEdit:
+    return SplatEdit(active=bool(e.active), shape=e.shape, mode=e.mode, invert=bool(e.invert),
+                     centre=tuple(e.centre), size=tuple(e.size), tint=tuple(e.tint), saturation=e.saturation,
+                     brightness=e.brightness, opacity=e.opacity, min_opacity=e.min_opacity, max_scale=e.max_scale)
+
 
 def frame_params_bytes(projection: _abi.ProjectionC, instance: SplatInstance, settings: RenderSettings,
                        base: int = 0, count: int = 0, rest_per_colour: int = 0) -> bytes:

@@ -1697,3 +1697,292 @@ void or_compare_images(const float* a, const float* b, uint32_t width, uint32_t 
     *max = mx;
     *over2 = o2;
 }
+
+/* ====================================================================== */
+/* Levels of detail (modules/lod, shaders/lrt/lod)                         */
+/* ====================================================================== */
+
+/* morton.slang:7-34 */
+static uint32_t spread_bits(uint32_t v) {
+    v &= 0x3FFu;
+    v = (v | (v << 16)) & 0x030000FFu;
+    v = (v | (v << 8)) & 0x0300F00Fu;
+    v = (v | (v << 4)) & 0x030C30C3u;
+    v = (v | (v << 2)) & 0x09249249u;
+    return v;
+}
+
+static uint32_t compact_bits(uint32_t v) {
+    v &= 0x09249249u;
+    v = (v | (v >> 2)) & 0x030C30C3u;
+    v = (v | (v >> 4)) & 0x0300F00Fu;
+    v = (v | (v >> 8)) & 0x030000FFu;
+    v = (v | (v >> 16)) & 0x000003FFu;
+    return v;
+}
+
+static uint32_t quantise_unit(float u) {
+    const float s = u > 0.0f ? (u < 1.0f ? u : 1.0f) : 0.0f;   /* saturate */
+    const uint32_t q = (uint32_t)(s * 1024.0f);
+    return q < 1023u ? q : 1023u;
+}
+
+uint32_t or_morton30(float ux, float uy, float uz) {
+    return (spread_bits(quantise_unit(ux)) << 2) | (spread_bits(quantise_unit(uy)) << 1) |
+           spread_bits(quantise_unit(uz));
+}
+
+/* lodMorton (lod_morton.slang:12-21) */
+void or_lod_morton(const float* positions, uint32_t n, const float lo[3], float extent, uint64_t* keys,
+                   uint32_t* values) {
+    const float e = fmax_(extent, 1e-20f);
+    for (uint32_t i = 0; i < n; ++i) {
+        const float* p = positions + (size_t)i * 4;
+        keys[i] = or_morton30((p[0] - lo[0]) / e, (p[1] - lo[1]) / e, (p[2] - lo[2]) / e);
+        values[i] = i;
+    }
+}
+
+/* lodBoundaries + PrefixSum + lodGroups (lod_boundaries.slang:12-21,
+ * lod_groups.slang:19-30): each splat's group at `level`, each group's first
+ * splat and cell code (top 3 level bits of the 30-bit code). Returns the groups. */
+uint32_t or_lod_groups(const uint64_t* keys, uint32_t n, uint32_t level, uint32_t* group, uint32_t* starts,
+                       uint32_t* cells) {
+    const uint32_t shift = 3u * (10u - level);
+    uint32_t groups = 0;
+    for (uint32_t i = 0; i < n; ++i) {
+        const uint32_t code = (uint32_t)keys[i];
+        const int boundary = i == 0 || (code >> shift) != ((uint32_t)keys[i - 1] >> shift);
+        if (boundary) {
+            if (starts) starts[groups] = i;
+            if (cells) cells[groups] = code >> shift;
+            groups++;
+        }
+        if (group) group[i] = groups - 1;
+    }
+    return groups;
+}
+
+/* lodLeafMoments (lod_leaf_moments.slang:21-64): per group, in splat order,
+ * W, S1, S2 (xx xy xz yy yz zz), base colour, then keep * 3 rest halves. */
+void or_lod_leaf_moments(const float* positions, const uint32_t* shape, const uint32_t* sh, uint32_t n,
+                         const uint32_t* starts, uint32_t groups, uint32_t keep, uint32_t sh_words, float* moments) {
+    const uint32_t stride = 13u + keep * 3u;
+    for (uint32_t g = 0; g < groups; ++g) {
+        const uint32_t first = starts[g], end = g + 1 < groups ? starts[g + 1] : n;
+        float* m = moments + (size_t)g * stride;
+        for (uint32_t k = 0; k < stride; ++k) m[k] = 0.0f;
+        for (uint32_t i = first; i < end; ++i) {
+            const float* p = positions + (size_t)i * 4;
+            const uint32_t* row = shape + (size_t)i * 4;
+            const float s[3] = {or_expf(low_half(row[1])), or_expf(high_half(row[1])), or_expf(low_half(row[2]))};
+            float q[4];
+            or_decode_quaternion(row[0], q);
+            const float x = q[0], y = q[1], z = q[2], w = q[3];
+            const float r[9] = {1.0f - 2.0f * (y * y + z * z), 2.0f * (x * y - w * z), 2.0f * (x * z + w * y),
+                                2.0f * (x * y + w * z), 1.0f - 2.0f * (x * x + z * z), 2.0f * (y * z - w * x),
+                                2.0f * (x * z - w * y), 2.0f * (y * z + w * x), 1.0f - 2.0f * (x * x + y * y)};
+            float mm[9], cov[9];
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c) mm[a * 3 + c] = r[a * 3 + c] * s[c];
+            for (int a = 0; a < 3; ++a)
+                for (int c = 0; c < 3; ++c)
+                    cov[a * 3 + c] = (mm[a * 3] * mm[c * 3] + mm[a * 3 + 1] * mm[c * 3 + 1]) + mm[a * 3 + 2] * mm[c * 3 + 2];
+            const float smallest = fmin_(fmin_(s[0], s[1]), s[2]);
+            const float area = s[0] * s[1] * s[2] / fmax_(smallest, 1e-20f);
+            const float wt = p[3] * area;
+            m[0] += wt;
+            m[1] += wt * p[0];
+            m[2] += wt * p[1];
+            m[3] += wt * p[2];
+            m[4] += wt * (cov[0] + p[0] * p[0]);
+            m[5] += wt * (cov[1] + p[0] * p[1]);
+            m[6] += wt * (cov[2] + p[0] * p[2]);
+            m[7] += wt * (cov[4] + p[1] * p[1]);
+            m[8] += wt * (cov[5] + p[1] * p[2]);
+            m[9] += wt * (cov[8] + p[2] * p[2]);
+            m[10] += wt * high_half(row[2]);
+            m[11] += wt * low_half(row[3]);
+            m[12] += wt * high_half(row[3]);
+            for (uint32_t h = 0; h < keep * 3; ++h) m[13 + h] += wt * half_at(sh + (size_t)i * sh_words, h);
+        }
+    }
+}
+
+/* lodMergeMoments (lod_merge_moments.slang:17-32) */
+void or_lod_merge_moments(const uint32_t* starts, uint32_t groups, const uint32_t* fine_group, uint32_t fine_groups,
+                          const float* fine_moments, uint32_t stride, float* moments) {
+    for (uint32_t g = 0; g < groups; ++g) {
+        const uint32_t first = fine_group[starts[g]];
+        const uint32_t end = g + 1 < groups ? fine_group[starts[g + 1]] : fine_groups;
+        for (uint32_t k = 0; k < stride; ++k) {
+            float sum = 0.0f;
+            for (uint32_t h = first; h < end; ++h) sum += fine_moments[(size_t)h * stride + k];
+            moments[(size_t)g * stride + k] = sum;
+        }
+    }
+}
+
+typedef struct { float a[3][3]; } m3;
+
+static m3 m3_mul(const m3* x, const m3* y) {
+    m3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) r.a[i][k] = (x->a[i][0] * y->a[0][k] + x->a[i][1] * y->a[1][k]) + x->a[i][2] * y->a[2][k];
+    return r;
+}
+
+static m3 m3_transposed(const m3* x) {
+    m3 r;
+    for (int i = 0; i < 3; ++i)
+        for (int k = 0; k < 3; ++k) r.a[i][k] = x->a[k][i];
+    return r;
+}
+
+static float sign_of(float x) { return x > 0.0f ? 1.0f : (x < 0.0f ? -1.0f : 0.0f); }
+
+/* jacobi (lod_finalize.slang:16-41): six cyclic sweeps over (0,1), (0,2), (1,2). */
+static void jacobi3(m3* a, m3* v) {
+    const m3 id = {{{1, 0, 0}, {0, 1, 0}, {0, 0, 1}}};
+    *v = id;
+    for (int sweep = 0; sweep < 6; ++sweep) {
+        for (int pq = 0; pq < 3; ++pq) {
+            const int p = pq == 2 ? 1 : 0;
+            const int q = pq == 0 ? 1 : 2;
+            const float apq = a->a[p][q];
+            if (fabsf(apq) < 1e-30f) continue;
+            const float theta = (a->a[q][q] - a->a[p][p]) / (2.0f * apq);
+            const float t = sign_of(theta + 1e-30f) / (fabsf(theta) + sqrtf(theta * theta + 1.0f));
+            const float c = 1.0f / sqrtf(t * t + 1.0f);
+            const float s = t * c;
+            m3 j = id;
+            j.a[p][p] = c;
+            j.a[q][q] = c;
+            j.a[p][q] = s;
+            j.a[q][p] = -s;
+            const m3 aj = m3_mul(a, &j);
+            const m3 jt = m3_transposed(&j);
+            *a = m3_mul(&jt, &aj);
+            *v = m3_mul(v, &j);
+        }
+    }
+}
+
+/* quaternionOfAxes (packing.slang:76-93) */
+static void quaternion_of_axes(const m3* m, float q[4]) {
+    const float trace = m->a[0][0] + m->a[1][1] + m->a[2][2];
+    if (trace > 0.0f) {
+        const float s = 0.5f / sqrtf(trace + 1.0f);
+        q[0] = (m->a[2][1] - m->a[1][2]) * s;
+        q[1] = (m->a[0][2] - m->a[2][0]) * s;
+        q[2] = (m->a[1][0] - m->a[0][1]) * s;
+        q[3] = 0.25f / s;
+    } else if (m->a[0][0] > m->a[1][1] && m->a[0][0] > m->a[2][2]) {
+        const float s = 2.0f * sqrtf(1.0f + m->a[0][0] - m->a[1][1] - m->a[2][2]);
+        q[0] = 0.25f * s;
+        q[1] = (m->a[1][0] + m->a[0][1]) / s;
+        q[2] = (m->a[0][2] + m->a[2][0]) / s;
+        q[3] = (m->a[2][1] - m->a[1][2]) / s;
+    } else if (m->a[1][1] > m->a[2][2]) {
+        const float s = 2.0f * sqrtf(1.0f - m->a[0][0] + m->a[1][1] - m->a[2][2]);
+        q[0] = (m->a[1][0] + m->a[0][1]) / s;
+        q[1] = 0.25f * s;
+        q[2] = (m->a[2][1] + m->a[1][2]) / s;
+        q[3] = (m->a[0][2] - m->a[2][0]) / s;
+    } else {
+        const float s = 2.0f * sqrtf(1.0f - m->a[0][0] - m->a[1][1] + m->a[2][2]);
+        q[0] = (m->a[0][2] + m->a[2][0]) / s;
+        q[1] = (m->a[2][1] + m->a[1][2]) / s;
+        q[2] = 0.25f * s;
+        q[3] = (m->a[1][0] - m->a[0][1]) / s;
+    }
+    const float len = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
+    for (int k = 0; k < 4; ++k) q[k] = q[k] / len;
+}
+
+/* lodFinalize (lod_finalize.slang:43-97): moments -> one packed Gaussian a group. */
+void or_lod_finalize(const float* moments, uint32_t groups, uint32_t keep, uint32_t sh_words, float* positions,
+                     uint32_t* shape, uint32_t* sh) {
+    const uint32_t stride = 13u + keep * 3u;
+    for (uint32_t g = 0; g < groups; ++g) {
+        const float* m = moments + (size_t)g * stride;
+        const float weight = fmax_(m[0], 1e-30f);
+        const float mu[3] = {m[1] / weight, m[2] / weight, m[3] / weight};
+        m3 cov = {{{m[4] / weight, m[5] / weight, m[6] / weight},
+                   {m[5] / weight, m[7] / weight, m[8] / weight},
+                   {m[6] / weight, m[8] / weight, m[9] / weight}}};
+        for (int i = 0; i < 3; ++i)
+            for (int k = 0; k < 3; ++k) cov.a[i][k] = cov.a[i][k] - mu[i] * mu[k];
+        m3 axes;
+        jacobi3(&cov, &axes);
+        const float det = axes.a[0][0] * (axes.a[1][1] * axes.a[2][2] - axes.a[1][2] * axes.a[2][1]) -
+                          axes.a[0][1] * (axes.a[1][0] * axes.a[2][2] - axes.a[1][2] * axes.a[2][0]) +
+                          axes.a[0][2] * (axes.a[1][0] * axes.a[2][1] - axes.a[1][1] * axes.a[2][0]);
+        if (det < 0.0f) {
+            axes.a[0][2] = -axes.a[0][2];
+            axes.a[1][2] = -axes.a[1][2];
+            axes.a[2][2] = -axes.a[2][2];
+        }
+        const float s[3] = {sqrtf(fmax_(cov.a[0][0], 1e-14f)), sqrtf(fmax_(cov.a[1][1], 1e-14f)),
+                            sqrtf(fmax_(cov.a[2][2], 1e-14f))};
+        const float smallest = fmin_(fmin_(s[0], s[1]), s[2]);
+        const float area = s[0] * s[1] * s[2] / fmax_(smallest, 1e-20f);
+        const float opacity = fmin_(m[0] / fmax_(area, 1e-30f), 0.99f);
+        float* pos = positions + (size_t)g * 4;
+        pos[0] = mu[0]; pos[1] = mu[1]; pos[2] = mu[2]; pos[3] = opacity;
+        float q[4];
+        quaternion_of_axes(&axes, q);
+        uint32_t* row = shape + (size_t)g * 4;
+        row[0] = or_encode_quaternion(q);
+        row[1] = or_pack_halves(half_safe(or_logf(s[0])), half_safe(or_logf(s[1])));
+        row[2] = or_pack_halves(half_safe(or_logf(s[2])), half_safe(m[10] / weight));
+        row[3] = or_pack_halves(half_safe(m[11] / weight), half_safe(m[12] / weight));
+        if (keep == 0) continue;
+        uint32_t word = 0, filled = 0, written = 0;
+        for (uint32_t h = 0; h < keep * 3; ++h) {
+            word |= (uint32_t)or_float_to_half(half_safe(m[13 + h] / weight)) << (filled * 16);
+            if (++filled == 2) {
+                sh[(size_t)g * sh_words + written++] = word;
+                word = 0;
+                filled = 0;
+            }
+        }
+        if (filled == 1) sh[(size_t)g * sh_words + written] = word;
+    }
+}
+
+/* projectedEdge (lod_cut.slang:13-24): the pixels a level-`level` cell spans
+ * at its nearest point to the eye. */
+float or_lod_projected_edge(const or_cut_params* p, uint32_t cell, uint32_t level) {
+    const float edge = p->extent / (float)(1u << level);
+    if (p->orthographic) return edge * p->pixels_per_unit;
+    const float lo[3] = {p->lo[0] + (float)compact_bits(cell >> 2) * edge, p->lo[1] + (float)compact_bits(cell >> 1) * edge,
+                         p->lo[2] + (float)compact_bits(cell) * edge};
+    float d[3];
+    for (int k = 0; k < 3; ++k) d[k] = clampf_(p->eye[k], lo[k], lo[k] + edge) - p->eye[k];
+    const float distance = fmax_(sqrtf((d[0] * d[0] + d[1] * d[1]) + d[2] * d[2]), 1e-6f);
+    return edge * p->pixels_per_unit / distance;
+}
+
+static int lod_small(const or_cut_params* p, uint32_t cell, uint32_t level) {
+    return p->threshold > 0.0f && or_lod_projected_edge(p, cell, level) <= p->threshold;
+}
+
+/* lodCutGroups, and lodCutFinest with every chunk resident (lod_cut.slang:
+ * 30-80): drawn when the cell is small and (below the coarsest stored level)
+ * its parent's is not. */
+void or_lod_cut_groups(const or_cut_params* p, const uint32_t* cells, uint32_t count, uint32_t level, int coarsest,
+                       uint32_t* selected) {
+    for (uint32_t g = 0; g < count; ++g) {
+        const int here = lod_small(p, cells[g], level);
+        const int above = !coarsest && lod_small(p, cells[g] >> 3, level - 1);
+        selected[g] = (here && !above) ? 1u : 0u;
+    }
+}
+
+/* lodCutSplats through lodCutFinest's state (lod_cut.slang:56-92): a splat is
+ * drawn when its finest-level cell is not small. */
+void or_lod_cut_splats(const or_cut_params* p, const uint32_t* groups, uint32_t n, const uint32_t* finest_cells,
+                       uint32_t finest_level, uint32_t* selected) {
+    for (uint32_t i = 0; i < n; ++i) selected[i] = lod_small(p, finest_cells[groups[i]], finest_level) ? 0u : 1u;
+}

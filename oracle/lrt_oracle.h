@@ -260,6 +260,40 @@ typedef struct {
 void or_diff_render_f64(const or_frame* frame, const or_diff_cloud* cloud, const uint32_t* px, const uint32_t* py,
                         uint32_t npix, double* colour, double* depth, int keyed, int threads);
 
+/* ---- levels of detail (modules/lod/src/Lod.cpp, shaders/lrt/lod) ------- */
+/* The 30-bit Morton code of a point at unit coordinates (morton.slang:27-30). */
+uint32_t or_morton30(float ux, float uy, float uz);
+/* lodMorton: keys of positions (float4) in the cube [lo, lo + extent]^3, values = index. */
+void or_lod_morton(const float* positions, uint32_t n, const float lo[3], float extent, uint64_t* keys,
+                   uint32_t* values);
+/* lodBoundaries + PrefixSum + lodGroups at `level` over Morton-sorted keys:
+ * group a splat, starts/cells a group (any may be NULL). Returns the groups. */
+uint32_t or_lod_groups(const uint64_t* keys, uint32_t n, uint32_t level, uint32_t* group, uint32_t* starts,
+                       uint32_t* cells);
+/* lodLeafMoments / lodMergeMoments: 13 + 3 keep floats a group. */
+void or_lod_leaf_moments(const float* positions, const uint32_t* shape, const uint32_t* sh, uint32_t n,
+                         const uint32_t* starts, uint32_t groups, uint32_t keep, uint32_t sh_words, float* moments);
+void or_lod_merge_moments(const uint32_t* starts, uint32_t groups, const uint32_t* fine_group, uint32_t fine_groups,
+                          const float* fine_moments, uint32_t stride, float* moments);
+/* lodFinalize: one packed Gaussian a group. */
+void or_lod_finalize(const float* moments, uint32_t groups, uint32_t keep, uint32_t sh_words, float* positions,
+                     uint32_t* shape, uint32_t* sh);
+/* The cut's view of a cloud (LodParams, lod_common.slang:40-62, and
+ * Lod.cpp:410-433): the bounds, the eye in the cloud's space, pixels a unit. */
+typedef struct {
+    float lo[3];
+    float extent;
+    float eye[3];
+    float pixels_per_unit;
+    float threshold;
+    uint32_t orthographic;
+} or_cut_params;
+float or_lod_projected_edge(const or_cut_params* p, uint32_t cell, uint32_t level);
+void or_lod_cut_groups(const or_cut_params* p, const uint32_t* cells, uint32_t count, uint32_t level, int coarsest,
+                       uint32_t* selected);
+void or_lod_cut_splats(const or_cut_params* p, const uint32_t* groups, uint32_t n, const uint32_t* finest_cells,
+                       uint32_t finest_level, uint32_t* selected);
+
 /* compareImages (modules/render/src/ReferenceRenderer.cpp:248-297 with
  * shaders/lrt/reference/image_compare.slang:27-51): 8-bit sRGB code-value
  * histogram of the worst channel; p99, max, pixels over 2. */

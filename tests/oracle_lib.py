@@ -117,6 +117,15 @@ _SIGS = {
                                        C.c_int, C.c_int]),
     "or_reference_render_f64": (None, [C.POINTER(OrFrame), C.POINTER(OrCloud), C.c_uint32, P, P, C.c_uint32, P, P,
                                        C.c_int, C.c_int]),
+    "or_morton30": (C.c_uint32, [C.c_float, C.c_float, C.c_float]),
+    "or_lod_morton": (None, [P, C.c_uint32, C.POINTER(C.c_float), C.c_float, P, P]),
+    "or_lod_groups": (C.c_uint32, [P, C.c_uint32, C.c_uint32, P, P, P]),
+    "or_lod_leaf_moments": (None, [P, P, P, C.c_uint32, P, C.c_uint32, C.c_uint32, C.c_uint32, P]),
+    "or_lod_merge_moments": (None, [P, C.c_uint32, P, C.c_uint32, P, C.c_uint32, P]),
+    "or_lod_finalize": (None, [P, C.c_uint32, C.c_uint32, C.c_uint32, P, P, P]),
+    "or_lod_projected_edge": (C.c_float, [C.c_void_p, C.c_uint32, C.c_uint32]),
+    "or_lod_cut_groups": (None, [C.c_void_p, P, C.c_uint32, C.c_uint32, C.c_int, P]),
+    "or_lod_cut_splats": (None, [C.c_void_p, P, C.c_uint32, P, C.c_uint32, P]),
     "or_compare_images": (None, [P, P, C.c_uint32, C.c_uint32, C.POINTER(C.c_uint32), C.POINTER(C.c_uint32),
                                  C.POINTER(C.c_uint64)]),
     "or_srgb_to_linear": (C.c_float, [C.c_float]),
@@ -351,3 +360,104 @@ def diff_render_f64(frame: OrFrame, params: dict, filter3d=None, threads=0) -> n
     colour = np.zeros((frame.height, frame.width, 4), np.float64)
     lib.or_diff_render_f64(C.byref(frame), C.byref(cloud), None, None, 0, ptr(colour), None, 1, threads)
     return colour
+
+
+# ---- levels of detail ---------------------------------------------------------------
+
+class OrCutParams(C.Structure):
+    _fields_ = [("lo", C.c_float * 3), ("extent", C.c_float), ("eye", C.c_float * 3), ("pixels_per_unit", C.c_float),
+                ("threshold", C.c_float), ("orthographic", C.c_uint32)]
+
+
+def lod_build(positions, shape, sh, rest_per_colour, max_group_fraction=0.5, coarsest_level=1):
+    """LodBuilder::build (modules/lod/src/Lod.cpp:79-263) over the oracle's stages:
+    bounds and extent, Morton keys and the stable 30-bit sort, the splats in
+    that order, the finest level worth storing, then from it up each stored
+    level's groups, moments and Gaussians. Returns a dict of host arrays."""
+    n = positions.shape[0]
+    assert n > 0
+    keep = rest_per_colour
+    words = 1 if keep == 0 else (keep * 3 + 1) // 2
+    lo = positions[:, :3].min(axis=0).astype(np.float32)
+    hi = positions[:, :3].max(axis=0).astype(np.float32)
+    extent = np.float32(0.0)
+    for k in range(3):
+        extent = max(extent, np.float32(hi[k] - lo[k]))
+    extent = np.float32(max(extent, np.float32(1e-6))) * np.float32(1.0001)
+    keys = np.zeros(n, np.uint64)
+    order = np.zeros(n, np.uint32)
+    lo_c = (C.c_float * 3)(*[float(v) for v in lo])
+    lib.or_lod_morton(ptr(np.ascontiguousarray(positions, np.float32)), n, lo_c, float(extent), ptr(keys), ptr(order))
+    lib.or_radix_sort(ptr(keys), ptr(order), n, 30)
+    pos = np.ascontiguousarray(positions[order], np.float32)
+    shp = np.ascontiguousarray(shape[order], np.uint32)
+    shs = np.ascontiguousarray(sh[order], np.uint32) if keep else np.zeros((n, 1), np.uint32)
+
+    def groups_at(r):
+        group = np.zeros(n, np.uint32)
+        starts = np.zeros(n, np.uint32)
+        cells = np.zeros(n, np.uint32)
+        g = lib.or_lod_groups(ptr(keys), n, r, ptr(group), ptr(starts), ptr(cells))
+        return group, starts[:g].copy(), cells[:g].copy()
+
+    coarsest = min(max(coarsest_level, 1), 10)
+    finest = coarsest
+    levels = {}
+    for r in range(coarsest, 11):
+        levels[r] = groups_at(r)
+        if np.float32(len(levels[r][1])) > np.float32(max_group_fraction) * np.float32(n) and r > coarsest:
+            del levels[r]
+            break
+        finest = r
+    stride = 13 + keep * 3
+    stored = []
+    fine_moments = None
+    for r in range(finest, coarsest - 1, -1):
+        group, starts, cells = levels[r]
+        gcount = len(starts)
+        moments = np.zeros((gcount, stride), np.float32)
+        if r == finest:
+            lib.or_lod_leaf_moments(ptr(pos), ptr(shp), ptr(shs), n, ptr(starts), gcount, keep, words, ptr(moments))
+        else:
+            fine_group = levels[r + 1][0]
+            lib.or_lod_merge_moments(ptr(starts), gcount, ptr(fine_group), len(levels[r + 1][1]), ptr(fine_moments),
+                                     stride, ptr(moments))
+        gp = np.zeros((gcount, 4), np.float32)
+        gs = np.zeros((gcount, 4), np.uint32)
+        gh = np.zeros((gcount, words), np.uint32)
+        lib.or_lod_finalize(ptr(moments), gcount, keep, words, ptr(gp), ptr(gs), ptr(gh))
+        stored.append(dict(level=r, positions=gp, shape=gs, sh=gh if keep else None, cells=cells, starts=starts,
+                           group=group, moments=moments))
+        fine_moments = moments
+    stored.reverse()
+    return dict(count=n, keep=keep, sh_words=words, lo=lo, extent=np.float32(extent), keys=keys, order=order,
+                positions=pos, shape=shp, sh=shs if keep else None, levels=stored,
+                groups=stored[-1]["group"])
+
+
+def lod_cut(lod, eye, pixels_per_unit, threshold, orthographic=False):
+    """CutSelector::select for one instance held in memory (Lod.cpp:346-546):
+    every level's selected Gaussians (coarsest first), then the selected splats
+    in Morton order. Returns (positions, shape, sh, merged drawn, splats drawn)."""
+    p = OrCutParams()
+    p.lo[:] = [float(v) for v in lod["lo"]]
+    p.extent = float(lod["extent"])
+    p.eye[:] = [float(v) for v in eye]
+    p.pixels_per_unit = float(pixels_per_unit)
+    p.threshold = float(threshold)
+    p.orthographic = 1 if orthographic else 0
+    parts = []
+    for k, level in enumerate(lod["levels"]):
+        sel = np.zeros(len(level["cells"]), np.uint32)
+        lib.or_lod_cut_groups(C.byref(p), ptr(level["cells"]), len(level["cells"]), level["level"], 1 if k == 0 else 0,
+                              ptr(sel))
+        parts.append((sel.astype(bool), level["positions"], level["shape"], level["sh"]))
+    finest = lod["levels"][-1]
+    sel = np.zeros(lod["count"], np.uint32)
+    lib.or_lod_cut_splats(C.byref(p), ptr(lod["groups"]), lod["count"], ptr(finest["cells"]), finest["level"], ptr(sel))
+    parts.append((sel.astype(bool), lod["positions"], lod["shape"], lod["sh"]))
+    positions = np.concatenate([pp[s] for s, pp, _, _ in parts])
+    shape = np.concatenate([ss[s] for s, _, ss, _ in parts])
+    sh = np.concatenate([hh[s] for s, _, _, hh in parts]) if lod["keep"] else None
+    merged = int(sum(s.sum() for s, _, _, _ in parts[:-1]))
+    return positions, shape, sh, merged, int(parts[-1][0].sum())
