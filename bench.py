@@ -334,6 +334,59 @@ def composite_leg(torch, raster, _abi, cfg, dev, settings, reps):
     return out
 
 
+def lod_leg(torch, raster, scenes, cfg, dev, settings, reps):
+    """Levels of detail on the benchmark scene (3,000,000 splats): the build's
+    wall time, then a far view at 1920x1080 (orbit radius 60) drawn whole and
+    through cuts at 2, 4 and 8 pixels: the cut's wall time (its one wait for
+    the counts included), the frame's, what was drawn, and the mean absolute
+    colour difference from the whole cloud's frame. Both frames are drawn
+    without the 3D filter, which merged Gaussians do not carry."""
+    settings = raster.RenderSettings(**{**settings.__dict__, "filter3d": False})
+    rast = raster.TileRasterizer()
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    lod = rast.lod_build(dev)
+    build_ms = (time.perf_counter() - t0) * 1e3
+    cam = scenes.orbit_camera(60.0, 0.0)
+    proj = raster.projection_for(cam, cfg.width, cfg.height)
+    full_t = raster.RenderTargets(cfg.width, cfg.height)
+    cut_t = raster.RenderTargets(cfg.width, cfg.height)
+    inst = [raster.SplatInstance(dev)]
+    for _ in range(2):
+        rast.render(proj, inst, settings, full_t)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(reps):
+        rast.render(proj, inst, settings, full_t)
+    torch.cuda.synchronize()
+    out = {"workload": "scene-lod-3M-1080p-far", "splats": lod.count, "merged": lod.merged,
+           "levels": [lv["level"] for lv in lod.levels], "build_ms": round(build_ms, 2),
+           "off": {"render_ms": round((time.perf_counter() - t0) * 1e3 / reps, 4)}}
+    for threshold in (2.0, 4.0, 8.0):
+        linst = [raster.LodInstance(lod)]
+        for _ in range(2):
+            drawn, stats = rast.lod_select(proj, linst, threshold, settings)
+            rast.render(proj, drawn, settings, cut_t)
+        torch.cuda.synchronize()
+        cut_ms = frame_ms = 0.0
+        for _ in range(reps):
+            t0 = time.perf_counter()
+            drawn, stats = rast.lod_select(proj, linst, threshold, settings)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            rast.render(proj, drawn, settings, cut_t)
+            torch.cuda.synchronize()
+            t2 = time.perf_counter()
+            cut_ms += (t1 - t0) * 1e3
+            frame_ms += (t2 - t1) * 1e3
+        mad = float((cut_t.colour - full_t.colour).abs().mean())
+        out[f"{threshold:g}px"] = {"drawn": stats[0].splats + stats[0].merged, "splats": stats[0].splats,
+                                   "merged": stats[0].merged, "cut_ms": round(cut_ms / reps, 4),
+                                   "render_ms": round(frame_ms / reps, 4), "mean_abs": round(mad, 6)}
+    lod.close()
+    return out
+
+
 def sort_stress_leg(torch, raster, scenes, _abi, reps):
     """Config 5: 6,000,000 splats at 3840x2160, 80% as config 1 and 20% large splats
     near the camera; the frame's stages timed, the sort's keys/s and HBM fraction."""
@@ -432,6 +485,7 @@ def main():
     ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")
     ap.add_argument("--load-reps", type=int, default=3, help="3M-record decodes timed; 0 skips it")
+    ap.add_argument("--lod-reps", type=int, default=5, help="LOD cuts and frames timed; 0 skips it")
     ap.add_argument("--composite-reps", type=int, default=5,
                     help="composite frames (1M discs with the 3M splats) timed; 0 skips it")
     ap.add_argument("--profile-frames", type=int, default=0,
@@ -543,6 +597,11 @@ def main():
         log("composite frames (3M splats + 1M discs, 1080p) ...")
         composite = composite_leg(torch, raster, _abi, cfg, dev, settings, args.composite_reps)
 
+    lod = None
+    if args.lod_reps > 0:
+        log("levels of detail (3M splats, far view at 1080p) ...")
+        lod = lod_leg(torch, raster, scenes, cfg, dev, settings, args.lod_reps)
+
     load = None
     if args.load_reps > 0:
         log("load path (3M PLY records) ...")
@@ -580,6 +639,7 @@ def main():
         "sort_stress": stress,
         "load": load,
         "composite": composite,
+        "lod": lod,
         "roofline": roofline,
         "cpu_baseline": cpu,
     }

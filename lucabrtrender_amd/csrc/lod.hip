@@ -392,6 +392,12 @@ __global__ __launch_bounds__(kBlock) void cutKernel(CutParams p, const uint32_t*
     selected[e] = drawn ? 1u : 0u;
 }
 
+// The drawn elements of a block of kBlock, compacted: they keep their order,
+// so their destinations are consecutive from the block's first (the exclusive
+// sum at its first element). The block lists them in LDS, copies their
+// positions and shapes one thread an element, then their harmonic rows one
+// word a thread across the block, so both sides of the row copy are runs of
+// consecutive words.
 __global__ __launch_bounds__(kBlock) void gatherKernel(uint32_t merged, uint32_t total,
                                                        const uint32_t* __restrict__ selected,
                                                        const uint32_t* __restrict__ dest,
@@ -403,16 +409,36 @@ __global__ __launch_bounds__(kBlock) void gatherKernel(uint32_t merged, uint32_t
                                                        const uint32_t* __restrict__ splatSh, uint32_t shWords,
                                                        float4* __restrict__ positions, uint4* __restrict__ shape,
                                                        uint32_t* __restrict__ sh) {
-    const uint32_t e = blockIdx.x * kBlock + threadIdx.x;
-    if (e >= total || selected[e] == 0) return;
-    const uint32_t j = dest[e];
-    const bool isMerged = e < merged;
-    const uint32_t k = isMerged ? e : e - merged;
-    positions[j] = isMerged ? mergedPositions[k] : splatPositions[k];
-    shape[j] = isMerged ? mergedShape[k] : splatShape[k];
-    if (sh != nullptr) {
-        const uint32_t* src = (isMerged ? mergedSh : splatSh) + static_cast<uint64_t>(k) * shWords;
-        for (uint32_t w = 0; w < shWords; ++w) sh[static_cast<uint64_t>(j) * shWords + w] = src[w];
+    __shared__ uint32_t sElement[kBlock];
+    const uint32_t first = blockIdx.x * kBlock;
+    const uint32_t last = first + kBlock - 1u < total ? first + kBlock - 1u : total - 1u;
+    const uint32_t base = dest[first];
+    const uint32_t count = dest[last] + selected[last] - base;
+    const uint32_t e = first + threadIdx.x;
+    if (e < total && selected[e] != 0) sElement[dest[e] - base] = e;
+    __syncthreads();
+    if (threadIdx.x < count) {
+        const uint32_t k = sElement[threadIdx.x];
+        const bool isMerged = k < merged;
+        positions[base + threadIdx.x] = isMerged ? mergedPositions[k] : splatPositions[k - merged];
+        shape[base + threadIdx.x] = isMerged ? mergedShape[k] : splatShape[k - merged];
+    }
+    if (sh == nullptr) return;
+    const uint32_t words = count * shWords;
+    uint32_t row = threadIdx.x / shWords, col = threadIdx.x - row * shWords;
+    const uint32_t rowStep = kBlock / shWords, colStep = kBlock - rowStep * shWords;
+    uint32_t* out = sh + static_cast<uint64_t>(base) * shWords;
+    for (uint32_t w = threadIdx.x; w < words; w += kBlock) {
+        const uint32_t k = sElement[row];
+        const uint32_t* src = k < merged ? mergedSh + static_cast<uint64_t>(k) * shWords
+                                         : splatSh + static_cast<uint64_t>(k - merged) * shWords;
+        out[w] = src[col];
+        row += rowStep;
+        col += colStep;
+        if (col >= shWords) {
+            col -= shWords;
+            row += 1;
+        }
     }
 }
 
