@@ -4,8 +4,9 @@
 // FP64 brute force with --technique reference); an EXR out, or timed frames.
 // Options, defaults (1920x1080, focal 35 mm, near 0.01, degree 3, eye 1.2 x
 // the clouds' extent in front of their centre) and the lines printed are the
-// reference's. Points, levels of detail and ray tracing are not this
-// library's: their options are refused.
+// reference's. --lod draws each cloud through its levels of detail, cut
+// every frame (CmdRender.cpp:275-398). Point files, the streamed .lrtc store
+// and ray tracing are not this library's: their options are refused.
 #include <hip/hip_runtime_api.h>
 
 #include <algorithm>
@@ -31,13 +32,14 @@ struct Options {
     unsigned degree = 3;
     bool noAntialias = false, stages = false;
     int repeat = 1;
+    double lod = 0.0;   // pixels a merged cell may span; 0 draws the splats
 };
 
 void usage(const char* command) {
     std::fprintf(stderr,
                  "usage: lrt %s --splats FILE [--splats FILE ...] [--technique raster|reference]\n"
                  "       [--eye X Y Z] [--target X Y Z] [--up X Y Z] [--rotate-x DEG] [--scale X Y Z]\n"
-                 "       [--focal MM] [--size WxH] [--near Z] [--degree 0..3] [--no-antialias]%s\n",
+                 "       [--focal MM] [--size WxH] [--near Z] [--degree 0..3] [--no-antialias] [--lod PX]%s\n",
                  command, std::strcmp(command, "bench") == 0 ? " [--repeat N] [--stages]" : " [-o|--output EXR]");
 }
 
@@ -95,9 +97,13 @@ bool parse(int argc, char** argv, bool bench, Options& o) {
             ok = numbers(argc, argv, i, 1, v) && (o.repeat = static_cast<int>(v[0]), true);
         } else if (bench && a == "--stages") {
             o.stages = true;
-        } else if (a == "--points" || a == "--lod" || a == "--stream-budget" || a == "--point-route" ||
-                   a == "--point-size" || a == "--point-pixels" || a == "--edl" || a == "--surface") {
-            std::fprintf(stderr, "%s: point clouds and levels of detail are not drawn by this build\n", a.c_str());
+        } else if (a == "--lod") {
+            std::vector<double> v;
+            ok = numbers(argc, argv, i, 1, v) && v[0] >= 0.0;
+            if (ok) o.lod = v[0];
+        } else if (a == "--points" || a == "--stream-budget" || a == "--point-route" || a == "--point-size" ||
+                   a == "--point-pixels" || a == "--edl" || a == "--surface") {
+            std::fprintf(stderr, "%s: point files and streamed .lrtc clouds are not read by this build\n", a.c_str());
             return false;
         } else {
             std::fprintf(stderr, "unknown option '%s'\n", a.c_str());
@@ -165,6 +171,10 @@ int run(const Options& o, bool bench) {
     }
     if (o.technique != "raster" && o.technique != "reference") {
         std::fprintf(stderr, "--technique wants raster or reference (this build has no ray tracer)\n");
+        return 1;
+    }
+    if (o.lod > 0.0 && o.technique != "raster") {
+        std::fprintf(stderr, "--lod draws splats through the rasteriser\n");
         return 1;
     }
     lrt_context* ctx = nullptr;
@@ -248,6 +258,34 @@ int run(const Options& o, bool bench) {
     settings.max_sh_degree = o.degree;
     settings.time_stages = o.stages ? 1 : 0;
 
+    // Levels of detail: built once, then cut every frame.
+    std::vector<lrt_lod_cloud*> lods;
+    struct LodGuard {
+        std::vector<lrt_lod_cloud*>& l;
+        ~LodGuard() {
+            for (lrt_lod_cloud* c : l) lrt_lod_destroy(c);
+        }
+    } lodGuard{lods};
+    std::vector<lrt_lod_instance> lodInstances;
+    if (o.lod > 0.0) {
+        const auto start = std::chrono::steady_clock::now();
+        for (const Cloud& c : clouds) {
+            lrt_lod_cloud* built = nullptr;
+            if (lrt_lod_build(ctx, &c.splats, nullptr, &built) != LRT_OK) {
+                std::fprintf(stderr, "%s\n", lrt_last_error(ctx));
+                return 1;
+            }
+            lods.push_back(built);
+            lrt_lod_instance li{};
+            li.cloud = built;
+            std::memcpy(li.object_to_world, model, sizeof model);
+            lrt_splat_edit_default(&li.edit);
+            lodInstances.push_back(li);
+        }
+        std::printf("levels of detail built in %.1f ms\n",
+                    std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count());
+    }
+
     const size_t pixels = size_t{width} * height;
     std::vector<float> colour(pixels * 4), depth(pixels);
     const int rounds = std::max(o.repeat, 1);
@@ -257,15 +295,44 @@ int run(const Options& o, bool bench) {
         lrt_render_targets targets{width, height, colourDev.as<float>(), depthDev.as<float>()};
         lrt_frame_stats best{};
         best.total_ms = 1e30;
+        double bestCutMs = 0.0;
+        lrt_cut_stats bestCut{};
+        std::vector<lrt_splat_instance> drawn(std::max(lodInstances.size(), size_t{1}));
         for (int round = 0; round < rounds; ++round) {
+            const lrt_splat_instance* list = instances.data();
+            uint32_t count = static_cast<uint32_t>(instances.size());
+            double cutMs = 0.0;
+            std::vector<lrt_cut_stats> cutStats(lodInstances.size());
+            if (!lodInstances.empty()) {
+                const auto start = std::chrono::steady_clock::now();
+                if (lrt_lod_select(ctx, &proj, lodInstances.data(), static_cast<uint32_t>(lodInstances.size()),
+                                   static_cast<float>(o.lod), drawn.data(), &count, cutStats.data()) != LRT_OK) {
+                    std::fprintf(stderr, "%s\n", lrt_last_error(ctx));
+                    return 1;
+                }
+                cutMs = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - start).count();
+                list = drawn.data();
+            }
             lrt_frame_stats stats{};
-            if (lrt_raster_render(ctx, &proj, instances.data(), static_cast<uint32_t>(instances.size()), &settings,
-                                  &targets, &stats) != LRT_OK) {
+            if (lrt_raster_render(ctx, &proj, list, count, &settings, &targets, &stats) != LRT_OK) {
                 std::fprintf(stderr, "%s\n", lrt_last_error(ctx));
                 return 1;
             }
-            if (stats.total_ms < best.total_ms) best = stats;
-            if (bench) std::printf("frame %d: %.2f ms\n", round, stats.total_ms);
+            if (stats.total_ms + cutMs < best.total_ms + bestCutMs) {
+                best = stats;
+                bestCutMs = cutMs;
+                if (!cutStats.empty()) bestCut = cutStats.front();
+            }
+            if (bench && !lodInstances.empty()) {
+                std::printf("frame %d: %.2f ms (cut %.2f + render %.2f)\n", round, stats.total_ms + cutMs, cutMs,
+                            stats.total_ms);
+            } else if (bench) {
+                std::printf("frame %d: %.2f ms\n", round, stats.total_ms);
+            }
+        }
+        if (!lodInstances.empty()) {
+            std::printf("cut: %u splats + %u merged of %u, %.2f ms; ", bestCut.splats, bestCut.merged,
+                        bestCut.available, bestCutMs);
         }
         std::printf("%u splats (%u visible), %u pairs, %ux%u: best %.2f ms", best.splats, best.visible, best.pairs,
                     width, height, best.total_ms);

@@ -1,6 +1,7 @@
 """The lrt command's argument handling, which needs no GPU: usage and help,
-unknown options and the reference's options this build does not draw
-(points, levels of detail) refused with status 1 before anything runs."""
+unknown options, the reference's options this build does not read (point
+files, streamed .lrtc clouds) and --lod outside the rasteriser refused with
+status 1 before anything runs."""
 import os
 import subprocess
 
@@ -23,13 +24,15 @@ def test_help_lists_the_reference_options(command):
     code, out = _run(command, "--help")
     assert code == 0
     for option in ("--splats", "--technique", "--eye", "--target", "--up", "--rotate-x", "--scale", "--focal",
-                   "--size", "--near", "--degree", "--no-antialias"):
+                   "--size", "--near", "--degree", "--no-antialias", "--lod"):
         assert option in out
     assert ("--repeat" in out and "--stages" in out) if command == "bench" else "--output" in out
 
 
 @pytest.mark.parametrize("args", [[], ["draw"], ["render", "--bogus"], ["render", "--eye", "1", "2"],
-                                  ["bench", "--points", "p.ply"], ["render", "--lod", "2"],
+                                  ["bench", "--points", "p.ply"], ["render", "--lod", "-1"],
+                                  ["render", "--splats", "x.ply", "--lod", "2", "--technique", "reference"],
+                                  ["bench", "--stream-budget", "1000"],
                                   ["render", "--repeat", "3"], ["bench", "-o", "x.exr"]])
 def test_bad_commands_and_options_exit_with_status_one(args):
     code, out = _run(*args)

@@ -74,3 +74,34 @@ def test_the_reference_technique_writes_the_fp64_brute_force(tmp_path):
     _, _, planes, _ = read_exr(out)
     r = planes["R"].view(np.float16).astype(np.float32)[::-1]
     assert np.max(np.abs(r - colour[..., 0])) <= 1e-2   # half rounding plus the raster's tolerance
+
+
+def test_lod_frames_cut_every_frame_and_say_so(tmp_path):
+    """CmdRender.cpp:279-417: built once, cut every frame; --lod with a threshold
+    every cell meets draws the coarsest level alone, and threshold 0.001 the
+    cloud as the Python cut draws it."""
+    code, text = _run("bench", "--splats", SH3, "--size", "160x120", "--repeat", "3", "--lod", "1000000")
+    assert code == 0 and re.search(r"^levels of detail built in [0-9.]+ ms$", text, re.M)
+    assert len(re.findall(r"^frame \d: [0-9.]+ ms \(cut [0-9.]+ \+ render [0-9.]+\)$", text, re.M)) == 3
+    m = re.search(r"^cut: 0 splats \+ (\d+) merged of 64, [0-9.]+ ms; (\d+) splats \(\d+ visible\)", text, re.M)
+    assert m and m.group(1) == m.group(2) and int(m.group(1)) > 0
+    out = str(tmp_path / "lod.exr")
+    code, text = _run("render", "--splats", SH3, "--size", "160x120", "--lod", "0.001", "-o", out)
+    assert code == 0 and "cut: 64 splats + 0 merged of 64" in text and f"wrote {out}" in text
+    rast = raster.TileRasterizer()
+    cloud, info = rast.decode_cloud(raster.read_splats(SH3), 3)
+    lod = rast.lod_build(cloud)
+    lo, hi = np.array(info.bounds_min, np.float64), np.array(info.bounds_max, np.float64)
+    target = (lo + hi) * 0.5
+    eye = target + np.array([0.0, 0.0, max(float(np.max(hi - lo)), 1e-3) * 1.2])
+    cam = raster.Camera.looking_at(tuple(eye), tuple(target))
+    cam.lens = raster.Lens(focal=35.0, near_z=0.01)
+    s = raster.RenderSettings(width=160, height=120)
+    drawn, _ = rast.lod_select(cam, [raster.LodInstance(lod)], 0.001, s)
+    t = raster.RenderTargets(160, 120)
+    rast.render(cam, drawn, s, t)
+    torch.cuda.synchronize()
+    _, _, planes, _ = read_exr(out)
+    want = t.colour.cpu().numpy()[::-1]
+    for k, name in enumerate("RGBA"):
+        np.testing.assert_array_equal(planes[name].view(np.float16), want[..., k].astype(np.float16))
