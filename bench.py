@@ -574,6 +574,14 @@ def main():
                 rast.render(proj, inst, vs, targets)
                 acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
             variants[f"batch{batch}_{'index_order' if order == 0 else 'heavy_first'}"] = round(float(acc_v[6] / 10), 4)
+    # Heavy tiles split across waves: the blend at each segment length (0: never split).
+    for segment in (0, 256, 512, 1024, 2048):
+        vs = raster.RenderSettings(**{**timed.__dict__, "blend_segment": segment})
+        acc_v = np.zeros(8)
+        for _ in range(10):
+            rast.render(proj, inst, vs, targets)
+            acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
+        variants[f"segment{segment}"] = round(float(acc_v[6] / 10), 4)
 
     fwd_bwd = None
     if args.fwd_bwd_steps > 0:

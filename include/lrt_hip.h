@@ -137,6 +137,13 @@ typedef struct {
     int32_t tile_order;           /* 0: tile index order; 1 (default): heaviest tiles launched first */
     int32_t persistent_blend;     /* batches: 1 = a persistent blend taking quadrants from a queue */
     int32_t use_graph;            /* batches: 1 = capture the launch sequence once and replay it */
+    uint32_t blend_segment;       /* a tile with more pairs is split into segments walked by waves of their
+                                     own and composited, a pixel that stops inside a segment walking it
+                                     again (0, the default: never). Frames without points, an under-layer
+                                     or a backward, with tile_order 1 */
+    int32_t keep_walk_state;      /* 1: keep each pixel's last contributor and final transmittance for
+                                     LRT_DEBUG_LAST_COUNT / LRT_DEBUG_FINAL_T (a differentiable frame
+                                     always keeps them; otherwise they cost the blend an instruction a pair) */
 } lrt_render_settings;
 
 /* render::RenderTargets (TileRasterizer.h:257-263): device images, bottom row
@@ -395,8 +402,8 @@ typedef enum {
     LRT_DEBUG_KEYS = 4,          /* u64 a pair, sorted: tile << 32 | bits(view z) */
     LRT_DEBUG_VALUES = 5,        /* u32 a pair, sorted: slot */
     LRT_DEBUG_RANGES = 6,        /* u32[2] a tile: [start, end) */
-    LRT_DEBUG_LAST_COUNT = 7,    /* u32 a pixel: pairs walked to the last contributor */
-    LRT_DEBUG_FINAL_T = 8,       /* float a pixel: final transmittance */
+    LRT_DEBUG_LAST_COUNT = 7,    /* u32 a pixel: pairs walked to the last contributor (kept frames only) */
+    LRT_DEBUG_FINAL_T = 8,       /* float a pixel: final transmittance (kept frames only) */
     LRT_DEBUG_RECTS = 9,         /* u32[2] a slot: x0 | y0 << 16, x1 | y1 << 16 */
     LRT_DEBUG_KERNEL_MS = 10,    /* float[8], last render with time_stages: GPU ms of project,
                                     scan, duplicate, sort histogram, sort passes, ranges,

@@ -70,6 +70,8 @@ lrt::RenderSettings toSettings(const lrt_render_settings& s) {
     out.dilation = s.dilation;
     out.filter3d = s.filter3d != 0;
     out.blendBatch = s.blend_batch == 0 ? 256u : s.blend_batch;
+    out.blendSegment = s.blend_segment;
+    out.keepWalkState = s.keep_walk_state != 0;
     out.heaviestTilesFirst = s.tile_order == 1;
     out.persistentBlend = s.persistent_blend != 0;
     out.useGraph = s.use_graph != 0;
@@ -274,6 +276,8 @@ void lrt_render_settings_default(lrt_render_settings* out) {
     out->dilation = s.dilation;
     out->filter3d = 0;
     out->blend_batch = 256;
+    out->blend_segment = s.blendSegment;
+    out->keep_walk_state = s.keepWalkState ? 1 : 0;
     out->tile_order = 1;
     out->persistent_blend = 0;
     out->use_graph = 0;

@@ -40,6 +40,8 @@
 // staging), and the walk tests that instead of the record's own z.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "splat_math.h"
 
@@ -53,6 +55,7 @@ constexpr uint32_t kPerLane = 2;         // pairs a lane stages per round
 constexpr uint32_t kRound = kLanes * kPerLane;
 constexpr uint32_t kGroup = 4;           // list records walked per step
 constexpr float kExpScale = -0.72134752044448170368f;   // -log2(e) / 2
+constexpr float kLog2Floor = -7.99435343685885793775f;   // log2(1 / 255): alpha's floor
 
 // The finished pixel: linearise rgb / coverage, the background behind, the
 // depth (reference splat_blend.slang:89-107, splat-only).
@@ -128,25 +131,54 @@ __device__ __forceinline__ Staged gather(const float4* __restrict__ proj, uint32
     return s;
 }
 
-// Quadrant `quadrant` of tile `tile` of the frame f, whose pairs are
-// [range.x, range.y) of pairSlots; colour and depth are the frame's images,
+// A quadrant's LDS list: records that reach it, in pair order, padded to a
+// multiple of kGroup with empty records (q far below 0: never pass the test).
+// One per kernel, whichever walks it runs.
+template <uint32_t LIST>
+struct BlendList {
+    float4 geom[LIST + kGroup];     // A, B, C, D of the expanded form
+    float4 conic[LIST + kGroup];    // E, F less the cutoff, o', view z
+    // rgb, then bits of the pair's place in the tile's range + 1; composite:
+    // the largest view z of the tile's pairs up to this one, kept or not
+    float4 colour[LIST + kGroup];
+};
+
+// A pixel's walk so far: premultiplied rgb, transmittance, depth sum, the
+// threshold depth (-1: not reached), the place of the last contributor, and
+// whether it still walks.
+struct WalkState {
+    float r, g, b, t, depthSum, depthAt;
+    uint32_t last;
+    bool active;
+};
+
+// What a walk leaves (MODE): kFinish writes the finished pixel; kPartial
+// writes the walk's state as it stands at the range's end (a segment of a
+// split tile, walked from an empty state) to partialA/partialB[lane];
+// kResume starts from *state (its active lanes) and leaves the state there.
+constexpr uint32_t kFinish = 0, kPartial = 1, kResume = 2;
+
+// Quadrant `quadrant` of tile `tile` of the frame f, walking the pairs
+// [range.x, range.y) of pairSlots (all of the tile's, or a segment of them;
+// places count from tileStart); colour and depth are the frame's images,
 // lastCount/finalT/accum its per-pixel state (each may be null).
-template <uint32_t DEPTH_MODE, uint32_t LIST, bool COMPOSITE>
-__device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t tile, uint32_t quadrant, uint2 range,
-                                              uint32_t slots, const uint32_t* __restrict__ pairSlots,
+template <uint32_t DEPTH_MODE, uint32_t LIST, bool COMPOSITE, uint32_t MODE = kFinish, bool TRACK = true>
+__device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const FrameParams& f, uint32_t tile,
+                                              uint32_t quadrant, uint2 range,
+                                              uint32_t tileStart, uint32_t slots,
+                                              const uint32_t* __restrict__ pairSlots,
                                               const float4* __restrict__ proj,
                                               const float4* __restrict__ underColour,
                                               const float* __restrict__ underDepth, float4* __restrict__ colour,
                                               float* __restrict__ depth, uint32_t* __restrict__ lastCount,
-                                              float* __restrict__ finalT, float4* __restrict__ accum) {
+                                              float* __restrict__ finalT, float4* __restrict__ accum,
+                                              WalkState* state = nullptr, float4* __restrict__ partialA = nullptr,
+                                              float2* __restrict__ partialB = nullptr) {
+    static_assert(!(COMPOSITE && MODE != kFinish), "a composite walk is not split");
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
-    // The list: records that reach this quadrant, in pair order, padded to a
-    // multiple of kGroup with empty records (cutoff -1: never pass the test).
-    __shared__ float4 sGeom[LIST + kGroup];     // A, B, C, D of the expanded form
-    __shared__ float4 sConic[LIST + kGroup];    // E, F less the cutoff, o', view z
-    // rgb, then bits of the pair's place in the tile's range + 1; composite:
-    // the largest view z of the tile's pairs up to this one, kept or not
-    __shared__ float4 sColour[LIST + kGroup];
+    float4* const sGeom = list.geom;
+    float4* const sConic = list.conic;
+    float4* const sColour = list.colour;
 
     const uint32_t lane = threadIdx.x;
     const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
@@ -174,6 +206,16 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
     float r = 0.0f, g = 0.0f, b = 0.0f, t = 1.0f, depthSum = 0.0f, depthAt = -1.0f;
     uint32_t last = 0;
     bool active = inside;
+    if constexpr (MODE == kResume) {
+        r = state->r;
+        g = state->g;
+        b = state->b;
+        t = state->t;
+        depthSum = state->depthSum;
+        depthAt = state->depthAt;
+        last = state->last;
+        active = inside && state->active;
+    }
     // Composite: what ends the walk from behind. A pixel with an under depth
     // above 0 stops at the first pair at or behind it (whether or not that pair
     // reaches the pixel: the reference tests the depth first), a disc that
@@ -224,14 +266,23 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                     const float fc = kExpScale * s.conic.z;
                     const float scaledCut = kExpScale * s.screen.w;
                     const float fd = -(2.0f * fa * ux + fb * vy), fe = -(2.0f * fc * vy + fb * ux);
-                    const float ff = (ux * (fa * ux + fb * vy) + fc * vy * vy) - scaledCut;
+                    const float fbase = ux * (fa * ux + fb * vy) + fc * vy * vy;
                     sGeom[to] = make_float4(fa, fb, fc, fd);
-                    // a disc (colour.w 1, composite only) is marked by o' < 0: its alpha never passes
-                    const bool disc = COMPOSITE && s.col.w > 0.5f;
-                    const float scaledOpacity = disc ? -1.0f : s.conic.w * __builtin_amdgcn_exp2f(scaledCut);
-                    sConic[to] = make_float4(fe, ff, scaledOpacity, s.screen.z);
+                    if constexpr (COMPOSITE) {
+                        // q = the form less the cutoff; alpha = o' 2^q. A disc
+                        // (colour.w 1) is marked by o' < 0: its alpha never passes.
+                        const bool disc = s.col.w > 0.5f;
+                        const float scaledOpacity = disc ? -1.0f : s.conic.w * __builtin_amdgcn_exp2f(scaledCut);
+                        sConic[to] = make_float4(fe, fbase - scaledCut, scaledOpacity, s.screen.z);
+                    } else {
+                        // q' = the form plus log2(opacity): alpha = 2^q' directly, and
+                        // both tests (inside the cutoff, alpha >= 1/255) are q' >= qmin
+                        const float logOpacity = __builtin_amdgcn_logf(s.conic.w);
+                        const float qmin = fmaxS(logOpacity + scaledCut, kLog2Floor);
+                        sConic[to] = make_float4(fe, fbase + logOpacity, qmin, s.screen.z);
+                    }
                     sColour[to] = make_float4(s.col.x, s.col.y, s.col.z,
-                                              COMPOSITE ? zReach : __uint_as_float(at - range.x + 1u));
+                                              COMPOSITE ? zReach : __uint_as_float(at - tileStart + 1u));
                 }
                 fill += static_cast<uint32_t>(__popcll(vote));
             }
@@ -242,7 +293,7 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
         const uint32_t padded = (fill + kGroup - 1u) & ~(kGroup - 1u);
         if (lane < padded - fill) {
             sGeom[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
-            sConic[fill + lane] = make_float4(0.0f, -1e30f, 0.0f, 0.0f);   // q far below 0: never passes
+            sConic[fill + lane] = make_float4(0.0f, -1e30f, 0.0f, 0.0f);   // q far below any threshold: never passes
             sColour[fill + lane] = make_float4(0.0f, 0.0f, 0.0f, COMPOSITE ? -3.4e38f : 0.0f);
         }
         __syncthreads();
@@ -264,9 +315,15 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                 if constexpr (COMPOSITE) active = active & !(underHit & (g2[k].w >= underZ));
                 const float q = __builtin_fmaf(pu, __builtin_fmaf(g0[k].x, pu, __builtin_fmaf(g0[k].y, pv, g0[k].w)),
                                                __builtin_fmaf(pv, __builtin_fmaf(g0[k].z, pv, g1[k].x), g1[k].y));
-                const float alpha = __builtin_fminf(0.99f, g1[k].z * __builtin_amdgcn_exp2f(q));
-                // bitwise, not short-circuit: no branch inside the walk
-                const bool ok = active & (q >= 0.0f) & (alpha >= 1.0f / 255.0f);
+                float alpha;
+                bool ok;   // bitwise, not short-circuit: no branch inside the walk
+                if constexpr (COMPOSITE) {
+                    alpha = __builtin_fminf(0.99f, g1[k].z * __builtin_amdgcn_exp2f(q));
+                    ok = active & (q >= 0.0f) & (alpha >= 1.0f / 255.0f);
+                } else {
+                    alpha = __builtin_fminf(0.99f, __builtin_amdgcn_exp2f(q));
+                    ok = active & (q >= g1[k].z);
+                }
                 const float a = ok ? alpha : 0.0f;
                 const float weight = t * a;
                 r = __builtin_fmaf(weight, g2[k].x, r);
@@ -274,7 +331,7 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
                 b = __builtin_fmaf(weight, g2[k].z, b);
                 depthSum = __builtin_fmaf(weight, g1[k].w, depthSum);
                 t = t - weight;   // T (1 - alpha)
-                if constexpr (!COMPOSITE) last = ok ? __float_as_uint(g2[k].w) : last;
+                if constexpr (!COMPOSITE && TRACK) last = ok ? __float_as_uint(g2[k].w) : last;
                 if constexpr (DEPTH_MODE == 1) {
                     depthAt = (ok && depthAt < 0.0f && 1.0f - t >= f.depthThreshold) ? g1[k].w : depthAt;
                 }
@@ -294,6 +351,14 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
         if (__ballot(active) == 0) break;
         __syncthreads();   // the list is refilled next
     }
+    if constexpr (MODE == kPartial) {
+        partialA[lane] = make_float4(r, g, b, depthSum);
+        partialB[lane] = make_float2(t, __uint_as_float(last));
+        return;
+    } else if constexpr (MODE == kResume) {
+        *state = WalkState{r, g, b, t, depthSum, depthAt, last, active};
+        return;
+    }
     if (!inside) return;
     if constexpr (COMPOSITE) {
         finishComposite(f, at0, r, g, b, t, depthSum, depthAt, opaqueHit, make_float4(oR, oG, oB, 1.0f), oZ,
@@ -303,7 +368,7 @@ __device__ __forceinline__ void blendQuadrant(const FrameParams& f, uint32_t til
     }
 }
 
-template <uint32_t DEPTH_MODE, uint32_t LIST, bool COMPOSITE>
+template <uint32_t DEPTH_MODE, uint32_t LIST, bool COMPOSITE, bool TRACK>
 __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t slots, const uint32_t* __restrict__ order,
                                                       const uint2* __restrict__ ranges,
                                                       const uint32_t* __restrict__ pairSlots,
@@ -319,8 +384,11 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     if (place >= tiles) return;
     const uint32_t tile = order != nullptr ? order[place] : place;
     if (tile >= tiles) return;   // an order entry that is not a tile: draw nothing
-    blendQuadrant<DEPTH_MODE, LIST, COMPOSITE>(f, tile, quadrant, ranges[tile], slots, pairSlots, proj, underColour,
-                                               underDepth, colour, depth, lastCount, finalT, accum);
+    __shared__ BlendList<LIST> list;
+    const uint2 rg = ranges[tile];
+    blendQuadrant<DEPTH_MODE, LIST, COMPOSITE, kFinish, TRACK>(list, f, tile, quadrant, rg, rg.x, slots, pairSlots, proj,
+                                                               underColour, underDepth, colour, depth, lastCount,
+                                                               finalT, accum);
 }
 
 // The view holding batch tile g: the last view whose tiles start at or before it.
@@ -338,7 +406,7 @@ __device__ __forceinline__ uint32_t viewOfTile(const ViewInfo* __restrict__ view
 // b + 24 of one XCD; persistent, each block takes tasks from a shared counter
 // until none are left (task t: tile t / 4, quadrant t % 4), so a block that
 // drew a light quadrant takes the next instead of leaving its slot idle.
-template <uint32_t DEPTH_MODE, uint32_t LIST, bool PERSISTENT>
+template <uint32_t DEPTH_MODE, uint32_t LIST, bool PERSISTENT, bool TRACK>
 __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __restrict__ params,
                                                            const ViewInfo* __restrict__ views, uint32_t viewCount,
                                                            uint32_t tiles, uint32_t tasks, uint32_t slots,
@@ -347,6 +415,7 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
                                                            const float4* __restrict__ proj,
                                                            uint32_t* __restrict__ lastCount,
                                                            float* __restrict__ finalT, uint32_t* __restrict__ queue) {
+    __shared__ BlendList<LIST> list;
     for (uint32_t round = 0;; ++round) {
         uint32_t task, g, quadrant;
         if constexpr (PERSISTENT) {
@@ -364,10 +433,13 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
         }
         if (g < tiles) {
             const ViewInfo& v = views[viewOfTile(views, viewCount, g)];
-            blendQuadrant<DEPTH_MODE, LIST, false>(params[v.params], g - v.tileBase, quadrant, ranges[g], slots,
+            const uint2 rg = ranges[g];
+            blendQuadrant<DEPTH_MODE, LIST, false, kFinish, TRACK>(list, params[v.params], g - v.tileBase, quadrant, rg,
+                                                                   rg.x, slots,
                                                    pairSlots, proj, nullptr, nullptr,
                                                    reinterpret_cast<float4*>(v.colour), v.depth,
-                                            lastCount + v.pixelBase, finalT + v.pixelBase, nullptr);
+                                            lastCount != nullptr ? lastCount + v.pixelBase : nullptr,
+                                            finalT != nullptr ? finalT + v.pixelBase : nullptr, nullptr);
         }
         if constexpr (!PERSISTENT) return;
     }
@@ -380,6 +452,7 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
 // descending, wave ascending) gives each wave its first place in every bucket,
 // and each wave walks its segment again handing out places.
 constexpr uint32_t kBuckets = 33;
+constexpr uint32_t kNoSlot = 0xFFFFFFFFu;   // a task that finishes its tile's pixels itself
 constexpr uint32_t kOrderWaves = 16;
 constexpr uint32_t kOrderCached = 32768;   // tiles whose bucket the first walk leaves in LDS for the second
 
@@ -387,8 +460,43 @@ __device__ __forceinline__ uint32_t bucketOf(uint2 rg) {
     return 32u - static_cast<uint32_t>(__clz(rg.y - rg.x));   // 0 for an empty tile
 }
 
+// The segments of a tile of `pairs` pairs: one, unless it holds more than
+// segmentPairs (0: never split).
+__device__ __forceinline__ uint32_t segmentsOf(uint32_t pairs, uint32_t segmentPairs) {
+    return segmentPairs == 0 || pairs <= segmentPairs ? 1u : (pairs + segmentPairs - 1u) / segmentPairs;
+}
+
+// Exclusive sums over the workgroup of three counts, and their totals.
+__device__ __forceinline__ void blockScan3(uint32_t v[3], uint32_t total[3], uint32_t (*sWave)[kOrderWaves]) {
+    const uint32_t lane = threadIdx.x & 63u, wave = threadIdx.x >> 6;
+    uint32_t inc[3] = {v[0], v[1], v[2]};
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+#pragma unroll
+        for (int k = 0; k < 3; ++k) {
+            const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(inc[k]), d, 64));
+            inc[k] += lane >= d ? o : 0u;
+        }
+    }
+    if (lane == 63)
+        for (int k = 0; k < 3; ++k) sWave[k][wave] = inc[k];
+    __syncthreads();
+    for (int k = 0; k < 3; ++k) {
+        uint32_t before = 0, all = 0;
+        for (uint32_t w = 0; w < kOrderWaves; ++w) {
+            before += w < wave ? sWave[k][w] : 0u;
+            all += sWave[k][w];
+        }
+        total[k] = all;
+        v[k] = before + inc[k] - v[k];
+    }
+}
+
 __global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2* __restrict__ ranges, uint32_t tiles,
-                                                                    uint32_t* __restrict__ order) {
+                                                                    uint32_t* __restrict__ order,
+                                                                    uint32_t segmentPairs, uint4* __restrict__ tasks,
+                                                                    uint4* __restrict__ combine,
+                                                                    uint32_t* __restrict__ planCounts) {
     __shared__ uint32_t sCount[kOrderWaves][kBuckets];
     __shared__ uint32_t sStart[kOrderWaves][kBuckets];
     __shared__ uint8_t sBucket[kOrderCached];
@@ -434,6 +542,143 @@ __global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2*
             todo &= ~vote;
         }
     }
+    if (tasks == nullptr) return;
+    // The split plan: tiles in that order, each cut into segmentsOf() segments
+    // of near-equal length. tasks[k] = (tile, segment begin, end, partial slot
+    // or kNoSlot); combine[m] = (tile, first slot, segments) for each split
+    // tile. Each thread takes a run of places; sums over the runs place them.
+    __shared__ uint32_t sWave[3][kOrderWaves];
+    __syncthreads();   // the order is written
+    const uint32_t chunk = (tiles + blockDim.x - 1u) / blockDim.x;
+    const uint32_t p0 = tid * chunk < tiles ? tid * chunk : tiles;
+    const uint32_t p1 = p0 + chunk < tiles ? p0 + chunk : tiles;
+    uint32_t v[3] = {0u, 0u, 0u};   // tasks, partial slots, split tiles
+    for (uint32_t p = p0; p < p1; ++p) {
+        const uint2 rg = ranges[order[p]];
+        const uint32_t n = segmentsOf(rg.y - rg.x, segmentPairs);
+        v[0] += n;
+        v[1] += n > 1 ? n : 0u;
+        v[2] += n > 1 ? 1u : 0u;
+    }
+    uint32_t total[3];
+    blockScan3(v, total, sWave);
+    for (uint32_t p = p0; p < p1; ++p) {
+        const uint32_t tile = order[p];
+        const uint2 rg = ranges[tile];
+        const uint32_t len = rg.y - rg.x;
+        const uint32_t n = segmentsOf(len, segmentPairs);
+        const uint32_t segLen = (len + n - 1u) / n;
+        for (uint32_t k = 0; k < n; ++k) {
+            const uint32_t begin = rg.x + k * segLen < rg.y ? rg.x + k * segLen : rg.y;
+            const uint32_t end = begin + segLen < rg.y ? begin + segLen : rg.y;
+            tasks[v[0] + k] = make_uint4(tile, begin, end, n > 1 ? v[1] + k : kNoSlot);
+        }
+        if (n > 1) {
+            combine[v[2]] = make_uint4(tile, v[1], n, 0u);
+            v[1] += n;
+            v[2] += 1;
+        }
+        v[0] += n;
+    }
+    if (tid == 0) {
+        planCounts[0] = total[0];
+        planCounts[1] = total[2];
+    }
+}
+
+// The tasks of a split plan: task k's four quadrants are blocks b, b + 8,
+// b + 16, b + 24 of one XCD. A task of an unsplit tile finishes its pixels; a
+// segment leaves its partial walk in its slot.
+template <uint32_t DEPTH_MODE, uint32_t LIST>
+__global__ __launch_bounds__(kLanes) void blendTasksKernel(FrameParams f, uint32_t slots, const uint4* __restrict__ tasks,
+                                                           const uint32_t* __restrict__ planCounts,
+                                                           const uint2* __restrict__ ranges,
+                                                           const uint32_t* __restrict__ pairSlots,
+                                                           const float4* __restrict__ proj,
+                                                           float4* __restrict__ colour, float* __restrict__ depth,
+                                                           uint32_t* __restrict__ lastCount,
+                                                           float* __restrict__ finalT, float4* __restrict__ partialA,
+                                                           float2* __restrict__ partialB) {
+    const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
+    const uint32_t task = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
+    if (task >= planCounts[0]) return;
+    __shared__ BlendList<LIST> list;
+    const uint4 tk = tasks[task];
+    const uint32_t tileStart = ranges[tk.x].x;
+    if (tk.w == kNoSlot) {
+        blendQuadrant<DEPTH_MODE, LIST, false, kFinish>(list, f, tk.x, quadrant, make_uint2(tk.y, tk.z), tileStart, slots,
+                                                        pairSlots, proj, nullptr, nullptr, colour, depth, lastCount,
+                                                        finalT, nullptr);
+    } else {
+        const uint64_t at = (static_cast<uint64_t>(tk.w) * 4u + quadrant) * kLanes;
+        blendQuadrant<DEPTH_MODE, LIST, false, kPartial>(list, f, tk.x, quadrant, make_uint2(tk.y, tk.z), tileStart, slots,
+                                                         pairSlots, proj, nullptr, nullptr, colour, depth, lastCount,
+                                                         finalT, nullptr, nullptr, partialA + at, partialB + at);
+    }
+}
+
+// A split tile's pixels, its segments' partial walks composited front to back:
+// rgb += T rgb_s, depth sum += T depth_s, T *= T_s. A pixel whose walk would
+// stop inside a segment (T below 1e-4, or the threshold depth crossed) walks
+// that segment again from its true state, so it stops where the one walk
+// would; the others take the segment whole.
+template <uint32_t DEPTH_MODE, uint32_t LIST>
+__global__ __launch_bounds__(kLanes) void combineKernel(FrameParams f, uint32_t slots, const uint4* __restrict__ combine,
+                                                        const uint32_t* __restrict__ planCounts,
+                                                        const uint2* __restrict__ ranges,
+                                                        const uint32_t* __restrict__ pairSlots,
+                                                        const float4* __restrict__ proj, float4* __restrict__ colour,
+                                                        float* __restrict__ depth, uint32_t* __restrict__ lastCount,
+                                                        float* __restrict__ finalT,
+                                                        const float4* __restrict__ partialA,
+                                                        const float2* __restrict__ partialB) {
+    const uint32_t quadrant = blockIdx.x & 3u;
+    const uint32_t c = blockIdx.x >> 2;
+    if (c >= planCounts[1]) return;
+    const uint4 ct = combine[c];
+    const uint32_t tile = ct.x, n = ct.z, lane = threadIdx.x;
+    const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
+    const uint32_t y0 = (tile / f.tilesX) * kTile + (quadrant >> 1) * 8u;
+    if (x0 >= f.width || y0 >= f.height) return;
+    const uint32_t px = x0 + (lane & 7u), py = y0 + (lane >> 3);
+    const bool inside = px < f.width && py < f.height && px < x0 + 8u && py < y0 + 8u;
+    const uint2 rg = ranges[tile];
+    const uint32_t len = rg.y - rg.x, segLen = (len + n - 1u) / n;
+    __shared__ BlendList<LIST> list;
+    WalkState st{0.0f, 0.0f, 0.0f, 1.0f, 0.0f, -1.0f, 0u, inside};
+    for (uint32_t k = 0; k < n; ++k) {
+        const uint64_t at = (static_cast<uint64_t>(ct.y + k) * 4u + quadrant) * kLanes + lane;
+        const float4 pa = partialA[at];
+        const float2 pb = partialB[at];
+        bool again = false;
+        if (st.active) {
+            const float tEnd = st.t * pb.x;
+            again = tEnd < 1e-4f;
+            if constexpr (DEPTH_MODE == 1) again = again || (st.depthAt < 0.0f && 1.0f - tEnd >= f.depthThreshold);
+            if (!again) {
+                st.r = __builtin_fmaf(st.t, pa.x, st.r);
+                st.g = __builtin_fmaf(st.t, pa.y, st.g);
+                st.b = __builtin_fmaf(st.t, pa.z, st.b);
+                st.depthSum = __builtin_fmaf(st.t, pa.w, st.depthSum);
+                st.t = tEnd;
+                st.last = __float_as_uint(pb.y) != 0u ? __float_as_uint(pb.y) : st.last;
+            }
+        }
+        if (__ballot(again) != 0) {
+            const uint32_t begin = rg.x + k * segLen < rg.y ? rg.x + k * segLen : rg.y;
+            const uint32_t end = begin + segLen < rg.y ? begin + segLen : rg.y;
+            WalkState w = st;
+            w.active = again;
+            blendQuadrant<DEPTH_MODE, LIST, false, kResume>(list, f, tile, quadrant, make_uint2(begin, end), rg.x, slots,
+                                                            pairSlots, proj, nullptr, nullptr, colour, depth,
+                                                            lastCount, finalT, nullptr, &w);
+            if (again) st = w;
+            __syncthreads();   // the walk's list is refilled by the next one
+        }
+    }
+    if (!inside) return;
+    finishPixel(f, px, py, st.r, st.g, st.b, st.t, st.depthSum, st.depthAt, st.last, colour, depth, lastCount, finalT,
+                nullptr);
 }
 
 template <uint32_t DEPTH_MODE, bool COMPOSITE>
@@ -442,16 +687,23 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
                  float4* colour, float* depth, uint32_t* lastCount, float* finalT, float4* accum,
                  hipStream_t stream) {
     // Four blocks a tile, the grid rounded up to whole groups of eight tiles.
+    // The last contributor is tracked only where it is kept.
     const uint32_t tiles = f.tilesX * f.tilesY;
     const uint32_t blocks = (tiles + 7u) / 8u * 32u;
+    const auto go = [&](auto listLen, auto track) {
+        blendKernel<DEPTH_MODE, decltype(listLen)::value, COMPOSITE, decltype(track)::value>
+            <<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour,
+                                            depth, lastCount, finalT, accum);
+    };
+    using L128 = std::integral_constant<uint32_t, 128>;
+    using L256 = std::integral_constant<uint32_t, 256>;
+    using Keep = std::true_type;
+    using Drop = std::false_type;
+    const bool track = !COMPOSITE && lastCount != nullptr;
     if (list <= 128) {
-        blendKernel<DEPTH_MODE, 128, COMPOSITE><<<blocks, kLanes, 0, stream>>>(
-            f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour, depth, lastCount, finalT,
-            accum);
+        if (track) go(L128{}, Keep{}); else go(L128{}, Drop{});
     } else {
-        blendKernel<DEPTH_MODE, 256, COMPOSITE><<<blocks, kLanes, 0, stream>>>(
-            f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour, depth, lastCount, finalT,
-            accum);
+        if (track) go(L256{}, Keep{}); else go(L256{}, Drop{});
     }
 }
 
@@ -459,17 +711,21 @@ template <uint32_t DEPTH_MODE, uint32_t LIST>
 void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* params, const ViewInfo* views,
                       uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges, const uint32_t* pairSlots,
                       const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue, hipStream_t stream) {
-    if (persistent) {
-        const uint32_t tasks = tiles * 4u;
-        const uint32_t grid = tasks < gridCap ? tasks : gridCap;
-        fill32(queue, 0u, 1, stream);
-        blendViewsKernel<DEPTH_MODE, LIST, true><<<grid, kLanes, 0, stream>>>(
-            params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
-    } else {
-        const uint32_t tasks = (tiles + 7u) / 8u * 32u;
-        blendViewsKernel<DEPTH_MODE, LIST, false><<<tasks, kLanes, 0, stream>>>(
-            params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
-    }
+    const auto go = [&](auto track) {
+        constexpr bool T = decltype(track)::value;
+        if (persistent) {
+            const uint32_t tasks = tiles * 4u;
+            const uint32_t grid = tasks < gridCap ? tasks : gridCap;
+            fill32(queue, 0u, 1, stream);
+            blendViewsKernel<DEPTH_MODE, LIST, true, T><<<grid, kLanes, 0, stream>>>(
+                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
+        } else {
+            const uint32_t tasks = (tiles + 7u) / 8u * 32u;
+            blendViewsKernel<DEPTH_MODE, LIST, false, T><<<tasks, kLanes, 0, stream>>>(
+                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
+        }
+    };
+    if (lastCount != nullptr) go(std::true_type{}); else go(std::false_type{});
 }
 
 }   // namespace
@@ -500,7 +756,38 @@ void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gri
 
 void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t stream) {
     if (tiles == 0) return;
-    tileOrderKernel<<<1, kOrderWaves * 64, 0, stream>>>(ranges, tiles, order);
+    tileOrderKernel<<<1, kOrderWaves * 64, 0, stream>>>(ranges, tiles, order, 0u, nullptr, nullptr, nullptr);
+}
+
+void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint32_t slots, const uint2* ranges,
+                const uint32_t* pairSlots, const float4* proj, const BlendPlan& plan, float4* colour, float* depth,
+                uint32_t* lastCount, float* finalT, hipStream_t stream) {
+    const uint32_t tiles = f.tilesX * f.tilesY;
+    if (tiles == 0) return;
+    tileOrderKernel<<<1, kOrderWaves * 64, 0, stream>>>(ranges, tiles, plan.order, segmentPairs, plan.tasks,
+                                                        plan.combine, plan.counts);
+    const uint32_t taskBlocks = (plan.taskCapacity + 7u) / 8u * 32u;
+    const uint32_t combineBlocks = plan.combineCapacity * 4u;
+    const auto launch = [&](auto depthMode, auto listLen) {
+        constexpr uint32_t D = decltype(depthMode)::value, L = decltype(listLen)::value;
+        blendTasksKernel<D, L><<<taskBlocks, kLanes, 0, stream>>>(f, slots, plan.tasks, plan.counts, ranges, pairSlots,
+                                                                  proj, colour, depth, lastCount, finalT,
+                                                                  plan.partialA, plan.partialB);
+        if (combineBlocks > 0) {
+            combineKernel<D, L><<<combineBlocks, kLanes, 0, stream>>>(f, slots, plan.combine, plan.counts, ranges,
+                                                                      pairSlots, proj, colour, depth, lastCount,
+                                                                      finalT, plan.partialA, plan.partialB);
+        }
+    };
+    using D0 = std::integral_constant<uint32_t, 0>;
+    using D1 = std::integral_constant<uint32_t, 1>;
+    using L128 = std::integral_constant<uint32_t, 128>;
+    using L256 = std::integral_constant<uint32_t, 256>;
+    if (f.depthMode == 0) {
+        if (list <= 128) launch(D0{}, L128{}); else launch(D0{}, L256{});
+    } else {
+        if (list <= 128) launch(D1{}, L128{}); else launch(D1{}, L256{});
+    }
 }
 
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,

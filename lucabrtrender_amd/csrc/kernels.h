@@ -134,6 +134,28 @@ void blendComposite(const FrameParams& f, uint32_t list, uint32_t slots, const u
                     const uint32_t* pairSlots, const float4* proj, const float4* underColour,
                     const float* underDepth, float4* colour, float* depth, hipStream_t stream);
 
+// A frame whose heavy tiles are split: the tiles heaviest first, each tile of
+// more than segmentPairs pairs cut into near-equal segments whose quadrants
+// are walked by their own waves (partial walks in partialA/partialB, 256
+// entries a slot), then composited front to back by one wave a quadrant, a
+// pixel that would stop inside a segment walking that segment again, so it
+// stops where the one walk would. order holds the tiles; tasks/combine/counts
+// the plan, sized for taskCapacity >= tiles + pairs / segmentPairs tasks and
+// combineCapacity >= the split tiles (slots: pairs / segmentPairs + split
+// tiles). Not for composite or training frames.
+struct BlendPlan {
+    uint32_t* order;
+    uint4* tasks;
+    uint4* combine;
+    uint32_t* counts;   // 2: tasks, split tiles
+    float4* partialA;
+    float2* partialB;
+    uint32_t taskCapacity, combineCapacity;
+};
+void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint32_t slots, const uint2* ranges,
+                const uint32_t* pairSlots, const float4* proj, const BlendPlan& plan, float4* colour, float* depth,
+                uint32_t* lastCount, float* finalT, hipStream_t stream);
+
 // A batch of views in one launch (device FrameParams and ViewInfo arrays;
 // views sorted by tileBase, `tiles` in all): as a grid of quadrant tasks, or
 // persistent (at most gridCap workgroups taking tasks from *queue, which is

@@ -428,16 +428,27 @@ __global__ __launch_bounds__(kBlock) void gatherKernel(uint32_t merged, uint32_t
     uint32_t row = threadIdx.x / shWords, col = threadIdx.x - row * shWords;
     const uint32_t rowStep = kBlock / shWords, colStep = kBlock - rowStep * shWords;
     uint32_t* out = sh + static_cast<uint64_t>(base) * shWords;
-    for (uint32_t w = threadIdx.x; w < words; w += kBlock) {
-        const uint32_t k = sElement[row];
-        const uint32_t* src = k < merged ? mergedSh + static_cast<uint64_t>(k) * shWords
-                                         : splatSh + static_cast<uint64_t>(k - merged) * shWords;
-        out[w] = src[col];
-        row += rowStep;
-        col += colStep;
-        if (col >= shWords) {
-            col -= shWords;
-            row += 1;
+    constexpr uint32_t kBatch = 8;   // loads in flight before the first store
+    for (uint32_t w0 = threadIdx.x; w0 < words; w0 += kBlock * kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            if (w0 + b * kBlock < words) {
+                const uint32_t k = sElement[row];
+                const uint32_t* src = k < merged ? mergedSh + static_cast<uint64_t>(k) * shWords
+                                                 : splatSh + static_cast<uint64_t>(k - merged) * shWords;
+                v[b] = src[col];
+            }
+            row += rowStep;
+            col += colStep;
+            if (col >= shWords) {
+                col -= shWords;
+                row += 1;
+            }
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < kBatch; ++b) {
+            if (w0 + b * kBlock < words) out[w0 + b * kBlock] = v[b];
         }
     }
 }

@@ -32,16 +32,26 @@ __device__ __forceinline__ void stageDrawnRows(const uint32_t* __restrict__ sh, 
     if (words == 0) return;
     const uint32_t total = rows * words;
     // Word w = tid + k * kBlock is word w % words of row w / words; both are
-    // stepped, not divided, per k.
+    // stepped, not divided, per k. kBatch loads are issued before any is
+    // stored, so their latencies overlap instead of adding up.
+    constexpr uint32_t kBatch = 8;
     uint32_t row = threadIdx.x / words, col = threadIdx.x - row * words;
     const uint32_t rowStep = kBlock / words, colStep = kBlock - rowStep * words;
-    for (uint32_t w = threadIdx.x; w < total; w += kBlock) {
-        sSh[w] = sh[static_cast<uint64_t>(first + sRowOf[row]) * words + col];
-        row += rowStep;
-        col += colStep;
-        if (col >= words) {
-            col -= words;
-            row += 1;
+    for (uint32_t w0 = threadIdx.x; w0 < total; w0 += kBlock * kBatch) {
+        uint32_t v[kBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; ++k) {
+            if (w0 + k * kBlock < total) v[k] = sh[static_cast<uint64_t>(first + sRowOf[row]) * words + col];
+            row += rowStep;
+            col += colStep;
+            if (col >= words) {
+                col -= words;
+                row += 1;
+            }
+        }
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; ++k) {
+            if (w0 + k * kBlock < total) sSh[w0 + k * kBlock] = v[k];
         }
     }
 }
@@ -166,7 +176,18 @@ __global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* 
         const uint32_t inBlock = count - first < kBlock ? count - first : kBlock;
         const uint32_t total = inBlock * words;
         const uint32_t* src = sh + static_cast<uint64_t>(first) * words;
-        for (uint32_t w = tid; w < total; w += kBlock) sSh[w] = src[w];
+        constexpr uint32_t kBatch = 8;   // loads in flight before the first store
+        for (uint32_t w0 = tid; w0 < total; w0 += kBlock * kBatch) {
+            uint32_t v[kBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                if (w0 + b * kBlock < total) v[b] = src[w0 + b * kBlock];
+            }
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                if (w0 + b * kBlock < total) sSh[w0 + b * kBlock] = v[b];
+            }
+        }
     }
     const bool live = i < count;
     const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);

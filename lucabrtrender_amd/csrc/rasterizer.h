@@ -107,6 +107,8 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
     float dilation = 0.3F;
     bool filter3d = false;
     uint32_t blendBatch = 256;          // records each quadrant's LDS list holds: 128 or 256
+    uint32_t blendSegment = 0;          // pairs a quadrant's wave walks before its tile is split (0: never)
+    bool keepWalkState = false;         // keep each pixel's last contributor and final transmittance (debug)
     bool heaviestTilesFirst = true;     // launch the tiles with the most pairs first
     bool persistentBlend = false;       // batches: a persistent blend taking quadrants from a queue
     bool useGraph = false;              // batches: replay the launch sequence as a captured hipGraph
@@ -272,6 +274,8 @@ private:
     uint32_t tileCapacity_ = 0;
     uint64_t pixelCapacity_ = 0;
     DeviceBuffer ranges_, tileOrder_, lastCount_, finalT_, accum_;
+    // Split blends: the plan's tasks and split tiles, and the segments' partial walks.
+    DeviceBuffer planTasks_, planCombine_, partials_;
     // Counters: [0] pairs, [1] visible, [2] overflow, [3] unused, [4] sort error, [5] scan error.
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
@@ -283,7 +287,7 @@ private:
     // What the last render left.
     uint32_t lastSplats_ = 0, lastVisible_ = 0, lastPairs_ = 0, lastTiles_ = 0;
     bool lastOrdered_ = false;
-    bool lastComposite_ = false;   // the composite blend keeps no per-pixel walk state
+    bool lastWalkDropped_ = false;   // the last frame kept no per-pixel walk state
     uint64_t lastPixels_ = 0;
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
     // Batches: the frame constants and view records (pinned host copies that

@@ -368,17 +368,43 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     mark(7);
 
     const uint32_t* order = nullptr;
-    if (settings.heaviestTilesFirst) {
+    // Heavy tiles split across waves: frames that keep no composite or
+    // training state, launched heaviest first.
+    const bool split = !training && !composite && settings.heaviestTilesFirst && settings.blendSegment > 0;
+    // Each pixel's last contributor and final transmittance: what a backward
+    // needs, kept otherwise only on request.
+    const bool keepWalk = !composite && (training || settings.keepWalkState);
+    uint32_t* lastCountOut = keepWalk ? lastCount_.as<uint32_t>() : nullptr;
+    float* finalTOut = keepWalk ? finalT_.as<float>() : nullptr;
+    if (settings.heaviestTilesFirst && !split) {
         kern::tileOrder(ranges_.as<uint2>(), tiles, tileOrder_.as<uint32_t>(), stream_);
         order = tileOrder_.as<uint32_t>();
     }
-    if (composite) {
+    if (split) {
+        const uint32_t seg = settings.blendSegment;
+        kern::BlendPlan plan{};
+        plan.taskCapacity = tiles + pairs / seg + 1u;
+        plan.combineCapacity = std::min(tiles, pairs / (seg + 1u) + 1u);
+        const uint64_t slotCapacity = uint64_t{pairs / seg} + plan.combineCapacity + 1u;
+        const uint64_t partialBytes = slotCapacity * 256u * 16u;
+        if (!(s = planTasks_.reserve(uint64_t{plan.taskCapacity} * 16, "blend.tasks")).ok()) return s;
+        if (!(s = planCombine_.reserve(uint64_t{plan.combineCapacity} * 16, "blend.combine")).ok()) return s;
+        if (!(s = partials_.reserve(partialBytes + slotCapacity * 256u * 8u, "blend.partials")).ok()) return s;
+        plan.order = tileOrder_.as<uint32_t>();
+        plan.tasks = planTasks_.as<uint4>();
+        plan.combine = planCombine_.as<uint4>();
+        plan.counts = counters + 6;
+        plan.partialA = partials_.as<float4>();
+        plan.partialB = reinterpret_cast<float2*>(partials_.as<uint8_t>() + partialBytes);
+        kern::blendSplit(first, settings.blendBatch, seg, splatCount, ranges_.as<uint2>(), pairSlots,
+                         proj_.as<float4>(), plan, targets.colour, targets.depth, lastCountOut, finalTOut, stream_);
+    } else if (composite) {
         kern::blendComposite(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots,
                              proj_.as<float4>(), under != nullptr ? under->colour : nullptr,
                              under != nullptr ? under->depth : nullptr, targets.colour, targets.depth, stream_);
     } else {
         kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots,
-                    proj_.as<float4>(), targets.colour, targets.depth, lastCount_.as<uint32_t>(), finalT_.as<float>(),
+                    proj_.as<float4>(), targets.colour, targets.depth, lastCountOut, finalTOut,
                     training ? accum_.as<float4>() : nullptr, stream_);
     }
     mark(8);
@@ -395,7 +421,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     lastPairs_ = pairs;
     lastTiles_ = tiles;
     lastOrdered_ = settings.heaviestTilesFirst;
-    lastComposite_ = composite;
+    lastWalkDropped_ = !keepWalk;
     lastPixels_ = uint64_t{settings.width} * settings.height;
     lastSorted_ = sorted;
     stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
@@ -467,8 +493,9 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint
     mark(7);
     // A persistent blend keeps about 24 one-wave workgroups on each CU.
     kern::blendViews(settings.depthMode, settings.blendBatch, settings.persistentBlend, computeUnits_ * 24u, params, views, viewCount, tiles, slots, ranges_.as<uint2>(),
-                     values_[sorted].as<uint32_t>(), proj_.as<float4>(), lastCount_.as<uint32_t>(),
-                     finalT_.as<float>(), counters + 6, stream);
+                     values_[sorted].as<uint32_t>(), proj_.as<float4>(),
+                     settings.keepWalkState ? lastCount_.as<uint32_t>() : nullptr,
+                     settings.keepWalkState ? finalT_.as<float>() : nullptr, counters + 6, stream);
     mark(8);
     (void)hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream);
     lastSorted_ = sorted;
@@ -665,7 +692,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
     lastPairs_ = hostCounters_[0];
     lastTiles_ = static_cast<uint32_t>(tiles);
     lastOrdered_ = false;
-    lastComposite_ = false;
+    lastWalkDropped_ = !settings.keepWalkState;
     lastPixels_ = pixels;
     if (statsOut != nullptr) {
         FrameStats stats;
@@ -957,8 +984,8 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     case 4: src = keys_[lastSorted_].get(); size = size_t{lastPairs_} * 8; break;
     case 5: src = values_[lastSorted_].get(); size = size_t{lastPairs_} * 4; break;
     case 6: src = ranges_.get(); size = size_t{lastTiles_} * 8; break;
-    case 7: src = lastCount_.get(); size = lastComposite_ ? 0 : size_t{lastPixels_} * 4; break;
-    case 8: src = finalT_.get(); size = lastComposite_ ? 0 : size_t{lastPixels_} * 4; break;
+    case 7: src = lastCount_.get(); size = lastWalkDropped_ ? 0 : size_t{lastPixels_} * 4; break;
+    case 8: src = finalT_.get(); size = lastWalkDropped_ ? 0 : size_t{lastPixels_} * 4; break;
     case 9: src = rects_.get(); size = size_t{lastSplats_} * 8; break;
     case 10: size = sizeof kernelMs_; break;
     case 11: src = tileOrder_.get(); size = lastOrdered_ ? size_t{lastTiles_} * 4 : 0; break;

@@ -124,6 +124,8 @@ class RenderSettings:
     tile_order: int = 1             # 0 tile index order, 1 heaviest tiles first (the default: shorter tail)
     persistent_blend: bool = False  # batches: a persistent blend taking quadrants from a queue
     use_graph: bool = False         # batches: capture the launch sequence once and replay it
+    blend_segment: int = 0          # a tile with more pairs is split across waves and composited (0: never)
+    keep_walk_state: bool = False   # keep each pixel's last contributor and final transmittance (debug)
 
     def to_c(self) -> _abi.RenderSettingsC:
         s = _abi.RenderSettingsC()
@@ -140,6 +142,8 @@ class RenderSettings:
         s.tile_order = self.tile_order
         s.persistent_blend = 1 if self.persistent_blend else 0
         s.use_graph = 1 if self.use_graph else 0
+        s.blend_segment = self.blend_segment
+        s.keep_walk_state = 1 if self.keep_walk_state else 0
         return s
 
 

@@ -39,8 +39,9 @@ def run_gpu(cam, settings, instances, rast=None):
     dev = [sc.to_device() for sc, _, _ in instances]
     inst = [raster.SplatInstance(d, tuple(np.asarray(o2w, dtype=np.float64).reshape(-1)), ed)
             for d, (_, o2w, ed) in zip(dev, instances)]
+    import dataclasses
     targets = raster.RenderTargets(settings.width, settings.height)
-    stats = rast.render(cam, inst, settings, targets)
+    stats = rast.render(cam, inst, dataclasses.replace(settings, keep_walk_state=True), targets)
     torch.cuda.synchronize()
     out = dict(stats=stats,
                counts=rast.debug(_abi.DEBUG_COUNTS),
@@ -54,6 +55,13 @@ def run_gpu(cam, settings, instances, rast=None):
                final_t=rast.debug(_abi.DEBUG_FINAL_T).reshape(settings.height, settings.width),
                colour=targets.colour.cpu().numpy(),
                depth=targets.depth.cpu().numpy())
+    # The same frame without the per-pixel walk state: the same images, bit for
+    # bit (the walk only stops tracking its last contributor).
+    fast = raster.RenderTargets(settings.width, settings.height)
+    rast.render(cam, inst, dataclasses.replace(settings, keep_walk_state=False), fast)
+    torch.cuda.synchronize()
+    assert np.array_equal(fast.colour.cpu().numpy().view(np.uint32), out["colour"].view(np.uint32))
+    assert np.array_equal(fast.depth.cpu().numpy().view(np.uint32), out["depth"].view(np.uint32))
     return out
 
 

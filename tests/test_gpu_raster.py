@@ -260,3 +260,27 @@ def test_tile_order_is_a_stable_heaviest_first_permutation(rast, cloud, size):
     s.tile_order = 0
     H.run_gpu(cam, s, inst, rast)
     assert len(rast.debug(_abi.DEBUG_TILE_ORDER)) == 0
+
+
+@pytest.mark.parametrize("segment", [16, 64, 300])
+@pytest.mark.parametrize("depth_mode", [0, 1])
+def test_split_tiles_draw_the_frame_the_one_walk_draws(rast, segment, depth_mode):
+    """Tiles of more than `segment` pairs walked in segments by waves of their
+    own and composited: the same images (to float association), the same last
+    contributor and final transmittance, as the one walk; and the oracle's frame."""
+    print(f"split blend segment {segment} depth mode {depth_mode}", flush=True)
+    cfg = scenes.config(1, count=20000)
+    inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
+    whole = H.run_gpu(cfg.camera, raster.RenderSettings(width=cfg.width, height=cfg.height, depth_mode=depth_mode,
+                                                        blend_segment=0), inst, rast)
+    s = raster.RenderSettings(width=cfg.width, height=cfg.height, depth_mode=depth_mode, blend_segment=segment)
+    gpu = H.run_gpu(cfg.camera, s, inst, rast)
+    split = int(np.sum(np.diff(gpu["ranges"], axis=1) > segment))
+    assert split > 0
+    np.testing.assert_allclose(gpu["colour"], whole["colour"], rtol=0, atol=3e-5)
+    np.testing.assert_allclose(gpu["depth"], whole["depth"], rtol=1e-4, atol=1e-4)
+    np.testing.assert_allclose(gpu["final_t"], whole["final_t"], rtol=1e-4, atol=1e-7)
+    assert np.mean(gpu["last_count"] == whole["last_count"]) >= 0.9999
+    ora = H.run_oracle(cfg.camera, s, inst)
+    H.assert_bit_exact(gpu, ora)
+    _image_checks(gpu, ora)
