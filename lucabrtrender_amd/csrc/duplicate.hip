@@ -44,74 +44,77 @@ __device__ __forceinline__ void emitPair(uint32_t at, uint64_t key, uint32_t s, 
     }
 }
 
-// Slot s's pairs: keys (tileBase + tile) << 32 | bits(view z) in the frame of
-// width x height with tilesX tiles a row, in the row-major order of its tile
-// rectangle, stopping at the count the projection gave. A rectangle of at most
-// kWaveFootprint tiles is walked by its own lane; larger ones by every active
-// lane together, one slot at a time, each pair's place the slot's pairs so far
-// plus its rank among this round's touched tiles. Must be reached by every
-// active lane (a lane with nothing to write passes touched 0).
-template <bool EMIT_INDEX>
-__device__ __forceinline__ void duplicateSlot(uint32_t s, uint32_t expected, uint32_t width, uint32_t height,
+// The wave's slots' pairs: keys (tileBase + tile) << 32 | bits(view z) in the
+// frame of width x height with tilesX tiles a row, each slot's in the
+// row-major order of its tile rectangle, stopping at the count the projection
+// gave. Every lane's rectangle is one run of a wave-wide candidate list
+// (countTouchedTilesWave); a round tests 64 consecutive candidates, and a
+// touched tile's pair goes to its slot's offset plus the slot's pairs so far
+// plus its rank among this round's touched tiles of the same slot -- the place
+// the one-lane row-major walk gives it. Every lane of the wave must take part
+// (a lane with nothing to write passes expected 0). PER_VIEW: the frame sizes
+// differ across lanes and travel with each candidate.
+template <bool EMIT_INDEX, bool PER_VIEW>
+__device__ __forceinline__ void duplicateWave(uint32_t s, uint32_t expected, uint32_t width, uint32_t height,
                                               uint32_t tilesX, uint32_t tileBase, const float4* __restrict__ proj,
                                               const uint2* __restrict__ rects, const uint32_t* __restrict__ offsets,
                                               uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
                                               uint32_t* __restrict__ pairSlot, uint32_t capacity,
                                               uint32_t* __restrict__ overflow) {
-    uint32_t first = 0, x0 = 1, y0 = 1, x1 = 0, y1 = 0;
+    const uint32_t lane = __lane_id();
+    uint32_t first = 0, x0 = 0, y0 = 0, w = 1, area = 0;
     float4 screen = make_float4(0.0f, 0.0f, 0.0f, 0.0f), conic = screen;
     if (expected != 0) {
         first = offsets[s];
         screen = proj[static_cast<uint64_t>(s) * 3];
         conic = proj[static_cast<uint64_t>(s) * 3 + 1];
         const uint2 rc = rects[s];
-        x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16, x1 = rc.y & 0xFFFFu, y1 = rc.y >> 16;
+        x0 = rc.x & 0xFFFFu, y0 = rc.x >> 16;
+        w = (rc.y & 0xFFFFu) - x0 + 1u;
+        area = w * ((rc.y >> 16) - y0 + 1u);
     }
-    const uint32_t w = x1 - x0 + 1u;
-    const uint32_t area = expected != 0 ? w * (y1 - y0 + 1u) : 0u;
-    const bool big = area > kWaveFootprint;
     const uint32_t depthBits = floatToBits(screen.z);
-    if (expected != 0 && !big) {
-        uint32_t written = 0;
-        for (uint32_t ty = y0; ty <= y1 && written < expected; ++ty) {
-            for (uint32_t tx = x0; tx <= x1 && written < expected; ++tx) {
-                if (!tileTouched(width, height, screen.x, screen.y, conic.x, conic.y, conic.z, screen.w, tx, ty)) {
-                    continue;
-                }
-                const uint64_t key = (static_cast<uint64_t>(tileBase + ty * tilesX + tx) << 32) | depthBits;
-                emitPair<EMIT_INDEX>(first + written, key, s, keys, values, pairSlot, capacity, overflow);
-                written += 1;
+    const WaveTiles wt = waveTiles(area);
+    const bool exact = wt.total >= (1u << 20);
+    const uint64_t below = (1ull << lane) - 1ull;
+    const uint32_t slotOfLane0 = s - lane;   // the wave's slots are consecutive
+    uint32_t written = 0;                    // this lane's slot's pairs so far
+    for (uint32_t base = 0; base < wt.total; base += 64) {
+        const uint32_t cand = base + lane;
+        const uint32_t o = ownerOf(wt.incl, cand);
+        const uint32_t oStart = shflU(wt.start, o), oW = shflU(w, o), ox0 = shflU(x0, o), oy0 = shflU(y0, o);
+        const float ocx = shflF(screen.x, o), ocy = shflF(screen.y, o), ocut = shflF(screen.w, o);
+        const float oa = shflF(conic.x, o), ob = shflF(conic.y, o), oc = shflF(conic.z, o);
+        uint32_t ow = width, oh = height, oTilesX = tilesX, oBase = tileBase;
+        if constexpr (PER_VIEW) {
+            ow = shflU(width, o);
+            oh = shflU(height, o);
+            oTilesX = shflU(tilesX, o);
+            oBase = shflU(tileBase, o);
+        }
+        const uint32_t oFirst = shflU(first, o), oExpected = shflU(expected, o), oWritten = shflU(written, o);
+        const uint32_t oDepth = shflU(depthBits, o);
+        bool hit = false;
+        uint32_t tx = 0, ty = 0;
+        if (cand < wt.total) {
+            uint32_t row, col;
+            rowColumn(cand - oStart, oW, exact, row, col);
+            tx = ox0 + col;
+            ty = oy0 + row;
+            hit = tileTouched(ow, oh, ocx, ocy, oa, ob, oc, ocut, tx, ty);
+        }
+        const uint64_t vote = __ballot(hit);
+        if (hit) {
+            // The owner's run begins at bit runLo of this round (at or below this lane).
+            const uint32_t runLo = oStart > base ? oStart - base : 0u;
+            const uint64_t sameRunBelow = below & ~((1ull << runLo) - 1ull);
+            const uint32_t place = oWritten + static_cast<uint32_t>(__popcll(vote & sameRunBelow));
+            if (place < oExpected) {
+                const uint64_t key = (static_cast<uint64_t>(oBase + ty * oTilesX + tx) << 32) | oDepth;
+                emitPair<EMIT_INDEX>(oFirst + place, key, slotOfLane0 + o, keys, values, pairSlot, capacity, overflow);
             }
         }
-    }
-    const ActiveLanes act = activeLanes();
-    const uint64_t below = (1ull << act.lane) - 1ull;
-    uint64_t todo = __ballot(big);
-    while (todo != 0) {
-        const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(todo));
-        todo &= todo - 1ull;
-        const uint32_t ls = shflU(s, leader), lfirst = shflU(first, leader), lexpected = shflU(expected, leader);
-        const uint32_t lw = shflU(w, leader), larea = shflU(area, leader);
-        const uint32_t lx0 = shflU(x0, leader), ly0 = shflU(y0, leader);
-        const uint32_t lwidth = shflU(width, leader), lheight = shflU(height, leader);
-        const uint32_t ltilesX = shflU(tilesX, leader), lbase = shflU(tileBase, leader);
-        const uint32_t ldepth = shflU(depthBits, leader);
-        const float lcx = shflF(screen.x, leader), lcy = shflF(screen.y, leader), lcut = shflF(screen.w, leader);
-        const float la = shflF(conic.x, leader), lb = shflF(conic.y, leader), lc = shflF(conic.z, leader);
-        uint32_t written = 0;
-        for (uint32_t k0 = 0; k0 < larea && written < lexpected; k0 += act.count) {
-            const uint32_t k = k0 + act.rank;
-            const uint32_t row = k / lw;
-            const uint32_t tx = lx0 + (k - row * lw), ty = ly0 + row;
-            const bool hit = k < larea && tileTouched(lwidth, lheight, lcx, lcy, la, lb, lc, lcut, tx, ty);
-            const uint64_t vote = __ballot(hit);
-            const uint32_t place = written + static_cast<uint32_t>(__popcll(vote & below));
-            if (hit && place < lexpected) {
-                const uint64_t key = (static_cast<uint64_t>(lbase + ty * ltilesX + tx) << 32) | ldepth;
-                emitPair<EMIT_INDEX>(lfirst + place, key, ls, keys, values, pairSlot, capacity, overflow);
-            }
-            written += static_cast<uint32_t>(__popcll(vote));
-        }
+        written += static_cast<uint32_t>(__popcll(vote & runMask(wt.start, wt.area, base)));
     }
 }
 
@@ -126,9 +129,9 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
                                                           uint32_t* __restrict__ pairSlot, uint32_t capacity,
                                                           uint32_t* __restrict__ overflow) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= slots) return;
-    duplicateSlot<EMIT_INDEX>(s, touched[s], f.width, f.height, f.tilesX, 0u, proj, rects, offsets, keys, values,
-                              pairSlot, capacity, overflow);
+    const uint32_t expected = s < slots ? touched[s] : 0u;   // every lane stays for the wave's walk
+    duplicateWave<EMIT_INDEX, false>(s, expected, f.width, f.height, f.tilesX, 0u, proj, rects, offsets, keys,
+                                     values, pairSlot, capacity, overflow);
 }
 
 // A batch of views: slot s belongs to view s / splatsPerView.
@@ -142,8 +145,7 @@ __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* _
                                                                uint32_t* __restrict__ values, uint32_t capacity,
                                                                uint32_t* __restrict__ overflow) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
-    if (s >= slots) return;
-    const uint32_t expected = touched[s];
+    const uint32_t expected = s < slots ? touched[s] : 0u;   // every lane stays for the wave's walk
     uint32_t width = 0, height = 0, tilesX = 0, tileBase = 0;
     if (expected != 0) {
         const ViewInfo& v = views[s / splatsPerView];
@@ -152,8 +154,8 @@ __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* _
         tilesX = v.tilesX;
         tileBase = v.tileBase;
     }
-    duplicateSlot<false>(s, expected, width, height, tilesX, tileBase, proj, rects, offsets, keys, values, nullptr,
-                         capacity, overflow);
+    duplicateWave<false, true>(s, expected, width, height, tilesX, tileBase, proj, rects, offsets, keys, values,
+                               nullptr, capacity, overflow);
 }
 
 }   // namespace

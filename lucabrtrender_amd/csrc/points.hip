@@ -35,29 +35,28 @@ __global__ __launch_bounds__(kBlock) void projectPointsKernel(FrameParams f, Poi
                                                               float4* __restrict__ proj, uint2* __restrict__ rects,
                                                               uint32_t* __restrict__ touchedOut) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= f.count) return;
+    const bool live = i < f.count;   // every lane stays for the wave's tile count
     const uint32_t slot = f.base + i;
     const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
-    const float4 p = positions[i];
+    const float4 p = live ? positions[i] : make_float4(0.0f, 0.0f, 0.0f, 1.0f);
     const float* m = f.m;
     const float tx = m[0] * p.x + m[1] * p.y + m[2] * p.z + m[3];
     const float ty = m[4] * p.x + m[5] * p.y + m[6] * p.z + m[7];
     const float tz = m[8] * p.x + m[9] * p.y + m[10] * p.z + m[11];
-    uint32_t touched = 0;
     uint32_t rect[4] = {1u, 1u, 0u, 0u};
     float cx = 0.0f, cy = 0.0f, k = 0.0f;
-    if (tz > f.nearZ && !(tz > f.farZ)) {
+    bool candidate = false;
+    if (live && tz > f.nearZ && !(tz > f.farZ)) {
         const float radius = pointRadius(f, style, tz);
         const float invZ = f.orthographic ? 1.0f : 1.0f / tz;
         cx = f.focalX * tx * invZ + f.centreX;
         cy = f.focalY * ty * invZ + f.centreY;
         k = 1.0f / (radius * radius);
         tileRectOfExtent(f.width, f.height, cx, cy, radius, radius, rect);
-        if (rect[0] <= rect[2] && rect[1] <= rect[3]) {
-            // every lane with a rectangle reaches the (wave-shared) count
-            touched = countTouchedTiles(f.width, f.height, cx, cy, k, 0.0f, k, 1.0f, rect);
-        }
+        candidate = rect[0] <= rect[2] && rect[1] <= rect[3];
     }
+    const uint32_t touched = countTouchedTilesWave(f.width, f.height, cx, cy, k, 0.0f, k, 1.0f, rect, candidate);
+    if (!live) return;
     if (touched == 0) {
         touchedOut[slot] = 0;
         rects[slot] = emptyRect;

@@ -78,8 +78,8 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
     uint4 row = make_uint4(0u, 0u, 0u, 0u);
     EditFate fate{true, false, 0.0f};
     SplatShape shapeOf{0.0f, 0.0f, 0.0f, 0.0f, 1.0f, 1.0f, 1.0f, {0.0f, 0.0f, 0.0f, 1.0f}};
-    Footprint fp;
-    uint32_t touched = 0;
+    Footprint fp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {1u, 1u, 0u, 0u}};
+    bool candidate = false;
     if (live) {
         const float4 p = positions[i];
         row = shape[i];
@@ -89,9 +89,12 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
             shapeOf = SplatShape{p.x, p.y, p.z, fate.opacity, sx, sy, sz, {0.0f, 0.0f, 0.0f, 0.0f}};
             if (f.useFilter3d) applyFilter3d(filter3d[i], shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
             decodeQuaternion(row.x, shapeOf.q);
-            touched = footprintOf(f, shapeOf, fp);
+            candidate = footprintOf(f, shapeOf, fp);
         }
     }
+    // Every lane of the wave counts together, each rectangle among the wave's.
+    const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
+                                                   fp.rect, candidate);
 
     // The drawn splats in splat order: row `at` of the staged harmonics.
     const bool drawn = touched > 0;

@@ -52,15 +52,15 @@ struct Footprint {
     uint32_t rect[4];           // inclusive tile rectangle x0, y0, x1, y1
 };
 
-// The splat's footprint from frame f; returns the tiles it touches (0:
-// culled, and then fp is partly written). Must be reached by every active
-// lane of the wave (the count of a large footprint is shared among them).
-__device__ __forceinline__ uint32_t footprintOf(const FrameParams& f, const SplatShape& s, Footprint& fp) {
+// The splat's footprint from frame f; false when it is culled or its tile
+// rectangle is empty (and then fp is partly written). The tiles it touches are
+// counted apart (touchedTiles, countTouchedTilesWave).
+__device__ __forceinline__ bool footprintOf(const FrameParams& f, const SplatShape& s, Footprint& fp) {
     const float* m = f.m;
     const float tx = m[0] * s.px + m[1] * s.py + m[2] * s.pz + m[3];
     const float ty = m[4] * s.px + m[5] * s.py + m[6] * s.pz + m[7];
     const float tz = m[8] * s.px + m[9] * s.py + m[10] * s.pz + m[11];
-    if (!(tz > f.nearZ) || tz > f.farZ) return 0;
+    if (!(tz > f.nearZ) || tz > f.farZ) return false;
     fp.tz = tz;
     float r[9];
     rotationOf(s.q, r);
@@ -113,15 +113,14 @@ __device__ __forceinline__ uint32_t footprintOf(const FrameParams& f, const Spla
     a = a + f.dilation;
     c = c + f.dilation;
     const float det = a * c - b * b;
-    if (!(det > 1e-12f)) return 0;
+    if (!(det > 1e-12f)) return false;
     float alpha = s.opacity;
     if (f.antialias) alpha = alpha * __builtin_sqrtf(fmaxS(detBefore, 0.0f) / det);
-    if (!(alpha >= 1.0f / 255.0f)) return 0;
+    if (!(alpha >= 1.0f / 255.0f)) return false;
     const float ca = c / det, cb = -b / det, cc = a / det;
     const float cutoff = cutoffPower(alpha);
     tileRect(f.width, f.height, cx, cy, ca, cb, cc, cutoff, fp.rect);
-    if (fp.rect[0] > fp.rect[2] || fp.rect[1] > fp.rect[3]) return 0;
-    const uint32_t touched = countTouchedTiles(f.width, f.height, cx, cy, ca, cb, cc, cutoff, fp.rect);
+    if (fp.rect[0] > fp.rect[2] || fp.rect[1] > fp.rect[3]) return false;
     fp.cx = cx;
     fp.cy = cy;
     fp.cutoff = cutoff;
@@ -129,7 +128,12 @@ __device__ __forceinline__ uint32_t footprintOf(const FrameParams& f, const Spla
     fp.cb = cb;
     fp.cc = cc;
     fp.alpha = alpha;
-    return touched;
+    return true;
+}
+
+// The tiles a footprint touches, counted by the lanes that reach the call.
+__device__ __forceinline__ uint32_t touchedTiles(const FrameParams& f, const Footprint& fp) {
+    return countTouchedTiles(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff, fp.rect);
 }
 
 // The unit direction from the eye to the splat's centre, in the cloud's own
@@ -159,7 +163,8 @@ template <class ColourOf>
 __device__ __forceinline__ uint32_t projectOne(const FrameParams& f, const SplatShape& s, ColourOf colourOf,
                                                float4* __restrict__ out, uint2* __restrict__ rectOut) {
     Footprint fp;
-    const uint32_t touched = footprintOf(f, s, fp);
+    if (!footprintOf(f, s, fp)) return 0;
+    const uint32_t touched = touchedTiles(f, fp);
     if (touched == 0) return 0;
     float dx, dy, dz;
     viewDirection(f, s, dx, dy, dz);

@@ -266,6 +266,96 @@ __device__ __forceinline__ uint32_t countTouchedTiles(uint32_t width, uint32_t h
     return touched;
 }
 
+// The wave's candidate tiles, flattened: lane L's rectangle (area[L] tiles in
+// row-major order) is candidates [start[L], start[L] + area[L]) of one list,
+// start the exclusive sum of the areas over the lanes. An iteration tests 64
+// consecutive candidates, one a lane, so a wave walks ceil(total / 64) rounds
+// however its areas are spread, instead of its largest area. Every lane of
+// the wave must take part (a lane with nothing to count brings area 0).
+struct WaveTiles {
+    uint32_t incl, start, area, total;
+};
+
+__device__ __forceinline__ WaveTiles waveTiles(uint32_t area) {
+    const uint32_t lane = __lane_id();
+    uint32_t incl = area;
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        const uint32_t o = static_cast<uint32_t>(__shfl_up(static_cast<int>(incl), d, 64));
+        incl += lane >= d ? o : 0u;
+    }
+    return WaveTiles{incl, incl - area, area, shflU(incl, 63)};
+}
+
+// The lane whose run holds candidate `cand`: the first whose inclusive end
+// passes it (a binary search over the monotone ends).
+__device__ __forceinline__ uint32_t ownerOf(uint32_t incl, uint32_t cand) {
+    uint32_t lo = 0;
+#pragma unroll
+    for (uint32_t step = 32; step > 0; step >>= 1) lo += shflU(incl, lo + step - 1u) <= cand ? step : 0u;
+    return lo;
+}
+
+// The bits of this lane's run that fall in the iteration from candidate `base`.
+__device__ __forceinline__ uint64_t runMask(uint32_t start, uint32_t area, uint32_t base) {
+    const int64_t lo = static_cast<int64_t>(start) - base, hi = lo + area;
+    const int64_t l = lo < 0 ? 0 : (lo > 64 ? 64 : lo), h = hi < 0 ? 0 : (hi > 64 ? 64 : hi);
+    if (l >= h) return 0ull;
+    const uint64_t upto = h == 64 ? ~0ull : (1ull << h) - 1ull;
+    return upto & ~((1ull << l) - 1ull);
+}
+
+// k / w and k % w for a candidate's place k in a rectangle w tiles wide: below
+// 2^20 a float estimate (relative error under 2^-22, so off by at most one),
+// corrected; exact division past that.
+__device__ __forceinline__ void rowColumn(uint32_t k, uint32_t w, bool exact, uint32_t& row, uint32_t& col) {
+    if (exact) {
+        row = k / w;
+        col = k - row * w;
+        return;
+    }
+    int q = static_cast<int>(static_cast<float>(k) * __builtin_amdgcn_rcpf(static_cast<float>(w)));
+    int r = static_cast<int>(k) - q * static_cast<int>(w);
+    if (r < 0) {
+        q -= 1;
+        r += static_cast<int>(w);
+    } else if (r >= static_cast<int>(w)) {
+        q += 1;
+        r -= static_cast<int>(w);
+    }
+    row = static_cast<uint32_t>(q);
+    col = static_cast<uint32_t>(r);
+}
+
+// countTouchedTiles for the whole wave at once, each lane's rectangle among
+// the flattened candidates: the same tests, the same count. `want` false: the
+// lane counts nothing (its rect is not read).
+__device__ __forceinline__ uint32_t countTouchedTilesWave(uint32_t width, uint32_t height, float cx, float cy,
+                                                          float a, float b, float c, float cutoff,
+                                                          const uint32_t rect[4], bool want) {
+    const uint32_t lane = __lane_id();
+    const uint32_t w = want ? rect[2] - rect[0] + 1u : 1u;
+    const WaveTiles wt = waveTiles(want ? w * (rect[3] - rect[1] + 1u) : 0u);
+    const bool exact = wt.total >= (1u << 20);
+    uint32_t touched = 0;
+    for (uint32_t base = 0; base < wt.total; base += 64) {
+        const uint32_t cand = base + lane;
+        const uint32_t o = ownerOf(wt.incl, cand);
+        const uint32_t oStart = shflU(wt.start, o), oW = shflU(w, o);
+        const uint32_t ox0 = shflU(want ? rect[0] : 0u, o), oy0 = shflU(want ? rect[1] : 0u, o);
+        const float ocx = shflF(cx, o), ocy = shflF(cy, o), oa = shflF(a, o), ob = shflF(b, o), oc = shflF(c, o);
+        const float ocut = shflF(cutoff, o);
+        bool hit = false;
+        if (cand < wt.total) {
+            uint32_t row, col;
+            rowColumn(cand - oStart, oW, exact, row, col);
+            hit = tileTouched(width, height, ocx, ocy, oa, ob, oc, ocut, ox0 + col, oy0 + row);
+        }
+        touched += static_cast<uint32_t>(__popcll(__ballot(hit) & runMask(wt.start, wt.area, base)));
+    }
+    return touched;
+}
+
 __device__ __forceinline__ uint32_t degreeOf(uint32_t rest) {
     return rest >= 15 ? 3 : rest >= 8 ? 2 : rest >= 3 ? 1 : 0;
 }
