@@ -153,7 +153,7 @@ def cpu_baseline(cfg, settings, seconds_budget=15.0):
 
 
 def fwd_bwd_leg(torch, raster, scenes, steps):
-    """Config 3: 1,000,000 FP32-parameter splats drawn as config 1 with SH degree 3,
+    """fwd-bwd-1M-720p: 1,000,000 FP32-parameter splats of the plain cube with SH degree 3,
     1280x720; loss = per-pixel L2 against the frame of the perturbed scene. A view
     is one differentiable frame, the loss gradient and the backward."""
     params = scenes.diff_params(1_000_000, 0, degree=3)
@@ -242,14 +242,14 @@ def _batch_against_sequence(torch, raster, rast, views, inst, settings, reps):
 
 
 def views_leg(torch, raster, scenes, reps):
-    """Config 4: one 2,000,000-splat clustered scene, 64 cameras on a sphere of radius
+    """views-fwd-2M-64: one 2,000,000-splat clustered scene, 64 cameras on a sphere of radius
     12 looking at its centre, each view's size drawn from {256x256, 512x384, 960x540,
     1280x720, 1920x1080}. The batch (one launch sequence, autotuned) against the plain
     sequence of 64 single-view frames."""
     import math
 
     import numpy as np
-    cfg = scenes.config(2, count=2_000_000)
+    cfg = scenes.workload("scene-fwd-3M-1080p", count=2_000_000)
     inst = [raster.SplatInstance(cfg.scene.to_device())]
     rng = np.random.default_rng(4)
     sizes = [(256, 256), (512, 384), (960, 540), (1280, 720), (1920, 1080)]
@@ -388,10 +388,10 @@ def lod_leg(torch, raster, scenes, cfg, dev, settings, reps):
 
 
 def sort_stress_leg(torch, raster, scenes, _abi, reps):
-    """Config 5: 6,000,000 splats at 3840x2160, 80% as config 1 and 20% large splats
+    """sort-stress-6M-2160p: 6,000,000 splats at 3840x2160, 80% of the plain cube and 20% large splats
     near the camera; the frame's stages timed, the sort's keys/s and HBM fraction."""
     import numpy as np
-    cfg = scenes.config(5)
+    cfg = scenes.workload("sort-stress-6M-2160p")
     dev = cfg.scene.to_device()
     settings = raster.RenderSettings(width=cfg.width, height=cfg.height, time_stages=True)
     targets = raster.RenderTargets(cfg.width, cfg.height)
@@ -481,15 +481,15 @@ def main():
     ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
     ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 128 or 256")
-    ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="config-3 fwd+bwd views timed; 0 skips it")
-    ap.add_argument("--stress-reps", type=int, default=3, help="config-5 frames timed; 0 skips it")
-    ap.add_argument("--views-reps", type=int, default=5, help="config-4 batches timed; 0 skips it")
+    ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="fwd+bwd views (1M splats, 720p) timed; 0 skips it")
+    ap.add_argument("--stress-reps", type=int, default=3, help="sort-stress frames (6M splats, 2160p) timed; 0 skips it")
+    ap.add_argument("--views-reps", type=int, default=5, help="64-view batches (2M splats) timed; 0 skips it")
     ap.add_argument("--load-reps", type=int, default=3, help="3M-record decodes timed; 0 skips it")
     ap.add_argument("--lod-reps", type=int, default=5, help="LOD cuts and frames timed; 0 skips it")
     ap.add_argument("--composite-reps", type=int, default=5,
                     help="composite frames (1M discs with the 3M splats) timed; 0 skips it")
     ap.add_argument("--profile-frames", type=int, default=0,
-                    help="only draw this many config-2 frames and one 256 MiB device copy (the counter "
+                    help="only draw this many 3M-splat 1080p frames and one 256 MiB device copy (the counter "
                          "calibration), for rocprofv3 --pmc; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -503,7 +503,7 @@ def main():
     from lucabrtrender_amd import _abi, raster, scenes
 
     if args.profile_frames > 0:
-        cfg = scenes.config(2, count=args.count)
+        cfg = scenes.workload("scene-fwd-3M-1080p", count=args.count)
         settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
                                          filter3d=cfg.filter3d, blend_batch=args.blend_batch)
         inst = [raster.SplatInstance(cfg.scene.to_device())]
@@ -519,7 +519,7 @@ def main():
         return
 
     t0 = time.perf_counter()
-    cfg = scenes.config(2, count=args.count)
+    cfg = scenes.workload("scene-fwd-3M-1080p", count=args.count)
     log(f"scene: {cfg.scene.count} splats generated in {time.perf_counter() - t0:.1f} s")
     settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
                                      filter3d=cfg.filter3d, blend_batch=args.blend_batch)

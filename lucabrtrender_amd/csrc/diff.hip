@@ -76,9 +76,9 @@ __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffC
                                                             uint2* __restrict__ rects,
                                                             uint32_t* __restrict__ touchedOut) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= f.count) return;
+    const bool live = i < f.count;   // every lane stays for the wave's tile count
     const uint32_t slot = f.base + i;
-    const Decoded d = decode(c, i, f.useFilter3d != 0);
+    const Decoded d = decode(c, live ? i : 0u, f.useFilter3d != 0);
     const uint32_t degree = shDegree(f);
     const uint32_t stride = (1u + c.restPerColour) * 3u;
     const float* row = c.sh + static_cast<uint64_t>(i) * stride;
@@ -94,7 +94,8 @@ __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffC
     };
     const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
     uint2 rect = emptyRect;
-    const uint32_t touched = projectOne(f, d.shape, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
+    const uint32_t touched = projectWave(f, d.shape, live, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
+    if (!live) return;
     rects[slot] = touched > 0 ? rect : emptyRect;
     touchedOut[slot] = touched;
 }
@@ -458,7 +459,7 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
     const float dCx = g[0], dCy = g[1], dCa = g[2], dCb = g[3], dCc = g[4], dAlpha = g[5];
     const float dCol[3] = {g[6], g[7], g[8]};
 
-    // Recompute the forward (projectOne) and walk it back.
+    // Recompute the forward (projectWave's footprint) and walk it back.
     const Decoded dec = decode(c, i, f.useFilter3d != 0);
     const SplatShape& s = dec.shape;
     const float* m = f.m;

@@ -232,17 +232,13 @@ __global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* 
         const uint32_t* src = reinterpret_cast<const uint32_t*>(params + v0);
         for (uint32_t w = tid; w < n * kWords; w += kBlock) sFWords[w] = src[w];
         __syncthreads();
-        if (!live) continue;
-        for (uint32_t v = 0; v < n; ++v) {
+        for (uint32_t v = 0; v < n; ++v) {   // every lane stays for the wave's tile count
             const FrameParams& f = sF[v];
             const uint32_t slot = f.base + i;
-            if (fate.removed) {
-                touchedOut[slot] = 0;
-                rects[slot] = emptyRect;
-                continue;
-            }
             uint2 rect = emptyRect;
-            const uint32_t touched = projectOne(f, shapeOf, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
+            const uint32_t touched = projectWave(f, shapeOf, live && !fate.removed, colourOf,
+                                                 proj + static_cast<uint64_t>(slot) * 3, &rect);
+            if (!live) continue;
             rects[slot] = touched > 0 ? rect : emptyRect;
             touchedOut[slot] = touched;
         }

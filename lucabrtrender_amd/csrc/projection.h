@@ -54,7 +54,7 @@ struct Footprint {
 
 // The splat's footprint from frame f; false when it is culled or its tile
 // rectangle is empty (and then fp is partly written). The tiles it touches are
-// counted apart (touchedTiles, countTouchedTilesWave).
+// counted apart, wave-wide (countTouchedTilesWave).
 __device__ __forceinline__ bool footprintOf(const FrameParams& f, const SplatShape& s, Footprint& fp) {
     const float* m = f.m;
     const float tx = m[0] * s.px + m[1] * s.py + m[2] * s.pz + m[3];
@@ -131,11 +131,6 @@ __device__ __forceinline__ bool footprintOf(const FrameParams& f, const SplatSha
     return true;
 }
 
-// The tiles a footprint touches, counted by the lanes that reach the call.
-__device__ __forceinline__ uint32_t touchedTiles(const FrameParams& f, const Footprint& fp) {
-    return countTouchedTiles(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff, fp.rect);
-}
-
 // The unit direction from the eye to the splat's centre, in the cloud's own
 // space: the argument of the colour's harmonics.
 __device__ __forceinline__ void viewDirection(const FrameParams& f, const SplatShape& s, float& dx, float& dy,
@@ -156,15 +151,19 @@ __device__ __forceinline__ void writeProjection(const Footprint& fp, const float
     *rectOut = make_uint2(fp.rect[0] | (fp.rect[1] << 16), fp.rect[2] | (fp.rect[3] << 16));
 }
 
-// Projects splat `slot`; returns the tiles it touches (0: culled, and then the
-// record is left alone). `colourOf(dx, dy, dz, rgb)` gives the colour for the
-// unit direction from the eye, called only for a splat that is drawn.
+// Projects one splat a lane, `want` false for a lane with none; returns the
+// tiles its footprint touches (0: culled, and then the record is left alone).
+// `colourOf(dx, dy, dz, rgb)` gives the colour for the unit direction from the
+// eye, called only for a splat that is drawn. Every lane of the wave must
+// reach the call: the tiles are counted wave-wide (countTouchedTilesWave).
 template <class ColourOf>
-__device__ __forceinline__ uint32_t projectOne(const FrameParams& f, const SplatShape& s, ColourOf colourOf,
-                                               float4* __restrict__ out, uint2* __restrict__ rectOut) {
-    Footprint fp;
-    if (!footprintOf(f, s, fp)) return 0;
-    const uint32_t touched = touchedTiles(f, fp);
+__device__ __forceinline__ uint32_t projectWave(const FrameParams& f, const SplatShape& s, bool want,
+                                                ColourOf colourOf, float4* __restrict__ out,
+                                                uint2* __restrict__ rectOut) {
+    Footprint fp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {1u, 1u, 0u, 0u}};
+    const bool candidate = want && footprintOf(f, s, fp);
+    const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
+                                                   fp.rect, candidate);
     if (touched == 0) return 0;
     float dx, dy, dz;
     viewDirection(f, s, dx, dy, dz);

@@ -200,70 +200,9 @@ __device__ __forceinline__ void tileRect(uint32_t width, uint32_t height, float 
     tileRectOfExtent(width, height, cx, cy, ex, ey, rect);
 }
 
-// Tile rectangles of more than this many tiles are walked by the whole wave;
-// smaller ones by their own lane. A lane-serial walk of a large footprint
-// would hold the other 63 lanes for its full length.
-constexpr uint32_t kWaveFootprint = 16;
-
 __device__ __forceinline__ float shflF(float v, uint32_t lane) { return __shfl(v, static_cast<int>(lane), 64); }
 __device__ __forceinline__ uint32_t shflU(uint32_t v, uint32_t lane) {
     return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(lane), 64));
-}
-
-// The lanes active at a call, as a set and as this lane's place in it: a wave
-// walk shares its tiles among exactly the lanes that reached it.
-struct ActiveLanes {
-    uint64_t mask;
-    uint32_t lane, rank, count;
-};
-__device__ __forceinline__ ActiveLanes activeLanes() {
-    ActiveLanes a;
-    a.mask = __ballot(1);
-    a.lane = __lane_id();
-    a.rank = static_cast<uint32_t>(__popcll(a.mask & ((1ull << a.lane) - 1ull)));
-    a.count = static_cast<uint32_t>(__popcll(a.mask));
-    return a;
-}
-
-// The number of tiles of rect (inclusive x0, y0, x1, y1) that the footprint
-// touches. Lanes with a rectangle of at most kWaveFootprint tiles count their
-// own; each larger one is then counted by all active lanes together, tile k of
-// its row-major order going to active lane k mod (active count). Same tests,
-// same count as the one-lane walk. Must be reached by every active lane.
-__device__ __forceinline__ uint32_t countTouchedTiles(uint32_t width, uint32_t height, float cx, float cy, float a,
-                                                      float b, float c, float cutoff, const uint32_t rect[4]) {
-    const uint32_t w = rect[2] - rect[0] + 1u;
-    const uint32_t area = w * (rect[3] - rect[1] + 1u);
-    const bool big = area > kWaveFootprint;
-    uint32_t touched = 0;
-    if (!big) {
-        for (uint32_t ty = rect[1]; ty <= rect[3]; ++ty)
-            for (uint32_t tx = rect[0]; tx <= rect[2]; ++tx)
-                touched += tileTouched(width, height, cx, cy, a, b, c, cutoff, tx, ty) ? 1u : 0u;
-    }
-    const ActiveLanes act = activeLanes();
-    uint64_t todo = __ballot(big);
-    while (todo != 0) {
-        const uint32_t leader = static_cast<uint32_t>(__builtin_ctzll(todo));
-        todo &= todo - 1ull;
-        const uint32_t lw = shflU(w, leader), larea = shflU(area, leader);
-        const uint32_t lx0 = shflU(rect[0], leader), ly0 = shflU(rect[1], leader);
-        const uint32_t lwidth = shflU(width, leader), lheight = shflU(height, leader);
-        const float lcx = shflF(cx, leader), lcy = shflF(cy, leader), la = shflF(a, leader);
-        const float lb = shflF(b, leader), lc = shflF(c, leader), lcut = shflF(cutoff, leader);
-        uint32_t count = 0;
-        for (uint32_t k0 = 0; k0 < larea; k0 += act.count) {
-            const uint32_t k = k0 + act.rank;
-            bool hit = false;
-            if (k < larea) {
-                const uint32_t row = k / lw;
-                hit = tileTouched(lwidth, lheight, lcx, lcy, la, lb, lc, lcut, lx0 + (k - row * lw), ly0 + row);
-            }
-            count += static_cast<uint32_t>(__popcll(__ballot(hit)));
-        }
-        if (act.lane == leader) touched = count;
-    }
-    return touched;
 }
 
 // The wave's candidate tiles, flattened: lane L's rectangle (area[L] tiles in
@@ -327,9 +266,10 @@ __device__ __forceinline__ void rowColumn(uint32_t k, uint32_t w, bool exact, ui
     col = static_cast<uint32_t>(r);
 }
 
-// countTouchedTiles for the whole wave at once, each lane's rectangle among
-// the flattened candidates: the same tests, the same count. `want` false: the
-// lane counts nothing (its rect is not read).
+// The number of tiles of each lane's rect (inclusive x0, y0, x1, y1) that its
+// footprint touches (tileTouched), every lane's rectangle among the wave's
+// flattened candidates. `want` false: the lane counts nothing (its rect is not
+// read). Every lane of the wave must reach the call.
 __device__ __forceinline__ uint32_t countTouchedTilesWave(uint32_t width, uint32_t height, float cx, float cy,
                                                           float a, float b, float c, float cutoff,
                                                           const uint32_t rect[4], bool want) {

@@ -1,20 +1,20 @@
-"""The seeded synthetic scenes and cameras of the benchmark configurations.
+"""The seeded synthetic scenes and cameras of the benchmark workloads.
 
 Scenes are generated and packed by liblrt_hip.so (lrt_synth_scene) into the
 GpuSplats layout in host memory. Each splat draws from its own seeded stream,
 so a scene of N splats is the first N splats of any larger one.
 
-  config 1  65,536 splats, means U[-1,1]^3 4 units down -Z of a camera at the
-            origin, log-scales N(-4, 0.5), opacity sigmoid(N(0,1)), degree 0,
-            640x480, 60 degree horizontal field of view.
-  config 2  3,000,000 splats in 256 anisotropic clusters (centres U[-10,10]^3,
+  plain-fwd-65k-640x480  65,536 splats, means U[-1,1]^3 4 units down -Z of a
+            camera at the origin, log-scales N(-4, 0.5), opacity sigmoid(N(0,1)),
+            degree 0, 640x480, 60 degree horizontal field of view (the plain cube).
+  scene-fwd-3M-1080p  3,000,000 splats in 256 anisotropic clusters (centres U[-10,10]^3,
             members N(centre, diag(U[0.05,1])^2)), log-scales N(-5, 0.8), degree 3
             (band l ~ N(0, 0.3 / l)), 1920x1080, camera on an orbit of radius 15
             about the origin, 60 degrees; Mip-Splatting 3D filter on, 2D filter 0.1.
-  config 3  1,000,000 splats drawn as config 1, degree 3, 1280x720.
-  config 5  6,000,000 splats at 3840x2160: 80% as config 1, 20% pulled toward
+  plain-fwd-1M-720p  1,000,000 splats of the plain cube, degree 3, 1280x720.
+  sort-stress-6M-2160p  6,000,000 splats at 3840x2160: 80% of the plain cube, 20% pulled toward
             the camera (into 0.2-1 of their distance) with log-scales N(-3.6, 0.3),
-            which puts about 7e8 (tile, splat) pairs in the frame (the config-1
+            which puts about 7e8 (tile, splat) pairs in the frame (the plain-cube
             part alone gives 51 a splat at this size); opacity sigmoid(N(1,1)),
             degree 3.
 """
@@ -142,7 +142,7 @@ SH_C0 = 0.28209479177387814
 
 def diff_params(count: int, seed: int = 0, degree: int = 3, offset=(0.0, 0.0, -4.0), log_scale=(-4.0, 0.5),
                 opacity_logit=(0.0, 1.0), sh_sd: float = 0.3) -> dict:
-    """FP32 parameters of a cloud drawn as config 1 (the differentiable frame's
+    """FP32 parameters of a plain cube (the differentiable frame's
     input): means U[-1,1]^3 + offset, log-scales N(mean, sd), quaternions of a
     4-D normal (uniform rotations), opacity logits N(mean, sd), dc from a base
     colour U[0,1] (base = 0.5 + C0 dc), rest band l ~ N(0, sh_sd / l)."""
@@ -171,7 +171,7 @@ def perturbed(params: dict, seed: int = 1, means_sd=0.01, logit_sd=0.2, sh_sd=0.
 
 
 @dataclass
-class Config:
+class Workload:
     name: str
     scene: HostScene
     camera: Camera
@@ -181,28 +181,32 @@ class Config:
     filter3d: bool = False
 
 
-def config(n: int, count: Optional[int] = None, seed: int = 0, view: float = 0.0) -> Config:
-    """The benchmark configuration n (1, 2, 3 or 5) as a scene, a camera and a frame size."""
-    if n == 1:
+WORKLOADS = ("plain-fwd-65k-640x480", "scene-fwd-3M-1080p", "plain-fwd-1M-720p", "sort-stress-6M-2160p")
+
+
+def workload(name: str, count: Optional[int] = None, seed: int = 0, view: float = 0.0) -> Workload:
+    """The benchmark workload `name` (one of WORKLOADS) as a scene, a camera and a frame size;
+    `count` draws the first `count` splats of its scene instead."""
+    if name == "plain-fwd-65k-640x480":
         scene = synth(0, count or 65536, seed, degree=0, offset=(0.0, 0.0, -4.0), log_scale=(-4.0, 0.5))
         cam = Camera.looking_at((0.0, 0.0, 0.0), (0.0, 0.0, -1.0))
         cam.lens = Lens(focal=focal_for_fov(60.0))
-        return Config("plain-fwd-65k-640x480", scene, cam, 640, 480)
-    if n == 2:
+        return Workload(name, scene, cam, 640, 480)
+    if name == "scene-fwd-3M-1080p":
         scene = synth(1, count or 3_000_000, seed, degree=3, log_scale=(-5.0, 0.8), clusters=256, box_half=10.0)
         cam = orbit_camera(15.0, view)
         cams = [orbit_camera(15.0, 360.0 * k / 16.0) for k in range(16)]
         scene.filter3d = filter3d_for(scene.positions, cams, 1920, 1080)
-        return Config("scene-fwd-3M-1080p", scene, cam, 1920, 1080, dilation=0.1, filter3d=True)
-    if n == 3:
+        return Workload(name, scene, cam, 1920, 1080, dilation=0.1, filter3d=True)
+    if name == "plain-fwd-1M-720p":
         scene = synth(0, count or 1_000_000, seed, degree=3, offset=(0.0, 0.0, -4.0), log_scale=(-4.0, 0.5))
         cam = Camera.looking_at((0.0, 0.0, 0.0), (0.0, 0.0, -1.0))
         cam.lens = Lens(focal=focal_for_fov(60.0))
-        return Config("fwd-1M-720p", scene, cam, 1280, 720)
-    if n == 5:
+        return Workload(name, scene, cam, 1280, 720)
+    if name == "sort-stress-6M-2160p":
         scene = synth(2, count or 6_000_000, seed, degree=3, offset=(0.0, 0.0, -4.0), log_scale=(-4.0, 0.5),
                       opacity_logit=(1.0, 1.0), near=(0.2, -3.6, 0.3))
         cam = Camera.looking_at((0.0, 0.0, 0.0), (0.0, 0.0, -1.0))
         cam.lens = Lens(focal=focal_for_fov(60.0))
-        return Config("sort-stress-6M-2160p", scene, cam, 3840, 2160)
-    raise ValueError(f"no configuration {n}")
+        return Workload(name, scene, cam, 3840, 2160)
+    raise ValueError(f"no workload {name!r}; the workloads are {WORKLOADS}")

@@ -10,8 +10,8 @@
 - The torch.autograd wrapper gives the same gradients as the C ABI, and the
   context refuses a backward that no differentiable frame precedes.
 
-The sub-scene is config 3's finite-difference case: 1,024 Gaussians drawn as
-config 1 with SH degree 3 and FP32 parameters, here at 128 x 96 with
+The sub-scene is the finite-difference case of the 1M-splat 720p fwd+bwd
+workload: 1,024 Gaussians of the plain cube with SH degree 3 and FP32 parameters, here at 128 x 96 with
 log-scales N(-3, 0.4) so the footprints span a few pixels of the small frame.
 """
 import numpy as np
@@ -169,7 +169,7 @@ def test_backward_needs_a_differentiable_frame(rast):
     s = raster.RenderSettings(width=32, height=32)
     dev = _device(p)
     grads = {k: torch.empty_like(v) for k, v in dev.items()}
-    cfg = scenes.config(1, count=256)
+    cfg = scenes.workload("plain-fwd-65k-640x480", count=256)
     rast.render(_camera(), [raster.SplatInstance(cfg.scene.to_device())], s, raster.RenderTargets(32, 32))
     with pytest.raises(LrtError) as err:
         rast.backward(torch.zeros((32, 32, 4), device="cuda"), grads)

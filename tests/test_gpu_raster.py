@@ -112,13 +112,13 @@ def test_settings_variants(rast, cloud, settings):
     _image_checks(gpu, ora)
 
 
-def test_config1_full_size(rast):
-    cfg = scenes.config(1)
+def test_plain_65k_full_size(rast):
+    cfg = scenes.workload("plain-fwd-65k-640x480")
     s = raster.RenderSettings(width=cfg.width, height=cfg.height)
     inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
     gpu = H.run_gpu(cfg.camera, s, inst, rast)
     ora = H.run_oracle(cfg.camera, s, inst)
-    print("config 1: pairs", ora["pairs"], "visible", ora["visible"])
+    print("plain 65k: pairs", ora["pairs"], "visible", ora["visible"])
     H.assert_bit_exact(gpu, ora)
     _image_checks(gpu, ora)
     # brute force on a pixel subset, exact view-z order and the rasteriser's key order
@@ -129,14 +129,14 @@ def test_config1_full_size(rast):
     ours = gpu["colour"][ys, xs]
     e99, emax = O.linear_error(ours, exact)
     k99, kmax = O.linear_error(ours, keyed)
-    print(f"config 1 vs brute force: exact order p99 {e99:.2e} max {emax:.2e}; key order p99 {k99:.2e} "
+    print(f"plain 65k vs brute force: exact order p99 {e99:.2e} max {emax:.2e}; key order p99 {k99:.2e} "
           f"max {kmax:.2e}")
     assert e99 <= 1e-3 and k99 <= 1e-3 and kmax <= 1e-2
 
 
 @pytest.mark.parametrize("count", [200_000])
-def test_config2_scene_scaled(rast, count):
-    cfg = scenes.config(2, count=count)
+def test_clustered_scene_scaled(rast, count):
+    cfg = scenes.workload("scene-fwd-3M-1080p", count=count)
     s = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation, filter3d=cfg.filter3d)
     inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
     gpu = H.run_gpu(cfg.camera, s, inst, rast)
@@ -145,26 +145,26 @@ def test_config2_scene_scaled(rast, count):
     _image_checks(gpu, ora)
 
 
-def test_config5_large_footprints_scaled(rast):
-    cfg = scenes.config(5, count=60_000)
+def test_sort_stress_large_footprints_scaled(rast):
+    cfg = scenes.workload("sort-stress-6M-2160p", count=60_000)
     s = raster.RenderSettings(width=cfg.width, height=cfg.height)
     inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
     gpu = H.run_gpu(cfg.camera, s, inst, rast)
     ora = H.run_oracle(cfg.camera, s, inst)
     t = ora["touched"]
-    print("config 5 scaled: pairs", ora["pairs"], "max tiles a splat", int(t.max()))
+    print("sort stress scaled: pairs", ora["pairs"], "max tiles a splat", int(t.max()))
     assert int(t.max()) > 256
     H.assert_bit_exact(gpu, ora)
     _image_checks(gpu, ora)
 
 
-def test_config2_full_size_bit_exact_and_deterministic(rast):
-    cfg = scenes.config(2)
+def test_clustered_3m_full_size_bit_exact_and_deterministic(rast):
+    cfg = scenes.workload("scene-fwd-3M-1080p")
     s = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation, filter3d=cfg.filter3d)
     inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
     gpu = H.run_gpu(cfg.camera, s, inst, rast)
     ora = H.run_oracle(cfg.camera, s, inst)
-    print("config 2: pairs", ora["pairs"], "visible", ora["visible"])
+    print("scene 3M: pairs", ora["pairs"], "visible", ora["visible"])
     H.assert_bit_exact(gpu, ora)
     _image_checks(gpu, ora)
     again = H.run_gpu(cfg.camera, s, inst, rast)
@@ -229,7 +229,7 @@ def test_every_batch_depth_and_tile_order_draws_the_same_frame(rast, cloud, batc
     ora = H.run_oracle(cam, s, inst)
     H.assert_bit_exact(gpu, ora)
     _image_checks(gpu, ora)
-    cfg = scenes.config(1, count=20000)
+    cfg = scenes.workload("plain-fwd-65k-640x480", count=20000)
     s1 = raster.RenderSettings(width=cfg.width, height=cfg.height, blend_batch=batch, tile_order=order, depth_mode=1)
     inst1 = [(cfg.scene, np.eye(4), raster.SplatEdit())]
     gpu = H.run_gpu(cfg.camera, s1, inst1, rast)
@@ -269,7 +269,7 @@ def test_split_tiles_draw_the_frame_the_one_walk_draws(rast, segment, depth_mode
     own and composited: the same images (to float association), the same last
     contributor and final transmittance, as the one walk; and the oracle's frame."""
     print(f"split blend segment {segment} depth mode {depth_mode}", flush=True)
-    cfg = scenes.config(1, count=20000)
+    cfg = scenes.workload("plain-fwd-65k-640x480", count=20000)
     inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
     whole = H.run_gpu(cfg.camera, raster.RenderSettings(width=cfg.width, height=cfg.height, depth_mode=depth_mode,
                                                         blend_segment=0), inst, rast)

@@ -64,7 +64,7 @@ def test_rasteriser_frame_against_the_fp64_brute_force_at_1080p(rast):
     reference's tile path shares, also max <= 1e-2. Splats whose depths tie
     under the 24-bit key but not exactly are blended in another order by the
     two, which is where the first comparison's largest differences sit."""
-    cfg = scenes.config(2, count=200_000)
+    cfg = scenes.workload("scene-fwd-3M-1080p", count=200_000)
     s = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation, filter3d=cfg.filter3d)
     inst = [raster.SplatInstance(cfg.scene.to_device())]
     targets = raster.RenderTargets(cfg.width, cfg.height)
@@ -75,7 +75,7 @@ def test_rasteriser_frame_against_the_fp64_brute_force_at_1080p(rast):
         torch.cuda.synchronize()
         d = np.abs(ours - colour.cpu().numpy()).reshape(-1)
         p99, mx = float(np.quantile(d, 0.99)), float(d.max())
-        print(f"config 2 (200k splats) 1920x1080 vs FP64 brute force, keyed {keyed}: p99 {p99:.2e} max {mx:.2e} "
+        print(f"clustered scene (200k splats) 1920x1080 vs FP64 brute force, keyed {keyed}: p99 {p99:.2e} max {mx:.2e} "
               f"over 1e-3: {np.mean(d > 1e-3):.2e}")
         assert p99 <= 1e-3
         if keyed:

@@ -108,8 +108,8 @@ def test_autotune_picks_one_of_its_candidates(cloud):
     assert out.pairs > 0
 
 
-def test_config2_scene_in_a_batch_of_eight_orbit_views():
-    cfg = scenes.config(2, count=200_000)
+def test_clustered_scene_in_a_batch_of_eight_orbit_views():
+    cfg = scenes.workload("scene-fwd-3M-1080p", count=200_000)
     dev = cfg.scene.to_device()
     inst = [raster.SplatInstance(dev)]
     settings = raster.RenderSettings(dilation=cfg.dilation, filter3d=cfg.filter3d, use_graph=True)

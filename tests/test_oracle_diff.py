@@ -32,7 +32,7 @@ def _frame(count, rest, linearise=True, filter3d=False, antialias=True, backgrou
 
 
 def _scene(count=300, seed=3):
-    # Larger than config 1's splats so that a 64x48 frame resolves their shapes.
+    # Larger than the plain cube's splats so that a 64x48 frame resolves their shapes.
     return scenes.diff_params(count, seed, degree=3, log_scale=(-2.6, 0.4))
 
 
