@@ -401,16 +401,16 @@ def sort_stress_leg(torch, raster, scenes, _abi, reps):
     acc = np.zeros(8)
     start = time.perf_counter()
     for _ in range(reps):
-        rast.render(cfg.camera, inst, settings, targets)
+        st = rast.render(cfg.camera, inst, settings, targets)
         acc += rast.debug(_abi.DEBUG_KERNEL_MS)
     wall = (time.perf_counter() - start) / reps
     ms = acc / reps
     n, visible, pairs, tiles = (int(v) for v in rast.debug(_abi.DEBUG_COUNTS))
-    passes = (24 + max(1, int(np.ceil(np.log2(max(tiles, 2))))) + 7) // 8
+    passes = st.sort_passes
     sort_ms = float(ms[3] + ms[4])
     sort_bytes = pairs * (8 + 24 * passes)
     return {"workload": "sort-stress-6M-2160p", "splats": n, "visible": visible, "pairs": pairs,
-            "sort_passes": passes, "sort_ms": round(sort_ms, 4),
+            "sort_passes": passes, "key_bits": st.key_bits, "sort_ms": round(sort_ms, 4),
             "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3),
             "sort_hbm_frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "duplicate_ms": round(float(ms[2]), 4), "blend_ms": round(float(ms[6]), 4),
@@ -545,12 +545,12 @@ def main():
     reps = max(10, min(args.steps, 50))
     acc = np.zeros(8)
     for _ in range(reps):
-        rast.render(proj, inst, timed, targets)
+        timed_stats = rast.render(proj, inst, timed, targets)
         acc += rast.debug(_abi.DEBUG_KERNEL_MS)
     kernel_ms = acc / reps
     counts = rast.debug(_abi.DEBUG_COUNTS)
     n, visible, pairs, tiles = (int(v) for v in counts)
-    passes = (24 + max(1, int(np.ceil(np.log2(max(tiles, 2))))) + 7) // 8
+    passes = timed_stats.sort_passes
     sbytes = stage_bytes(n, visible, pairs, tiles, cfg.width * cfg.height, cfg.scene.sh_words, passes)
     stages = {}
     for k, name in enumerate(STAGES):
@@ -635,7 +635,7 @@ def main():
         "data": "synthetic: seeded 3M-splat clustered scene, packed fp16 shape/harmonics",
         "config": {"workload": "scene-fwd-3M-1080p", "splats": n, "visible": visible, "pairs": pairs,
                    "width": cfg.width, "height": cfg.height, "tiles": tiles, "sh_degree": 3, "filter3d": True,
-                   "dilation": cfg.dilation, "sort_passes": passes},
+                   "dilation": cfg.dilation, "sort_passes": passes, "key_bits": timed_stats.key_bits},
         "stages": stages,
         "gpu_ms_per_frame": round(float(kernel_ms[7]), 4),
         "blend_ms_by_setting": variants,

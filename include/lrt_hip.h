@@ -159,10 +159,15 @@ typedef struct {
  * 64-bit (tile, depth) sort replaces the reference's depth sort + tile sort
  * and is reported as tile_sort_ms. total_ms is the call's wall clock, as the
  * reference's; the stage figures (with time_stages) are GPU time from HIP
- * events, counts_ms including the pair-count readback's wait. */
+ * events, counts_ms including the pair-count readback's wait. key_bits and
+ * sort_passes (no counterpart in the reference) describe the frame's sort: its
+ * virtual key is the tile id over the depth key (bits(view z) >> 7) less the
+ * frame's least drawn one, in bitsFor(tiles) + bitsFor(depth-key range) bits,
+ * eight a pass (a batch of views keeps all 24 depth bits). */
 typedef struct {
     uint32_t splats, visible, pairs;
     double project_ms, depth_sort_ms, counts_ms, emit_ms, tile_sort_ms, blend_ms, total_ms;
+    uint32_t key_bits, sort_passes;
 } lrt_frame_stats;
 
 void lrt_render_settings_default(lrt_render_settings* out);

@@ -115,7 +115,7 @@ class FrameStatsC(C.Structure):
         ("splats", C.c_uint32), ("visible", C.c_uint32), ("pairs", C.c_uint32),
         ("project_ms", C.c_double), ("depth_sort_ms", C.c_double), ("counts_ms", C.c_double),
         ("emit_ms", C.c_double), ("tile_sort_ms", C.c_double), ("blend_ms", C.c_double),
-        ("total_ms", C.c_double),
+        ("total_ms", C.c_double), ("key_bits", C.c_uint32), ("sort_passes", C.c_uint32),
     ]
 
 

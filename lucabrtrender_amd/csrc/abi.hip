@@ -152,6 +152,8 @@ void toStats(const lrt::FrameStats& fs, lrt_frame_stats* stats) {
     stats->tile_sort_ms = fs.tileSortMs;
     stats->blend_ms = fs.blendMs;
     stats->total_ms = fs.totalMs;
+    stats->key_bits = fs.keyBits;
+    stats->sort_passes = fs.sortPasses;
 }
 
 lrt_splat_encoding toEncodingC(const lrt::io::SplatEncoding& e) {
@@ -396,6 +398,8 @@ lrt_status renderFrame(lrt_context* ctx, const lrt_projection* projection, const
         stats->tile_sort_ms = fs.tileSortMs;
         stats->blend_ms = fs.blendMs;
         stats->total_ms = fs.totalMs;
+        stats->key_bits = fs.keyBits;
+        stats->sort_passes = fs.sortPasses;
     }
     return finish(ctx, s);
 }
@@ -512,6 +516,8 @@ lrt_status lrt_raster_render_diff(lrt_context* ctx, const lrt_projection* projec
         stats->tile_sort_ms = fs.tileSortMs;
         stats->blend_ms = fs.blendMs;
         stats->total_ms = fs.totalMs;
+        stats->key_bits = fs.keyBits;
+        stats->sort_passes = fs.sortPasses;
     }
     return finish(ctx, s);
 }

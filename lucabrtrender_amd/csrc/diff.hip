@@ -72,9 +72,12 @@ __device__ __forceinline__ Decoded decode(const DiffCloud& c, uint32_t i, bool u
     return d;
 }
 
+static_assert(kBlock == kProjectBlock, "one depth-range entry a workgroup of kProjectBlock slots");
+
 __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffCloud c, float4* __restrict__ proj,
                                                             uint2* __restrict__ rects,
-                                                            uint32_t* __restrict__ touchedOut) {
+                                                            uint32_t* __restrict__ touchedOut, DepthBlocks depth) {
+    __shared__ uint32_t sDepth[2 * (kBlock / 64)];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < f.count;   // every lane stays for the wave's tile count
     const uint32_t slot = f.base + i;
@@ -94,7 +97,13 @@ __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffC
     };
     const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
     uint2 rect = emptyRect;
-    const uint32_t touched = projectWave(f, d.shape, live, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect);
+    float tz = 0.0f;
+    const uint32_t touched = projectWave(f, d.shape, live, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect, &tz);
+    if (depth.entries != nullptr) {
+        waveDepthRange(touched > 0, tz, sDepth);
+        __syncthreads();
+        storeDepthRange(depth, sDepth);
+    }
     if (!live) return;
     rects[slot] = touched > 0 ? rect : emptyRect;
     touchedOut[slot] = touched;
@@ -658,9 +667,9 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
 }   // namespace
 
 void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uint2* rects, uint32_t* touched,
-                 hipStream_t stream) {
+                 DepthBlocks depth, hipStream_t stream) {
     if (f.count == 0) return;
-    projectDiffKernel<<<(f.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(f, cloud, proj, rects, touched);
+    projectDiffKernel<<<projectBlocks(f.count), kBlock, 0, stream>>>(f, cloud, proj, rects, touched, depth);
 }
 
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,

@@ -53,14 +53,18 @@ __device__ __forceinline__ void emitPair(uint32_t at, uint64_t key, uint32_t s, 
 // plus its rank among this round's touched tiles of the same slot -- the place
 // the one-lane row-major walk gives it. Every lane of the wave must take part
 // (a lane with nothing to write passes expected 0). PER_VIEW: the frame sizes
-// differ across lanes and travel with each candidate.
+// differ across lanes and travel with each candidate. With depthRange
+// non-null, a slot whose depth key (bits(z) >> 7) lies outside
+// [depthRange[0], depthRange[1]] is counted in *outOfRange.
 template <bool EMIT_INDEX, bool PER_VIEW>
 __device__ __forceinline__ void duplicateWave(uint32_t s, uint32_t expected, uint32_t width, uint32_t height,
                                               uint32_t tilesX, uint32_t tileBase, const float4* __restrict__ proj,
                                               const uint2* __restrict__ rects, const uint32_t* __restrict__ offsets,
                                               uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
                                               uint32_t* __restrict__ pairSlot, uint32_t capacity,
-                                              uint32_t* __restrict__ overflow) {
+                                              uint32_t* __restrict__ overflow,
+                                              const uint32_t* __restrict__ depthRange,
+                                              uint32_t* __restrict__ outOfRange) {
     const uint32_t lane = __lane_id();
     uint32_t first = 0, x0 = 0, y0 = 0, w = 1, area = 0;
     float4 screen = make_float4(0.0f, 0.0f, 0.0f, 0.0f), conic = screen;
@@ -74,6 +78,10 @@ __device__ __forceinline__ void duplicateWave(uint32_t s, uint32_t expected, uin
         area = w * ((rc.y >> 16) - y0 + 1u);
     }
     const uint32_t depthBits = floatToBits(screen.z);
+    if (depthRange != nullptr && expected != 0) {
+        const uint32_t depthKey = (depthBits & 0x7FFFFFFFu) >> 7;
+        if (depthKey < depthRange[0] || depthKey > depthRange[1]) atomicAdd(outOfRange, 1u);
+    }
     const WaveTiles wt = waveTiles(area);
     const bool exact = wt.total >= (1u << 20);
     const uint64_t below = (1ull << lane) - 1ull;
@@ -127,11 +135,13 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
                                                           uint64_t* __restrict__ keys,
                                                           uint32_t* __restrict__ values,
                                                           uint32_t* __restrict__ pairSlot, uint32_t capacity,
-                                                          uint32_t* __restrict__ overflow) {
+                                                          uint32_t* __restrict__ overflow,
+                                                          const uint32_t* __restrict__ depthRange,
+                                                          uint32_t* __restrict__ outOfRange) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t expected = s < slots ? touched[s] : 0u;   // every lane stays for the wave's walk
     duplicateWave<EMIT_INDEX, false>(s, expected, f.width, f.height, f.tilesX, 0u, proj, rects, offsets, keys,
-                                     values, pairSlot, capacity, overflow);
+                                     values, pairSlot, capacity, overflow, depthRange, outOfRange);
 }
 
 // A batch of views: slot s belongs to view s / splatsPerView.
@@ -155,22 +165,23 @@ __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* _
         tileBase = v.tileBase;
     }
     duplicateWave<false, true>(s, expected, width, height, tilesX, tileBase, proj, rects, offsets, keys, values,
-                               nullptr, capacity, overflow);
+                               nullptr, capacity, overflow, nullptr, nullptr);
 }
 
 }   // namespace
 
 void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
                const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
-               uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream) {
+               uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream,
+               const uint32_t* depthRange, uint32_t* outOfRange) {
     if (slots == 0) return;
     const uint32_t blocks = (slots + kBlock - 1) / kBlock;
     if (pairSlot != nullptr) {
         duplicateKernel<true><<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values,
-                                                             pairSlot, capacity, overflow);
+                                                             pairSlot, capacity, overflow, depthRange, outOfRange);
     } else {
         duplicateKernel<false><<<blocks, kBlock, 0, stream>>>(f, slots, proj, rects, touched, offsets, keys, values,
-                                                              nullptr, capacity, overflow);
+                                                              nullptr, capacity, overflow, depthRange, outOfRange);
     }
 }
 

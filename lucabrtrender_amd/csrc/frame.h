@@ -74,4 +74,19 @@ struct ProjRecord {
     float colour[4];
 };
 
+// ---- depth-key ranges ------------------------------------------------------
+// The depth keys (bits(view z) >> 7) of a frame's drawn slots, folded per
+// projecting workgroup of kProjectBlock slots: words 2e and 2e + 1 of
+// `entries` are the least and greatest over workgroup e's drawn slots
+// (0xFFFFFFFF and 0 for none), e = first + the workgroup's index. Every
+// workgroup of a launch writes its entry, so a frame's launches fill entries
+// [0, their workgroups) and the scan reduces them to the frame's range. A null
+// `entries` writes nothing.
+constexpr uint32_t kProjectBlock = 256;
+struct DepthBlocks {
+    uint32_t* entries;
+    uint32_t first;
+};
+inline uint32_t projectBlocks(uint32_t count) { return (count + kProjectBlock - 1) / kProjectBlock; }
+
 }   // namespace lrt

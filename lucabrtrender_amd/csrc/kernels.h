@@ -19,18 +19,24 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
 // ---- scan ----------------------------------------------------------------
 // Exclusive prefix sum of `n` u32 into `out` (may not alias `in`), the total
 // into *total (saturating at 2^32 - 1) and the number of non-zero inputs into
-// *nonzero. `status` holds scanStatusWords(n) u64 words, cleared here.
+// *nonzero. `status` holds scanStatusWords(n) u64 words, cleared here. With
+// depthRange non-null, the depth-key entries [0, depthEntries) of the frame's
+// projecting workgroups (DepthBlocks) are reduced into depthRange[0] (least)
+// and depthRange[1] (greatest).
 uint32_t scanPartitions(uint32_t n);
 inline uint64_t scanStatusWords(uint32_t n) { return uint64_t{scanPartitions(n)} * 3 / 2 + 2; }
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
-                   uint32_t* nonzero, uint32_t* error, hipStream_t stream);
+                   uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks = nullptr,
+                   uint32_t depthEntries = 0, uint32_t* depthRange = nullptr);
 
 // ---- projection -----------------------------------------------------------
 // One instance: f.count splats from index 0 of the cloud arrays, written to
 // slots f.base + i of proj (3 float4 a slot), rects (uint2 a slot: x0 | y0 << 16,
-// x1 | y1 << 16) and touched (u32 a slot).
+// x1 | y1 << 16) and touched (u32 a slot); each of its projectBlocks(f.count)
+// workgroups' depth-key range into `depth`.
 void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
-             const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream);
+             const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, DepthBlocks depth,
+             hipStream_t stream);
 // The same instance (count splats) for `views` views at once: view v's
 // constants are params[v] (device), its slots start at params[v].base.
 void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,
@@ -39,18 +45,22 @@ void projectViews(const FrameParams* params, uint32_t views, uint32_t count, con
 // One point instance as opaque discs: f.count points (positions float4 xyz 1;
 // colours two words a point, f16 r | g, b | a, 8-byte aligned, read only
 // without style.constantColour) into slots f.base + i, in the same record,
-// rectangle and count layout as a splat's, colour.w = 1.
+// rectangle and count layout as a splat's, colour.w = 1; depth-key ranges into
+// `depth` as project's are.
 void projectPoints(const FrameParams& f, const PointParams& style, const float4* positions, const uint32_t* colours,
-                   float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream);
+                   float4* proj, uint2* rects, uint32_t* touched, DepthBlocks depth, hipStream_t stream);
 
 // ---- duplication ------------------------------------------------------------
 // For each of `slots` slots with touched > 0: its pairs at offsets[slot].., keys
 // (tile << 32 | bits(view z)) and values (slot). With pairSlot non-null the
 // value is instead the pair's own index and pairSlot[index] its slot. Writes
-// past `capacity` are dropped and counted in *overflow.
+// past `capacity` are dropped and counted in *overflow. With depthRange
+// non-null, a slot whose depth key lies outside [depthRange[0], depthRange[1]]
+// is counted in *outOfRange (the sort's key layout would not hold it).
 void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const uint2* rects,
                const uint32_t* touched, const uint32_t* offsets, uint64_t* keys, uint32_t* values,
-               uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream);
+               uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream,
+               const uint32_t* depthRange = nullptr, uint32_t* outOfRange = nullptr);
 // A batch of views (device ViewInfo array): slot s is view s / splatsPerView's,
 // its keys carry the view's tileBase.
 void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
@@ -61,8 +71,17 @@ void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slot
 // Which bits of a 64-bit key the digits come from.
 enum class KeyMode : uint32_t {
     Plain = 0,        // bits [bitLo, bitLo + keyBits)
-    TileDepth = 1,    // (key >> 32) << 24 | (key & 0x7FFFFFFF) >> 7: tile id over the
-                      // top 24 bits of a positive float depth; keyBits = tileBits + 24
+    TileDepth = 1,    // (key >> 32) << depthBits | ((key & 0x7FFFFFFF) >> 7) - depthMin:
+                      // tile id over the top 24 bits of a positive float depth, less
+                      // the frame's least; keyBits = tileBits + depthBits
+};
+
+// TileDepth's depth field: the frame's depth keys lie in [depthMin, depthMin +
+// 2^depthBits), so the virtual key orders as (tile, depth key) does in fewer
+// bits. The default is the whole 24-bit field.
+struct TileDepthLayout {
+    uint32_t depthMin = 0;
+    uint32_t depthBits = 24;
 };
 
 struct SortWorkspace {
@@ -85,7 +104,8 @@ SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32
 // the capacity that bounds it (the launch is sized for n; nothing waits on the host).
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
                KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
-               hipEvent_t afterHistogram = nullptr, const uint32_t* nDevice = nullptr);
+               hipEvent_t afterHistogram = nullptr, const uint32_t* nDevice = nullptr,
+               TileDepthLayout depth = {});
 
 // ---- load: raw splat records into packed splats --------------------------------
 // The decode's view of an io::SplatEncoding: field offsets in floats, modes
@@ -202,7 +222,7 @@ struct DiffGrads {
     float* sh;
 };
 void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uint2* rects, uint32_t* touched,
-                 hipStream_t stream);
+                 DepthBlocks depth, hipStream_t stream);
 // sortedSlots[i] = pairSlot[pairIndex[i]] for i < pairs (or < *nDevice, pairs
 // then being the capacity).
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,

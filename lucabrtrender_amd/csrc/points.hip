@@ -19,7 +19,7 @@ namespace {
 
 using namespace lrt::dev;
 
-constexpr uint32_t kBlock = 256;
+constexpr uint32_t kBlock = kProjectBlock;
 
 // The disc's radius in pixels, never below half a pixel: a point that projects
 // smaller still covers the pixel it lands in.
@@ -33,7 +33,8 @@ __global__ __launch_bounds__(kBlock) void projectPointsKernel(FrameParams f, Poi
                                                               const float4* __restrict__ positions,
                                                               const uint2* __restrict__ colours,
                                                               float4* __restrict__ proj, uint2* __restrict__ rects,
-                                                              uint32_t* __restrict__ touchedOut) {
+                                                              uint32_t* __restrict__ touchedOut, DepthBlocks depth) {
+    __shared__ uint32_t sDepth[2 * (kBlock / 64)];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     const bool live = i < f.count;   // every lane stays for the wave's tile count
     const uint32_t slot = f.base + i;
@@ -56,6 +57,11 @@ __global__ __launch_bounds__(kBlock) void projectPointsKernel(FrameParams f, Poi
         candidate = rect[0] <= rect[2] && rect[1] <= rect[3];
     }
     const uint32_t touched = countTouchedTilesWave(f.width, f.height, cx, cy, k, 0.0f, k, 1.0f, rect, candidate);
+    if (depth.entries != nullptr) {
+        waveDepthRange(touched > 0, tz, sDepth);
+        __syncthreads();
+        storeDepthRange(depth, sDepth);
+    }
     if (!live) return;
     if (touched == 0) {
         touchedOut[slot] = 0;
@@ -80,10 +86,10 @@ __global__ __launch_bounds__(kBlock) void projectPointsKernel(FrameParams f, Poi
 }   // namespace
 
 void projectPoints(const FrameParams& f, const PointParams& style, const float4* positions, const uint32_t* colours,
-                   float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream) {
+                   float4* proj, uint2* rects, uint32_t* touched, DepthBlocks depth, hipStream_t stream) {
     if (f.count == 0) return;
-    projectPointsKernel<<<(f.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        f, style, positions, reinterpret_cast<const uint2*>(colours), proj, rects, touched);
+    projectPointsKernel<<<projectBlocks(f.count), kBlock, 0, stream>>>(
+        f, style, positions, reinterpret_cast<const uint2*>(colours), proj, rects, touched, depth);
 }
 
 }   // namespace lrt::kern

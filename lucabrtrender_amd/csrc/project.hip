@@ -22,7 +22,7 @@ namespace {
 
 using namespace lrt::dev;
 
-constexpr uint32_t kBlock = 256;
+constexpr uint32_t kBlock = kProjectBlock;
 constexpr uint32_t kMaxShWords = 23;   // 45 halves at degree 3, rgb per basis
 
 // The coefficient words of the drawn splats, staged: row r of sSh holds the
@@ -60,10 +60,12 @@ __device__ __forceinline__ void stageDrawnRows(const uint32_t* __restrict__ sh, 
 __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4* __restrict__ positions,
                                              const uint4* __restrict__ shape, const uint32_t* __restrict__ sh,
                                              const float* __restrict__ filter3d, float4* __restrict__ proj,
-                                             uint2* __restrict__ rects, uint32_t* __restrict__ touchedOut) {
+                                             uint2* __restrict__ rects, uint32_t* __restrict__ touchedOut,
+                                             DepthBlocks depth) {
     __shared__ uint32_t sSh[kBlock * kMaxShWords];
     __shared__ uint32_t sRowOf[kBlock];
     __shared__ uint32_t sDrawn[kBlock / 64];
+    __shared__ uint32_t sDepth[2 * (kBlock / 64)];
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u, wave = tid >> 6;
     const uint32_t first = blockIdx.x * kBlock;
@@ -100,7 +102,9 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
     const bool drawn = touched > 0;
     const uint64_t drawnLanes = __ballot(drawn);
     if (lane == 0) sDrawn[wave] = static_cast<uint32_t>(__popcll(drawnLanes));
+    if (depth.entries != nullptr) waveDepthRange(drawn, fp.tz, sDepth);
     __syncthreads();
+    storeDepthRange(depth, sDepth);
     uint32_t at = static_cast<uint32_t>(__popcll(drawnLanes & ((1ull << lane) - 1ull)));
     uint32_t rows = 0;
 #pragma unroll
@@ -146,8 +150,8 @@ __global__ __launch_bounds__(kBlock) void projectKernel(FrameParams f, const flo
                                                         const uint32_t* __restrict__ sh,
                                                         const float* __restrict__ filter3d,
                                                         float4* __restrict__ proj, uint2* __restrict__ rects,
-                                                        uint32_t* __restrict__ touchedOut) {
-    projectBlock(f, positions, shape, sh, filter3d, proj, rects, touchedOut);
+                                                        uint32_t* __restrict__ touchedOut, DepthBlocks depth) {
+    projectBlock(f, positions, shape, sh, filter3d, proj, rects, touchedOut, depth);
 }
 
 constexpr uint32_t kViewsStaged = 32;   // view constants held in LDS at a time
@@ -236,8 +240,9 @@ __global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* 
             const FrameParams& f = sF[v];
             const uint32_t slot = f.base + i;
             uint2 rect = emptyRect;
+            float tz;
             const uint32_t touched = projectWave(f, shapeOf, live && !fate.removed, colourOf,
-                                                 proj + static_cast<uint64_t>(slot) * 3, &rect);
+                                                 proj + static_cast<uint64_t>(slot) * 3, &rect, &tz);
             if (!live) continue;
             rects[slot] = touched > 0 ? rect : emptyRect;
             touchedOut[slot] = touched;
@@ -248,10 +253,11 @@ __global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* 
 }   // namespace
 
 void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
-             const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, hipStream_t stream) {
+             const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, DepthBlocks depth,
+             hipStream_t stream) {
     if (f.count == 0) return;
-    const uint32_t blocks = (f.count + kBlock - 1) / kBlock;
-    projectKernel<<<blocks, kBlock, 0, stream>>>(f, positions, shape, sh, filter3d, proj, rects, touched);
+    const uint32_t blocks = projectBlocks(f.count);
+    projectKernel<<<blocks, kBlock, 0, stream>>>(f, positions, shape, sh, filter3d, proj, rects, touched, depth);
 }
 
 void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,

@@ -154,16 +154,18 @@ __device__ __forceinline__ void writeProjection(const Footprint& fp, const float
 // Projects one splat a lane, `want` false for a lane with none; returns the
 // tiles its footprint touches (0: culled, and then the record is left alone).
 // `colourOf(dx, dy, dz, rgb)` gives the colour for the unit direction from the
-// eye, called only for a splat that is drawn. Every lane of the wave must
-// reach the call: the tiles are counted wave-wide (countTouchedTilesWave).
+// eye, called only for a splat that is drawn; *tz is its view depth. Every
+// lane of the wave must reach the call: the tiles are counted wave-wide
+// (countTouchedTilesWave).
 template <class ColourOf>
 __device__ __forceinline__ uint32_t projectWave(const FrameParams& f, const SplatShape& s, bool want,
                                                 ColourOf colourOf, float4* __restrict__ out,
-                                                uint2* __restrict__ rectOut) {
+                                                uint2* __restrict__ rectOut, float* tz) {
     Footprint fp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {1u, 1u, 0u, 0u}};
     const bool candidate = want && footprintOf(f, s, fp);
     const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
                                                    fp.rect, candidate);
+    *tz = fp.tz;
     if (touched == 0) return 0;
     float dx, dy, dz;
     viewDirection(f, s, dx, dy, dz);

@@ -122,6 +122,8 @@ struct RenderTargets {   // caller-owned device images, bottom row first
 
 struct FrameStats {   // render::FrameStats (TileRasterizer.h:249-255)
     uint32_t splats = 0, visible = 0, pairs = 0;
+    // The sort's key: tile bits plus the bits of the frame's depth-key range, and its passes.
+    uint32_t keyBits = 0, sortPasses = 0;
     double projectMs = 0, depthSortMs = 0, countsMs = 0, emitMs = 0, tileSortMs = 0, blendMs = 0, totalMs = 0;
 };
 
@@ -260,6 +262,9 @@ private:
     // Per slot.
     uint32_t splatCapacity_ = 0;
     DeviceBuffer proj_, rects_, touched_, offsets_, scanStatus_;
+    // The projecting workgroups' depth-key ranges (DepthBlocks), two words each.
+    DeviceBuffer depthBlocks_;
+    uint32_t depthEntries_ = 0;   // the entries the frame's projections wrote
     // Per pair, ping-pong.
     uint32_t pairCapacity_ = 0;
     DeviceBuffer keys_[2], values_[2], sortWork_;
@@ -276,7 +281,9 @@ private:
     DeviceBuffer ranges_, tileOrder_, lastCount_, finalT_, accum_;
     // Split blends: the plan's tasks and split tiles, and the segments' partial walks.
     DeviceBuffer planTasks_, planCombine_, partials_;
-    // Counters: [0] pairs, [1] visible, [2] overflow, [3] unused, [4] sort error, [5] scan error.
+    // Counters: [0] pairs, [1] visible, [2] overflow, [3] unused, [4] sort error, [5] scan error,
+    // [6..7] the split blend's plan counts (the views' queue), [8] least and [9] greatest depth key
+    // of the frame's drawn slots, [10] drawn slots outside that range.
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
     hipEvent_t events_[9] = {};

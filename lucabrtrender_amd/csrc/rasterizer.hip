@@ -225,6 +225,17 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     if (!(s = reserveSplats(std::max(splatCount, 1u))).ok()) return s;
     if (!(s = reserveTargets(settings.width, settings.height, tiles, false)).ok()) return s;
 
+    // One depth-key range entry for each projecting workgroup of the frame.
+    uint64_t entries = 0;
+    for (const SplatInstance& instance : instances) {
+        if (instance.splats != nullptr) entries += projectBlocks(instance.splats->count);
+    }
+    for (const PointInstance& instance : points) {
+        if (instance.points != nullptr) entries += projectBlocks(instance.points->count);
+    }
+    if (!(s = depthBlocks_.reserve(std::max<uint64_t>(entries, 1) * 8, "frame.depthBlocks")).ok()) return s;
+    DepthBlocks depth{depthBlocks_.as<uint32_t>(), 0};
+
     if (settings.timeStages) (void)hipEventRecord(events_[0], stream_);
     // Project every instance into its own slots.
     uint32_t base = 0;
@@ -239,7 +250,8 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         }
         if (cloud == nullptr || cloud->count == 0) continue;
         kern::project(f, cloud->positions, cloud->shape, cloud->sh, cloud->filter3d, proj_.as<float4>(),
-                      rects_.as<uint2>(), touched_.as<uint32_t>(), stream_);
+                      rects_.as<uint2>(), touched_.as<uint32_t>(), depth, stream_);
+        depth.first += projectBlocks(cloud->count);
         base += cloud->count;
     }
     // Then the points, as discs, in the slots after the splats'.
@@ -256,9 +268,11 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         style.constantColour = instance.style.constantColour ? 1u : 0u;
         for (int k = 0; k < 3; ++k) style.colour[k] = instance.style.colour[static_cast<size_t>(k)];
         kern::projectPoints(f, style, cloud->positions, cloud->colours, proj_.as<float4>(), rects_.as<uint2>(),
-                            touched_.as<uint32_t>(), stream_);
+                            touched_.as<uint32_t>(), depth, stream_);
+        depth.first += projectBlocks(cloud->count);
         base += cloud->count;
     }
+    depthEntries_ = depth.first;
     if (!haveFirst) {
         SplatInstance none;
         first = frameParamsFor(projection, none, settings, 0);
@@ -296,8 +310,12 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
     instance.splats = &shapeOnly;
     instance.objectToWorld = objectToWorld;
     const FrameParams f = frameParamsFor(projection, instance, settings, 0);
+    if (!(s = depthBlocks_.reserve(std::max<uint64_t>(projectBlocks(cloud.count), 1) * 8, "frame.depthBlocks")).ok())
+        return s;
     if (settings.timeStages) (void)hipEventRecord(events_[0], stream_);
-    kern::projectDiff(f, cloud, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(), stream_);
+    kern::projectDiff(f, cloud, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
+                      DepthBlocks{depthBlocks_.as<uint32_t>(), 0}, stream_);
+    depthEntries_ = projectBlocks(cloud.count);
     diffReady_ = false;
     if (!(s = finishFrame(f, cloud.count, settings, targets, statsOut, true)).ok()) return s;
     diffCloud_ = cloud;
@@ -311,7 +329,6 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
                                    const RenderTargets* under) {
     const auto wallStart = std::chrono::steady_clock::now();
     const uint32_t tiles = first.tilesX * first.tilesY;
-    const uint32_t keyBits = 24 + bitsFor(tiles);
     uint32_t* counters = counters_.as<uint32_t>();
     const bool timed = settings.timeStages;
     FrameStats stats;
@@ -324,12 +341,14 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     mark(1);
     Status s;
 
-    // How many pairs and how many visible: one scan, one readback.
-    kern::fill32(counters, 0u, 8, stream_);
+    // How many pairs, how many visible and the drawn depth keys' range: one
+    // scan, one readback.
+    kern::fill32(counters, 0u, 12, stream_);
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), splatCount, scanStatus_.as<uint64_t>(),
-                        counters + 0, counters + 1, counters + 5, stream_);
+                        counters + 0, counters + 1, counters + 5, stream_, depthBlocks_.as<uint32_t>(), depthEntries_,
+                        counters + 8);
     mark(2);
-    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "counts")).ok())
+    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream_), "counts")).ok())
         return s;
     if (!(s = check(hipStreamSynchronize(stream_), "counts")).ok()) return s;
     const auto countsDone = std::chrono::steady_clock::now();
@@ -341,20 +360,33 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     }
     stats.visible = visible;
     stats.pairs = pairs;
+    // The sort's key: the tile over the depth key less the frame's least, in
+    // as many bits as the drawn keys span (the same order as tile << 24 | key).
+    kern::TileDepthLayout layout;
+    if (pairs > 0) {
+        const uint32_t lo = hostCounters_[8], hi = hostCounters_[9];
+        if (lo > hi) return Status::error(ErrorCode::InternalError, "pairs without a drawn depth key");
+        layout.depthMin = lo;
+        layout.depthBits = bitsFor(hi - lo + 1u);
+    }
+    const uint32_t keyBits = bitsFor(tiles) + layout.depthBits;
+    stats.keyBits = keyBits;
+    stats.sortPasses = kern::sortPasses(keyBits);
     if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) return s;
 
     // Duplicate into 64-bit keys, in slot order.
     mark(3);
     kern::duplicate(first, splatCount, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
                     offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(),
-                    training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_, counters + 2, stream_);
+                    training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_, counters + 2, stream_,
+                    counters + 8, counters + 10);
     mark(4);
 
     // One stable sort on (tile, depth >> 7), then the ranges.
     const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairs, kern::KeyMode::TileDepth, 0, keyBits, ws,
-                                       stream_, timed ? events_[5] : nullptr);
+                                       stream_, timed ? events_[5] : nullptr, nullptr, layout);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_);
@@ -408,11 +440,14 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
                     training ? accum_.as<float4>() : nullptr, stream_);
     }
     mark(8);
-    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "frame")).ok())
+    if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream_), "frame")).ok())
         return s;
     if (!(s = check(hipStreamSynchronize(stream_), "frame")).ok()) return s;
     if (!(s = check(hipGetLastError(), "frame")).ok()) return s;
     if (hostCounters_[2] != 0) return Status::error(ErrorCode::InternalError, "the duplication overran its pairs");
+    if (hostCounters_[10] != 0) {
+        return Status::error(ErrorCode::InternalError, "a drawn splat's depth key fell outside the frame's range");
+    }
     if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
     const auto wallEnd = std::chrono::steady_clock::now();
 
@@ -699,6 +734,8 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         stats.splats = static_cast<uint32_t>(slots);
         stats.visible = lastVisible_;
         stats.pairs = lastPairs_;
+        stats.keyBits = 24 + bitsFor(static_cast<uint32_t>(tiles));   // the batch's sort keeps every depth bit
+        stats.sortPasses = kern::sortPasses(stats.keyBits);
         stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
         *statsOut = stats;
     }

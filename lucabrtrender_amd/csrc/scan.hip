@@ -166,20 +166,47 @@ __global__ __launch_bounds__(kBlock) void scanKernel(const uint32_t* __restrict_
     }
 }
 
-// One workgroup: the partitions' non-zero counts added up.
+// One workgroup: the partitions' non-zero counts added up; with depthRange
+// non-null, the depth-key entries (two words each: least, greatest) folded.
 __global__ __launch_bounds__(1024) void sumPartsKernel(const uint32_t* __restrict__ parts, uint32_t count,
-                                                       uint32_t* __restrict__ out) {
-    __shared__ uint32_t sWave[1024 / 64];
+                                                       uint32_t* __restrict__ out,
+                                                       const uint2* __restrict__ depthBlocks, uint32_t depthEntries,
+                                                       uint32_t* __restrict__ depthRange) {
+    __shared__ uint32_t sWave[3][1024 / 64];
     uint32_t sum = 0;
     for (uint32_t i = threadIdx.x; i < count; i += 1024) sum += parts[i];
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+    if (depthRange != nullptr) {
+        for (uint32_t e = threadIdx.x; e < depthEntries; e += 1024) {
+            const uint2 r = depthBlocks[e];
+            lo = min(lo, r.x);
+            hi = max(hi, r.y);
+        }
+    }
 #pragma unroll
-    for (uint32_t d = 1; d < 64; d <<= 1) sum += static_cast<uint32_t>(__shfl_xor(static_cast<int>(sum), static_cast<int>(d), 64));
-    if ((threadIdx.x & 63u) == 0) sWave[threadIdx.x >> 6] = sum;
+    for (uint32_t d = 1; d < 64; d <<= 1) {
+        sum += static_cast<uint32_t>(__shfl_xor(static_cast<int>(sum), static_cast<int>(d), 64));
+        lo = min(lo, static_cast<uint32_t>(__shfl_xor(static_cast<int>(lo), static_cast<int>(d), 64)));
+        hi = max(hi, static_cast<uint32_t>(__shfl_xor(static_cast<int>(hi), static_cast<int>(d), 64)));
+    }
+    if ((threadIdx.x & 63u) == 0) {
+        sWave[0][threadIdx.x >> 6] = sum;
+        sWave[1][threadIdx.x >> 6] = lo;
+        sWave[2][threadIdx.x >> 6] = hi;
+    }
     __syncthreads();
     if (threadIdx.x == 0) {
-        uint32_t all = 0;
-        for (uint32_t w = 0; w < 1024 / 64; ++w) all += sWave[w];
+        uint32_t all = 0, allLo = 0xFFFFFFFFu, allHi = 0u;
+        for (uint32_t w = 0; w < 1024 / 64; ++w) {
+            all += sWave[0][w];
+            allLo = min(allLo, sWave[1][w]);
+            allHi = max(allHi, sWave[2][w]);
+        }
         *out = all;
+        if (depthRange != nullptr) {
+            depthRange[0] = allLo;
+            depthRange[1] = allHi;
+        }
     }
 }
 
@@ -195,12 +222,17 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream) {
 uint32_t scanPartitions(uint32_t n) { return (n + kPart - 1) / kPart; }
 
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
-                   uint32_t* nonzero, uint32_t* error, hipStream_t stream) {
+                   uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks,
+                   uint32_t depthEntries, uint32_t* depthRange) {
     const uint32_t parts = scanPartitions(n);
     fill32(reinterpret_cast<uint32_t*>(status), 0u, static_cast<uint64_t>(parts) * 2, stream);
     if (n == 0) {
         fill32(total, 0u, 1, stream);
         fill32(nonzero, 0u, 1, stream);
+        if (depthRange != nullptr) {
+            fill32(depthRange, 0xFFFFFFFFu, 1, stream);
+            fill32(depthRange + 1, 0u, 1, stream);
+        }
         return;
     }
     // Each partition's non-zero count after its look-back words.
@@ -208,7 +240,8 @@ void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* stat
     // 16-byte loads and stores only where both arrays allow them.
     const bool vectorised = (reinterpret_cast<uintptr_t>(in) % 16 == 0) && (reinterpret_cast<uintptr_t>(out) % 16 == 0);
     scanKernel<<<parts, kBlock, 0, stream>>>(in, out, n, status, total, nonzeroParts, error, vectorised);
-    sumPartsKernel<<<1, 1024, 0, stream>>>(nonzeroParts, parts, nonzero);
+    sumPartsKernel<<<1, 1024, 0, stream>>>(nonzeroParts, parts, nonzero,
+                                           reinterpret_cast<const uint2*>(depthBlocks), depthEntries, depthRange);
 }
 
 }   // namespace lrt::kern

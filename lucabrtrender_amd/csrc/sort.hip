@@ -58,10 +58,17 @@ __device__ __forceinline__ uint32_t pairCount(uint32_t n, const uint32_t* __rest
     return d < n ? d : n;
 }
 
+// TileDepth's virtual key: the tile id over the depth key less the frame's
+// least (a key outside the layout's range is the duplication's error, never sorted).
+__device__ __forceinline__ uint64_t tileDepthKey(uint64_t key, uint32_t depthMin, uint32_t depthBits) {
+    const uint32_t depth = ((static_cast<uint32_t>(key) & 0x7FFFFFFFu) >> 7) - depthMin;
+    return ((key >> 32) << depthBits) | depth;
+}
+
 template <KeyMode MODE>
-__device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64_t mask) {
+__device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64_t mask, TileDepthLayout depth) {
     if constexpr (MODE == KeyMode::TileDepth) {
-        return (((key >> 32) << 24) | ((static_cast<uint32_t>(key) & 0x7FFFFFFFu) >> 7)) & mask;
+        return tileDepthKey(key, depth.depthMin, depth.depthBits) & mask;
     } else {
         return (key >> bitLo) & mask;
     }
@@ -76,7 +83,7 @@ __device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64
 template <KeyMode MODE>
 __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
                                                               const uint32_t* __restrict__ nDevice, uint32_t bitLo,
-                                                              uint64_t mask, uint32_t passes,
+                                                              uint64_t mask, TileDepthLayout depth, uint32_t passes,
                                                               uint32_t* __restrict__ hist) {
     __shared__ uint32_t sHist[kMaxPasses * kRadix];
     const uint32_t n = pairCount(cap, nDevice);
@@ -92,7 +99,7 @@ __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
             const uint32_t i = w0 + u * stride + lane;   // n < 2^30: no wrap
-            k[u] = i < n ? sortKey<MODE>(keys[i], bitLo, mask) : 0ull;
+            k[u] = i < n ? sortKey<MODE>(keys[i], bitLo, mask, depth) : 0ull;
         }
 #pragma unroll
         for (uint32_t u = 0; u < 4; ++u) {
@@ -179,33 +186,28 @@ __global__ __launch_bounds__(1024) void digitOffsetsKernel(const uint32_t* __res
     }
 }
 
-// Where one pass's 8-bit digit sits in a 64-bit key. TileDepth: the digit of
-// the virtual key at bit `shift` is the low word's bits [shift + 7, +8) below
-// bit 24 of the virtual key (the depth's top 24 bits) and the high word's
-// bits [shift - 24, +8) above it (the tile id). Plain: bits [bitLo + shift, +8).
+// Where one pass's 8-bit digit sits. TileDepth: bits [shift, +8) of the
+// virtual key (tileDepthKey). Plain: bits [bitLo + shift, +8) of the key.
 // `mask` keeps the digit's bits below keyBits.
 struct PassDigit {
-    uint32_t hiWord;   // TileDepth: the digit is in the high word
-    uint32_t shift;    // TileDepth: within the chosen word; Plain: within the key
+    uint32_t shift;
     uint32_t mask;
+    uint32_t depthMin, depthBits;   // TileDepth's layout
 };
 
 template <KeyMode MODE>
 __device__ __forceinline__ uint32_t digitOf(uint64_t key, const PassDigit& pd) {
     if constexpr (MODE == KeyMode::TileDepth) {
-        const uint32_t w = pd.hiWord != 0 ? static_cast<uint32_t>(key >> 32) : static_cast<uint32_t>(key);
-        return (w >> pd.shift) & pd.mask;
+        return static_cast<uint32_t>(tileDepthKey(key, pd.depthMin, pd.depthBits) >> pd.shift) & pd.mask;
     } else {
         return static_cast<uint32_t>(key >> pd.shift) & pd.mask;
     }
 }
 
-PassDigit passDigit(KeyMode mode, uint32_t bitLo, uint64_t mask, uint32_t shift) {
+PassDigit passDigit(KeyMode mode, uint32_t bitLo, uint64_t mask, uint32_t shift, TileDepthLayout depth) {
     const uint32_t m = static_cast<uint32_t>((mask >> shift) & 255u);
-    if (mode == KeyMode::TileDepth) {
-        return shift >= 24 ? PassDigit{1u, shift - 24, m} : PassDigit{0u, shift + 7, m};
-    }
-    return PassDigit{0u, bitLo + shift, m};
+    if (mode == KeyMode::TileDepth) return PassDigit{shift, m, depth.depthMin, depth.depthBits};
+    return PassDigit{bitLo + shift, m, 0u, 0u};
 }
 
 template <KeyMode MODE>
@@ -401,8 +403,8 @@ uint32_t histogramRows(uint32_t n) {
 
 template <KeyMode MODE>
 void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
-                  const uint32_t* nDevice, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws,
-                  hipStream_t stream, hipEvent_t afterHistogram) {
+                  const uint32_t* nDevice, uint32_t bitLo, uint32_t keyBits, TileDepthLayout depth,
+                  const SortWorkspace& ws, hipStream_t stream, hipEvent_t afterHistogram) {
     const uint32_t passes = sortPasses(keyBits);
     const uint32_t parts = sortPartitions(n);
     const uint64_t mask = keyBits >= 64 ? ~0ull : ((1ull << keyBits) - 1ull);
@@ -414,7 +416,8 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
         clearStatusKernel<<<grid, kBlock, 0, stream>>>(ws.status, passes, parts, n, nDevice);
     }
     const uint32_t rows = histogramRows(n);
-    histogramKernel<MODE><<<rows, kHistBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, passes, ws.histogram);
+    histogramKernel<MODE><<<rows, kHistBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, depth, passes,
+                                                          ws.histogram);
     digitOffsetsKernel<<<passes, 1024, 0, stream>>>(ws.histogram, rows, passes, ws.offsets);
     if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
     const uint64_t* kin = keys;
@@ -423,7 +426,7 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     uint32_t* vout = valuesAlt;
     for (uint32_t p = 0; p < passes; ++p) {
         onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(kin, vin, kout, vout, n, nDevice,
-                                                           passDigit(MODE, bitLo, mask, 8 * p),
+                                                           passDigit(MODE, bitLo, mask, 8 * p, depth),
                                                            ws.offsets + p * kRadix,
                                                            ws.status + static_cast<uint64_t>(p) * parts * kRadix,
                                                            ws.error);
@@ -453,16 +456,16 @@ SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32
 
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n, KeyMode mode,
                uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
-               hipEvent_t afterHistogram, const uint32_t* nDevice) {
+               hipEvent_t afterHistogram, const uint32_t* nDevice, TileDepthLayout depth) {
     if ((nDevice == nullptr && n < 2) || n == 0 || keyBits == 0) {
         if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
         return false;
     }
     if (mode == KeyMode::TileDepth) {
-        launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, ws, stream,
-                                         afterHistogram);
+        launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, depth, ws,
+                                         stream, afterHistogram);
     } else {
-        launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, ws, stream,
+        launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, depth, ws, stream,
                                      afterHistogram);
     }
     return (sortPasses(keyBits) & 1u) != 0u;

@@ -205,6 +205,45 @@ __device__ __forceinline__ uint32_t shflU(uint32_t v, uint32_t lane) {
     return static_cast<uint32_t>(__shfl(static_cast<int>(v), static_cast<int>(lane), 64));
 }
 
+__device__ __forceinline__ uint32_t waveMin(uint32_t v) {
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) v = min(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), d, 64)));
+    return v;
+}
+__device__ __forceinline__ uint32_t waveMax(uint32_t v) {
+#pragma unroll
+    for (uint32_t d = 1; d < 64; d <<= 1) v = max(v, static_cast<uint32_t>(__shfl_xor(static_cast<int>(v), d, 64)));
+    return v;
+}
+
+// The wave's part of its workgroup's depth-key range: the least and greatest
+// depth key (bits(tz) >> 7) of its drawn lanes, written by lane 0 to
+// sWaves[2 w] and sWaves[2 w + 1] (w the wave in the workgroup). Every lane
+// of the wave must take part.
+__device__ __forceinline__ void waveDepthRange(bool drawn, float tz, uint32_t* sWaves) {
+    const uint32_t key = (floatToBits(tz) & 0x7FFFFFFFu) >> 7;
+    const uint32_t lo = waveMin(drawn ? key : 0xFFFFFFFFu), hi = waveMax(drawn ? key : 0u);
+    if (__lane_id() == 0) {
+        sWaves[2 * (threadIdx.x >> 6)] = lo;
+        sWaves[2 * (threadIdx.x >> 6) + 1] = hi;
+    }
+}
+
+// After a barrier that follows every wave's waveDepthRange: thread 0 writes
+// the workgroup's entry (kProjectBlock / 64 waves folded).
+__device__ __forceinline__ void storeDepthRange(const DepthBlocks& depth, const uint32_t* sWaves) {
+    if (depth.entries == nullptr || threadIdx.x != 0) return;
+    uint32_t lo = 0xFFFFFFFFu, hi = 0u;
+#pragma unroll
+    for (uint32_t w = 0; w < kProjectBlock / 64; ++w) {
+        lo = min(lo, sWaves[2 * w]);
+        hi = max(hi, sWaves[2 * w + 1]);
+    }
+    uint32_t* e = depth.entries + 2ull * (depth.first + blockIdx.x);
+    e[0] = lo;
+    e[1] = hi;
+}
+
 // The wave's candidate tiles, flattened: lane L's rectangle (area[L] tiles in
 // row-major order) is candidates [start[L], start[L] + area[L]) of one list,
 // start the exclusive sum of the areas over the lanes. An iteration tests 64

@@ -306,6 +306,8 @@ class FrameStats:
     tile_sort_ms: float = 0.0
     blend_ms: float = 0.0
     total_ms: float = 0.0
+    key_bits: int = 0       # the sort's virtual key: tile bits + the frame's depth-key range bits
+    sort_passes: int = 0    # eight key bits a pass
 
 
 class RenderTargets:

@@ -86,6 +86,23 @@ def assert_bit_exact(gpu, ora):
     np.testing.assert_array_equal((gpu["keys"] >> np.uint64(32)).astype(np.uint32), ora["pair_tiles"])
     np.testing.assert_array_equal(gpu["values"], ora["pair_splats"])
     np.testing.assert_array_equal(gpu["ranges"], ora["ranges"])
+    if gpu.get("stats") is not None and ora["pairs"] > 0:
+        assert gpu["stats"].key_bits == expected_key_bits(ora), (gpu["stats"].key_bits, expected_key_bits(ora))
+        assert gpu["stats"].sort_passes == (gpu["stats"].key_bits + 7) // 8
+
+
+def bits_for(values: int) -> int:
+    """The bits a key needs to tell `values` values apart (at least one)."""
+    return max(1, (int(values) - 1).bit_length())
+
+
+def expected_key_bits(ora) -> int:
+    """The frame's sort key width: the tile id over the drawn slots' depth keys
+    (bits(view z) >> 7) less their least, each field as wide as its range."""
+    vis = ora["touched"] > 0
+    depth_keys = (ora["proj"][vis][:, 2].view(np.uint32) & np.uint32(0x7FFFFFFF)) >> np.uint32(7)
+    span = int(depth_keys.max()) - int(depth_keys.min()) + 1
+    return bits_for(len(ora["ranges"])) + bits_for(span)
 
 
 def oracle_style(st: raster.PointStyle) -> O.OrPointStyle:
