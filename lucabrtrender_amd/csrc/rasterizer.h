@@ -287,6 +287,7 @@ private:
     DeviceBuffer counters_;
     uint32_t* hostCounters_ = nullptr;   // pinned
     hipEvent_t events_[9] = {};
+    hipEvent_t countsCopied_ = nullptr;   // the pair-count readback has landed (no timing)
     // GPU milliseconds of the last timed render: project, scan, duplicate, sort
     // histogram, sort passes, ranges, blend, their sum.
     float kernelMs_[8] = {};

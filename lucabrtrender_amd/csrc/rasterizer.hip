@@ -58,6 +58,7 @@ TileRasterizer::~TileRasterizer() {
     (void)hipSetDevice(device_);
     for (hipEvent_t& e : events_)
         if (e != nullptr) (void)hipEventDestroy(e);
+    if (countsCopied_ != nullptr) (void)hipEventDestroy(countsCopied_);
     if (hostCounters_ != nullptr) (void)hipHostFree(hostCounters_);
     if (hostParams_ != nullptr) (void)hipHostFree(hostParams_);
     if (hostViews_ != nullptr) (void)hipHostFree(hostViews_);
@@ -88,6 +89,8 @@ Status TileRasterizer::create(int device, hipStream_t stream) {
         s = check(hipEventCreate(&e), "hipEventCreate");
         if (!s.ok()) return s;
     }
+    s = check(hipEventCreateWithFlags(&countsCopied_, hipEventDisableTiming), "hipEventCreate");
+    if (!s.ok()) return s;
     int cus = 0;
     s = check(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, device_), "device attribute");
     if (!s.ok()) return s;
@@ -350,13 +353,35 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     mark(2);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream_), "counts")).ok())
         return s;
-    if (!(s = check(hipStreamSynchronize(stream_), "counts")).ok()) return s;
+    if (!(s = check(hipEventRecord(countsCopied_, stream_), "counts")).ok()) return s;
+    // The duplication needs no count from the host: queued behind the readback
+    // at the pair buffers' present capacity, it runs while the counts travel
+    // and the host wakes. A frame with more pairs than that (or the first)
+    // runs it again once they have grown.
+    const auto duplicateInto = [&] {
+        mark(3);
+        kern::duplicate(first, splatCount, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
+                        offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(),
+                        training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_, counters + 2, stream_,
+                        counters + 8, counters + 10);
+        mark(4);
+    };
+    const bool early = pairCapacity_ > 0 && (!training || pairSlot_.bytes() >= uint64_t{pairCapacity_} * 4);
+    if (early) duplicateInto();
+    // A frame given up here leaves nothing of its own running on the stream.
+    const auto abandon = [&](Status failure) {
+        (void)hipStreamSynchronize(stream_);
+        return failure;
+    };
+    if (!(s = check(hipEventSynchronize(countsCopied_), "counts")).ok()) return abandon(s);
     const auto countsDone = std::chrono::steady_clock::now();
     const uint32_t pairs = splatCount > 0 ? hostCounters_[0] : 0u;
     const uint32_t visible = splatCount > 0 ? hostCounters_[1] : 0u;
-    if (hostCounters_[5] != 0) return Status::error(ErrorCode::InternalError, "the scan's look-back gave up");
+    if (hostCounters_[5] != 0) {
+        return abandon(Status::error(ErrorCode::InternalError, "the scan's look-back gave up"));
+    }
     if (pairs > kMaxPairs) {
-        return Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 (tile, splat) pairs in one frame");
+        return abandon(Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 (tile, splat) pairs in one frame"));
     }
     stats.visible = visible;
     stats.pairs = pairs;
@@ -365,22 +390,23 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     kern::TileDepthLayout layout;
     if (pairs > 0) {
         const uint32_t lo = hostCounters_[8], hi = hostCounters_[9];
-        if (lo > hi) return Status::error(ErrorCode::InternalError, "pairs without a drawn depth key");
+        if (lo > hi) return abandon(Status::error(ErrorCode::InternalError, "pairs without a drawn depth key"));
         layout.depthMin = lo;
         layout.depthBits = bitsFor(hi - lo + 1u);
     }
     const uint32_t keyBits = bitsFor(tiles) + layout.depthBits;
     stats.keyBits = keyBits;
     stats.sortPasses = kern::sortPasses(keyBits);
-    if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) return s;
-
-    // Duplicate into 64-bit keys, in slot order.
-    mark(3);
-    kern::duplicate(first, splatCount, proj_.as<float4>(), rects_.as<uint2>(), touched_.as<uint32_t>(),
-                    offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(),
-                    training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_, counters + 2, stream_,
-                    counters + 8, counters + 10);
-    mark(4);
+    // Duplicate into 64-bit keys, in slot order (again, if the early run lacked room).
+    if (!early || pairs > pairCapacity_) {
+        // The early run must be done before its buffers may be replaced.
+        if (early && !(s = check(hipStreamSynchronize(stream_), "duplicate")).ok()) return s;
+        if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) return s;
+        kern::fill32(counters + 2, 0u, 1, stream_);   // the early run's dropped pairs
+        duplicateInto();
+    } else if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) {
+        return abandon(s);
+    }
 
     // One stable sort on (tile, depth >> 7), then the ranges.
     const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);

@@ -284,3 +284,20 @@ def test_split_tiles_draw_the_frame_the_one_walk_draws(rast, segment, depth_mode
     ora = H.run_oracle(cfg.camera, s, inst)
     H.assert_bit_exact(gpu, ora)
     _image_checks(gpu, ora)
+
+
+def test_a_frame_that_outgrows_the_pair_buffers_is_the_oracle_s():
+    """The duplication runs behind the pair-count readback at the buffers'
+    present size; a frame with more pairs runs it again after they grow."""
+    print("pair buffers grown between frames", flush=True)
+    r = raster.TileRasterizer()
+    cfg = scenes.workload("plain-fwd-65k-640x480")
+    s = raster.RenderSettings(width=cfg.width, height=cfg.height)
+    small = scenes.workload("plain-fwd-65k-640x480", count=500)
+    H.run_gpu(small.camera, s, [(small.scene, np.eye(4), raster.SplatEdit())], r)
+    inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
+    gpu = H.run_gpu(cfg.camera, s, inst, r)
+    ora = H.run_oracle(cfg.camera, s, inst)
+    assert ora["pairs"] > 20 * 500
+    H.assert_bit_exact(gpu, ora)
+    _image_checks(gpu, ora)
