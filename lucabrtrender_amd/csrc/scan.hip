@@ -177,10 +177,19 @@ __global__ __launch_bounds__(1024) void sumPartsKernel(const uint32_t* __restric
     for (uint32_t i = threadIdx.x; i < count; i += 1024) sum += parts[i];
     uint32_t lo = 0xFFFFFFFFu, hi = 0u;
     if (depthRange != nullptr) {
-        for (uint32_t e = threadIdx.x; e < depthEntries; e += 1024) {
-            const uint2 r = depthBlocks[e];
-            lo = min(lo, r.x);
-            hi = max(hi, r.y);
+        constexpr uint32_t kBatch = 8;   // loads in flight before the first fold
+        for (uint32_t e0 = threadIdx.x; e0 < depthEntries; e0 += 1024 * kBatch) {
+            uint2 r[kBatch];
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                const uint32_t e = e0 + b * 1024;
+                r[b] = e < depthEntries ? depthBlocks[e] : make_uint2(0xFFFFFFFFu, 0u);
+            }
+#pragma unroll
+            for (uint32_t b = 0; b < kBatch; ++b) {
+                lo = min(lo, r[b].x);
+                hi = max(hi, r[b].y);
+            }
         }
     }
 #pragma unroll

@@ -84,8 +84,14 @@ template <KeyMode MODE>
 __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
                                                               const uint32_t* __restrict__ nDevice, uint32_t bitLo,
                                                               uint64_t mask, TileDepthLayout depth, uint32_t passes,
-                                                              uint32_t* __restrict__ hist) {
+                                                              uint32_t* __restrict__ hist,
+                                                              uint32_t* __restrict__ clear, uint64_t clearWords) {
     __shared__ uint32_t sHist[kMaxPasses * kRadix];
+    // The passes' look-back words, zeroed here rather than by a launch of their own.
+    for (uint64_t w = uint64_t{blockIdx.x} * kHistBlock + threadIdx.x; w < clearWords;
+         w += uint64_t{gridDim.x} * kHistBlock) {
+        clear[w] = 0u;
+    }
     const uint32_t n = pairCount(cap, nDevice);
     for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kHistBlock) sHist[j] = 0;
     __syncthreads();
@@ -408,16 +414,16 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     const uint32_t passes = sortPasses(keyBits);
     const uint32_t parts = sortPartitions(n);
     const uint64_t mask = keyBits >= 64 ? ~0ull : ((1ull << keyBits) - 1ull);
-    if (nDevice == nullptr) {
-        fill32(ws.status, 0u, static_cast<uint64_t>(passes) * parts * kRadix, stream);
-    } else {
+    // Without a device count the histogram kernel zeroes the look-back words.
+    const uint64_t clearWords = nDevice == nullptr ? static_cast<uint64_t>(passes) * parts * kRadix : 0u;
+    if (nDevice != nullptr) {
         const uint64_t words = static_cast<uint64_t>(passes) * parts * kRadix;
         const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((words + kBlock - 1) / kBlock, 4096));
         clearStatusKernel<<<grid, kBlock, 0, stream>>>(ws.status, passes, parts, n, nDevice);
     }
     const uint32_t rows = histogramRows(n);
     histogramKernel<MODE><<<rows, kHistBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, depth, passes,
-                                                          ws.histogram);
+                                                          ws.histogram, ws.status, clearWords);
     digitOffsetsKernel<<<passes, 1024, 0, stream>>>(ws.histogram, rows, passes, ws.offsets);
     if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
     const uint64_t* kin = keys;
