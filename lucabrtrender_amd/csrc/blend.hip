@@ -506,41 +506,73 @@ __global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2*
     const uint32_t segment = (tiles + kOrderWaves * 64 - 1) / (kOrderWaves * 64) * 64;   // whole chunks of 64
     const uint32_t begin = wave * segment;
     const uint32_t end = begin + segment < tiles ? begin + segment : tiles;
-    for (uint32_t t = begin + lane; t < end; t += 64) {
-        const uint32_t bucket = bucketOf(ranges[t]);
-        if (t < kOrderCached) sBucket[t] = static_cast<uint8_t>(bucket);
-        atomicAdd(&sCount[wave][bucket], 1u);
+    // Eight loads in flight before the first count: one trip to memory for a
+    // wave's tiles of a 1080p frame instead of one a chunk.
+    constexpr uint32_t kOrderBatch = 8;
+    for (uint32_t t0 = begin + lane; t0 < end; t0 += 64 * kOrderBatch) {
+        uint2 rg[kOrderBatch];
+#pragma unroll
+        for (uint32_t b = 0; b < kOrderBatch; ++b) {
+            const uint32_t t = t0 + 64 * b;
+            rg[b] = t < end ? ranges[t] : make_uint2(0u, 0u);
+        }
+#pragma unroll
+        for (uint32_t b = 0; b < kOrderBatch; ++b) {
+            const uint32_t t = t0 + 64 * b;
+            if (t < end) {
+                const uint32_t bucket = bucketOf(rg[b]);
+                if (t < kOrderCached) sBucket[t] = static_cast<uint8_t>(bucket);
+                atomicAdd(&sCount[wave][bucket], 1u);
+            }
+        }
+    }
+    __syncthreads();
+    // Bucket k's first place: the tiles of every heavier bucket; then wave w's
+    // within it: the bucket's tiles in the waves before w.
+    __shared__ uint32_t sTotal[kBuckets];
+    if (tid < kBuckets) {
+        uint32_t sum = 0;
+        for (uint32_t ww = 0; ww < kOrderWaves; ++ww) sum += sCount[ww][tid];
+        sTotal[tid] = sum;
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t running = 0;
+        for (uint32_t k = kBuckets; k-- > 0;) {
+            const uint32_t n = sTotal[k];
+            sTotal[k] = running;
+            running += n;
+        }
     }
     __syncthreads();
     for (uint32_t e = tid; e < kOrderWaves * kBuckets; e += blockDim.x) {
         const uint32_t w = e / kBuckets, k = e % kBuckets;
-        uint32_t start = 0;
-        for (uint32_t kk = k + 1; kk < kBuckets; ++kk)
-            for (uint32_t ww = 0; ww < kOrderWaves; ++ww) start += sCount[ww][kk];
+        uint32_t start = sTotal[k];
         for (uint32_t ww = 0; ww < w; ++ww) start += sCount[ww][k];
         sStart[w][k] = start;
     }
     __syncthreads();
-    // Lane k owns the next free place of bucket k in a register and hands it
-    // out by shuffle: places never pass between lanes through memory, which the
-    // compiler would be free to cache per lane.
-    uint32_t next = lane < kBuckets ? sStart[wave][lane] : 0u;
+    // Each lane's rank among the lanes of its bucket (six ballots on the
+    // bucket's bits give the lanes that share it); the group's last lane then
+    // moves the bucket's next place on. The places live in LDS, read and written
+    // through a volatile pointer so that every round sees the last one's.
+    volatile uint32_t* next = &sStart[wave][0];
     const uint64_t below = (1ull << lane) - 1ull;
+    static_assert(kBuckets < 64, "a bucket (and the no-tile mark kBuckets) fits six bits");
     for (uint32_t base = begin; base < end; base += 64) {
         const uint32_t t = base + lane;
         const bool mine = t < end;
         const uint32_t bucket = !mine ? kBuckets : t < kOrderCached ? sBucket[t] : bucketOf(ranges[t]);
-        // One round per bucket present among these 64 tiles.
-        uint64_t todo = __ballot(mine);
-        while (todo != 0) {
-            const int first = static_cast<int>(__builtin_ctzll(todo));
-            const uint32_t k = static_cast<uint32_t>(__shfl(static_cast<int>(bucket), first, 64));
-            const uint64_t vote = __ballot(mine && bucket == k);
-            const uint32_t start = static_cast<uint32_t>(__shfl(static_cast<int>(next), static_cast<int>(k), 64));
-            if (mine && bucket == k) order[start + static_cast<uint32_t>(__popcll(vote & below))] = t;
-            if (lane == k) next += static_cast<uint32_t>(__popcll(vote));
-            todo &= ~vote;
+        uint64_t same = ~0ull;
+#pragma unroll
+        for (uint32_t bit = 0; bit < 6; ++bit) {
+            const bool set = ((bucket >> bit) & 1u) != 0u;
+            const uint64_t vote = __ballot(set);
+            same &= set ? vote : ~vote;
         }
+        const uint32_t place = mine ? next[bucket] + static_cast<uint32_t>(__popcll(same & below)) : 0u;
+        if (mine) order[place] = t;
+        if (mine && (same & ~below & ~(1ull << lane)) == 0) next[bucket] = place + 1u;
     }
     if (tasks == nullptr) return;
     // The split plan: tiles in that order, each cut into segmentsOf() segments
