@@ -48,10 +48,11 @@ def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes):
         "sort_histogram": pairs * 8,
         # each pass reads and writes a 12-byte pair
         "sort_passes": pairs * 24 * passes,
-        # keys read, two u32 a tile written
-        "ranges": pairs * 8 + tiles * 8,
-        # each pair's slot and record read once a tile, the pixel out (colour 16 + depth 4 + state 8)
-        "blend": pairs * (4 + 48) + pixels * 28,
+        # keys read, two u32 a tile written; the ordering reads them again and writes a u32 a tile
+        "ranges": pairs * 8 + tiles * 8 + tiles * 12,
+        # each pair's slot and record read once a tile, the pixel out (colour 16 + depth 4; the
+        # walk state is kept only on request)
+        "blend": pairs * (4 + 48) + pixels * 20,
     }
 
 

@@ -411,8 +411,9 @@ typedef enum {
     LRT_DEBUG_FINAL_T = 8,       /* float a pixel: final transmittance (kept frames only) */
     LRT_DEBUG_RECTS = 9,         /* u32[2] a slot: x0 | y0 << 16, x1 | y1 << 16 */
     LRT_DEBUG_KERNEL_MS = 10,    /* float[8], last render with time_stages: GPU ms of project,
-                                    scan, duplicate, sort histogram, sort passes, ranges,
-                                    blend, and their sum (HIP events on the context's stream) */
+                                    scan, duplicate, sort histogram, sort passes, ranges (with
+                                    the heaviest-first tile ordering), blend, and their sum (HIP
+                                    events on the context's stream) */
     LRT_DEBUG_TILE_ORDER = 11    /* u32 a tile: the order the blend took the tiles in (empty
                                     when the last render used index order) */
 } lrt_debug_buffer;

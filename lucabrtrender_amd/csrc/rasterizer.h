@@ -289,7 +289,7 @@ private:
     hipEvent_t events_[9] = {};
     hipEvent_t countsCopied_ = nullptr;   // the pair-count readback has landed (no timing)
     // GPU milliseconds of the last timed render: project, scan, duplicate, sort
-    // histogram, sort passes, ranges, blend, their sum.
+    // histogram, sort passes, ranges (with the heaviest-first ordering), blend, their sum.
     float kernelMs_[8] = {};
 
     // What the last render left.

@@ -423,7 +423,6 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
                           sortedSlots_.as<uint32_t>(), stream_);
         pairSlots = sortedSlots_.as<uint32_t>();
     }
-    mark(7);
 
     const uint32_t* order = nullptr;
     // Heavy tiles split across waves: frames that keep no composite or
@@ -438,6 +437,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
         kern::tileOrder(ranges_.as<uint2>(), tiles, tileOrder_.as<uint32_t>(), stream_);
         order = tileOrder_.as<uint32_t>();
     }
+    mark(7);   // the blend stage: the blend launch alone (a split blend's plan included)
     if (split) {
         const uint32_t seg = settings.blendSegment;
         kern::BlendPlan plan{};
