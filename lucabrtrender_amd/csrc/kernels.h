@@ -22,12 +22,14 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
 // *nonzero. `status` holds scanStatusWords(n) u64 words, cleared here. With
 // depthRange non-null, the depth-key entries [0, depthEntries) of the frame's
 // projecting workgroups (DepthBlocks) are reduced into depthRange[0] (least)
-// and depthRange[1] (greatest).
+// and depthRange[1] (greatest). alsoZero's first alsoZeroWords words are
+// zeroed by the same launch that clears the look-back words.
 uint32_t scanPartitions(uint32_t n);
 inline uint64_t scanStatusWords(uint32_t n) { return uint64_t{scanPartitions(n)} * 3 / 2 + 2; }
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks = nullptr,
-                   uint32_t depthEntries = 0, uint32_t* depthRange = nullptr);
+                   uint32_t depthEntries = 0, uint32_t* depthRange = nullptr, uint32_t* alsoZero = nullptr,
+                   uint32_t alsoZeroWords = 0);
 
 // ---- projection -----------------------------------------------------------
 // One instance: f.count splats from index 0 of the cloud arrays, written to
@@ -100,12 +102,14 @@ SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32
 // result is in (keys, values) after an even number of passes and in
 // (keysAlt, valuesAlt) after an odd one; returns true for the latter.
 // afterHistogram (may be null) is recorded between the histogram and the passes.
+// alsoZero's first alsoZeroWords words are zeroed before the passes (by the
+// histogram launch, or by a fill when there is nothing to sort).
 // With nDevice non-null the count is *nDevice, read by the kernels, and n is
 // the capacity that bounds it (the launch is sized for n; nothing waits on the host).
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
                KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
                hipEvent_t afterHistogram = nullptr, const uint32_t* nDevice = nullptr,
-               TileDepthLayout depth = {});
+               TileDepthLayout depth = {}, uint32_t* alsoZero = nullptr, uint64_t alsoZeroWords = 0);
 
 // ---- load: raw splat records into packed splats --------------------------------
 // The decode's view of an io::SplatEncoding: field offsets in floats, modes
@@ -131,8 +135,9 @@ void splatBounds(const float4* positions, uint32_t count, float* partial, float*
 // ---- ranges -----------------------------------------------------------------
 // ranges[t] = [start, end) of tile t's run in the sorted keys, (0, 0) when empty.
 // With nDevice non-null the count is *nDevice and `pairs` its capacity.
+// zeroed: the caller has already set every range to (0, 0).
 void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream,
-                const uint32_t* nDevice = nullptr);
+                const uint32_t* nDevice = nullptr, bool zeroed = false);
 
 // ---- blend ------------------------------------------------------------------
 // Tile indices, heaviest tile (most pairs) first, for the blend's launch order.

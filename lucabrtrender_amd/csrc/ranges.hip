@@ -63,8 +63,8 @@ __global__ __launch_bounds__(kBlock) void rangesKernel(const uint64_t* __restric
 }   // namespace
 
 void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream,
-                const uint32_t* nDevice) {
-    fill32(reinterpret_cast<uint32_t*>(ranges), 0u, static_cast<uint64_t>(tiles) * 2, stream);
+                const uint32_t* nDevice, bool zeroed) {
+    if (!zeroed) fill32(reinterpret_cast<uint32_t*>(ranges), 0u, static_cast<uint64_t>(tiles) * 2, stream);
     if (pairs == 0) return;
     const uint64_t threads = (uint64_t{pairs} + kPer - 1) / kPer;
     rangesKernel<<<static_cast<uint32_t>((threads + kBlock - 1) / kBlock), kBlock, 0, stream>>>(keys, pairs, nDevice,

@@ -346,10 +346,9 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
 
     // How many pairs, how many visible and the drawn depth keys' range: one
     // scan, one readback.
-    kern::fill32(counters, 0u, 12, stream_);
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), splatCount, scanStatus_.as<uint64_t>(),
                         counters + 0, counters + 1, counters + 5, stream_, depthBlocks_.as<uint32_t>(), depthEntries_,
-                        counters + 8);
+                        counters + 8, counters, 12);   // the counters zeroed with the look-back words
     mark(2);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream_), "counts")).ok())
         return s;
@@ -412,10 +411,11 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairs, kern::KeyMode::TileDepth, 0, keyBits, ws,
-                                       stream_, timed ? events_[5] : nullptr, nullptr, layout);
+                                       stream_, timed ? events_[5] : nullptr, nullptr, layout,
+                                       reinterpret_cast<uint32_t*>(ranges_.as<uint2>()), uint64_t{tiles} * 2);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
-    kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_);
+    kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_, nullptr, true);
     // The blend reads slots; a training frame's sort carried pair indices.
     const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
     if (training) {

@@ -12,6 +12,8 @@
 // in[i-1] in u32 arithmetic, total = the sum of all.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
+
 #include "kernels.h"
 
 namespace lrt::kern {
@@ -24,6 +26,18 @@ constexpr uint64_t kAggregate = 1ull << 62;
 constexpr uint64_t kInclusive = 2ull << 62;
 constexpr uint64_t kValueMask = (1ull << 62) - 1;
 constexpr uint32_t kSpinLimit = 1u << 18;
+
+// Zeroes words [0, n0) of a and [0, n1) of b: two small clears in one launch.
+__global__ void zero2Kernel(uint32_t* a, uint64_t n0, uint32_t* b, uint64_t n1) {
+    const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
+    for (uint64_t i = static_cast<uint64_t>(blockIdx.x) * blockDim.x + threadIdx.x; i < n0 + n1; i += stride) {
+        if (i < n0) {
+            a[i] = 0u;
+        } else {
+            b[i - n0] = 0u;
+        }
+    }
+}
 
 __global__ void fill32Kernel(uint32_t* dst, uint32_t value, uint64_t count) {
     const uint64_t stride = static_cast<uint64_t>(gridDim.x) * blockDim.x;
@@ -232,9 +246,14 @@ uint32_t scanPartitions(uint32_t n) { return (n + kPart - 1) / kPart; }
 
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks,
-                   uint32_t depthEntries, uint32_t* depthRange) {
+                   uint32_t depthEntries, uint32_t* depthRange, uint32_t* alsoZero, uint32_t alsoZeroWords) {
     const uint32_t parts = scanPartitions(n);
-    fill32(reinterpret_cast<uint32_t*>(status), 0u, static_cast<uint64_t>(parts) * 2, stream);
+    {
+        const uint64_t words = static_cast<uint64_t>(parts) * 2 + alsoZeroWords;
+        const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((words + 255) / 256, 4096));
+        zero2Kernel<<<grid, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(status), static_cast<uint64_t>(parts) * 2,
+                                              alsoZero, alsoZero != nullptr ? alsoZeroWords : 0u);
+    }
     if (n == 0) {
         fill32(total, 0u, 1, stream);
         fill32(nonzero, 0u, 1, stream);

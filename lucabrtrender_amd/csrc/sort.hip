@@ -85,12 +85,18 @@ __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __
                                                               const uint32_t* __restrict__ nDevice, uint32_t bitLo,
                                                               uint64_t mask, TileDepthLayout depth, uint32_t passes,
                                                               uint32_t* __restrict__ hist,
-                                                              uint32_t* __restrict__ clear, uint64_t clearWords) {
+                                                              uint32_t* __restrict__ clear, uint64_t clearWords,
+                                                              uint32_t* __restrict__ clear2, uint64_t clear2Words) {
     __shared__ uint32_t sHist[kMaxPasses * kRadix];
-    // The passes' look-back words, zeroed here rather than by a launch of their own.
-    for (uint64_t w = uint64_t{blockIdx.x} * kHistBlock + threadIdx.x; w < clearWords;
+    // The passes' look-back words (and the caller's region), zeroed here rather
+    // than by launches of their own.
+    for (uint64_t w = uint64_t{blockIdx.x} * kHistBlock + threadIdx.x; w < clearWords + clear2Words;
          w += uint64_t{gridDim.x} * kHistBlock) {
-        clear[w] = 0u;
+        if (w < clearWords) {
+            clear[w] = 0u;
+        } else {
+            clear2[w - clearWords] = 0u;
+        }
     }
     const uint32_t n = pairCount(cap, nDevice);
     for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kHistBlock) sHist[j] = 0;
@@ -410,7 +416,8 @@ uint32_t histogramRows(uint32_t n) {
 template <KeyMode MODE>
 void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
                   const uint32_t* nDevice, uint32_t bitLo, uint32_t keyBits, TileDepthLayout depth,
-                  const SortWorkspace& ws, hipStream_t stream, hipEvent_t afterHistogram) {
+                  const SortWorkspace& ws, hipStream_t stream, hipEvent_t afterHistogram, uint32_t* alsoZero,
+                  uint64_t alsoZeroWords) {
     const uint32_t passes = sortPasses(keyBits);
     const uint32_t parts = sortPartitions(n);
     const uint64_t mask = keyBits >= 64 ? ~0ull : ((1ull << keyBits) - 1ull);
@@ -423,7 +430,8 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
     }
     const uint32_t rows = histogramRows(n);
     histogramKernel<MODE><<<rows, kHistBlock, 0, stream>>>(keys, n, nDevice, bitLo, mask, depth, passes,
-                                                          ws.histogram, ws.status, clearWords);
+                                                          ws.histogram, ws.status, clearWords, alsoZero,
+                                                          alsoZero != nullptr ? alsoZeroWords : 0u);
     digitOffsetsKernel<<<passes, 1024, 0, stream>>>(ws.histogram, rows, passes, ws.offsets);
     if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
     const uint64_t* kin = keys;
@@ -462,17 +470,19 @@ SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32
 
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n, KeyMode mode,
                uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
-               hipEvent_t afterHistogram, const uint32_t* nDevice, TileDepthLayout depth) {
+               hipEvent_t afterHistogram, const uint32_t* nDevice, TileDepthLayout depth, uint32_t* alsoZero,
+               uint64_t alsoZeroWords) {
     if ((nDevice == nullptr && n < 2) || n == 0 || keyBits == 0) {
+        if (alsoZero != nullptr) fill32(alsoZero, 0u, alsoZeroWords, stream);
         if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
         return false;
     }
     if (mode == KeyMode::TileDepth) {
         launchPasses<KeyMode::TileDepth>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, depth, ws,
-                                         stream, afterHistogram);
+                                         stream, afterHistogram, alsoZero, alsoZeroWords);
     } else {
         launchPasses<KeyMode::Plain>(keys, values, keysAlt, valuesAlt, n, nDevice, bitLo, keyBits, depth, ws, stream,
-                                     afterHistogram);
+                                     afterHistogram, alsoZero, alsoZeroWords);
     }
     return (sortPasses(keyBits) & 1u) != 0u;
 }
