@@ -6,6 +6,7 @@ import os
 import re
 
 import numpy as np
+import pytest
 
 from lucabrtrender_amd import _abi, raster
 
@@ -90,3 +91,44 @@ def test_synthetic_scenes_are_seeded_and_prefix_stable():
     assert np.array_equal(a.positions[:400], b.positions) and np.array_equal(a.sh[:400], b.sh)
     assert not np.array_equal(b.positions, c.positions)
     assert a.sh.shape == (1000, 23)
+
+
+# Each ctypes mirror and the header's struct it mirrors.
+_MIRRORS = {
+    "Lens": "lrt_lens", "CameraC": "lrt_camera", "ProjectionC": "lrt_projection", "Splats": "lrt_splats",
+    "SplatEdit": "lrt_splat_edit", "SplatInstanceC": "lrt_splat_instance", "PointsC": "lrt_points",
+    "PointStyleC": "lrt_point_style", "PointInstanceC": "lrt_point_instance",
+    "RenderSettingsC": "lrt_render_settings", "RenderTargetsC": "lrt_render_targets", "ViewC": "lrt_view",
+    "FrameStatsC": "lrt_frame_stats", "DiffSplatsC": "lrt_diff_splats", "DiffGradsC": "lrt_diff_grads",
+    "LodBuildSettingsC": "lrt_lod_build_settings", "LodInfoC": "lrt_lod_info",
+    "LodInstanceC": "lrt_lod_instance", "CutStatsC": "lrt_cut_stats", "SplatEncodingC": "lrt_splat_encoding",
+    "CloudInfoC": "lrt_cloud_info",
+}
+
+
+def test_every_ctypes_mirror_has_the_header_s_layout(tmp_path):
+    """Size and every field's offset of each struct, as gcc lays the header
+    out, against the ctypes mirror the bindings pass."""
+    import shutil
+    import subprocess
+    from lucabrtrender_amd import _abi
+    gcc = shutil.which("gcc")
+    if gcc is None:
+        pytest.skip("no C compiler")
+    lines = ['#include <stddef.h>', '#include <stdio.h>', '#include "lrt_hip.h"', "int main(void) {"]
+    expected = []
+    for py, c in _MIRRORS.items():
+        cls = getattr(_abi, py)
+        lines.append(f'printf("%zu\\n", sizeof({c}));')
+        expected.append(C.sizeof(cls))
+        for name, *_ in cls._fields_:
+            lines.append(f'printf("%zu\\n", offsetof({c}, {name}));')
+            expected.append(getattr(cls, name).offset)
+    lines.append("return 0; }")
+    src = tmp_path / "layout.c"
+    src.write_text("\n".join(lines))
+    exe = tmp_path / "layout"
+    include = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "include")
+    subprocess.run([gcc, "-std=c11", "-I", include, str(src), "-o", str(exe)], check=True)
+    got = [int(v) for v in subprocess.run([str(exe)], check=True, capture_output=True, text=True).stdout.split()]
+    assert got == expected
