@@ -163,7 +163,8 @@ typedef struct {
  * sort_passes (no counterpart in the reference) describe the frame's sort: its
  * virtual key is the tile id over the depth key (bits(view z) >> 7) less the
  * frame's least drawn one, in bitsFor(tiles) + bitsFor(depth-key range) bits,
- * eight a pass (a batch of views keeps all 24 depth bits). */
+ * eight a pass (a batch of views, which reads no counts back before its sort,
+ * bounds the depth keys by its views' least near and greatest far plane). */
 typedef struct {
     uint32_t splats, visible, pairs;
     double project_ms, depth_sort_ms, counts_ms, emit_ms, tile_sort_ms, blend_ms, total_ms;

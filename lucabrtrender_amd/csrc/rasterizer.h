@@ -265,6 +265,7 @@ private:
     // The projecting workgroups' depth-key ranges (DepthBlocks), two words each.
     DeviceBuffer depthBlocks_;
     uint32_t depthEntries_ = 0;   // the entries the frame's projections wrote
+    kern::TileDepthLayout viewsLayout_;   // the batch's sort key depth field (from its near and far planes)
     // Per pair, ping-pong.
     uint32_t pairCapacity_ = 0;
     DeviceBuffer keys_[2], values_[2], sortWork_;

@@ -9,9 +9,11 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <bit>
 #include <chrono>
 #include <cstdint>
 #include <cstring>
+#include <limits>
 #include <string>
 
 #include "kernels.h"
@@ -543,11 +545,11 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint
     mark(4);
     // The pair count (counters[0], from the scan) stays on the device: the
     // sort and the ranges read it, launched for the capacity.
-    const uint32_t keyBits = 24 + bitsFor(tiles);
+    const uint32_t keyBits = bitsFor(tiles) + viewsLayout_.depthBits;
     const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairCapacity_, kern::KeyMode::TileDepth, 0,
-                                       keyBits, ws, stream, timed ? events_[5] : nullptr, counters + 0);
+                                       keyBits, ws, stream, timed ? events_[5] : nullptr, counters + 0, viewsLayout_);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream, counters + 0);
@@ -639,6 +641,25 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
     // its slots from v * splatsPerView + the instance's base.
     uint32_t tileBase = 0;
     uint64_t pixelBase = 0;
+    // A view draws only splats with near < z <= far (as floats, as the
+    // projection tests them), so the batch's depth keys lie between the keys of
+    // its least near and greatest far plane: the sort's depth field, fixed
+    // before any count comes back. Planes that bound nothing keep all 24 bits.
+    float nearest = std::numeric_limits<float>::infinity(), farthest = 0.0f;
+    bool bounded = viewCount > 0;
+    for (uint32_t v = 0; v < viewCount; ++v) {
+        const float nearZ = static_cast<float>(views[v].projection.nearZ);
+        const float farZ = static_cast<float>(views[v].projection.farZ);
+        bounded = bounded && nearZ > 0.0f && farZ >= nearZ && farZ < std::numeric_limits<float>::infinity();
+        nearest = std::min(nearest, nearZ);
+        farthest = std::max(farthest, farZ);
+    }
+    viewsLayout_ = kern::TileDepthLayout{};
+    if (bounded) {
+        const auto depthKey = [](float z) { return (std::bit_cast<uint32_t>(z) & 0x7FFFFFFFu) >> 7; };
+        viewsLayout_.depthMin = depthKey(nearest);
+        viewsLayout_.depthBits = bitsFor(depthKey(farthest) - depthKey(nearest) + 1u);
+    }
     for (uint32_t v = 0; v < viewCount; ++v) {
         const View& view = views[v];
         RenderSettings vs = settings;
@@ -679,7 +700,8 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
             std::string key = std::to_string(viewCount) + "/" + std::to_string(splatsPerView) + "/" +
                               std::to_string(tiles) + "/" + std::to_string(pairCapacity_) + "/" +
                               std::to_string(settings.blendBatch) + "/" + std::to_string(settings.persistentBlend) +
-                              "/" + std::to_string(settings.depthMode);
+                              "/" + std::to_string(settings.depthMode) + "/" + std::to_string(viewsLayout_.depthMin) +
+                              "/" + std::to_string(viewsLayout_.depthBits);
             const auto addr = [&key](const void* p) { key += "/" + std::to_string(reinterpret_cast<uintptr_t>(p)); };
             for (const SplatInstance& instance : instances) {
                 const SplatCloud* c = instance.splats;
@@ -760,7 +782,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         stats.splats = static_cast<uint32_t>(slots);
         stats.visible = lastVisible_;
         stats.pairs = lastPairs_;
-        stats.keyBits = 24 + bitsFor(static_cast<uint32_t>(tiles));   // the batch's sort keeps every depth bit
+        stats.keyBits = bitsFor(static_cast<uint32_t>(tiles)) + viewsLayout_.depthBits;
         stats.sortPasses = kern::sortPasses(stats.keyBits);
         stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
         *statsOut = stats;
