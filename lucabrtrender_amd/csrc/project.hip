@@ -85,11 +85,13 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
     if (live) {
         const float4 p = positions[i];
         row = shape[i];
+        // Issued with the two loads above: one trip to memory, not two.
+        const float filterOf = f.useFilter3d ? filter3d[i] : 0.0f;
         const float sx = expF(lowHalf(row.y)), sy = expF(highHalf(row.y)), sz = expF(lowHalf(row.z));
         fate = editFate(f.edit, p.x, p.y, p.z, p.w, sx, sy, sz);
         if (!fate.removed) {
             shapeOf = SplatShape{p.x, p.y, p.z, fate.opacity, sx, sy, sz, {0.0f, 0.0f, 0.0f, 0.0f}};
-            if (f.useFilter3d) applyFilter3d(filter3d[i], shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
+            if (f.useFilter3d) applyFilter3d(filterOf, shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
             decodeQuaternion(row.x, shapeOf.q);
             candidate = footprintOf(f, shapeOf, fp);
         }
@@ -205,10 +207,11 @@ __global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* 
     if (live) {
         p = positions[i];
         row = shape[i];
+        const float filterOf = f0.useFilter3d ? filter3d[i] : 0.0f;   // issued with the loads above
         const float sx = expF(lowHalf(row.y)), sy = expF(highHalf(row.y)), sz = expF(lowHalf(row.z));
         fate = editFate(f0.edit, p.x, p.y, p.z, p.w, sx, sy, sz);
         shapeOf = SplatShape{p.x, p.y, p.z, fate.opacity, sx, sy, sz, {0.0f, 0.0f, 0.0f, 0.0f}};
-        if (f0.useFilter3d) applyFilter3d(filter3d[i], shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
+        if (f0.useFilter3d) applyFilter3d(filterOf, shapeOf.sx, shapeOf.sy, shapeOf.sz, shapeOf.opacity);
         decodeQuaternion(row.x, shapeOf.q);
     }
     __syncthreads();
