@@ -207,6 +207,9 @@ Status readSplatPly(const std::string& path, RawSplats& out) {
     if (vertex->hasList) return Status::error(ErrorCode::Unsupported, "'" + path + "': a list in the vertex element");
     if (vertex->count > 0xFFFFFFFFull) return Status::error(ErrorCode::Unsupported, "'" + path + "': more than 2^32 splats");
     // The vertex element's offset: every element before it must be fixed-size.
+    // Counts come from the header, so every product and sum is checked: a
+    // header whose sizes wrap 64 bits or pass the file's end is refused.
+    const Status ends = Status::error(ErrorCode::IoFailure, "'" + path + "': the file ends before its vertices do");
     uint64_t start = header.bytes;
     for (const PlyElement& e : header.elements) {
         if (e.name == "vertex") break;
@@ -214,10 +217,16 @@ Status readSplatPly(const std::string& path, RawSplats& out) {
             return Status::error(ErrorCode::Unsupported,
                                  "'" + path + "': a list element ('" + e.name + "') before 'vertex' makes its offset unknowable");
         }
-        start += uint64_t{e.stride} * e.count;
+        uint64_t bytes = 0;
+        if (__builtin_mul_overflow(uint64_t{e.stride}, e.count, &bytes) || __builtin_add_overflow(start, bytes, &start) ||
+            start > file.size()) {
+            return ends;
+        }
     }
-    if (start + uint64_t{vertex->stride} * vertex->count > file.size()) {
-        return Status::error(ErrorCode::IoFailure, "'" + path + "': the file ends before its vertices do");
+    uint64_t vertexBytes = 0, vertexEnd = 0;
+    if (__builtin_mul_overflow(uint64_t{vertex->stride}, vertex->count, &vertexBytes) ||
+        __builtin_add_overflow(start, vertexBytes, &vertexEnd) || vertexEnd > file.size()) {
+        return ends;
     }
     static const char* const kRequired[] = {"x",     "y",     "z",     "opacity", "scale_0", "scale_1", "scale_2",
                                             "rot_0", "rot_1", "rot_2", "rot_3",   "f_dc_0",  "f_dc_1",  "f_dc_2"};

@@ -134,10 +134,12 @@ _SIGS = {
 }
 
 
-def load():
-    if not os.path.exists(ORACLE_PATH):
-        raise RuntimeError(f"{ORACLE_PATH} is not built: run `make -C oracle`")
-    lib = C.CDLL(ORACLE_PATH)
+def load(path: str = ORACLE_PATH):
+    """The oracle at `path` with its signatures set (a planted-mistake build of
+    the same source loads the same way)."""
+    if not os.path.exists(path):
+        raise RuntimeError(f"{path} is not built: run `make -C oracle`")
+    lib = C.CDLL(path)
     for name, (res, args) in _SIGS.items():
         fn = getattr(lib, name)
         fn.restype = res
@@ -278,8 +280,10 @@ def reference_points(frames, clouds, n, pframes, points, npts, threads=0):
     return colour, depth
 
 
-def raster(frames, clouds, n, threads=0, want_image=True):
-    """The reference's whole frame restated: dict of host arrays."""
+def raster(frames, clouds, n, threads=0, want_image=True, using=None):
+    """The reference's whole frame restated: dict of host arrays. `using`: another
+    build of the oracle (a planted mistake) instead of the shipped one."""
+    lib_ = using or lib
     total = sum(frames[k].count for k in range(n))
     f0 = frames[0]
     tiles = f0.tiles_x * f0.tiles_y
@@ -294,11 +298,11 @@ def raster(frames, clouds, n, threads=0, want_image=True):
     final_t = np.zeros((f0.height, f0.width), np.float32) if want_image else None
     stats = OrStats()
     # First pass sizes the pairs.
-    pairs = lib.or_raster(frames, clouds, n, ptr(proj), ptr(touched), ptr(keys), None, None, 0, None, None, None,
+    pairs = lib_.or_raster(frames, clouds, n, ptr(proj), ptr(touched), ptr(keys), None, None, 0, None, None, None,
                           None, None, C.byref(stats), threads)
     tiles_out = np.zeros(max(pairs, 1), np.uint32)
     splats_out = np.zeros(max(pairs, 1), np.uint32)
-    lib.or_raster(frames, clouds, n, ptr(proj), ptr(touched), ptr(keys), ptr(tiles_out), ptr(splats_out), pairs,
+    lib_.or_raster(frames, clouds, n, ptr(proj), ptr(touched), ptr(keys), ptr(tiles_out), ptr(splats_out), pairs,
                   ptr(ranges), ptr(colour), ptr(depth), ptr(last), ptr(final_t), C.byref(stats), threads)
     del px
     return dict(proj=proj[:total], touched=touched[:total], depth_keys=keys[:total], pair_tiles=tiles_out[:pairs],

@@ -155,6 +155,27 @@ def test_refusals_carry_the_reference_codes(tmp_path):
     assert _refusal(tmp_path / "missing.ply") == IO_FAILURE
 
 
+def test_a_header_whose_sizes_wrap_is_refused(tmp_path):
+    """Element counts come from the header: an element before the vertices whose
+    size puts the vertex offset just under 2^64 (so offset + vertex bytes wraps
+    to a small number), and one whose stride * count wraps, are both refused
+    with IoFailure instead of reading outside the file."""
+    props = "".join(f"property float {n}\n" for n in PLY_NAMES)
+    body = b"\x00" * (4 * len(PLY_NAMES) + 64)
+    probe = f"ply\nformat binary_little_endian 1.0\nelement junk {{}}\nproperty uchar u\nelement vertex 1\n{props}end_header\n"
+    digits = len(str(2 ** 64 - 1))
+    hdr = len(probe.format("9" * digits))
+    count = 2 ** 64 - 16 - hdr
+    assert len(str(count)) == digits
+    (tmp_path / "wrap_add.ply").write_bytes(probe.format(count).encode() + body)
+    assert _refusal(tmp_path / "wrap_add.ply") == IO_FAILURE
+    junk = "".join(f"property double d{k}\n" for k in range(8))     # a 64-byte stride
+    header = (f"ply\nformat binary_little_endian 1.0\nelement junk {2 ** 62 + 1}\n{junk}element vertex 1\n"
+              f"{props}end_header\n")
+    (tmp_path / "wrap_mul.ply").write_bytes(header.encode() + body)
+    assert _refusal(tmp_path / "wrap_mul.ply") == IO_FAILURE
+
+
 def _splat_record():
     """test_loading.cpp:168-172."""
     return struct.pack("<6f8B", 0.5, -0.5, 2.0, 0.25, 0.5, 1.0, 255, 0, 51, 255, 255, 128, 128, 128)
