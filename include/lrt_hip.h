@@ -46,7 +46,7 @@ typedef struct lrt_context lrt_context;
 
 /* ---- host math: cameras (no device needed) ------------------------------ */
 
-/* lrt::render::Lens (modules/render/include/lrt/render/Camera.h:73-94), the
+/* lrt::render::Lens (modules/render/include/lrt/render/Camera.h:23-44), the
  * fields the rasteriser reads. lrt_lens_default fills the reference's defaults. */
 typedef struct {
     int32_t orthographic;
@@ -60,13 +60,13 @@ typedef struct {
     double window_roll;      /* degrees */
 } lrt_lens;
 
-/* lrt::render::Camera (Camera.h:96-103): object-to-world, row-major 4x4. */
+/* lrt::render::Camera (Camera.h:46-53): object-to-world, row-major 4x4. */
 typedef struct {
     double camera_to_world[16];
     lrt_lens lens;
 } lrt_camera;
 
-/* lrt::render::Projection (Camera.h:106-115), the fields the rasteriser reads. */
+/* lrt::render::Projection (Camera.h:56-80), the fields the rasteriser reads. */
 typedef struct {
     double world_to_view[16];
     double focal_x, focal_y, centre_x, centre_y, near_z, far_z;
@@ -101,7 +101,7 @@ typedef struct {
     uint32_t sh_words;          /* (rest_per_colour * 3 + 1) / 2, 1 at degree 0 */
 } lrt_splats;
 
-/* render::SplatEdit (modules/render/include/lrt/render/TileRasterizer.h:158-179). */
+/* render::SplatEdit (modules/render/include/lrt/render/TileRasterizer.h:29-50). */
 typedef struct {
     int32_t active;
     uint32_t shape;      /* 0 box, 1 sphere */
@@ -113,14 +113,14 @@ typedef struct {
     float saturation, brightness, opacity, min_opacity, max_scale;
 } lrt_splat_edit;
 
-/* render::SplatInstance (TileRasterizer.h:181-196), without relighting. */
+/* render::SplatInstance (TileRasterizer.h:52-67), without relighting. */
 typedef struct {
     const lrt_splats* splats;
     double object_to_world[16];
     lrt_splat_edit edit;
 } lrt_splat_instance;
 
-/* render::RenderSettings (TileRasterizer.h:234-247) plus two filter settings the
+/* render::RenderSettings (TileRasterizer.h:105-118) plus two filter settings the
  * reference fixes: dilation (its +0.3 px^2) and filter3d (off). */
 typedef struct {
     uint32_t width, height;
@@ -146,7 +146,7 @@ typedef struct {
                                      always keeps them; otherwise they cost the blend an instruction a pair) */
 } lrt_render_settings;
 
-/* render::RenderTargets (TileRasterizer.h:257-263): device images, bottom row
+/* render::RenderTargets (TileRasterizer.h:129-134): device images, bottom row
  * first, width * height each. colour: float4 premultiplied linear; depth:
  * float view z (may be NULL). The caller allocates them at the settings' size. */
 typedef struct {
@@ -155,7 +155,7 @@ typedef struct {
     float* depth;
 } lrt_render_targets;
 
-/* render::FrameStats (TileRasterizer.h:249-255). depth_sort_ms is 0: one
+/* render::FrameStats (TileRasterizer.h:120-126). depth_sort_ms is 0: one
  * 64-bit (tile, depth) sort replaces the reference's depth sort + tile sort
  * and is reported as tile_sort_ms. total_ms is the call's wall clock, as the
  * reference's; the stage figures (with time_stages) are GPU time from HIP
@@ -232,9 +232,9 @@ void lrt_point_style_default(lrt_point_style* out);
 
 /* TileRasterizer::render(projection, instances, settings, targets, points,
  * under) (TileRasterizer.h:143-155, TileRasterizer.cpp:188-484 with :355-377
- * and :464-475): point instances drawn as opaque discs (points_project.slang:
- * 33-86) in the same depth order as the splats, their slots after the
- * splats'; `under` (may be NULL; the frame's size, colour and depth both set,
+ * and :464-475): point instances drawn as opaque discs
+ * (points_project.slang:33-86) in the same depth order as the splats, their
+ * slots after the splats'; `under` (may be NULL; the frame's size, colour and depth both set,
  * device memory as the targets') is what lies under the splats -- each
  * pixel's walk ends at the first pair at or behind its under depth (when that
  * is above 0) and the under colour lies behind. Points or an under-layer
@@ -249,7 +249,7 @@ lrt_status lrt_raster_render_composite(lrt_context* ctx, const lrt_projection* p
                                        lrt_render_targets* targets, lrt_frame_stats* stats);
 
 /* ---- levels of detail ----------------------------------------------------------------
- * lrt::lod (modules/lod/include/lrt/lod/Lod.h:36-150; shaders/lrt/lod/ kernels)
+ * lrt::lod (modules/lod/include/lrt/lod/Lod.h:40-143; shaders/lrt/lod/ kernels)
  * for a cloud held whole in device memory: the reference's streamed store
  * (chunks, residency, .lrtc) is not carried. The build sorts the splats by the
  * Morton code of their position in the bounding cube and keeps, for each
@@ -260,7 +260,7 @@ lrt_status lrt_raster_render_composite(lrt_context* ctx, const lrt_projection* p
  * threshold 0 draws every splat and nothing merged. */
 typedef struct lrt_lod_cloud lrt_lod_cloud;
 
-/* LodBuildSettings (Lod.h:83-93) without the streaming chunk size. */
+/* LodBuildSettings (Lod.h:78-87) without the streaming chunk size. */
 typedef struct {
     float max_group_fraction;    /* the finest stored level's cells hold >= 1 / this splats on average */
     uint32_t coarsest_level;     /* 1 .. 10 */
@@ -268,7 +268,7 @@ typedef struct {
 
 void lrt_lod_build_settings_default(lrt_lod_build_settings* out);
 
-/* LodBuilder::build (Lod.h:95-106, Lod.cpp:79-263) on the context's stream.
+/* LodBuilder::build (Lod.h:89-99, Lod.cpp:81-283) on the context's stream.
  * Synchronous. The cloud (device pointers, as for lrt_raster_render) is read
  * only; the levels own their device memory until lrt_lod_destroy. settings
  * may be NULL (the defaults). An empty cloud is LRT_INVALID_ARGUMENT. */
@@ -276,7 +276,7 @@ lrt_status lrt_lod_build(lrt_context* ctx, const lrt_splats* cloud, const lrt_lo
                          lrt_lod_cloud** out);
 void lrt_lod_destroy(lrt_lod_cloud* lod);
 
-/* What a built cloud holds (LodCloud, Lod.h:48-81): per stored level, coarsest
+/* What a built cloud holds (LodCloud, Lod.h:51-76): per stored level, coarsest
  * first, its octree level, its first merged Gaussian and its group count. */
 typedef struct {
     uint32_t count, rest_per_colour, sh_words;
@@ -293,7 +293,7 @@ lrt_status lrt_lod_splats(const lrt_lod_cloud* lod, lrt_splats* out, const uint3
 lrt_status lrt_lod_gaussians(const lrt_lod_cloud* lod, uint32_t level_index, lrt_splats* out,
                              const uint32_t** cells);
 
-/* LodInstance (Lod.h:108-113): has_threshold 1 uses `threshold` over the call's. */
+/* LodInstance (Lod.h:101-106): has_threshold 1 uses `threshold` over the call's. */
 typedef struct {
     const lrt_lod_cloud* cloud;
     double object_to_world[16];
@@ -302,12 +302,12 @@ typedef struct {
     float threshold;
 } lrt_lod_instance;
 
-/* CutStats (Lod.h:115-126) without the streaming needs. */
+/* CutStats (Lod.h:108-117) without the streaming needs. */
 typedef struct {
     uint32_t splats, merged, available;
 } lrt_cut_stats;
 
-/* CutSelector::select (Lod.h:135-141, Lod.cpp:346-546): each instance's cut
+/* CutSelector::select (Lod.h:131-134, Lod.cpp:350-547): each instance's cut
  * for this view as splat instances for lrt_raster_render, in order, an
  * instance that draws nothing left out. out has room for `count` entries and
  * *out_count gets how many were written; stats (may be NULL) gets one entry an
@@ -320,7 +320,7 @@ lrt_status lrt_lod_select(lrt_context* ctx, const lrt_projection* projection, co
 
 /* ---- the brute force in FP64 -------------------------------------------------------
  * ReferenceRenderer (modules/render/src/ReferenceRenderer.cpp:54-160;
- * reference_project.slang:31-105, reference_blend.slang:129-223) on the GPU in
+ * reference_project.slang:31-105, reference_blend.slang:24-119) on the GPU in
  * double, without its per-pixel hit cap: every splat of every instance in
  * (view z, index) order, or with keyed in the rasteriser's (24-bit depth key,
  * slot) order, walked per pixel. colour: 4 doubles a pixel, depth: 1 (may be
@@ -461,9 +461,9 @@ lrt_status lrt_write_exr(const char* path, uint32_t width, uint32_t height, cons
                          int32_t half);
 
 /* ---- splat files --------------------------------------------------------------
- * io::RawSplats and io::SplatEncoding (modules/io/include/lrt/io/RawSplats.h:
- * 15-80): a file as one float a property a record, in the file's order, and
- * where each field sits. Modes are numbered as RawSplats.h:42-58 numbers them:
+ * io::RawSplats and io::SplatEncoding
+ * (modules/io/include/lrt/io/RawSplats.h:15-77): a file as one float a property
+ * a record, in the file's order, and where each field sits. Modes are numbered as RawSplats.h:42-58 numbers them:
  *   opacity_mode   0 logit, 1 linear, 2 byte (v / 255)
  *   scale_mode     0 log, 1 linear (log(max(v, 1e-30)))
  *   colour_mode    0 SH DC (0.5 + C0 v), 1 linear, 2 byte (v / 255)

@@ -33,7 +33,7 @@ namespace lrt::lod {
 constexpr uint32_t kLevels = 10;        // 30-bit Morton codes
 constexpr uint32_t kMomentsHead = 13;   // W, S1 (3), S2 (6), base colour (3)
 
-struct LodBuildSettings {   // Lod.h:83-93
+struct LodBuildSettings {   // Lod.h:78-87
     float maxGroupFraction = 0.5F;   // the finest stored level's cells hold >= 1 / this splats on average
     uint32_t coarsestLevel = 1;
 };
@@ -44,7 +44,7 @@ struct LodLevel {
     uint32_t level = 0, first = 0, count = 0;
 };
 
-// A cloud with its levels of detail (Lod.h:48-81, held whole on the device).
+// A cloud with its levels of detail (Lod.h:51-76, held whole on the device).
 // Owns its device memory.
 struct LodCloud {
     uint32_t count = 0, restPerColour = 0, shWords = 1;
@@ -62,24 +62,24 @@ struct LodCloud {
     SplatCloud gaussians(size_t level) const;
 };
 
-// Builds the levels of `cloud` on `stream` (Lod.cpp:79-263). Synchronous.
+// Builds the levels of `cloud` on `stream` (Lod.cpp:81-283). Synchronous.
 Status build(int device, hipStream_t stream, const SplatCloud& cloud, const LodBuildSettings& settings,
              LodCloud& out);
 
-struct LodInstance {   // Lod.h:108-113
+struct LodInstance {   // Lod.h:101-106
     const LodCloud* cloud = nullptr;
     Mat4 objectToWorld;
     SplatEditParams edit;
     std::optional<float> threshold;   // its own, over the call's
 };
 
-struct CutStats {   // Lod.h:115-126, without the streaming needs
+struct CutStats {   // Lod.h:108-117, without the streaming needs
     uint32_t splats = 0;      // the cloud's own splats drawn
     uint32_t merged = 0;      // merged Gaussians drawn
     uint32_t available = 0;   // splats in the cloud
 };
 
-// CutSelector (Lod.h:128-150, Lod.cpp:346-546): each instance's cut for a view
+// CutSelector (Lod.h:119-143, Lod.cpp:350-547): each instance's cut for a view
 // as splat instances the rasteriser draws. The clouds they point at belong to
 // the selector and are rewritten by its next call; the gathers are queued on
 // `stream` (a render on the same stream sees them), after one wait for the

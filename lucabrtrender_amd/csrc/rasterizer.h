@@ -1,6 +1,6 @@
 // The C++20 host side of the splat tile rasteriser: the shape of the
 // reference's lrt::render::TileRasterizer (modules/render/include/lrt/render/
-// TileRasterizer.h:265-322) over HIP instead of slang-rhi. It owns every
+// TileRasterizer.h:136-193) over HIP instead of slang-rhi. It owns every
 // scratch buffer (grown by half again from 1024 and never shrunk, as the
 // reference's reserve* do, TileRasterizer.cpp:26-32,105-177), queues the
 // kernels of kernels.h on one stream and waits where the reference waits.
@@ -37,7 +37,7 @@ Mat4 rotationZ(double degrees);
 Mat4 lookAt(const Vec3& eye, const Vec3& target, const Vec3& up);
 Vec3 transformPoint(const Mat4& m, const Vec3& p);
 
-struct Lens {   // modules/render/include/lrt/render/Camera.h:73-94
+struct Lens {   // modules/render/include/lrt/render/Camera.h:23-44
     bool orthographic = false;
     double focal = 50.0, haperture = 24.576, vaperture = 18.672, nearZ = 0.1, farZ = 10000.0;
     double windowTranslate[2] = {0.0, 0.0};
@@ -45,13 +45,13 @@ struct Lens {   // modules/render/include/lrt/render/Camera.h:73-94
     double windowRoll = 0.0;
 };
 
-struct Camera {   // Camera.h:96-103
+struct Camera {   // Camera.h:46-53
     Mat4 cameraToWorld;
     Lens lens;
     static Camera lookingAt(const Vec3& eye, const Vec3& target, const Vec3& up = {0.0, 1.0, 0.0});
 };
 
-struct Projection {   // Camera.h:106-115
+struct Projection {   // Camera.h:56-80
     Mat4 worldToView;
     double focalX = 0, focalY = 0, centreX = 0, centreY = 0, nearZ = 0.1, farZ = 10000.0;
     bool orthographic = false;
@@ -95,7 +95,7 @@ struct PointInstance {   // render::PointInstance (Points.h:35-39)
     PointStyle style;
 };
 
-struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:234-247)
+struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:105-118)
     uint32_t width = 1920, height = 1080;
     bool antialias = true;
     uint32_t maxShDegree = 3;
@@ -120,7 +120,7 @@ struct RenderTargets {   // caller-owned device images, bottom row first
     float* depth = nullptr;
 };
 
-struct FrameStats {   // render::FrameStats (TileRasterizer.h:249-255)
+struct FrameStats {   // render::FrameStats (TileRasterizer.h:120-126)
     uint32_t splats = 0, visible = 0, pairs = 0;
     // The sort's key: tile bits plus the bits of the frame's depth-key range, and its passes.
     uint32_t keyBits = 0, sortPasses = 0;

@@ -29,7 +29,7 @@ def _mat(m) -> "C.Array":
 
 @dataclass
 class Lens:
-    """render::Lens (Camera.h:73-94); defaults are the reference's."""
+    """render::Lens (Camera.h:23-44); defaults are the reference's."""
     orthographic: bool = False
     focal: float = 50.0
     haperture: float = 24.576
@@ -53,7 +53,7 @@ class Lens:
 
 @dataclass
 class Camera:
-    """render::Camera (Camera.h:96-103)."""
+    """render::Camera (Camera.h:46-53)."""
     camera_to_world: tuple = IDENTITY
     lens: Lens = field(default_factory=Lens)
 
@@ -82,7 +82,7 @@ def projection_for(camera: Camera, width: int, height: int) -> _abi.ProjectionC:
 
 @dataclass
 class SplatEdit:
-    """render::SplatEdit (TileRasterizer.h:158-179)."""
+    """render::SplatEdit (TileRasterizer.h:29-50)."""
     active: bool = False
     shape: int = 0
     mode: int = 2
@@ -108,7 +108,7 @@ class SplatEdit:
 
 @dataclass
 class RenderSettings:
-    """render::RenderSettings (TileRasterizer.h:234-247) + dilation / filter3d."""
+    """render::RenderSettings (TileRasterizer.h:105-118) + dilation / filter3d."""
     width: int = 1920
     height: int = 1080
     antialias: bool = True
@@ -287,7 +287,7 @@ class PointInstance:
 
 @dataclass
 class SplatInstance:
-    """render::SplatInstance (TileRasterizer.h:181-196), without relighting."""
+    """render::SplatInstance (TileRasterizer.h:52-67), without relighting."""
     splats: Optional[GpuSplats]
     object_to_world: tuple = IDENTITY
     edit: SplatEdit = field(default_factory=SplatEdit)
@@ -295,7 +295,7 @@ class SplatInstance:
 
 @dataclass
 class FrameStats:
-    """render::FrameStats (TileRasterizer.h:249-255)."""
+    """render::FrameStats (TileRasterizer.h:120-126)."""
     splats: int = 0
     visible: int = 0
     pairs: int = 0
@@ -437,7 +437,7 @@ def _instances_c(instances: Sequence[SplatInstance]):
 
 
 class TileRasterizer:
-    """render::TileRasterizer (TileRasterizer.h:265-322) on one HIP stream."""
+    """render::TileRasterizer (TileRasterizer.h:136-193) on one HIP stream."""
 
     def __init__(self, device: int = 0, stream=None):
         import torch
@@ -630,7 +630,7 @@ class TileRasterizer:
         return out
 
     def lod_build(self, splats, settings: Optional["LodBuildSettings"] = None) -> "LodCloud":
-        """LodBuilder::build (Lod.cpp:79-263): the cloud's levels of detail."""
+        """LodBuilder::build (Lod.cpp:81-283): the cloud's levels of detail."""
         c = (settings or LodBuildSettings()).to_c()
         h = C.c_void_p()
         self._check(lib.lrt_lod_build(self._ctx, C.byref(splats.c), C.byref(c), C.byref(h)))
@@ -638,7 +638,7 @@ class TileRasterizer:
 
     def lod_select(self, camera_or_projection, instances: Sequence["LodInstance"], threshold: float,
                    settings: RenderSettings):
-        """CutSelector::select (Lod.cpp:346-546): (splat instances to render, one
+        """CutSelector::select (Lod.cpp:350-547): (splat instances to render, one
         CutStats an instance). The instances' clouds are the context's and are
         rewritten by the next call."""
         proj = camera_or_projection
@@ -665,7 +665,7 @@ class TileRasterizer:
 
 @dataclass
 class LodBuildSettings:
-    """LodBuildSettings (modules/lod/include/lrt/lod/Lod.h:83-93) without chunks."""
+    """LodBuildSettings (modules/lod/include/lrt/lod/Lod.h:78-87) without chunks."""
     max_group_fraction: float = 0.5
     coarsest_level: int = 1
 
@@ -675,7 +675,7 @@ class LodBuildSettings:
 
 @dataclass
 class CutStats:
-    """CutStats (Lod.h:115-126) without the streaming needs."""
+    """CutStats (Lod.h:108-117) without the streaming needs."""
     splats: int = 0
     merged: int = 0
     available: int = 0
@@ -703,7 +703,7 @@ class SplatsView:
 
 
 class LodCloud:
-    """A cloud with its levels of detail (LodCloud, Lod.h:48-81), owned by the library."""
+    """A cloud with its levels of detail (LodCloud, Lod.h:51-76), owned by the library."""
 
     def __init__(self, handle: C.c_void_p, rast: "TileRasterizer"):
         self._h = handle
@@ -753,7 +753,7 @@ class LodCloud:
 
 @dataclass
 class LodInstance:
-    """LodInstance (Lod.h:108-113): threshold None takes the call's."""
+    """LodInstance (Lod.h:101-106): threshold None takes the call's."""
     cloud: Optional[LodCloud]
     object_to_world: tuple = IDENTITY
     edit: SplatEdit = field(default_factory=SplatEdit)

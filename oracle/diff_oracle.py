@@ -10,7 +10,7 @@ so torch.autograd gives its exact FP64 gradient:
   cutoff min(2 ln(max(255 alpha, 1.0001)), 9)) from FP32 parameters: scale
   exp(log s), opacity sigmoid(logit), unit quaternion q / |q|, colour
   max(0.5 + C0 dc + rest(dir), 0) with sh.slang:25-50's basis;
-- walk: reference_blend.slang:129-223 over every splat in the rasteriser's
+- walk: reference_blend.slang:24-119 over every splat in the rasteriser's
   keyed order (bits(float z) >> 7, then index): skip past the cutoff, alpha =
   min(0.99, opacity exp(-power / 2)), skip below 1/255, stop once T < 1e-4,
   then linearise and add the background.

@@ -567,7 +567,7 @@ static float evaluate_rest(const uint32_t* row, uint32_t degree, float x, float 
     return result;
 }
 
-/* sh.slang:227-229 */
+/* sh.slang:89-91 */
 static inline uint32_t degree_of(uint32_t rest) {
     return rest >= 15 ? 3 : rest >= 8 ? 2 : rest >= 3 ? 1 : 0;
 }
@@ -1457,7 +1457,7 @@ static void ref_points_project_f32(const or_frame* f, const or_points* pts, uint
     }
 }
 
-/* The walk of reference_blend.slang:129-223, over every splat in (z, index)
+/* The walk of reference_blend.slang:24-119, over every splat in (z, index)
  * order: the same sequence its per-pixel insertion sort builds. */
 static void reference_f32(const or_frame* frames, const or_cloud* clouds, uint32_t instances,
                           const or_frame* point_frames, const or_points* points, uint32_t point_instances,

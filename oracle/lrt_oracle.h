@@ -256,7 +256,7 @@ typedef struct {
 } or_diff_cloud;
 /* The brute force of such a cloud in FP64 (frame.base must be 0): scales
  * exp(log s), opacity sigmoid(logit), base colour 0.5 + C0 dc, then
- * reference_project.slang:31-105 and the walk of reference_blend.slang:129-223. */
+ * reference_project.slang:31-105 and the walk of reference_blend.slang:24-119. */
 void or_diff_render_f64(const or_frame* frame, const or_diff_cloud* cloud, const uint32_t* px, const uint32_t* py,
                         uint32_t npix, double* colour, double* depth, int keyed, int threads);
 

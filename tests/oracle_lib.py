@@ -374,7 +374,7 @@ class OrCutParams(C.Structure):
 
 
 def lod_build(positions, shape, sh, rest_per_colour, max_group_fraction=0.5, coarsest_level=1):
-    """LodBuilder::build (modules/lod/src/Lod.cpp:79-263) over the oracle's stages:
+    """LodBuilder::build (modules/lod/src/Lod.cpp:81-283) over the oracle's stages:
     bounds and extent, Morton keys and the stable 30-bit sort, the splats in
     that order, the finest level worth storing, then from it up each stored
     level's groups, moments and Gaussians. Returns a dict of host arrays."""
@@ -440,7 +440,7 @@ def lod_build(positions, shape, sh, rest_per_colour, max_group_fraction=0.5, coa
 
 
 def lod_cut(lod, eye, pixels_per_unit, threshold, orthographic=False):
-    """CutSelector::select for one instance held in memory (Lod.cpp:346-546):
+    """CutSelector::select for one instance held in memory (Lod.cpp:350-547):
     every level's selected Gaussians (coarsest first), then the selected splats
     in Morton order. Returns (positions, shape, sh, merged drawn, splats drawn)."""
     p = OrCutParams()
