@@ -189,6 +189,40 @@ def render(frame, means, log_scales, quats, logits, sh, filter3d=None, chunk: in
     return torch.cat(out).reshape(frame.height, frame.width, 4)
 
 
+def decisions(frame, params: dict, filter3d=None, chunk: int = 2048):
+    """The walk's discrete choices for float32 parameters: (the visible splats'
+    keyed order, a [pixels, visible] bool of the pairs that pass the cutoff and
+    alpha-floor tests and come before the T < 1e-4 stop). The image is smooth
+    in the parameters wherever these stay the same; a finite difference whose
+    two ends differ here straddles a step of the forward."""
+    names = ("means", "log_scales", "quats", "opacity_logits", "sh")
+    with torch.no_grad():
+        t = [torch.from_numpy(np.asarray(params[k], dtype=np.float32)) for k in names]
+        f3 = torch.from_numpy(np.asarray(filter3d, dtype=np.float32)) if filter3d is not None else None
+        rec = project(frame, *t, f3)
+        order = keyed_order(rec)
+        ys, xs = torch.meshgrid(torch.arange(frame.height), torch.arange(frame.width), indexing="ij")
+        xs, ys = xs.reshape(-1), ys.reshape(-1)
+        masks = []
+        for k in range(0, xs.numel(), chunk):
+            px = xs[k:k + chunk].to(torch.float64)[:, None] + 0.5
+            py = ys[k:k + chunk].to(torch.float64)[:, None] + 0.5
+            dx = px - rec["cx"][order][None, :]
+            dy = py - rec["cy"][order][None, :]
+            power = rec["a"][order] * dx * dx + 2.0 * rec["b"][order] * dx * dy + rec["c"][order] * dy * dy
+            a_pix = torch.clamp(rec["alpha"][order][None, :] * torch.exp(-0.5 * power), max=0.99)
+            ok = (power <= rec["cutoff"][order][None, :]) & (a_pix >= 1.0 / 255.0)
+            a_eff = torch.where(ok, a_pix, torch.zeros_like(a_pix))
+            ones = torch.ones((a_eff.shape[0], 1), dtype=torch.float64)
+            t_before = torch.cat([ones, torch.cumprod(1.0 - a_eff, dim=1)[:, :-1]], dim=1)
+            masks.append((ok & (t_before >= 1e-4)).numpy())
+    return order.numpy(), np.concatenate(masks)
+
+
+def same_decisions(a, b) -> bool:
+    return a[0].shape == b[0].shape and np.array_equal(a[0], b[0]) and np.array_equal(a[1], b[1])
+
+
 def gradients(frame, params: dict, grad_colour: np.ndarray, filter3d=None, chunk: int = 1024) -> dict:
     """d(sum(colour * grad_colour))/d(parameter) in FP64 by autodiff, per parameter name.
 

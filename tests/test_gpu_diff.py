@@ -100,30 +100,46 @@ def test_forward_and_gradients_against_the_fp64_oracle(rast, case):
 
 
 def test_gradients_against_fp64_central_differences(rast):
+    """The 8 largest gradient components of each group against FP64 central
+    differences of the C brute force. The forward is piecewise smooth: a pixel
+    whose power crosses a splat's cutoff (capped at 9, where alpha is still
+    above 1/255), the alpha floor or the T < 1e-4 stop makes a step. A
+    difference whose two ends change one of those choices (D.decisions) is
+    excluded, as shown to straddle a step; the rest must agree to 1e-3 at worst."""
     p = scenes.diff_params(1024, 21, degree=3, log_scale=(-3.0, 0.4))
     n = p["means"].shape[0]
     s = raster.RenderSettings(width=W, height=HGT)
     g = np.random.default_rng(7).normal(0.0, 1.0, (HGT, W, 4))
     _, grads = _gpu_grads(rast, p, s, g)
     frame = _oracle_frame(s, n, 15)
+    base_choices = D.decisions(frame, p)
 
     def loss(q):
         return float((O.diff_render_f64(frame, q) * g).sum())
 
+    excluded = 0
     for name in GROUPS:
         flat = grads[name].reshape(-1)
-        rel = []
+        rel, steps = [], []
         for idx in np.argsort(-np.abs(flat))[:8]:
             base = p[name].reshape(-1)[idx]
             h = 1e-5 * max(1.0, abs(float(base)))
             plus, minus = {k: v.copy() for k, v in p.items()}, {k: v.copy() for k, v in p.items()}
             plus[name].reshape(-1)[idx] = np.float32(base + h)
             minus[name].reshape(-1)[idx] = np.float32(base - h)
+            if not (D.same_decisions(D.decisions(frame, plus), base_choices) and
+                    D.same_decisions(D.decisions(frame, minus), base_choices)):
+                steps.append(int(idx))
+                continue
             step = float(plus[name].reshape(-1)[idx]) - float(minus[name].reshape(-1)[idx])
             fd = (loss(plus) - loss(minus)) / step
             rel.append(abs(float(flat[idx]) - fd) / abs(fd))
-        print(f"  {name:15s} vs FP64 central differences: median {np.median(rel):.2e} max {np.max(rel):.2e}")
-        assert np.median(rel) < 1e-2, (name, rel)
+        excluded += len(steps)
+        print(f"  {name:15s} vs FP64 central differences: {len(rel)} smooth, max {np.max(rel):.2e} "
+              f"median {np.median(rel):.2e}; straddling a step: {steps}")
+        assert len(rel) >= 6, (name, steps)
+        assert np.max(rel) < 1e-3, (name, rel)
+    assert excluded <= 4
 
 
 def test_autograd_wrapper_matches_the_c_abi(rast):

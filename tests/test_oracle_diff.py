@@ -96,3 +96,43 @@ def test_autodiff_matches_central_differences(linearise, filter3d):
         # the agreement is at the level of the differences' own rounding.
         assert np.median(rel) < 1e-7 and rel.max() < 1e-5, (name, rel)
     assert checked == 12 * len(GROUPS)
+
+
+def test_the_step_detector_flags_the_difference_that_straddles_a_cutoff():
+    """The differentiable frame's FD sub-scene at 128 x 96: the y of splat 18's
+    mean sits 4.3e-5 in power outside the 3-sigma cutoff cap (power 9) at
+    pixel (64, 31), where its alpha is still 2.3/255. A step of -1e-5 brings
+    that pixel in: D.decisions must change there and only there, the one-sided
+    difference on that side must grow as 1/h (a jump, not a slope), and on the
+    smooth side it must equal the autodiff gradient."""
+    w, h = 128, 96
+    p = scenes.diff_params(1024, 21, degree=3, log_scale=(-3.0, 0.4))
+    cam = O.camera(O.look_at((0.0, 0.0, 0.0), (0.0, 0.0, -1.0)), focal=scenes.focal_for_fov(60.0))
+    frame = O.diff_frame(O.projection_for(cam, w, h), O.settings(w, h), 1024, 15)
+    g = np.random.default_rng(7).normal(0.0, 1.0, (h, w, 4))
+    base_choices = D.decisions(frame, p)
+    ad = float(D.gradients(frame, p, g)["means"].reshape(-1)[55])
+
+    def moved(step):
+        q = {k: v.copy() for k, v in p.items()}
+        q["means"].reshape(-1)[55] = np.float32(p["means"].reshape(-1)[55] + step)
+        return q
+
+    def loss(q):
+        return float((O.diff_render_f64(frame, q) * g).sum())
+
+    down = D.decisions(frame, moved(-1e-5))
+    assert np.array_equal(down[0], base_choices[0])
+    flips = np.argwhere(down[1] != base_choices[1])
+    assert [(int(a) % w, int(a) // w, int(base_choices[0][b])) for a, b in flips] == [(64, 31, 18)]
+    assert D.same_decisions(D.decisions(frame, moved(1e-5)), base_choices)
+    l0 = loss(p)
+    jumps = []
+    for step in (1e-5, 1e-6):
+        lo, hi = moved(-step), moved(step)
+        below = (l0 - loss(lo)) / (float(p["means"].reshape(-1)[55]) - float(lo["means"].reshape(-1)[55]))
+        above = (loss(hi) - l0) / (float(hi["means"].reshape(-1)[55]) - float(p["means"].reshape(-1)[55]))
+        assert abs(above - ad) < 1e-3 * abs(ad), (above, ad)
+        jumps.append((below - ad) * step)
+    # the same jump in the loss at both step sizes: a step of the forward
+    assert abs(jumps[0]) > 1e-3 and abs(jumps[1] / jumps[0] - 1.0) < 0.05, jumps
