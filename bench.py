@@ -247,24 +247,9 @@ def views_leg(torch, raster, scenes, reps):
     12 looking at its centre, each view's size drawn from {256x256, 512x384, 960x540,
     1280x720, 1920x1080}. The batch (one launch sequence, autotuned) against the plain
     sequence of 64 single-view frames."""
-    import math
-
-    import numpy as np
     cfg = scenes.workload("scene-fwd-3M-1080p", count=2_000_000)
     inst = [raster.SplatInstance(cfg.scene.to_device())]
-    rng = np.random.default_rng(4)
-    sizes = [(256, 256), (512, 384), (960, 540), (1280, 720), (1920, 1080)]
-    views = []
-    golden = math.pi * (3.0 - math.sqrt(5.0))
-    for k in range(64):
-        y = 1.0 - 2.0 * (k + 0.5) / 64.0
-        r = math.sqrt(1.0 - y * y)
-        eye = (12.0 * r * math.cos(golden * k), 12.0 * y, 12.0 * r * math.sin(golden * k))
-        up = (0.0, 0.0, 1.0) if abs(y) > 0.95 else (0.0, 1.0, 0.0)
-        cam = raster.Camera.looking_at(eye, (0.0, 0.0, 0.0), up)
-        cam.lens = raster.Lens(focal=scenes.focal_for_fov(60.0))
-        w, h = sizes[int(rng.integers(0, len(sizes)))]
-        views.append(raster.View(cam, raster.RenderTargets(w, h)))
+    views = [raster.View(cam, raster.RenderTargets(w, h)) for cam, (w, h) in scenes.sphere_views(64, 12.0, seed=4)]
     settings = raster.RenderSettings(dilation=cfg.dilation, filter3d=cfg.filter3d)
     out = _batch_against_sequence(torch, raster, raster.TileRasterizer(), views, inst, settings, reps)
     return {"workload": "views-fwd-2M-64", "splats": 2_000_000, **out}

@@ -398,6 +398,24 @@ lrt_status lrt_raster_render_diff(lrt_context* ctx, const lrt_projection* projec
 lrt_status lrt_raster_backward(lrt_context* ctx, const float* grad_colour, const lrt_diff_grads* grads,
                                float* millis);
 
+/* Many views of one FP32-parameter cloud in one launch sequence with one wait
+ * (no counterpart in the reference, which draws one camera a render() call and
+ * has no backward): each view's images are those lrt_raster_render_diff would
+ * draw for it alone, bit for bit, and the context keeps what
+ * lrt_raster_backward_views needs until its next render. settings.width/height
+ * are ignored (each view's targets give its size); settings.use_graph captures
+ * the sequence as a hipGraph once per batch shape and replays it. */
+lrt_status lrt_raster_render_diff_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
+                                        const lrt_diff_splats* splats, const double object_to_world[16],
+                                        const lrt_render_settings* settings, lrt_frame_stats* stats);
+/* Gradients of the last lrt_raster_render_diff_views' parameters, summed over
+ * its views in view order (no atomics): grad_colours is a host array of
+ * view_count device pointers, view v's dL/d(colour) (one float4 a pixel of
+ * that view, bottom row first). view_count must be the batch's. millis as
+ * lrt_raster_backward's. Synchronous. */
+lrt_status lrt_raster_backward_views(lrt_context* ctx, const float* const* grad_colours, uint32_t view_count,
+                                     const lrt_diff_grads* grads, float* millis);
+
 /* What the last render left on the device, copied to host, for bit-exact
  * checks. *needed (may be NULL) receives the byte size; bytes may be 0 to ask. */
 typedef enum {

@@ -210,6 +210,11 @@ SIGNATURES = {
                                          C.POINTER(C.c_double), C.POINTER(RenderSettingsC),
                                          C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
     "lrt_raster_backward": (C.c_int, [C.c_void_p, C.c_void_p, C.POINTER(DiffGradsC), C.POINTER(C.c_float)]),
+    "lrt_raster_render_diff_views": (C.c_int, [C.c_void_p, C.POINTER(ViewC), C.c_uint32, C.POINTER(DiffSplatsC),
+                                               C.POINTER(C.c_double), C.POINTER(RenderSettingsC),
+                                               C.POINTER(FrameStatsC)]),
+    "lrt_raster_backward_views": (C.c_int, [C.c_void_p, C.POINTER(C.c_void_p), C.c_uint32, C.POINTER(DiffGradsC),
+                                            C.POINTER(C.c_float)]),
     "lrt_raster_debug": (C.c_int, [C.c_void_p, C.c_int32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "lrt_frame_params": (C.c_int, [C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.POINTER(RenderSettingsC),
                                    C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),

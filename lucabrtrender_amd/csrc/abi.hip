@@ -530,6 +530,40 @@ lrt_status lrt_raster_backward(lrt_context* ctx, const float* grad_colour, const
     return finish(ctx, ctx->raster.backward(reinterpret_cast<const float4*>(grad_colour), g, millis));
 }
 
+lrt_status lrt_raster_render_diff_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
+                                        const lrt_diff_splats* splats, const double object_to_world[16],
+                                        const lrt_render_settings* settings, lrt_frame_stats* stats) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (splats == nullptr || settings == nullptr) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    std::vector<lrt::View> list;
+    std::vector<lrt::SplatCloud> clouds;
+    std::vector<lrt::SplatInstance> inst;
+    lrt::Status s = toBatch(views, view_count, nullptr, 0, list, clouds, inst);
+    if (!s.ok()) return finish(ctx, s);
+    const lrt::kern::DiffCloud cloud{splats->means, splats->log_scales, splats->quats, splats->opacity_logits,
+                                     splats->sh, splats->filter3d, splats->count, splats->rest_per_colour};
+    lrt::Mat4 o2w;
+    if (object_to_world != nullptr) o2w = toMat(object_to_world);
+    lrt::FrameStats fs;
+    s = ctx->raster.renderDiffViews(list, cloud, o2w, toSettings(*settings), &fs);
+    if (s.ok()) toStats(fs, stats);
+    return finish(ctx, s);
+}
+
+lrt_status lrt_raster_backward_views(lrt_context* ctx, const float* const* grad_colours, uint32_t view_count,
+                                     const lrt_diff_grads* grads, float* millis) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (grads == nullptr || (view_count > 0 && grad_colours == nullptr)) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    std::vector<const float4*> list(view_count);
+    for (uint32_t v = 0; v < view_count; ++v) list[v] = reinterpret_cast<const float4*>(grad_colours[v]);
+    const lrt::kern::DiffGrads g{grads->means, grads->log_scales, grads->quats, grads->opacity_logits, grads->sh};
+    return finish(ctx, ctx->raster.backwardViews(list, g, millis));
+}
+
 lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed) {
     if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
     return finish(ctx, ctx->raster.debugCopy(which, host, bytes, needed));

@@ -391,16 +391,6 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
                                                                finalT, accum);
 }
 
-// The view holding batch tile g: the last view whose tiles start at or before it.
-__device__ __forceinline__ uint32_t viewOfTile(const ViewInfo* __restrict__ views, uint32_t viewCount, uint32_t g) {
-    uint32_t lo = 0, hi = viewCount;   // views[lo].tileBase <= g < views[hi].tileBase
-    while (hi - lo > 1) {
-        const uint32_t mid = (lo + hi) >> 1;
-        if (views[mid].tileBase <= g) lo = mid; else hi = mid;
-    }
-    return lo;
-}
-
 // Every quadrant of every view of a batch. As a grid (PERSISTENT false) block b
 // is task b, the four quadrants of a tile going to blocks b, b + 8, b + 16,
 // b + 24 of one XCD; persistent, each block takes tasks from a shared counter
@@ -414,7 +404,8 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
                                                            const uint32_t* __restrict__ pairSlots,
                                                            const float4* __restrict__ proj,
                                                            uint32_t* __restrict__ lastCount,
-                                                           float* __restrict__ finalT, uint32_t* __restrict__ queue) {
+                                                           float* __restrict__ finalT, float4* __restrict__ accum,
+                                                           uint32_t* __restrict__ queue) {
     __shared__ BlendList<LIST> list;
     for (uint32_t round = 0;; ++round) {
         uint32_t task, g, quadrant;
@@ -439,7 +430,8 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
                                                    pairSlots, proj, nullptr, nullptr,
                                                    reinterpret_cast<float4*>(v.colour), v.depth,
                                             lastCount != nullptr ? lastCount + v.pixelBase : nullptr,
-                                            finalT != nullptr ? finalT + v.pixelBase : nullptr, nullptr);
+                                            finalT != nullptr ? finalT + v.pixelBase : nullptr,
+                                            accum != nullptr ? accum + v.pixelBase : nullptr);
         }
         if constexpr (!PERSISTENT) return;
     }
@@ -742,7 +734,8 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
 template <uint32_t DEPTH_MODE, uint32_t LIST>
 void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* params, const ViewInfo* views,
                       uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges, const uint32_t* pairSlots,
-                      const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue, hipStream_t stream) {
+                      const float4* proj, uint32_t* lastCount, float* finalT, float4* accum, uint32_t* queue,
+                      hipStream_t stream) {
     const auto go = [&](auto track) {
         constexpr bool T = decltype(track)::value;
         if (persistent) {
@@ -750,11 +743,11 @@ void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* para
             const uint32_t grid = tasks < gridCap ? tasks : gridCap;
             fill32(queue, 0u, 1, stream);
             blendViewsKernel<DEPTH_MODE, LIST, true, T><<<grid, kLanes, 0, stream>>>(
-                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
+                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, accum, queue);
         } else {
             const uint32_t tasks = (tiles + 7u) / 8u * 32u;
             blendViewsKernel<DEPTH_MODE, LIST, false, T><<<tasks, kLanes, 0, stream>>>(
-                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, queue);
+                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, accum, queue);
         }
     };
     if (lastCount != nullptr) go(std::true_type{}); else go(std::false_type{});
@@ -764,24 +757,24 @@ void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* para
 
 void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gridCap, const FrameParams* params,
                 const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges,
-                const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue,
-                hipStream_t stream) {
+                const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
+                uint32_t* queue, hipStream_t stream) {
     if (tiles == 0 || viewCount == 0) return;
     if (depthMode == 0) {
         if (list <= 128) {
             launchBlendViews<0, 128>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, queue, stream);
+                                     proj, lastCount, finalT, accum, queue, stream);
         } else {
             launchBlendViews<0, 256>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, queue, stream);
+                                     proj, lastCount, finalT, accum, queue, stream);
         }
     } else {
         if (list <= 128) {
             launchBlendViews<1, 128>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, queue, stream);
+                                     proj, lastCount, finalT, accum, queue, stream);
         } else {
             launchBlendViews<1, 256>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, queue, stream);
+                                     proj, lastCount, finalT, accum, queue, stream);
         }
     }
 }

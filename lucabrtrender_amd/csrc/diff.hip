@@ -72,6 +72,18 @@ __device__ __forceinline__ Decoded decode(const DiffCloud& c, uint32_t i, bool u
     return d;
 }
 
+// The splat's harmonic coefficients its frames use: dc rgb, and the rest
+// bases up to the degree (zero past it).
+__device__ __forceinline__ void loadHarmonics(const DiffCloud& c, uint32_t i, uint32_t degree, float dc[3],
+                                              float coeff[45]) {
+    const float* row = c.sh + static_cast<uint64_t>(i) * ((1u + c.restPerColour) * 3u);
+    const uint32_t used = restCount(degree) * 3u;
+#pragma unroll
+    for (int ch = 0; ch < 3; ++ch) dc[ch] = row[ch];
+#pragma unroll
+    for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
+}
+
 static_assert(kBlock == kProjectBlock, "one depth-range entry a workgroup of kProjectBlock slots");
 
 __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffCloud c, float4* __restrict__ proj,
@@ -107,6 +119,56 @@ __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffC
     if (!live) return;
     rects[slot] = touched > 0 ? rect : emptyRect;
     touchedOut[slot] = touched;
+}
+
+constexpr uint32_t kViewsStaged = 32;   // view constants held in LDS at a time
+static_assert(sizeof(FrameParams) % 4 == 0, "FrameParams is staged as words");
+
+// One cloud seen by many views, splat-major: each thread decodes its splat and
+// reads its 48 harmonic coefficients once, then projects it for every view
+// (the views' constants staged in LDS), each view's colour by the same
+// arithmetic as projectDiffKernel's.
+__global__ __launch_bounds__(kBlock) void projectDiffViewsKernel(const FrameParams* __restrict__ params,
+                                                                 uint32_t views, DiffCloud c,
+                                                                 float4* __restrict__ proj,
+                                                                 uint2* __restrict__ rects,
+                                                                 uint32_t* __restrict__ touchedOut) {
+    constexpr uint32_t kWords = sizeof(FrameParams) / 4;
+    __shared__ uint32_t sFWords[kViewsStaged * kWords];   // FrameParams as words (no constructor in LDS)
+    const FrameParams* sF = reinterpret_cast<const FrameParams*>(sFWords);
+    const FrameParams& f0 = params[0];   // what every view shares: the filter, the harmonics' degree
+    const uint32_t tid = threadIdx.x;
+    const uint32_t i = blockIdx.x * kBlock + tid;
+    const bool live = i < c.count;   // every lane stays for the wave's tile counts
+    const Decoded d = decode(c, live ? i : 0u, f0.useFilter3d != 0);
+    const uint32_t degree = shDegree(f0);
+    float dc[3], coeff[45];
+    loadHarmonics(c, live ? i : 0u, degree, dc, coeff);
+    const auto colourOf = [&](float dx, float dy, float dz, float colour[3]) {
+        float rest[3];
+        evaluateRest(coeff, degree, dx, dy, dz, rest);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) colour[ch] = fmaxS((0.5f + kShC0 * dc[ch]) + rest[ch], 0.0f);
+    };
+    const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
+    for (uint32_t v0 = 0; v0 < views; v0 += kViewsStaged) {
+        const uint32_t n = views - v0 < kViewsStaged ? views - v0 : kViewsStaged;
+        __syncthreads();   // the last group's constants are done with
+        const uint32_t* src = reinterpret_cast<const uint32_t*>(params + v0);
+        for (uint32_t w = tid; w < n * kWords; w += kBlock) sFWords[w] = src[w];
+        __syncthreads();
+        for (uint32_t v = 0; v < n; ++v) {
+            const FrameParams& f = sF[v];
+            const uint32_t slot = f.base + i;
+            uint2 rect = emptyRect;
+            float tz;
+            const uint32_t touched =
+                projectWave(f, d.shape, live, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect, &tz);
+            if (!live) continue;
+            rects[slot] = touched > 0 ? rect : emptyRect;
+            touchedOut[slot] = touched;
+        }
+    }
 }
 
 __global__ __launch_bounds__(kBlock) void gatherSlotsKernel(const uint32_t* __restrict__ pairIndex,
@@ -158,16 +220,18 @@ __device__ __forceinline__ void srgbToLinearGrad(float x, float& value, float& s
 constexpr uint32_t kWaves = 4;
 constexpr uint32_t kBatch = 128;   // pairs staged per step, by the first 128 threads
 
-__global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uint32_t slots,
-                                                              const uint2* __restrict__ ranges,
-                                                              const uint32_t* __restrict__ sortedSlots,
-                                                              const uint32_t* __restrict__ pairIndex,
-                                                              const float4* __restrict__ proj,
-                                                              const float4* __restrict__ accum,
-                                                              const uint32_t* __restrict__ lastCount,
-                                                              const float* __restrict__ finalT,
-                                                              const float4* __restrict__ gradColour,
-                                                              float4* __restrict__ pairGrad) {
+// One tile of frame f (its pairs [range.x, range.y) of the sorted order): the
+// workgroup's four waves take its four quadrants. accum, lastCount, finalT and
+// gradColour are the frame's own per-pixel arrays.
+__device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t tile, uint2 range, uint32_t slots,
+                                                  const uint32_t* __restrict__ sortedSlots,
+                                                  const uint32_t* __restrict__ pairIndex,
+                                                  const float4* __restrict__ proj,
+                                                  const float4* __restrict__ accum,
+                                                  const uint32_t* __restrict__ lastCount,
+                                                  const float* __restrict__ finalT,
+                                                  const float4* __restrict__ gradColour,
+                                                  float4* __restrict__ pairGrad) {
     __shared__ float4 sGeom[kBatch];     // centre.xy, cutoff, alpha
     __shared__ float4 sConic[kBatch];    // a, 2b, c, (unused)
     __shared__ float4 sColour[kBatch];   // rgb, bits of the pair's place in the tile's range
@@ -180,13 +244,11 @@ __global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uin
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
     const uint32_t wave = tid >> 6;
-    const uint32_t tile = blockIdx.x;
     const uint32_t tileX = tile % f.tilesX;
     const uint32_t tileY = tile / f.tilesX;
     const uint32_t px = tileX * kTile + (wave & 1u) * 8u + (lane & 7u);
     const uint32_t py = tileY * kTile + (wave >> 1) * 8u + (lane >> 3);
     const bool inside = px < f.width && py < f.height;
-    const uint2 range = ranges[tile];
     const float pxf = static_cast<float>(px) + 0.5f;
     const float pyf = static_cast<float>(py) + 0.5f;
     float boxLo[kWaves][2], boxHi[kWaves][2];
@@ -378,6 +440,43 @@ __global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uin
     }
 }
 
+__global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uint32_t slots,
+                                                              const uint2* __restrict__ ranges,
+                                                              const uint32_t* __restrict__ sortedSlots,
+                                                              const uint32_t* __restrict__ pairIndex,
+                                                              const float4* __restrict__ proj,
+                                                              const float4* __restrict__ accum,
+                                                              const uint32_t* __restrict__ lastCount,
+                                                              const float* __restrict__ finalT,
+                                                              const float4* __restrict__ gradColour,
+                                                              float4* __restrict__ pairGrad) {
+    blendBackwardTile(f, blockIdx.x, ranges[blockIdx.x], slots, sortedSlots, pairIndex, proj, accum, lastCount,
+                      finalT, gradColour, pairGrad);
+}
+
+// Every tile of a batch of views: workgroup g takes batch tile g, in its view's
+// frame and per-pixel arrays (ViewInfo.pixelBase), with that view's colour
+// gradient.
+__global__ __launch_bounds__(kBlock) void blendBackwardViewsKernel(const FrameParams* __restrict__ params,
+                                                                   const ViewInfo* __restrict__ views,
+                                                                   uint32_t viewCount, uint32_t slots,
+                                                                   const uint2* __restrict__ ranges,
+                                                                   const uint32_t* __restrict__ sortedSlots,
+                                                                   const uint32_t* __restrict__ pairIndex,
+                                                                   const float4* __restrict__ proj,
+                                                                   const float4* __restrict__ accum,
+                                                                   const uint32_t* __restrict__ lastCount,
+                                                                   const float* __restrict__ finalT,
+                                                                   const float4* const* __restrict__ gradColours,
+                                                                   float4* __restrict__ pairGrad) {
+    const uint32_t g = blockIdx.x;
+    const uint32_t v = viewOfTile(views, viewCount, g);
+    const ViewInfo& info = views[v];
+    const FrameParams f = params[info.params];
+    blendBackwardTile(f, g - info.tileBase, ranges[g], slots, sortedSlots, pairIndex, proj, accum + info.pixelBase,
+                      lastCount + info.pixelBase, finalT + info.pixelBase, gradColours[v], pairGrad);
+}
+
 // ---- preprocess backward ------------------------------------------------------
 
 // The rest of the colour and its gradients: given dL/d(rest) per channel,
@@ -433,43 +532,36 @@ __device__ __forceinline__ void evaluateRestGrad(const float* k, uint32_t degree
     }
 }
 
-__global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, DiffCloud c,
-                                                                const uint32_t* __restrict__ touched,
-                                                                const uint32_t* __restrict__ offsets,
-                                                                const float4* __restrict__ pairGrad,
-                                                                DiffGrads out) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= f.count) return;
-    const uint32_t slot = f.base + i;
-    const uint32_t stride = (1u + c.restPerColour) * 3u;
-    float* gMean = out.means + static_cast<uint64_t>(i) * 3;
-    float* gScale = out.logScales + static_cast<uint64_t>(i) * 3;
-    float* gQuat = out.quats + static_cast<uint64_t>(i) * 4;
-    float* gSh = out.sh + static_cast<uint64_t>(i) * stride;
-    const uint32_t n = touched[slot];
-    // The splat's screen-space gradient: its run of pair records.
-    float g[9] = {0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f};
-    if (n > 0) {
-        const float4* run = pairGrad + static_cast<uint64_t>(offsets[slot]) * 3;
-        for (uint32_t e = 0; e < n; ++e) {
-            const float4 a = run[e * 3], b = run[e * 3 + 1], d = run[e * 3 + 2];
-            g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
-            g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
-            g[8] += d.x;
-        }
-    }
-    if (n == 0) {
-        for (int k = 0; k < 3; ++k) gMean[k] = gScale[k] = 0.0f;
-        for (int k = 0; k < 4; ++k) gQuat[k] = 0.0f;
-        out.logits[i] = 0.0f;
-        for (uint32_t k = 0; k < stride; ++k) gSh[k] = 0.0f;
-        return;
-    }
+// A splat's parameter gradients, summed over the views that drew it. The
+// harmonic entries past the cloud's own stride stay zero and are not written.
+struct SplatGrad {
+    float mean[3], scale[3], quat[4], logit;
+    float sh[48];   // dc rgb, then 15 rest bases rgb
+};
+
+__device__ __forceinline__ void zero(SplatGrad& g) {
+#pragma unroll
+    for (int k = 0; k < 3; ++k) g.mean[k] = g.scale[k] = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 4; ++k) g.quat[k] = 0.0f;
+    g.logit = 0.0f;
+#pragma unroll
+    for (int k = 0; k < 48; ++k) g.sh[k] = 0.0f;
+}
+
+// The screen-space gradient g of splat i in frame f (d centre.xy, d conic a b
+// c, d alpha, d rgb) walked back to its parameters and added into acc: conic
+// -> 2D covariance (dilation, antialiasing factor), A = J W M, M = R diag(s),
+// the clamped Jacobian, the centre, the rotation of the normalised
+// quaternion, the 3D filter, exp and sigmoid, and the harmonics through the
+// view direction.
+__device__ __forceinline__ void splatBackward(const FrameParams& f, const DiffCloud& c, uint32_t i,
+                                              const Decoded& dec, const float dc[3], const float coeff[45],
+                                              const float g[9], SplatGrad& acc) {
     const float dCx = g[0], dCy = g[1], dCa = g[2], dCb = g[3], dCc = g[4], dAlpha = g[5];
     const float dCol[3] = {g[6], g[7], g[8]};
 
     // Recompute the forward (projectWave's footprint) and walk it back.
-    const Decoded dec = decode(c, i, f.useFilter3d != 0);
     const SplatShape& s = dec.shape;
     const float* m = f.m;
     const float tx = m[0] * s.px + m[1] * s.py + m[2] * s.pz + m[3];
@@ -579,8 +671,8 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
         }
     }
 #pragma unroll
-    for (int j = 0; j < 3; ++j) gScale[j] = dRaw[j] * dec.s[j];   // s = exp(log s)
-    out.logits[i] = dSigmoid * dec.sigmoid * (1.0f - dec.sigmoid);
+    for (int j = 0; j < 3; ++j) acc.scale[j] = acc.scale[j] + dRaw[j] * dec.s[j];   // s = exp(log s)
+    acc.logit = acc.logit + dSigmoid * dec.sigmoid * (1.0f - dec.sigmoid);
 
     // R(q), q normalised
     {
@@ -596,10 +688,7 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
         if (dec.qLen > 0.0f) {
             const float proj1 = dq[0] * s.q[0] + dq[1] * s.q[1] + dq[2] * s.q[2] + dq[3] * s.q[3];
 #pragma unroll
-            for (int k = 0; k < 4; ++k) gQuat[k] = (dq[k] - s.q[k] * proj1) / dec.qLen;
-        } else {
-#pragma unroll
-            for (int k = 0; k < 4; ++k) gQuat[k] = 0.0f;
+            for (int k = 0; k < 4; ++k) acc.quat[k] = acc.quat[k] + (dq[k] - s.q[k] * proj1) / dec.qLen;
         }
     }
 
@@ -637,31 +726,109 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
 
     // colour = max(0.5 + C0 dc + rest(dir), 0), dir = (p - eye) / |p - eye|
     const uint32_t degree = shDegree(f);
-    const float* row = c.sh + static_cast<uint64_t>(i) * stride;
     const float dxo = s.px - f.eye[0], dyo = s.py - f.eye[1], dzo = s.pz - f.eye[2];
     const float len = __builtin_sqrtf((dxo * dxo + dyo * dyo) + dzo * dzo);
     const float dir[3] = {dxo / len, dyo / len, dzo / len};
-    float coeff[45];
-    const uint32_t used = restCount(degree) * 3u;
-#pragma unroll
-    for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
     float rest[3];
     evaluateRest(coeff, degree, dir[0], dir[1], dir[2], rest);
     float dPre[3];
 #pragma unroll
     for (int ch = 0; ch < 3; ++ch) {
-        const float pre = (0.5f + kShC0 * row[ch]) + rest[ch];
+        const float pre = (0.5f + kShC0 * dc[ch]) + rest[ch];
         dPre[ch] = pre > 0.0f ? dCol[ch] : 0.0f;
-        gSh[ch] = kShC0 * dPre[ch];
+        acc.sh[ch] = acc.sh[ch] + kShC0 * dPre[ch];
     }
     float dCoeff[45], dDir[3];
     evaluateRestGrad(coeff, degree, dir[0], dir[1], dir[2], dPre, dCoeff, dDir);
 #pragma unroll
-    for (uint32_t k = 0; k < 45; ++k)
-        if (k + 3u < stride) gSh[3 + k] = dCoeff[k];
+    for (uint32_t k = 0; k < 45; ++k) acc.sh[3 + k] = acc.sh[3 + k] + dCoeff[k];
     const float along = dDir[0] * dir[0] + dDir[1] * dir[1] + dDir[2] * dir[2];
 #pragma unroll
-    for (int k = 0; k < 3; ++k) gMean[k] = dP[k] + (dDir[k] - dir[k] * along) / len;
+    for (int k = 0; k < 3; ++k) acc.mean[k] = acc.mean[k] + (dP[k] + (dDir[k] - dir[k] * along) / len);
+}
+
+// The screen-space gradient of slot `slot`: the sum of its run of pair records
+// (its pairs are contiguous in slot-major order).
+__device__ __forceinline__ void slotGradient(const float4* __restrict__ pairGrad, uint32_t first, uint32_t n,
+                                             float g[9]) {
+#pragma unroll
+    for (int k = 0; k < 9; ++k) g[k] = 0.0f;
+    const float4* run = pairGrad + static_cast<uint64_t>(first) * 3;
+    for (uint32_t e = 0; e < n; ++e) {
+        const float4 a = run[e * 3], b = run[e * 3 + 1], d = run[e * 3 + 2];
+        g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
+        g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
+        g[8] += d.x;
+    }
+}
+
+__device__ __forceinline__ void storeGrad(const SplatGrad& acc, uint32_t i, uint32_t stride, const DiffGrads& out) {
+    float* gMean = out.means + static_cast<uint64_t>(i) * 3;
+    float* gScale = out.logScales + static_cast<uint64_t>(i) * 3;
+    float* gQuat = out.quats + static_cast<uint64_t>(i) * 4;
+    float* gSh = out.sh + static_cast<uint64_t>(i) * stride;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gMean[k] = acc.mean[k];
+        gScale[k] = acc.scale[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gQuat[k] = acc.quat[k];
+    out.logits[i] = acc.logit;
+#pragma unroll
+    for (uint32_t k = 0; k < 48; ++k)
+        if (k < stride) gSh[k] = acc.sh[k];
+}
+
+__global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, DiffCloud c,
+                                                                const uint32_t* __restrict__ touched,
+                                                                const uint32_t* __restrict__ offsets,
+                                                                const float4* __restrict__ pairGrad,
+                                                                DiffGrads out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= f.count) return;
+    const uint32_t slot = f.base + i;
+    SplatGrad acc;
+    zero(acc);
+    const uint32_t n = touched[slot];
+    if (n > 0) {
+        float g[9];
+        slotGradient(pairGrad, offsets[slot], n, g);
+        const Decoded dec = decode(c, i, f.useFilter3d != 0);
+        float dc[3], coeff[45];
+        loadHarmonics(c, i, shDegree(f), dc, coeff);
+        splatBackward(f, c, i, dec, dc, coeff, g, acc);
+    }
+    storeGrad(acc, i, (1u + c.restPerColour) * 3u, out);
+}
+
+// A batch of views of one cloud: each splat decodes its parameters once and
+// adds its gradient in every view that drew it, in view order (no atomics).
+// View v's frame is params[v]; the splat's slot there is params[v].base + i.
+__global__ __launch_bounds__(kBlock) void projectBackwardViewsKernel(const FrameParams* __restrict__ params,
+                                                                     uint32_t views, DiffCloud c,
+                                                                     const uint32_t* __restrict__ touched,
+                                                                     const uint32_t* __restrict__ offsets,
+                                                                     const float4* __restrict__ pairGrad,
+                                                                     DiffGrads out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= c.count) return;
+    const FrameParams& f0 = params[0];
+    const Decoded dec = decode(c, i, f0.useFilter3d != 0);
+    float dc[3], coeff[45];
+    loadHarmonics(c, i, shDegree(f0), dc, coeff);
+    SplatGrad acc;
+    zero(acc);
+    for (uint32_t v = 0; v < views; ++v) {
+        const FrameParams& f = params[v];
+        const uint32_t slot = f.base + i;
+        const uint32_t n = touched[slot];
+        if (n == 0) continue;
+        float g[9];
+        slotGradient(pairGrad, offsets[slot], n, g);
+        splatBackward(f, c, i, dec, dc, coeff, g, acc);
+    }
+    storeGrad(acc, i, (1u + c.restPerColour) * 3u, out);
 }
 
 }   // namespace
@@ -670,6 +837,13 @@ void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uin
                  DepthBlocks depth, hipStream_t stream) {
     if (f.count == 0) return;
     projectDiffKernel<<<projectBlocks(f.count), kBlock, 0, stream>>>(f, cloud, proj, rects, touched, depth);
+}
+
+void projectDiffViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, float4* proj, uint2* rects,
+                      uint32_t* touched, hipStream_t stream) {
+    if (cloud.count == 0 || views == 0) return;
+    projectDiffViewsKernel<<<(cloud.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(params, views, cloud, proj, rects,
+                                                                                      touched);
 }
 
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
@@ -686,6 +860,25 @@ void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, co
     if (tiles == 0) return;
     blendBackwardKernel<<<tiles, kBlock, 0, stream>>>(f, slots, ranges, sortedSlots, pairIndex, proj, accum,
                                                       lastCount, finalT, gradColour, pairGrad);
+}
+
+void blendBackwardViews(const FrameParams* params, const ViewInfo* views, uint32_t viewCount, uint32_t tiles,
+                        uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots, const uint32_t* pairIndex,
+                        const float4* proj, const float4* accum, const uint32_t* lastCount, const float* finalT,
+                        const float4* const* gradColours, float4* pairGrad, hipStream_t stream) {
+    if (tiles == 0 || viewCount == 0) return;
+    blendBackwardViewsKernel<<<tiles, kBlock, 0, stream>>>(params, views, viewCount, slots, ranges, sortedSlots,
+                                                           pairIndex, proj, accum, lastCount, finalT, gradColours,
+                                                           pairGrad);
+}
+
+void projectBackwardViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, const uint32_t* touched,
+                          const uint32_t* offsets, const float4* pairGrad, const DiffGrads& grads,
+                          hipStream_t stream) {
+    if (cloud.count == 0 || views == 0) return;
+    projectBackwardViewsKernel<<<(cloud.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(params, views, cloud,
+                                                                                          touched, offsets, pairGrad,
+                                                                                          grads);
 }
 
 void projectBackward(const FrameParams& f, const DiffCloud& cloud, const uint32_t* touched, const uint32_t* offsets,

@@ -145,6 +145,7 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
 }
 
 // A batch of views: slot s belongs to view s / splatsPerView.
+template <bool EMIT_INDEX>
 __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* __restrict__ views,
                                                                uint32_t splatsPerView, uint32_t slots,
                                                                const float4* __restrict__ proj,
@@ -152,20 +153,24 @@ __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* _
                                                                const uint32_t* __restrict__ touched,
                                                                const uint32_t* __restrict__ offsets,
                                                                uint64_t* __restrict__ keys,
-                                                               uint32_t* __restrict__ values, uint32_t capacity,
-                                                               uint32_t* __restrict__ overflow) {
+                                                               uint32_t* __restrict__ values,
+                                                               uint32_t* __restrict__ pairSlot, uint32_t capacity,
+                                                               uint32_t* __restrict__ overflow,
+                                                               uint32_t* __restrict__ outOfRange) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t expected = s < slots ? touched[s] : 0u;   // every lane stays for the wave's walk
     uint32_t width = 0, height = 0, tilesX = 0, tileBase = 0;
+    const uint32_t* depthRange = nullptr;
     if (expected != 0) {
         const ViewInfo& v = views[s / splatsPerView];
         width = v.width;
         height = v.height;
         tilesX = v.tilesX;
         tileBase = v.tileBase;
+        if (outOfRange != nullptr) depthRange = &v.depthLo;   // depthLo, depthHi
     }
-    duplicateWave<false, true>(s, expected, width, height, tilesX, tileBase, proj, rects, offsets, keys, values,
-                               nullptr, capacity, overflow, nullptr, nullptr);
+    duplicateWave<EMIT_INDEX, true>(s, expected, width, height, tilesX, tileBase, proj, rects, offsets, keys, values,
+                                    pairSlot, capacity, overflow, depthRange, outOfRange);
 }
 
 }   // namespace
@@ -187,10 +192,19 @@ void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const u
 
 void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
                     const uint2* rects, const uint32_t* touched, const uint32_t* offsets, uint64_t* keys,
-                    uint32_t* values, uint32_t capacity, uint32_t* overflow, hipStream_t stream) {
+                    uint32_t* values, uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow,
+                    uint32_t* outOfRange, hipStream_t stream) {
     if (slots == 0 || splatsPerView == 0) return;
-    duplicateViewsKernel<<<(slots + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        views, splatsPerView, slots, proj, rects, touched, offsets, keys, values, capacity, overflow);
+    const uint32_t blocks = (slots + kBlock - 1) / kBlock;
+    if (pairSlot != nullptr) {
+        duplicateViewsKernel<true><<<blocks, kBlock, 0, stream>>>(views, splatsPerView, slots, proj, rects, touched,
+                                                                  offsets, keys, values, pairSlot, capacity, overflow,
+                                                                  outOfRange);
+    } else {
+        duplicateViewsKernel<false><<<blocks, kBlock, 0, stream>>>(views, splatsPerView, slots, proj, rects, touched,
+                                                                   offsets, keys, values, nullptr, capacity, overflow,
+                                                                   outOfRange);
+    }
 }
 
 }   // namespace lrt::kern

@@ -61,6 +61,9 @@ struct ViewInfo {
     uint64_t pixelBase;
     float* colour;         // float4 a pixel, the caller's
     float* depth;          // may be null
+    // The depth keys (bits(z) >> 7) the batch's sort key holds: a drawn slot
+    // outside [depthLo, depthHi] is an error the duplication counts.
+    uint32_t depthLo, depthHi;
 };
 
 // What the projection leaves for the blend, one record per slot: the

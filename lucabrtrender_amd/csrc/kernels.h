@@ -64,10 +64,13 @@ void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const u
                uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow, hipStream_t stream,
                const uint32_t* depthRange = nullptr, uint32_t* outOfRange = nullptr);
 // A batch of views (device ViewInfo array): slot s is view s / splatsPerView's,
-// its keys carry the view's tileBase.
+// its keys carry the view's tileBase; pairSlot as duplicate()'s. With
+// outOfRange non-null, a drawn slot whose depth key lies outside its view's
+// [depthLo, depthHi] is counted there.
 void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
                     const uint2* rects, const uint32_t* touched, const uint32_t* offsets, uint64_t* keys,
-                    uint32_t* values, uint32_t capacity, uint32_t* overflow, hipStream_t stream);
+                    uint32_t* values, uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow,
+                    uint32_t* outOfRange, hipStream_t stream);
 
 // ---- sort -------------------------------------------------------------------
 // Which bits of a 64-bit key the digits come from.
@@ -184,11 +187,12 @@ void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint
 // A batch of views in one launch (device FrameParams and ViewInfo arrays;
 // views sorted by tileBase, `tiles` in all): as a grid of quadrant tasks, or
 // persistent (at most gridCap workgroups taking tasks from *queue, which is
-// cleared here). lastCount/finalT hold the batch's pixels (ViewInfo.pixelBase).
+// cleared here). lastCount/finalT/accum (each may be null) hold the batch's
+// pixels (ViewInfo.pixelBase); accum as blend()'s.
 void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gridCap, const FrameParams* params,
                 const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges,
-                const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, uint32_t* queue,
-                hipStream_t stream);
+                const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
+                uint32_t* queue, hipStream_t stream);
 
 // ---- brute force in FP64 -----------------------------------------------------------
 // Bytes of one FP64 brute-force record.
@@ -228,6 +232,10 @@ struct DiffGrads {
 };
 void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uint2* rects, uint32_t* touched,
                  DepthBlocks depth, hipStream_t stream);
+// The same cloud for `views` views at once, splat-major: view v's constants
+// are params[v] (device), its slots start at params[v].base.
+void projectDiffViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, float4* proj, uint2* rects,
+                      uint32_t* touched, hipStream_t stream);
 // sortedSlots[i] = pairSlot[pairIndex[i]] for i < pairs (or < *nDevice, pairs
 // then being the capacity).
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
@@ -237,6 +245,19 @@ void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t p
 void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
                    const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
                    const float* finalT, const float4* gradColour, float4* pairGrad, hipStream_t stream);
+// The blend backward of every tile of a batch (device FrameParams and ViewInfo
+// arrays as blendViews'; accum/lastCount/finalT hold the batch's pixels at
+// ViewInfo.pixelBase); gradColours: a device array of one colour-gradient
+// pointer a view.
+void blendBackwardViews(const FrameParams* params, const ViewInfo* views, uint32_t viewCount, uint32_t tiles,
+                        uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots, const uint32_t* pairIndex,
+                        const float4* proj, const float4* accum, const uint32_t* lastCount, const float* finalT,
+                        const float4* const* gradColours, float4* pairGrad, hipStream_t stream);
+// Each splat's runs of pair records in every view -> gradients of its
+// parameters summed over the views in view order.
+void projectBackwardViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, const uint32_t* touched,
+                          const uint32_t* offsets, const float4* pairGrad, const DiffGrads& grads,
+                          hipStream_t stream);
 // Each splat's run of pair records -> gradients of its parameters.
 void projectBackward(const FrameParams& f, const DiffCloud& cloud, const uint32_t* touched, const uint32_t* offsets,
                      const float4* pairGrad, const DiffGrads& grads, hipStream_t stream);

@@ -200,6 +200,18 @@ public:
     Status autotuneViews(std::span<const View> views, std::span<const SplatInstance> instances,
                          RenderSettings& settings, float* ms);
 
+    // Many views of one FP32-parameter cloud: renderViews' launch sequence with
+    // renderDiff's differentiable stages (pairs carry their slot-major index,
+    // each pixel keeps its raw rgb, last contributor and final transmittance),
+    // for backwardViews(). Each view's images are those renderDiff would draw
+    // for it alone, bit for bit.
+    Status renderDiffViews(std::span<const View> views, const kern::DiffCloud& cloud, const Mat4& objectToWorld,
+                           const RenderSettings& settings, FrameStats* stats);
+    // Gradients of the last renderDiffViews' parameters, summed over its views
+    // in view order, from each view's dL/d(colour) (device, one float4 a pixel
+    // of that view, bottom row first). millis as backward()'s.
+    Status backwardViews(std::span<const float4* const> gradColours, const kern::DiffGrads& grads, float* millis);
+
     // The differentiable frame: one FP32-parameter cloud, drawn by the same
     // stages as render(); it keeps what backward() needs.
     Status renderDiff(const Projection& projection, const kern::DiffCloud& cloud, const Mat4& objectToWorld,
@@ -244,9 +256,13 @@ private:
     Status reservePairs(uint32_t count, uint32_t keyBits, bool training = false);
     Status reserveTargets(uint32_t width, uint32_t height, uint32_t tiles, bool training);
     Status checkFrame(const RenderSettings& settings, const RenderTargets& targets) const;
+    // A batch of views of the instances, or (diff non-null, one instance giving
+    // the cloud's place) of an FP32-parameter cloud kept for backwardViews().
+    Status runViews(std::span<const View> views, std::span<const SplatInstance> instances,
+                    const kern::DiffCloud* diff, const RenderSettings& settings, FrameStats* stats);
     // The launch sequence of a batch (no waits), on `stream`.
-    void enqueueViews(std::span<const SplatInstance> instances, uint32_t viewCount, uint32_t splatsPerView,
-                      uint32_t tiles, const RenderSettings& settings, hipStream_t stream);
+    void enqueueViews(std::span<const SplatInstance> instances, const kern::DiffCloud* diff, uint32_t viewCount,
+                      uint32_t splatsPerView, uint32_t tiles, const RenderSettings& settings, hipStream_t stream);
     // Everything after the projection: scan, duplicate, sort, ranges, blend
     // (the composite blend when `composite`, over `under` when non-null).
     Status finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
@@ -309,10 +325,16 @@ private:
     hipStream_t captureStream_ = nullptr;
     hipGraphExec_t graph_ = nullptr;
     std::string graphKey_;
-    // The last differentiable frame.
+    // The last differentiable frame, or batch of views (its view count).
     bool diffReady_ = false;
+    bool diffViewsReady_ = false;
+    uint32_t diffViews_ = 0;
     kern::DiffCloud diffCloud_{};
     FrameParams diffFrame_{};
+    // A batch backward's colour-gradient pointers, one a view (pinned host, device).
+    const float4** hostGradPtrs_ = nullptr;
+    uint32_t gradPtrCapacity_ = 0;
+    DeviceBuffer viewGrads_;
 };
 
 }   // namespace lrt

@@ -64,6 +64,7 @@ TileRasterizer::~TileRasterizer() {
     if (hostCounters_ != nullptr) (void)hipHostFree(hostCounters_);
     if (hostParams_ != nullptr) (void)hipHostFree(hostParams_);
     if (hostViews_ != nullptr) (void)hipHostFree(hostViews_);
+    if (hostGradPtrs_ != nullptr) (void)hipHostFree(hostGradPtrs_);
     if (graph_ != nullptr) (void)hipGraphExecDestroy(graph_);
     if (captureStream_ != nullptr) (void)hipStreamDestroy(captureStream_);
 }
@@ -283,6 +284,7 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
         first = frameParamsFor(projection, none, settings, 0);
     }
     diffReady_ = false;
+    diffViewsReady_ = false;
     const bool composite = under != nullptr || !points.empty();
     return finishFrame(first, splatCount, settings, targets, statsOut, false, composite, under);
 }
@@ -322,6 +324,7 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
                       DepthBlocks{depthBlocks_.as<uint32_t>(), 0}, stream_);
     depthEntries_ = projectBlocks(cloud.count);
     diffReady_ = false;
+    diffViewsReady_ = false;
     if (!(s = finishFrame(f, cloud.count, settings, targets, statsOut, true)).ok()) return s;
     diffCloud_ = cloud;
     diffFrame_ = f;
@@ -509,13 +512,14 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     return {};
 }
 
-void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint32_t viewCount,
-                                  uint32_t splatsPerView, uint32_t tiles, const RenderSettings& settings,
-                                  hipStream_t stream) {
+void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, const kern::DiffCloud* diff,
+                                  uint32_t viewCount, uint32_t splatsPerView, uint32_t tiles,
+                                  const RenderSettings& settings, hipStream_t stream) {
     const uint32_t entries = viewCount * static_cast<uint32_t>(std::max<size_t>(instances.size(), 1));
     uint32_t* counters = counters_.as<uint32_t>();
     auto* params = viewParams_.as<FrameParams>();
     auto* views = viewInfo_.as<ViewInfo>();
+    const bool training = diff != nullptr;
     // Stage events as render()'s (kernelMs_), only when not capturing.
     const bool timed = settings.timeStages && stream == stream_;
     const auto mark = [&](int k) {
@@ -525,23 +529,30 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint
     (void)hipMemcpyAsync(views, hostViews_, sizeof(ViewInfo) * viewCount, hipMemcpyHostToDevice, stream);
     mark(0);
     // Project: one launch per instance covers every view.
-    for (size_t k = 0; k < instances.size(); ++k) {
-        const SplatCloud* cloud = instances[k].splats;
-        if (cloud == nullptr || cloud->count == 0) continue;
-        kern::projectViews(params + k * viewCount, viewCount, cloud->count, cloud->positions, cloud->shape,
-                           cloud->sh, cloud->filter3d, proj_.as<float4>(), rects_.as<uint2>(),
-                           touched_.as<uint32_t>(), stream);
+    if (training) {
+        kern::projectDiffViews(params, viewCount, *diff, proj_.as<float4>(), rects_.as<uint2>(),
+                               touched_.as<uint32_t>(), stream);
+    } else {
+        for (size_t k = 0; k < instances.size(); ++k) {
+            const SplatCloud* cloud = instances[k].splats;
+            if (cloud == nullptr || cloud->count == 0) continue;
+            kern::projectViews(params + k * viewCount, viewCount, cloud->count, cloud->positions, cloud->shape,
+                               cloud->sh, cloud->filter3d, proj_.as<float4>(), rects_.as<uint2>(),
+                               touched_.as<uint32_t>(), stream);
+        }
     }
     const uint32_t slots = viewCount * splatsPerView;
     mark(1);
-    kern::fill32(counters, 0u, 8, stream);
+    kern::fill32(counters, 0u, 11, stream);
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), slots, scanStatus_.as<uint64_t>(),
                         counters + 0, counters + 1, counters + 5, stream);
     mark(2);
     mark(3);
+    // Every drawn slot's depth key is checked against the sort key's field.
     kern::duplicateViews(views, splatsPerView, slots, proj_.as<float4>(), rects_.as<uint2>(),
                          touched_.as<uint32_t>(), offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(),
-                         values_[0].as<uint32_t>(), pairCapacity_, counters + 2, stream);
+                         values_[0].as<uint32_t>(), training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_,
+                         counters + 2, counters + 10, stream);
     mark(4);
     // The pair count (counters[0], from the scan) stays on the device: the
     // sort and the ranges read it, launched for the capacity.
@@ -553,34 +564,27 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, uint
     mark(6);
     const int sorted = inAlt ? 1 : 0;
     kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream, counters + 0);
+    // The blend reads slots; a training batch's sort carried pair indices.
+    const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
+    if (training) {
+        kern::gatherSlots(values_[sorted].as<uint32_t>(), pairSlot_.as<uint32_t>(), pairCapacity_,
+                          sortedSlots_.as<uint32_t>(), stream, counters + 0);
+        pairSlots = sortedSlots_.as<uint32_t>();
+    }
     mark(7);
     // A persistent blend keeps about 24 one-wave workgroups on each CU.
-    kern::blendViews(settings.depthMode, settings.blendBatch, settings.persistentBlend, computeUnits_ * 24u, params, views, viewCount, tiles, slots, ranges_.as<uint2>(),
-                     values_[sorted].as<uint32_t>(), proj_.as<float4>(),
-                     settings.keepWalkState ? lastCount_.as<uint32_t>() : nullptr,
-                     settings.keepWalkState ? finalT_.as<float>() : nullptr, counters + 6, stream);
+    const bool keepWalk = training || settings.keepWalkState;
+    kern::blendViews(settings.depthMode, settings.blendBatch, settings.persistentBlend, computeUnits_ * 24u, params,
+                     views, viewCount, tiles, slots, ranges_.as<uint2>(), pairSlots, proj_.as<float4>(),
+                     keepWalk ? lastCount_.as<uint32_t>() : nullptr, keepWalk ? finalT_.as<float>() : nullptr,
+                     training ? accum_.as<float4>() : nullptr, counters + 6, stream);
     mark(8);
-    (void)hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream);
+    (void)hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream);
     lastSorted_ = sorted;
 }
 
 Status TileRasterizer::renderViews(std::span<const View> views, std::span<const SplatInstance> instances,
                                    const RenderSettings& settings, FrameStats* statsOut) {
-    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
-    if (views.empty()) return Status::error(ErrorCode::InvalidArgument, "a batch needs a view");
-    uint64_t pixels = 0, tiles = 0;
-    for (const View& v : views) {
-        const RenderTargets& t = v.targets;
-        if (t.width == 0 || t.height == 0 || t.colour == nullptr) {
-            return Status::error(ErrorCode::InvalidArgument, "a view without a size or a colour target");
-        }
-        if (t.width > 0xFFFFu * kTile || t.height > 0xFFFFu * kTile) {
-            return Status::error(ErrorCode::Unsupported, "a frame wider or taller than 1048560 pixels");
-        }
-        pixels += uint64_t{t.width} * t.height;
-        tiles += uint64_t{(t.width + kTile - 1) / kTile} * ((t.height + kTile - 1) / kTile);
-    }
-    uint64_t splats = 0;
     for (const SplatInstance& instance : instances) {
         if (instance.splats == nullptr) continue;
         const SplatCloud& c = *instance.splats;
@@ -599,7 +603,59 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         if (settings.filter3d && c.count > 0 && c.filter3d == nullptr) {
             return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and a cloud has no filter3d");
         }
-        splats += c.count;
+    }
+    return runViews(views, instances, nullptr, settings, statsOut);
+}
+
+Status TileRasterizer::renderDiffViews(std::span<const View> views, const kern::DiffCloud& cloud,
+                                       const Mat4& objectToWorld, const RenderSettings& settings,
+                                       FrameStats* statsOut) {
+    if (cloud.count > 0 && (cloud.means == nullptr || cloud.logScales == nullptr || cloud.quats == nullptr ||
+                            cloud.logits == nullptr || cloud.sh == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "a differentiable cloud with a missing parameter");
+    }
+    if (cloud.restPerColour != 0 && cloud.restPerColour != 3 && cloud.restPerColour != 8 &&
+        cloud.restPerColour != 15) {
+        return Status::error(ErrorCode::InvalidArgument, "restPerColour must be 0, 3, 8 or 15");
+    }
+    if (settings.filter3d && cloud.count > 0 && cloud.filter3d == nullptr) {
+        return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and the cloud has no filter3d");
+    }
+    // The cloud's place in the frame constants: its count, harmonics and transform.
+    SplatCloud shapeOnly;
+    shapeOnly.count = cloud.count;
+    shapeOnly.restPerColour = cloud.restPerColour;
+    SplatInstance instance;
+    instance.splats = &shapeOnly;
+    instance.objectToWorld = objectToWorld;
+    Status s = runViews(views, std::span<const SplatInstance>(&instance, 1), &cloud, settings, statsOut);
+    if (!s.ok()) return s;
+    diffCloud_ = cloud;
+    diffViews_ = static_cast<uint32_t>(views.size());
+    diffViewsReady_ = true;
+    return {};
+}
+
+Status TileRasterizer::runViews(std::span<const View> views, std::span<const SplatInstance> instances,
+                                const kern::DiffCloud* diff, const RenderSettings& settings, FrameStats* statsOut) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (views.empty()) return Status::error(ErrorCode::InvalidArgument, "a batch needs a view");
+    const bool training = diff != nullptr;
+    uint64_t pixels = 0, tiles = 0;
+    for (const View& v : views) {
+        const RenderTargets& t = v.targets;
+        if (t.width == 0 || t.height == 0 || t.colour == nullptr) {
+            return Status::error(ErrorCode::InvalidArgument, "a view without a size or a colour target");
+        }
+        if (t.width > 0xFFFFu * kTile || t.height > 0xFFFFu * kTile) {
+            return Status::error(ErrorCode::Unsupported, "a frame wider or taller than 1048560 pixels");
+        }
+        pixels += uint64_t{t.width} * t.height;
+        tiles += uint64_t{(t.width + kTile - 1) / kTile} * ((t.height + kTile - 1) / kTile);
+    }
+    uint64_t splats = 0;
+    for (const SplatInstance& instance : instances) {
+        if (instance.splats != nullptr) splats += instance.splats->count;
     }
     const uint64_t slots = splats * views.size();
     if (slots > 0xFFFFFFFFull || tiles > (1ull << 24) || views.size() > 0xFFFFu) {
@@ -610,6 +666,9 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
     const uint32_t entries = viewCount * static_cast<uint32_t>(std::max<size_t>(instances.size(), 1));
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
+    // The scratch this batch may regrow holds the last differentiable frame.
+    diffReady_ = false;
+    diffViewsReady_ = false;
     if (!(s = reserveSplats(static_cast<uint32_t>(std::max<uint64_t>(slots, 1)))).ok()) return s;
     if (tiles > tileCapacity_) {
         if (!(s = ranges_.reserve(tiles * 8, "tile.ranges")).ok()) return s;
@@ -621,6 +680,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         if (!(s = finalT_.reserve(pixels * 4, "pixel.finalT")).ok()) return s;
         pixelCapacity_ = pixels;
     }
+    if (training && !(s = accum_.reserve(pixels * 16, "pixel.accum")).ok()) return s;
     if (entries > batchCapacity_) {
         if (hostParams_ != nullptr) (void)hipHostFree(hostParams_);
         if (hostViews_ != nullptr) (void)hipHostFree(hostViews_);
@@ -660,6 +720,9 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         viewsLayout_.depthMin = depthKey(nearest);
         viewsLayout_.depthBits = bitsFor(depthKey(farthest) - depthKey(nearest) + 1u);
     }
+    const uint32_t depthHi = viewsLayout_.depthBits >= 24
+                                 ? 0x00FFFFFFu
+                                 : viewsLayout_.depthMin + ((1u << viewsLayout_.depthBits) - 1u);
     for (uint32_t v = 0; v < viewCount; ++v) {
         const View& view = views[v];
         RenderSettings vs = settings;
@@ -683,24 +746,26 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         info.pixelBase = pixelBase;
         info.colour = reinterpret_cast<float*>(view.targets.colour);
         info.depth = view.targets.depth;
+        info.depthLo = viewsLayout_.depthMin;
+        info.depthHi = depthHi;
         tileBase += info.tilesX * info.tilesY;
         pixelBase += uint64_t{info.width} * info.height;
     }
     const uint32_t keyBits = 24 + bitsFor(static_cast<uint32_t>(tiles));
     const auto wallStart = std::chrono::steady_clock::now();
-    diffReady_ = false;
     // The pair capacity is the last batch's; a batch with more pairs overflows
     // the duplication, which the host sees after the wait: then the capacity
     // grows to the count and the batch runs again.
     for (int attempt = 0; attempt < 2; ++attempt) {
-        if (!(s = reservePairs(std::max(pairCapacity_, 1u), keyBits)).ok()) return s;
+        if (!(s = reservePairs(std::max(pairCapacity_, 1u), keyBits, training)).ok()) return s;
         if (settings.useGraph) {
             // A captured graph holds its launches' sizes and every buffer
             // address: any change of either needs a new capture.
             std::string key = std::to_string(viewCount) + "/" + std::to_string(splatsPerView) + "/" +
                               std::to_string(tiles) + "/" + std::to_string(pairCapacity_) + "/" +
                               std::to_string(settings.blendBatch) + "/" + std::to_string(settings.persistentBlend) +
-                              "/" + std::to_string(settings.depthMode) + "/" + std::to_string(viewsLayout_.depthMin) +
+                              "/" + std::to_string(settings.keepWalkState) + "/" + std::to_string(training) + "/" +
+                              std::to_string(settings.depthMode) + "/" + std::to_string(viewsLayout_.depthMin) +
                               "/" + std::to_string(viewsLayout_.depthBits);
             const auto addr = [&key](const void* p) { key += "/" + std::to_string(reinterpret_cast<uintptr_t>(p)); };
             for (const SplatInstance& instance : instances) {
@@ -713,9 +778,16 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
                 addr(c->filter3d);
                 key += ":" + std::to_string(c->count);
             }
+            if (training) {
+                for (const void* p : {static_cast<const void*>(diff->means), static_cast<const void*>(diff->logScales),
+                                      static_cast<const void*>(diff->quats), static_cast<const void*>(diff->logits),
+                                      static_cast<const void*>(diff->sh), static_cast<const void*>(diff->filter3d)})
+                    addr(p);
+                key += ":" + std::to_string(diff->count) + ":" + std::to_string(diff->restPerColour);
+            }
             for (const DeviceBuffer* b : {&proj_, &rects_, &touched_, &offsets_, &scanStatus_, &keys_[0], &keys_[1],
                                           &values_[0], &values_[1], &sortWork_, &ranges_, &lastCount_, &finalT_,
-                                          &counters_, &viewParams_, &viewInfo_})
+                                          &counters_, &viewParams_, &viewInfo_, &pairSlot_, &sortedSlots_, &accum_})
                 addr(b->get());
             addr(hostParams_);
             addr(hostViews_);
@@ -730,7 +802,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
                 if (!(s = check(hipStreamBeginCapture(captureStream_, hipStreamCaptureModeThreadLocal),
                                 "begin capture")).ok())
                     return s;
-                enqueueViews(instances, viewCount, splatsPerView, static_cast<uint32_t>(tiles), settings,
+                enqueueViews(instances, diff, viewCount, splatsPerView, static_cast<uint32_t>(tiles), settings,
                              captureStream_);
                 hipGraph_t graph = nullptr;
                 if (!(s = check(hipStreamEndCapture(captureStream_, &graph), "end capture")).ok()) return s;
@@ -744,7 +816,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
             }
             if (!(s = check(hipGraphLaunch(graph_, stream_), "graph launch")).ok()) return s;
         } else {
-            enqueueViews(instances, viewCount, splatsPerView, static_cast<uint32_t>(tiles), settings, stream_);
+            enqueueViews(instances, diff, viewCount, splatsPerView, static_cast<uint32_t>(tiles), settings, stream_);
         }
         if (!(s = check(hipStreamSynchronize(stream_), "batch")).ok()) return s;
         if (!(s = check(hipGetLastError(), "batch")).ok()) return s;
@@ -755,7 +827,10 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
         }
         if (hostCounters_[2] == 0) break;
         if (attempt == 1) return Status::error(ErrorCode::InternalError, "the duplication overran its pairs");
-        if (!(s = reservePairs(pairs, keyBits)).ok()) return s;
+        if (!(s = reservePairs(pairs, keyBits, training)).ok()) return s;
+    }
+    if (hostCounters_[10] != 0) {
+        return Status::error(ErrorCode::InternalError, "a drawn splat's depth key fell outside the batch's planes");
     }
     if (hostCounters_[4] != 0) return Status::error(ErrorCode::InternalError, "the sort's look-back gave up");
     const auto wallEnd = std::chrono::steady_clock::now();
@@ -775,7 +850,7 @@ Status TileRasterizer::renderViews(std::span<const View> views, std::span<const 
     lastPairs_ = hostCounters_[0];
     lastTiles_ = static_cast<uint32_t>(tiles);
     lastOrdered_ = false;
-    lastWalkDropped_ = !settings.keepWalkState;
+    lastWalkDropped_ = !(training || settings.keepWalkState);
     lastPixels_ = pixels;
     if (statsOut != nullptr) {
         FrameStats stats;
@@ -918,9 +993,64 @@ Status TileRasterizer::backward(const float4* gradColour, const kern::DiffGrads&
     return {};
 }
 
+Status TileRasterizer::backwardViews(std::span<const float4* const> gradColours, const kern::DiffGrads& grads,
+                                     float* millis) {
+    if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
+    if (!diffViewsReady_) {
+        return Status::error(ErrorCode::InvalidArgument, "no differentiable batch to take the backward of");
+    }
+    if (gradColours.size() != diffViews_) {
+        return Status::error(ErrorCode::InvalidArgument, "one colour gradient a view of the batch is needed");
+    }
+    for (const float4* g : gradColours) {
+        if (g == nullptr) return Status::error(ErrorCode::InvalidArgument, "no colour gradient");
+    }
+    if (diffCloud_.count > 0 && (grads.means == nullptr || grads.logScales == nullptr || grads.quats == nullptr ||
+                                 grads.logits == nullptr || grads.sh == nullptr)) {
+        return Status::error(ErrorCode::InvalidArgument, "a missing gradient buffer");
+    }
+    Status s = check(hipSetDevice(device_), "hipSetDevice");
+    if (!s.ok()) return s;
+    if (!(s = pairGrad_.reserve(uint64_t{std::max(pairCapacity_, 1u)} * 48, "pair.grad")).ok()) return s;
+    if (diffViews_ > gradPtrCapacity_) {
+        if (hostGradPtrs_ != nullptr) (void)hipHostFree(hostGradPtrs_);
+        hostGradPtrs_ = nullptr;
+        gradPtrCapacity_ = 0;
+        if (!(s = check(hipHostMalloc(reinterpret_cast<void**>(&hostGradPtrs_), sizeof(float4*) * diffViews_,
+                                      hipHostMallocDefault), "hipHostMalloc")).ok())
+            return s;
+        if (!(s = viewGrads_.reserve(sizeof(float4*) * diffViews_, "batch.gradColours")).ok()) return s;
+        gradPtrCapacity_ = diffViews_;
+    }
+    // The pinned table is rewritten only once the last backward's copy of it is done.
+    if (!(s = check(hipStreamSynchronize(stream_), "backward")).ok()) return s;
+    for (uint32_t v = 0; v < diffViews_; ++v) hostGradPtrs_[v] = gradColours[v];
+    if (!(s = check(hipMemcpyAsync(viewGrads_.get(), hostGradPtrs_, sizeof(float4*) * diffViews_,
+                                   hipMemcpyHostToDevice, stream_), "gradient table")).ok())
+        return s;
+    const auto* params = viewParams_.as<FrameParams>();
+    if (millis != nullptr) (void)hipEventRecord(events_[0], stream_);
+    kern::blendBackwardViews(params, viewInfo_.as<ViewInfo>(), diffViews_, lastTiles_, lastSplats_, ranges_.as<uint2>(),
+                             sortedSlots_.as<uint32_t>(), values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(),
+                             accum_.as<float4>(), lastCount_.as<uint32_t>(), finalT_.as<float>(),
+                             viewGrads_.as<const float4*>(), pairGrad_.as<float4>(), stream_);
+    if (millis != nullptr) (void)hipEventRecord(events_[1], stream_);
+    kern::projectBackwardViews(params, diffViews_, diffCloud_, touched_.as<uint32_t>(), offsets_.as<uint32_t>(),
+                               pairGrad_.as<float4>(), grads, stream_);
+    if (millis != nullptr) (void)hipEventRecord(events_[2], stream_);
+    if (!(s = check(hipStreamSynchronize(stream_), "backward")).ok()) return s;
+    if (!(s = check(hipGetLastError(), "backward")).ok()) return s;
+    if (millis != nullptr) {
+        (void)hipEventElapsedTime(&millis[0], events_[0], events_[1]);
+        (void)hipEventElapsedTime(&millis[1], events_[1], events_[2]);
+    }
+    return {};
+}
+
 Status TileRasterizer::prefixSum(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count) {
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
     diffReady_ = false;   // the scratch it may regrow holds the last frame
+    diffViewsReady_ = false;
     if (in == nullptr || out == nullptr || total == nullptr) return Status::error(ErrorCode::InvalidArgument, "null buffer");
     if (in == out) return Status::error(ErrorCode::InvalidArgument, "the scan's input and output must differ");
     Status s = check(hipSetDevice(device_), "hipSetDevice");
@@ -965,6 +1095,7 @@ Status TileRasterizer::decodeCloud(const io::RawSplats& raw, uint32_t maxDegree,
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
     diffReady_ = false;   // the per-splat scratch it reuses holds the last frame
+    diffViewsReady_ = false;
     lastSplats_ = lastVisible_ = lastPairs_ = lastTiles_ = 0;
     const uint64_t recordBytes = uint64_t{e.floatsPerRecord} * 4;
     constexpr uint64_t kSliceBytes = 256ull << 20;

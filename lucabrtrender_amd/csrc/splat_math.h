@@ -424,4 +424,15 @@ __device__ __forceinline__ float srgbToLinear(float c) {
     return c <= 0.04045f ? c / 12.92f : expF(2.4f * logF((c + 0.055f) / 1.055f));
 }
 
+// The view of a batch that holds batch tile g: the last view whose tiles
+// start at or before it (views in ascending tileBase).
+__device__ __forceinline__ uint32_t viewOfTile(const ViewInfo* __restrict__ views, uint32_t viewCount, uint32_t g) {
+    uint32_t lo = 0, hi = viewCount;   // views[lo].tileBase <= g < views[hi].tileBase
+    while (hi - lo > 1) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (views[mid].tileBase <= g) lo = mid; else hi = mid;
+    }
+    return lo;
+}
+
 }   // namespace lrt::dev

@@ -597,6 +597,31 @@ class TileRasterizer:
         self._check(lib.lrt_raster_backward(self._ctx, C.c_void_p(grad_colour.data_ptr()), C.byref(g), ms))
         return float(ms[0]), float(ms[1])
 
+    def render_diff_views(self, views: Sequence[View], splats: DiffSplats, settings: RenderSettings,
+                          object_to_world=IDENTITY) -> FrameStats:
+        """Every view of one FP32-parameter cloud in one launch sequence (each view's images as
+        render_diff draws them), kept for backward_views()."""
+        st = _abi.FrameStatsC()
+        self._diff_serial = getattr(self, "_diff_serial", 0) + 1
+        self._check(lib.lrt_raster_render_diff_views(self._ctx, self._views_c(views), len(views), C.byref(splats.c),
+                                                     _mat(object_to_world), C.byref(settings.to_c()), C.byref(st)))
+        return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
+
+    def backward_views(self, grad_colours: Sequence, grads: dict) -> tuple:
+        """Gradients of the last render_diff_views' parameters, summed over its views, from
+        each view's dL/d(colour) [H_v, W_v, 4] (in view order). grads as backward()'s.
+        Returns the GPU ms of the blend's and the preprocess's backward."""
+        import torch
+        for g in grad_colours:
+            if g.dtype != torch.float32 or not g.is_contiguous() or not g.is_cuda:
+                raise LrtError(3, "each colour gradient must be a contiguous float32 CUDA tensor")
+        ptrs = (C.c_void_p * max(1, len(grad_colours)))(*[g.data_ptr() for g in grad_colours])
+        g = _abi.DiffGradsC(grads["means"].data_ptr(), grads["log_scales"].data_ptr(), grads["quats"].data_ptr(),
+                            grads["opacity_logits"].data_ptr(), grads["sh"].data_ptr())
+        ms = (C.c_float * 2)()
+        self._check(lib.lrt_raster_backward_views(self._ctx, ptrs, len(grad_colours), C.byref(g), ms))
+        return float(ms[0]), float(ms[1])
+
     def debug(self, which: int) -> np.ndarray:
         """What the last render left on the device (LRT_DEBUG_*), copied to host."""
         need = C.c_size_t()

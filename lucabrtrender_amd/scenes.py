@@ -160,6 +160,51 @@ def diff_params(count: int, seed: int = 0, degree: int = 3, offset=(0.0, 0.0, -4
     return dict(means=means, log_scales=log_scales, quats=quats, opacity_logits=logits, sh=sh)
 
 
+def diff_params_clustered(count: int, seed: int = 0, degree: int = 3, clusters: int = 256, box_half: float = 10.0,
+                          cluster_sd=(0.05, 1.0), log_scale=(-5.0, 0.8), opacity_logit=(0.0, 1.0),
+                          sh_sd: float = 0.3) -> dict:
+    """FP32 parameters of the clustered scene (the batched fwd+bwd input): cluster
+    centres U[-box_half, box_half]^3, members N(centre, diag(U[cluster_sd])^2),
+    log-scales N(log_scale), quaternions of a 4-D normal, opacity logits
+    N(opacity_logit), dc from a base colour U[0,1], rest band l ~ N(0, sh_sd / l)."""
+    rng = np.random.default_rng(seed)
+    rest = REST[degree]
+    centres = rng.uniform(-box_half, box_half, (clusters, 3))
+    sds = rng.uniform(cluster_sd[0], cluster_sd[1], (clusters, 3))
+    member = rng.integers(0, clusters, count)
+    means = (centres[member] + rng.normal(0.0, 1.0, (count, 3)) * sds[member]).astype(np.float32)
+    log_scales = rng.normal(log_scale[0], log_scale[1], (count, 3)).astype(np.float32)
+    quats = rng.normal(0.0, 1.0, (count, 4)).astype(np.float32)
+    logits = rng.normal(opacity_logit[0], opacity_logit[1], count).astype(np.float32)
+    sh = np.zeros((count, 1 + rest, 3), np.float32)
+    sh[:, 0, :] = (rng.uniform(0.0, 1.0, (count, 3)) - 0.5) / SH_C0
+    band = [1, 1, 1, 2, 2, 2, 2, 2, 3, 3, 3, 3, 3, 3, 3]
+    for k in range(rest):
+        sh[:, 1 + k, :] = rng.normal(0.0, sh_sd / band[k], (count, 3))
+    return dict(means=means, log_scales=log_scales, quats=quats, opacity_logits=logits, sh=sh)
+
+
+VIEW_SIZES = ((256, 256), (512, 384), (960, 540), (1280, 720), (1920, 1080))
+
+
+def sphere_views(count: int = 64, radius: float = 12.0, seed: int = 4, sizes=VIEW_SIZES, fov: float = 60.0):
+    """`count` cameras on a sphere of `radius` about the origin (a Fibonacci
+    lattice, each looking at the centre), each view's size drawn from `sizes`
+    by a seeded generator: a list of (Camera, (width, height))."""
+    rng = np.random.default_rng(seed)
+    golden = math.pi * (3.0 - math.sqrt(5.0))
+    out = []
+    for k in range(count):
+        y = 1.0 - 2.0 * (k + 0.5) / count
+        r = math.sqrt(1.0 - y * y)
+        eye = (radius * r * math.cos(golden * k), radius * y, radius * r * math.sin(golden * k))
+        up = (0.0, 0.0, 1.0) if abs(y) > 0.95 else (0.0, 1.0, 0.0)
+        cam = Camera.looking_at(eye, (0.0, 0.0, 0.0), up)
+        cam.lens = Lens(focal=focal_for_fov(fov))
+        out.append((cam, tuple(sizes[int(rng.integers(0, len(sizes)))])))
+    return out
+
+
 def perturbed(params: dict, seed: int = 1, means_sd=0.01, logit_sd=0.2, sh_sd=0.05) -> dict:
     """The target scene of the fwd+bwd benchmark: means + N(0, 0.01), logits + N(0, 0.2), SH + N(0, 0.05)."""
     rng = np.random.default_rng(seed)
