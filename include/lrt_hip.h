@@ -364,7 +364,9 @@ lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t 
  * No counterpart in the reference, which has no backward: the entries that the
  * boundary's lrt_raster_backward stands for. One cloud of FP32 parameters in
  * device memory, drawn by the same stages as lrt_raster_render; the context
- * keeps what the backward needs until its next render. */
+ * keeps what the backward needs until its next render. The backward reads the
+ * parameter buffers (and filter3d) again: they must stay allocated and
+ * unchanged from the differentiable render to its backward. */
 typedef struct {
     const float* means;           /* N x 3, the cloud's own space */
     const float* log_scales;      /* N x 3: scale = exp(log scale) */

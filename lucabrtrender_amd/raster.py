@@ -577,6 +577,10 @@ class TileRasterizer:
                                 targets.depth.data_ptr() if targets.depth is not None else None)
         st = _abi.FrameStatsC()
         self._diff_serial = getattr(self, "_diff_serial", 0) + 1
+        # The backward reads the parameters (and the 3D filter) again: their
+        # tensors stay referenced here until the next differentiable frame, so
+        # a caller's temporary is not freed and its memory reused in between.
+        self._diff_inputs = splats
         self._check(lib.lrt_raster_render_diff(self._ctx, C.byref(proj), C.byref(splats.c), _mat(object_to_world),
                                                C.byref(settings.to_c()), C.byref(t), C.byref(st)))
         return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
@@ -603,6 +607,7 @@ class TileRasterizer:
         render_diff draws them), kept for backward_views()."""
         st = _abi.FrameStatsC()
         self._diff_serial = getattr(self, "_diff_serial", 0) + 1
+        self._diff_inputs = splats   # read again by backward_views (see render_diff)
         self._check(lib.lrt_raster_render_diff_views(self._ctx, self._views_c(views), len(views), C.byref(splats.c),
                                                      _mat(object_to_world), C.byref(settings.to_c()), C.byref(st)))
         return FrameStats(**{name: getattr(st, name) for name, _ in _abi.FrameStatsC._fields_})
