@@ -34,7 +34,18 @@ def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
 
-def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes):
+def sort_pass_bytes(pairs, passes, key_bits):
+    """HBM bytes of a sort's passes: each reads and writes its pairs. A key of at
+    most 40 bits leaves the first pass as u32 (12 bytes in, 8 out; 8 each way
+    after), a wider one moves 12 bytes each way every pass."""
+    if passes == 0:
+        return 0
+    if key_bits <= 40:
+        return pairs * (12 + 8 + 16 * (passes - 1))
+    return pairs * 24 * passes
+
+
+def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes, key_bits):
     """Algorithmic HBM bytes of each kernel for one frame."""
     return {
         # every splat's packed record and 3D filter read, rect (8) and count (4)
@@ -46,8 +57,8 @@ def stage_bytes(n, visible, pairs, tiles, pixels, sh_words, passes):
         "duplicate": visible * (16 + 16 + 8 + 4 + 4) + pairs * 12,
         # every key read once
         "sort_histogram": pairs * 8,
-        # each pass reads and writes a 12-byte pair
-        "sort_passes": pairs * 24 * passes,
+        # each pass reads and writes its pairs (8 or 12 bytes each way)
+        "sort_passes": sort_pass_bytes(pairs, passes, key_bits),
         # keys read, two u32 a tile written; the ordering reads them again and writes a u32 a tile
         "ranges": pairs * 8 + tiles * 8 + tiles * 12,
         # each pair's slot and record read once a tile, the pixel out (colour 16 + depth 4; the
@@ -76,6 +87,30 @@ def measured_traffic_bytes(stage, passes):
             mb = float(cells[4].replace(",", ""))
             return int(mb * 1e6 * (passes if stage == "sort_passes" else 1))
     return None
+
+
+# FP32 operations of one step of the blend's walk for one pixel (csrc/blend.hip,
+# the splat-only body): the exponent's quadratic form (5 FMA), exp2, the 0.99
+# clamp, T * alpha, the rgb and depth sums (4 FMA), T - weight. An FMA counts 2.
+WALK_FLOPS = 5 * 2 + 1 + 1 + 1 + 4 * 2 + 1
+
+
+def roofline_of(kernel, stages, passes):
+    """The dominant kernel against the ceiling that bounds it: the blend against
+    MI355X's FP32 VALU peak (its walk is VALU-bound; achieved = walk steps x 64
+    pixels x WALK_FLOPS / the launch's time), any other kernel against HBM
+    (achieved = algorithmic bytes / time). traffic: the kernel's measured HBM
+    bytes a launch (rocprofv3 counters, the committed traffic summary)."""
+    st = stages[kernel]
+    traffic = measured_traffic_bytes(kernel, passes)
+    if kernel == "blend":
+        achieved = st["TFLOP/s"]
+        return {"kernel": kernel, "bound": "valu", "achieved": achieved, "peak": FP32_PEAK_TFLOPS, "unit": "TFLOP/s",
+                "frac": round(achieved / FP32_PEAK_TFLOPS, 4), "traffic": traffic,
+                "walk_steps": st["walk_steps"], "flops_per_step": 64 * WALK_FLOPS,
+                "hbm": {"algorithmic_bytes": st["bytes"], "GB/s": st["GB/s"], "frac": st["hbm_frac"]}}
+    return {"kernel": kernel, "bound": "hbm", "achieved": st["GB/s"], "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": st["hbm_frac"], "traffic": traffic}
 
 
 def copy_bandwidth_gbs(torch):
@@ -112,7 +147,10 @@ def cpu_baseline(cfg, settings, seconds_budget=15.0):
         cores = len(os.sched_getaffinity(0))
     except AttributeError:
         cores = os.cpu_count() or 1
-    threads = max(1, min(16, cores))
+    # Every core this process may run on, up to the host share its launcher
+    # grants (OMP_NUM_THREADS when set).
+    share = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    threads = max(1, min(cores, share) if share > 0 else cores)
     from lucabrtrender_amd import raster
     frames, clouds, n = H.oracle_frames(cfg.camera, settings, [(cfg.scene, np.eye(4), raster.SplatEdit())])
     f0 = frames[0]
@@ -148,6 +186,7 @@ def cpu_baseline(cfg, settings, seconds_budget=15.0):
     except OSError:
         pass
     return {"value": done / elapsed, "unit": "frames/s", "cores": threads, "kind": "port",
+            "host_cores": os.cpu_count(), "cores_available": cores,
             "sample": f"{done} whole frames of the same 3M-splat 1080p scene ({elapsed:.1f} s), the oracle's C "
                       f"restatement of the reference forward path (project, 24-bit depth sort, emit, tile "
                       f"sort, ranges, blend) on {threads} OpenMP threads, {model}"}
@@ -196,11 +235,109 @@ def fwd_bwd_leg(torch, raster, scenes, steps):
         rast.render_diff(cam, splats, settings, targets)
     torch.cuda.synchronize()
     fwd = (time.perf_counter() - start) / steps
+    # The same scene packed as the plain forward draws it (fp16 shape and
+    # harmonics, linear opacity, base colour 0.5 + C0 dc), through that forward.
+    import numpy as np
+    n = params["means"].shape[0]
+    fields = np.concatenate([params["means"], 1.0 / (1.0 + np.exp(-params["opacity_logits"][:, None])),
+                             params["log_scales"], params["quats"], 0.5 + scenes.SH_C0 * params["sh"][:, 0, :],
+                             params["sh"][:, 1:, :].reshape(n, -1)], axis=1).astype(np.float32)
+    packed = scenes.pack_splats(fields, 15).to_device()
+    inst = [raster.SplatInstance(packed)]
+    plain_t = raster.RenderTargets(1280, 720)
+    for _ in range(3):
+        plain_stats = rast.render(cam, inst, settings, plain_t)
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(steps):
+        rast.render(cam, inst, settings, plain_t)
+    torch.cuda.synchronize()
+    plain = (time.perf_counter() - start) / steps
     return {"workload": "fwd-bwd-1M-720p", "splats": 1_000_000, "visible": stats.visible, "pairs": stats.pairs,
             "ms_per_view": round(per_view * 1e3, 4), "forward_ms": round(fwd * 1e3, 4),
             "blend_backward_ms": round(bwd[0] / steps, 4), "project_backward_ms": round(bwd[1] / steps, 4),
             "fwd_bwd_over_fwd": round(per_view / fwd, 3),
+            "plain_forward_ms": round(plain * 1e3, 4), "plain_forward_pairs": plain_stats.pairs,
+            "fwd_bwd_over_plain_fwd": round(per_view / plain, 3),
             "data": "synthetic: seeded FP32 parameters, target from means+N(0,0.01), logits+N(0,0.2), SH+N(0,0.05)"}
+
+
+def batch_fwd_bwd_leg(torch, raster, scenes, reps):
+    """batch-fwd-bwd-2M-64: one 2,000,000-Gaussian clustered scene of FP32 parameters
+    (SH degree 3, 3D filter on, 2D filter 0.1), 64 cameras on a sphere of radius 12
+    at sizes drawn from {256x256, 512x384, 960x540, 1280x720, 1920x1080}. Loss:
+    per-pixel L2 against each view's frame of the perturbed scene. A step is the
+    batched differentiable forward of all 64 views, the loss gradients and the
+    batched backward (gradients summed over the views); against it the plain
+    sequence of 64 single-view forward + backward steps."""
+    params = scenes.diff_params_clustered(2_000_000, 0, degree=3)
+    target_p = scenes.perturbed(params, seed=1)
+    views_spec = scenes.sphere_views(64, 12.0, seed=4)
+    cams = [c for c, _ in views_spec]
+    f3 = torch.from_numpy(scenes.filter3d_for(params["means"], cams[::4], 1920, 1080)).cuda()
+    settings = raster.RenderSettings(dilation=0.1, filter3d=True)
+    rast = raster.TileRasterizer()
+
+    def device(p):
+        d = {k: torch.from_numpy(v).cuda() for k, v in p.items()}
+        return d, raster.DiffSplats(d["means"], d["log_scales"], d["quats"], d["opacity_logits"], d["sh"],
+                                    filter3d=f3)
+
+    views = [raster.View(c, raster.RenderTargets(w, h)) for c, (w, h) in views_spec]
+    _, tsplats = device(target_p)
+    rast.render_diff_views(views, tsplats, settings)
+    targets = [v.targets.colour.clone() for v in views]
+    dev, splats = device(params)
+    grads = {k: torch.empty_like(v) for k, v in dev.items()}
+
+    def step():
+        st = rast.render_diff_views(views, splats, settings)
+        gcols = [2.0 * (v.targets.colour - t) for v, t in zip(views, targets)]
+        return st, rast.backward_views(gcols, grads)
+
+    for _ in range(2):
+        step()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    bwd = [0.0, 0.0]
+    for _ in range(reps):
+        stats, ms = step()
+        bwd[0] += ms[0]
+        bwd[1] += ms[1]
+    torch.cuda.synchronize()
+    batch = (time.perf_counter() - start) / reps
+    start = time.perf_counter()
+    for _ in range(reps):
+        rast.render_diff_views(views, splats, settings)
+    torch.cuda.synchronize()
+    batch_fwd = (time.perf_counter() - start) / reps
+    # The same views one at a time: forward, loss gradient, backward.
+    single = [raster.RenderSettings(**{**settings.__dict__, "width": v.targets.width, "height": v.targets.height})
+              for v in views]
+    one = {k: torch.empty_like(v) for k, v in dev.items()}
+
+    def sequence():
+        for v, s_, t in zip(views, single, targets):
+            rast.render_diff(v.camera, splats, s_, v.targets)
+            rast.backward(2.0 * (v.targets.colour - t), one)
+
+    sequence()
+    torch.cuda.synchronize()
+    start = time.perf_counter()
+    for _ in range(reps):
+        sequence()
+    torch.cuda.synchronize()
+    seq = (time.perf_counter() - start) / reps
+    n = len(views)
+    return {"workload": "batch-fwd-bwd-2M-64", "splats": 2_000_000, "views": n,
+            "pixels": sum(v.targets.width * v.targets.height for v in views), "pairs": stats.pairs,
+            "batch_fwd_bwd_ms": round(batch * 1e3, 3), "fwd_bwd_ms_per_view": round(batch * 1e3 / n, 4),
+            "batch_fwd_ms": round(batch_fwd * 1e3, 3), "fwd_ms_per_view": round(batch_fwd * 1e3 / n, 4),
+            "fwd_bwd_over_fwd": round(batch / batch_fwd, 3),
+            "blend_backward_ms": round(bwd[0] / reps, 3), "project_backward_ms": round(bwd[1] / reps, 3),
+            "sequence_fwd_bwd_ms": round(seq * 1e3, 3), "speedup_over_sequence": round(seq / batch, 3),
+            "data": "synthetic: seeded clustered FP32 parameters; target from means+N(0,0.01), logits+N(0,0.2), "
+                    "SH+N(0,0.05)"}
 
 
 def _batch_against_sequence(torch, raster, rast, views, inst, settings, reps):
@@ -394,11 +531,15 @@ def sort_stress_leg(torch, raster, scenes, _abi, reps):
     n, visible, pairs, tiles = (int(v) for v in rast.debug(_abi.DEBUG_COUNTS))
     passes = st.sort_passes
     sort_ms = float(ms[3] + ms[4])
-    sort_bytes = pairs * (8 + 24 * passes)
+    pass_bytes = sort_pass_bytes(pairs, passes, st.key_bits)
+    sort_bytes = pairs * 8 + pass_bytes   # the histogram reads every key once
     return {"workload": "sort-stress-6M-2160p", "splats": n, "visible": visible, "pairs": pairs,
             "sort_passes": passes, "key_bits": st.key_bits, "sort_ms": round(sort_ms, 4),
             "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3),
+            "sort_bytes": int(sort_bytes), "sort_GB/s": round(sort_bytes / (sort_ms * 1e-3) / 1e9, 1),
             "sort_hbm_frac": round(sort_bytes / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "passes_ms": round(float(ms[4]), 4),
+            "passes_hbm_frac": round(pass_bytes / (float(ms[4]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "duplicate_ms": round(float(ms[2]), 4), "blend_ms": round(float(ms[6]), 4),
             "gpu_ms_per_frame": round(float(ms[7]), 4), "wall_ms_per_frame": round(wall * 1e3, 4)}
 
@@ -470,6 +611,8 @@ def main():
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="fwd+bwd views (1M splats, 720p) timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="sort-stress frames (6M splats, 2160p) timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="64-view batches (2M splats) timed; 0 skips it")
+    ap.add_argument("--batch-bwd-reps", type=int, default=3,
+                    help="batched fwd+bwd steps (2M splats, 64 views) timed; 0 skips it")
     ap.add_argument("--load-reps", type=int, default=3, help="3M-record decodes timed; 0 skips it")
     ap.add_argument("--lod-reps", type=int, default=5, help="LOD cuts and frames timed; 0 skips it")
     ap.add_argument("--composite-reps", type=int, default=5,
@@ -537,7 +680,8 @@ def main():
     counts = rast.debug(_abi.DEBUG_COUNTS)
     n, visible, pairs, tiles = (int(v) for v in counts)
     passes = timed_stats.sort_passes
-    sbytes = stage_bytes(n, visible, pairs, tiles, cfg.width * cfg.height, cfg.scene.sh_words, passes)
+    sbytes = stage_bytes(n, visible, pairs, tiles, cfg.width * cfg.height, cfg.scene.sh_words, passes,
+                         timed_stats.key_bits)
     stages = {}
     for k, name in enumerate(STAGES):
         ms = float(kernel_ms[k])
@@ -546,9 +690,12 @@ def main():
                         "hbm_frac": round(gbs / HBM_PEAK_GBS, 4)}
     dominant = max(STAGES, key=lambda s: stages[s]["ms"])
     sort_ms = stages["sort_histogram"]["ms"] + stages["sort_passes"]["ms"]
-    roofline = {"kernel": dominant, "bound": "hbm", "achieved": stages[dominant]["GB/s"], "peak": HBM_PEAK_GBS,
-                "unit": "GB/s", "frac": stages[dominant]["hbm_frac"],
-                "traffic": measured_traffic_bytes(dominant, passes)}
+    # The blend's walk steps (list records walked, each for a wave's 64 pixels),
+    # counted by the same blend run again.
+    walk_steps = int(rast.debug(_abi.DEBUG_WALK_STEPS)[0])
+    stages["blend"]["walk_steps"] = walk_steps
+    stages["blend"]["TFLOP/s"] = round(walk_steps * 64 * WALK_FLOPS / (stages["blend"]["ms"] * 1e-3) / 1e12, 3)
+    roofline = roofline_of(dominant, stages, passes)
     copy_gbs = copy_bandwidth_gbs(torch)
     # The blend at each batch depth and launch order, for the record.
     variants = {}
@@ -573,6 +720,10 @@ def main():
     if args.fwd_bwd_steps > 0:
         log("fwd+bwd (1M splats, 720p) ...")
         fwd_bwd = fwd_bwd_leg(torch, raster, scenes, args.fwd_bwd_steps)
+    batch_fwd_bwd = None
+    if args.batch_bwd_reps > 0:
+        log("batched fwd+bwd (2M splats, 64 views) ...")
+        batch_fwd_bwd = batch_fwd_bwd_leg(torch, raster, scenes, args.batch_bwd_reps)
     batched = None
     if args.views_reps > 0:
         log("batched views (2M splats, 64 views) ...")
@@ -628,6 +779,7 @@ def main():
         "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3) if sort_ms > 0 else None,
         "copy_GB/s": round(copy_gbs, 1),
         "fwd_bwd": fwd_bwd,
+        "batch_fwd_bwd": batch_fwd_bwd,
         "views": batched,
         "views_3m_1080p": batched_1080p,
         "sort_stress": stress,

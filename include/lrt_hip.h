@@ -330,6 +330,15 @@ lrt_status lrt_reference_render(lrt_context* ctx, const lrt_projection* projecti
                                 const lrt_splat_instance* instances, uint32_t instance_count,
                                 const lrt_render_settings* settings, double* colour, double* depth,
                                 int32_t keyed);
+/* The same brute force on a subset of the frame's pixels (the reference's
+ * renderer draws whole frames; a subset is what a full-size check can afford):
+ * pixels (device, pixel_count u32, each y * width + x, bottom row first);
+ * colour receives 4 doubles and depth (may be NULL) 1 for each listed pixel,
+ * in list order. */
+lrt_status lrt_reference_render_pixels(lrt_context* ctx, const lrt_projection* projection,
+                                       const lrt_splat_instance* instances, uint32_t instance_count,
+                                       const lrt_render_settings* settings, const uint32_t* pixels,
+                                       uint32_t pixel_count, double* colour, double* depth, int32_t keyed);
 
 /* ---- batches of views -------------------------------------------------------------
  * No single counterpart in the reference, which draws one camera a render()
@@ -435,8 +444,12 @@ typedef enum {
                                     scan, duplicate, sort histogram, sort passes, ranges (with
                                     the heaviest-first tile ordering), blend, and their sum (HIP
                                     events on the context's stream) */
-    LRT_DEBUG_TILE_ORDER = 11    /* u32 a tile: the order the blend took the tiles in (empty
+    LRT_DEBUG_TILE_ORDER = 11,   /* u32 a tile: the order the blend took the tiles in (empty
                                     when the last render used index order) */
+    LRT_DEBUG_WALK_STEPS = 12    /* u32[1]: the last lrt_raster_render's blend run again (into
+                                    scratch, not the targets) counting the LDS-list records its
+                                    quadrant waves walked; each step is one record for the
+                                    wave's 64 pixels. Empty after other kinds of frame. */
 } lrt_debug_buffer;
 lrt_status lrt_raster_debug(lrt_context* ctx, int32_t which, void* host, size_t bytes, size_t* needed);
 

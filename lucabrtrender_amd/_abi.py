@@ -19,7 +19,7 @@ STATUS_NAMES = {
 }
 
 DEBUG_COUNTS, DEBUG_TOUCHED, DEBUG_OFFSETS, DEBUG_PROJ, DEBUG_KEYS, DEBUG_VALUES, DEBUG_RANGES, \
-    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS, DEBUG_KERNEL_MS, DEBUG_TILE_ORDER = range(12)
+    DEBUG_LAST_COUNT, DEBUG_FINAL_T, DEBUG_RECTS, DEBUG_KERNEL_MS, DEBUG_TILE_ORDER, DEBUG_WALK_STEPS = range(13)
 
 
 class Lens(C.Structure):
@@ -202,6 +202,9 @@ SIGNATURES = {
                                               C.POINTER(RenderTargetsC), C.POINTER(FrameStatsC)]),
     "lrt_reference_render": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.c_uint32,
                                        C.POINTER(RenderSettingsC), C.c_void_p, C.c_void_p, C.c_int32]),
+    "lrt_reference_render_pixels": (C.c_int, [C.c_void_p, C.POINTER(ProjectionC), C.POINTER(SplatInstanceC),
+                                              C.c_uint32, C.POINTER(RenderSettingsC), C.c_void_p, C.c_uint32,
+                                              C.c_void_p, C.c_void_p, C.c_int32]),
     "lrt_raster_render_views": (C.c_int, [C.c_void_p, C.POINTER(ViewC), C.c_uint32, C.POINTER(SplatInstanceC),
                                           C.c_uint32, C.POINTER(RenderSettingsC), C.POINTER(FrameStatsC)]),
     "lrt_autotune_views": (C.c_int, [C.c_void_p, C.POINTER(ViewC), C.c_uint32, C.POINTER(SplatInstanceC), C.c_uint32,

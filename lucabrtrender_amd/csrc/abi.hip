@@ -449,6 +449,25 @@ lrt_status lrt_reference_render(lrt_context* ctx, const lrt_projection* projecti
                                                    depth, keyed != 0));
 }
 
+lrt_status lrt_reference_render_pixels(lrt_context* ctx, const lrt_projection* projection,
+                                       const lrt_splat_instance* instances, uint32_t instance_count,
+                                       const lrt_render_settings* settings, const uint32_t* pixels,
+                                       uint32_t pixel_count, double* colour, double* depth, int32_t keyed) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    if (projection == nullptr || settings == nullptr || (instance_count > 0 && instances == nullptr) ||
+        (pixel_count > 0 && pixels == nullptr)) {
+        return finish(ctx, lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null argument"));
+    }
+    if (pixel_count == 0) return finish(ctx, {});
+    std::vector<lrt::View> none;
+    std::vector<lrt::SplatCloud> clouds;
+    std::vector<lrt::SplatInstance> inst;
+    lrt::Status s = toBatch(nullptr, 0, instances, instance_count, none, clouds, inst);
+    if (!s.ok()) return finish(ctx, s);
+    return finish(ctx, ctx->raster.referenceRender(toProjection(*projection), inst, toSettings(*settings), colour,
+                                                   depth, keyed != 0, pixels, pixel_count));
+}
+
 lrt_status lrt_raster_render_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
                                    const lrt_splat_instance* instances, uint32_t instance_count,
                                    const lrt_render_settings* settings, lrt_frame_stats* stats) {

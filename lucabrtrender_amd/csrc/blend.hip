@@ -173,7 +173,8 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
                                               float* __restrict__ depth, uint32_t* __restrict__ lastCount,
                                               float* __restrict__ finalT, float4* __restrict__ accum,
                                               WalkState* state = nullptr, float4* __restrict__ partialA = nullptr,
-                                              float2* __restrict__ partialB = nullptr) {
+                                              float2* __restrict__ partialB = nullptr,
+                                              uint32_t* __restrict__ walkSteps = nullptr) {
     static_assert(!(COMPOSITE && MODE != kFinish), "a composite walk is not split");
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
     float4* const sGeom = list.geom;
@@ -229,6 +230,7 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
     bool opaqueHit = false;
     float oR = 0.0f, oG = 0.0f, oB = 0.0f, oZ = 0.0f;
     float zCarry = -3.4e38f;   // the largest view z of the pairs staged so far
+    uint32_t walked = 0;       // list records walked (kGroup a step)
     uint32_t next = range.x;   // the first pair of the round in `staged`
     while (next < range.y) {
         // 1-2. stage rounds into the list while a whole round still fits
@@ -300,6 +302,7 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
         // 3. walk the list: all kGroup records' reads issued together, then the
         // pixels blend them in order
         for (uint32_t j = 0; j < padded; j += kGroup) {
+            walked += kGroup;
             float4 g0[kGroup], g1[kGroup], g2[kGroup];
 #pragma unroll
             for (uint32_t k = 0; k < kGroup; ++k) {
@@ -351,6 +354,9 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
         if (__ballot(active) == 0) break;
         __syncthreads();   // the list is refilled next
     }
+    // The walk's records, counted on request (the blend's VALU work is
+    // 64 lanes x these steps).
+    if (walkSteps != nullptr && lane == 0) atomicAdd(walkSteps, walked);
     if constexpr (MODE == kPartial) {
         partialA[lane] = make_float4(r, g, b, depthSum);
         partialB[lane] = make_float2(t, __uint_as_float(last));
@@ -377,7 +383,8 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
                                                       const float* __restrict__ underDepth,
                                                       float4* __restrict__ colour, float* __restrict__ depth,
                                                       uint32_t* __restrict__ lastCount,
-                                                      float* __restrict__ finalT, float4* __restrict__ accum) {
+                                                      float* __restrict__ finalT, float4* __restrict__ accum,
+                                                      uint32_t* __restrict__ walkSteps) {
     const uint32_t tiles = f.tilesX * f.tilesY;
     const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
     const uint32_t place = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
@@ -388,7 +395,7 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     const uint2 rg = ranges[tile];
     blendQuadrant<DEPTH_MODE, LIST, COMPOSITE, kFinish, TRACK>(list, f, tile, quadrant, rg, rg.x, slots, pairSlots, proj,
                                                                underColour, underDepth, colour, depth, lastCount,
-                                                               finalT, accum);
+                                                               finalT, accum, nullptr, nullptr, nullptr, walkSteps);
 }
 
 // Every quadrant of every view of a batch. As a grid (PERSISTENT false) block b
@@ -709,7 +716,7 @@ template <uint32_t DEPTH_MODE, bool COMPOSITE>
 void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint32_t* order, const uint2* ranges,
                  const uint32_t* pairSlots, const float4* proj, const float4* underColour, const float* underDepth,
                  float4* colour, float* depth, uint32_t* lastCount, float* finalT, float4* accum,
-                 hipStream_t stream) {
+                 hipStream_t stream, uint32_t* walkSteps = nullptr) {
     // Four blocks a tile, the grid rounded up to whole groups of eight tiles.
     // The last contributor is tracked only where it is kept.
     const uint32_t tiles = f.tilesX * f.tilesY;
@@ -717,7 +724,7 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
     const auto go = [&](auto listLen, auto track) {
         blendKernel<DEPTH_MODE, decltype(listLen)::value, COMPOSITE, decltype(track)::value>
             <<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour,
-                                            depth, lastCount, finalT, accum);
+                                            depth, lastCount, finalT, accum, walkSteps);
     };
     using L128 = std::integral_constant<uint32_t, 128>;
     using L256 = std::integral_constant<uint32_t, 256>;
@@ -817,14 +824,14 @@ void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint
 
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-           float* finalT, float4* accum, hipStream_t stream) {
+           float* finalT, float4* accum, hipStream_t stream, uint32_t* walkSteps) {
     if (f.tilesX * f.tilesY == 0) return;
     if (f.depthMode == 0) {
         launchBlend<0, false>(list, f, slots, order, ranges, pairSlots, proj, nullptr, nullptr, colour, depth,
-                              lastCount, finalT, accum, stream);
+                              lastCount, finalT, accum, stream, walkSteps);
     } else {
         launchBlend<1, false>(list, f, slots, order, ranges, pairSlots, proj, nullptr, nullptr, colour, depth,
-                              lastCount, finalT, accum, stream);
+                              lastCount, finalT, accum, stream, walkSteps);
     }
 }
 

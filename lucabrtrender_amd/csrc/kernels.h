@@ -89,6 +89,16 @@ struct TileDepthLayout {
     uint32_t depthBits = 24;
 };
 
+// A TileDepth sort of at most kNarrowMaxBits key bits leaves its keys as u32
+// (in the u64 buffers' first half-words): the virtual key shifted right by
+// narrowShift(keyBits), so the tile id is the stored key >> (depthBits -
+// narrowShift(keyBits)). Wider TileDepth sorts and Plain sorts keep u64 keys.
+constexpr uint32_t kNarrowMaxBits = 40;
+constexpr bool sortNarrows(KeyMode mode, uint32_t keyBits) {
+    return mode == KeyMode::TileDepth && keyBits <= kNarrowMaxBits;
+}
+constexpr uint32_t narrowShift(uint32_t keyBits) { return keyBits > 32 ? keyBits - 32 : 0u; }
+
 struct SortWorkspace {
     uint32_t* histogram;   // one row of passes * 256 digit counts per histogram workgroup
     uint32_t* offsets;     // passes * 256
@@ -103,7 +113,8 @@ uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits);
 SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32_t* error);
 // Stable LSD radix sort, eight bits a pass, one onesweep launch a pass. The
 // result is in (keys, values) after an even number of passes and in
-// (keysAlt, valuesAlt) after an odd one; returns true for the latter.
+// (keysAlt, valuesAlt) after an odd one; returns true for the latter. The
+// result's keys are u32 when sortNarrows(mode, keyBits) (see above).
 // afterHistogram (may be null) is recorded between the histogram and the passes.
 // alsoZero's first alsoZeroWords words are zeroed before the passes (by the
 // histogram launch, or by a fill when there is nothing to sort).
@@ -141,6 +152,13 @@ void splatBounds(const float4* positions, uint32_t count, float* partial, float*
 // zeroed: the caller has already set every range to (0, 0).
 void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream,
                 const uint32_t* nDevice = nullptr, bool zeroed = false);
+// The same over a narrowed sort's u32 keys: the tile is key >> tileShift.
+void tileRangesNarrow(const uint32_t* keys, uint32_t tileShift, uint32_t pairs, uint32_t tiles, uint2* ranges,
+                      hipStream_t stream, const uint32_t* nDevice = nullptr, bool zeroed = false);
+// u64 keys tile << 32 | bits(view z) rebuilt from a narrowed sort's u32 keys
+// and the pairs' slots (the view z read from each slot's record), for readback.
+void widenKeys(const uint32_t* keys, uint32_t tileShift, const uint32_t* slots, const float4* proj, uint32_t pairs,
+               uint64_t* out, hipStream_t stream);
 
 // ---- blend ------------------------------------------------------------------
 // Tile indices, heaviest tile (most pairs) first, for the blend's launch order.
@@ -152,7 +170,9 @@ void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t
 // linearisation and the background.
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-           float* finalT, float4* accum, hipStream_t stream);
+           float* finalT, float4* accum, hipStream_t stream, uint32_t* walkSteps = nullptr);
+// (walkSteps, when not null, receives the sum over quadrant waves of the list
+// records each walked: the (pixel, record) steps are 64 times that.)
 // The composite walk: a record with colour.w = 1 (a disc) that reaches a pixel
 // ends its walk with its colour behind the splats; with underDepth non-null a
 // pixel whose under depth is above 0 ends its walk at the first record at or
@@ -204,9 +224,11 @@ void referenceProject(const FrameParams& f, const float4* positions, const uint4
                       const float* filter3d, void* records, uint64_t* keys, uint32_t* values, uint32_t* live,
                       bool keyed, hipStream_t stream);
 // Every pixel over the first *live slots of `order`, in FP64: colour 4 doubles
-// a pixel, depth one (may be null).
+// a pixel, depth one (may be null). With pixelList, only the listCount pixels
+// it names (y * width + x), output k for pixelList[k].
 void referenceBlend(const FrameParams& f, const void* records, const uint32_t* order, const uint32_t* live,
-                    double* colour, double* depth, hipStream_t stream);
+                    double* colour, double* depth, hipStream_t stream, const uint32_t* pixelList = nullptr,
+                    uint32_t listCount = 0);
 
 // ---- differentiable path -------------------------------------------------------
 // An FP32-parameter cloud (device pointers): means N x 3, log scales N x 3,

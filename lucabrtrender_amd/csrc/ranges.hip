@@ -2,7 +2,7 @@
 // boundary is written by exactly one thread; tiles with no pairs keep (0, 0).
 // The reference's splatTilesClear + splatRanges (shaders/lrt/splat/
 // splat_tiles_clear.slang:8-17, splat_ranges.slang:11-25), with the tile read
-// from the high word of the 64-bit key.
+// from the high word of the 64-bit key or from a narrowed u32 key.
 #include <hip/hip_runtime.h>
 
 #include "kernels.h"
@@ -20,11 +20,17 @@ __device__ __forceinline__ uint32_t fromLane(uint32_t v, uint32_t missing) {
                                                              0xF, 0xF, false));
 }
 
-// Thread g takes pairs [4g, 4g + 4) with two 16-byte loads; the tile before
-// its first pair and after its last come from the neighbouring lanes (DPP
-// wave shifts), the wave's two end lanes loading theirs. Keys must be 16-byte
-// aligned (the rasteriser's sort buffers are).
-__global__ __launch_bounds__(kBlock) void rangesKernel(const uint64_t* __restrict__ keys, uint32_t cap,
+// The tile of a sorted key: the high word of a u64 key, or a narrowed u32
+// key shifted right by tileShift.
+__device__ __forceinline__ uint32_t tileOf(uint64_t key, uint32_t) { return static_cast<uint32_t>(key >> 32); }
+__device__ __forceinline__ uint32_t tileOf(uint32_t key, uint32_t tileShift) { return key >> tileShift; }
+
+// Thread g takes pairs [4g, 4g + 4) with 16-byte loads (two of u64 keys, one
+// of u32); the tile before its first pair and after its last come from the
+// neighbouring lanes (DPP wave shifts), the wave's two end lanes loading
+// theirs. Keys must be 16-byte aligned (the rasteriser's sort buffers are).
+template <typename Key>
+__global__ __launch_bounds__(kBlock) void rangesKernel(const Key* __restrict__ keys, uint32_t tileShift, uint32_t cap,
                                                        const uint32_t* __restrict__ nDevice, uint32_t tiles,
                                                        uint2* __restrict__ ranges) {
     uint32_t pairs = cap;
@@ -35,20 +41,28 @@ __global__ __launch_bounds__(kBlock) void rangesKernel(const uint64_t* __restric
     if (i0 - lane * kPer >= pairs) return;
     uint32_t t[kPer];
     if (i0 + kPer <= pairs) {
-        const uint4* k4 = reinterpret_cast<const uint4*>(keys + i0);
-        const uint4 a = k4[0], b = k4[1];
-        t[0] = a.y;
-        t[1] = a.w;
-        t[2] = b.y;
-        t[3] = b.w;
+        if constexpr (sizeof(Key) == 8) {
+            const uint4* k4 = reinterpret_cast<const uint4*>(keys + i0);
+            const uint4 a = k4[0], b = k4[1];
+            t[0] = a.y;
+            t[1] = a.w;
+            t[2] = b.y;
+            t[3] = b.w;
+        } else {
+            const uint4 a = *reinterpret_cast<const uint4*>(keys + i0);
+            t[0] = a.x >> tileShift;
+            t[1] = a.y >> tileShift;
+            t[2] = a.z >> tileShift;
+            t[3] = a.w >> tileShift;
+        }
     } else {
 #pragma unroll
-        for (uint32_t j = 0; j < kPer; ++j) t[j] = i0 + j < pairs ? static_cast<uint32_t>(keys[i0 + j] >> 32) : kNone;
+        for (uint32_t j = 0; j < kPer; ++j) t[j] = i0 + j < pairs ? tileOf(keys[i0 + j], tileShift) : kNone;
     }
     uint32_t before = fromLane<0x138>(t[kPer - 1], kNone);   // wave_shr:1, lane - 1's last
     uint32_t after = fromLane<0x130>(t[0], kNone);           // wave_shl:1, lane + 1's first
-    if (lane == 0) before = i0 > 0 && i0 - 1 < pairs ? static_cast<uint32_t>(keys[i0 - 1] >> 32) : kNone;
-    if (lane == 63) after = i0 + kPer < pairs ? static_cast<uint32_t>(keys[i0 + kPer] >> 32) : kNone;
+    if (lane == 0) before = i0 > 0 && i0 - 1 < pairs ? tileOf(keys[i0 - 1], tileShift) : kNone;
+    if (lane == 63) after = i0 + kPer < pairs ? tileOf(keys[i0 + kPer], tileShift) : kNone;
 #pragma unroll
     for (uint32_t j = 0; j < kPer; ++j) {
         const uint32_t tile = t[j];
@@ -60,15 +74,45 @@ __global__ __launch_bounds__(kBlock) void rangesKernel(const uint64_t* __restric
     }
 }
 
+__global__ __launch_bounds__(kBlock) void widenKeysKernel(const uint32_t* __restrict__ keys, uint32_t tileShift,
+                                                          const uint32_t* __restrict__ slots,
+                                                          const float4* __restrict__ proj, uint32_t pairs,
+                                                          uint64_t* __restrict__ out) {
+    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    if (i >= pairs) return;
+    const uint64_t tile = keys[i] >> tileShift;
+    const float z = proj[static_cast<uint64_t>(slots[i]) * 3].z;
+    out[i] = tile << 32 | __float_as_uint(z);
+}
+
+}   // namespace
+
+namespace {
+template <typename Key>
+void launchRanges(const Key* keys, uint32_t tileShift, uint32_t pairs, uint32_t tiles, uint2* ranges,
+                  hipStream_t stream, const uint32_t* nDevice, bool zeroed) {
+    if (!zeroed) fill32(reinterpret_cast<uint32_t*>(ranges), 0u, static_cast<uint64_t>(tiles) * 2, stream);
+    if (pairs == 0) return;
+    const uint64_t threads = (uint64_t{pairs} + kPer - 1) / kPer;
+    rangesKernel<Key><<<static_cast<uint32_t>((threads + kBlock - 1) / kBlock), kBlock, 0, stream>>>(
+        keys, tileShift, pairs, nDevice, tiles, ranges);
+}
 }   // namespace
 
 void tileRanges(const uint64_t* keys, uint32_t pairs, uint32_t tiles, uint2* ranges, hipStream_t stream,
                 const uint32_t* nDevice, bool zeroed) {
-    if (!zeroed) fill32(reinterpret_cast<uint32_t*>(ranges), 0u, static_cast<uint64_t>(tiles) * 2, stream);
+    launchRanges(keys, 32u, pairs, tiles, ranges, stream, nDevice, zeroed);
+}
+
+void tileRangesNarrow(const uint32_t* keys, uint32_t tileShift, uint32_t pairs, uint32_t tiles, uint2* ranges,
+                      hipStream_t stream, const uint32_t* nDevice, bool zeroed) {
+    launchRanges(keys, tileShift, pairs, tiles, ranges, stream, nDevice, zeroed);
+}
+
+void widenKeys(const uint32_t* keys, uint32_t tileShift, const uint32_t* slots, const float4* proj, uint32_t pairs,
+               uint64_t* out, hipStream_t stream) {
     if (pairs == 0) return;
-    const uint64_t threads = (uint64_t{pairs} + kPer - 1) / kPer;
-    rangesKernel<<<static_cast<uint32_t>((threads + kBlock - 1) / kBlock), kBlock, 0, stream>>>(keys, pairs, nDevice,
-                                                                                               tiles, ranges);
+    widenKeysKernel<<<(pairs + kBlock - 1) / kBlock, kBlock, 0, stream>>>(keys, tileShift, slots, proj, pairs, out);
 }
 
 }   // namespace lrt::kern

@@ -220,8 +220,11 @@ public:
     // every instance in (view z, index) order, or with keyed in the
     // rasteriser's (24-bit depth key, slot) order, per pixel, no cap. colour: 4
     // doubles a pixel, depth 1 (may be null), device, settings' size.
+    // With pixels (device, pixelCount entries y * width + x) only those pixels,
+    // output k for pixels[k].
     Status referenceRender(const Projection& projection, std::span<const SplatInstance> instances,
-                           const RenderSettings& settings, double* colour, double* depth, bool keyed);
+                           const RenderSettings& settings, double* colour, double* depth, bool keyed,
+                           const uint32_t* pixels = nullptr, uint32_t pixelCount = 0);
 
     // Gradients of the last renderDiff's parameters given dL/d(colour) (one
     // float4 a pixel, bottom row first). millis (may be null) receives the GPU
@@ -315,6 +318,17 @@ private:
     bool lastWalkDropped_ = false;   // the last frame kept no per-pixel walk state
     uint64_t lastPixels_ = 0;
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
+    bool lastNarrow_ = false;       // the sorted keys are u32 (kern::sortNarrows)
+    uint32_t lastTileShift_ = 0;    // their tile is key >> lastTileShift_
+    bool lastSlotsSorted_ = false;  // the sorted values are pair indices; sortedSlots_ holds the slots
+    // The last plain (one view, splats only, not split) blend's launch, for
+    // the walk-step count of LRT_DEBUG_WALK_STEPS.
+    bool lastBlendPlain_ = false;
+    FrameParams lastBlendFrame_{};
+    uint32_t lastBlendList_ = 0, lastBlendSlots_ = 0;
+    const uint32_t* lastBlendOrder_ = nullptr;
+    const uint32_t* lastBlendPairSlots_ = nullptr;
+    DeviceBuffer walkSteps_;
     // Batches: the frame constants and view records (pinned host copies that
     // a launch uploads, and their device arrays), the capture stream and the
     // graph of the last batch shape.

@@ -285,6 +285,7 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     }
     diffReady_ = false;
     diffViewsReady_ = false;
+    lastBlendPlain_ = false;
     const bool composite = under != nullptr || !points.empty();
     return finishFrame(first, splatCount, settings, targets, statsOut, false, composite, under);
 }
@@ -325,6 +326,7 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
     depthEntries_ = projectBlocks(cloud.count);
     diffReady_ = false;
     diffViewsReady_ = false;
+    lastBlendPlain_ = false;
     if (!(s = finishFrame(f, cloud.count, settings, targets, statsOut, true)).ok()) return s;
     diffCloud_ = cloud;
     diffFrame_ = f;
@@ -420,7 +422,14 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
                                        reinterpret_cast<uint32_t*>(ranges_.as<uint2>()), uint64_t{tiles} * 2);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
-    kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_, nullptr, true);
+    lastNarrow_ = kern::sortNarrows(kern::KeyMode::TileDepth, keyBits);
+    lastTileShift_ = layout.depthBits - kern::narrowShift(keyBits);
+    if (lastNarrow_) {
+        kern::tileRangesNarrow(keys_[sorted].as<uint32_t>(), lastTileShift_, pairs, tiles, ranges_.as<uint2>(),
+                               stream_, nullptr, true);
+    } else {
+        kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_, nullptr, true);
+    }
     // The blend reads slots; a training frame's sort carried pair indices.
     const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
     if (training) {
@@ -470,6 +479,12 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
                     proj_.as<float4>(), targets.colour, targets.depth, lastCountOut, finalTOut,
                     training ? accum_.as<float4>() : nullptr, stream_);
     }
+    lastBlendPlain_ = !split && !composite && !training;   // a training frame's accum_ is kept
+    lastBlendFrame_ = first;
+    lastBlendList_ = settings.blendBatch;
+    lastBlendSlots_ = splatCount;
+    lastBlendOrder_ = order;
+    lastBlendPairSlots_ = pairSlots;
     mark(8);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream_), "frame")).ok())
         return s;
@@ -490,6 +505,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     lastWalkDropped_ = !keepWalk;
     lastPixels_ = uint64_t{settings.width} * settings.height;
     lastSorted_ = sorted;
+    lastSlotsSorted_ = training;
     stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
     if (timed) {
         // project, scan, duplicate, sort histogram, sort passes, ranges, blend
@@ -563,7 +579,15 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
                                        keyBits, ws, stream, timed ? events_[5] : nullptr, counters + 0, viewsLayout_);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
-    kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream, counters + 0);
+    lastNarrow_ = kern::sortNarrows(kern::KeyMode::TileDepth, keyBits);
+    lastTileShift_ = viewsLayout_.depthBits - kern::narrowShift(keyBits);
+    if (lastNarrow_) {
+        kern::tileRangesNarrow(keys_[sorted].as<uint32_t>(), lastTileShift_, pairCapacity_, tiles, ranges_.as<uint2>(),
+                               stream, counters + 0);
+    } else {
+        kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream,
+                         counters + 0);
+    }
     // The blend reads slots; a training batch's sort carried pair indices.
     const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
     if (training) {
@@ -581,6 +605,7 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
     mark(8);
     (void)hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream);
     lastSorted_ = sorted;
+    lastSlotsSorted_ = training;
 }
 
 Status TileRasterizer::renderViews(std::span<const View> views, std::span<const SplatInstance> instances,
@@ -669,6 +694,7 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
     // The scratch this batch may regrow holds the last differentiable frame.
     diffReady_ = false;
     diffViewsReady_ = false;
+    lastBlendPlain_ = false;
     if (!(s = reserveSplats(static_cast<uint32_t>(std::max<uint64_t>(slots, 1)))).ok()) return s;
     if (tiles > tileCapacity_) {
         if (!(s = ranges_.reserve(tiles * 8, "tile.ranges")).ok()) return s;
@@ -866,7 +892,8 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
 }
 
 Status TileRasterizer::referenceRender(const Projection& projection, std::span<const SplatInstance> instances,
-                                       const RenderSettings& settings, double* colour, double* depth, bool keyed) {
+                                       const RenderSettings& settings, double* colour, double* depth, bool keyed,
+                                       const uint32_t* pixels, uint32_t pixelCount) {
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
     if (settings.width == 0 || settings.height == 0 || colour == nullptr) {
         return Status::error(ErrorCode::InvalidArgument, "a frame needs a size and a colour target");
@@ -921,7 +948,7 @@ Status TileRasterizer::referenceRender(const Projection& projection, std::span<c
                                      refKeys_[1].as<uint64_t>(), refValues_[1].as<uint32_t>(),
                                      static_cast<uint32_t>(total), kern::KeyMode::Plain, 0, 64, ws, stream_);
     kern::referenceBlend(first, refRecords_.get(), refValues_[alt ? 1 : 0].as<uint32_t>(), counters + 7, colour,
-                         depth, stream_);
+                         depth, stream_, pixels, pixels != nullptr ? pixelCount : 0u);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 32, hipMemcpyDeviceToHost, stream_), "reference")).ok())
         return s;
     if (!(s = check(hipStreamSynchronize(stream_), "reference")).ok()) return s;
@@ -1051,6 +1078,7 @@ Status TileRasterizer::prefixSum(const uint32_t* in, uint32_t* out, uint32_t* to
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
     diffReady_ = false;   // the scratch it may regrow holds the last frame
     diffViewsReady_ = false;
+    lastBlendPlain_ = false;
     if (in == nullptr || out == nullptr || total == nullptr) return Status::error(ErrorCode::InvalidArgument, "null buffer");
     if (in == out) return Status::error(ErrorCode::InvalidArgument, "the scan's input and output must differ");
     Status s = check(hipSetDevice(device_), "hipSetDevice");
@@ -1096,6 +1124,7 @@ Status TileRasterizer::decodeCloud(const io::RawSplats& raw, uint32_t maxDegree,
     if (!s.ok()) return s;
     diffReady_ = false;   // the per-splat scratch it reuses holds the last frame
     diffViewsReady_ = false;
+    lastBlendPlain_ = false;
     lastSplats_ = lastVisible_ = lastPairs_ = lastTiles_ = 0;
     const uint64_t recordBytes = uint64_t{e.floatsPerRecord} * 4;
     constexpr uint64_t kSliceBytes = 256ull << 20;
@@ -1197,7 +1226,7 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     case 1: src = touched_.get(); size = size_t{lastSplats_} * 4; break;
     case 2: src = offsets_.get(); size = size_t{lastSplats_} * 4; break;
     case 3: src = proj_.get(); size = size_t{lastSplats_} * sizeof(ProjRecord); break;
-    case 4: src = keys_[lastSorted_].get(); size = size_t{lastPairs_} * 8; break;
+    case 4: src = keys_[lastSorted_].get(); size = size_t{lastPairs_} * 8; break;   // widened below when narrow
     case 5: src = values_[lastSorted_].get(); size = size_t{lastPairs_} * 4; break;
     case 6: src = ranges_.get(); size = size_t{lastTiles_} * 8; break;
     case 7: src = lastCount_.get(); size = lastWalkDropped_ ? 0 : size_t{lastPixels_} * 4; break;
@@ -1205,6 +1234,7 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     case 9: src = rects_.get(); size = size_t{lastSplats_} * 8; break;
     case 10: size = sizeof kernelMs_; break;
     case 11: src = tileOrder_.get(); size = lastOrdered_ ? size_t{lastTiles_} * 4 : 0; break;
+    case 12: size = lastBlendPlain_ ? 4 : 0; break;
     default: return Status::error(ErrorCode::InvalidArgument, "unknown debug buffer " + std::to_string(which));
     }
     if (needed != nullptr) *needed = size;
@@ -1221,6 +1251,25 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     if (size == 0) return {};
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
+    if (which == 12) {
+        // The same blend again, its images into scratch, counting the walk.
+        const uint64_t pixels = uint64_t{lastBlendFrame_.width} * lastBlendFrame_.height;
+        if (!(s = accum_.reserve(std::max<uint64_t>(pixels, 1) * 16, "walk.colour")).ok()) return s;
+        if (!(s = walkSteps_.reserve(4, "walk.steps")).ok()) return s;
+        kern::fill32(walkSteps_.as<uint32_t>(), 0u, 1, stream_);
+        kern::blend(lastBlendFrame_, lastBlendList_, lastBlendSlots_, lastBlendOrder_, ranges_.as<uint2>(),
+                    lastBlendPairSlots_, proj_.as<float4>(), accum_.as<float4>(), nullptr, nullptr, nullptr, nullptr,
+                    stream_, walkSteps_.as<uint32_t>());
+        src = walkSteps_.get();
+    }
+    if (which == 4 && lastNarrow_) {
+        // The sort left u32 keys: rebuild tile << 32 | bits(view z) from them and
+        // the pairs' slots into the other key buffer (free once a frame is done).
+        const uint32_t* slots = lastSlotsSorted_ ? sortedSlots_.as<uint32_t>() : values_[lastSorted_].as<uint32_t>();
+        kern::widenKeys(keys_[lastSorted_].as<uint32_t>(), lastTileShift_, slots, proj_.as<float4>(), lastPairs_,
+                        keys_[1 - lastSorted_].as<uint64_t>(), stream_);
+        src = keys_[1 - lastSorted_].get();
+    }
     if (!(s = check(hipMemcpyAsync(host, src, size, hipMemcpyDeviceToHost, stream_), "debug copy")).ok()) return s;
     return check(hipStreamSynchronize(stream_), "debug copy");
 }

@@ -192,13 +192,16 @@ __device__ __forceinline__ double srgbToLinearF64(double c) {
 __global__ __launch_bounds__(kBlock) void refBlendKernel(FrameParams f, const RefRecord* __restrict__ recs,
                                                          const uint32_t* __restrict__ order,
                                                          const uint32_t* __restrict__ liveCount,
+                                                         const uint32_t* __restrict__ pixelList, uint32_t listCount,
                                                          double* __restrict__ colour, double* __restrict__ depth) {
+    // Output n is pixel n of the frame, or pixel pixelList[n] when a list is given.
     const uint32_t n = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t pixels = f.width * f.height;
     const uint32_t live = *liveCount;
-    const bool inside = n < pixels;
-    const double pxf = static_cast<double>(inside ? n % f.width : 0u) + 0.5;
-    const double pyf = static_cast<double>(inside ? n / f.width : 0u) + 0.5;
+    const bool inside = pixelList != nullptr ? n < listCount && pixelList[n] < pixels : n < pixels;
+    const uint32_t at = pixelList != nullptr ? (n < listCount ? pixelList[n] : 0u) : n;
+    const double pxf = static_cast<double>(inside ? at % f.width : 0u) + 0.5;
+    const double pyf = static_cast<double>(inside ? at / f.width : 0u) + 0.5;
     double r = 0.0, g = 0.0, b = 0.0, t = 1.0, dsum = 0.0, dat = -1.0;
     bool active = inside;
     for (uint32_t k = 0; k < live; ++k) {
@@ -250,11 +253,12 @@ void referenceProject(const FrameParams& f, const float4* positions, const uint4
 }
 
 void referenceBlend(const FrameParams& f, const void* records, const uint32_t* order, const uint32_t* live,
-                    double* colour, double* depth, hipStream_t stream) {
-    const uint32_t pixels = f.width * f.height;
+                    double* colour, double* depth, hipStream_t stream, const uint32_t* pixelList,
+                    uint32_t listCount) {
+    const uint32_t pixels = pixelList != nullptr ? listCount : f.width * f.height;
     if (pixels == 0) return;
     refBlendKernel<<<(pixels + kBlock - 1) / kBlock, kBlock, 0, stream>>>(
-        f, static_cast<const RefRecord*>(records), order, live, colour, depth);
+        f, static_cast<const RefRecord*>(records), order, live, pixelList, listCount, colour, depth);
 }
 
 }   // namespace lrt::kern

@@ -29,6 +29,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <type_traits>
 
 #include "kernels.h"
 
@@ -198,40 +199,61 @@ __global__ __launch_bounds__(1024) void digitOffsetsKernel(const uint32_t* __res
     }
 }
 
-// Where one pass's 8-bit digit sits. TileDepth: bits [shift, +8) of the
-// virtual key (tileDepthKey). Plain: bits [bitLo + shift, +8) of the key.
+// How a pass reads its keys, takes its 8-bit digit and writes its keys:
+//   Plain64:      u64 in and out; the digit is bits [shift, +8) of the key.
+//   TileDepth64:  u64 in and out, the duplication's keys; the digit is bits
+//                 [shift, +8) of the virtual key (tileDepthKey).
+//   TileNarrow:   u64 in (the duplication's keys), u32 out: the virtual key
+//                 shifted right by `narrow` (its low `narrow` bits are this
+//                 pass's, consumed once it is done); the digit as TileDepth64's.
+//   Narrow32:     u32 in and out (TileNarrow's output); the digit is bits
+//                 [shift, +8) of the stored key.
 // `mask` keeps the digit's bits below keyBits.
+enum class PassKind : uint32_t { Plain64, TileDepth64, TileNarrow, Narrow32 };
+
 struct PassDigit {
     uint32_t shift;
     uint32_t mask;
     uint32_t depthMin, depthBits;   // TileDepth's layout
+    uint32_t narrow;                // TileNarrow: the bits dropped from the virtual key
 };
 
-template <KeyMode MODE>
-__device__ __forceinline__ uint32_t digitOf(uint64_t key, const PassDigit& pd) {
-    if constexpr (MODE == KeyMode::TileDepth) {
-        return static_cast<uint32_t>(tileDepthKey(key, pd.depthMin, pd.depthBits) >> pd.shift) & pd.mask;
+template <PassKind K>
+struct PassKeys {
+    using In = std::conditional_t<K == PassKind::Narrow32, uint32_t, uint64_t>;
+    using Out = std::conditional_t<K == PassKind::TileNarrow || K == PassKind::Narrow32, uint32_t, uint64_t>;
+};
+
+// The digit of an input key and the key as the pass writes it.
+template <PassKind K>
+__device__ __forceinline__ uint32_t digitOf(typename PassKeys<K>::In key, const PassDigit& pd,
+                                            typename PassKeys<K>::Out& out) {
+    if constexpr (K == PassKind::TileDepth64 || K == PassKind::TileNarrow) {
+        const uint64_t v = tileDepthKey(key, pd.depthMin, pd.depthBits);
+        if constexpr (K == PassKind::TileNarrow) {
+            out = static_cast<uint32_t>(v >> pd.narrow);
+        } else {
+            out = key;
+        }
+        return static_cast<uint32_t>(v >> pd.shift) & pd.mask;
     } else {
+        out = key;
         return static_cast<uint32_t>(key >> pd.shift) & pd.mask;
     }
 }
 
-PassDigit passDigit(KeyMode mode, uint32_t bitLo, uint64_t mask, uint32_t shift, TileDepthLayout depth) {
-    const uint32_t m = static_cast<uint32_t>((mask >> shift) & 255u);
-    if (mode == KeyMode::TileDepth) return PassDigit{shift, m, depth.depthMin, depth.depthBits};
-    return PassDigit{bitLo + shift, m, 0u, 0u};
-}
-
-template <KeyMode MODE>
-__global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restrict__ keysIn,
+template <PassKind K>
+__global__ __launch_bounds__(kBlock) void onesweepKernel(const typename PassKeys<K>::In* __restrict__ keysIn,
                                                          const uint32_t* __restrict__ valsIn,
-                                                         uint64_t* __restrict__ keysOut,
+                                                         typename PassKeys<K>::Out* __restrict__ keysOut,
                                                          uint32_t* __restrict__ valsOut, uint32_t cap,
                                                          const uint32_t* __restrict__ nDevice, PassDigit pd,
                                                          const uint32_t* __restrict__ digitOffsets,
                                                          uint32_t* __restrict__ status,
                                                          uint32_t* __restrict__ error) {
-    __shared__ uint64_t sKeys[kPart];
+    using KIn = typename PassKeys<K>::In;
+    using KOut = typename PassKeys<K>::Out;
+    __shared__ KOut sKeys[kPart];
     __shared__ uint32_t sVals[kPart];
     // Per-wave digit counts; once the scatter positions are in registers, the
     // digits of sKeys (bytes), so the write-out need not recompute them.
@@ -256,7 +278,8 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
     if (base >= n) return;
     const uint32_t waveFirst = base + wave * (kKpt * 64);
 
-    uint64_t k[kKpt];
+    KIn kin[kKpt];
+    KOut k[kKpt];
     uint32_t v[kKpt];
     uint32_t d[kKpt];
     uint32_t r[kKpt];
@@ -266,11 +289,11 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
 #pragma unroll
     for (uint32_t j = 0; j < kKpt; ++j) {
         const uint32_t idx = min(waveFirst + j * 64 + lane, last);
-        k[j] = keysIn[idx];
+        kin[j] = keysIn[idx];
         v[j] = valsIn[idx];
     }
 #pragma unroll
-    for (uint32_t j = 0; j < kKpt; ++j) d[j] = digitOf<MODE>(k[j], pd);
+    for (uint32_t j = 0; j < kKpt; ++j) d[j] = digitOf<K>(kin[j], pd, k[j]);
 
     // Rank within the wave: peers = lanes holding the same digit, the AND over
     // the digit's bits of the bit's vote (bit set) or its complement (clear);
@@ -395,7 +418,7 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const uint64_t* __restr
         const uint32_t i = tid + j * kBlock;
         if (i < count) {
             const uint32_t out = sGlobal[sDigit[i]] + i;
-            const uint64_t key = sKeys[i];
+            const KOut key = sKeys[i];
             const uint32_t val = sVals[i];
             if (out < n) {
                 keysOut[out] = key;
@@ -434,19 +457,41 @@ void launchPasses(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t*
                                                           alsoZero != nullptr ? alsoZeroWords : 0u);
     digitOffsetsKernel<<<passes, 1024, 0, stream>>>(ws.histogram, rows, passes, ws.offsets);
     if (afterHistogram != nullptr) (void)hipEventRecord(afterHistogram, stream);
-    const uint64_t* kin = keys;
+    // TileDepth keys of at most kNarrowMaxBits bits leave the first pass as
+    // u32 (the virtual key less its low narrowShift bits, which that pass
+    // consumes), so the later passes move 8 bytes a pair instead of 12.
+    const bool narrow = MODE == KeyMode::TileDepth && keyBits <= kNarrowMaxBits;
+    const uint32_t dropped = narrowShift(keyBits);
+    const void* kin = keys;
     const uint32_t* vin = values;
-    uint64_t* kout = keysAlt;
+    void* kout = keysAlt;
     uint32_t* vout = valuesAlt;
     for (uint32_t p = 0; p < passes; ++p) {
-        onesweepKernel<MODE><<<parts, kBlock, 0, stream>>>(kin, vin, kout, vout, n, nDevice,
-                                                           passDigit(MODE, bitLo, mask, 8 * p, depth),
-                                                           ws.offsets + p * kRadix,
-                                                           ws.status + static_cast<uint64_t>(p) * parts * kRadix,
-                                                           ws.error);
-        const uint64_t* nk = kout;
+        PassDigit pd{8 * p, static_cast<uint32_t>((mask >> (8 * p)) & 255u), depth.depthMin, depth.depthBits, dropped};
+        const uint32_t* offsets = ws.offsets + p * kRadix;
+        uint32_t* status = ws.status + static_cast<uint64_t>(p) * parts * kRadix;
+        if constexpr (MODE == KeyMode::Plain) {
+            pd.shift = bitLo + 8 * p;
+            onesweepKernel<PassKind::Plain64><<<parts, kBlock, 0, stream>>>(
+                static_cast<const uint64_t*>(kin), vin, static_cast<uint64_t*>(kout), vout, n, nDevice, pd, offsets,
+                status, ws.error);
+        } else if (!narrow) {
+            onesweepKernel<PassKind::TileDepth64><<<parts, kBlock, 0, stream>>>(
+                static_cast<const uint64_t*>(kin), vin, static_cast<uint64_t*>(kout), vout, n, nDevice, pd, offsets,
+                status, ws.error);
+        } else if (p == 0) {
+            onesweepKernel<PassKind::TileNarrow><<<parts, kBlock, 0, stream>>>(
+                static_cast<const uint64_t*>(kin), vin, static_cast<uint32_t*>(kout), vout, n, nDevice, pd, offsets,
+                status, ws.error);
+        } else {
+            pd.shift = 8 * p - dropped;
+            onesweepKernel<PassKind::Narrow32><<<parts, kBlock, 0, stream>>>(
+                static_cast<const uint32_t*>(kin), vin, static_cast<uint32_t*>(kout), vout, n, nDevice, pd, offsets,
+                status, ws.error);
+        }
+        const void* nk = kout;
         const uint32_t* nv = vout;
-        kout = const_cast<uint64_t*>(kin);
+        kout = const_cast<void*>(kin);
         vout = const_cast<uint32_t*>(vin);
         kin = nk;
         vin = nv;

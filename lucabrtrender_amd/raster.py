@@ -538,6 +538,22 @@ class TileRasterizer:
                                              1 if keyed else 0))
         return colour, depth
 
+    def reference_render_pixels(self, camera_or_projection, instances: Sequence[SplatInstance],
+                                settings: RenderSettings, pixels, keyed: bool = False):
+        """The FP64 brute force at the listed pixels only (pixels: int32/uint32 CUDA
+        tensor of y * width + x): (colour [n, 4], depth [n]) float64 CUDA tensors."""
+        import torch
+        proj = camera_or_projection
+        if isinstance(proj, Camera):
+            proj = projection_for(proj, settings.width, settings.height)
+        n = int(pixels.numel())
+        colour = torch.empty((max(n, 1), 4), dtype=torch.float64, device="cuda")
+        depth = torch.empty((max(n, 1),), dtype=torch.float64, device="cuda")
+        self._check(lib.lrt_reference_render_pixels(self._ctx, C.byref(proj), _instances_c(instances), len(instances),
+                                                    C.byref(settings.to_c()), pixels.data_ptr(), n,
+                                                    colour.data_ptr(), depth.data_ptr(), 1 if keyed else 0))
+        return colour[:n], depth[:n]
+
     def _views_c(self, views: Sequence[View]):
         arr = (_abi.ViewC * max(1, len(views)))()
         for k, v in enumerate(views):

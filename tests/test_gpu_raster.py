@@ -301,3 +301,25 @@ def test_a_frame_that_outgrows_the_pair_buffers_is_the_oracle_s():
     assert ora["pairs"] > 20 * 500
     H.assert_bit_exact(gpu, ora)
     _image_checks(gpu, ora)
+
+
+def test_walk_steps_are_counted_by_the_same_blend(rast):
+    """LRT_DEBUG_WALK_STEPS runs the last frame's blend again and counts its
+    quadrant waves' list steps: positive, no more than every quadrant walking its
+    whole tile (each list fill padded to a group of four), the same on each call, and
+    leaving the frame's images as they were."""
+    cfg = scenes.workload("scene-fwd-3M-1080p", count=100_000)
+    s = raster.RenderSettings(width=640, height=360, dilation=cfg.dilation, filter3d=cfg.filter3d)
+    inst = [raster.SplatInstance(cfg.scene.to_device())]
+    t = raster.RenderTargets(640, 360)
+    rast.render(cfg.camera, inst, s, t)
+    before = t.colour.clone()
+    steps = rast.debug(_abi.DEBUG_WALK_STEPS)
+    assert steps.size == 1
+    ranges = rast.debug(_abi.DEBUG_RANGES).reshape(-1, 2).astype(np.int64)
+    pairs = ranges[:, 1] - ranges[:, 0]
+    bound = int((4 * (pairs + 4 * (pairs // 128 + 1))).sum())   # each list fill padded to a group of 4
+    print(f"walk steps {int(steps[0])}, bound {bound}")
+    assert 0 < int(steps[0]) <= bound
+    assert int(rast.debug(_abi.DEBUG_WALK_STEPS)[0]) == int(steps[0])
+    assert bool((t.colour == before).all())
