@@ -248,8 +248,8 @@ void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* stat
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks,
                    uint32_t depthEntries, uint32_t* depthRange, uint32_t* alsoZero, uint32_t alsoZeroWords) {
     const uint32_t parts = scanPartitions(n);
-    {
-        const uint64_t words = static_cast<uint64_t>(parts) * 2 + alsoZeroWords;
+    if (const uint64_t words = static_cast<uint64_t>(parts) * 2 + alsoZeroWords; words > 0) {
+        // (an empty scan with nothing else to clear launches nothing: a grid of 0 is an error)
         const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((words + 255) / 256, 4096));
         zero2Kernel<<<grid, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(status), static_cast<uint64_t>(parts) * 2,
                                               alsoZero, alsoZero != nullptr ? alsoZeroWords : 0u);

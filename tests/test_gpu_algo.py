@@ -34,6 +34,17 @@ def test_prefix_sum_of_every_size(rast, count):
     assert int(total.cpu().numpy().view(np.uint32)[0]) == int(c.astype(np.uint64).sum())
 
 
+def test_an_empty_prefix_sum_sets_the_total_to_zero(rast):
+    import torch
+    inp = torch.ones(4, dtype=torch.int32, device="cuda")
+    out = torch.full((4,), 7, dtype=torch.int32, device="cuda")
+    total = torch.full((1,), 9, dtype=torch.int32, device="cuda")
+    from lucabrtrender_amd import _abi
+    assert _abi.lib.lrt_prefix_sum(rast._ctx, inp.data_ptr(), out.data_ptr(), total.data_ptr(), 0) == _abi.LRT_OK
+    torch.cuda.synchronize()
+    assert int(total.cpu()[0]) == 0 and out.cpu().tolist() == [7, 7, 7, 7]
+
+
 def test_prefix_sum_on_unaligned_views(rast):
     import torch
     c = PT.counts_generate(50001, seed=9)
