@@ -498,24 +498,34 @@ lrt_status lrt_write_exr(const char* path, uint32_t width, uint32_t height, cons
  * (modules/io/include/lrt/io/RawSplats.h:15-77): a file as one float a property
  * a record, in the file's order, and where each field sits. Modes are numbered as RawSplats.h:42-58 numbers them:
  *   opacity_mode   0 logit, 1 linear, 2 byte (v / 255)
- *   scale_mode     0 log, 1 linear (log(max(v, 1e-30)))
- *   colour_mode    0 SH DC (0.5 + C0 v), 1 linear, 2 byte (v / 255)
- *   rotation_mode  0 float (w, x, y, z), 1 byte ((v - 128) / 128 each)
+ *   scale_mode     0 log, 1 linear (log(max(v, 1e-30))), 2 SPZ byte (v / 16 - 10)
+ *   colour_mode    0 SH DC (0.5 + C0 v), 1 linear, 2 byte (v / 255),
+ *                  3 SPZ byte (0.5 + C0 ((v / 255 - 0.5) / 0.15))
+ *   rotation_mode  0 float (w, x, y, z), 1 byte ((v - 128) / 128 each),
+ *                  2 SPZ v2 (x y z bytes v / 127.5 - 1 at rot_x..rot_z, w from unit length),
+ *                  3 SPZ v3 smallest three (a u32: low 16 bits at rot_x, high 16 at rot_y)
  *   rest_mode      0 float, 1 byte ((v - 128) / 128)
  * rest_colour_outer 1: rest coefficients rrr.. ggg.. bbb.. with
- * file_per_colour a colour (a 3DGS PLY); 0: rgb per basis. */
+ * file_per_colour a colour (a 3DGS PLY); 0: rgb per basis. Positions are
+ * multiplied by 2^-fractional_bits (SPZ's fixed point); flip_yz 1: the file
+ * is right-up-back (SPZ), read right-down-front (y and z of positions and
+ * rotations, and the rest coefficients odd in y xor z, change sign). */
 typedef struct lrt_raw_splats lrt_raw_splats;
 typedef struct {
     uint32_t floats_per_record;
     uint32_t x, y, z, opacity, scale0, scale1, scale2, rot_w, rot_x, rot_y, rot_z, dc0, dc1, dc2;
     uint32_t rest_base, rest_per_colour, file_per_colour, rest_colour_outer;
     uint32_t opacity_mode, scale_mode, colour_mode, rotation_mode, rest_mode;
+    uint32_t fractional_bits, flip_yz;
 } lrt_splat_encoding;
 
-/* io::readSplats (modules/io/src/SplatReaders.cpp:32-195) by extension: a
- * binary little-endian 3DGS PLY or a .splat file, read on the host. .spz and
- * .sog are refused (LRT_UNSUPPORTED): this build has neither zstd nor an image
- * decoder. The reference's codes and messages otherwise: not a PLY, no
+/* io::readSplats (modules/io/src/SplatReaders.cpp:32-328) by extension: a
+ * binary little-endian 3DGS PLY, a .splat file or an SPZ file of version 2 or
+ * 3 (one gzip stream), read on the host. SPZ version 4 (zstd streams), SPZ
+ * version 1 (float16 positions) and .sog are refused (LRT_UNSUPPORTED): this
+ * build has neither zstd nor an image decoder; an SPZ whose header or streams
+ * do not hold together is LRT_IO_FAILURE. The reference's codes and messages
+ * otherwise: not a PLY, no
  * vertices, a missing 3DGS property -> LRT_INVALID_ARGUMENT; ASCII or
  * big-endian, a list in the vertex element, harmonics not contiguous or not a
  * whole degree -> LRT_UNSUPPORTED; a file shorter than its vertices ->

@@ -169,7 +169,8 @@ class SceneSpecC(C.Structure):
 # Every symbol include/lrt_hip.h and include/lrt_synth.h declare, with its signature.
 ENCODING_FIELDS = ("floats_per_record", "x", "y", "z", "opacity", "scale0", "scale1", "scale2", "rot_w", "rot_x",
                    "rot_y", "rot_z", "dc0", "dc1", "dc2", "rest_base", "rest_per_colour", "file_per_colour",
-                   "rest_colour_outer", "opacity_mode", "scale_mode", "colour_mode", "rotation_mode", "rest_mode")
+                   "rest_colour_outer", "opacity_mode", "scale_mode", "colour_mode", "rotation_mode", "rest_mode",
+                   "fractional_bits", "flip_yz")
 
 
 class SplatEncodingC(C.Structure):

@@ -170,6 +170,8 @@ lrt_splat_encoding toEncodingC(const lrt::io::SplatEncoding& e) {
     c.colour_mode = static_cast<uint32_t>(e.colourMode);
     c.rotation_mode = static_cast<uint32_t>(e.rotationMode);
     c.rest_mode = static_cast<uint32_t>(e.restMode);
+    c.fractional_bits = e.fractionalBits;
+    c.flip_yz = e.flipYZ;
     return c;
 }
 
@@ -178,9 +180,10 @@ lrt::Status fromEncodingC(const lrt_splat_encoding& c, lrt::io::SplatEncoding& e
     using lrt::ErrorCode;
     using lrt::Status;
     using E = lrt::io::SplatEncoding;
-    if (c.opacity_mode > 2 || c.scale_mode > 1 || c.colour_mode > 2 || c.rotation_mode > 1 || c.rest_mode > 1) {
+    if (c.opacity_mode > 2 || c.scale_mode > 2 || c.colour_mode > 3 || c.rotation_mode > 3 || c.rest_mode > 1) {
         return Status::error(ErrorCode::InvalidArgument, "an encoding mode this build does not decode");
     }
+    if (c.fractional_bits > 24) return Status::error(ErrorCode::InvalidArgument, "fractional_bits above 24");
     if (c.rest_per_colour != 0 && c.rest_per_colour != 3 && c.rest_per_colour != 8 && c.rest_per_colour != 15) {
         return Status::error(ErrorCode::InvalidArgument, "rest_per_colour must be 0, 3, 8 or 15");
     }
@@ -196,6 +199,8 @@ lrt::Status fromEncodingC(const lrt_splat_encoding& c, lrt::io::SplatEncoding& e
     e.colourMode = static_cast<E::Colour>(c.colour_mode);
     e.rotationMode = static_cast<E::Rotation>(c.rotation_mode);
     e.restMode = static_cast<E::Rest>(c.rest_mode);
+    e.fractionalBits = c.fractional_bits;
+    e.flipYZ = c.flip_yz != 0 ? 1u : 0u;
     return {};
 }
 

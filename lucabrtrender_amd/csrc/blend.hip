@@ -57,6 +57,16 @@ constexpr uint32_t kGroup = 4;           // list records walked per step
 constexpr float kExpScale = -0.72134752044448170368f;   // -log2(e) / 2
 constexpr float kLog2Floor = -7.99435343685885793775f;   // log2(1 / 255): alpha's floor
 
+// The sRGB decode (reference color.slang:7-10) on the hardware log2 and exp2,
+// the division by the coverage a reciprocal: a few ulps, as the walk's own
+// exponent (the blend's images match the oracle within the image tolerances,
+// not bit for bit; the projection, counts and keys stay exact).
+__device__ __forceinline__ float srgbToLinearFast(float c) {
+    c = fmaxS(c, 0.0f);
+    return c <= 0.04045f ? c * (1.0f / 12.92f)
+                         : __builtin_amdgcn_exp2f(2.4f * __builtin_amdgcn_logf((c + 0.055f) * (1.0f / 1.055f)));
+}
+
 // The finished pixel: linearise rgb / coverage, the background behind, the
 // depth (reference splat_blend.slang:89-107, splat-only).
 __device__ __forceinline__ void finishPixel(const FrameParams& f, uint32_t px, uint32_t py, float r, float g, float b,
@@ -67,9 +77,10 @@ __device__ __forceinline__ void finishPixel(const FrameParams& f, uint32_t px, u
     const float coverage = 1.0f - t;
     if (accum != nullptr) accum[py * f.width + px] = make_float4(r, g, b, depthSum);
     if (f.linearise && coverage > 1e-6f) {
-        r = srgbToLinear(r / coverage) * coverage;
-        g = srgbToLinear(g / coverage) * coverage;
-        b = srgbToLinear(b / coverage) * coverage;
+        const float inv = __builtin_amdgcn_rcpf(coverage);
+        r = srgbToLinearFast(r * inv) * coverage;
+        g = srgbToLinearFast(g * inv) * coverage;
+        b = srgbToLinearFast(b * inv) * coverage;
     }
     const uint32_t at = py * f.width + px;
     colour[at] = make_float4(r + t * f.bg[0], g + t * f.bg[1], b + t * f.bg[2], coverage + t * f.bg[3]);
@@ -95,9 +106,10 @@ __device__ __forceinline__ void finishComposite(const FrameParams& f, uint32_t a
     }
     const float coverage = 1.0f - t;
     if (f.linearise && coverage > 1e-6f) {
-        r = srgbToLinear(r / coverage) * coverage;
-        g = srgbToLinear(g / coverage) * coverage;
-        b = srgbToLinear(b / coverage) * coverage;
+        const float inv = __builtin_amdgcn_rcpf(coverage);
+        r = srgbToLinearFast(r * inv) * coverage;
+        g = srgbToLinearFast(g * inv) * coverage;
+        b = srgbToLinearFast(b * inv) * coverage;
     }
     float4 behind = make_float4(f.bg[0], f.bg[1], f.bg[2], f.bg[3]);
     if (opaqueHit) {

@@ -134,6 +134,8 @@ struct DecodeParams {
     uint32_t x, y, z, opacity, scale0, scale1, scale2, rotW, rotX, rotY, rotZ, dc0, dc1, dc2;
     uint32_t restBase, filePerColour, restColourOuter;
     uint32_t opacityMode, scaleMode, colourMode, rotationMode, restMode;
+    float positionScale;   // positions are multiplied by it (SPZ: 2^-fractional bits)
+    uint32_t flipYZ;       // the file is right-up-back: y and z change sign
 };
 // Validates records [0, p.count) of `raw` (device), scans the flags into
 // `dest` (kept count into *kept) and packs each kept record to base + its

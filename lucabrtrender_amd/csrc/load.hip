@@ -35,17 +35,77 @@ __device__ __forceinline__ float decodeOpacity(float v, uint32_t mode) {
     return v;
 }
 __device__ __forceinline__ float decodeLogScale(float v, uint32_t mode) {
+    if (mode == 2) return v / 16.0f - 10.0f;   // SPZ byte
     return mode == 0 ? v : logF(fmaxS(v, 1e-30f));
 }
 __device__ __forceinline__ float decodeColour(float v, uint32_t mode) {
     if (mode == 0) return 0.5f + kShC0 * v;
     if (mode == 2) return v / 255.0f;
+    if (mode == 3) return 0.5f + kShC0 * ((v / 255.0f - 0.5f) / 0.15f);   // SPZ byte: its DC scale 0.15
     return v;
 }
 __device__ __forceinline__ float byteUnit(float v, uint32_t mode) { return mode == 1 ? (v - 128.0f) / 128.0f : v; }
 
+// A file stored right-up-back (SPZ) is read right-down-front, a half turn
+// about x: y and z of positions and rotations change sign, and so does a rest
+// coefficient whose basis is odd in y xor z (3DGS basis order).
+__device__ __forceinline__ float restFlipSign(uint32_t basis) {
+    constexpr uint32_t kNegative = 0b010110101001011u;   // bases 0 1 3 6 8 10 11 13 (bit = basis)
+    return ((kNegative >> (basis < 14u ? basis : 14u)) & 1u) != 0u ? -1.0f : 1.0f;
+}
+
+__device__ __forceinline__ void decodePosition(const float* __restrict__ r, const DecodeParams& p, float v[3]) {
+    v[0] = r[p.x] * p.positionScale;
+    v[1] = r[p.y] * p.positionScale;
+    v[2] = r[p.z] * p.positionScale;
+    if (p.flipYZ != 0) {
+        v[1] = -v[1];
+        v[2] = -v[2];
+    }
+}
+
+// The record's rotation as (x, y, z, w), not yet normalised
+// (splat_encoding.slang decodeQuaternion).
+__device__ __forceinline__ void decodeRotation(const float* __restrict__ r, const DecodeParams& p, float q[4]) {
+    if (p.rotationMode == 2) {
+        // SPZ v2: x y z bytes, w from unit length.
+        q[0] = r[p.rotX] / 127.5f - 1.0f;
+        q[1] = r[p.rotY] / 127.5f - 1.0f;
+        q[2] = r[p.rotZ] / 127.5f - 1.0f;
+        q[3] = __builtin_sqrtf(fmaxS(0.0f, 1.0f - ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2])));
+    } else if (p.rotationMode == 3) {
+        // SPZ v3: smallest three in one u32 held as two 16-bit halves: two bits
+        // name the largest component, then the others from the last down, a
+        // 9-bit magnitude of sqrt(1/2) and a sign bit each.
+        uint32_t comp = static_cast<uint32_t>(r[p.rotX]) | (static_cast<uint32_t>(r[p.rotY]) << 16);
+        const uint32_t largest = comp >> 30;
+        float sum = 0.0f;
+        for (int i = 3; i >= 0; --i) {
+            if (static_cast<uint32_t>(i) == largest) {
+                continue;
+            }
+            const float magnitude = 0.70710678f * static_cast<float>(comp & 511u) / 511.0f;
+            q[i] = ((comp >> 9) & 1u) != 0u ? -magnitude : magnitude;
+            sum = sum + q[i] * q[i];
+            comp >>= 10;
+        }
+        q[largest] = __builtin_sqrtf(fmaxS(0.0f, 1.0f - sum));
+    } else {
+        q[0] = byteUnit(r[p.rotX], p.rotationMode);
+        q[1] = byteUnit(r[p.rotY], p.rotationMode);
+        q[2] = byteUnit(r[p.rotZ], p.rotationMode);
+        q[3] = byteUnit(r[p.rotW], p.rotationMode);
+    }
+    if (p.flipYZ != 0) {
+        q[1] = -q[1];
+        q[2] = -q[2];
+    }
+}
+
 __device__ __forceinline__ bool keepRecord(const float* __restrict__ r, const DecodeParams& p) {
-    const float px = r[p.x], py = r[p.y], pz = r[p.z];
+    float pos[3];
+    decodePosition(r, p, pos);
+    const float px = pos[0], py = pos[1], pz = pos[2];
     const float a = decodeOpacity(r[p.opacity], p.opacityMode);
     const float l0 = decodeLogScale(r[p.scale0], p.scaleMode);
     const float l1 = decodeLogScale(r[p.scale1], p.scaleMode);
@@ -73,9 +133,11 @@ __global__ __launch_bounds__(kBlock) void decodeKernel(const float* __restrict__
     const float* r = raw + static_cast<uint64_t>(i) * p.stride;
     const uint32_t j = base + dest[i];
     const float a = decodeOpacity(r[p.opacity], p.opacityMode);
-    positions[j] = make_float4(r[p.x], r[p.y], r[p.z], clampS(a, 0.0f, 1.0f));
-    float q[4] = {byteUnit(r[p.rotX], p.rotationMode), byteUnit(r[p.rotY], p.rotationMode),
-                  byteUnit(r[p.rotZ], p.rotationMode), byteUnit(r[p.rotW], p.rotationMode)};
+    float pos[3];
+    decodePosition(r, p, pos);
+    positions[j] = make_float4(pos[0], pos[1], pos[2], clampS(a, 0.0f, 1.0f));
+    float q[4];
+    decodeRotation(r, p, q);
     const float qLength = __builtin_sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
     if (!(qLength > 1e-8f)) {
         q[0] = 0.0f;
@@ -100,7 +162,8 @@ __global__ __launch_bounds__(kBlock) void decodeKernel(const float* __restrict__
         for (uint32_t colour = 0; colour < 3; ++colour) {
             const uint32_t field = p.restColourOuter != 0 ? p.restBase + colour * p.filePerColour + basis
                                                           : p.restBase + basis * 3 + colour;
-            const float value = byteUnit(r[field], p.restMode);
+            float value = byteUnit(r[field], p.restMode);
+            if (p.flipYZ != 0) value *= restFlipSign(basis);
             word |= static_cast<uint32_t>(pack::floatToHalf(pack::halfSafe(value))) << (filled * 16);
             if (++filled == 2) {
                 row[at++] = word;

@@ -1148,6 +1148,8 @@ Status TileRasterizer::decodeCloud(const io::RawSplats& raw, uint32_t maxDegree,
     p.colourMode = static_cast<uint32_t>(e.colourMode);
     p.rotationMode = static_cast<uint32_t>(e.rotationMode);
     p.restMode = static_cast<uint32_t>(e.restMode);
+    p.positionScale = 1.0f / static_cast<float>(1u << std::min(e.fractionalBits, 24u));   // exact: a power of two
+    p.flipYZ = e.flipYZ;
     uint32_t* counters = counters_.as<uint32_t>();
     uint32_t written = 0;
     float uploadMs = 0.0F, decodeMs = 0.0F;

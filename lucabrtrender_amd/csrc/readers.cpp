@@ -15,6 +15,8 @@
 #include <sstream>
 #include <thread>
 
+#include <zlib.h>
+
 namespace lrt::io {
 namespace {
 
@@ -182,9 +184,7 @@ Status readSplats(const std::string& path, RawSplats& out) {
     const std::string ext = lowerExtension(path);
     if (ext == ".splat") return readDotSplat(path, out);
     if (ext == ".ply") return readSplatPly(path, out);
-    if (ext == ".spz") {
-        return Status::error(ErrorCode::Unsupported, "'" + path + "': this build reads no .spz (zstd was not found)");
-    }
+    if (ext == ".spz") return readSpz(path, out);
     if (ext == ".sog" || fileName(path) == "meta.json") {
         return Status::error(ErrorCode::Unsupported, "'" + path + "': this build reads no SOG image sets");
     }
@@ -346,6 +346,143 @@ Status readDotSplat(const std::string& path, RawSplats& out) {
             float* o = dst + i * 14;
             std::memcpy(o, record, 24);   // six float32, as stored
             for (int k = 0; k < 8; ++k) o[6 + k] = static_cast<float>(record[24 + k]);
+        }
+    });
+    out = std::move(raw);
+    return {};
+}
+
+// SPZ (Niantic's packed splats), the reference's readSpz
+// (modules/io/src/SplatReaders.cpp:242-328) over its loadSpzPacked: versions 1
+// to 3 are one gzip stream holding a 16-byte header (magic "NGSP", version,
+// points, SH degree, fractional bits, flags, a spare byte) and then the
+// attribute streams one after another: positions (three 24-bit fixed-point
+// integers a point; version 1's float16 positions were never released and are
+// refused), alphas (a byte), colours (3 bytes), log scales (3 bytes),
+// rotations (3 bytes in version 2, a smallest-three u32 in version 3) and the
+// rest coefficients (3 bytes a basis, rgb per basis). Version 4 starts with the
+// magic itself and keeps each stream in zstd: refused, as this build has no
+// zstd. The records carry the bytes as values (a 24-bit integer is exact in a
+// float; the smallest-three word as its two 16-bit halves), at most degree 3
+// kept, and the encoding says how the GPU decodes them.
+Status readSpz(const std::string& path, RawSplats& out) {
+    constexpr uint32_t kMagic = 0x5053474eu;   // "NGSP"
+    MappedFile file;
+    Status s = file.open(path);
+    if (!s.ok()) return s;
+    const auto* bytes = reinterpret_cast<const unsigned char*>(file.data());
+    const std::string where = "'" + path + "': ";
+    if (file.size() >= 4) {
+        uint32_t first = 0;
+        std::memcpy(&first, bytes, 4);
+        if (first == kMagic) {
+            return Status::error(ErrorCode::Unsupported, where + "SPZ version 4 streams are zstd, which this build does not read");
+        }
+    }
+    // The whole gzip stream, inflated.
+    std::vector<unsigned char> data;
+    {
+        z_stream z{};
+        if (inflateInit2(&z, 16 + MAX_WBITS) != Z_OK) return Status::error(ErrorCode::IoFailure, where + "zlib");
+        z.next_in = const_cast<Bytef*>(reinterpret_cast<const Bytef*>(bytes));
+        z.avail_in = static_cast<uInt>(std::min<uint64_t>(file.size(), 0xFFFFFFFFull));
+        std::vector<unsigned char> chunk(1u << 20);
+        int res = Z_OK;
+        while (res == Z_OK) {
+            z.next_out = chunk.data();
+            z.avail_out = static_cast<uInt>(chunk.size());
+            res = inflate(&z, Z_NO_FLUSH);
+            if (res != Z_OK && res != Z_STREAM_END) break;
+            data.insert(data.end(), chunk.data(), chunk.data() + (chunk.size() - z.avail_out));
+        }
+        inflateEnd(&z);
+        if (res != Z_STREAM_END) return Status::error(ErrorCode::IoFailure, where + "not a readable SPZ file (gzip)");
+    }
+    if (data.size() < 16) return Status::error(ErrorCode::IoFailure, where + "not a readable SPZ file");
+    uint32_t magic = 0, version = 0, points = 0;
+    std::memcpy(&magic, data.data(), 4);
+    std::memcpy(&version, data.data() + 4, 4);
+    std::memcpy(&points, data.data() + 8, 4);
+    const uint32_t shDegree = data[12], fractionalBits = data[13];
+    if (magic != kMagic) return Status::error(ErrorCode::IoFailure, where + "not a readable SPZ file (magic)");
+    if (version < 1 || version > 3) {
+        return Status::error(ErrorCode::IoFailure, where + "SPZ version " + std::to_string(version) + " is not supported");
+    }
+    if (version == 1) {
+        return Status::error(ErrorCode::Unsupported, where + "SPZ float16 positions (a format never released)");
+    }
+    if (points == 0) return Status::error(ErrorCode::IoFailure, where + "not a readable SPZ file (no points)");
+    if (shDegree > 4 || fractionalBits > 24) {
+        return Status::error(ErrorCode::IoFailure, where + "not a readable SPZ file (header)");
+    }
+    const uint32_t shDim = shDegree >= 4 ? 24u : shDegree == 3 ? 15u : shDegree == 2 ? 8u : shDegree == 1 ? 3u : 0u;
+    const bool smallestThree = version >= 3;
+    const uint64_t n = points;
+    const uint64_t rotBytes = smallestThree ? 4 : 3;
+    const uint64_t need = 16 + n * (9 + 1 + 3 + 3 + rotBytes + uint64_t{shDim} * 3);
+    if (data.size() < need) {
+        return Status::error(ErrorCode::IoFailure,
+                             where + "SPZ attribute streams shorter than " + std::to_string(n) + " points");
+    }
+    const unsigned char* pos = data.data() + 16;
+    const unsigned char* alpha = pos + n * 9;
+    const unsigned char* colour = alpha + n;
+    const unsigned char* scale = colour + n * 3;
+    const unsigned char* rot = scale + n * 3;
+    const unsigned char* rest = rot + n * rotBytes;
+
+    RawSplats raw;
+    raw.source = path;
+    raw.count = points;
+    SplatEncoding& enc = raw.encoding;
+    const uint32_t kept = std::min(shDim, 15u);   // the engine evaluates up to degree 3
+    enc.floatsPerRecord = 14 + kept * 3;
+    enc.x = 0; enc.y = 1; enc.z = 2;
+    enc.opacity = 3;
+    enc.scale0 = 4; enc.scale1 = 5; enc.scale2 = 6;
+    enc.dc0 = 7; enc.dc1 = 8; enc.dc2 = 9;
+    enc.rotX = 10; enc.rotY = 11; enc.rotZ = 12; enc.rotW = 13;
+    enc.restBase = 14;
+    enc.restPerColour = kept;
+    enc.filePerColour = kept;
+    enc.restColourOuter = 0;   // rgb per basis
+    enc.opacityMode = SplatEncoding::Opacity::Byte;
+    enc.scaleMode = SplatEncoding::Scale::SpzByte;
+    enc.colourMode = SplatEncoding::Colour::SpzByte;
+    enc.rotationMode = smallestThree ? SplatEncoding::Rotation::SmallestThree : SplatEncoding::Rotation::FirstThree;
+    enc.restMode = SplatEncoding::Rest::Byte;
+    enc.fractionalBits = fractionalBits;
+    enc.flipYZ = 1;
+    raw.records.assign(n * enc.floatsPerRecord, 0.0f);
+    float* const dst = raw.records.data();
+    const uint32_t stride = enc.floatsPerRecord;
+    parallelBlocks(n, [&](uint64_t first, uint64_t last) {
+        for (uint64_t i = first; i < last; ++i) {
+            float* o = dst + i * stride;
+            const unsigned char* p = pos + i * 9;
+            for (int k = 0; k < 3; ++k) {
+                // A little-endian 24-bit two's-complement integer, kept unscaled.
+                int32_t fixed = p[k * 3] | (p[k * 3 + 1] << 8) | (p[k * 3 + 2] << 16);
+                if ((fixed & 0x800000) != 0) fixed |= static_cast<int32_t>(0xFF000000u);
+                o[k] = static_cast<float>(fixed);
+            }
+            o[3] = alpha[i];
+            for (int k = 0; k < 3; ++k) {
+                o[4 + k] = scale[i * 3 + k];
+                o[7 + k] = colour[i * 3 + k];
+            }
+            const unsigned char* r = rot + i * rotBytes;
+            if (smallestThree) {
+                o[10] = static_cast<float>(r[0] | (r[1] << 8));
+                o[11] = static_cast<float>(r[2] | (r[3] << 8));
+            } else {
+                o[10] = r[0];
+                o[11] = r[1];
+                o[12] = r[2];
+            }
+            // The first `kept` bases; the file's own count says where the next point starts.
+            const unsigned char* h = rest + i * uint64_t{shDim} * 3;
+            for (uint32_t k = 0; k < kept * 3; ++k) o[14 + k] = h[k];
         }
     });
     out = std::move(raw);

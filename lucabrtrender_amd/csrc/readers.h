@@ -5,10 +5,11 @@
 // record, in the file's order, plus where each field sits and how it is
 // encoded; the decode into packed splats runs on the GPU (load.hip).
 //
-// Read: binary little-endian PLY (3DGS properties) and .splat (32-byte
-// records). .spz needs zstd, which this build does not have, and .sog is a
+// Read: binary little-endian PLY (3DGS properties), .splat (32-byte records)
+// and SPZ up to version 3 (one gzip stream, through zlib). SPZ version 4 keeps
+// its streams in zstd, which this build does not have, and .sog is a
 // device-decoded image set: both are refused as Unsupported, as the reference
-// refuses them without their libraries.
+// refuses what its build lacks a library for.
 #pragma once
 
 #include <cstdint>
@@ -33,15 +34,24 @@ struct SplatEncoding {
     uint32_t restColourOuter = 1;   // 1: rrr.. ggg.. bbb.. (3DGS PLY); 0: rgb per basis
     // Modes, numbered as RawSplats.h:42-58 numbers them.
     enum class Opacity : uint32_t { Logit = 0, Linear = 1, Byte = 2 };
-    enum class Scale : uint32_t { Log = 0, Linear = 1 };
-    enum class Colour : uint32_t { ShDc = 0, Linear = 1, Byte = 2 };
-    enum class Rotation : uint32_t { Float = 0, Byte = 1 };
+    // SpzByte: log scale = byte / 16 - 10.
+    enum class Scale : uint32_t { Log = 0, Linear = 1, SpzByte = 2 };
+    // SpzByte: SH DC = (byte / 255 - 0.5) / 0.15.
+    enum class Colour : uint32_t { ShDc = 0, Linear = 1, Byte = 2, SpzByte = 3 };
+    // FirstThree: x y z bytes, v / 127.5 - 1, w from unit length (SPZ v2).
+    // SmallestThree: a u32 as its low 16 bits at rotX and high 16 at rotY (SPZ v3).
+    enum class Rotation : uint32_t { Float = 0, Byte = 1, FirstThree = 2, SmallestThree = 3 };
     enum class Rest : uint32_t { Float = 0, Byte = 1 };
     Opacity opacityMode = Opacity::Logit;
     Scale scaleMode = Scale::Log;
     Colour colourMode = Colour::ShDc;
     Rotation rotationMode = Rotation::Float;
     Rest restMode = Rest::Float;
+    // Positions are multiplied by 2^-fractionalBits (SPZ's fixed point).
+    uint32_t fractionalBits = 0;
+    // The file is right-up-back (SPZ); the engine reads right-down-front: y and
+    // z of positions, rotations and harmonics change sign.
+    uint32_t flipYZ = 0;
 };
 
 struct RawSplats {
@@ -75,9 +85,10 @@ struct PlyHeader {
 // PlyHeader.cpp's parsePlyHeader: the header in the first bytes of a file.
 Status parsePlyHeader(const char* text, uint64_t size, PlyHeader& out);
 
-// readSplats by extension (.ply, .splat; .spz and .sog refused).
+// readSplats by extension (.ply, .splat, .spz; .sog refused).
 Status readSplats(const std::string& path, RawSplats& out);
 Status readSplatPly(const std::string& path, RawSplats& out);
 Status readDotSplat(const std::string& path, RawSplats& out);
+Status readSpz(const std::string& path, RawSplats& out);
 
 }   // namespace lrt::io

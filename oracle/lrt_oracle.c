@@ -212,6 +212,42 @@ static float decode_colour(float v, uint32_t mode) {
 }
 static float decode_byte_unit(float v, uint32_t mode) { return mode == 1 ? (v - 128.0f) / 128.0f : v; }
 
+/* splat_encoding.slang restFlipSign: the sign a rest coefficient takes when y
+ * and z change sign, per basis in 3DGS order. */
+static const float kRestFlip[15] = {-1.0f, -1.0f, 1.0f, -1.0f, 1.0f, 1.0f, -1.0f, 1.0f,
+                                    -1.0f, 1.0f, -1.0f, -1.0f, 1.0f, -1.0f, 1.0f};
+
+/* splat_encoding.slang decodeQuaternion: (x, y, z, w), not normalised. */
+static void decode_rotation(const float* r, const or_encoding* e, float q[4]) {
+    if (e->rotation_mode == 2) {
+        q[0] = r[e->rot_x] / 127.5f - 1.0f;
+        q[1] = r[e->rot_y] / 127.5f - 1.0f;
+        q[2] = r[e->rot_z] / 127.5f - 1.0f;
+        q[3] = sqrtf(fmax_(0.0f, 1.0f - ((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2])));
+    } else if (e->rotation_mode == 3) {
+        uint32_t comp = (uint32_t)r[e->rot_x] | ((uint32_t)r[e->rot_y] << 16);
+        const uint32_t largest = comp >> 30;
+        float sum = 0.0f;
+        for (int i = 3; i >= 0; --i) {
+            if ((uint32_t)i == largest) continue;
+            const float magnitude = OR_SQRT_HALF * (float)(comp & 511u) / 511.0f;
+            q[i] = ((comp >> 9) & 1u) ? -magnitude : magnitude;
+            sum = sum + q[i] * q[i];
+            comp >>= 10;
+        }
+        q[largest] = sqrtf(fmax_(0.0f, 1.0f - sum));
+    } else {
+        q[0] = decode_byte_unit(r[e->rot_x], e->rotation_mode);
+        q[1] = decode_byte_unit(r[e->rot_y], e->rotation_mode);
+        q[2] = decode_byte_unit(r[e->rot_z], e->rotation_mode);
+        q[3] = decode_byte_unit(r[e->rot_w], e->rotation_mode);
+    }
+    if (e->flip_yz) {
+        q[1] = -q[1];
+        q[2] = -q[2];
+    }
+}
+
 /* splat_validate.slang:16-29, splat_decode.slang:17-87: kept when the
  * position and log scales are finite, |log scale| < 60 and opacity >= 1/255;
  * compacted in record order (GpuClouds.cpp:151-185). */
@@ -221,7 +257,12 @@ uint32_t or_decode_records(const float* records, uint32_t count, const or_encodi
     uint32_t kept = 0;
     for (uint32_t i = 0; i < count; ++i) {
         const float* r = records + (size_t)i * e->floats_per_record;
-        const float px = r[e->x], py = r[e->y], pz = r[e->z];
+        const float scale = 1.0f / (float)(1u << (e->fractional_bits < 24u ? e->fractional_bits : 24u));
+        float px = r[e->x] * scale, py = r[e->y] * scale, pz = r[e->z] * scale;
+        if (e->flip_yz) {
+            py = -py;
+            pz = -pz;
+        }
         const float a = decode_opacity(r[e->opacity], e->opacity_mode);
         const float l0 = decode_log_scale(r[e->scale0], e->scale_mode);
         const float l1 = decode_log_scale(r[e->scale1], e->scale_mode);
@@ -234,8 +275,8 @@ uint32_t or_decode_records(const float* records, uint32_t count, const or_encodi
         positions[j * 4 + 1] = py;
         positions[j * 4 + 2] = pz;
         positions[j * 4 + 3] = clampf_(a, 0.0f, 1.0f);
-        float q[4] = {decode_byte_unit(r[e->rot_x], e->rotation_mode), decode_byte_unit(r[e->rot_y], e->rotation_mode),
-                      decode_byte_unit(r[e->rot_z], e->rotation_mode), decode_byte_unit(r[e->rot_w], e->rotation_mode)};
+        float q[4];
+        decode_rotation(r, e, q);
         const float qlen = sqrtf(((q[0] * q[0] + q[1] * q[1]) + q[2] * q[2]) + q[3] * q[3]);
         if (!(qlen > 1e-8f)) {
             q[0] = 0.0f; q[1] = 0.0f; q[2] = 0.0f; q[3] = 1.0f;
@@ -253,7 +294,8 @@ uint32_t or_decode_records(const float* records, uint32_t count, const or_encodi
             for (uint32_t colour = 0; colour < 3; ++colour) {
                 const uint32_t field = e->rest_colour_outer ? e->rest_base + colour * e->file_per_colour + basis
                                                             : e->rest_base + basis * 3 + colour;
-                const float value = decode_byte_unit(r[field], e->rest_mode);
+                float value = decode_byte_unit(r[field], e->rest_mode);
+                if (e->flip_yz) value *= kRestFlip[basis < 14u ? basis : 14u];
                 word |= (uint32_t)or_float_to_half(half_safe(value)) << (filled * 16);
                 if (++filled == 2) {
                     sh[(size_t)j * sh_words + at++] = word;

@@ -58,21 +58,26 @@ uint32_t or_pack_halves(float lo, float hi);
  *   scale_mode     0 log, 1 linear (log(max(v, 1e-30))), 2 SPZ byte (v / 16 - 10)
  *   colour_mode    0 SH DC (0.5 + C0 v), 1 linear, 2 byte (v / 255),
  *                  3 SPZ byte (0.5 + C0 ((v / 255 - 0.5) / 0.15))
- *   rotation_mode  0 float (w, x, y, z), 1 byte ((v - 128) / 128 each)
+ *   rotation_mode  0 float (w, x, y, z), 1 byte ((v - 128) / 128 each),
+ *                  2 SPZ v2 (x y z bytes v / 127.5 - 1, w from unit length),
+ *                  3 SPZ v3 smallest three (a u32: low 16 bits at rot_x, high 16 at rot_y)
  *   rest_mode      0 float, 1 byte ((v - 128) / 128)
+ * Positions are multiplied by 2^-fractional_bits; flip_yz: the record is
+ * right-up-back (SPZ) and y and z change sign in positions and rotations, and
+ * in the rest coefficients odd in y xor z (splat_encoding.slang:35-112).
  * All zero is a 3DGS PLY (tests/render/SplatFixtures.h CloudBuilder). */
 typedef struct {
     uint32_t floats_per_record;
     uint32_t x, y, z, opacity, scale0, scale1, scale2, rot_w, rot_x, rot_y, rot_z;
     uint32_t dc0, dc1, dc2, rest_base, file_per_colour, rest_colour_outer;
     uint32_t opacity_mode, scale_mode, colour_mode, rotation_mode, rest_mode;
+    uint32_t fractional_bits, flip_yz;
 } or_encoding;
 
 /* splat_validate + scan + splat_decode for one slice (shaders/lrt/scene/
  * splat_validate.slang:16-29, splat_decode.slang:17-87, splat_encoding.slang:
  * 28-96, GpuClouds.cpp:151-227). Writes compacted packed splats; returns the
- * number kept. The SPZ rotation forms and the y/z flip are not restated: no
- * reader here produces them. */
+ * number kept. */
 uint32_t or_decode_records(const float* records, uint32_t count, const or_encoding* e,
                            uint32_t keep_per_colour, float* positions, uint32_t* shape,
                            uint32_t* sh, uint32_t sh_words);

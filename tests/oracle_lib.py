@@ -53,7 +53,8 @@ class OrEncoding(C.Structure):
     _fields_ = [(n, C.c_uint32) for n in ("floats_per_record", "x", "y", "z", "opacity", "scale0", "scale1",
                                           "scale2", "rot_w", "rot_x", "rot_y", "rot_z", "dc0", "dc1", "dc2",
                                           "rest_base", "file_per_colour", "rest_colour_outer", "opacity_mode",
-                                          "scale_mode", "colour_mode", "rotation_mode", "rest_mode")]
+                                          "scale_mode", "colour_mode", "rotation_mode", "rest_mode",
+                                          "fractional_bits", "flip_yz")]
 
 
 class OrCloud(C.Structure):

@@ -73,6 +73,7 @@ CITED = {
     "modules/io/include/lrt/io/RawSplats.h:15-77": ("modules/io/include/lrt/io/RawSplats.h", "struct SplatEncoding"),
     "RawSplats.h:42-58": ("modules/io/include/lrt/io/RawSplats.h", "enum class Opacity"),
     "modules/io/src/SplatReaders.cpp:32-195": ("modules/io/src/SplatReaders.cpp", "readSplats("),
+    "modules/io/src/SplatReaders.cpp:32-328": ("modules/io/src/SplatReaders.cpp", "readSplats("),
     "modules/scene/src/GpuClouds.cpp:151-227": ("modules/scene/src/GpuClouds.cpp", "CloudLoader::decodeSlice("),
     "modules/scene/src/GpuClouds.cpp:199-227": ("modules/scene/src/GpuClouds.cpp", "CloudLoader::upload("),
 }

@@ -123,3 +123,20 @@ def test_nothing_surviving_is_an_invalid_argument(rast):
     with pytest.raises(LrtError) as err:
         rast.decode_cloud(raw, 3)
     assert err.value.code == 3
+
+
+@pytest.mark.parametrize("version,degree,bits", [(3, 1, 12), (3, 4, 10), (2, 3, 8), (2, 0, 16)])
+def test_spz_files_decode_as_the_oracle_does(rast, tmp_path, version, degree, bits):
+    """SPZ (version 2: first-three rotation; 3: smallest three), every degree
+    path including degree 4 kept up to 3, fixed point at several widths,
+    right-up-back flipped: the GPU decode against the oracle's, bit for bit."""
+    from test_spz import _random_spz, write_spz
+    rng = np.random.default_rng(100 + version * 10 + degree)
+    n = 20_000
+    streams, _ = _random_spz(rng, version, n, degree, bits)
+    path = str(tmp_path / "r.spz")
+    write_spz(path, version, n, degree, bits, streams)
+    raw = raster.read_splats(path)
+    print(f"SPZ v{version} degree {degree}, {bits} fractional bits", flush=True)
+    for max_degree in (3, 1):
+        _check(rast, raw, max_degree)

@@ -149,7 +149,9 @@ def test_refusals_carry_the_reference_codes(tmp_path):
     assert _refusal(tmp_path / "i.ply") == UNSUPPORTED               # a list element before the vertices
     (tmp_path / "j.splat").write_bytes(b"\x00" * 33)
     assert _refusal(tmp_path / "j.splat") == INVALID
-    for name in ("k.spz", "l.sog", "m.xyz"):
+    (tmp_path / "k.spz").write_bytes(b"\x00" * 32)
+    assert _refusal(tmp_path / "k.spz") == IO_FAILURE                  # not a gzip stream
+    for name in ("l.sog", "m.xyz"):
         (tmp_path / name).write_bytes(b"\x00" * 32)
         assert _refusal(tmp_path / name) == UNSUPPORTED
     assert _refusal(tmp_path / "missing.ply") == IO_FAILURE
