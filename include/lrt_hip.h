@@ -467,6 +467,20 @@ lrt_status lrt_frame_params(const lrt_projection* projection, const lrt_splat_in
  * in and out must differ. Queued on the context's stream, not waited for. */
 lrt_status lrt_prefix_sum(lrt_context* ctx, const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count);
 
+/* The prefix sum of lrt_prefix_sum (same contract, same kernels) with no
+ * allocation on any call and no context. No counterpart in the reference: it
+ * is for callers that cannot allocate device memory (code under HIP graph
+ * capture, tight serving loops, harnesses that forbid hipMalloc). Its scratch
+ * (look-back words and counters, about 768 KiB) is a static array in the
+ * library's code object, one copy a device, held until the process ends.
+ * count is at most LRT_PREFIX_SUM_STATIC_MAX (2^28; LRT_INVALID_ARGUMENT past
+ * it). Runs on the current device, queued on `stream` (a hipStream_t; NULL is
+ * the default stream), not waited for. Calls that overlap on one device share
+ * that scratch and race: serialise them (one stream, or events). Messages:
+ * lrt_last_error(NULL). */
+#define LRT_PREFIX_SUM_STATIC_MAX (1u << 28)
+lrt_status lrt_prefix_sum_static(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count, void* stream);
+
 /* gpu::RadixSort::sort (modules/gpu/include/lrt/gpu/algo/RadixSort.h:56-57):
  * stable sort of count (key, value) pairs by the key's low key_bits bits
  * (1..64), eight bits a pass. Keys are u64 here (the reference splits them in

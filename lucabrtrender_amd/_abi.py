@@ -223,6 +223,7 @@ SIGNATURES = {
     "lrt_frame_params": (C.c_int, [C.POINTER(ProjectionC), C.POINTER(SplatInstanceC), C.POINTER(RenderSettingsC),
                                    C.c_uint32, C.c_void_p, C.c_size_t, C.POINTER(C.c_size_t)]),
     "lrt_prefix_sum": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32]),
+    "lrt_prefix_sum_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "lrt_radix_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.POINTER(C.c_int32)]),
     "lrt_context_synchronize": (C.c_int, [C.c_void_p]),

@@ -618,6 +618,27 @@ lrt_status lrt_prefix_sum(lrt_context* ctx, const uint32_t* in, uint32_t* out, u
     return finish(ctx, ctx->raster.prefixSum(in, out, total, count));
 }
 
+lrt_status lrt_prefix_sum_static(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count, void* stream) {
+    static_assert(LRT_PREFIX_SUM_STATIC_MAX == lrt::kern::kStaticScanMax, "the header's limit is the scan's");
+    if (in == nullptr || out == nullptr || total == nullptr) {
+        return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument, "null buffer"));
+    }
+    if (in == out) {
+        return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument, "the scan's input and output must differ"));
+    }
+    if (count > LRT_PREFIX_SUM_STATIC_MAX) {
+        return noContext(lrt::Status::error(lrt::ErrorCode::InvalidArgument,
+                                            "count " + std::to_string(count) + " exceeds the static scan's maximum " +
+                                                std::to_string(LRT_PREFIX_SUM_STATIC_MAX)));
+    }
+    const hipError_t e = lrt::kern::scanExclusiveStatic(in, out, count, total, static_cast<hipStream_t>(stream));
+    if (e != hipSuccess) {
+        return noContext(lrt::Status::error(lrt::ErrorCode::DeviceFailure,
+                                            std::string("static prefix sum: ") + hipGetErrorString(e)));
+    }
+    return noContext({});
+}
+
 lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, uint64_t* scratch_keys,
                           uint32_t* scratch_values, uint32_t count, uint32_t key_bits, int32_t* result_in_scratch) {
     if (ctx == nullptr) return LRT_INVALID_ARGUMENT;

@@ -30,6 +30,12 @@ void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* stat
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks = nullptr,
                    uint32_t depthEntries = 0, uint32_t* depthRange = nullptr, uint32_t* alsoZero = nullptr,
                    uint32_t alsoZeroWords = 0);
+// The same scan with its scratch in a static device array instead of `status`
+// and counters: no allocation on any call. n at most kStaticScanMax
+// (hipErrorInvalidValue past it). Calls that overlap on one device share that
+// array and race.
+constexpr uint32_t kStaticScanMax = 1u << 28;
+hipError_t scanExclusiveStatic(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, hipStream_t stream);
 
 // ---- projection -----------------------------------------------------------
 // One instance: f.count splats from index 0 of the cloud arrays, written to

@@ -13,6 +13,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <mutex>
 
 #include "kernels.h"
 
@@ -243,6 +244,50 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream) {
 }
 
 uint32_t scanPartitions(uint32_t n) { return (n + kPart - 1) / kPart; }
+
+// ---- the prefix sum that allocates nothing ------------------------------------
+// Its look-back words, per-partition counts and counters live in one array in
+// the code object, each device holding its own copy: the status words of
+// kStaticScanMax elements, then one word holding the non-zero count (low u32)
+// and the error flags (high u32).
+namespace {
+constexpr uint64_t kStaticStatusWords = uint64_t{(kStaticScanMax + kPart - 1) / kPart} * 3 / 2 + 2;
+constexpr uint64_t kStaticScratchWords = kStaticStatusWords + 1;
+static __device__ __align__(256) uint64_t scratch[kStaticScratchWords];
+
+constexpr int kStaticDevices = 64;
+std::mutex gScratchMutex;
+uint64_t* gScratch[kStaticDevices] = {};   // each device's address of `scratch`
+
+// The current device's address of `scratch`, looked up once per device with
+// that device current.
+hipError_t staticScratch(uint64_t** out) {
+    int device = 0;
+    hipError_t e = hipGetDevice(&device);
+    if (e != hipSuccess) return e;
+    if (device < 0 || device >= kStaticDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(gScratchMutex);
+    if (gScratch[device] == nullptr) {
+        void* ptr = nullptr;
+        e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return e;
+        gScratch[device] = static_cast<uint64_t*>(ptr);
+    }
+    *out = gScratch[device];
+    return hipSuccess;
+}
+}   // namespace
+
+hipError_t scanExclusiveStatic(const uint32_t* in, uint32_t* out, uint32_t n, uint32_t* total, hipStream_t stream) {
+    if (n > kStaticScanMax) return hipErrorInvalidValue;
+    uint64_t* ptr = nullptr;
+    const hipError_t e = staticScratch(&ptr);
+    if (e != hipSuccess) return e;
+    uint32_t* counters = reinterpret_cast<uint32_t*>(ptr + kStaticStatusWords);
+    // The error flags are OR-ed by the scan: cleared with the look-back words.
+    scanExclusive(in, out, n, ptr, total, counters, counters + 1, stream, nullptr, 0, nullptr, counters + 1, 1);
+    return hipGetLastError();
+}
 
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks,

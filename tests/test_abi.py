@@ -75,6 +75,17 @@ def test_calls_without_a_context_are_invalid_arguments():
     assert lib.lrt_raster_backward(None, None, C.byref(_abi.DiffGradsC()), None) == 3
 
 
+def test_the_static_prefix_sum_refuses_bad_arguments_before_any_device_call():
+    lib = _abi.lib
+    a, b, t = C.c_void_p(4096), C.c_void_p(8192), C.c_void_p(12288)   # never dereferenced: refused first
+    assert lib.lrt_prefix_sum_static(None, b, t, 16, None) == 3
+    assert lib.lrt_prefix_sum_static(a, a, t, 16, None) == 3
+    assert "differ" in lib.lrt_last_error(None).decode()
+    assert lib.lrt_prefix_sum_static(a, b, t, (1 << 28) + 1, None) == 3
+    msg = lib.lrt_last_error(None).decode()
+    assert "exceeds" in msg and str(1 << 28) in msg
+
+
 def test_frame_params_need_no_device():
     cam = raster.Camera.looking_at((1.0, 2.0, 3.0), (0.0, 0.0, 0.0))
     p = raster.projection_for(cam, 640, 480)
