@@ -74,8 +74,8 @@ KERNEL_OF_STAGE = {"project": "projectKernel", "scan": "scanKernel", "duplicate"
 
 def measured_traffic_bytes(stage, passes):
     """HBM bytes one launch of the stage's kernel moved, from the committed
-    rocprofv3 --pmc summary (FETCH_SIZE and WRITE_SIZE scaled by a device copy
-    of known size), or None without one."""
+    rocprofv3 --pmc summary (32 x the DRAM read and write request counts in
+    32-byte units), or None without one."""
     path = os.path.join(ROOT, "profiles", "traffic_scene-fwd-3M-1080p.txt")
     try:
         text = open(path).read()
