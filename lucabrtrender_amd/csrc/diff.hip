@@ -189,17 +189,25 @@ __device__ __forceinline__ float dppMove(float v) {
     return __int_as_float(__builtin_amdgcn_update_dpp(0, __float_as_int(v), CTRL, ROW_MASK, 0xF, false));
 }
 
-// Sum over the 64 lanes, landing in lane 63: pairs, quads, half rows and rows
-// by quad_perm and the row mirrors, then row 0 into row 1 and row 2 into row 3
-// (row_bcast:15), then rows 0-1 into rows 2-3 (row_bcast:31). Every lane must
-// be active.
+// The same move leaving the rows outside row_mask undefined (one instruction
+// fewer than reading 0 there): for steps whose other rows no later step reads.
+template <int CTRL, int ROW_MASK>
+__device__ __forceinline__ float dppMoveRows(float v) {
+    return __int_as_float(__builtin_amdgcn_mov_dpp(__float_as_int(v), CTRL, ROW_MASK, 0xF, false));
+}
+
+// Sum over the 64 lanes, landing in lane 63 (the other lanes hold partial or
+// undefined values): pairs, quads, half rows and rows by quad_perm and the row
+// mirrors, then row 0 into row 1 and row 2 into row 3 (row_bcast:15), then
+// rows 0-1 into rows 2-3 (row_bcast:31). Lane 63 reads only rows the earlier
+// steps wrote. Every lane must be active.
 __device__ __forceinline__ float waveSum(float v) {
-    v += dppMove<0xB1, 0xF>(v);    // quad_perm [1, 0, 3, 2]
-    v += dppMove<0x4E, 0xF>(v);    // quad_perm [2, 3, 0, 1]
-    v += dppMove<0x141, 0xF>(v);   // row_half_mirror
-    v += dppMove<0x140, 0xF>(v);   // row_mirror
-    v += dppMove<0x142, 0xA>(v);   // row_bcast:15
-    v += dppMove<0x143, 0xC>(v);   // row_bcast:31
+    v += dppMove<0xB1, 0xF>(v);       // quad_perm [1, 0, 3, 2]
+    v += dppMove<0x4E, 0xF>(v);       // quad_perm [2, 3, 0, 1]
+    v += dppMove<0x141, 0xF>(v);      // row_half_mirror
+    v += dppMove<0x140, 0xF>(v);      // row_mirror
+    v += dppMoveRows<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
+    v += dppMoveRows<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
     return v;
 }
 
@@ -231,7 +239,7 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
                                                   const uint32_t* __restrict__ lastCount,
                                                   const float* __restrict__ finalT,
                                                   const float4* __restrict__ gradColour,
-                                                  float4* __restrict__ pairGrad) {
+                                                  PairGrads pairGrad) {
     __shared__ float4 sGeom[kBatch];     // centre.xy, cutoff, alpha
     __shared__ float4 sConic[kBatch];    // a, 2b, c, (unused)
     __shared__ float4 sColour[kBatch];   // rgb, bits of the pair's place in the tile's range
@@ -299,8 +307,10 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
     const uint32_t maxLast = sMaxLast < range.y - range.x ? sMaxLast : range.y - range.x;
     // Pairs past every pixel's last contributor get zero gradients.
     for (uint32_t pos = range.x + maxLast + tid; pos < range.y; pos += kBlock) {
-        float4* out = pairGrad + static_cast<uint64_t>(pairIndex[pos]) * 3;
-        out[0] = out[1] = out[2] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        const uint32_t at = pairIndex[pos];
+        pairGrad.ab[2 * uint64_t{at}] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pairGrad.ab[2 * uint64_t{at} + 1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+        pairGrad.c[at] = 0.0f;
     }
 
     float t = tFinal, behind[3] = {0.0f, 0.0f, 0.0f};
@@ -430,10 +440,10 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
                     }
                 }
             }
-            float4* out = pairGrad + static_cast<uint64_t>(pairIndex[top - 1u - tid]) * 3;
-            out[0] = acc[0];
-            out[1] = acc[1];
-            out[2] = acc[2];
+            const uint32_t at = pairIndex[top - 1u - tid];
+            pairGrad.ab[2 * uint64_t{at}] = acc[0];
+            pairGrad.ab[2 * uint64_t{at} + 1] = acc[1];
+            pairGrad.c[at] = acc[2].x;
         }
         __syncthreads();
         top = lo;
@@ -449,9 +459,13 @@ __global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uin
                                                               const uint32_t* __restrict__ lastCount,
                                                               const float* __restrict__ finalT,
                                                               const float4* __restrict__ gradColour,
-                                                              float4* __restrict__ pairGrad) {
-    blendBackwardTile(f, blockIdx.x, ranges[blockIdx.x], slots, sortedSlots, pairIndex, proj, accum, lastCount,
-                      finalT, gradColour, pairGrad);
+                                                              PairGrads pairGrad,
+                                                              const uint32_t* __restrict__ order) {
+    // Heaviest tiles first when an order is given: a long tile starts early
+    // instead of running alone at the end.
+    const uint32_t tile = order != nullptr ? order[blockIdx.x] : blockIdx.x;
+    blendBackwardTile(f, tile, ranges[tile], slots, sortedSlots, pairIndex, proj, accum, lastCount, finalT,
+                      gradColour, pairGrad);
 }
 
 // Every tile of a batch of views: workgroup g takes batch tile g, in its view's
@@ -468,8 +482,9 @@ __global__ __launch_bounds__(kBlock) void blendBackwardViewsKernel(const FramePa
                                                                    const uint32_t* __restrict__ lastCount,
                                                                    const float* __restrict__ finalT,
                                                                    const float4* const* __restrict__ gradColours,
-                                                                   float4* __restrict__ pairGrad) {
-    const uint32_t g = blockIdx.x;
+                                                                   PairGrads pairGrad,
+                                                                   const uint32_t* __restrict__ order) {
+    const uint32_t g = order != nullptr ? order[blockIdx.x] : blockIdx.x;
     const uint32_t v = viewOfTile(views, viewCount, g);
     const ViewInfo& info = views[v];
     const FrameParams f = params[info.params];
@@ -749,16 +764,18 @@ __device__ __forceinline__ void splatBackward(const FrameParams& f, const DiffCl
 
 // The screen-space gradient of slot `slot`: the sum of its run of pair records
 // (its pairs are contiguous in slot-major order).
-__device__ __forceinline__ void slotGradient(const float4* __restrict__ pairGrad, uint32_t first, uint32_t n,
-                                             float g[9]) {
+__device__ __forceinline__ void slotGradient(const PairGrads& pairGrad, uint32_t first, uint32_t n, float g[9]) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) g[k] = 0.0f;
-    const float4* run = pairGrad + static_cast<uint64_t>(first) * 3;
+    const float4* ra = pairGrad.ab + 2 * uint64_t{first};
+    const float* rc = pairGrad.c + first;
+#pragma unroll 4
     for (uint32_t e = 0; e < n; ++e) {
-        const float4 a = run[e * 3], b = run[e * 3 + 1], d = run[e * 3 + 2];
+        const float4 a = ra[2 * e], b = ra[2 * e + 1];
+        const float d = rc[e];
         g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
         g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
-        g[8] += d.x;
+        g[8] += d;
     }
 }
 
@@ -783,8 +800,7 @@ __device__ __forceinline__ void storeGrad(const SplatGrad& acc, uint32_t i, uint
 __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, DiffCloud c,
                                                                 const uint32_t* __restrict__ touched,
                                                                 const uint32_t* __restrict__ offsets,
-                                                                const float4* __restrict__ pairGrad,
-                                                                DiffGrads out) {
+                                                                PairGrads pairGrad, DiffGrads out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= f.count) return;
     const uint32_t slot = f.base + i;
@@ -809,8 +825,7 @@ __global__ __launch_bounds__(kBlock) void projectBackwardViewsKernel(const Frame
                                                                      uint32_t views, DiffCloud c,
                                                                      const uint32_t* __restrict__ touched,
                                                                      const uint32_t* __restrict__ offsets,
-                                                                     const float4* __restrict__ pairGrad,
-                                                                     DiffGrads out) {
+                                                                     PairGrads pairGrad, DiffGrads out) {
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
     if (i >= c.count) return;
     const FrameParams& f0 = params[0];
@@ -855,25 +870,27 @@ void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t p
 
 void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
                    const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
-                   const float* finalT, const float4* gradColour, float4* pairGrad, hipStream_t stream) {
+                   const float* finalT, const float4* gradColour, PairGrads pairGrad, hipStream_t stream,
+                   const uint32_t* order) {
     const uint32_t tiles = f.tilesX * f.tilesY;
     if (tiles == 0) return;
     blendBackwardKernel<<<tiles, kBlock, 0, stream>>>(f, slots, ranges, sortedSlots, pairIndex, proj, accum,
-                                                      lastCount, finalT, gradColour, pairGrad);
+                                                      lastCount, finalT, gradColour, pairGrad, order);
 }
 
 void blendBackwardViews(const FrameParams* params, const ViewInfo* views, uint32_t viewCount, uint32_t tiles,
                         uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots, const uint32_t* pairIndex,
                         const float4* proj, const float4* accum, const uint32_t* lastCount, const float* finalT,
-                        const float4* const* gradColours, float4* pairGrad, hipStream_t stream) {
+                        const float4* const* gradColours, PairGrads pairGrad, hipStream_t stream,
+                        const uint32_t* order) {
     if (tiles == 0 || viewCount == 0) return;
     blendBackwardViewsKernel<<<tiles, kBlock, 0, stream>>>(params, views, viewCount, slots, ranges, sortedSlots,
                                                            pairIndex, proj, accum, lastCount, finalT, gradColours,
-                                                           pairGrad);
+                                                           pairGrad, order);
 }
 
 void projectBackwardViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, const uint32_t* touched,
-                          const uint32_t* offsets, const float4* pairGrad, const DiffGrads& grads,
+                          const uint32_t* offsets, PairGrads pairGrad, const DiffGrads& grads,
                           hipStream_t stream) {
     if (cloud.count == 0 || views == 0) return;
     projectBackwardViewsKernel<<<(cloud.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(params, views, cloud,
@@ -882,7 +899,7 @@ void projectBackwardViews(const FrameParams* params, uint32_t views, const DiffC
 }
 
 void projectBackward(const FrameParams& f, const DiffCloud& cloud, const uint32_t* touched, const uint32_t* offsets,
-                     const float4* pairGrad, const DiffGrads& grads, hipStream_t stream) {
+                     PairGrads pairGrad, const DiffGrads& grads, hipStream_t stream) {
     if (f.count == 0) return;
     projectBackwardKernel<<<(f.count + kBlock - 1) / kBlock, kBlock, 0, stream>>>(f, cloud, touched, offsets,
                                                                                   pairGrad, grads);

@@ -270,26 +270,43 @@ void projectDiffViews(const FrameParams* params, uint32_t views, const DiffCloud
 // then being the capacity).
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
                  hipStream_t stream, const uint32_t* nDevice = nullptr);
-// dL/d(colour) per pixel -> one screen-space gradient record per pair (3 float4:
-// d centre.xy, d conic a b; d conic c, d alpha, d r g; d b) at pairGrad[index].
+// Each pair's nine screen-space gradient terms, 36 bytes a pair, in one buffer
+// of `capacity` pairs: a 32-byte record ab[2i], ab[2i + 1] = d (centre x,
+// centre y, conic a, conic b), d (conic c, alpha, r, g), then an array c[i] =
+// d b. The blend backward writes pairs at scattered places; a whole aligned
+// 32-byte record is one full DRAM sector, where two separate float4 arrays
+// would leave twice the partly written sectors.
+struct PairGrads {
+    float4* ab;
+    float* c;
+};
+constexpr uint64_t kPairGradBytes = 36;
+inline PairGrads pairGradsIn(void* buffer, uint32_t capacity) {
+    auto* ab = static_cast<float4*>(buffer);
+    return PairGrads{ab, reinterpret_cast<float*>(ab + 2ull * capacity)};
+}
+// dL/d(colour) per pixel -> one screen-space gradient record per pair, at the
+// pair's index in pairGrad.
 void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
                    const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
-                   const float* finalT, const float4* gradColour, float4* pairGrad, hipStream_t stream);
+                   const float* finalT, const float4* gradColour, PairGrads pairGrad, hipStream_t stream,
+                   const uint32_t* order = nullptr);
 // The blend backward of every tile of a batch (device FrameParams and ViewInfo
 // arrays as blendViews'; accum/lastCount/finalT hold the batch's pixels at
 // ViewInfo.pixelBase); gradColours: a device array of one colour-gradient
-// pointer a view.
+// pointer a view. order (may be null): the tiles in launch order (tileOrder's).
 void blendBackwardViews(const FrameParams* params, const ViewInfo* views, uint32_t viewCount, uint32_t tiles,
                         uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots, const uint32_t* pairIndex,
                         const float4* proj, const float4* accum, const uint32_t* lastCount, const float* finalT,
-                        const float4* const* gradColours, float4* pairGrad, hipStream_t stream);
+                        const float4* const* gradColours, PairGrads pairGrad, hipStream_t stream,
+                        const uint32_t* order = nullptr);
 // Each splat's runs of pair records in every view -> gradients of its
 // parameters summed over the views in view order.
 void projectBackwardViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, const uint32_t* touched,
-                          const uint32_t* offsets, const float4* pairGrad, const DiffGrads& grads,
+                          const uint32_t* offsets, PairGrads pairGrad, const DiffGrads& grads,
                           hipStream_t stream);
 // Each splat's run of pair records -> gradients of its parameters.
 void projectBackward(const FrameParams& f, const DiffCloud& cloud, const uint32_t* touched, const uint32_t* offsets,
-                     const float4* pairGrad, const DiffGrads& grads, hipStream_t stream);
+                     PairGrads pairGrad, const DiffGrads& grads, hipStream_t stream);
 
 }   // namespace lrt::kern

@@ -1002,14 +1002,18 @@ Status TileRasterizer::backward(const float4* gradColour, const kern::DiffGrads&
     }
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
-    if (!(s = pairGrad_.reserve(uint64_t{std::max(pairCapacity_, 1u)} * 48, "pair.grad")).ok()) return s;
+    if (!(s = pairGrad_.reserve(uint64_t{std::max(pairCapacity_, 1u)} * kern::kPairGradBytes, "pair.grad")).ok()) return s;
+    const kern::PairGrads pairGrads = kern::pairGradsIn(pairGrad_.get(), std::max(pairCapacity_, 1u));
     if (millis != nullptr) (void)hipEventRecord(events_[0], stream_);
+    // One view's tiles in raster order: neighbouring tiles share splats in L2,
+    // and on a 1M-splat 720p frame that beat heaviest-first by about 10%.
     kern::blendBackward(diffFrame_, lastSplats_, ranges_.as<uint2>(), sortedSlots_.as<uint32_t>(),
                         values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(), accum_.as<float4>(),
-                        lastCount_.as<uint32_t>(), finalT_.as<float>(), gradColour, pairGrad_.as<float4>(), stream_);
+                        lastCount_.as<uint32_t>(), finalT_.as<float>(), gradColour, pairGrads, stream_,
+                        nullptr);
     if (millis != nullptr) (void)hipEventRecord(events_[1], stream_);
     kern::projectBackward(diffFrame_, diffCloud_, touched_.as<uint32_t>(), offsets_.as<uint32_t>(),
-                          pairGrad_.as<float4>(), grads, stream_);
+                          pairGrads, grads, stream_);
     if (millis != nullptr) (void)hipEventRecord(events_[2], stream_);
     if (!(s = check(hipStreamSynchronize(stream_), "backward")).ok()) return s;
     if (!(s = check(hipGetLastError(), "backward")).ok()) return s;
@@ -1038,7 +1042,8 @@ Status TileRasterizer::backwardViews(std::span<const float4* const> gradColours,
     }
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
-    if (!(s = pairGrad_.reserve(uint64_t{std::max(pairCapacity_, 1u)} * 48, "pair.grad")).ok()) return s;
+    if (!(s = pairGrad_.reserve(uint64_t{std::max(pairCapacity_, 1u)} * kern::kPairGradBytes, "pair.grad")).ok()) return s;
+    const kern::PairGrads pairGrads = kern::pairGradsIn(pairGrad_.get(), std::max(pairCapacity_, 1u));
     if (diffViews_ > gradPtrCapacity_) {
         if (hostGradPtrs_ != nullptr) (void)hipHostFree(hostGradPtrs_);
         hostGradPtrs_ = nullptr;
@@ -1057,13 +1062,16 @@ Status TileRasterizer::backwardViews(std::span<const float4* const> gradColours,
         return s;
     const auto* params = viewParams_.as<FrameParams>();
     if (millis != nullptr) (void)hipEventRecord(events_[0], stream_);
+    // The batch's tiles heaviest first: a view of few, long tiles (a small view
+    // close to the cloud) would otherwise finish alone at the end.
+    kern::tileOrder(ranges_.as<uint2>(), lastTiles_, tileOrder_.as<uint32_t>(), stream_);
     kern::blendBackwardViews(params, viewInfo_.as<ViewInfo>(), diffViews_, lastTiles_, lastSplats_, ranges_.as<uint2>(),
                              sortedSlots_.as<uint32_t>(), values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(),
                              accum_.as<float4>(), lastCount_.as<uint32_t>(), finalT_.as<float>(),
-                             viewGrads_.as<const float4*>(), pairGrad_.as<float4>(), stream_);
+                             viewGrads_.as<const float4*>(), pairGrads, stream_, tileOrder_.as<uint32_t>());
     if (millis != nullptr) (void)hipEventRecord(events_[1], stream_);
     kern::projectBackwardViews(params, diffViews_, diffCloud_, touched_.as<uint32_t>(), offsets_.as<uint32_t>(),
-                               pairGrad_.as<float4>(), grads, stream_);
+                               pairGrads, grads, stream_);
     if (millis != nullptr) (void)hipEventRecord(events_[2], stream_);
     if (!(s = check(hipStreamSynchronize(stream_), "backward")).ok()) return s;
     if (!(s = check(hipGetLastError(), "backward")).ok()) return s;
