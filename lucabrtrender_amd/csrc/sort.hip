@@ -46,9 +46,9 @@ constexpr uint32_t kAggregate = 1u << 30;
 constexpr uint32_t kInclusive = 2u << 30;
 constexpr uint32_t kValueMask = (1u << 30) - 1;
 constexpr uint32_t kSpinLimit = 1u << 18;
-constexpr uint32_t kLookBack = 16;   // predecessor partitions read a look-back step
+constexpr uint32_t kLookBack = 8;    // predecessor partitions read a look-back step
 constexpr uint32_t kHistBlock = 1024;
-constexpr uint32_t kHistRows = 256;     // histogram workgroups at most (one row of counts each)
+constexpr uint32_t kHistRows = 512;     // histogram workgroups at most (one row of counts each)
 constexpr uint32_t kHistKeys = 8192;    // keys a histogram workgroup below that cap
 
 // The pairs to sort: the host's count, or the one a kernel left in device
@@ -79,8 +79,10 @@ __device__ __forceinline__ uint64_t sortKey(uint64_t key, uint32_t bitLo, uint64
 // hist[b * passes * 256 + pass * 256 + digit]. The 64 lanes of a wave hold 64
 // consecutive keys, and consecutive keys often share a digit (a splat's pairs
 // share its depth; neighbouring tiles share the tile id's high bits), so each
-// run of equal digits among neighbouring lanes is added once, by its first
-// lane, with its length: no same-address LDS atomics within a wave.
+// run of equal digits among neighbouring lanes is added as two terms: minus
+// its first lane by that lane, and its last lane plus one by that lane. A run
+// costs two LDS atomics whatever its length, and no lane searches for the
+// run's end.
 template <KeyMode MODE>
 __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __restrict__ keys, uint32_t cap,
                                                               const uint32_t* __restrict__ nDevice, uint32_t bitLo,
@@ -103,7 +105,6 @@ __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __
     for (uint32_t j = threadIdx.x; j < passes * kRadix; j += kHistBlock) sHist[j] = 0;
     __syncthreads();
     const uint32_t lane = threadIdx.x & 63u;
-    const uint64_t after = lane == 63 ? 0ull : ~0ull << (lane + 1);   // lanes above this one
     const uint32_t stride = gridDim.x * kHistBlock;
     // The loop bound is the wave's first index, so whole waves iterate together.
     const uint32_t waveStart = blockIdx.x * kHistBlock + (threadIdx.x & ~63u);
@@ -119,16 +120,20 @@ __global__ __launch_bounds__(kHistBlock) void histogramKernel(const uint64_t* __
             const uint32_t first = w0 + u * stride;
             if (first >= n) break;   // uniform across the wave
             const uint32_t live = n - first < 64u ? n - first : 64u;   // lanes [0, live) hold keys
+            const bool in = lane < live;
+            const bool lastLive = lane + 1 == live;
             for (uint32_t p = 0; p < passes; ++p) {
                 const uint32_t dg = static_cast<uint32_t>((k[u] >> (8 * p)) & 255u);
-                // The previous lane's digit (lane 0 reads a value unequal to its own).
-                const uint32_t prev = static_cast<uint32_t>(
-                    __builtin_amdgcn_update_dpp(static_cast<int>(dg ^ 1u), static_cast<int>(dg), 0x138, 0xF, 0xF, false));
-                const bool head = lane < live && (lane == 0 || prev != dg);
-                const uint64_t heads = __ballot(head);
-                const uint64_t next = heads & after;
-                const uint32_t end = next != 0 ? static_cast<uint32_t>(__builtin_ctzll(next)) : live;
-                if (head) atomicAdd(&sHist[p * kRadix + dg], end - lane);
+                // The digits of the lanes below and above (wave_shr:1 and
+                // wave_shl:1); lanes 0 and 63 read a value unequal to their own.
+                const int other = static_cast<int>(dg ^ 1u);
+                const uint32_t prev =
+                    static_cast<uint32_t>(__builtin_amdgcn_update_dpp(other, static_cast<int>(dg), 0x138, 0xF, 0xF, false));
+                const uint32_t next =
+                    static_cast<uint32_t>(__builtin_amdgcn_update_dpp(other, static_cast<int>(dg), 0x130, 0xF, 0xF, false));
+                uint32_t* bin = &sHist[p * kRadix + dg];
+                if (in && prev != dg) atomicAdd(bin, 0u - lane);
+                if (in && (next != dg || lastLive)) atomicAdd(bin, lane + 1u);
             }
         }
     }
@@ -340,35 +345,43 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const typename PassKeys
     if (part > 0) {
         // Look back kLookBack partitions a step: the window's loads are in
         // flight together, so a walk of L partitions costs about L / kLookBack
-        // round trips to L2 instead of L. Partition 0 is inclusive from the
-        // start, so no walk reaches below it.
-        int64_t p = static_cast<int64_t>(part) - 1;
+        // round trips to L2 instead of L. Eight measured best on sorts of 2M
+        // and of 694M pairs: a wider window's status reads (1 KiB a
+        // partition) cost more than the round trips it saves. Partition 0 is
+        // inclusive from the start, so no walk reaches below it.
+        static_assert(kLookBack < 32, "a window's flags fit one word");
+        int32_t p = static_cast<int32_t>(part) - 1;   // partitions < 2^31
+        const uint32_t* column = status + tid;         // digit tid's word of partition 0
         uint32_t spins = 0;
         while (true) {
+            // Words below partition 0 read partition 0's (every load issued,
+            // no branch) and count as inclusive zeros.
             uint32_t s[kLookBack];
 #pragma unroll
             for (uint32_t j = 0; j < kLookBack; ++j) {
-                s[j] = p - j >= 0 ? __hip_atomic_load(status + static_cast<uint64_t>(p - j) * kRadix + tid,
-                                                      __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT)
-                                  : kInclusive;
+                const int32_t q = p - static_cast<int32_t>(j);
+                const uint32_t at = static_cast<uint32_t>(q > 0 ? q : 0) * kRadix;
+                const uint32_t word = __hip_atomic_load(column + at, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                s[j] = q >= 0 ? word : kInclusive;
             }
-            uint32_t taken = 0;
-            bool open = true, done = false;
+            // The window is summed from its top down to its first unpublished
+            // word, or through its first inclusive word if that comes first
+            // (bit j of each mask is word j's). Flags are 1 (aggregate) or 2
+            // (inclusive), never 3.
+            uint32_t published = 0, inclusive = 0;
 #pragma unroll
             for (uint32_t j = 0; j < kLookBack; ++j) {
-                const uint32_t flag = s[j] & ~kValueMask;
-                if (open && flag == 0) open = false;   // not yet published: wait for it
-                if (open) {
-                    prefix += s[j] & kValueMask;
-                    taken += 1;
-                    if (flag == kInclusive) {
-                        done = true;
-                        open = false;
-                    }
-                }
+                published |= (s[j] >= kAggregate ? 1u : 0u) << j;
+                inclusive |= (s[j] >> 31) << j;
             }
+            const uint32_t unpublished = static_cast<uint32_t>(__builtin_ctz(~published));   // kLookBack if none
+            const uint32_t firstInclusive = static_cast<uint32_t>(__builtin_ctz(inclusive | (1u << kLookBack)));
+            const bool done = firstInclusive < unpublished;
+            const uint32_t taken = done ? firstInclusive + 1 : unpublished;
+#pragma unroll
+            for (uint32_t j = 0; j < kLookBack; ++j) prefix += j < taken ? s[j] & kValueMask : 0u;
             if (done) break;
-            p -= taken;
+            p -= static_cast<int32_t>(taken);
             if (taken == 0) {
                 // The partition waited on has a lower workgroup id, so it is
                 // running or next on its XCD; the bound only guards a broken launch.
