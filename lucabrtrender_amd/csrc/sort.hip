@@ -2,7 +2,7 @@
 // the keys for every digit at once, then one onesweep launch per 8-bit digit.
 //
 // A onesweep workgroup (256 threads, 4 wave64s) takes partition blockIdx.x of
-// 3840 pairs and
+// 5120 pairs and
 //   1. ranks its keys by digit within each wave with ballot-built match masks
 //      (eight __ballot per 64 keys), keeping a per-wave digit histogram in LDS;
 //   2. turns the four wave histograms into per-wave offsets and a block digit
@@ -38,8 +38,11 @@ namespace {
 
 constexpr uint32_t kBlock = 256;
 constexpr uint32_t kWaves = kBlock / 64;
-constexpr uint32_t kKpt = 15;
-constexpr uint32_t kPart = kBlock * kKpt;   // 3840 pairs a partition
+constexpr uint32_t kKpt = 20;
+// 5120 pairs a partition: on a 694M-pair sort 20 a thread beat 12, 15, 18,
+// 24 and 30 (longer digit runs and fewer look-backs against the occupancy
+// that registers and LDS allow).
+constexpr uint32_t kPart = kBlock * kKpt;
 constexpr uint32_t kRadix = 256;
 constexpr uint32_t kMaxPasses = 8;
 constexpr uint32_t kAggregate = 1u << 30;
@@ -262,12 +265,13 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const typename PassKeys
     __shared__ uint32_t sVals[kPart];
     // Per-wave digit counts; once the scatter positions are in registers, the
     // digits of sKeys (bytes), so the write-out need not recompute them.
-    __shared__ uint32_t sWave[kWaves][kRadix];
+    constexpr uint32_t kCountWords = kWaves * kRadix > (kPart + 3) / 4 ? kWaves * kRadix : (kPart + 3) / 4;
+    __shared__ uint32_t sCounts[kCountWords];
     __shared__ uint32_t sLocal[kRadix];
     __shared__ uint32_t sGlobal[kRadix];
     __shared__ uint32_t sScan[kWaves];
-    static_assert(sizeof(sWave) >= kPart, "the digit bytes fit where the wave counts were");
-    uint8_t* sDigit = reinterpret_cast<uint8_t*>(&sWave[0][0]);
+    auto sWave = reinterpret_cast<uint32_t (*)[kRadix]>(sCounts);
+    uint8_t* sDigit = reinterpret_cast<uint8_t*>(sCounts);
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & 63u;
@@ -320,8 +324,9 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const typename PassKeys
         const uint32_t count = static_cast<uint32_t>(__popc(lo) + __popc(hi));
         // The count a leader stores is read by other lanes of the wave at the
         // next key: relaxed atomics, so the compiler keeps no per-lane copy.
-        uint32_t prev = 0;
-        if (valid) prev = __hip_atomic_load(&sWave[wave][d[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
+        // Every lane reads (a lane past n holds the digit of a clamped load),
+        // so no branch holds the read back.
+        const uint32_t prev = __hip_atomic_load(&sWave[wave][d[j]], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
         r[j] = prev + before;
         if (valid && before == 0) {
             __hip_atomic_store(&sWave[wave][d[j]], prev + count, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_WORKGROUP);
@@ -410,35 +415,41 @@ __global__ __launch_bounds__(kBlock) void onesweepKernel(const typename PassKeys
     bool misplaced = false;
 #pragma unroll
     for (uint32_t j = 0; j < kKpt; ++j) {
-        if (waveFirst + j * 64 + lane < n) {
-            const uint32_t at = r[j];
-            if (at < kPart) {
-                sKeys[at] = k[j];
-                sVals[at] = v[j];
-                sDigit[at] = static_cast<uint8_t>(d[j]);
-            }
-            misplaced |= at >= kPart;
+        const bool valid = waveFirst + j * 64 + lane < n;
+        const uint32_t at = r[j];
+        if (valid & (at < kPart)) {
+            sKeys[at] = k[j];
+            sVals[at] = v[j];
+            sDigit[at] = static_cast<uint8_t>(d[j]);
         }
+        misplaced |= valid & (at >= kPart);
     }
     if (misplaced) atomicOr(error, 2u);
     __syncthreads();
-    // Unrolled, stores predicated, a bad position flagged once after the loop:
-    // the LDS reads of all kKpt pairs can be in flight together.
+    // Every place's LDS reads are issued before any store, unconditionally
+    // (places past the count read stale entries of the partition's own LDS,
+    // digits below 256, and store nothing); only the stores are predicated,
+    // and a bad position is flagged once after the loop.
     const uint32_t count = n - base < kPart ? n - base : kPart;
+    uint32_t out[kKpt];
+#pragma unroll
+    for (uint32_t j = 0; j < kKpt; ++j) out[j] = sDigit[tid + j * kBlock];
+#pragma unroll
+    for (uint32_t j = 0; j < kKpt; ++j) out[j] = sGlobal[out[j]] + tid + j * kBlock;
+#pragma unroll
+    for (uint32_t j = 0; j < kKpt; ++j) {
+        k[j] = sKeys[tid + j * kBlock];
+        v[j] = sVals[tid + j * kBlock];
+    }
     bool bad = false;
 #pragma unroll
     for (uint32_t j = 0; j < kKpt; ++j) {
-        const uint32_t i = tid + j * kBlock;
-        if (i < count) {
-            const uint32_t out = sGlobal[sDigit[i]] + i;
-            const KOut key = sKeys[i];
-            const uint32_t val = sVals[i];
-            if (out < n) {
-                keysOut[out] = key;
-                valsOut[out] = val;
-            }
-            bad |= out >= n;
+        const bool inside = tid + j * kBlock < count;
+        if (inside & (out[j] < n)) {
+            keysOut[out[j]] = k[j];
+            valsOut[out[j]] = v[j];
         }
+        bad |= inside & (out[j] >= n);
     }
     if (bad) atomicOr(error, 4u);
 }
