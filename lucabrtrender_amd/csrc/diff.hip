@@ -17,6 +17,8 @@
 // pair at that index; each splat then sums its own contiguous run of records.
 #include <hip/hip_runtime.h>
 
+#include <type_traits>
+
 #include "kernels.h"
 #include "projection.h"
 #include "splat_math.h"
@@ -82,6 +84,53 @@ __device__ __forceinline__ void loadHarmonics(const DiffCloud& c, uint32_t i, ui
     for (int ch = 0; ch < 3; ++ch) dc[ch] = row[ch];
 #pragma unroll
     for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
+}
+
+// A workgroup's rows of a row-major float array (stride floats a row, at most
+// kRowPitchMax - 1) moved between global memory and LDS, row r at LDS word
+// r * pitch with pitch = stride | 1: consecutive threads move consecutive
+// words of global memory (coalesced; kBatch words in flight a thread), and a
+// thread reading its own row of LDS meets no bank conflict (odd pitch).
+constexpr uint32_t kRowPitchMax = 49;
+
+__device__ __forceinline__ uint32_t rowPitch(uint32_t stride) { return stride | 1u; }
+
+template <bool LOAD>
+__device__ __forceinline__ void moveRows(std::conditional_t<LOAD, const float*, float*> global, uint32_t stride,
+                                         uint32_t rows, float* rowsLds) {
+    constexpr uint32_t kBatch = 8;
+    const uint32_t total = rows * stride, pitch = rowPitch(stride);
+    uint32_t row = threadIdx.x / stride, col = threadIdx.x - row * stride;
+    const uint32_t rowStep = kBlock / stride, colStep = kBlock - rowStep * stride;
+    for (uint32_t e0 = threadIdx.x; e0 < total; e0 += kBlock * kBatch) {
+        uint32_t at[kBatch];
+#pragma unroll
+        for (uint32_t k = 0; k < kBatch; ++k) {
+            at[k] = row * pitch + col;
+            row += rowStep;
+            col += colStep;
+            if (col >= stride) {
+                col -= stride;
+                row += 1;
+            }
+        }
+        float v[kBatch];
+        if constexpr (LOAD) {
+#pragma unroll
+            for (uint32_t k = 0; k < kBatch; ++k) v[k] = e0 + k * kBlock < total ? global[e0 + k * kBlock] : 0.0f;
+#pragma unroll
+            for (uint32_t k = 0; k < kBatch; ++k) {
+                if (e0 + k * kBlock < total) rowsLds[at[k]] = v[k];
+            }
+        } else {
+#pragma unroll
+            for (uint32_t k = 0; k < kBatch; ++k) v[k] = e0 + k * kBlock < total ? rowsLds[at[k]] : 0.0f;
+#pragma unroll
+            for (uint32_t k = 0; k < kBatch; ++k) {
+                if (e0 + k * kBlock < total) global[e0 + k * kBlock] = v[k];
+            }
+        }
+    }
 }
 
 static_assert(kBlock == kProjectBlock, "one depth-range entry a workgroup of kProjectBlock slots");
@@ -762,21 +811,79 @@ __device__ __forceinline__ void splatBackward(const FrameParams& f, const DiffCl
     for (int k = 0; k < 3; ++k) acc.mean[k] = acc.mean[k] + (dP[k] + (dDir[k] - dir[k] * along) / len);
 }
 
-// The screen-space gradient of slot `slot`: the sum of its run of pair records
-// (its pairs are contiguous in slot-major order).
-__device__ __forceinline__ void slotGradient(const PairGrads& pairGrad, uint32_t first, uint32_t n, float g[9]) {
+// The screen-space gradients of a workgroup's kBlock consecutive slots, one a
+// thread: each is the sum of its run of pair records, [first, first + n), in
+// pair order. The slots' runs are consecutive in slot-major order, so together
+// they are one range of records, [begin, end); the workgroup streams it through
+// LDS kBlock records at a time with coalesced loads (the next chunk's loads in
+// flight while this one is summed), and each thread adds the records of its own
+// run from LDS. Every thread of the workgroup must call it (n = 0 for one with
+// no slot); begin/end are the same for all.
+struct PairChunk {
+    float4 a[kBlock];
+    float4 b[kBlock];
+    float c[kBlock];
+};
+
+__device__ __forceinline__ void slotGradients(const PairGrads& pairGrad, uint32_t begin, uint32_t end,
+                                              uint32_t first, uint32_t n, PairChunk& chunk, float g[9]) {
 #pragma unroll
     for (int k = 0; k < 9; ++k) g[k] = 0.0f;
-    const float4* ra = pairGrad.ab + 2 * uint64_t{first};
-    const float* rc = pairGrad.c + first;
-#pragma unroll 4
-    for (uint32_t e = 0; e < n; ++e) {
-        const float4 a = ra[2 * e], b = ra[2 * e + 1];
-        const float d = rc[e];
-        g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
-        g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
-        g[8] += d;
+    const uint32_t tid = threadIdx.x;
+    float4 na = make_float4(0.0f, 0.0f, 0.0f, 0.0f), nb = na;
+    float nc = 0.0f;
+    if (begin + tid < end) {
+        const uint64_t at = begin + tid;
+        na = pairGrad.ab[2 * at];
+        nb = pairGrad.ab[2 * at + 1];
+        nc = pairGrad.c[at];
     }
+    for (uint32_t c0 = begin; c0 < end; c0 += kBlock) {
+        chunk.a[tid] = na;
+        chunk.b[tid] = nb;
+        chunk.c[tid] = nc;
+        __syncthreads();
+        if (c0 + kBlock + tid < end) {
+            const uint64_t at = c0 + kBlock + tid;
+            na = pairGrad.ab[2 * at];
+            nb = pairGrad.ab[2 * at + 1];
+            nc = pairGrad.c[at];
+        }
+        const uint32_t lo = first > c0 ? first : c0;
+        const uint32_t hi = first + n < c0 + kBlock ? first + n : c0 + kBlock;
+        for (uint32_t q = lo; q < hi; ++q) {
+            const float4 a = chunk.a[q - c0], b = chunk.b[q - c0];
+            const float d = chunk.c[q - c0];
+            g[0] += a.x; g[1] += a.y; g[2] += a.z; g[3] += a.w;
+            g[4] += b.x; g[5] += b.y; g[6] += b.z; g[7] += b.w;
+            g[8] += d;
+        }
+        __syncthreads();
+    }
+}
+
+// The record range of a workgroup's slots: the first thread's run start and
+// the last slot's run end (runs are consecutive, so that is every run).
+__device__ __forceinline__ uint2 blockRunRange(uint32_t first, uint32_t n, bool lastSlot, uint32_t* sRange) {
+    if (threadIdx.x == 0) sRange[0] = first;
+    if (lastSlot) sRange[1] = first + n;
+    __syncthreads();
+    return make_uint2(sRange[0], sRange[1]);
+}
+
+// Every gradient but the harmonics' (those go through LDS, moveRows).
+__device__ __forceinline__ void storeShapeGrad(const SplatGrad& acc, uint32_t i, const DiffGrads& out) {
+    float* gMean = out.means + static_cast<uint64_t>(i) * 3;
+    float* gScale = out.logScales + static_cast<uint64_t>(i) * 3;
+    float* gQuat = out.quats + static_cast<uint64_t>(i) * 4;
+#pragma unroll
+    for (int k = 0; k < 3; ++k) {
+        gMean[k] = acc.mean[k];
+        gScale[k] = acc.scale[k];
+    }
+#pragma unroll
+    for (int k = 0; k < 4; ++k) gQuat[k] = acc.quat[k];
+    out.logits[i] = acc.logit;
 }
 
 __device__ __forceinline__ void storeGrad(const SplatGrad& acc, uint32_t i, uint32_t stride, const DiffGrads& out) {
@@ -801,21 +908,48 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
                                                                 const uint32_t* __restrict__ touched,
                                                                 const uint32_t* __restrict__ offsets,
                                                                 PairGrads pairGrad, DiffGrads out) {
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= f.count) return;
+    // The pair chunks, then the workgroup's harmonic rows, then their gradients.
+    __shared__ float sRows[kBlock * kRowPitchMax];
+    __shared__ uint32_t sRange[2];
+    static_assert(sizeof(PairChunk) <= sizeof(sRows), "the pair chunk fits the rows' LDS");
+    PairChunk& chunk = *reinterpret_cast<PairChunk*>(sRows);
+    const uint32_t blockFirst = blockIdx.x * kBlock;
+    const uint32_t i = blockFirst + threadIdx.x;
+    const bool live = i < f.count;
+    const uint32_t rows = f.count - blockFirst < kBlock ? f.count - blockFirst : kBlock;
     const uint32_t slot = f.base + i;
+    const uint32_t n = live ? touched[slot] : 0u;
+    const uint32_t first = live ? offsets[slot] : 0u;
+    const uint2 range = blockRunRange(first, n, threadIdx.x == rows - 1u, sRange);
+    float g[9];
+    slotGradients(pairGrad, range.x, range.y, first, n, chunk, g);
+    __syncthreads();   // the chunk is done with
+    const uint32_t stride = (1u + c.restPerColour) * 3u, pitch = rowPitch(stride);
+    moveRows<true>(c.sh + static_cast<uint64_t>(blockFirst) * stride, stride, rows, sRows);
+    __syncthreads();
     SplatGrad acc;
     zero(acc);
-    const uint32_t n = touched[slot];
     if (n > 0) {
-        float g[9];
-        slotGradient(pairGrad, offsets[slot], n, g);
         const Decoded dec = decode(c, i, f.useFilter3d != 0);
+        const float* row = sRows + threadIdx.x * pitch;
+        const uint32_t used = restCount(shDegree(f)) * 3u;
         float dc[3], coeff[45];
-        loadHarmonics(c, i, shDegree(f), dc, coeff);
+#pragma unroll
+        for (int ch = 0; ch < 3; ++ch) dc[ch] = row[ch];
+#pragma unroll
+        for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
         splatBackward(f, c, i, dec, dc, coeff, g, acc);
     }
-    storeGrad(acc, i, (1u + c.restPerColour) * 3u, out);
+    __syncthreads();   // every row is read
+    if (live) {
+        float* row = sRows + threadIdx.x * pitch;
+#pragma unroll
+        for (uint32_t k = 0; k < 48; ++k)
+            if (k < stride) row[k] = acc.sh[k];
+        storeShapeGrad(acc, i, out);
+    }
+    __syncthreads();
+    moveRows<false>(out.sh + static_cast<uint64_t>(blockFirst) * stride, stride, rows, sRows);
 }
 
 // A batch of views of one cloud: each splat decodes its parameters once and
@@ -826,24 +960,32 @@ __global__ __launch_bounds__(kBlock) void projectBackwardViewsKernel(const Frame
                                                                      const uint32_t* __restrict__ touched,
                                                                      const uint32_t* __restrict__ offsets,
                                                                      PairGrads pairGrad, DiffGrads out) {
+    __shared__ PairChunk chunk;
+    __shared__ uint32_t sRange[2];
     const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    if (i >= c.count) return;
+    const bool live = i < c.count;
+    const uint32_t lastLive = (blockIdx.x * kBlock + kBlock < c.count ? blockIdx.x * kBlock + kBlock : c.count) - 1u;
     const FrameParams& f0 = params[0];
-    const Decoded dec = decode(c, i, f0.useFilter3d != 0);
-    float dc[3], coeff[45];
-    loadHarmonics(c, i, shDegree(f0), dc, coeff);
+    Decoded dec{};
+    float dc[3] = {0.0f, 0.0f, 0.0f}, coeff[45];
+    if (live) {
+        dec = decode(c, i, f0.useFilter3d != 0);
+        loadHarmonics(c, i, shDegree(f0), dc, coeff);
+    }
     SplatGrad acc;
     zero(acc);
     for (uint32_t v = 0; v < views; ++v) {
         const FrameParams& f = params[v];
         const uint32_t slot = f.base + i;
-        const uint32_t n = touched[slot];
-        if (n == 0) continue;
+        const uint32_t n = live ? touched[slot] : 0u;
+        const uint32_t first = live ? offsets[slot] : 0u;
+        const uint2 range = blockRunRange(first, n, i == lastLive, sRange);
         float g[9];
-        slotGradient(pairGrad, offsets[slot], n, g);
-        splatBackward(f, c, i, dec, dc, coeff, g, acc);
+        slotGradients(pairGrad, range.x, range.y, first, n, chunk, g);
+        if (n > 0) splatBackward(f, c, i, dec, dc, coeff, g, acc);
+        __syncthreads();   // sRange is rewritten for the next view
     }
-    storeGrad(acc, i, (1u + c.restPerColour) * 3u, out);
+    if (live) storeGrad(acc, i, (1u + c.restPerColour) * 3u, out);
 }
 
 }   // namespace
