@@ -886,24 +886,6 @@ __device__ __forceinline__ void storeShapeGrad(const SplatGrad& acc, uint32_t i,
     out.logits[i] = acc.logit;
 }
 
-__device__ __forceinline__ void storeGrad(const SplatGrad& acc, uint32_t i, uint32_t stride, const DiffGrads& out) {
-    float* gMean = out.means + static_cast<uint64_t>(i) * 3;
-    float* gScale = out.logScales + static_cast<uint64_t>(i) * 3;
-    float* gQuat = out.quats + static_cast<uint64_t>(i) * 4;
-    float* gSh = out.sh + static_cast<uint64_t>(i) * stride;
-#pragma unroll
-    for (int k = 0; k < 3; ++k) {
-        gMean[k] = acc.mean[k];
-        gScale[k] = acc.scale[k];
-    }
-#pragma unroll
-    for (int k = 0; k < 4; ++k) gQuat[k] = acc.quat[k];
-    out.logits[i] = acc.logit;
-#pragma unroll
-    for (uint32_t k = 0; k < 48; ++k)
-        if (k < stride) gSh[k] = acc.sh[k];
-}
-
 __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, DiffCloud c,
                                                                 const uint32_t* __restrict__ touched,
                                                                 const uint32_t* __restrict__ offsets,
@@ -930,15 +912,10 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
     SplatGrad acc;
     zero(acc);
     if (n > 0) {
+        // The coefficients read from the thread's LDS row (the degree's bases only).
         const Decoded dec = decode(c, i, f.useFilter3d != 0);
         const float* row = sRows + threadIdx.x * pitch;
-        const uint32_t used = restCount(shDegree(f)) * 3u;
-        float dc[3], coeff[45];
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) dc[ch] = row[ch];
-#pragma unroll
-        for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
-        splatBackward(f, c, i, dec, dc, coeff, g, acc);
+        splatBackward(f, c, i, dec, row, row + 3, g, acc);
     }
     __syncthreads();   // every row is read
     if (live) {
@@ -955,37 +932,48 @@ __global__ __launch_bounds__(kBlock) void projectBackwardKernel(FrameParams f, D
 // A batch of views of one cloud: each splat decodes its parameters once and
 // adds its gradient in every view that drew it, in view order (no atomics).
 // View v's frame is params[v]; the splat's slot there is params[v].base + i.
+// The harmonic rows stay in LDS for every view (then hold their gradients), so
+// the coefficients take no registers across the views.
 __global__ __launch_bounds__(kBlock) void projectBackwardViewsKernel(const FrameParams* __restrict__ params,
                                                                      uint32_t views, DiffCloud c,
                                                                      const uint32_t* __restrict__ touched,
                                                                      const uint32_t* __restrict__ offsets,
                                                                      PairGrads pairGrad, DiffGrads out) {
+    __shared__ float sRows[kBlock * kRowPitchMax];
     __shared__ PairChunk chunk;
     __shared__ uint32_t sRange[2];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
+    const uint32_t blockFirst = blockIdx.x * kBlock;
+    const uint32_t i = blockFirst + threadIdx.x;
     const bool live = i < c.count;
-    const uint32_t lastLive = (blockIdx.x * kBlock + kBlock < c.count ? blockIdx.x * kBlock + kBlock : c.count) - 1u;
+    const uint32_t rows = c.count - blockFirst < kBlock ? c.count - blockFirst : kBlock;
     const FrameParams& f0 = params[0];
+    const uint32_t stride = (1u + c.restPerColour) * 3u, pitch = rowPitch(stride);
+    moveRows<true>(c.sh + static_cast<uint64_t>(blockFirst) * stride, stride, rows, sRows);
     Decoded dec{};
-    float dc[3] = {0.0f, 0.0f, 0.0f}, coeff[45];
-    if (live) {
-        dec = decode(c, i, f0.useFilter3d != 0);
-        loadHarmonics(c, i, shDegree(f0), dc, coeff);
-    }
+    if (live) dec = decode(c, i, f0.useFilter3d != 0);
+    float* row = sRows + threadIdx.x * pitch;
     SplatGrad acc;
     zero(acc);
+    __syncthreads();   // the rows are staged
     for (uint32_t v = 0; v < views; ++v) {
         const FrameParams& f = params[v];
         const uint32_t slot = f.base + i;
         const uint32_t n = live ? touched[slot] : 0u;
         const uint32_t first = live ? offsets[slot] : 0u;
-        const uint2 range = blockRunRange(first, n, i == lastLive, sRange);
+        const uint2 range = blockRunRange(first, n, threadIdx.x == rows - 1u, sRange);
         float g[9];
         slotGradients(pairGrad, range.x, range.y, first, n, chunk, g);
-        if (n > 0) splatBackward(f, c, i, dec, dc, coeff, g, acc);
+        if (n > 0) splatBackward(f, c, i, dec, row, row + 3, g, acc);
         __syncthreads();   // sRange is rewritten for the next view
     }
-    if (live) storeGrad(acc, i, (1u + c.restPerColour) * 3u, out);
+    if (live) {
+#pragma unroll
+        for (uint32_t k = 0; k < 48; ++k)
+            if (k < stride) row[k] = acc.sh[k];
+        storeShapeGrad(acc, i, out);
+    }
+    __syncthreads();
+    moveRows<false>(out.sh + static_cast<uint64_t>(blockFirst) * stride, stride, rows, sRows);
 }
 
 }   // namespace
