@@ -250,14 +250,57 @@ __device__ __forceinline__ float dppMoveRows(float v) {
 // mirrors, then row 0 into row 1 and row 2 into row 3 (row_bcast:15), then
 // rows 0-1 into rows 2-3 (row_bcast:31). Lane 63 reads only rows the earlier
 // steps wrote. Every lane must be active.
-__device__ __forceinline__ float waveSum(float v) {
+__device__ __forceinline__ float rowSum(float v) {
     v += dppMove<0xB1, 0xF>(v);       // quad_perm [1, 0, 3, 2]
     v += dppMove<0x4E, 0xF>(v);       // quad_perm [2, 3, 0, 1]
     v += dppMove<0x141, 0xF>(v);      // row_half_mirror
     v += dppMove<0x140, 0xF>(v);      // row_mirror
+    return v;                         // every lane holds its row's sum
+}
+
+__device__ __forceinline__ float waveSum(float v) {
+    v = rowSum(v);
     v += dppMoveRows<0x142, 0xA>(v);  // row_bcast:15 into rows 1 and 3
     v += dppMoveRows<0x143, 0xC>(v);  // row_bcast:31 into rows 2 and 3
     return v;
+}
+
+// Two terms' halves (lanes 0-31 and 32-63) summed at once: permlane32_swap
+// leaves a's low half and b's low half in one register and both high halves in
+// the other, so one add gives a's half sums in lanes 0-31 and b's in 32-63.
+__device__ __forceinline__ float halvesOfTwo(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane32_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// The same for rows: permlane16_swap leaves rows 0 and 2 of a and of b in one
+// register and rows 1 and 3 in the other, so one add gives row pairs summed:
+// rows 0-3 of the result hold a's rows 0+1, b's rows 0+1, a's rows 2+3 and
+// b's rows 2+3.
+__device__ __forceinline__ float rowsOfTwo(float a, float b) {
+    const auto r = __builtin_amdgcn_permlane16_swap(__float_as_uint(a), __float_as_uint(b), false, false);
+    return __uint_as_float(r[0]) + __uint_as_float(r[1]);
+}
+
+// A record's nine terms, each summed over the wave's 64 lanes, stored to
+// out[0..8] by the lanes that end with them. Terms 0-7 go in pairs through
+// halvesOfTwo, then the pairs' results in pairs through rowsOfTwo, which
+// leaves four terms a register, one a row (rows 0-3: terms 0, 2, 1, 3 of the
+// four), for the in-row steps; term 8 takes the whole-wave sum. 26
+// instructions where nine whole-wave sums take 54. Every lane must be active.
+__device__ __forceinline__ void waveSum9(const float v[9], float* out) {
+    const float w0 = halvesOfTwo(v[0], v[1]), w1 = halvesOfTwo(v[2], v[3]);
+    const float w2 = halvesOfTwo(v[4], v[5]), w3 = halvesOfTwo(v[6], v[7]);
+    const float x0 = rowSum(rowsOfTwo(w0, w1)), x1 = rowSum(rowsOfTwo(w2, w3));
+    const float s8 = waveSum(v[8]);
+    const uint32_t lane = __lane_id();
+    if ((lane & 15u) == 0u) {
+        const uint32_t row = lane >> 4;
+        const uint32_t at = ((row & 1u) << 1) | (row >> 1);
+        out[at] = x0;
+        out[4 + at] = x1;
+    }
+    if (lane == 63) out[8] = s8;
 }
 
 // sRGB decode and its derivative, for the blend's linearisation
@@ -461,38 +504,37 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
                 }
             }
             if (__ballot(ok) != 0) {
-#pragma unroll
-                for (int e = 0; e < 9; ++e) v[e] = waveSum(v[e]);
-            }
-            if (lane == 63) {
-                sPart[wave][k][0] = make_float4(v[0], v[1], v[2], v[3]);
-                sPart[wave][k][1] = make_float4(v[4], v[5], v[6], v[7]);
-                sPart[wave][k][2] = make_float4(v[8], 0.0f, 0.0f, 0.0f);
+                waveSum9(v, reinterpret_cast<float*>(&sPart[wave][k][0]));
+            } else if (lane == 63) {
+                sPart[wave][k][0] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                sPart[wave][k][1] = make_float4(0.0f, 0.0f, 0.0f, 0.0f);
+                sPart[wave][k][2].x = 0.0f;
             }
         }
         __syncthreads();
         // 4. the four quadrants' sums, one record per pair
         if (tid < n) {
             const uint32_t m = sMask[tid];
-            float4 acc[3] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f),
-                             make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+            float4 acc[2] = {make_float4(0.0f, 0.0f, 0.0f, 0.0f), make_float4(0.0f, 0.0f, 0.0f, 0.0f)};
+            float acc8 = 0.0f;
 #pragma unroll
             for (uint32_t q = 0; q < kWaves; ++q) {
                 if ((m >> q) & 1u) {
 #pragma unroll
-                    for (int r = 0; r < 3; ++r) {
+                    for (int r = 0; r < 2; ++r) {
                         const float4 p = sPart[q][tid][r];
                         acc[r].x += p.x;
                         acc[r].y += p.y;
                         acc[r].z += p.z;
                         acc[r].w += p.w;
                     }
+                    acc8 += sPart[q][tid][2].x;
                 }
             }
             const uint32_t at = pairIndex[top - 1u - tid];
             pairGrad.ab[2 * uint64_t{at}] = acc[0];
             pairGrad.ab[2 * uint64_t{at} + 1] = acc[1];
-            pairGrad.c[at] = acc[2].x;
+            pairGrad.c[at] = acc8;
         }
         __syncthreads();
         top = lo;
