@@ -135,38 +135,55 @@ __device__ __forceinline__ void moveRows(std::conditional_t<LOAD, const float*, 
 
 static_assert(kBlock == kProjectBlock, "one depth-range entry a workgroup of kProjectBlock slots");
 
+// The footprints first; then the workgroup's harmonic rows, half the
+// workgroup's at a time, staged through LDS with coalesced loads (a half with
+// no drawn splat loads nothing), and each drawn splat's colour from its row.
 __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffCloud c, float4* __restrict__ proj,
                                                             uint2* __restrict__ rects,
                                                             uint32_t* __restrict__ touchedOut, DepthBlocks depth) {
+    constexpr uint32_t kHalf = kBlock / 2;
+    __shared__ float sRows[kHalf * kRowPitchMax];
     __shared__ uint32_t sDepth[2 * (kBlock / 64)];
-    const uint32_t i = blockIdx.x * kBlock + threadIdx.x;
-    const bool live = i < f.count;   // every lane stays for the wave's tile count
+    const uint32_t blockFirst = blockIdx.x * kBlock;
+    const uint32_t i = blockFirst + threadIdx.x;
+    const bool live = i < f.count;   // every lane stays for the wave's tile count and the barriers
     const uint32_t slot = f.base + i;
     const Decoded d = decode(c, live ? i : 0u, f.useFilter3d != 0);
+    Footprint fp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {1u, 1u, 0u, 0u}};
+    const bool candidate = live && footprintOf(f, d.shape, fp);
+    const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
+                                                   fp.rect, candidate);
+    if (depth.entries != nullptr) waveDepthRange(touched > 0, fp.tz, sDepth);
     const uint32_t degree = shDegree(f);
-    const uint32_t stride = (1u + c.restPerColour) * 3u;
-    const float* row = c.sh + static_cast<uint64_t>(i) * stride;
-    const auto colourOf = [&](float dx, float dy, float dz, float colour[3]) {
-        float coeff[45];
-        const uint32_t used = restCount(degree) * 3u;
+    const uint32_t stride = (1u + c.restPerColour) * 3u, pitch = rowPitch(stride);
+    const uint32_t rows = f.count - blockFirst < kBlock ? f.count - blockFirst : kBlock;
+    const uint32_t half = threadIdx.x / kHalf;
+    float colour[3] = {0.0f, 0.0f, 0.0f};
 #pragma unroll
-        for (uint32_t k = 0; k < 45; ++k) coeff[k] = k < used ? row[3 + k] : 0.0f;
-        float rest[3];
-        evaluateRest(coeff, degree, dx, dy, dz, rest);
-#pragma unroll
-        for (int ch = 0; ch < 3; ++ch) colour[ch] = fmaxS((0.5f + kShC0 * row[ch]) + rest[ch], 0.0f);
-    };
-    const uint2 emptyRect = make_uint2(1u | (1u << 16), 0u);
-    uint2 rect = emptyRect;
-    float tz = 0.0f;
-    const uint32_t touched = projectWave(f, d.shape, live, colourOf, proj + static_cast<uint64_t>(slot) * 3, &rect, &tz);
-    if (depth.entries != nullptr) {
-        waveDepthRange(touched > 0, tz, sDepth);
+    for (uint32_t h = 0; h < 2; ++h) {
+        // A barrier: the last half's rows are read, and the depth words are written.
+        if (!__syncthreads_or(half == h && touched > 0)) continue;
+        const uint32_t first = h * kHalf;
+        moveRows<true>(c.sh + static_cast<uint64_t>(blockFirst + first) * stride, stride,
+                       rows - first < kHalf ? rows - first : kHalf, sRows);
         __syncthreads();
-        storeDepthRange(depth, sDepth);
+        if (half == h && touched > 0) {
+            const float* row = sRows + (threadIdx.x - first) * pitch;
+            float dx, dy, dz;
+            viewDirection(f, d.shape, dx, dy, dz);
+            float rest[3];
+            evaluateRest(row + 3, degree, dx, dy, dz, rest);
+#pragma unroll
+            for (int ch = 0; ch < 3; ++ch) colour[ch] = fmaxS((0.5f + kShC0 * row[ch]) + rest[ch], 0.0f);
+        }
     }
+    if (depth.entries != nullptr) storeDepthRange(depth, sDepth);
     if (!live) return;
-    rects[slot] = touched > 0 ? rect : emptyRect;
+    if (touched > 0) {
+        writeProjection(fp, colour, proj + static_cast<uint64_t>(slot) * 3, rects + slot);
+    } else {
+        rects[slot] = make_uint2(1u | (1u << 16), 0u);
+    }
     touchedOut[slot] = touched;
 }
 
