@@ -64,6 +64,7 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
                                              DepthBlocks depth) {
     __shared__ uint32_t sSh[kBlock * kMaxShWords];
     __shared__ uint32_t sRowOf[kBlock];
+    __shared__ uint8_t sGraded[kBlock];
     __shared__ uint32_t sDrawn[kBlock / 64];
     __shared__ uint32_t sDepth[2 * (kBlock / 64)];
     const uint32_t tid = threadIdx.x;
@@ -114,10 +115,43 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
         if (w < wave) at += sDrawn[w];
         rows += sDrawn[w];
     }
-    if (drawn) sRowOf[at] = tid;
+    if (drawn) {
+        sRowOf[at] = tid;
+        sGraded[at] = fate.graded ? 1u : 0u;
+    }
     __syncthreads();
     stageDrawnRows(sh + static_cast<uint64_t>(first) * words, 0u, words, rows, sRowOf, sSh);
     __syncthreads();
+
+    // The colours: thread r takes the workgroup's r-th drawn splat (harmonic
+    // row r), so the harmonics run on full waves however many splats were
+    // culled, and writes the record's colour; the splat's own thread writes
+    // the rest of the record.
+    if (tid < rows) {
+        const uint32_t j = first + sRowOf[tid];
+        const float4 p = positions[j];
+        const uint4 own = shape[j];
+        // Unpacked with constant indices so the 45 coefficients stay in registers.
+        const uint32_t* shRow = sSh + tid * words;
+        float coeff[45];
+        const uint32_t halves = degree == 0 ? 0u : (degree == 1 ? 9u : degree == 2 ? 24u : 45u);
+#pragma unroll
+        for (uint32_t k = 0; k < 45; ++k) {
+            const uint32_t wd = k < halves ? shRow[k >> 1] : 0u;
+            coeff[k] = (k & 1u) == 0u ? lowHalf(wd) : highHalf(wd);
+        }
+        const SplatShape centre{p.x, p.y, p.z, 0.0f, 0.0f, 0.0f, 0.0f, {0.0f, 0.0f, 0.0f, 1.0f}};
+        float dx, dy, dz;
+        viewDirection(f, centre, dx, dy, dz);
+        float rest[3];
+        evaluateRest(coeff, degree, dx, dy, dz, rest);
+        float colour[3];
+        colour[0] = fmaxS(highHalf(own.z) + rest[0], 0.0f);
+        colour[1] = fmaxS(lowHalf(own.w) + rest[1], 0.0f);
+        colour[2] = fmaxS(highHalf(own.w) + rest[2], 0.0f);
+        if (sGraded[tid] != 0) editGrade(f.edit, colour);
+        proj[static_cast<uint64_t>(f.base + j) * 3 + 2] = make_float4(colour[0], colour[1], colour[2], 0.0f);
+    }
 
     if (!live) return;
     if (!drawn) {
@@ -125,25 +159,7 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
         rects[slot] = emptyRect;
         return;
     }
-    // Unpacked with constant indices so the 45 coefficients stay in registers.
-    const uint32_t* shRow = sSh + at * words;
-    float coeff[45];
-    const uint32_t halves = degree == 0 ? 0u : (degree == 1 ? 9u : degree == 2 ? 24u : 45u);
-#pragma unroll
-    for (uint32_t k = 0; k < 45; ++k) {
-        const uint32_t wd = k < halves ? shRow[k >> 1] : 0u;
-        coeff[k] = (k & 1u) == 0u ? lowHalf(wd) : highHalf(wd);
-    }
-    float dx, dy, dz;
-    viewDirection(f, shapeOf, dx, dy, dz);
-    float rest[3];
-    evaluateRest(coeff, degree, dx, dy, dz, rest);
-    float colour[3];
-    colour[0] = fmaxS(highHalf(row.z) + rest[0], 0.0f);
-    colour[1] = fmaxS(lowHalf(row.w) + rest[1], 0.0f);
-    colour[2] = fmaxS(highHalf(row.w) + rest[2], 0.0f);
-    if (fate.graded) editGrade(f.edit, colour);
-    writeProjection(fp, colour, proj + static_cast<uint64_t>(slot) * 3, rects + slot);
+    writeFootprint(fp, proj + static_cast<uint64_t>(slot) * 3, rects + slot);
     touchedOut[slot] = touched;
 }
 

@@ -142,13 +142,20 @@ __device__ __forceinline__ void viewDirection(const FrameParams& f, const SplatS
     dz = dzo / len;
 }
 
+// A drawn splat's record but its colour (float4s 0 and 1 of the three) and its
+// packed tile rectangle.
+__device__ __forceinline__ void writeFootprint(const Footprint& fp, float4* __restrict__ out,
+                                               uint2* __restrict__ rectOut) {
+    out[0] = make_float4(fp.cx, fp.cy, fp.tz, fp.cutoff);
+    out[1] = make_float4(fp.ca, fp.cb, fp.cc, fp.alpha);
+    *rectOut = make_uint2(fp.rect[0] | (fp.rect[1] << 16), fp.rect[2] | (fp.rect[3] << 16));
+}
+
 // The three float4 of a drawn splat's record and its packed tile rectangle.
 __device__ __forceinline__ void writeProjection(const Footprint& fp, const float colour[3], float4* __restrict__ out,
                                                 uint2* __restrict__ rectOut) {
-    out[0] = make_float4(fp.cx, fp.cy, fp.tz, fp.cutoff);
-    out[1] = make_float4(fp.ca, fp.cb, fp.cc, fp.alpha);
+    writeFootprint(fp, out, rectOut);
     out[2] = make_float4(colour[0], colour[1], colour[2], 0.0f);
-    *rectOut = make_uint2(fp.rect[0] | (fp.rect[1] << 16), fp.rect[2] | (fp.rect[3] << 16));
 }
 
 // Projects one splat a lane, `want` false for a lane with none; returns the
