@@ -82,14 +82,15 @@ __device__ __forceinline__ float logF(float x) {
     return ef * 0.693145751953125f + (ef * 1.428606765330187e-06f + lm);
 }
 
-// fp16 -> fp32, exact.
+// fp16 -> fp32, exact, a NaN keeping its payload bits (sign, all-ones
+// exponent, mantissa << 13). The hardware conversion (v_cvt_f32_f16) gives
+// exactly that for every half but a signalling NaN, which it quiets (sets
+// bit 22); for a NaN, bit 22 is put back from the half's top mantissa bit.
+// Branch-free.
 __device__ __forceinline__ float halfToFloat(uint32_t h) {
-    const uint32_t sign = (h & 0x8000u) << 16;
-    const uint32_t e = (h >> 10) & 31u;
-    const uint32_t m = h & 1023u;
-    if (e == 0) return bitsToFloat(floatToBits(static_cast<float>(m) * 5.9604644775390625e-08f) | sign);
-    if (e == 31) return bitsToFloat(sign | 0x7F800000u | (m << 13));
-    return bitsToFloat(sign | ((e + 112u) << 23) | (m << 13));
+    const float hw = static_cast<float>(__builtin_bit_cast(_Float16, static_cast<uint16_t>(h)));
+    const uint32_t payload = (floatToBits(hw) & ~0x00400000u) | ((h & 0x200u) << 13);
+    return hw != hw ? bitsToFloat(payload) : hw;
 }
 __device__ __forceinline__ float lowHalf(uint32_t w) { return halfToFloat(w & 0xFFFFu); }
 __device__ __forceinline__ float highHalf(uint32_t w) { return halfToFloat(w >> 16); }
