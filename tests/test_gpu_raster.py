@@ -323,3 +323,30 @@ def test_walk_steps_are_counted_by_the_same_blend(rast):
     assert 0 < int(steps[0]) <= bound
     assert int(rast.debug(_abi.DEBUG_WALK_STEPS)[0]) == int(steps[0])
     assert bool((t.colour == before).all())
+
+
+def test_every_half_pattern_decodes_as_the_oracle_does(rast):
+    """All 65,536 fp16 patterns (zeros, subnormals, infinities, quiet and
+    signalling NaNs among them) as the 3,000-splat scene's harmonic
+    coefficients, and a cycle of special halves as its base colours: the
+    records, counts, pairs and ranges agree bit for bit with the oracle, whose
+    decode is the exact bit-level one (the GPU converts in hardware and puts a
+    NaN's quiet bit back from its payload)."""
+    import dataclasses
+    cloud = F.random_cloud(3000, 7).decode(15)   # degree 3: 23 words a splat, 138,000 halves
+    halves = np.resize(np.arange(65536, dtype=np.uint32), cloud.sh.size * 2)
+    sh = (halves[0::2] | (halves[1::2] << np.uint32(16))).reshape(cloud.sh.shape).astype(np.uint32)
+    special = np.array([0x0000, 0x8000, 0x0001, 0x03FF, 0x8001, 0x7BFF, 0x7C00, 0xFC00, 0x7C01, 0x7E00, 0xFDFF,
+                        0x3C00, 0x3555], np.uint32)
+    shape = cloud.shape.copy()
+    n = shape.shape[0]
+    pick = special[np.arange(3 * n) % special.size].reshape(n, 3)
+    shape[:, 2] = (shape[:, 2] & np.uint32(0xFFFF)) | (pick[:, 0] << np.uint32(16))   # z log-scale kept
+    shape[:, 3] = pick[:, 1] | (pick[:, 2] << np.uint32(16))
+    scene = dataclasses.replace(cloud, shape=shape, sh=sh)
+    cam, _ = CAMERAS["perspective"]
+    s = raster.RenderSettings(width=W, height=HGT)
+    inst = [(scene, np.eye(4), raster.SplatEdit())]
+    gpu = H.run_gpu(cam, s, inst, rast)
+    ora = H.run_oracle(cam, s, inst)
+    H.assert_bit_exact(gpu, ora)
