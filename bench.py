@@ -620,6 +620,10 @@ def main():
     ap.add_argument("--profile-frames", type=int, default=0,
                     help="only draw this many 3M-splat 1080p frames and one 256 MiB device copy (the counter "
                          "calibration), for rocprofv3 --pmc; prints no JSON")
+    ap.add_argument("--profile-stress", type=int, default=0,
+                    help="only draw this many sort-stress frames (6M splats, 2160p), for rocprofv3; prints no JSON")
+    ap.add_argument("--profile-fwd-bwd", type=int, default=0,
+                    help="only run this many fwd+bwd views (1M splats, 720p), for rocprofv3; prints no JSON")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py measures one GPU: this build runs no multi-GPU path (asked for --gpus {args.gpus}); "
@@ -631,6 +635,21 @@ def main():
 
     from lucabrtrender_amd import _abi, raster, scenes
 
+    if args.profile_stress > 0:
+        cfg = scenes.workload("sort-stress-6M-2160p")
+        inst = [raster.SplatInstance(cfg.scene.to_device())]
+        settings = raster.RenderSettings(width=cfg.width, height=cfg.height)
+        targets = raster.RenderTargets(cfg.width, cfg.height)
+        rast = raster.TileRasterizer()
+        for _ in range(args.profile_stress):
+            stats = rast.render(cfg.camera, inst, settings, targets)
+        torch.cuda.synchronize()
+        log(f"{args.profile_stress} sort-stress frames drawn ({stats.pairs} pairs, {stats.sort_passes} passes)")
+        return
+    if args.profile_fwd_bwd > 0:
+        fwd_bwd_leg(torch, raster, scenes, args.profile_fwd_bwd)
+        log(f"{args.profile_fwd_bwd} fwd+bwd views run")
+        return
     if args.profile_frames > 0:
         cfg = scenes.workload("scene-fwd-3M-1080p", count=args.count)
         settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
