@@ -152,6 +152,10 @@ FrameParams frameParamsFor(const Projection& p, const SplatInstance& instance, c
     f.centreY = static_cast<float>(p.centreY);
     f.nearZ = static_cast<float>(p.nearZ);
     f.farZ = static_cast<float>(p.farZ);
+    f.limLoX = (-0.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
+    f.limHiX = (1.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
+    f.limLoY = (-0.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
+    f.limHiY = (1.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
     f.orthographic = p.orthographic ? 1u : 0u;
     f.antialias = s.antialias ? 1u : 0u;
     const Mat4 objectToView = p.worldToView * instance.objectToWorld;

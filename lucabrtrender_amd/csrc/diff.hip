@@ -706,10 +706,7 @@ __device__ __forceinline__ void splatBackward(const FrameParams& f, const DiffCl
         jx0 = f.focalX; jx2 = 0.0f;
         jy1 = f.focalY; jy2 = 0.0f;
     } else {
-        const float limLoX = (-0.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
-        const float limHiX = (1.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
-        const float limLoY = (-0.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
-        const float limHiY = (1.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
+        const float limLoX = f.limLoX, limHiX = f.limHiX, limLoY = f.limLoY, limHiY = f.limHiY;
         const float ux = tx * invZ, uy = ty * invZ;
         clampX = ux < limLoX || ux > limHiX;
         clampY = uy < limLoY || uy > limHiY;

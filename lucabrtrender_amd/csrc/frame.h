@@ -25,6 +25,10 @@ struct SplatEditParams {   // shaders/lrt/common/edit.slang:23-36
 struct FrameParams {
     uint32_t width, height, tilesX, tilesY;
     float focalX, focalY, centreX, centreY, nearZ, farZ;
+    // Where the Jacobian is linearised at most: 1.3x the frustum, in units of
+    // view x / z and y / z (set with the lens; the same float operations the
+    // preprocess would make a splat).
+    float limLoX, limHiX, limLoY, limHiY;
     uint32_t orthographic, antialias;
     float m[12];          // object -> view, rows of a 3x4
     float eye[3];         // the eye in the cloud's own space

@@ -80,12 +80,8 @@ __device__ __forceinline__ bool footprintOf(const FrameParams& f, const SplatSha
         cx = f.focalX * tx * invZ + f.centreX;
         cy = f.focalY * ty * invZ + f.centreY;
         // The Jacobian is linearised at a point clamped to 1.3x the frustum.
-        const float limLoX = (-0.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
-        const float limHiX = (1.15f * static_cast<float>(f.width) - f.centreX) / f.focalX;
-        const float limLoY = (-0.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
-        const float limHiY = (1.15f * static_cast<float>(f.height) - f.centreY) / f.focalY;
-        const float ctx = clampS(tx * invZ, limLoX, limHiX) * tz;
-        const float cty = clampS(ty * invZ, limLoY, limHiY) * tz;
+        const float ctx = clampS(tx * invZ, f.limLoX, f.limHiX) * tz;
+        const float cty = clampS(ty * invZ, f.limLoY, f.limHiY) * tz;
         jx0 = f.focalX * invZ;
         jx2 = -f.focalX * ctx * invZ * invZ;
         jy1 = f.focalY * invZ;

@@ -95,6 +95,16 @@ __device__ __forceinline__ float halfToFloat(uint32_t h) {
 __device__ __forceinline__ float lowHalf(uint32_t w) { return halfToFloat(w & 0xFFFFu); }
 __device__ __forceinline__ float highHalf(uint32_t w) { return halfToFloat(w >> 16); }
 
+// s / 1023 correctly rounded, for an integer s in [0, 1023]: the product with
+// the rounded reciprocal, corrected once by FMA, equals the division for each
+// of the 1,024 values (checked exhaustively in exact arithmetic). Three
+// instructions instead of a division's ten.
+__device__ __forceinline__ float div1023(float s) {
+    constexpr float kR = 1.0f / 1023.0f;
+    const float q = s * kR;
+    return __builtin_fmaf(__builtin_fmaf(-q, 1023.0f, s), kR, q);
+}
+
 // Smallest-three quaternion (reference packing.slang:52-69): three 10-bit
 // components in [-sqrt(1/2), sqrt(1/2)], the largest from unit length.
 __device__ __forceinline__ void decodeQuaternion(uint32_t word, float q[4]) {
@@ -103,9 +113,9 @@ __device__ __forceinline__ void decodeQuaternion(uint32_t word, float q[4]) {
     float s0 = static_cast<float>((word >> 22) & 1023u);
     float s1 = static_cast<float>((word >> 12) & 1023u);
     float s2 = static_cast<float>((word >> 2) & 1023u);
-    s0 = (s0 / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
-    s1 = (s1 / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
-    s2 = (s2 / 1023.0f * 2.0f - 1.0f) * kSqrtHalf;
+    s0 = (div1023(s0) * 2.0f - 1.0f) * kSqrtHalf;
+    s1 = (div1023(s1) * 2.0f - 1.0f) * kSqrtHalf;
+    s2 = (div1023(s2) * 2.0f - 1.0f) * kSqrtHalf;
     const float rest = __builtin_sqrtf(fmaxS(0.0f, 1.0f - ((s0 * s0 + s1 * s1) + s2 * s2)));
     // Place the three stored components around the largest one.
     q[0] = largest == 0 ? rest : s0;

@@ -492,6 +492,10 @@ void or_frame_for(const or_projection* p, const or_settings* s, const double obj
     f->centre_y = (float)p->centre_y;
     f->near_z = (float)p->near_z;
     f->far_z = (float)p->far_z;
+    f->lim_lo_x = (-0.15f * (float)f->width - f->centre_x) / f->focal_x;
+    f->lim_hi_x = (1.15f * (float)f->width - f->centre_x) / f->focal_x;
+    f->lim_lo_y = (-0.15f * (float)f->height - f->centre_y) / f->focal_y;
+    f->lim_hi_y = (1.15f * (float)f->height - f->centre_y) / f->focal_y;
     f->orthographic = p->orthographic ? 1u : 0u;
     f->antialias = s->antialias ? 1u : 0u;
     double ov[16], inv[16];

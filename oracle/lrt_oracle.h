@@ -110,6 +110,10 @@ typedef struct {
 typedef struct {
     uint32_t width, height, tiles_x, tiles_y;
     float focal_x, focal_y, centre_x, centre_y, near_z, far_z;
+    /* The Jacobian's clamp, 1.3x the frustum in units of x/z and y/z: the
+     * expressions the projection evaluates (frame.slang's splatProject
+     * clamp), kept here so the frame block matches the library's byte for byte. */
+    float lim_lo_x, lim_hi_x, lim_lo_y, lim_hi_y;
     uint32_t orthographic, antialias;
     float m[12];
     float eye[3];

@@ -34,7 +34,8 @@ class OrEdit(C.Structure):
 class OrFrame(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32),
                 ("focal_x", C.c_float), ("focal_y", C.c_float), ("centre_x", C.c_float), ("centre_y", C.c_float),
-                ("near_z", C.c_float), ("far_z", C.c_float), ("orthographic", C.c_uint32),
+                ("near_z", C.c_float), ("far_z", C.c_float), ("lim_lo_x", C.c_float), ("lim_hi_x", C.c_float),
+                ("lim_lo_y", C.c_float), ("lim_hi_y", C.c_float), ("orthographic", C.c_uint32),
                 ("antialias", C.c_uint32), ("m", C.c_float * 12), ("eye", C.c_float * 3), ("count", C.c_uint32),
                 ("base", C.c_uint32), ("rest_per_colour", C.c_uint32), ("sh_words", C.c_uint32),
                 ("sh_limit", C.c_uint32), ("tile_bits", C.c_uint32), ("depth_mode", C.c_uint32),
