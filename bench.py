@@ -390,7 +390,7 @@ def _batch_against_sequence(torch, raster, rast, views, inst, settings, reps):
             "speedup": round(seq_s / batch_s, 3),
             "stage_ms_eager": {name: round(float(acc[k] / reps), 3) for k, name in enumerate(STAGES)},
             "tuned": {"blend_batch": tuned.blend_batch, "persistent_blend": tuned.persistent_blend,
-                      "use_graph": tuned.use_graph}}
+                      "use_graph": tuned.use_graph, "tile_size": tuned.tile_size}}
 
 
 def views_leg(torch, raster, scenes, reps):
@@ -629,6 +629,7 @@ def main():
     ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
     ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 128 or 256")
+    ap.add_argument("--tile-size", type=int, default=16, help="pixels a tile's side: 8, 16 (the reference's) or 32")
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="fwd+bwd views (1M splats, 720p) timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="sort-stress frames (6M splats, 2160p) timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="64-view batches (2M splats) timed; 0 skips it")
@@ -691,7 +692,7 @@ def main():
     cfg = scenes.workload("scene-fwd-3M-1080p", count=args.count)
     log(f"scene: {cfg.scene.count} splats generated in {time.perf_counter() - t0:.1f} s")
     settings = raster.RenderSettings(width=cfg.width, height=cfg.height, dilation=cfg.dilation,
-                                     filter3d=cfg.filter3d, blend_batch=args.blend_batch)
+                                     filter3d=cfg.filter3d, blend_batch=args.blend_batch, tile_size=args.tile_size)
     dev = cfg.scene.to_device()
     inst = [raster.SplatInstance(dev)]
     targets = raster.RenderTargets(cfg.width, cfg.height)
@@ -755,6 +756,26 @@ def main():
             rast.render(proj, inst, vs, targets)
             acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
         variants[f"segment{segment}"] = round(float(acc_v[6] / 10), 4)
+    # The frame at each tile size: the same images, different pair counts.
+    tile_sizes = {}
+    for tile in (8, 16, 32):
+        vs = raster.RenderSettings(**{**settings.__dict__, "tile_size": tile})
+        for _ in range(3):
+            rast.render(proj, inst, vs, targets)
+        torch.cuda.synchronize()
+        t1 = time.perf_counter()
+        for _ in range(30):
+            st = rast.render(proj, inst, vs, targets)
+        torch.cuda.synchronize()
+        tfps = 30 / (time.perf_counter() - t1)
+        acc_v = np.zeros(8)
+        for _ in range(10):
+            rast.render(proj, inst, raster.RenderSettings(**{**vs.__dict__, "time_stages": True}), targets)
+            acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
+        tile_sizes[str(tile)] = {"frames_per_s": round(tfps, 1), "pairs": int(st.pairs),
+                                 "stage_ms": {name: round(float(acc_v[k] / 10), 4) for k, name in enumerate(STAGES)}}
+    log("tile sizes: " + ", ".join(f"{t}: {v['frames_per_s']} frames/s, {v['pairs']} pairs"
+                                   for t, v in tile_sizes.items()))
 
     fwd_bwd = None
     if args.fwd_bwd_steps > 0:
@@ -812,10 +833,12 @@ def main():
         "data": "synthetic: seeded 3M-splat clustered scene, packed fp16 shape/harmonics",
         "config": {"workload": "scene-fwd-3M-1080p", "splats": n, "visible": visible, "pairs": pairs,
                    "width": cfg.width, "height": cfg.height, "tiles": tiles, "sh_degree": 3, "filter3d": True,
-                   "dilation": cfg.dilation, "sort_passes": passes, "key_bits": timed_stats.key_bits},
+                   "dilation": cfg.dilation, "sort_passes": passes, "key_bits": timed_stats.key_bits,
+                   "tile_size": settings.tile_size},
         "stages": stages,
         "gpu_ms_per_frame": round(float(kernel_ms[7]), 4),
         "blend_ms_by_setting": variants,
+        "tile_sizes": tile_sizes,
         "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3) if sort_ms > 0 else None,
         "copy_GB/s": round(copy_gbs, 1),
         "fwd_bwd": fwd_bwd,

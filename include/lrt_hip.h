@@ -144,6 +144,8 @@ typedef struct {
     int32_t keep_walk_state;      /* 1: keep each pixel's last contributor and final transmittance for
                                      LRT_DEBUG_LAST_COUNT / LRT_DEBUG_FINAL_T (a differentiable frame
                                      always keeps them; otherwise they cost the blend an instruction a pair) */
+    uint32_t tile_size;           /* pixels a square tile's side: 8, 16 or 32 (0, the default: 16, the
+                                     reference's frame.slang:31). Differentiable frames take 16 only */
 } lrt_render_settings;
 
 /* render::RenderTargets (TileRasterizer.h:129-134): device images, bottom row
@@ -361,10 +363,10 @@ lrt_status lrt_raster_render_views(lrt_context* ctx, const lrt_view* views, uint
                                    const lrt_render_settings* settings, lrt_frame_stats* stats);
 
 /* The autotuner: times the batch under each launch choice this build offers
- * (list length 128/256, grid or persistent blend, graph replay or not) and
- * writes the fastest into *settings (its other fields are kept). ms (host, may
- * be NULL) receives the winner's batch time. Tile size (16) and radix width (8
- * bits) are fixed in this build. */
+ * (tile size 8/16/32, list length 128/256, grid or persistent blend, graph
+ * replay or not) and writes the fastest into *settings (its other fields are
+ * kept). ms (host, may be NULL) receives the winner's batch time. The radix
+ * width (8 bits) is fixed in this build. */
 lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
                               const lrt_splat_instance* instances, uint32_t instance_count,
                               lrt_render_settings* settings, float* ms);

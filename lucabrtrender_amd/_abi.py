@@ -98,7 +98,7 @@ class RenderSettingsC(C.Structure):
         ("linearise", C.c_int32), ("time_stages", C.c_int32),
         ("dilation", C.c_float), ("filter3d", C.c_int32), ("blend_batch", C.c_uint32), ("tile_order", C.c_int32),
         ("persistent_blend", C.c_int32), ("use_graph", C.c_int32), ("blend_segment", C.c_uint32),
-        ("keep_walk_state", C.c_int32),
+        ("keep_walk_state", C.c_int32), ("tile_size", C.c_uint32),
     ]
 
 

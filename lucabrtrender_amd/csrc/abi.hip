@@ -75,6 +75,7 @@ lrt::RenderSettings toSettings(const lrt_render_settings& s) {
     out.heaviestTilesFirst = s.tile_order == 1;
     out.persistentBlend = s.persistent_blend != 0;
     out.useGraph = s.use_graph != 0;
+    out.tileSize = s.tile_size == 0 ? 16u : s.tile_size;
     return out;
 }
 
@@ -288,6 +289,7 @@ void lrt_render_settings_default(lrt_render_settings* out) {
     out->tile_order = 1;
     out->persistent_blend = 0;
     out->use_graph = 0;
+    out->tile_size = s.tileSize;
 }
 
 void lrt_splat_edit_default(lrt_splat_edit* out) {
@@ -506,6 +508,7 @@ lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t 
     settings->blend_batch = best.blendBatch;
     settings->persistent_blend = best.persistentBlend ? 1 : 0;
     settings->use_graph = best.useGraph ? 1 : 0;
+    settings->tile_size = best.tileSize;
     if (ms != nullptr) *ms = bestMs;
     return finish(ctx, s);
 }

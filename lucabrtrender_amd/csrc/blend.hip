@@ -170,6 +170,17 @@ struct WalkState {
 // kResume starts from *state (its active lanes) and leaves the state there.
 constexpr uint32_t kFinish = 0, kPartial = 1, kResume = 2;
 
+// A tile of side 2^tileShift holds 2^quadBits 8x8 quadrants (1, 4 or 16),
+// numbered row-major inside it.
+__device__ __forceinline__ uint32_t quadBits(uint32_t tileShift) { return 2u * (tileShift - 3u); }
+
+// The first pixel of quadrant `quadrant` of tile `tile` of the frame f.
+__device__ __forceinline__ uint2 quadrantOrigin(const FrameParams& f, uint32_t tile, uint32_t quadrant) {
+    const uint32_t side = f.tileShift - 3u;
+    return make_uint2(((tile % f.tilesX) << f.tileShift) + ((quadrant & ((1u << side) - 1u)) << 3),
+                      ((tile / f.tilesX) << f.tileShift) + ((quadrant >> side) << 3));
+}
+
 // Quadrant `quadrant` of tile `tile` of the frame f, walking the pairs
 // [range.x, range.y) of pairSlots (all of the tile's, or a segment of them;
 // places count from tileStart); colour and depth are the frame's images,
@@ -194,8 +205,8 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
     float4* const sColour = list.colour;
 
     const uint32_t lane = threadIdx.x;
-    const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
-    const uint32_t y0 = (tile / f.tilesX) * kTile + (quadrant >> 1) * 8u;
+    const uint2 origin = quadrantOrigin(f, tile, quadrant);
+    const uint32_t x0 = origin.x, y0 = origin.y;
     if (x0 >= f.width || y0 >= f.height) return;   // a quadrant past the frame's edge has no pixels
     const uint32_t x1 = x0 + 8u < f.width ? x0 + 8u : f.width;
     const uint32_t y1 = y0 + 8u < f.height ? y0 + 8u : f.height;
@@ -398,8 +409,9 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
                                                       float* __restrict__ finalT, float4* __restrict__ accum,
                                                       uint32_t* __restrict__ walkSteps) {
     const uint32_t tiles = f.tilesX * f.tilesY;
-    const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
-    const uint32_t place = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
+    const uint32_t qBits = quadBits(f.tileShift);
+    const uint32_t quadrant = (blockIdx.x >> 3) & ((1u << qBits) - 1u);
+    const uint32_t place = (blockIdx.x >> (3u + qBits)) * 8u + (blockIdx.x & 7u);
     if (place >= tiles) return;
     const uint32_t tile = order != nullptr ? order[place] : place;
     if (tile >= tiles) return;   // an order entry that is not a tile: draw nothing
@@ -410,15 +422,17 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
                                                                finalT, accum, nullptr, nullptr, nullptr, walkSteps);
 }
 
-// Every quadrant of every view of a batch. As a grid (PERSISTENT false) block b
-// is task b, the four quadrants of a tile going to blocks b, b + 8, b + 16,
-// b + 24 of one XCD; persistent, each block takes tasks from a shared counter
-// until none are left (task t: tile t / 4, quadrant t % 4), so a block that
-// drew a light quadrant takes the next instead of leaving its slot idle.
+// Every quadrant of every view of a batch (2^qBits quadrants a tile). As a
+// grid (PERSISTENT false) block b is task b, the quadrants of a tile going to
+// blocks b, b + 8, b + 16, ... of one XCD; persistent, each block takes tasks
+// from a shared counter until none are left (task t: tile t >> qBits,
+// quadrant t & (2^qBits - 1)), so a block that drew a light quadrant takes the
+// next instead of leaving its slot idle.
 template <uint32_t DEPTH_MODE, uint32_t LIST, bool PERSISTENT, bool TRACK>
 __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __restrict__ params,
                                                            const ViewInfo* __restrict__ views, uint32_t viewCount,
-                                                           uint32_t tiles, uint32_t tasks, uint32_t slots,
+                                                           uint32_t tiles, uint32_t qBits, uint32_t tasks,
+                                                           uint32_t slots,
                                                            const uint2* __restrict__ ranges,
                                                            const uint32_t* __restrict__ pairSlots,
                                                            const float4* __restrict__ proj,
@@ -433,13 +447,13 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
             if (threadIdx.x == 0) taken = atomicAdd(queue, 1u);
             task = static_cast<uint32_t>(__shfl(static_cast<int>(taken), 0, 64));
             if (task >= tasks) return;
-            g = task >> 2;
-            quadrant = task & 3u;
+            g = task >> qBits;
+            quadrant = task & ((1u << qBits) - 1u);
             if (round > 0) __syncthreads();   // the last task's LDS list is done with
         } else {
             task = blockIdx.x;
-            g = (task >> 5) * 8u + (task & 7u);
-            quadrant = (task >> 3) & 3u;
+            g = (task >> (3u + qBits)) * 8u + (task & 7u);
+            quadrant = (task >> 3) & ((1u << qBits) - 1u);
         }
         if (g < tiles) {
             const ViewInfo& v = views[viewOfTile(views, viewCount, g)];
@@ -629,8 +643,8 @@ __global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2*
     }
 }
 
-// The tasks of a split plan: task k's four quadrants are blocks b, b + 8,
-// b + 16, b + 24 of one XCD. A task of an unsplit tile finishes its pixels; a
+// The tasks of a split plan: task k's quadrants are blocks b, b + 8, b + 16,
+// ... of one XCD. A task of an unsplit tile finishes its pixels; a
 // segment leaves its partial walk in its slot.
 template <uint32_t DEPTH_MODE, uint32_t LIST>
 __global__ __launch_bounds__(kLanes) void blendTasksKernel(FrameParams f, uint32_t slots, const uint4* __restrict__ tasks,
@@ -642,8 +656,9 @@ __global__ __launch_bounds__(kLanes) void blendTasksKernel(FrameParams f, uint32
                                                            uint32_t* __restrict__ lastCount,
                                                            float* __restrict__ finalT, float4* __restrict__ partialA,
                                                            float2* __restrict__ partialB) {
-    const uint32_t quadrant = (blockIdx.x >> 3) & 3u;
-    const uint32_t task = (blockIdx.x >> 5) * 8u + (blockIdx.x & 7u);
+    const uint32_t qBits = quadBits(f.tileShift);
+    const uint32_t quadrant = (blockIdx.x >> 3) & ((1u << qBits) - 1u);
+    const uint32_t task = (blockIdx.x >> (3u + qBits)) * 8u + (blockIdx.x & 7u);
     if (task >= planCounts[0]) return;
     __shared__ BlendList<LIST> list;
     const uint4 tk = tasks[task];
@@ -653,7 +668,7 @@ __global__ __launch_bounds__(kLanes) void blendTasksKernel(FrameParams f, uint32
                                                         pairSlots, proj, nullptr, nullptr, colour, depth, lastCount,
                                                         finalT, nullptr);
     } else {
-        const uint64_t at = (static_cast<uint64_t>(tk.w) * 4u + quadrant) * kLanes;
+        const uint64_t at = ((static_cast<uint64_t>(tk.w) << qBits) + quadrant) * kLanes;
         blendQuadrant<DEPTH_MODE, LIST, false, kPartial>(list, f, tk.x, quadrant, make_uint2(tk.y, tk.z), tileStart, slots,
                                                          pairSlots, proj, nullptr, nullptr, colour, depth, lastCount,
                                                          finalT, nullptr, nullptr, partialA + at, partialB + at);
@@ -675,13 +690,14 @@ __global__ __launch_bounds__(kLanes) void combineKernel(FrameParams f, uint32_t 
                                                         float* __restrict__ finalT,
                                                         const float4* __restrict__ partialA,
                                                         const float2* __restrict__ partialB) {
-    const uint32_t quadrant = blockIdx.x & 3u;
-    const uint32_t c = blockIdx.x >> 2;
+    const uint32_t qBits = quadBits(f.tileShift);
+    const uint32_t quadrant = blockIdx.x & ((1u << qBits) - 1u);
+    const uint32_t c = blockIdx.x >> qBits;
     if (c >= planCounts[1]) return;
     const uint4 ct = combine[c];
     const uint32_t tile = ct.x, n = ct.z, lane = threadIdx.x;
-    const uint32_t x0 = (tile % f.tilesX) * kTile + (quadrant & 1u) * 8u;
-    const uint32_t y0 = (tile / f.tilesX) * kTile + (quadrant >> 1) * 8u;
+    const uint2 origin = quadrantOrigin(f, tile, quadrant);
+    const uint32_t x0 = origin.x, y0 = origin.y;
     if (x0 >= f.width || y0 >= f.height) return;
     const uint32_t px = x0 + (lane & 7u), py = y0 + (lane >> 3);
     const bool inside = px < f.width && py < f.height && px < x0 + 8u && py < y0 + 8u;
@@ -690,7 +706,7 @@ __global__ __launch_bounds__(kLanes) void combineKernel(FrameParams f, uint32_t 
     __shared__ BlendList<LIST> list;
     WalkState st{0.0f, 0.0f, 0.0f, 1.0f, 0.0f, -1.0f, 0u, inside};
     for (uint32_t k = 0; k < n; ++k) {
-        const uint64_t at = (static_cast<uint64_t>(ct.y + k) * 4u + quadrant) * kLanes + lane;
+        const uint64_t at = ((static_cast<uint64_t>(ct.y + k) << qBits) + quadrant) * kLanes + lane;
         const float4 pa = partialA[at];
         const float2 pb = partialB[at];
         bool again = false;
@@ -729,10 +745,10 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
                  const uint32_t* pairSlots, const float4* proj, const float4* underColour, const float* underDepth,
                  float4* colour, float* depth, uint32_t* lastCount, float* finalT, float4* accum,
                  hipStream_t stream, uint32_t* walkSteps = nullptr) {
-    // Four blocks a tile, the grid rounded up to whole groups of eight tiles.
+    // A block a quadrant, the grid rounded up to whole groups of eight tiles.
     // The last contributor is tracked only where it is kept.
     const uint32_t tiles = f.tilesX * f.tilesY;
-    const uint32_t blocks = (tiles + 7u) / 8u * 32u;
+    const uint32_t blocks = (tiles + 7u) / 8u * (8u << (2u * (f.tileShift - 3u)));
     const auto go = [&](auto listLen, auto track) {
         blendKernel<DEPTH_MODE, decltype(listLen)::value, COMPOSITE, decltype(track)::value>
             <<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour,
@@ -752,21 +768,24 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
 
 template <uint32_t DEPTH_MODE, uint32_t LIST>
 void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* params, const ViewInfo* views,
-                      uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges, const uint32_t* pairSlots,
-                      const float4* proj, uint32_t* lastCount, float* finalT, float4* accum, uint32_t* queue,
-                      hipStream_t stream) {
+                      uint32_t viewCount, uint32_t tiles, uint32_t tileShift, uint32_t slots, const uint2* ranges,
+                      const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
+                      uint32_t* queue, hipStream_t stream) {
+    const uint32_t qBits = 2u * (tileShift - 3u);
     const auto go = [&](auto track) {
         constexpr bool T = decltype(track)::value;
         if (persistent) {
-            const uint32_t tasks = tiles * 4u;
+            const uint32_t tasks = tiles << qBits;
             const uint32_t grid = tasks < gridCap ? tasks : gridCap;
             fill32(queue, 0u, 1, stream);
             blendViewsKernel<DEPTH_MODE, LIST, true, T><<<grid, kLanes, 0, stream>>>(
-                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, accum, queue);
+                params, views, viewCount, tiles, qBits, tasks, slots, ranges, pairSlots, proj, lastCount, finalT,
+                accum, queue);
         } else {
-            const uint32_t tasks = (tiles + 7u) / 8u * 32u;
+            const uint32_t tasks = (tiles + 7u) / 8u * (8u << qBits);
             blendViewsKernel<DEPTH_MODE, LIST, false, T><<<tasks, kLanes, 0, stream>>>(
-                params, views, viewCount, tiles, tasks, slots, ranges, pairSlots, proj, lastCount, finalT, accum, queue);
+                params, views, viewCount, tiles, qBits, tasks, slots, ranges, pairSlots, proj, lastCount, finalT,
+                accum, queue);
         }
     };
     if (lastCount != nullptr) go(std::true_type{}); else go(std::false_type{});
@@ -775,24 +794,25 @@ void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* para
 }   // namespace
 
 void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gridCap, const FrameParams* params,
-                const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges,
+                const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t tileShift, uint32_t slots,
+                const uint2* ranges,
                 const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
                 uint32_t* queue, hipStream_t stream) {
     if (tiles == 0 || viewCount == 0) return;
     if (depthMode == 0) {
         if (list <= 128) {
-            launchBlendViews<0, 128>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+            launchBlendViews<0, 128>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
                                      proj, lastCount, finalT, accum, queue, stream);
         } else {
-            launchBlendViews<0, 256>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+            launchBlendViews<0, 256>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
                                      proj, lastCount, finalT, accum, queue, stream);
         }
     } else {
         if (list <= 128) {
-            launchBlendViews<1, 128>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+            launchBlendViews<1, 128>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
                                      proj, lastCount, finalT, accum, queue, stream);
         } else {
-            launchBlendViews<1, 256>(persistent, gridCap, params, views, viewCount, tiles, slots, ranges, pairSlots,
+            launchBlendViews<1, 256>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
                                      proj, lastCount, finalT, accum, queue, stream);
         }
     }
@@ -810,8 +830,9 @@ void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint
     if (tiles == 0) return;
     tileOrderKernel<<<1, kOrderWaves * 64, 0, stream>>>(ranges, tiles, plan.order, segmentPairs, plan.tasks,
                                                         plan.combine, plan.counts);
-    const uint32_t taskBlocks = (plan.taskCapacity + 7u) / 8u * 32u;
-    const uint32_t combineBlocks = plan.combineCapacity * 4u;
+    const uint32_t qBits = 2u * (f.tileShift - 3u);
+    const uint32_t taskBlocks = (plan.taskCapacity + 7u) / 8u * (8u << qBits);
+    const uint32_t combineBlocks = plan.combineCapacity << qBits;
     const auto launch = [&](auto depthMode, auto listLen) {
         constexpr uint32_t D = decltype(depthMode)::value, L = decltype(listLen)::value;
         blendTasksKernel<D, L><<<taskBlocks, kLanes, 0, stream>>>(f, slots, plan.tasks, plan.counts, ranges, pairSlots,

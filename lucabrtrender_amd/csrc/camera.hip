@@ -144,8 +144,9 @@ FrameParams frameParamsFor(const Projection& p, const SplatInstance& instance, c
     FrameParams f{};
     f.width = s.width;
     f.height = s.height;
-    f.tilesX = (s.width + kTile - 1) / kTile;
-    f.tilesY = (s.height + kTile - 1) / kTile;
+    f.tileShift = tileShiftOf(s.tileSize);
+    f.tilesX = tilesAlong(s.width, f.tileShift);
+    f.tilesY = tilesAlong(s.height, f.tileShift);
     f.focalX = static_cast<float>(p.focalX);
     f.focalY = static_cast<float>(p.focalY);
     f.centreX = static_cast<float>(p.centreX);

@@ -152,7 +152,7 @@ __global__ __launch_bounds__(kBlock) void projectDiffKernel(FrameParams f, DiffC
     Footprint fp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {1u, 1u, 0u, 0u}};
     const bool candidate = live && footprintOf(f, d.shape, fp);
     const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
-                                                   fp.rect, candidate);
+                                                   fp.rect, candidate, f.tileShift);
     if (depth.entries != nullptr) waveDepthRange(touched > 0, fp.tz, sDepth);
     const uint32_t degree = shDegree(f);
     const uint32_t stride = (1u + c.restPerColour) * 3u, pitch = rowPitch(stride);

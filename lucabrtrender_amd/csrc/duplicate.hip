@@ -58,7 +58,8 @@ __device__ __forceinline__ void emitPair(uint32_t at, uint64_t key, uint32_t s, 
 // [depthRange[0], depthRange[1]] is counted in *outOfRange.
 template <bool EMIT_INDEX, bool PER_VIEW>
 __device__ __forceinline__ void duplicateWave(uint32_t s, uint32_t expected, uint32_t width, uint32_t height,
-                                              uint32_t tilesX, uint32_t tileBase, const float4* __restrict__ proj,
+                                              uint32_t tilesX, uint32_t tileShift, uint32_t tileBase,
+                                              const float4* __restrict__ proj,
                                               const uint2* __restrict__ rects, const uint32_t* __restrict__ offsets,
                                               uint64_t* __restrict__ keys, uint32_t* __restrict__ values,
                                               uint32_t* __restrict__ pairSlot, uint32_t capacity,
@@ -109,7 +110,7 @@ __device__ __forceinline__ void duplicateWave(uint32_t s, uint32_t expected, uin
             rowColumn(cand - oStart, oW, exact, row, col);
             tx = ox0 + col;
             ty = oy0 + row;
-            hit = tileTouched(ow, oh, ocx, ocy, oa, ob, oc, ocut, tx, ty);
+            hit = tileTouched(ow, oh, ocx, ocy, oa, ob, oc, ocut, tx, ty, tileShift);
         }
         const uint64_t vote = __ballot(hit);
         if (hit) {
@@ -140,14 +141,14 @@ __global__ __launch_bounds__(kBlock) void duplicateKernel(FrameParams f, uint32_
                                                           uint32_t* __restrict__ outOfRange) {
     const uint32_t s = blockIdx.x * kBlock + threadIdx.x;
     const uint32_t expected = s < slots ? touched[s] : 0u;   // every lane stays for the wave's walk
-    duplicateWave<EMIT_INDEX, false>(s, expected, f.width, f.height, f.tilesX, 0u, proj, rects, offsets, keys,
-                                     values, pairSlot, capacity, overflow, depthRange, outOfRange);
+    duplicateWave<EMIT_INDEX, false>(s, expected, f.width, f.height, f.tilesX, f.tileShift, 0u, proj, rects, offsets,
+                                     keys, values, pairSlot, capacity, overflow, depthRange, outOfRange);
 }
 
 // A batch of views: slot s belongs to view s / splatsPerView.
 template <bool EMIT_INDEX>
 __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* __restrict__ views,
-                                                               uint32_t splatsPerView, uint32_t slots,
+                                                               uint32_t splatsPerView, uint32_t tileShift, uint32_t slots,
                                                                const float4* __restrict__ proj,
                                                                const uint2* __restrict__ rects,
                                                                const uint32_t* __restrict__ touched,
@@ -169,8 +170,8 @@ __global__ __launch_bounds__(kBlock) void duplicateViewsKernel(const ViewInfo* _
         tileBase = v.tileBase;
         if (outOfRange != nullptr) depthRange = &v.depthLo;   // depthLo, depthHi
     }
-    duplicateWave<EMIT_INDEX, true>(s, expected, width, height, tilesX, tileBase, proj, rects, offsets, keys, values,
-                                    pairSlot, capacity, overflow, depthRange, outOfRange);
+    duplicateWave<EMIT_INDEX, true>(s, expected, width, height, tilesX, tileShift, tileBase, proj, rects, offsets, keys,
+                                    values, pairSlot, capacity, overflow, depthRange, outOfRange);
 }
 
 }   // namespace
@@ -190,18 +191,21 @@ void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const u
     }
 }
 
-void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
+void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t tileShift, uint32_t slots,
+                    const float4* proj,
                     const uint2* rects, const uint32_t* touched, const uint32_t* offsets, uint64_t* keys,
                     uint32_t* values, uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow,
                     uint32_t* outOfRange, hipStream_t stream) {
     if (slots == 0 || splatsPerView == 0) return;
     const uint32_t blocks = (slots + kBlock - 1) / kBlock;
     if (pairSlot != nullptr) {
-        duplicateViewsKernel<true><<<blocks, kBlock, 0, stream>>>(views, splatsPerView, slots, proj, rects, touched,
+        duplicateViewsKernel<true><<<blocks, kBlock, 0, stream>>>(views, splatsPerView, tileShift, slots, proj, rects,
+                                                                  touched,
                                                                   offsets, keys, values, pairSlot, capacity, overflow,
                                                                   outOfRange);
     } else {
-        duplicateViewsKernel<false><<<blocks, kBlock, 0, stream>>>(views, splatsPerView, slots, proj, rects, touched,
+        duplicateViewsKernel<false><<<blocks, kBlock, 0, stream>>>(views, splatsPerView, tileShift, slots, proj, rects,
+                                                                   touched,
                                                                    offsets, keys, values, nullptr, capacity, overflow,
                                                                    outOfRange);
     }

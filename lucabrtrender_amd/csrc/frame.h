@@ -14,6 +14,11 @@ namespace lrt {
 
 inline constexpr uint32_t kTile = 16;   // frame.slang:31
 
+// A tile size's shift (8, 16 or 32 pixels; anything else is refused before a
+// frame is set up) and a frame's tiles along a side of `pixels`.
+inline constexpr uint32_t tileShiftOf(uint32_t tileSize) { return tileSize == 8 ? 3u : tileSize == 32 ? 5u : 4u; }
+inline constexpr uint32_t tilesAlong(uint32_t pixels, uint32_t shift) { return (pixels + (1u << shift) - 1u) >> shift; }
+
 struct SplatEditParams {   // shaders/lrt/common/edit.slang:23-36
     uint32_t active = 0, shape = 0, mode = 2, invert = 0;
     float centre[3] = {0, 0, 0};
@@ -24,6 +29,7 @@ struct SplatEditParams {   // shaders/lrt/common/edit.slang:23-36
 
 struct FrameParams {
     uint32_t width, height, tilesX, tilesY;
+    uint32_t tileShift;   // tiles are 1 << tileShift pixels square: 3, 4 (the reference's 16) or 5
     float focalX, focalY, centreX, centreY, nearZ, farZ;
     // Where the Jacobian is linearised at most: 1.3x the frustum, in units of
     // view x / z and y / z (set with the lens; the same float operations the

@@ -73,7 +73,8 @@ void duplicate(const FrameParams& f, uint32_t slots, const float4* proj, const u
 // its keys carry the view's tileBase; pairSlot as duplicate()'s. With
 // outOfRange non-null, a drawn slot whose depth key lies outside its view's
 // [depthLo, depthHi] is counted there.
-void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t slots, const float4* proj,
+void duplicateViews(const ViewInfo* views, uint32_t splatsPerView, uint32_t tileShift, uint32_t slots,
+                    const float4* proj,
                     const uint2* rects, const uint32_t* touched, const uint32_t* offsets, uint64_t* keys,
                     uint32_t* values, uint32_t* pairSlot, uint32_t capacity, uint32_t* overflow,
                     uint32_t* outOfRange, hipStream_t stream);
@@ -192,8 +193,8 @@ void blendComposite(const FrameParams& f, uint32_t list, uint32_t slots, const u
 
 // A frame whose heavy tiles are split: the tiles heaviest first, each tile of
 // more than segmentPairs pairs cut into near-equal segments whose quadrants
-// are walked by their own waves (partial walks in partialA/partialB, 256
-// entries a slot), then composited front to back by one wave a quadrant, a
+// are walked by their own waves (partial walks in partialA/partialB, one
+// entry a pixel of a tile in each slot), then composited front to back by one wave a quadrant, a
 // pixel that would stop inside a segment walking that segment again, so it
 // stops where the one walk would. order holds the tiles; tasks/combine/counts
 // the plan, sized for taskCapacity >= tiles + pairs / segmentPairs tasks and
@@ -215,10 +216,12 @@ void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint
 // A batch of views in one launch (device FrameParams and ViewInfo arrays;
 // views sorted by tileBase, `tiles` in all): as a grid of quadrant tasks, or
 // persistent (at most gridCap workgroups taking tasks from *queue, which is
-// cleared here). lastCount/finalT/accum (each may be null) hold the batch's
-// pixels (ViewInfo.pixelBase); accum as blend()'s.
+// cleared here). Every view's tiles are 2^tileShift pixels a side.
+// lastCount/finalT/accum (each may be null) hold the batch's pixels
+// (ViewInfo.pixelBase); accum as blend()'s.
 void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gridCap, const FrameParams* params,
-                const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t slots, const uint2* ranges,
+                const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t tileShift, uint32_t slots,
+                const uint2* ranges,
                 const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
                 uint32_t* queue, hipStream_t stream);
 

@@ -53,10 +53,11 @@ __global__ __launch_bounds__(kBlock) void projectPointsKernel(FrameParams f, Poi
         cx = f.focalX * tx * invZ + f.centreX;
         cy = f.focalY * ty * invZ + f.centreY;
         k = 1.0f / (radius * radius);
-        tileRectOfExtent(f.width, f.height, cx, cy, radius, radius, rect);
+        tileRectOfExtent(f.width, f.height, cx, cy, radius, radius, rect, f.tileShift);
         candidate = rect[0] <= rect[2] && rect[1] <= rect[3];
     }
-    const uint32_t touched = countTouchedTilesWave(f.width, f.height, cx, cy, k, 0.0f, k, 1.0f, rect, candidate);
+    const uint32_t touched = countTouchedTilesWave(f.width, f.height, cx, cy, k, 0.0f, k, 1.0f, rect, candidate,
+                                                   f.tileShift);
     if (depth.entries != nullptr) {
         waveDepthRange(touched > 0, tz, sDepth);
         __syncthreads();

@@ -99,7 +99,7 @@ __device__ __forceinline__ void projectBlock(const FrameParams& f, const float4*
     }
     // Every lane of the wave counts together, each rectangle among the wave's.
     const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
-                                                   fp.rect, candidate);
+                                                   fp.rect, candidate, f.tileShift);
 
     // The drawn splats in splat order: row `at` of the staged harmonics.
     const bool drawn = touched > 0;

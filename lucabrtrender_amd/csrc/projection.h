@@ -115,7 +115,7 @@ __device__ __forceinline__ bool footprintOf(const FrameParams& f, const SplatSha
     if (!(alpha >= 1.0f / 255.0f)) return false;
     const float ca = c / det, cb = -b / det, cc = a / det;
     const float cutoff = cutoffPower(alpha);
-    tileRect(f.width, f.height, cx, cy, ca, cb, cc, cutoff, fp.rect);
+    tileRect(f.width, f.height, cx, cy, ca, cb, cc, cutoff, fp.rect, f.tileShift);
     if (fp.rect[0] > fp.rect[2] || fp.rect[1] > fp.rect[3]) return false;
     fp.cx = cx;
     fp.cy = cy;
@@ -167,7 +167,7 @@ __device__ __forceinline__ uint32_t projectWave(const FrameParams& f, const Spla
     Footprint fp{0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, 0.0f, {1u, 1u, 0u, 0u}};
     const bool candidate = want && footprintOf(f, s, fp);
     const uint32_t touched = countTouchedTilesWave(f.width, f.height, fp.cx, fp.cy, fp.ca, fp.cb, fp.cc, fp.cutoff,
-                                                   fp.rect, candidate);
+                                                   fp.rect, candidate, f.tileShift);
     *tz = fp.tz;
     if (touched == 0) return 0;
     float dx, dy, dz;

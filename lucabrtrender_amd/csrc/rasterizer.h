@@ -112,6 +112,7 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:105-118)
     bool heaviestTilesFirst = true;     // launch the tiles with the most pairs first
     bool persistentBlend = false;       // batches: a persistent blend taking quadrants from a queue
     bool useGraph = false;              // batches: replay the launch sequence as a captured hipGraph
+    uint32_t tileSize = 16;             // pixels a square tile's side: 8, 16 (the reference's) or 32
 };
 
 struct RenderTargets {   // caller-owned device images, bottom row first

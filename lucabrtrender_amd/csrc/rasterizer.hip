@@ -166,8 +166,12 @@ Status TileRasterizer::checkFrame(const RenderSettings& settings, const RenderTa
     if (settings.width == 0 || settings.height == 0) {
         return Status::error(ErrorCode::InvalidArgument, "a frame needs a size");
     }
-    if (settings.width > 0xFFFFu * kTile || settings.height > 0xFFFFu * kTile) {
-        return Status::error(ErrorCode::Unsupported, "a frame wider or taller than 1048560 pixels");
+    if (settings.tileSize != 8 && settings.tileSize != 16 && settings.tileSize != 32) {
+        return Status::error(ErrorCode::InvalidArgument, "a tile size other than 8, 16 or 32 pixels");
+    }
+    const uint32_t shift = tileShiftOf(settings.tileSize);
+    if (tilesAlong(settings.width, shift) > 0xFFFFu || tilesAlong(settings.height, shift) > 0xFFFFu) {
+        return Status::error(ErrorCode::Unsupported, "a frame over 65535 tiles wide or tall");
     }
     if (targets.width != settings.width || targets.height != settings.height || targets.colour == nullptr) {
         return Status::error(ErrorCode::InvalidArgument, "the targets are not the frame's size");
@@ -226,7 +230,8 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     }
     if (total > 0xFFFFFFFFull) return Status::error(ErrorCode::Unsupported, "more than 2^32 splats in one frame");
     const uint32_t splatCount = static_cast<uint32_t>(total);
-    const uint32_t tiles = ((settings.width + kTile - 1) / kTile) * ((settings.height + kTile - 1) / kTile);
+    const uint32_t shift = tileShiftOf(settings.tileSize);
+    const uint32_t tiles = tilesAlong(settings.width, shift) * tilesAlong(settings.height, shift);
     if (!(s = check(hipSetDevice(device_), "hipSetDevice")).ok()) return s;
     if (!(s = reserveSplats(std::max(splatCount, 1u))).ok()) return s;
     if (!(s = reserveTargets(settings.width, settings.height, tiles, false)).ok()) return s;
@@ -295,6 +300,9 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
                                   FrameStats* statsOut) {
     Status s = checkFrame(settings, targets);
     if (!s.ok()) return s;
+    if (settings.tileSize != kTile) {
+        return Status::error(ErrorCode::Unsupported, "a differentiable frame with tiles other than 16 pixels");
+    }
     if (cloud.count > 0 && (cloud.means == nullptr || cloud.logScales == nullptr || cloud.quats == nullptr ||
                             cloud.logits == nullptr || cloud.sh == nullptr)) {
         return Status::error(ErrorCode::InvalidArgument, "a differentiable cloud with a missing parameter");
@@ -306,7 +314,8 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
     if (settings.filter3d && cloud.count > 0 && cloud.filter3d == nullptr) {
         return Status::error(ErrorCode::InvalidArgument, "the 3D filter is on and the cloud has no filter3d");
     }
-    const uint32_t tiles = ((settings.width + kTile - 1) / kTile) * ((settings.height + kTile - 1) / kTile);
+    const uint32_t shift = tileShiftOf(settings.tileSize);
+    const uint32_t tiles = tilesAlong(settings.width, shift) * tilesAlong(settings.height, shift);
     if (!(s = check(hipSetDevice(device_), "hipSetDevice")).ok()) return s;
     if (!(s = reserveSplats(std::max(cloud.count, 1u))).ok()) return s;
     if (!(s = reserveTargets(settings.width, settings.height, tiles, true)).ok()) return s;
@@ -458,10 +467,11 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
         plan.taskCapacity = tiles + pairs / seg + 1u;
         plan.combineCapacity = std::min(tiles, pairs / (seg + 1u) + 1u);
         const uint64_t slotCapacity = uint64_t{pairs / seg} + plan.combineCapacity + 1u;
-        const uint64_t partialBytes = slotCapacity * 256u * 16u;
+        const uint64_t tilePixels = uint64_t{settings.tileSize} * settings.tileSize;
+        const uint64_t partialBytes = slotCapacity * tilePixels * 16u;
         if (!(s = planTasks_.reserve(uint64_t{plan.taskCapacity} * 16, "blend.tasks")).ok()) return s;
         if (!(s = planCombine_.reserve(uint64_t{plan.combineCapacity} * 16, "blend.combine")).ok()) return s;
-        if (!(s = partials_.reserve(partialBytes + slotCapacity * 256u * 8u, "blend.partials")).ok()) return s;
+        if (!(s = partials_.reserve(partialBytes + slotCapacity * tilePixels * 8u, "blend.partials")).ok()) return s;
         plan.order = tileOrder_.as<uint32_t>();
         plan.tasks = planTasks_.as<uint4>();
         plan.combine = planCombine_.as<uint4>();
@@ -565,7 +575,7 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
     mark(2);
     mark(3);
     // Every drawn slot's depth key is checked against the sort key's field.
-    kern::duplicateViews(views, splatsPerView, slots, proj_.as<float4>(), rects_.as<uint2>(),
+    kern::duplicateViews(views, splatsPerView, tileShiftOf(settings.tileSize), slots, proj_.as<float4>(), rects_.as<uint2>(),
                          touched_.as<uint32_t>(), offsets_.as<uint32_t>(), keys_[0].as<uint64_t>(),
                          values_[0].as<uint32_t>(), training ? pairSlot_.as<uint32_t>() : nullptr, pairCapacity_,
                          counters + 2, counters + 10, stream);
@@ -599,7 +609,8 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
     // A persistent blend keeps about 24 one-wave workgroups on each CU.
     const bool keepWalk = training || settings.keepWalkState;
     kern::blendViews(settings.depthMode, settings.blendBatch, settings.persistentBlend, computeUnits_ * 24u, params,
-                     views, viewCount, tiles, slots, ranges_.as<uint2>(), pairSlots, proj_.as<float4>(),
+                     views, viewCount, tiles, tileShiftOf(settings.tileSize), slots, ranges_.as<uint2>(), pairSlots,
+                     proj_.as<float4>(),
                      keepWalk ? lastCount_.as<uint32_t>() : nullptr, keepWalk ? finalT_.as<float>() : nullptr,
                      training ? accum_.as<float4>() : nullptr, counters + 6, stream);
     mark(8);
@@ -666,17 +677,24 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
     if (views.empty()) return Status::error(ErrorCode::InvalidArgument, "a batch needs a view");
     const bool training = diff != nullptr;
+    if (settings.tileSize != 8 && settings.tileSize != 16 && settings.tileSize != 32) {
+        return Status::error(ErrorCode::InvalidArgument, "a tile size other than 8, 16 or 32 pixels");
+    }
+    if (training && settings.tileSize != kTile) {
+        return Status::error(ErrorCode::Unsupported, "a differentiable frame with tiles other than 16 pixels");
+    }
+    const uint32_t shift = tileShiftOf(settings.tileSize);
     uint64_t pixels = 0, tiles = 0;
     for (const View& v : views) {
         const RenderTargets& t = v.targets;
         if (t.width == 0 || t.height == 0 || t.colour == nullptr) {
             return Status::error(ErrorCode::InvalidArgument, "a view without a size or a colour target");
         }
-        if (t.width > 0xFFFFu * kTile || t.height > 0xFFFFu * kTile) {
-            return Status::error(ErrorCode::Unsupported, "a frame wider or taller than 1048560 pixels");
+        if (tilesAlong(t.width, shift) > 0xFFFFu || tilesAlong(t.height, shift) > 0xFFFFu) {
+            return Status::error(ErrorCode::Unsupported, "a frame over 65535 tiles wide or tall");
         }
         pixels += uint64_t{t.width} * t.height;
-        tiles += uint64_t{(t.width + kTile - 1) / kTile} * ((t.height + kTile - 1) / kTile);
+        tiles += uint64_t{tilesAlong(t.width, shift)} * tilesAlong(t.height, shift);
     }
     uint64_t splats = 0;
     for (const SplatInstance& instance : instances) {
@@ -765,8 +783,8 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
         ViewInfo& info = hostViews_[v];
         info.width = view.targets.width;
         info.height = view.targets.height;
-        info.tilesX = (info.width + kTile - 1) / kTile;
-        info.tilesY = (info.height + kTile - 1) / kTile;
+        info.tilesX = tilesAlong(info.width, shift);
+        info.tilesY = tilesAlong(info.height, shift);
         info.tileBase = tileBase;
         info.params = v;
         info.pixelBase = pixelBase;
@@ -791,7 +809,8 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
                               std::to_string(tiles) + "/" + std::to_string(pairCapacity_) + "/" +
                               std::to_string(settings.blendBatch) + "/" + std::to_string(settings.persistentBlend) +
                               "/" + std::to_string(settings.keepWalkState) + "/" + std::to_string(training) + "/" +
-                              std::to_string(settings.depthMode) + "/" + std::to_string(viewsLayout_.depthMin) +
+                              std::to_string(settings.depthMode) + "/" + std::to_string(settings.tileSize) + "/" +
+                              std::to_string(viewsLayout_.depthMin) +
                               "/" + std::to_string(viewsLayout_.depthBits);
             const auto addr = [&key](const void* p) { key += "/" + std::to_string(reinterpret_cast<uintptr_t>(p)); };
             for (const SplatInstance& instance : instances) {
@@ -961,10 +980,12 @@ Status TileRasterizer::autotuneViews(std::span<const View> views, std::span<cons
                                      RenderSettings& settings, float* ms) {
     RenderSettings best = settings;
     double bestMs = -1.0;
+    for (const uint32_t tile : {8u, 16u, 32u})
     for (const uint32_t list : {128u, 256u}) {
         for (const bool persistent : {false, true}) {
             for (const bool graph : {false, true}) {
                 RenderSettings trial = settings;
+                trial.tileSize = tile;
                 trial.blendBatch = list;
                 trial.persistentBlend = persistent;
                 trial.useGraph = graph;

@@ -173,42 +173,44 @@ __device__ __forceinline__ bool quadrantReach(float a, float b, float c, float c
     return centreInside || degenerate || best <= cutoff + slack;
 }
 
-// Does tile (tx, ty) hold a pixel centre inside the footprint (frame.slang:133-138)?
+// Does tile (tx, ty), 1 << tileShift pixels square, hold a pixel centre inside
+// the footprint (frame.slang:133-138, whose tiles are 16 pixels: tileShift 4)?
 __device__ __forceinline__ bool tileTouched(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
-                                            float c, float cutoff, uint32_t tx, uint32_t ty) {
-    const float fx = static_cast<float>(tx * kTile) + 0.5f;
-    const float fy = static_cast<float>(ty * kTile) + 0.5f;
-    const uint32_t ex = (tx + 1u) * kTile < width ? (tx + 1u) * kTile : width;
-    const uint32_t ey = (ty + 1u) * kTile < height ? (ty + 1u) * kTile : height;
+                                            float c, float cutoff, uint32_t tx, uint32_t ty, uint32_t tileShift) {
+    const float fx = static_cast<float>(tx << tileShift) + 0.5f;
+    const float fy = static_cast<float>(ty << tileShift) + 0.5f;
+    const uint32_t ex = (tx + 1u) << tileShift < width ? (tx + 1u) << tileShift : width;
+    const uint32_t ey = (ty + 1u) << tileShift < height ? (ty + 1u) << tileShift : height;
     const float lx = static_cast<float>(ex - 1u) + 0.5f;
     const float ly = static_cast<float>(ey - 1u) + 0.5f;
     return minPowerOverBox(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
 }
 
 // The tile rectangle, inclusive, of a footprint with half-widths ex, ey about
-// (cx, cy) (frame.slang:149-161); empty (x0 > x1) when it is off screen.
+// (cx, cy) (frame.slang:149-161), in tiles of 1 << tileShift pixels; empty
+// (x0 > x1) when it is off screen.
 __device__ __forceinline__ void tileRectOfExtent(uint32_t width, uint32_t height, float cx, float cy, float ex,
-                                                 float ey, uint32_t rect[4]) {
+                                                 float ey, uint32_t rect[4], uint32_t tileShift) {
     const float x1 = cx - ex, y1 = cy - ey, x2 = cx + ex, y2 = cy + ey;
     if (x2 < 0.0f || y2 < 0.0f || x1 >= static_cast<float>(width) || y1 >= static_cast<float>(height)) {
         rect[0] = 1; rect[1] = 1; rect[2] = 0; rect[3] = 0;
         return;
     }
-    rect[0] = static_cast<uint32_t>(fmaxS(x1, 0.0f)) / kTile;
-    rect[1] = static_cast<uint32_t>(fmaxS(y1, 0.0f)) / kTile;
-    rect[2] = static_cast<uint32_t>(fminS(x2, static_cast<float>(width) - 1.0f)) / kTile;
-    rect[3] = static_cast<uint32_t>(fminS(y2, static_cast<float>(height) - 1.0f)) / kTile;
+    rect[0] = static_cast<uint32_t>(fmaxS(x1, 0.0f)) >> tileShift;
+    rect[1] = static_cast<uint32_t>(fmaxS(y1, 0.0f)) >> tileShift;
+    rect[2] = static_cast<uint32_t>(fminS(x2, static_cast<float>(width) - 1.0f)) >> tileShift;
+    rect[3] = static_cast<uint32_t>(fminS(y2, static_cast<float>(height) - 1.0f)) >> tileShift;
 }
 
 // The tile rectangle of a Gaussian's footprint: its extent from the 2D
 // covariance, sqrt(cutoff sigma_xx) and sqrt(cutoff sigma_yy) (frame.slang:142-146).
 __device__ __forceinline__ void tileRect(uint32_t width, uint32_t height, float cx, float cy, float a, float b,
-                                         float c, float cutoff, uint32_t rect[4]) {
+                                         float c, float cutoff, uint32_t rect[4], uint32_t tileShift) {
     const float det = a * c - b * b;
     const float inv = det > 1e-20f ? 1.0f / det : 0.0f;
     const float ex = __builtin_sqrtf(fmaxS(cutoff * c * inv, 0.0f));
     const float ey = __builtin_sqrtf(fmaxS(cutoff * a * inv, 0.0f));
-    tileRectOfExtent(width, height, cx, cy, ex, ey, rect);
+    tileRectOfExtent(width, height, cx, cy, ex, ey, rect, tileShift);
 }
 
 __device__ __forceinline__ float shflF(float v, uint32_t lane) { return __shfl(v, static_cast<int>(lane), 64); }
@@ -319,10 +321,11 @@ __device__ __forceinline__ void rowColumn(uint32_t k, uint32_t w, bool exact, ui
 // The number of tiles of each lane's rect (inclusive x0, y0, x1, y1) that its
 // footprint touches (tileTouched), every lane's rectangle among the wave's
 // flattened candidates. `want` false: the lane counts nothing (its rect is not
-// read). Every lane of the wave must reach the call.
+// read). Tiles are 1 << tileShift pixels square. Every lane of the wave must
+// reach the call.
 __device__ __forceinline__ uint32_t countTouchedTilesWave(uint32_t width, uint32_t height, float cx, float cy,
                                                           float a, float b, float c, float cutoff,
-                                                          const uint32_t rect[4], bool want) {
+                                                          const uint32_t rect[4], bool want, uint32_t tileShift) {
     const uint32_t lane = __lane_id();
     const uint32_t w = want ? rect[2] - rect[0] + 1u : 1u;
     const WaveTiles wt = waveTiles(want ? w * (rect[3] - rect[1] + 1u) : 0u);
@@ -339,7 +342,7 @@ __device__ __forceinline__ uint32_t countTouchedTilesWave(uint32_t width, uint32
         if (cand < wt.total) {
             uint32_t row, col;
             rowColumn(cand - oStart, oW, exact, row, col);
-            hit = tileTouched(width, height, ocx, ocy, oa, ob, oc, ocut, ox0 + col, oy0 + row);
+            hit = tileTouched(width, height, ocx, ocy, oa, ob, oc, ocut, ox0 + col, oy0 + row, tileShift);
         }
         touched += static_cast<uint32_t>(__popcll(__ballot(hit) & runMask(wt.start, wt.area, base)));
     }

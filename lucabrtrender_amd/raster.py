@@ -126,6 +126,7 @@ class RenderSettings:
     use_graph: bool = False         # batches: capture the launch sequence once and replay it
     blend_segment: int = 0          # a tile with more pairs is split across waves and composited (0: never)
     keep_walk_state: bool = False   # keep each pixel's last contributor and final transmittance (debug)
+    tile_size: int = 16             # pixels a square tile's side: 8, 16 or 32 (differentiable frames: 16)
 
     def to_c(self) -> _abi.RenderSettingsC:
         s = _abi.RenderSettingsC()
@@ -144,6 +145,7 @@ class RenderSettings:
         s.use_graph = 1 if self.use_graph else 0
         s.blend_segment = self.blend_segment
         s.keep_walk_state = 1 if self.keep_walk_state else 0
+        s.tile_size = self.tile_size
         return s
 
 
@@ -581,7 +583,7 @@ class TileRasterizer:
         self._check(lib.lrt_autotune_views(self._ctx, self._views_c(views), len(views), _instances_c(instances),
                                            len(instances), C.byref(c), C.byref(ms)))
         return dataclasses.replace(settings, blend_batch=c.blend_batch, persistent_blend=bool(c.persistent_blend),
-                                   use_graph=bool(c.use_graph)), float(ms.value)
+                                   use_graph=bool(c.use_graph), tile_size=c.tile_size), float(ms.value)
 
     def render_diff(self, camera_or_projection, splats: DiffSplats, settings: RenderSettings,
                     targets: RenderTargets, object_to_world=IDENTITY) -> FrameStats:

@@ -484,8 +484,9 @@ void or_frame_for(const or_projection* p, const or_settings* s, const double obj
     memset(f, 0, sizeof *f);
     f->width = s->width;
     f->height = s->height;
-    f->tiles_x = (s->width + 15) / 16;
-    f->tiles_y = (s->height + 15) / 16;
+    f->tile_shift = s->tile_size == 8 ? 3u : s->tile_size == 32 ? 5u : 4u;
+    f->tiles_x = (s->width + (1u << f->tile_shift) - 1u) >> f->tile_shift;
+    f->tiles_y = (s->height + (1u << f->tile_shift) - 1u) >> f->tile_shift;
     f->focal_x = (float)p->focal_x;
     f->focal_y = (float)p->focal_y;
     f->centre_x = (float)p->centre_x;
@@ -550,10 +551,11 @@ static float min_power_over_box(float a, float b, float c, float lox, float loy,
 /* frame.slang:133-138 */
 static inline int tile_touched(const or_frame* f, float cx, float cy, float a, float b, float c,
                                float cutoff, uint32_t tx, uint32_t ty) {
-    const float fx = (float)(tx * 16u) + 0.5f;
-    const float fy = (float)(ty * 16u) + 0.5f;
-    const uint32_t ex = (tx + 1u) * 16u < f->width ? (tx + 1u) * 16u : f->width;
-    const uint32_t ey = (ty + 1u) * 16u < f->height ? (ty + 1u) * 16u : f->height;
+    const uint32_t side = 1u << f->tile_shift;
+    const float fx = (float)(tx * side) + 0.5f;
+    const float fy = (float)(ty * side) + 0.5f;
+    const uint32_t ex = (tx + 1u) * side < f->width ? (tx + 1u) * side : f->width;
+    const uint32_t ey = (ty + 1u) * side < f->height ? (ty + 1u) * side : f->height;
     const float lx = (float)(ex - 1u) + 0.5f;
     const float ly = (float)(ey - 1u) + 0.5f;
     return min_power_over_box(a, b, c, fx - cx, fy - cy, lx - cx, ly - cy) <= cutoff;
@@ -574,10 +576,10 @@ static inline void tile_rect(const or_frame* f, float cx, float cy, float ex, fl
         r[0] = 1; r[1] = 1; r[2] = 0; r[3] = 0;
         return;
     }
-    r[0] = (uint32_t)fmax_(x1, 0.0f) / 16u;
-    r[1] = (uint32_t)fmax_(y1, 0.0f) / 16u;
-    r[2] = (uint32_t)fmin_(x2, (float)f->width - 1.0f) / 16u;
-    r[3] = (uint32_t)fmin_(y2, (float)f->height - 1.0f) / 16u;
+    r[0] = (uint32_t)fmax_(x1, 0.0f) >> f->tile_shift;
+    r[1] = (uint32_t)fmax_(y1, 0.0f) >> f->tile_shift;
+    r[2] = (uint32_t)fmin_(x2, (float)f->width - 1.0f) >> f->tile_shift;
+    r[3] = (uint32_t)fmin_(y2, (float)f->height - 1.0f) >> f->tile_shift;
 }
 
 /* sh.slang:8-13 */
@@ -961,7 +963,7 @@ static void blend_pixel(const or_frame* f, uint32_t px, uint32_t py, const uint3
                         const uint32_t* pair_splats, const or_proj_record* proj, const float* under_colour,
                         const float* under_depth, int composite, float* colour, float* depth,
                         uint32_t* last_count, float* final_t) {
-    const uint32_t tile = (py / 16u) * f->tiles_x + px / 16u;
+    const uint32_t tile = (py >> f->tile_shift) * f->tiles_x + (px >> f->tile_shift);
     const uint32_t start = ranges[tile * 2], end = ranges[tile * 2 + 1];
     const float pxf = (float)px + 0.5f, pyf = (float)py + 0.5f;
     const size_t at = (size_t)py * f->width + px;

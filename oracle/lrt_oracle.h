@@ -109,6 +109,7 @@ typedef struct {
  * 3D filter variance per splat). */
 typedef struct {
     uint32_t width, height, tiles_x, tiles_y;
+    uint32_t tile_shift;   /* log2 of the tile's side: 3, 4 (the reference's 16 pixels) or 5 */
     float focal_x, focal_y, centre_x, centre_y, near_z, far_z;
     /* The Jacobian's clamp, 1.3x the frustum in units of x/z and y/z: the
      * expressions the projection evaluates (frame.slang's splatProject
@@ -137,6 +138,7 @@ typedef struct {
     int32_t linearise;
     float dilation;              /* 0.3 reproduces the reference */
     int32_t filter3d;            /* 0 reproduces the reference */
+    uint32_t tile_size;          /* 8, 16 or 32 pixels a tile's side; 0 or 16 reproduces the reference */
 } or_settings;
 
 void or_look_at(const double eye[3], const double target[3], const double up[3], double out[16]);

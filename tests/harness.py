@@ -23,7 +23,7 @@ def oracle_edit(e: raster.SplatEdit) -> O.OrEdit:
 
 def oracle_settings(s: raster.RenderSettings) -> O.OrSettings:
     return O.settings(s.width, s.height, s.antialias, s.max_sh_degree, s.background, s.depth_mode,
-                      s.depth_threshold, s.linearise, s.dilation, s.filter3d)
+                      s.depth_threshold, s.linearise, s.dilation, s.filter3d, s.tile_size)
 
 
 def oracle_frames(cam, settings, instances):

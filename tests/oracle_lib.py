@@ -33,7 +33,7 @@ class OrEdit(C.Structure):
 
 class OrFrame(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("tiles_x", C.c_uint32), ("tiles_y", C.c_uint32),
-                ("focal_x", C.c_float), ("focal_y", C.c_float), ("centre_x", C.c_float), ("centre_y", C.c_float),
+                ("tile_shift", C.c_uint32), ("focal_x", C.c_float), ("focal_y", C.c_float), ("centre_x", C.c_float), ("centre_y", C.c_float),
                 ("near_z", C.c_float), ("far_z", C.c_float), ("lim_lo_x", C.c_float), ("lim_hi_x", C.c_float),
                 ("lim_lo_y", C.c_float), ("lim_hi_y", C.c_float), ("orthographic", C.c_uint32),
                 ("antialias", C.c_uint32), ("m", C.c_float * 12), ("eye", C.c_float * 3), ("count", C.c_uint32),
@@ -47,7 +47,7 @@ class OrSettings(C.Structure):
     _fields_ = [("width", C.c_uint32), ("height", C.c_uint32), ("antialias", C.c_int32),
                 ("max_sh_degree", C.c_uint32), ("background", C.c_float * 4), ("depth_mode", C.c_int32),
                 ("depth_threshold", C.c_float), ("linearise", C.c_int32), ("dilation", C.c_float),
-                ("filter3d", C.c_int32)]
+                ("filter3d", C.c_int32), ("tile_size", C.c_uint32)]
 
 
 class OrEncoding(C.Structure):
@@ -186,12 +186,13 @@ def projection_for(cam: OrCamera, width: int, height: int) -> OrProjection:
 
 
 def settings(width, height, antialias=True, max_sh_degree=3, background=(0, 0, 0, 0), depth_mode=0,
-             depth_threshold=0.5, linearise=True, dilation=0.3, filter3d=False) -> OrSettings:
+             depth_threshold=0.5, linearise=True, dilation=0.3, filter3d=False, tile_size=16) -> OrSettings:
     s = OrSettings()
     s.width, s.height, s.antialias, s.max_sh_degree = width, height, 1 if antialias else 0, max_sh_degree
     s.background[:] = [float(v) for v in background]
     s.depth_mode, s.depth_threshold, s.linearise = depth_mode, depth_threshold, 1 if linearise else 0
     s.dilation, s.filter3d = dilation, 1 if filter3d else 0
+    s.tile_size = tile_size
     return s
 
 

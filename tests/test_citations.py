@@ -26,6 +26,7 @@ CITED = {
     "modules/core/include/lrt/core/Error.h:14-26": ("modules/core/include/lrt/core/Error.h", "enum class ErrorCode"),
     "modules/render/include/lrt/render/Camera.h:23-44": (R + "Camera.h", "struct Lens"),
     "Camera.h:46-53": (R + "Camera.h", "struct Camera"),
+    "frame.slang:31": ("shaders/lrt/splat/frame.slang", "kTile = 16"),
     "Camera.h:56-80": (R + "Camera.h", "struct Projection"),
     "modules/render/src/Camera.cpp:9-13": ("modules/render/src/Camera.cpp", "Camera::lookingAt"),
     "Transform.h:376-400": ("modules/aofx/sdk/include/aofx/Transform.h", "lookAt("),

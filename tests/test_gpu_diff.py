@@ -190,3 +190,14 @@ def test_backward_needs_a_differentiable_frame(rast):
     with pytest.raises(LrtError) as err:
         rast.backward(torch.zeros((32, 32, 4), device="cuda"), grads)
     assert err.value.code == 3
+    # The backward walks 16-pixel tiles: other tile sizes are refused up front.
+    splats = raster.DiffSplats(dev["means"], dev["log_scales"], dev["quats"], dev["opacity_logits"], dev["sh"])
+    for tile in (8, 32):
+        with pytest.raises(LrtError) as err:
+            rast.render_diff(_camera(), splats, raster.RenderSettings(width=32, height=32, tile_size=tile),
+                             raster.RenderTargets(32, 32))
+        assert err.value.name == "Unsupported", err.value.name
+    with pytest.raises(LrtError) as err:
+        rast.render(_camera(), [raster.SplatInstance(cfg.scene.to_device())],
+                    raster.RenderSettings(width=32, height=32, tile_size=12), raster.RenderTargets(32, 32))
+    assert err.value.code == 3

@@ -238,6 +238,38 @@ def test_every_batch_depth_and_tile_order_draws_the_same_frame(rast, cloud, batc
     _image_checks(gpu, ora)
 
 
+@pytest.mark.parametrize("tile", [8, 32])
+@pytest.mark.parametrize("name", ["perspective", "frustum-edge", "no-aa-window"])
+def test_tile_sizes_bin_as_the_oracle_and_draw_the_16_pixel_frame(rast, cloud, name, tile):
+    """At 8- and 32-pixel tiles the counts, records, pairs and ranges are the
+    oracle's at that tile size, bit for bit, and the images are the 16-pixel
+    frame's: the quadrants are the same 8x8 blocks and each walks the splats
+    that reach it in the same (depth key, index) order."""
+    print(f"tile size {tile} camera {name}", flush=True)
+    cam, extra = CAMERAS[name]
+    inst = [(cloud, np.eye(4), raster.SplatEdit())]
+    base = H.run_gpu(cam, raster.RenderSettings(width=W, height=HGT, **extra), inst, rast)
+    s = raster.RenderSettings(width=W, height=HGT, tile_size=tile, **extra)
+    gpu = H.run_gpu(cam, s, inst, rast)
+    ora = H.run_oracle(cam, s, inst)
+    assert len(ora["ranges"]) == (-(-W // tile)) * (-(-HGT // tile))
+    H.assert_bit_exact(gpu, ora)
+    _image_checks(gpu, ora)
+    np.testing.assert_array_equal(gpu["colour"].view(np.uint32), base["colour"].view(np.uint32))
+    np.testing.assert_array_equal(gpu["depth"].view(np.uint32), base["depth"].view(np.uint32))
+    np.testing.assert_array_equal(gpu["final_t"].view(np.uint32), base["final_t"].view(np.uint32))
+    # The 1080p benchmark camera over a clustered scene, heavy tiles split.
+    cfg = scenes.workload("plain-fwd-65k-640x480", count=20000)
+    inst1 = [(cfg.scene, np.eye(4), raster.SplatEdit())]
+    for segment in (0, 64):
+        s1 = raster.RenderSettings(width=cfg.width, height=cfg.height, tile_size=tile, blend_segment=segment,
+                                   depth_mode=1)
+        gpu = H.run_gpu(cfg.camera, s1, inst1, rast)
+        ora = H.run_oracle(cfg.camera, s1, inst1)
+        H.assert_bit_exact(gpu, ora)
+        _image_checks(gpu, ora)
+
+
 def _heaviest_first(ranges):
     """Tiles by the bit length of their pair count, descending, index order within one length."""
     counts = (ranges[:, 1].astype(np.int64) - ranges[:, 0].astype(np.int64))

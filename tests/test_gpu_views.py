@@ -43,13 +43,15 @@ def _batch(rast, cams, sizes, inst, settings):
     return [(v.targets.colour.cpu().numpy(), v.targets.depth.cpu().numpy()) for v in views], stats
 
 
-@pytest.mark.parametrize("persistent,graph,depth_mode", [(False, False, 0), (True, False, 0), (False, True, 1),
-                                                         (True, True, 0)])
-def test_each_view_is_the_single_view_frame(cloud, persistent, graph, depth_mode):
+@pytest.mark.parametrize("persistent,graph,depth_mode,tile", [(False, False, 0, 16), (True, False, 0, 16),
+                                                              (False, True, 1, 16), (True, True, 0, 16),
+                                                              (False, False, 0, 8), (True, True, 1, 8),
+                                                              (True, False, 1, 32), (False, True, 0, 32)])
+def test_each_view_is_the_single_view_frame(cloud, persistent, graph, depth_mode, tile):
     rast = raster.TileRasterizer()
     inst = [raster.SplatInstance(cloud)]
     settings = raster.RenderSettings(persistent_blend=persistent, use_graph=graph, depth_mode=depth_mode,
-                                     background=(0.1, 0.0, 0.2, 0.3))
+                                     background=(0.1, 0.0, 0.2, 0.3), tile_size=tile)
     cams = [_cam(k, len(SIZES)) for k in range(len(SIZES))]
     out, stats = _batch(rast, cams, SIZES, inst, settings)
     total = 0
@@ -103,7 +105,7 @@ def test_autotune_picks_one_of_its_candidates(cloud):
     inst = [raster.SplatInstance(cloud)]
     views = [raster.View(_cam(k, 4), raster.RenderTargets(*SIZES[k])) for k in range(4)]
     best, ms = rast.autotune_views(views, inst, raster.RenderSettings())
-    assert best.blend_batch in (128, 256) and ms > 0.0
+    assert best.blend_batch in (128, 256) and best.tile_size in (8, 16, 32) and ms > 0.0
     out = rast.render_views(views, inst, best)
     assert out.pairs > 0
 

@@ -117,12 +117,14 @@ def test_antialiasing_pays_back_the_dilation_energy():
     assert without > 0.0 and energy(True) < 0.6 * without
 
 
-def test_binning_invariants(cloud):
+@pytest.mark.parametrize("tile", [16, 8, 32])
+def test_binning_invariants(cloud, tile):
     """Every pixel centre a splat reaches (its power at that centre, in double,
     below its cutoff) lies in a tile whose range holds the splat; each tile's run
-    holds only its own pairs, in (24-bit depth key, slot) order."""
+    holds only its own pairs, in (24-bit depth key, slot) order. At the
+    reference's 16-pixel tiles and at this library's 8 and 32."""
     cam, _, _ = SECTIONS["perspective"]
-    s = raster.RenderSettings(width=W, height=HGT)
+    s = raster.RenderSettings(width=W, height=HGT, tile_size=tile)
     r = H.run_oracle(cam, s, [(cloud, np.eye(4), raster.SplatEdit())])
     f = r["frames"][0]
     tiles, splats, ranges, proj = r["pair_tiles"], r["pair_splats"], r["ranges"], r["proj"]
@@ -139,7 +141,8 @@ def test_binning_invariants(cloud):
         members.setdefault(int(tiles[k]), set()).add(int(splats[k]))
     ys, xs = np.mgrid[0:HGT, 0:W]
     px, py = xs.reshape(-1) + 0.5, ys.reshape(-1) + 0.5
-    tile_of = (ys.reshape(-1) // 16) * f.tiles_x + xs.reshape(-1) // 16
+    assert f.tiles_x == -(-W // tile) and f.tiles_y == -(-HGT // tile)
+    tile_of = (ys.reshape(-1) // tile) * f.tiles_x + xs.reshape(-1) // tile
     for i in np.nonzero(r["touched"])[0][:600]:
         sc, co = proj[i, :4].astype(np.float64), proj[i, 4:8].astype(np.float64)
         dx, dy = px - sc[0], py - sc[1]
@@ -147,6 +150,21 @@ def test_binning_invariants(cloud):
         reached = np.unique(tile_of[power <= sc[3] * (1 - 1e-6)])
         for t in reached:
             assert i in members.get(int(t), ()), (i, t)
+
+
+@pytest.mark.parametrize("name", list(SECTIONS))
+def test_tile_size_changes_the_pairs_not_the_image(cloud, name):
+    """Every splat reaching a pixel is in its tile's list at any tile size and
+    each list walks in (depth key, index) order, so 8- and 32-pixel tiles draw
+    the 16-pixel frame bit for bit, from about 3x and 0.4x its pairs."""
+    cam, extra, _ = SECTIONS[name]
+    inst = [(cloud, np.eye(4), raster.SplatEdit())]
+    runs = {t: H.run_oracle(cam, raster.RenderSettings(width=W, height=HGT, tile_size=t, **extra), inst)
+            for t in (16, 8, 32)}
+    for t in (8, 32):
+        np.testing.assert_array_equal(runs[t]["colour"].view(np.uint32), runs[16]["colour"].view(np.uint32))
+        np.testing.assert_array_equal(runs[t]["depth"].view(np.uint32), runs[16]["depth"].view(np.uint32))
+    assert runs[8]["pairs"] > 2 * runs[16]["pairs"] > 4 * runs[32]["pairs"]
 
 
 def test_splat_edit_renders_as_the_brute_force_renders_it(cloud):
