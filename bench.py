@@ -628,7 +628,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=10)
     ap.add_argument("--count", type=int, default=3_000_000, help="splats (the workload is 3,000,000)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
-    ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 128 or 256")
+    ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 64, 128 or 256")
     ap.add_argument("--tile-size", type=int, default=16, help="pixels a tile's side: 8, 16 (the reference's) or 32")
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="fwd+bwd views (1M splats, 720p) timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="sort-stress frames (6M splats, 2160p) timed; 0 skips it")
@@ -740,7 +740,7 @@ def main():
     copy_gbs = copy_bandwidth_gbs(torch)
     # The blend at each batch depth and launch order, for the record.
     variants = {}
-    for batch in (128, 256):
+    for batch in (64, 128, 256):
         for order in (0, 1):
             vs = raster.RenderSettings(**{**timed.__dict__, "blend_batch": batch, "tile_order": order})
             acc_v = np.zeros(8)

@@ -133,7 +133,8 @@ typedef struct {
     int32_t time_stages;          /* per-stage HIP event timing into lrt_frame_stats */
     float dilation;               /* 0.3 reproduces the reference */
     int32_t filter3d;             /* 1: read lrt_splats.filter3d */
-    uint32_t blend_batch;         /* records each 8x8 quadrant's LDS list holds: 128 or 256 (0, the default: 256) */
+    uint32_t blend_batch;         /* records each 8x8 quadrant's LDS list holds: 64, 128 or 256 (0, the
+                                     default: 256; other values take the next of those up, 256 at most) */
     int32_t tile_order;           /* 0: tile index order; 1 (default): heaviest tiles launched first */
     int32_t persistent_blend;     /* batches: 1 = a persistent blend taking quadrants from a queue */
     int32_t use_graph;            /* batches: 1 = capture the launch sequence once and replay it */
@@ -363,7 +364,7 @@ lrt_status lrt_raster_render_views(lrt_context* ctx, const lrt_view* views, uint
                                    const lrt_render_settings* settings, lrt_frame_stats* stats);
 
 /* The autotuner: times the batch under each launch choice this build offers
- * (tile size 8/16/32, list length 128/256, grid or persistent blend, graph
+ * (tile size 8/16/32, list length 64/128/256, grid or persistent blend, graph
  * replay or not) and writes the fastest into *settings (its other fields are
  * kept). ms (host, may be NULL) receives the winner's batch time. The radix
  * width (8 bits) is fixed in this build. */

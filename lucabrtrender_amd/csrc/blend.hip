@@ -51,8 +51,6 @@ namespace {
 using namespace lrt::dev;
 
 constexpr uint32_t kLanes = 64;          // one wave: an 8x8 quadrant
-constexpr uint32_t kPerLane = 2;         // pairs a lane stages per round
-constexpr uint32_t kRound = kLanes * kPerLane;
 constexpr uint32_t kGroup = 4;           // list records walked per step
 constexpr float kExpScale = -0.72134752044448170368f;   // -log2(e) / 2
 constexpr float kLog2Floor = -7.99435343685885793775f;   // log2(1 / 255): alpha's floor
@@ -199,6 +197,9 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
                                               float2* __restrict__ partialB = nullptr,
                                               uint32_t* __restrict__ walkSteps = nullptr) {
     static_assert(!(COMPOSITE && MODE != kFinish), "a composite walk is not split");
+    // Pairs a lane stages a round: two, or one for a 64-record list.
+    constexpr uint32_t kPerLane = LIST >= 2 * kLanes ? 2u : 1u;
+    constexpr uint32_t kRound = kLanes * kPerLane;
     static_assert(LIST % kRound == 0 && LIST >= kRound, "list length");
     float4* const sGeom = list.geom;
     float4* const sConic = list.conic;
@@ -754,12 +755,15 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
             <<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour,
                                             depth, lastCount, finalT, accum, walkSteps);
     };
+    using L64 = std::integral_constant<uint32_t, 64>;
     using L128 = std::integral_constant<uint32_t, 128>;
     using L256 = std::integral_constant<uint32_t, 256>;
     using Keep = std::true_type;
     using Drop = std::false_type;
     const bool track = !COMPOSITE && lastCount != nullptr;
-    if (list <= 128) {
+    if (list <= 64) {
+        if (track) go(L64{}, Keep{}); else go(L64{}, Drop{});
+    } else if (list <= 128) {
         if (track) go(L128{}, Keep{}); else go(L128{}, Drop{});
     } else {
         if (track) go(L256{}, Keep{}); else go(L256{}, Drop{});
@@ -799,22 +803,20 @@ void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gri
                 const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
                 uint32_t* queue, hipStream_t stream) {
     if (tiles == 0 || viewCount == 0) return;
+    const auto go = [&](auto depthMode, auto listLen) {
+        launchBlendViews<decltype(depthMode)::value, decltype(listLen)::value>(
+            persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots, proj, lastCount,
+            finalT, accum, queue, stream);
+    };
+    using D0 = std::integral_constant<uint32_t, 0>;
+    using D1 = std::integral_constant<uint32_t, 1>;
+    using L64 = std::integral_constant<uint32_t, 64>;
+    using L128 = std::integral_constant<uint32_t, 128>;
+    using L256 = std::integral_constant<uint32_t, 256>;
     if (depthMode == 0) {
-        if (list <= 128) {
-            launchBlendViews<0, 128>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, accum, queue, stream);
-        } else {
-            launchBlendViews<0, 256>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, accum, queue, stream);
-        }
+        if (list <= 64) go(D0{}, L64{}); else if (list <= 128) go(D0{}, L128{}); else go(D0{}, L256{});
     } else {
-        if (list <= 128) {
-            launchBlendViews<1, 128>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, accum, queue, stream);
-        } else {
-            launchBlendViews<1, 256>(persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots,
-                                     proj, lastCount, finalT, accum, queue, stream);
-        }
+        if (list <= 64) go(D1{}, L64{}); else if (list <= 128) go(D1{}, L128{}); else go(D1{}, L256{});
     }
 }
 
@@ -846,12 +848,13 @@ void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint
     };
     using D0 = std::integral_constant<uint32_t, 0>;
     using D1 = std::integral_constant<uint32_t, 1>;
+    using L64 = std::integral_constant<uint32_t, 64>;
     using L128 = std::integral_constant<uint32_t, 128>;
     using L256 = std::integral_constant<uint32_t, 256>;
     if (f.depthMode == 0) {
-        if (list <= 128) launch(D0{}, L128{}); else launch(D0{}, L256{});
+        if (list <= 64) launch(D0{}, L64{}); else if (list <= 128) launch(D0{}, L128{}); else launch(D0{}, L256{});
     } else {
-        if (list <= 128) launch(D1{}, L128{}); else launch(D1{}, L256{});
+        if (list <= 64) launch(D1{}, L64{}); else if (list <= 128) launch(D1{}, L128{}); else launch(D1{}, L256{});
     }
 }
 

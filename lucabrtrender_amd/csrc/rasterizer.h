@@ -106,7 +106,7 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:105-118)
     bool timeStages = false;
     float dilation = 0.3F;
     bool filter3d = false;
-    uint32_t blendBatch = 256;          // records each quadrant's LDS list holds: 128 or 256
+    uint32_t blendBatch = 256;          // records each quadrant's LDS list holds: 64, 128 or 256
     uint32_t blendSegment = 0;          // pairs a quadrant's wave walks before its tile is split (0: never)
     bool keepWalkState = false;         // keep each pixel's last contributor and final transmittance (debug)
     bool heaviestTilesFirst = true;     // launch the tiles with the most pairs first

@@ -981,7 +981,7 @@ Status TileRasterizer::autotuneViews(std::span<const View> views, std::span<cons
     RenderSettings best = settings;
     double bestMs = -1.0;
     for (const uint32_t tile : {8u, 16u, 32u})
-    for (const uint32_t list : {128u, 256u}) {
+    for (const uint32_t list : {64u, 128u, 256u}) {
         for (const bool persistent : {false, true}) {
             for (const bool graph : {false, true}) {
                 RenderSettings trial = settings;

@@ -120,7 +120,7 @@ class RenderSettings:
     time_stages: bool = False
     dilation: float = 0.3
     filter3d: bool = False
-    blend_batch: int = 256          # records each 8x8 quadrant's LDS list holds: 128 or 256
+    blend_batch: int = 256          # records each 8x8 quadrant's LDS list holds: 64, 128 or 256
     tile_order: int = 1             # 0 tile index order, 1 heaviest tiles first (the default: shorter tail)
     persistent_blend: bool = False  # batches: a persistent blend taking quadrants from a queue
     use_graph: bool = False         # batches: capture the launch sequence once and replay it

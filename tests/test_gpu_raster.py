@@ -219,7 +219,7 @@ def test_errors_carry_the_reference_codes(rast, cloud):
     assert rast.debug(_abi.DEBUG_COUNTS)[2] == gpu["counts"][2]
 
 
-@pytest.mark.parametrize("batch,order", [(128, 0), (128, 1), (256, 0), (256, 1)])
+@pytest.mark.parametrize("batch,order", [(64, 0), (64, 1), (128, 0), (128, 1), (256, 0), (256, 1)])
 def test_every_batch_depth_and_tile_order_draws_the_same_frame(rast, cloud, batch, order):
     """Batch depth and launch order are performance parameters only."""
     cam, _ = CAMERAS["perspective"]

@@ -50,8 +50,10 @@ def _batch(rast, cams, sizes, inst, settings):
 def test_each_view_is_the_single_view_frame(cloud, persistent, graph, depth_mode, tile):
     rast = raster.TileRasterizer()
     inst = [raster.SplatInstance(cloud)]
+    # 8-pixel tiles with 64-record lists, the smallest of each.
     settings = raster.RenderSettings(persistent_blend=persistent, use_graph=graph, depth_mode=depth_mode,
-                                     background=(0.1, 0.0, 0.2, 0.3), tile_size=tile)
+                                     background=(0.1, 0.0, 0.2, 0.3), tile_size=tile,
+                                     blend_batch=64 if tile == 8 else 256)
     cams = [_cam(k, len(SIZES)) for k in range(len(SIZES))]
     out, stats = _batch(rast, cams, SIZES, inst, settings)
     total = 0
@@ -105,7 +107,7 @@ def test_autotune_picks_one_of_its_candidates(cloud):
     inst = [raster.SplatInstance(cloud)]
     views = [raster.View(_cam(k, 4), raster.RenderTargets(*SIZES[k])) for k in range(4)]
     best, ms = rast.autotune_views(views, inst, raster.RenderSettings())
-    assert best.blend_batch in (128, 256) and best.tile_size in (8, 16, 32) and ms > 0.0
+    assert best.blend_batch in (64, 128, 256) and best.tile_size in (8, 16, 32) and ms > 0.0
     out = rast.render_views(views, inst, best)
     assert out.pairs > 0
 
