@@ -35,12 +35,13 @@ def log(*a):
 
 
 def sort_pass_bytes(pairs, passes, key_bits):
-    """HBM bytes of a sort's passes: each reads and writes its pairs. A key of at
-    most 40 bits leaves the first pass as u32 (12 bytes in, 8 out; 8 each way
-    after), a wider one moves 12 bytes each way every pass."""
+    """HBM bytes of a sort's passes: each reads and writes its pairs. A key at
+    most 32 bits wider than its first digit (ceil(key bits / passes) bits) leaves
+    the first pass as u32 (12 bytes in, 8 out; 8 each way after), a wider one
+    moves 12 bytes each way every pass."""
     if passes == 0:
         return 0
-    if key_bits <= 40:
+    if key_bits <= 32 + -(-key_bits // passes):
         return pairs * (12 + 8 + 16 * (passes - 1))
     return pairs * 24 * passes
 
@@ -551,6 +552,15 @@ def sort_stress_leg(torch, raster, scenes, _abi, reps):
     defined, defined_passes = defined_sort_bytes(pairs, tiles)
     pass_bytes = sort_pass_bytes(pairs, passes, st.key_bits)
     moved = pairs * 8 + pass_bytes   # the histogram reads every key once
+    # The sort at each digit width the library offers (the default above).
+    by_radix = {}
+    for radix in (8, 9, 10):
+        rs = raster.RenderSettings(**{**settings.__dict__, "radix_bits": radix})
+        acc_r = np.zeros(8)
+        for _ in range(reps):
+            rst = rast.render(cfg.camera, inst, rs, targets)
+            acc_r += rast.debug(_abi.DEBUG_KERNEL_MS)
+        by_radix[str(radix)] = {"passes": rst.sort_passes, "sort_ms": round(float(acc_r[3] + acc_r[4]) / reps, 4)}
     return {"workload": "sort-stress-6M-2160p", "splats": n, "visible": visible, "pairs": pairs,
             "sort_passes": passes, "key_bits": st.key_bits, "sort_ms": round(sort_ms, 4),
             "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3),
@@ -561,6 +571,7 @@ def sort_stress_leg(torch, raster, scenes, _abi, reps):
             "moved_hbm_frac": round(moved / (sort_ms * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
             "passes_ms": round(float(ms[4]), 4),
             "passes_moved_hbm_frac": round(pass_bytes / (float(ms[4]) * 1e-3) / 1e9 / HBM_PEAK_GBS, 4),
+            "sort_ms_by_radix": by_radix,
             "duplicate_ms": round(float(ms[2]), 4), "blend_ms": round(float(ms[6]), 4),
             "gpu_ms_per_frame": round(float(ms[7]), 4), "wall_ms_per_frame": round(wall * 1e3, 4)}
 

@@ -147,6 +147,8 @@ typedef struct {
                                      always keeps them; otherwise they cost the blend an instruction a pair) */
     uint32_t tile_size;           /* pixels a square tile's side: 8, 16 or 32 (0, the default: 16, the
                                      reference's frame.slang:31). Differentiable frames take 16 only */
+    uint32_t radix_bits;          /* the sort's widest digit: 8, 9 or 10 bits (0, the default: 9 where that
+                                     saves a pass, else 8); the sorted pairs are the same */
 } lrt_render_settings;
 
 /* render::RenderTargets (TileRasterizer.h:129-134): device images, bottom row
@@ -365,9 +367,9 @@ lrt_status lrt_raster_render_views(lrt_context* ctx, const lrt_view* views, uint
 
 /* The autotuner: times the batch under each launch choice this build offers
  * (tile size 8/16/32, list length 64/128/256, grid or persistent blend, graph
- * replay or not) and writes the fastest into *settings (its other fields are
- * kept). ms (host, may be NULL) receives the winner's batch time. The radix
- * width (8 bits) is fixed in this build. */
+ * replay or not), then the sort's digit width (8/9/10 bits) under the fastest,
+ * and writes the winner into *settings (its other fields are kept). ms (host,
+ * may be NULL) receives the winner's batch time. */
 lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t view_count,
                               const lrt_splat_instance* instances, uint32_t instance_count,
                               lrt_render_settings* settings, float* ms);
@@ -486,12 +488,19 @@ lrt_status lrt_prefix_sum_static(const uint32_t* in, uint32_t* out, uint32_t* to
 
 /* gpu::RadixSort::sort (modules/gpu/include/lrt/gpu/algo/RadixSort.h:56-57):
  * stable sort of count (key, value) pairs by the key's low key_bits bits
- * (1..64), eight bits a pass. Keys are u64 here (the reference splits them in
+ * (1..64), ceil(key_bits / 8) passes of at most eight bits. Keys are u64 here (the reference splits them in
  * keysLo/keysHi). scratch_* are same-sized buffers the passes alternate with;
  * the result is left in keys/values, or in scratch_* when *result_in_scratch is
  * set (an odd pass count; the reference copies back instead). Queued, not waited. */
 lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, uint64_t* scratch_keys,
                           uint32_t* scratch_values, uint32_t count, uint32_t key_bits, int32_t* result_in_scratch);
+
+/* lrt_radix_sort with digits up to radix_bits (8..10) wide: ceil(key_bits /
+ * radix_bits) passes, the same sorted result (no counterpart in the reference,
+ * whose digits are 8 bits). */
+lrt_status lrt_radix_sort_digits(lrt_context* ctx, uint64_t* keys, uint32_t* values, uint64_t* scratch_keys,
+                                 uint32_t* scratch_values, uint32_t count, uint32_t key_bits, uint32_t radix_bits,
+                                 int32_t* result_in_scratch);
 
 /* Waits for everything queued on the context's stream. */
 lrt_status lrt_context_synchronize(lrt_context* ctx);

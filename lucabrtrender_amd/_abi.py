@@ -98,7 +98,7 @@ class RenderSettingsC(C.Structure):
         ("linearise", C.c_int32), ("time_stages", C.c_int32),
         ("dilation", C.c_float), ("filter3d", C.c_int32), ("blend_batch", C.c_uint32), ("tile_order", C.c_int32),
         ("persistent_blend", C.c_int32), ("use_graph", C.c_int32), ("blend_segment", C.c_uint32),
-        ("keep_walk_state", C.c_int32), ("tile_size", C.c_uint32),
+        ("keep_walk_state", C.c_int32), ("tile_size", C.c_uint32), ("radix_bits", C.c_uint32),
     ]
 
 
@@ -226,6 +226,8 @@ SIGNATURES = {
     "lrt_prefix_sum_static": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32, C.c_void_p]),
     "lrt_radix_sort": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
                                  C.c_uint32, C.POINTER(C.c_int32)]),
+    "lrt_radix_sort_digits": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_void_p, C.c_uint32,
+                                        C.c_uint32, C.c_uint32, C.POINTER(C.c_int32)]),
     "lrt_context_synchronize": (C.c_int, [C.c_void_p]),
     "lrt_copy_to_host": (C.c_int, [C.c_void_p, C.c_void_p, C.c_void_p, C.c_size_t]),
     "lrt_lod_build_settings_default": (None, [C.POINTER(LodBuildSettingsC)]),

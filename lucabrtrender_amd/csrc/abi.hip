@@ -76,6 +76,7 @@ lrt::RenderSettings toSettings(const lrt_render_settings& s) {
     out.persistentBlend = s.persistent_blend != 0;
     out.useGraph = s.use_graph != 0;
     out.tileSize = s.tile_size == 0 ? 16u : s.tile_size;
+    out.radixBits = s.radix_bits;
     return out;
 }
 
@@ -290,6 +291,7 @@ void lrt_render_settings_default(lrt_render_settings* out) {
     out->persistent_blend = 0;
     out->use_graph = 0;
     out->tile_size = s.tileSize;
+    out->radix_bits = s.radixBits;
 }
 
 void lrt_splat_edit_default(lrt_splat_edit* out) {
@@ -509,6 +511,7 @@ lrt_status lrt_autotune_views(lrt_context* ctx, const lrt_view* views, uint32_t 
     settings->persistent_blend = best.persistentBlend ? 1 : 0;
     settings->use_graph = best.useGraph ? 1 : 0;
     settings->tile_size = best.tileSize;
+    settings->radix_bits = best.radixBits;
     if (ms != nullptr) *ms = bestMs;
     return finish(ctx, s);
 }
@@ -648,6 +651,17 @@ lrt_status lrt_radix_sort(lrt_context* ctx, uint64_t* keys, uint32_t* values, ui
     bool alt = false;
     const lrt::Status s =
         ctx->raster.radixSort(keys, values, scratch_keys, scratch_values, count, key_bits, &alt);
+    if (result_in_scratch != nullptr) *result_in_scratch = alt ? 1 : 0;
+    return finish(ctx, s);
+}
+
+lrt_status lrt_radix_sort_digits(lrt_context* ctx, uint64_t* keys, uint32_t* values, uint64_t* scratch_keys,
+                                 uint32_t* scratch_values, uint32_t count, uint32_t key_bits, uint32_t radix_bits,
+                                 int32_t* result_in_scratch) {
+    if (ctx == nullptr) return LRT_INVALID_ARGUMENT;
+    bool alt = false;
+    const lrt::Status s =
+        ctx->raster.radixSort(keys, values, scratch_keys, scratch_values, count, key_bits, &alt, radix_bits);
     if (result_in_scratch != nullptr) *result_in_scratch = alt ? 1 : 0;
     return finish(ctx, s);
 }

@@ -96,32 +96,45 @@ struct TileDepthLayout {
     uint32_t depthBits = 24;
 };
 
-// A TileDepth sort of at most kNarrowMaxBits key bits leaves its keys as u32
-// (in the u64 buffers' first half-words): the virtual key shifted right by
-// narrowShift(keyBits), so the tile id is the stored key >> (depthBits -
-// narrowShift(keyBits)). Wider TileDepth sorts and Plain sorts keep u64 keys.
-constexpr uint32_t kNarrowMaxBits = 40;
-constexpr bool sortNarrows(KeyMode mode, uint32_t keyBits) {
-    return mode == KeyMode::TileDepth && keyBits <= kNarrowMaxBits;
-}
+// A sort's digits. radixBits (8, 9 or 10; 0 picks 9 where that saves a pass
+// and 8 otherwise) bounds a digit: the key takes ceil(keyBits / radixBits)
+// passes of keyBits / passes bits, the first keyBits % passes of them one bit
+// wider (33 bits: 9 + 8 + 8 + 8 at radix 9, 7 + 7 + 7 + 6 + 6 at radix 8).
+// Pass p's digit is bits [shift[p], shift[p] + width[p]) of the key; its digit
+// tables hold 2^max(8, width[p]) entries, at bins[p] of a row of totalBins.
+constexpr uint32_t kMaxSortPasses = 8;
+struct SortPlan {
+    uint32_t passes = 0, totalBins = 0;
+    uint32_t width[kMaxSortPasses] = {}, shift[kMaxSortPasses] = {}, bins[kMaxSortPasses] = {};
+};
+SortPlan sortPlan(uint32_t keyBits, uint32_t radixBits);
+uint32_t sortPasses(uint32_t keyBits, uint32_t radixBits = 0);
+
+// A TileDepth sort whose key exceeds 32 bits by no more than its first digit
+// leaves its keys as u32 (in the u64 buffers' first half-words): the virtual
+// key shifted right by narrowShift(keyBits), the bits that first pass consumes,
+// so the tile id is the stored key >> (depthBits - narrowShift(keyBits)).
+// Wider TileDepth sorts and Plain sorts keep u64 keys.
+bool sortNarrows(KeyMode mode, uint32_t keyBits, uint32_t radixBits = 0);
 constexpr uint32_t narrowShift(uint32_t keyBits) { return keyBits > 32 ? keyBits - 32 : 0u; }
 
 struct SortWorkspace {
-    uint32_t* histogram;   // one row of passes * 256 digit counts per histogram workgroup
-    uint32_t* offsets;     // passes * 256
-    uint32_t* status;      // passes * partitions * 256
+    uint32_t* histogram;   // one row of the plan's totalBins digit counts per histogram workgroup
+    uint32_t* offsets;     // totalBins
+    uint32_t* status;      // partitions * totalBins
     uint32_t* error;       // 1
 };
 uint32_t sortPartitions(uint32_t n);
-uint32_t sortPasses(uint32_t keyBits);
-// Workspace words for a sort of capacity n on keyBits bits, and its layout in
-// `work` (which holds at least that many words).
-uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits);
-SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32_t* error);
-// Stable LSD radix sort, eight bits a pass, one onesweep launch a pass. The
-// result is in (keys, values) after an even number of passes and in
-// (keysAlt, valuesAlt) after an odd one; returns true for the latter. The
-// result's keys are u32 when sortNarrows(mode, keyBits) (see above).
+// Workspace words for a sort of capacity n on keyBits bits (up to maxKeyBits
+// bits for the second form: the most any key width up to it needs), and its
+// layout in `work` (which holds at least that many words).
+uint64_t sortWorkspaceWords(uint32_t n, uint32_t keyBits, uint32_t radixBits = 0);
+uint64_t sortWorkspaceWordsUpTo(uint32_t n, uint32_t maxKeyBits, uint32_t radixBits = 0);
+SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32_t* error, uint32_t radixBits = 0);
+// Stable LSD radix sort, one onesweep launch a pass (digits as sortPlan lays
+// them out). The result is in (keys, values) after an even number of passes
+// and in (keysAlt, valuesAlt) after an odd one; returns true for the latter.
+// The result's keys are u32 when sortNarrows(mode, keyBits, radixBits).
 // afterHistogram (may be null) is recorded between the histogram and the passes.
 // alsoZero's first alsoZeroWords words are zeroed before the passes (by the
 // histogram launch, or by a fill when there is nothing to sort).
@@ -130,7 +143,8 @@ SortWorkspace sortWorkspace(uint32_t* work, uint32_t n, uint32_t keyBits, uint32
 bool sortPairs(uint64_t* keys, uint32_t* values, uint64_t* keysAlt, uint32_t* valuesAlt, uint32_t n,
                KeyMode mode, uint32_t bitLo, uint32_t keyBits, const SortWorkspace& ws, hipStream_t stream,
                hipEvent_t afterHistogram = nullptr, const uint32_t* nDevice = nullptr,
-               TileDepthLayout depth = {}, uint32_t* alsoZero = nullptr, uint64_t alsoZeroWords = 0);
+               TileDepthLayout depth = {}, uint32_t* alsoZero = nullptr, uint64_t alsoZeroWords = 0,
+               uint32_t radixBits = 0);
 
 // ---- load: raw splat records into packed splats --------------------------------
 // The decode's view of an io::SplatEncoding: field offsets in floats, modes

@@ -113,6 +113,7 @@ struct RenderSettings {   // render::RenderSettings (TileRasterizer.h:105-118)
     bool persistentBlend = false;       // batches: a persistent blend taking quadrants from a queue
     bool useGraph = false;              // batches: replay the launch sequence as a captured hipGraph
     uint32_t tileSize = 16;             // pixels a square tile's side: 8, 16 (the reference's) or 32
+    uint32_t radixBits = 0;             // the sort's widest digit: 8, 9 or 10 (0: 9 where it saves a pass, else 8)
 };
 
 struct RenderTargets {   // caller-owned device images, bottom row first
@@ -232,10 +233,11 @@ public:
     // ms of the blend backward and of the preprocess backward.
     Status backward(const float4* gradColour, const kern::DiffGrads& grads, float* millis);
 
-    // PrefixSum::apply and RadixSort::sort over caller buffers, on this stream.
+    // PrefixSum::apply and RadixSort::sort over caller buffers, on this stream
+    // (the sort's digits at most radixBits wide: 8, the reference's, to 10).
     Status prefixSum(const uint32_t* in, uint32_t* out, uint32_t* total, uint32_t count);
     Status radixSort(uint64_t* keys, uint32_t* values, uint64_t* scratchKeys, uint32_t* scratchValues,
-                     uint32_t count, uint32_t keyBits, bool* resultInScratch);
+                     uint32_t count, uint32_t keyBits, bool* resultInScratch, uint32_t radixBits = 8);
 
     // What the last render left (LRT_DEBUG_* of lrt_hip.h).
     Status debugCopy(int which, void* host, size_t bytes, size_t* needed);
@@ -257,7 +259,7 @@ public:
 
 private:
     Status reserveSplats(uint32_t count);
-    Status reservePairs(uint32_t count, uint32_t keyBits, bool training = false);
+    Status reservePairs(uint32_t count, uint32_t keyBits, bool training = false, uint32_t radixBits = 0);
     Status reserveTargets(uint32_t width, uint32_t height, uint32_t tiles, bool training);
     Status checkFrame(const RenderSettings& settings, const RenderTargets& targets) const;
     // A batch of views of the instances, or (diff non-null, one instance giving

@@ -122,7 +122,7 @@ Status TileRasterizer::reserveSplats(uint32_t count) {
     return {};
 }
 
-Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits, bool training) {
+Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits, bool training, uint32_t radixBits) {
     Status s;
     if (count > pairCapacity_) {
         const uint64_t n = std::min<uint64_t>(grow(pairCapacity_, count), kMaxPairs);
@@ -136,7 +136,7 @@ Status TileRasterizer::reservePairs(uint32_t count, uint32_t keyBits, bool train
         if (!(s = pairSlot_.reserve(uint64_t{pairCapacity_} * 4, "pair.slot")).ok()) return s;
         if (!(s = sortedSlots_.reserve(uint64_t{pairCapacity_} * 4, "pair.sortedSlots")).ok()) return s;
     }
-    const uint64_t words = kern::sortWorkspaceWords(pairCapacity_, std::max<uint32_t>(keyBits, 40));
+    const uint64_t words = kern::sortWorkspaceWordsUpTo(pairCapacity_, std::max<uint32_t>(keyBits, 40), radixBits);
     if (words > sortWorkWords_) {
         if (!(s = sortWork_.reserve(words * 4, "pair.sortWorkspace")).ok()) return s;
         sortWorkWords_ = words;
@@ -168,6 +168,9 @@ Status TileRasterizer::checkFrame(const RenderSettings& settings, const RenderTa
     }
     if (settings.tileSize != 8 && settings.tileSize != 16 && settings.tileSize != 32) {
         return Status::error(ErrorCode::InvalidArgument, "a tile size other than 8, 16 or 32 pixels");
+    }
+    if (settings.radixBits != 0 && (settings.radixBits < 8 || settings.radixBits > 10)) {
+        return Status::error(ErrorCode::InvalidArgument, "a sort digit other than 8, 9 or 10 bits (or 0)");
     }
     const uint32_t shift = tileShiftOf(settings.tileSize);
     if (tilesAlong(settings.width, shift) > 0xFFFFu || tilesAlong(settings.height, shift) > 0xFFFFu) {
@@ -411,27 +414,29 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     }
     const uint32_t keyBits = bitsFor(tiles) + layout.depthBits;
     stats.keyBits = keyBits;
-    stats.sortPasses = kern::sortPasses(keyBits);
+    stats.sortPasses = kern::sortPasses(keyBits, settings.radixBits);
     // Duplicate into 64-bit keys, in slot order (again, if the early run lacked room).
     if (!early || pairs > pairCapacity_) {
         // The early run must be done before its buffers may be replaced.
         if (early && !(s = check(hipStreamSynchronize(stream_), "duplicate")).ok()) return s;
-        if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) return s;
+        if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training, settings.radixBits)).ok()) return s;
         kern::fill32(counters + 2, 0u, 1, stream_);   // the early run's dropped pairs
         duplicateInto();
-    } else if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training)).ok()) {
+    } else if (!(s = reservePairs(std::max(pairs, 1u), keyBits, training, settings.radixBits)).ok()) {
         return abandon(s);
     }
 
     // One stable sort on (tile, depth >> 7), then the ranges.
-    const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
+    const kern::SortWorkspace ws =
+        kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4, settings.radixBits);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairs, kern::KeyMode::TileDepth, 0, keyBits, ws,
                                        stream_, timed ? events_[5] : nullptr, nullptr, layout,
-                                       reinterpret_cast<uint32_t*>(ranges_.as<uint2>()), uint64_t{tiles} * 2);
+                                       reinterpret_cast<uint32_t*>(ranges_.as<uint2>()), uint64_t{tiles} * 2,
+                                       settings.radixBits);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
-    lastNarrow_ = kern::sortNarrows(kern::KeyMode::TileDepth, keyBits);
+    lastNarrow_ = kern::sortNarrows(kern::KeyMode::TileDepth, keyBits, settings.radixBits);
     lastTileShift_ = layout.depthBits - kern::narrowShift(keyBits);
     if (lastNarrow_) {
         kern::tileRangesNarrow(keys_[sorted].as<uint32_t>(), lastTileShift_, pairs, tiles, ranges_.as<uint2>(),
@@ -583,13 +588,15 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
     // The pair count (counters[0], from the scan) stays on the device: the
     // sort and the ranges read it, launched for the capacity.
     const uint32_t keyBits = bitsFor(tiles) + viewsLayout_.depthBits;
-    const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4);
+    const kern::SortWorkspace ws =
+        kern::sortWorkspace(sortWork_.as<uint32_t>(), pairCapacity_, keyBits, counters + 4, settings.radixBits);
     const bool inAlt = kern::sortPairs(keys_[0].as<uint64_t>(), values_[0].as<uint32_t>(), keys_[1].as<uint64_t>(),
                                        values_[1].as<uint32_t>(), pairCapacity_, kern::KeyMode::TileDepth, 0,
-                                       keyBits, ws, stream, timed ? events_[5] : nullptr, counters + 0, viewsLayout_);
+                                       keyBits, ws, stream, timed ? events_[5] : nullptr, counters + 0, viewsLayout_,
+                                       nullptr, 0, settings.radixBits);
     mark(6);
     const int sorted = inAlt ? 1 : 0;
-    lastNarrow_ = kern::sortNarrows(kern::KeyMode::TileDepth, keyBits);
+    lastNarrow_ = kern::sortNarrows(kern::KeyMode::TileDepth, keyBits, settings.radixBits);
     lastTileShift_ = viewsLayout_.depthBits - kern::narrowShift(keyBits);
     if (lastNarrow_) {
         kern::tileRangesNarrow(keys_[sorted].as<uint32_t>(), lastTileShift_, pairCapacity_, tiles, ranges_.as<uint2>(),
@@ -679,6 +686,9 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
     const bool training = diff != nullptr;
     if (settings.tileSize != 8 && settings.tileSize != 16 && settings.tileSize != 32) {
         return Status::error(ErrorCode::InvalidArgument, "a tile size other than 8, 16 or 32 pixels");
+    }
+    if (settings.radixBits != 0 && (settings.radixBits < 8 || settings.radixBits > 10)) {
+        return Status::error(ErrorCode::InvalidArgument, "a sort digit other than 8, 9 or 10 bits (or 0)");
     }
     if (training && settings.tileSize != kTile) {
         return Status::error(ErrorCode::Unsupported, "a differentiable frame with tiles other than 16 pixels");
@@ -801,7 +811,7 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
     // the duplication, which the host sees after the wait: then the capacity
     // grows to the count and the batch runs again.
     for (int attempt = 0; attempt < 2; ++attempt) {
-        if (!(s = reservePairs(std::max(pairCapacity_, 1u), keyBits, training)).ok()) return s;
+        if (!(s = reservePairs(std::max(pairCapacity_, 1u), keyBits, training, settings.radixBits)).ok()) return s;
         if (settings.useGraph) {
             // A captured graph holds its launches' sizes and every buffer
             // address: any change of either needs a new capture.
@@ -810,6 +820,7 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
                               std::to_string(settings.blendBatch) + "/" + std::to_string(settings.persistentBlend) +
                               "/" + std::to_string(settings.keepWalkState) + "/" + std::to_string(training) + "/" +
                               std::to_string(settings.depthMode) + "/" + std::to_string(settings.tileSize) + "/" +
+                              std::to_string(settings.radixBits) + "/" +
                               std::to_string(viewsLayout_.depthMin) +
                               "/" + std::to_string(viewsLayout_.depthBits);
             const auto addr = [&key](const void* p) { key += "/" + std::to_string(reinterpret_cast<uintptr_t>(p)); };
@@ -872,7 +883,7 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
         }
         if (hostCounters_[2] == 0) break;
         if (attempt == 1) return Status::error(ErrorCode::InternalError, "the duplication overran its pairs");
-        if (!(s = reservePairs(pairs, keyBits, training)).ok()) return s;
+        if (!(s = reservePairs(pairs, keyBits, training, settings.radixBits)).ok()) return s;
     }
     if (hostCounters_[10] != 0) {
         return Status::error(ErrorCode::InternalError, "a drawn splat's depth key fell outside the batch's planes");
@@ -903,7 +914,7 @@ Status TileRasterizer::runViews(std::span<const View> views, std::span<const Spl
         stats.visible = lastVisible_;
         stats.pairs = lastPairs_;
         stats.keyBits = bitsFor(static_cast<uint32_t>(tiles)) + viewsLayout_.depthBits;
-        stats.sortPasses = kern::sortPasses(stats.keyBits);
+        stats.sortPasses = kern::sortPasses(stats.keyBits, settings.radixBits);
         stats.totalMs = std::chrono::duration<double, std::milli>(wallEnd - wallStart).count();
         *statsOut = stats;
     }
@@ -1004,6 +1015,25 @@ Status TileRasterizer::autotuneViews(std::span<const View> views, std::span<cons
                     best = trial;
                 }
             }
+        }
+    }
+    // The sort's digit width under the fastest launch choices.
+    const RenderSettings launch = best;
+    for (const uint32_t radix : {8u, 9u, 10u}) {
+        RenderSettings trial = launch;
+        trial.radixBits = radix;
+        Status s = renderViews(views, instances, trial, nullptr);
+        if (!s.ok()) return s;
+        double fastest = -1.0;
+        for (int rep = 0; rep < 3; ++rep) {
+            const auto t0 = std::chrono::steady_clock::now();
+            if (!(s = renderViews(views, instances, trial, nullptr)).ok()) return s;
+            const double t = std::chrono::duration<double, std::milli>(std::chrono::steady_clock::now() - t0).count();
+            if (fastest < 0.0 || t < fastest) fastest = t;
+        }
+        if (fastest < bestMs) {
+            bestMs = fastest;
+            best = trial;
         }
     }
     settings = best;
@@ -1226,23 +1256,25 @@ Status TileRasterizer::decodeCloud(const io::RawSplats& raw, uint32_t maxDegree,
 }
 
 Status TileRasterizer::radixSort(uint64_t* keys, uint32_t* values, uint64_t* scratchKeys, uint32_t* scratchValues,
-                                 uint32_t count, uint32_t keyBits, bool* resultInScratch) {
+                                 uint32_t count, uint32_t keyBits, bool* resultInScratch, uint32_t radixBits) {
     if (!created_) return Status::error(ErrorCode::InvalidArgument, "the rasteriser was not created");
     if (keyBits < 1 || keyBits > 64) return Status::error(ErrorCode::InvalidArgument, "keyBits must be 1..64");
+    if (radixBits < 8 || radixBits > 10) return Status::error(ErrorCode::InvalidArgument, "radixBits must be 8..10");
     if (count > kMaxPairs) return Status::error(ErrorCode::Unsupported, "more than 2^30 - 1 pairs");
     if (count > 0 && (keys == nullptr || values == nullptr || scratchKeys == nullptr || scratchValues == nullptr)) {
         return Status::error(ErrorCode::InvalidArgument, "null buffer");
     }
     Status s = check(hipSetDevice(device_), "hipSetDevice");
     if (!s.ok()) return s;
-    const uint64_t words = kern::sortWorkspaceWords(count, keyBits);
+    const uint64_t words = kern::sortWorkspaceWords(count, keyBits, radixBits);
     if (words > sortWorkWords_) {
         if (!(s = sortWork_.reserve(words * 4, "pair.sortWorkspace")).ok()) return s;
         sortWorkWords_ = words;
     }
-    const kern::SortWorkspace ws = kern::sortWorkspace(sortWork_.as<uint32_t>(), count, keyBits, counters_.as<uint32_t>() + 4);
+    const kern::SortWorkspace ws =
+        kern::sortWorkspace(sortWork_.as<uint32_t>(), count, keyBits, counters_.as<uint32_t>() + 4, radixBits);
     const bool alt = kern::sortPairs(keys, values, scratchKeys, scratchValues, count, kern::KeyMode::Plain, 0, keyBits,
-                                     ws, stream_);
+                                     ws, stream_, nullptr, nullptr, {}, nullptr, 0, radixBits);
     if (resultInScratch != nullptr) *resultInScratch = alt;
     return check(hipGetLastError(), "radix sort");
 }

@@ -127,6 +127,7 @@ class RenderSettings:
     blend_segment: int = 0          # a tile with more pairs is split across waves and composited (0: never)
     keep_walk_state: bool = False   # keep each pixel's last contributor and final transmittance (debug)
     tile_size: int = 16             # pixels a square tile's side: 8, 16 or 32 (differentiable frames: 16)
+    radix_bits: int = 0             # the sort's widest digit: 8, 9 or 10 (0: 9 where that saves a pass, else 8)
 
     def to_c(self) -> _abi.RenderSettingsC:
         s = _abi.RenderSettingsC()
@@ -146,6 +147,7 @@ class RenderSettings:
         s.blend_segment = self.blend_segment
         s.keep_walk_state = 1 if self.keep_walk_state else 0
         s.tile_size = self.tile_size
+        s.radix_bits = self.radix_bits
         return s
 
 
@@ -583,7 +585,8 @@ class TileRasterizer:
         self._check(lib.lrt_autotune_views(self._ctx, self._views_c(views), len(views), _instances_c(instances),
                                            len(instances), C.byref(c), C.byref(ms)))
         return dataclasses.replace(settings, blend_batch=c.blend_batch, persistent_blend=bool(c.persistent_blend),
-                                   use_graph=bool(c.use_graph), tile_size=c.tile_size), float(ms.value)
+                                   use_graph=bool(c.use_graph), tile_size=c.tile_size,
+                                   radix_bits=c.radix_bits), float(ms.value)
 
     def render_diff(self, camera_or_projection, splats: DiffSplats, settings: RenderSettings,
                     targets: RenderTargets, object_to_world=IDENTITY) -> FrameStats:
@@ -660,11 +663,17 @@ class TileRasterizer:
         """PrefixSum::apply (PrefixSum.h:43-44) on torch uint32/int32 device tensors."""
         self._check(lib.lrt_prefix_sum(self._ctx, inp.data_ptr(), out.data_ptr(), total.data_ptr(), inp.numel()))
 
-    def radix_sort(self, keys, values, scratch_keys, scratch_values, key_bits: int) -> bool:
-        """RadixSort::sort (RadixSort.h:56-57); returns True when the result is in the scratch pair."""
+    def radix_sort(self, keys, values, scratch_keys, scratch_values, key_bits: int, radix_bits: int = 8) -> bool:
+        """RadixSort::sort (RadixSort.h:56-57), digits up to radix_bits (8..10) wide; returns True
+        when the result is in the scratch pair."""
         flag = C.c_int32()
-        self._check(lib.lrt_radix_sort(self._ctx, keys.data_ptr(), values.data_ptr(), scratch_keys.data_ptr(),
-                                       scratch_values.data_ptr(), keys.numel(), key_bits, C.byref(flag)))
+        if radix_bits == 8:
+            self._check(lib.lrt_radix_sort(self._ctx, keys.data_ptr(), values.data_ptr(), scratch_keys.data_ptr(),
+                                           scratch_values.data_ptr(), keys.numel(), key_bits, C.byref(flag)))
+        else:
+            self._check(lib.lrt_radix_sort_digits(self._ctx, keys.data_ptr(), values.data_ptr(),
+                                                  scratch_keys.data_ptr(), scratch_values.data_ptr(), keys.numel(),
+                                                  key_bits, radix_bits, C.byref(flag)))
         return bool(flag.value)
 
     def synchronize(self):

@@ -43,7 +43,7 @@ def run_gpu(cam, settings, instances, rast=None):
     targets = raster.RenderTargets(settings.width, settings.height)
     stats = rast.render(cam, inst, dataclasses.replace(settings, keep_walk_state=True), targets)
     torch.cuda.synchronize()
-    out = dict(stats=stats,
+    out = dict(stats=stats, radix_bits=settings.radix_bits,
                counts=rast.debug(_abi.DEBUG_COUNTS),
                touched=rast.debug(_abi.DEBUG_TOUCHED),
                offsets=rast.debug(_abi.DEBUG_OFFSETS),
@@ -88,7 +88,14 @@ def assert_bit_exact(gpu, ora):
     np.testing.assert_array_equal(gpu["ranges"], ora["ranges"])
     if gpu.get("stats") is not None and ora["pairs"] > 0:
         assert gpu["stats"].key_bits == expected_key_bits(ora), (gpu["stats"].key_bits, expected_key_bits(ora))
-        assert gpu["stats"].sort_passes == (gpu["stats"].key_bits + 7) // 8
+        assert gpu["stats"].sort_passes == expected_passes(gpu["stats"].key_bits, gpu.get("radix_bits", 0))
+
+
+def expected_passes(key_bits: int, radix_bits: int = 0) -> int:
+    """Sort passes: ceil(key bits / digit bits), the digit 9 bits by default
+    where that saves a pass over 8."""
+    radix = radix_bits or (9 if -(-key_bits // 9) < -(-key_bits // 8) else 8)
+    return -(-key_bits // radix)
 
 
 def bits_for(values: int) -> int:

@@ -73,6 +73,26 @@ def test_radix_sort_orders_keeps_ties_stable_and_loses_nothing(rast, count, key_
     assert np.array_equal(rv.cpu().numpy().view(np.uint32), want_v)
 
 
+@pytest.mark.parametrize("count,key_bits,radix_bits", [(777, 18, 9), (1_000_003, 27, 9), (100_000, 19, 10),
+                                                      (200_001, 30, 10), (3_000_001, 37, 10), (50_000, 64, 9),
+                                                      (2_000_000, 33, 9), (2_000_000, 33, 8)])
+@pytest.mark.parametrize("mode", range(5))
+def test_wider_digits_sort_the_same(rast, count, key_bits, radix_bits, mode):
+    """Digits of 9 and 10 bits (and 33 bits as 7 + 7 + 7 + 6 + 6): the same
+    stable order as numpy's, in ceil(key bits / digit bits) passes."""
+    import torch
+    keys, values = PT.sort_generate(count, mode, key_bits)
+    want_k, want_v = PT.stable_sorted(keys, values, key_bits)
+    k, v = _dev(keys), _dev(values)
+    k2, v2 = torch.empty_like(k), torch.empty_like(v)
+    in_scratch = rast.radix_sort(k, v, k2, v2, key_bits, radix_bits)
+    torch.cuda.synchronize()
+    assert in_scratch == (-(-key_bits // radix_bits) % 2 == 1)
+    rk, rv = (k2, v2) if in_scratch else (k, v)
+    assert np.array_equal(rk.cpu().numpy().view(np.uint64), want_k)
+    assert np.array_equal(rv.cpu().numpy().view(np.uint32), want_v)
+
+
 def test_sorting_ten_million_pairs_timed(rast):
     import torch
     keys, values = PT.sort_generate(10_000_000, 0, 32)

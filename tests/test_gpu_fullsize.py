@@ -19,9 +19,11 @@ pytestmark = pytest.mark.gpu
 CHUNK = 1 << 26
 
 
-def test_sort_stress_full_size_sort_properties():
+@pytest.mark.parametrize("radix", [0, 8])
+def test_sort_stress_full_size_sort_properties(radix):
+    """radix 0 sorts the 33-bit key as 9 + 8 + 8 + 8 bits, radix 8 as 7 + 7 + 7 + 6 + 6."""
     cfg = scenes.workload("sort-stress-6M-2160p")
-    s = raster.RenderSettings(width=cfg.width, height=cfg.height)
+    s = raster.RenderSettings(width=cfg.width, height=cfg.height, radix_bits=radix)
     rast = raster.TileRasterizer()
     targets = raster.RenderTargets(cfg.width, cfg.height)
     st = rast.render(cfg.camera, [raster.SplatInstance(cfg.scene.to_device())], s, targets)

@@ -270,6 +270,21 @@ def test_tile_sizes_bin_as_the_oracle_and_draw_the_16_pixel_frame(rast, cloud, n
         _image_checks(gpu, ora)
 
 
+@pytest.mark.parametrize("radix", [8, 9, 10])
+@pytest.mark.parametrize("tile", [8, 16])
+def test_sort_digit_widths_give_the_oracle_s_pairs(rast, radix, tile):
+    """The frame's sort at digits of up to 8, 9 and 10 bits: the same pairs and
+    ranges as the oracle's, bit for bit, in the passes the widths make."""
+    print(f"radix {radix} tile {tile}", flush=True)
+    cfg = scenes.workload("plain-fwd-65k-640x480", count=20000)
+    inst = [(cfg.scene, np.eye(4), raster.SplatEdit())]
+    s = raster.RenderSettings(width=cfg.width, height=cfg.height, tile_size=tile, radix_bits=radix)
+    gpu = H.run_gpu(cfg.camera, s, inst, rast)
+    ora = H.run_oracle(cfg.camera, s, inst)
+    H.assert_bit_exact(gpu, ora)
+    _image_checks(gpu, ora)
+
+
 def _heaviest_first(ranges):
     """Tiles by the bit length of their pair count, descending, index order within one length."""
     counts = (ranges[:, 1].astype(np.int64) - ranges[:, 0].astype(np.int64))
