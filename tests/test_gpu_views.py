@@ -66,6 +66,33 @@ def test_each_view_is_the_single_view_frame(cloud, persistent, graph, depth_mode
     assert stats.splats == cloud.count * len(SIZES)
 
 
+@pytest.mark.parametrize("graph", [False, True])
+@pytest.mark.parametrize("far", [60.0, float("inf")])
+def test_views_with_their_own_near_and_far_planes(cloud, graph, far):
+    """The batch's depth field spans the least near plane's key to the greatest
+    far plane's (all 24 bits when a far plane is infinite): views with planes of
+    their own (near planes that cut into the cloud, an orthographic view) each
+    draw their single-view frame, bit for bit."""
+    planes = [(0.5, 50.0, False), (0.01, 1e4 if far < 1e30 else far, False), (2.0, far, False), (4.5, 6.0, False),
+              (0.1, 30.0, True)]
+    cams = []
+    for k, (near, far, ortho) in enumerate(planes):
+        c = _cam(k, len(planes))
+        c.lens = raster.Lens(focal=5.0 if ortho else 30.0 + 3.0 * k, orthographic=ortho, near_z=near, far_z=far)
+        cams.append(c)
+    inst = [raster.SplatInstance(cloud)]
+    settings = raster.RenderSettings(use_graph=graph)
+    rast = raster.TileRasterizer()
+    out, stats = _batch(rast, cams, SIZES, inst, settings)
+    total = 0
+    for cam, size, (colour, depth) in zip(cams, SIZES, out):
+        c1, d1, pairs = _single(raster.TileRasterizer(), cam, size, inst, settings)
+        total += pairs
+        np.testing.assert_array_equal(colour.view(np.uint32), c1.view(np.uint32))
+        np.testing.assert_array_equal(depth.view(np.uint32), d1.view(np.uint32))
+    assert stats.pairs == total and total > 0
+
+
 def test_two_instances_and_the_batch_debug_buffers(cloud):
     rast = raster.TileRasterizer()
     o2w = np.diag([0.5, 0.5, 0.5, 1.0])
