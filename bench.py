@@ -641,6 +641,7 @@ def main():
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="CPU-baseline budget; 0 skips it")
     ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 64, 128 or 256")
     ap.add_argument("--tile-size", type=int, default=16, help="pixels a tile's side: 8, 16 (the reference's) or 32")
+    ap.add_argument("--sweeps", type=int, default=1, help="1: the blend-setting and tile-size comparisons; 0 skips them")
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="fwd+bwd views (1M splats, 720p) timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="sort-stress frames (6M splats, 2160p) timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="64-view batches (2M splats) timed; 0 skips it")
@@ -749,44 +750,45 @@ def main():
     stages["blend"]["TFLOP/s"] = round(walk_steps * 64 * WALK_FLOPS / (stages["blend"]["ms"] * 1e-3) / 1e12, 3)
     roofline = roofline_of(dominant, stages, passes)
     copy_gbs = copy_bandwidth_gbs(torch)
-    # The blend at each batch depth and launch order, for the record.
-    variants = {}
-    for batch in (64, 128, 256):
-        for order in (0, 1):
-            vs = raster.RenderSettings(**{**timed.__dict__, "blend_batch": batch, "tile_order": order})
+    # The blend at each batch depth and launch order, split blends and the frame
+    # at each tile size, for the record (--sweeps 0 skips them).
+    variants, tile_sizes = {}, {}
+    if args.sweeps:
+        for batch in (64, 128, 256):
+            for order in (0, 1):
+                vs = raster.RenderSettings(**{**timed.__dict__, "blend_batch": batch, "tile_order": order})
+                acc_v = np.zeros(8)
+                for _ in range(10):
+                    rast.render(proj, inst, vs, targets)
+                    acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
+                variants[f"batch{batch}_{'index_order' if order == 0 else 'heavy_first'}"] = round(float(acc_v[6] / 10), 4)
+        # Heavy tiles split across waves: the blend at each segment length (0: never split).
+        for segment in (0, 256, 512, 1024, 2048):
+            vs = raster.RenderSettings(**{**timed.__dict__, "blend_segment": segment})
             acc_v = np.zeros(8)
             for _ in range(10):
                 rast.render(proj, inst, vs, targets)
                 acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
-            variants[f"batch{batch}_{'index_order' if order == 0 else 'heavy_first'}"] = round(float(acc_v[6] / 10), 4)
-    # Heavy tiles split across waves: the blend at each segment length (0: never split).
-    for segment in (0, 256, 512, 1024, 2048):
-        vs = raster.RenderSettings(**{**timed.__dict__, "blend_segment": segment})
-        acc_v = np.zeros(8)
-        for _ in range(10):
-            rast.render(proj, inst, vs, targets)
-            acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
-        variants[f"segment{segment}"] = round(float(acc_v[6] / 10), 4)
-    # The frame at each tile size: the same images, different pair counts.
-    tile_sizes = {}
-    for tile in (8, 16, 32):
-        vs = raster.RenderSettings(**{**settings.__dict__, "tile_size": tile})
-        for _ in range(3):
-            rast.render(proj, inst, vs, targets)
-        torch.cuda.synchronize()
-        t1 = time.perf_counter()
-        for _ in range(30):
-            st = rast.render(proj, inst, vs, targets)
-        torch.cuda.synchronize()
-        tfps = 30 / (time.perf_counter() - t1)
-        acc_v = np.zeros(8)
-        for _ in range(10):
-            rast.render(proj, inst, raster.RenderSettings(**{**vs.__dict__, "time_stages": True}), targets)
-            acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
-        tile_sizes[str(tile)] = {"frames_per_s": round(tfps, 1), "pairs": int(st.pairs),
-                                 "stage_ms": {name: round(float(acc_v[k] / 10), 4) for k, name in enumerate(STAGES)}}
-    log("tile sizes: " + ", ".join(f"{t}: {v['frames_per_s']} frames/s, {v['pairs']} pairs"
-                                   for t, v in tile_sizes.items()))
+            variants[f"segment{segment}"] = round(float(acc_v[6] / 10), 4)
+        # The frame at each tile size: the same images, different pair counts.
+        for tile in (8, 16, 32):
+            vs = raster.RenderSettings(**{**settings.__dict__, "tile_size": tile})
+            for _ in range(3):
+                rast.render(proj, inst, vs, targets)
+            torch.cuda.synchronize()
+            t1 = time.perf_counter()
+            for _ in range(30):
+                st = rast.render(proj, inst, vs, targets)
+            torch.cuda.synchronize()
+            tfps = 30 / (time.perf_counter() - t1)
+            acc_v = np.zeros(8)
+            for _ in range(10):
+                rast.render(proj, inst, raster.RenderSettings(**{**vs.__dict__, "time_stages": True}), targets)
+                acc_v += rast.debug(_abi.DEBUG_KERNEL_MS)
+            tile_sizes[str(tile)] = {"frames_per_s": round(tfps, 1), "pairs": int(st.pairs),
+                                     "stage_ms": {name: round(float(acc_v[k] / 10), 4) for k, name in enumerate(STAGES)}}
+        log("tile sizes: " + ", ".join(f"{t}: {v['frames_per_s']} frames/s, {v['pairs']} pairs"
+                                       for t, v in tile_sizes.items()))
 
     fwd_bwd = None
     if args.fwd_bwd_steps > 0:
