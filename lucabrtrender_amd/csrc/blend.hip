@@ -195,7 +195,8 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
                                               float* __restrict__ finalT, float4* __restrict__ accum,
                                               WalkState* state = nullptr, float4* __restrict__ partialA = nullptr,
                                               float2* __restrict__ partialB = nullptr,
-                                              uint32_t* __restrict__ walkSteps = nullptr) {
+                                              uint32_t* __restrict__ walkSteps = nullptr,
+                                              const uint32_t* __restrict__ slotOfPair = nullptr) {
     static_assert(!(COMPOSITE && MODE != kFinish), "a composite walk is not split");
     // Pairs a lane stages a round: two, or one for a 64-record list.
     constexpr uint32_t kPerLane = LIST >= 2 * kLanes ? 2u : 1u;
@@ -223,7 +224,12 @@ __device__ __forceinline__ void blendQuadrant(BlendList<LIST>& list, const Frame
     const uint64_t below = (1ull << lane) - 1ull;
 
     // Round k's records sit in `staged` while round k + 1's are gathered.
-    auto slotAt = [&](uint32_t at) -> uint32_t { return at < range.y ? pairSlots[at] : 0xFFFFFFFFu; };
+    // A training frame's sort carried pair indices: each pair's slot is read through it.
+    auto slotAt = [&](uint32_t at) -> uint32_t {
+        if (at >= range.y) return 0xFFFFFFFFu;
+        const uint32_t v = pairSlots[at];
+        return slotOfPair != nullptr ? slotOfPair[v] : v;
+    };
     Staged staged[kPerLane];
 #pragma unroll
     for (uint32_t u = 0; u < kPerLane; ++u) staged[u] = gather(proj, slotAt(range.x + u * kLanes + lane), slots);
@@ -408,7 +414,8 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
                                                       float4* __restrict__ colour, float* __restrict__ depth,
                                                       uint32_t* __restrict__ lastCount,
                                                       float* __restrict__ finalT, float4* __restrict__ accum,
-                                                      uint32_t* __restrict__ walkSteps) {
+                                                      uint32_t* __restrict__ walkSteps,
+                                                      const uint32_t* __restrict__ slotOfPair) {
     const uint32_t tiles = f.tilesX * f.tilesY;
     const uint32_t qBits = quadBits(f.tileShift);
     const uint32_t quadrant = (blockIdx.x >> 3) & ((1u << qBits) - 1u);
@@ -420,7 +427,8 @@ __global__ __launch_bounds__(kLanes) void blendKernel(FrameParams f, uint32_t sl
     const uint2 rg = ranges[tile];
     blendQuadrant<DEPTH_MODE, LIST, COMPOSITE, kFinish, TRACK>(list, f, tile, quadrant, rg, rg.x, slots, pairSlots, proj,
                                                                underColour, underDepth, colour, depth, lastCount,
-                                                               finalT, accum, nullptr, nullptr, nullptr, walkSteps);
+                                                               finalT, accum, nullptr, nullptr, nullptr, walkSteps,
+                                                               slotOfPair);
 }
 
 // Every quadrant of every view of a batch (2^qBits quadrants a tile). As a
@@ -439,7 +447,8 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
                                                            const float4* __restrict__ proj,
                                                            uint32_t* __restrict__ lastCount,
                                                            float* __restrict__ finalT, float4* __restrict__ accum,
-                                                           uint32_t* __restrict__ queue) {
+                                                           uint32_t* __restrict__ queue,
+                                                           const uint32_t* __restrict__ slotOfPair) {
     __shared__ BlendList<LIST> list;
     for (uint32_t round = 0;; ++round) {
         uint32_t task, g, quadrant;
@@ -459,13 +468,11 @@ __global__ __launch_bounds__(kLanes) void blendViewsKernel(const FrameParams* __
         if (g < tiles) {
             const ViewInfo& v = views[viewOfTile(views, viewCount, g)];
             const uint2 rg = ranges[g];
-            blendQuadrant<DEPTH_MODE, LIST, false, kFinish, TRACK>(list, params[v.params], g - v.tileBase, quadrant, rg,
-                                                                   rg.x, slots,
-                                                   pairSlots, proj, nullptr, nullptr,
-                                                   reinterpret_cast<float4*>(v.colour), v.depth,
-                                            lastCount != nullptr ? lastCount + v.pixelBase : nullptr,
-                                            finalT != nullptr ? finalT + v.pixelBase : nullptr,
-                                            accum != nullptr ? accum + v.pixelBase : nullptr);
+            blendQuadrant<DEPTH_MODE, LIST, false, kFinish, TRACK>(
+                list, params[v.params], g - v.tileBase, quadrant, rg, rg.x, slots, pairSlots, proj, nullptr, nullptr,
+                reinterpret_cast<float4*>(v.colour), v.depth, lastCount != nullptr ? lastCount + v.pixelBase : nullptr,
+                finalT != nullptr ? finalT + v.pixelBase : nullptr, accum != nullptr ? accum + v.pixelBase : nullptr,
+                nullptr, nullptr, nullptr, nullptr, slotOfPair);
         }
         if constexpr (!PERSISTENT) return;
     }
@@ -745,7 +752,7 @@ template <uint32_t DEPTH_MODE, bool COMPOSITE>
 void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint32_t* order, const uint2* ranges,
                  const uint32_t* pairSlots, const float4* proj, const float4* underColour, const float* underDepth,
                  float4* colour, float* depth, uint32_t* lastCount, float* finalT, float4* accum,
-                 hipStream_t stream, uint32_t* walkSteps = nullptr) {
+                 hipStream_t stream, uint32_t* walkSteps = nullptr, const uint32_t* slotOfPair = nullptr) {
     // A block a quadrant, the grid rounded up to whole groups of eight tiles.
     // The last contributor is tracked only where it is kept.
     const uint32_t tiles = f.tilesX * f.tilesY;
@@ -753,7 +760,7 @@ void launchBlend(uint32_t list, const FrameParams& f, uint32_t slots, const uint
     const auto go = [&](auto listLen, auto track) {
         blendKernel<DEPTH_MODE, decltype(listLen)::value, COMPOSITE, decltype(track)::value>
             <<<blocks, kLanes, 0, stream>>>(f, slots, order, ranges, pairSlots, proj, underColour, underDepth, colour,
-                                            depth, lastCount, finalT, accum, walkSteps);
+                                            depth, lastCount, finalT, accum, walkSteps, slotOfPair);
     };
     using L64 = std::integral_constant<uint32_t, 64>;
     using L128 = std::integral_constant<uint32_t, 128>;
@@ -774,7 +781,7 @@ template <uint32_t DEPTH_MODE, uint32_t LIST>
 void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* params, const ViewInfo* views,
                       uint32_t viewCount, uint32_t tiles, uint32_t tileShift, uint32_t slots, const uint2* ranges,
                       const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
-                      uint32_t* queue, hipStream_t stream) {
+                      uint32_t* queue, hipStream_t stream, const uint32_t* slotOfPair) {
     const uint32_t qBits = 2u * (tileShift - 3u);
     const auto go = [&](auto track) {
         constexpr bool T = decltype(track)::value;
@@ -784,12 +791,12 @@ void launchBlendViews(bool persistent, uint32_t gridCap, const FrameParams* para
             fill32(queue, 0u, 1, stream);
             blendViewsKernel<DEPTH_MODE, LIST, true, T><<<grid, kLanes, 0, stream>>>(
                 params, views, viewCount, tiles, qBits, tasks, slots, ranges, pairSlots, proj, lastCount, finalT,
-                accum, queue);
+                accum, queue, slotOfPair);
         } else {
             const uint32_t tasks = (tiles + 7u) / 8u * (8u << qBits);
             blendViewsKernel<DEPTH_MODE, LIST, false, T><<<tasks, kLanes, 0, stream>>>(
                 params, views, viewCount, tiles, qBits, tasks, slots, ranges, pairSlots, proj, lastCount, finalT,
-                accum, queue);
+                accum, queue, slotOfPair);
         }
     };
     if (lastCount != nullptr) go(std::true_type{}); else go(std::false_type{});
@@ -801,12 +808,12 @@ void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gri
                 const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t tileShift, uint32_t slots,
                 const uint2* ranges,
                 const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
-                uint32_t* queue, hipStream_t stream) {
+                uint32_t* queue, hipStream_t stream, const uint32_t* slotOfPair) {
     if (tiles == 0 || viewCount == 0) return;
     const auto go = [&](auto depthMode, auto listLen) {
         launchBlendViews<decltype(depthMode)::value, decltype(listLen)::value>(
             persistent, gridCap, params, views, viewCount, tiles, tileShift, slots, ranges, pairSlots, proj, lastCount,
-            finalT, accum, queue, stream);
+            finalT, accum, queue, stream, slotOfPair);
     };
     using D0 = std::integral_constant<uint32_t, 0>;
     using D1 = std::integral_constant<uint32_t, 1>;
@@ -860,14 +867,14 @@ void blendSplit(const FrameParams& f, uint32_t list, uint32_t segmentPairs, uint
 
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-           float* finalT, float4* accum, hipStream_t stream, uint32_t* walkSteps) {
+           float* finalT, float4* accum, hipStream_t stream, uint32_t* walkSteps, const uint32_t* slotOfPair) {
     if (f.tilesX * f.tilesY == 0) return;
     if (f.depthMode == 0) {
         launchBlend<0, false>(list, f, slots, order, ranges, pairSlots, proj, nullptr, nullptr, colour, depth,
-                              lastCount, finalT, accum, stream, walkSteps);
+                              lastCount, finalT, accum, stream, walkSteps, slotOfPair);
     } else {
         launchBlend<1, false>(list, f, slots, order, ranges, pairSlots, proj, nullptr, nullptr, colour, depth,
-                              lastCount, finalT, accum, stream, walkSteps);
+                              lastCount, finalT, accum, stream, walkSteps, slotOfPair);
     }
 }
 

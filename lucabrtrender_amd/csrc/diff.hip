@@ -341,7 +341,7 @@ constexpr uint32_t kBatch = 128;   // pairs staged per step, by the first 128 th
 // workgroup's four waves take its four quadrants. accum, lastCount, finalT and
 // gradColour are the frame's own per-pixel arrays.
 __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t tile, uint2 range, uint32_t slots,
-                                                  const uint32_t* __restrict__ sortedSlots,
+                                                  const uint32_t* __restrict__ pairSlot,
                                                   const uint32_t* __restrict__ pairIndex,
                                                   const float4* __restrict__ proj,
                                                   const float4* __restrict__ accum,
@@ -436,7 +436,7 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
             if (tid < n) {
                 const uint32_t pos = top - 1u - tid;
                 place = pos - range.x;
-                const uint32_t slot = sortedSlots[pos];
+                const uint32_t slot = pairSlot[pairIndex[pos]];
                 if (slot < slots) {
                     const uint64_t s3 = static_cast<uint64_t>(slot) * 3;
                     screen = proj[s3];
@@ -560,7 +560,7 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
 
 __global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uint32_t slots,
                                                               const uint2* __restrict__ ranges,
-                                                              const uint32_t* __restrict__ sortedSlots,
+                                                              const uint32_t* __restrict__ pairSlot,
                                                               const uint32_t* __restrict__ pairIndex,
                                                               const float4* __restrict__ proj,
                                                               const float4* __restrict__ accum,
@@ -572,7 +572,7 @@ __global__ __launch_bounds__(kBlock) void blendBackwardKernel(FrameParams f, uin
     // Heaviest tiles first when an order is given: a long tile starts early
     // instead of running alone at the end.
     const uint32_t tile = order != nullptr ? order[blockIdx.x] : blockIdx.x;
-    blendBackwardTile(f, tile, ranges[tile], slots, sortedSlots, pairIndex, proj, accum, lastCount, finalT,
+    blendBackwardTile(f, tile, ranges[tile], slots, pairSlot, pairIndex, proj, accum, lastCount, finalT,
                       gradColour, pairGrad);
 }
 
@@ -583,7 +583,7 @@ __global__ __launch_bounds__(kBlock) void blendBackwardViewsKernel(const FramePa
                                                                    const ViewInfo* __restrict__ views,
                                                                    uint32_t viewCount, uint32_t slots,
                                                                    const uint2* __restrict__ ranges,
-                                                                   const uint32_t* __restrict__ sortedSlots,
+                                                                   const uint32_t* __restrict__ pairSlot,
                                                                    const uint32_t* __restrict__ pairIndex,
                                                                    const float4* __restrict__ proj,
                                                                    const float4* __restrict__ accum,
@@ -596,7 +596,7 @@ __global__ __launch_bounds__(kBlock) void blendBackwardViewsKernel(const FramePa
     const uint32_t v = viewOfTile(views, viewCount, g);
     const ViewInfo& info = views[v];
     const FrameParams f = params[info.params];
-    blendBackwardTile(f, g - info.tileBase, ranges[g], slots, sortedSlots, pairIndex, proj, accum + info.pixelBase,
+    blendBackwardTile(f, g - info.tileBase, ranges[g], slots, pairSlot, pairIndex, proj, accum + info.pixelBase,
                       lastCount + info.pixelBase, finalT + info.pixelBase, gradColours[v], pairGrad);
 }
 
@@ -1054,23 +1054,23 @@ void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t p
                                                                             sortedSlots);
 }
 
-void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
+void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* pairSlot,
                    const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
                    const float* finalT, const float4* gradColour, PairGrads pairGrad, hipStream_t stream,
                    const uint32_t* order) {
     const uint32_t tiles = f.tilesX * f.tilesY;
     if (tiles == 0) return;
-    blendBackwardKernel<<<tiles, kBlock, 0, stream>>>(f, slots, ranges, sortedSlots, pairIndex, proj, accum,
+    blendBackwardKernel<<<tiles, kBlock, 0, stream>>>(f, slots, ranges, pairSlot, pairIndex, proj, accum,
                                                       lastCount, finalT, gradColour, pairGrad, order);
 }
 
 void blendBackwardViews(const FrameParams* params, const ViewInfo* views, uint32_t viewCount, uint32_t tiles,
-                        uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots, const uint32_t* pairIndex,
+                        uint32_t slots, const uint2* ranges, const uint32_t* pairSlot, const uint32_t* pairIndex,
                         const float4* proj, const float4* accum, const uint32_t* lastCount, const float* finalT,
                         const float4* const* gradColours, PairGrads pairGrad, hipStream_t stream,
                         const uint32_t* order) {
     if (tiles == 0 || viewCount == 0) return;
-    blendBackwardViewsKernel<<<tiles, kBlock, 0, stream>>>(params, views, viewCount, slots, ranges, sortedSlots,
+    blendBackwardViewsKernel<<<tiles, kBlock, 0, stream>>>(params, views, viewCount, slots, ranges, pairSlot,
                                                            pairIndex, proj, accum, lastCount, finalT, gradColours,
                                                            pairGrad, order);
 }

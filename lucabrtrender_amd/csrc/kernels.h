@@ -187,13 +187,16 @@ void widenKeys(const uint32_t* keys, uint32_t tileShift, const uint32_t* slots, 
 // Tile indices, heaviest tile (most pairs) first, for the blend's launch order.
 void tileOrder(const uint2* ranges, uint32_t tiles, uint32_t* order, hipStream_t stream);
 // Per pixel, front to back over its tile's pairs; one wave per 8x8 quadrant,
-// whose LDS list holds `list` (128 or 256) records; tiles in `order` (null:
-// index order); slots bounds the pair values. lastCount/finalT/accum may be
-// null; accum receives each pixel's raw rgb and depth sum before the
-// linearisation and the background.
+// whose LDS list holds `list` (64, 128 or 256) records; tiles in `order`
+// (null: index order); slots bounds the pair slots. pairSlots holds the
+// sorted pairs' slots, or with slotOfPair non-null their pair indices, each
+// pair's slot then slotOfPair[index]. lastCount/finalT/accum may be null;
+// accum receives each pixel's raw rgb and depth sum before the linearisation
+// and the background.
 void blend(const FrameParams& f, uint32_t list, uint32_t slots, const uint32_t* order, const uint2* ranges,
            const uint32_t* pairSlots, const float4* proj, float4* colour, float* depth, uint32_t* lastCount,
-           float* finalT, float4* accum, hipStream_t stream, uint32_t* walkSteps = nullptr);
+           float* finalT, float4* accum, hipStream_t stream, uint32_t* walkSteps = nullptr,
+           const uint32_t* slotOfPair = nullptr);
 // (walkSteps, when not null, receives the sum over quadrant waves of the list
 // records each walked: the (pixel, record) steps are 64 times that.)
 // The composite walk: a record with colour.w = 1 (a disc) that reaches a pixel
@@ -237,7 +240,7 @@ void blendViews(uint32_t depthMode, uint32_t list, bool persistent, uint32_t gri
                 const ViewInfo* views, uint32_t viewCount, uint32_t tiles, uint32_t tileShift, uint32_t slots,
                 const uint2* ranges,
                 const uint32_t* pairSlots, const float4* proj, uint32_t* lastCount, float* finalT, float4* accum,
-                uint32_t* queue, hipStream_t stream);
+                uint32_t* queue, hipStream_t stream, const uint32_t* slotOfPair = nullptr);
 
 // ---- brute force in FP64 -----------------------------------------------------------
 // Bytes of one FP64 brute-force record.
@@ -284,7 +287,8 @@ void projectDiff(const FrameParams& f, const DiffCloud& cloud, float4* proj, uin
 void projectDiffViews(const FrameParams* params, uint32_t views, const DiffCloud& cloud, float4* proj, uint2* rects,
                       uint32_t* touched, hipStream_t stream);
 // sortedSlots[i] = pairSlot[pairIndex[i]] for i < pairs (or < *nDevice, pairs
-// then being the capacity).
+// then being the capacity): the debug readback's sorted slots of a training
+// frame (its blends read pairSlot[pairIndex[i]] as they go).
 void gatherSlots(const uint32_t* pairIndex, const uint32_t* pairSlot, uint32_t pairs, uint32_t* sortedSlots,
                  hipStream_t stream, const uint32_t* nDevice = nullptr);
 // Each pair's nine screen-space gradient terms, 36 bytes a pair, in one buffer
@@ -303,8 +307,9 @@ inline PairGrads pairGradsIn(void* buffer, uint32_t capacity) {
     return PairGrads{ab, reinterpret_cast<float*>(ab + 2ull * capacity)};
 }
 // dL/d(colour) per pixel -> one screen-space gradient record per pair, at the
-// pair's index in pairGrad.
-void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots,
+// pair's index in pairGrad. pairIndex: the sorted pairs' indices; pairSlot:
+// each pair's slot, by pair index.
+void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, const uint32_t* pairSlot,
                    const uint32_t* pairIndex, const float4* proj, const float4* accum, const uint32_t* lastCount,
                    const float* finalT, const float4* gradColour, PairGrads pairGrad, hipStream_t stream,
                    const uint32_t* order = nullptr);
@@ -313,7 +318,7 @@ void blendBackward(const FrameParams& f, uint32_t slots, const uint2* ranges, co
 // ViewInfo.pixelBase); gradColours: a device array of one colour-gradient
 // pointer a view. order (may be null): the tiles in launch order (tileOrder's).
 void blendBackwardViews(const FrameParams* params, const ViewInfo* views, uint32_t viewCount, uint32_t tiles,
-                        uint32_t slots, const uint2* ranges, const uint32_t* sortedSlots, const uint32_t* pairIndex,
+                        uint32_t slots, const uint2* ranges, const uint32_t* pairSlot, const uint32_t* pairIndex,
                         const float4* proj, const float4* accum, const uint32_t* lastCount, const float* finalT,
                         const float4* const* gradColours, PairGrads pairGrad, hipStream_t stream,
                         const uint32_t* order = nullptr);

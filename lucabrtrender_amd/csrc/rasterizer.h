@@ -323,7 +323,7 @@ private:
     int lastSorted_ = 0;   // which of keys_/values_ hold the sorted pairs
     bool lastNarrow_ = false;       // the sorted keys are u32 (kern::sortNarrows)
     uint32_t lastTileShift_ = 0;    // their tile is key >> lastTileShift_
-    bool lastSlotsSorted_ = false;  // the sorted values are pair indices; sortedSlots_ holds the slots
+    bool lastSlotsSorted_ = false;  // the sorted values are pair indices (their slots: pairSlot_)
     // The last plain (one view, splats only, not split) blend's launch, for
     // the walk-step count of LRT_DEBUG_WALK_STEPS.
     bool lastBlendPlain_ = false;

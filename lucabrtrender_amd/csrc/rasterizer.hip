@@ -444,13 +444,11 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     } else {
         kern::tileRanges(keys_[sorted].as<uint64_t>(), pairs, tiles, ranges_.as<uint2>(), stream_, nullptr, true);
     }
-    // The blend reads slots; a training frame's sort carried pair indices.
+    // A training frame's sort carried pair indices: its blend reads each
+    // pair's slot through them (a separate gather of the sorted slots cost more,
+    // its random reads all on the critical path).
     const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
-    if (training) {
-        kern::gatherSlots(values_[sorted].as<uint32_t>(), pairSlot_.as<uint32_t>(), pairs,
-                          sortedSlots_.as<uint32_t>(), stream_);
-        pairSlots = sortedSlots_.as<uint32_t>();
-    }
+    const uint32_t* slotOfPair = training ? pairSlot_.as<uint32_t>() : nullptr;
 
     const uint32_t* order = nullptr;
     // Heavy tiles split across waves: frames that keep no composite or
@@ -492,7 +490,7 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
     } else {
         kern::blend(first, settings.blendBatch, splatCount, order, ranges_.as<uint2>(), pairSlots,
                     proj_.as<float4>(), targets.colour, targets.depth, lastCountOut, finalTOut,
-                    training ? accum_.as<float4>() : nullptr, stream_);
+                    training ? accum_.as<float4>() : nullptr, stream_, nullptr, slotOfPair);
     }
     lastBlendPlain_ = !split && !composite && !training;   // a training frame's accum_ is kept
     lastBlendFrame_ = first;
@@ -605,13 +603,10 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
         kern::tileRanges(keys_[sorted].as<uint64_t>(), pairCapacity_, tiles, ranges_.as<uint2>(), stream,
                          counters + 0);
     }
-    // The blend reads slots; a training batch's sort carried pair indices.
+    // A training batch's sort carried pair indices: the blend reads each pair's
+    // slot through them.
     const uint32_t* pairSlots = values_[sorted].as<uint32_t>();
-    if (training) {
-        kern::gatherSlots(values_[sorted].as<uint32_t>(), pairSlot_.as<uint32_t>(), pairCapacity_,
-                          sortedSlots_.as<uint32_t>(), stream, counters + 0);
-        pairSlots = sortedSlots_.as<uint32_t>();
-    }
+    const uint32_t* slotOfPair = training ? pairSlot_.as<uint32_t>() : nullptr;
     mark(7);
     // A persistent blend keeps about 24 one-wave workgroups on each CU.
     const bool keepWalk = training || settings.keepWalkState;
@@ -619,7 +614,7 @@ void TileRasterizer::enqueueViews(std::span<const SplatInstance> instances, cons
                      views, viewCount, tiles, tileShiftOf(settings.tileSize), slots, ranges_.as<uint2>(), pairSlots,
                      proj_.as<float4>(),
                      keepWalk ? lastCount_.as<uint32_t>() : nullptr, keepWalk ? finalT_.as<float>() : nullptr,
-                     training ? accum_.as<float4>() : nullptr, counters + 6, stream);
+                     training ? accum_.as<float4>() : nullptr, counters + 6, stream, slotOfPair);
     mark(8);
     (void)hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream);
     lastSorted_ = sorted;
@@ -1058,7 +1053,7 @@ Status TileRasterizer::backward(const float4* gradColour, const kern::DiffGrads&
     if (millis != nullptr) (void)hipEventRecord(events_[0], stream_);
     // One view's tiles in raster order: neighbouring tiles share splats in L2,
     // and on a 1M-splat 720p frame that beat heaviest-first by about 10%.
-    kern::blendBackward(diffFrame_, lastSplats_, ranges_.as<uint2>(), sortedSlots_.as<uint32_t>(),
+    kern::blendBackward(diffFrame_, lastSplats_, ranges_.as<uint2>(), pairSlot_.as<uint32_t>(),
                         values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(), accum_.as<float4>(),
                         lastCount_.as<uint32_t>(), finalT_.as<float>(), gradColour, pairGrads, stream_,
                         nullptr);
@@ -1117,7 +1112,7 @@ Status TileRasterizer::backwardViews(std::span<const float4* const> gradColours,
     // close to the cloud) would otherwise finish alone at the end.
     kern::tileOrder(ranges_.as<uint2>(), lastTiles_, tileOrder_.as<uint32_t>(), stream_);
     kern::blendBackwardViews(params, viewInfo_.as<ViewInfo>(), diffViews_, lastTiles_, lastSplats_, ranges_.as<uint2>(),
-                             sortedSlots_.as<uint32_t>(), values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(),
+                             pairSlot_.as<uint32_t>(), values_[lastSorted_].as<uint32_t>(), proj_.as<float4>(),
                              accum_.as<float4>(), lastCount_.as<uint32_t>(), finalT_.as<float>(),
                              viewGrads_.as<const float4*>(), pairGrads, stream_, tileOrder_.as<uint32_t>());
     if (millis != nullptr) (void)hipEventRecord(events_[1], stream_);
@@ -1328,7 +1323,12 @@ Status TileRasterizer::debugCopy(int which, void* host, size_t bytes, size_t* ne
     if (which == 4 && lastNarrow_) {
         // The sort left u32 keys: rebuild tile << 32 | bits(view z) from them and
         // the pairs' slots into the other key buffer (free once a frame is done).
-        const uint32_t* slots = lastSlotsSorted_ ? sortedSlots_.as<uint32_t>() : values_[lastSorted_].as<uint32_t>();
+        const uint32_t* slots = values_[lastSorted_].as<uint32_t>();
+        if (lastSlotsSorted_) {
+            // A training frame's sorted values are pair indices: their slots first.
+            kern::gatherSlots(slots, pairSlot_.as<uint32_t>(), lastPairs_, sortedSlots_.as<uint32_t>(), stream_);
+            slots = sortedSlots_.as<uint32_t>();
+        }
         kern::widenKeys(keys_[lastSorted_].as<uint32_t>(), lastTileShift_, slots, proj_.as<float4>(), lastPairs_,
                         keys_[1 - lastSorted_].as<uint64_t>(), stream_);
         src = keys_[1 - lastSorted_].get();
