@@ -369,6 +369,21 @@ def diff_render_f64(frame: OrFrame, params: dict, filter3d=None, threads=0) -> n
     return colour
 
 
+def diff_render_f64_pixels(frame: OrFrame, params: dict, px, py, keyed=True, filter3d=None, threads=0) -> np.ndarray:
+    """or_diff_render_f64 at the pixels (px[k], py[k]) only: FP64 colours [len(px), 4]."""
+    arrs = {k: np.ascontiguousarray(params[k], dtype=np.float32) for k in
+            ("means", "log_scales", "quats", "opacity_logits", "sh")}
+    f3 = np.ascontiguousarray(filter3d, dtype=np.float32) if filter3d is not None else None
+    cloud = OrDiffCloud(ptr(arrs["means"]), ptr(arrs["log_scales"]), ptr(arrs["quats"]), ptr(arrs["opacity_logits"]),
+                        ptr(arrs["sh"]), ptr(f3), arrs["means"].shape[0], arrs["sh"].shape[1] - 1)
+    xs = np.ascontiguousarray(px, dtype=np.uint32)
+    ys = np.ascontiguousarray(py, dtype=np.uint32)
+    colour = np.zeros((xs.size, 4), np.float64)
+    lib.or_diff_render_f64(C.byref(frame), C.byref(cloud), ptr(xs), ptr(ys), xs.size, ptr(colour), None,
+                           1 if keyed else 0, threads)
+    return colour
+
+
 # ---- levels of detail ---------------------------------------------------------------
 
 class OrCutParams(C.Structure):

@@ -201,3 +201,88 @@ def test_backward_needs_a_differentiable_frame(rast):
         rast.render(_camera(), [raster.SplatInstance(cfg.scene.to_device())],
                     raster.RenderSettings(width=32, height=32, tile_size=12), raster.RenderTargets(32, 32))
     assert err.value.code == 3
+
+
+def test_full_size_1m_720p_on_a_pixel_subset(rast):
+    """The 1M-splat 720p fwd+bwd workload at its full size, on 64 pixels the
+    cloud covers (coverage above 0.5, seeded): the HIP colours there against the
+    FP64 brute force in the tile path's order (p99 <= 1e-3, max <= 1e-2), and
+    the HIP gradients of sum over those pixels of colour * G against FP64
+    central differences, for the four largest components of each group. The
+    differences are taken over the splats that reach the 64 pixels (their
+    colours there are the whole cloud's: the walk keeps their order); a
+    component whose difference moves by more than 1e-2 between the steps h and
+    h/4 straddles a step of the forward and is left out (at most one a group),
+    the rest must agree to 1e-3 (measured: 3e-6 at worst; the image 2e-6)."""
+    n, width, height = 1_000_000, 1280, 720
+    p = scenes.diff_params(n, 0, degree=3)
+    s = raster.RenderSettings(width=width, height=height)
+    dev = _device(p)
+    splats = raster.DiffSplats(dev["means"], dev["log_scales"], dev["quats"], dev["opacity_logits"], dev["sh"])
+    targets = raster.RenderTargets(width, height)
+    rast.render_diff(_camera(), splats, s, targets)
+    torch.cuda.synchronize()
+    img = targets.colour.cpu().numpy()
+    rng = np.random.default_rng(5)
+    ys, xs = np.nonzero(img[..., 3] > 0.5)
+    pick = rng.choice(xs.size, 64, replace=False)
+    px, py = xs[pick].astype(np.uint32), ys[pick].astype(np.uint32)
+    frame = _oracle_frame(s, n, 15)
+    ref = O.diff_render_f64_pixels(frame, p, px, py, keyed=True)
+    d = np.abs(img[py, px].astype(np.float64) - ref).reshape(-1)
+    print(f"1M splats, 64 pixels vs FP64 brute force: p99 {np.quantile(d, 0.99):.2e} max {d.max():.2e}", flush=True)
+    assert np.quantile(d, 0.99) <= 1e-3 and d.max() <= 1e-2
+
+    g = np.zeros((height, width, 4))
+    g[py, px] = rng.normal(0.0, 1.0, (64, 4))
+    grads = {k: torch.empty_like(v) for k, v in dev.items()}
+    rast.backward(torch.from_numpy(g.astype(np.float32)).cuda(), grads)
+    torch.cuda.synchronize()
+    grads = {k: v.cpu().numpy() for k, v in grads.items()}
+
+    # The splats that reach any of the 64 pixels (their power there within the
+    # cutoff plus a margin the differences' steps cannot cross), in index order.
+    t = {k: torch.from_numpy(np.ascontiguousarray(p[key], dtype=np.float32))
+         for k, key in (("means", "means"), ("log_scales", "log_scales"), ("quats", "quats"),
+                        ("logits", "opacity_logits"), ("sh", "sh"))}
+    rec = D.project(frame, t["means"], t["log_scales"], t["quats"], t["logits"], t["sh"])
+    reach = np.zeros(n, bool)
+    for x, y in zip(px, py):
+        dx = float(x) + 0.5 - rec["cx"]
+        dy = float(y) + 0.5 - rec["cy"]
+        power = rec["a"] * dx * dx + 2.0 * rec["b"] * dx * dy + rec["c"] * dy * dy
+        reach |= (rec["valid"] & (power <= rec["cutoff"] + 1.0)).numpy()
+    keep = np.nonzero(reach)[0]
+    sub = {k: np.ascontiguousarray(v[keep]) for k, v in p.items()}
+    sub_frame = _oracle_frame(s, keep.size, 15)
+    gsub = g[py, px]
+    assert np.abs(O.diff_render_f64_pixels(sub_frame, sub, px, py) - ref).max() < 1e-12
+    print(f"  {keep.size} splats reach the 64 pixels", flush=True)
+
+    def loss(q):
+        return float((O.diff_render_f64_pixels(sub_frame, q, px, py) * gsub).sum())
+
+    position = {int(k): j for j, k in enumerate(keep)}
+    for name in GROUPS:
+        flat = grads[name].reshape(-1)
+        per = flat.size // n
+        rel, steps = [], []
+        for idx in np.argsort(-np.abs(flat))[:4]:
+            j = position[int(idx) // per]
+            at = j * per + int(idx) % per
+            base = float(sub[name].reshape(-1)[at])
+            fd = []
+            for h in (1e-4 * max(1.0, abs(base)), 2.5e-5 * max(1.0, abs(base))):
+                plus, minus = {k: v.copy() for k, v in sub.items()}, {k: v.copy() for k, v in sub.items()}
+                plus[name].reshape(-1)[at] = np.float32(base + h)
+                minus[name].reshape(-1)[at] = np.float32(base - h)
+                step = float(plus[name].reshape(-1)[at]) - float(minus[name].reshape(-1)[at])
+                fd.append((loss(plus) - loss(minus)) / step)
+            if abs(fd[0] - fd[1]) > 1e-2 * abs(fd[1]):
+                steps.append(int(idx))
+                continue
+            rel.append(abs(float(flat[idx]) - fd[1]) / abs(fd[1]))
+        print(f"  {name:15s} vs FP64 central differences: {len(rel)} smooth, max {np.max(rel):.2e}; "
+              f"straddling a step: {steps}", flush=True)
+        assert len(rel) >= 3, (name, steps)
+        assert np.max(rel) < 1e-3, (name, rel)
