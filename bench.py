@@ -642,6 +642,8 @@ def main():
     ap.add_argument("--blend-batch", type=int, default=256, help="records in each quadrant's LDS list: 64, 128 or 256")
     ap.add_argument("--tile-size", type=int, default=16, help="pixels a tile's side: 8, 16 (the reference's) or 32")
     ap.add_argument("--sweeps", type=int, default=1, help="1: the blend-setting and tile-size comparisons; 0 skips them")
+    ap.add_argument("--image-pixels", type=int, default=65536,
+                    help="pixels of the timed frame checked against the FP64 brute force; 0 skips it")
     ap.add_argument("--fwd-bwd-steps", type=int, default=20, help="fwd+bwd views (1M splats, 720p) timed; 0 skips it")
     ap.add_argument("--stress-reps", type=int, default=3, help="sort-stress frames (6M splats, 2160p) timed; 0 skips it")
     ap.add_argument("--views-reps", type=int, default=5, help="64-view batches (2M splats) timed; 0 skips it")
@@ -790,6 +792,23 @@ def main():
         log("tile sizes: " + ", ".join(f"{t}: {v['frames_per_s']} frames/s, {v['pairs']} pairs"
                                        for t, v in tile_sizes.items()))
 
+    # The timed frame against the FP64 brute force over all splats on a seeded
+    # pixel subset, in the rasteriser's order (24-bit depth key, slot) and in
+    # the brute force's own (exact view z, index).
+    image_vs_fp64 = None
+    if args.image_pixels > 0:
+        rast.render(proj, inst, settings, targets)
+        npix = min(args.image_pixels, cfg.width * cfg.height)
+        pix = np.random.default_rng(11).choice(cfg.width * cfg.height, npix, replace=False).astype(np.int32)
+        ours = targets.colour.reshape(-1, 4).cpu().numpy().astype(np.float64)[pix]
+        image_vs_fp64 = {"pixels": int(npix)}
+        for keyed in (True, False):
+            colour, _ = rast.reference_render_pixels(proj, inst, settings, torch.from_numpy(pix).cuda(), keyed=keyed)
+            d = np.abs(ours - colour.cpu().numpy()).reshape(-1)
+            image_vs_fp64["keyed_order" if keyed else "exact_order"] = {
+                "p99": float(np.quantile(d, 0.99)), "max": float(d.max()), "over_1e-2": float(np.mean(d > 1e-2))}
+        log(f"image vs FP64: {image_vs_fp64}")
+
     fwd_bwd = None
     if args.fwd_bwd_steps > 0:
         log("fwd+bwd (1M splats, 720p) ...")
@@ -852,6 +871,7 @@ def main():
         "gpu_ms_per_frame": round(float(kernel_ms[7]), 4),
         "blend_ms_by_setting": variants,
         "tile_sizes": tile_sizes,
+        "image_vs_fp64": image_vs_fp64,
         "sort_gkeys_per_s": round(pairs / (sort_ms * 1e-3) / 1e9, 3) if sort_ms > 0 else None,
         "copy_GB/s": round(copy_gbs, 1),
         "fwd_bwd": fwd_bwd,
