@@ -1,7 +1,9 @@
-// Blend: one wave64 workgroup per 8x8 quadrant of a 16x16 tile, one lane per
-// pixel. The wave walks its tile's sorted pairs front to back in rounds of 128:
-//   1. each lane takes two pairs of the round (their records gathered one round
-//      ahead, so the loads are in flight while the previous round is worked)
+// Blend: one wave64 workgroup per 8x8 quadrant of a tile (a 16x16 tile has
+// four, an 8x8 one, a 32x32 sixteen), one lane per pixel. The wave walks its
+// tile's sorted pairs front to back in rounds of 128 (64 for 64-record lists):
+//   1. each lane takes two pairs of the round (one for 64-record lists; their
+//      records gathered one round ahead, so the loads are in flight while the
+//      previous round is worked)
 //      and tests each against the quadrant's pixel-centre box with a
 //      conservative bound on the Gaussian's power (quadrantReach);
 //   2. the pairs that pass are compacted (ballot + popcount) into the wave's
@@ -19,10 +21,9 @@
 // from a quarter pixel to 300 pixels wide, centres up to 3.5 sigma outside the
 // quadrant).
 // No wave ever waits for another: quadrants finish as soon as their own pixels
-// do, and a quadrant with few pairs frees its slot for the next. The four
-// quadrants of one tile are blocks b, b + 8, b + 16, b + 24, which the
-// dispatcher deals to one XCD, so their gathers of the tile's records share
-// that XCD's L2.
+// do, and a quadrant with few pairs frees its slot for the next. The
+// quadrants of one tile are blocks b, b + 8, b + 16, ..., which the dispatcher
+// deals to one XCD, so their gathers of the tile's records share that XCD's L2.
 //
 // Per pixel it is the reference's splatBlend walk (shaders/lrt/splat/
 // splat_blend.slang:23-108, splat-only entry :110-114): skip past the cutoff,
