@@ -26,10 +26,24 @@ void fill32(uint32_t* dst, uint32_t value, uint64_t count, hipStream_t stream);
 // zeroed by the same launch that clears the look-back words.
 uint32_t scanPartitions(uint32_t n);
 inline uint64_t scanStatusWords(uint32_t n) { return uint64_t{scanPartitions(n)} * 3 / 2 + 2; }
+// With zeroed set, the caller has already cleared the look-back words (the
+// first scanZeroWords(n) u32 words of status) and alsoZero with a launch that
+// runs anyway, and the scan launches no clear of its own.
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks = nullptr,
                    uint32_t depthEntries = 0, uint32_t* depthRange = nullptr, uint32_t* alsoZero = nullptr,
-                   uint32_t alsoZeroWords = 0);
+                   uint32_t alsoZeroWords = 0, bool zeroed = false);
+// The u32 words of `status` a scan of n elements needs zeroed before it runs.
+inline uint32_t scanZeroWords(uint32_t n) { return scanPartitions(n) * 2; }
+
+// Words a launch zeroes on the side (either span may be empty): a later
+// launch's look-back words and counters, cleared by one that runs anyway.
+struct ZeroSpans {
+    uint32_t* a = nullptr;
+    uint32_t aWords = 0;
+    uint32_t* b = nullptr;
+    uint32_t bWords = 0;
+};
 // The same scan with its scratch in a static device array instead of `status`
 // and counters: no allocation on any call. n at most kStaticScanMax
 // (hipErrorInvalidValue past it). Calls that overlap on one device share that
@@ -44,7 +58,7 @@ hipError_t scanExclusiveStatic(const uint32_t* in, uint32_t* out, uint32_t n, ui
 // workgroups' depth-key range into `depth`.
 void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
              const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, DepthBlocks depth,
-             hipStream_t stream);
+             hipStream_t stream, ZeroSpans zero = {});
 // The same instance (count splats) for `views` views at once: view v's
 // constants are params[v] (device), its slots start at params[v].base.
 void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,

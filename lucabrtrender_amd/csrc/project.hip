@@ -168,7 +168,13 @@ __global__ __launch_bounds__(kBlock) void projectKernel(FrameParams f, const flo
                                                         const uint32_t* __restrict__ sh,
                                                         const float* __restrict__ filter3d,
                                                         float4* __restrict__ proj, uint2* __restrict__ rects,
-                                                        uint32_t* __restrict__ touchedOut, DepthBlocks depth) {
+                                                        uint32_t* __restrict__ touchedOut, DepthBlocks depth,
+                                                        ZeroSpans zero) {
+    // The side clears: grid-stride, so any grid covers them.
+    for (uint32_t w = blockIdx.x * kBlock + threadIdx.x; w < zero.aWords || w < zero.bWords; w += gridDim.x * kBlock) {
+        if (w < zero.aWords) zero.a[w] = 0u;
+        if (w < zero.bWords) zero.b[w] = 0u;
+    }
     projectBlock(f, positions, shape, sh, filter3d, proj, rects, touchedOut, depth);
 }
 
@@ -273,10 +279,11 @@ __global__ __launch_bounds__(kBlock) void projectViewsKernel(const FrameParams* 
 
 void project(const FrameParams& f, const float4* positions, const uint4* shape, const uint32_t* sh,
              const float* filter3d, float4* proj, uint2* rects, uint32_t* touched, DepthBlocks depth,
-             hipStream_t stream) {
+             hipStream_t stream, ZeroSpans zero) {
     if (f.count == 0) return;
     const uint32_t blocks = projectBlocks(f.count);
-    projectKernel<<<blocks, kBlock, 0, stream>>>(f, positions, shape, sh, filter3d, proj, rects, touched, depth);
+    projectKernel<<<blocks, kBlock, 0, stream>>>(f, positions, shape, sh, filter3d, proj, rects, touched, depth,
+                                                 zero);
 }
 
 void projectViews(const FrameParams* params, uint32_t views, uint32_t count, const float4* positions,

@@ -271,9 +271,11 @@ private:
                       uint32_t splatsPerView, uint32_t tiles, const RenderSettings& settings, hipStream_t stream);
     // Everything after the projection: scan, duplicate, sort, ranges, blend
     // (the composite blend when `composite`, over `under` when non-null).
+    // scanZeroed: the projection already cleared the scan's look-back words
+    // and the counters.
     Status finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
                        RenderTargets& targets, FrameStats* stats, bool training, bool composite = false,
-                       const RenderTargets* under = nullptr);
+                       const RenderTargets* under = nullptr, bool scanZeroed = false);
     Status check(hipError_t e, const char* what);
 
     int device_ = 0;

@@ -255,6 +255,10 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     uint32_t base = 0;
     FrameParams first{};
     bool haveFirst = false;
+    // The scan's look-back words and the counters are cleared by the first
+    // projection launch rather than by a launch of their own.
+    uint32_t* counters = counters_.as<uint32_t>();
+    bool scanZeroed = false;
     for (const SplatInstance& instance : instances) {
         const SplatCloud* cloud = instance.splats;
         FrameParams f = frameParamsFor(projection, instance, settings, base);
@@ -263,8 +267,14 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
             haveFirst = true;
         }
         if (cloud == nullptr || cloud->count == 0) continue;
+        kern::ZeroSpans zero;
+        if (!scanZeroed) {
+            zero = kern::ZeroSpans{reinterpret_cast<uint32_t*>(scanStatus_.as<uint64_t>()),
+                                   kern::scanZeroWords(splatCount), counters, 12};
+            scanZeroed = true;
+        }
         kern::project(f, cloud->positions, cloud->shape, cloud->sh, cloud->filter3d, proj_.as<float4>(),
-                      rects_.as<uint2>(), touched_.as<uint32_t>(), depth, stream_);
+                      rects_.as<uint2>(), touched_.as<uint32_t>(), depth, stream_, zero);
         depth.first += projectBlocks(cloud->count);
         base += cloud->count;
     }
@@ -295,7 +305,7 @@ Status TileRasterizer::render(const Projection& projection, std::span<const Spla
     diffViewsReady_ = false;
     lastBlendPlain_ = false;
     const bool composite = under != nullptr || !points.empty();
-    return finishFrame(first, splatCount, settings, targets, statsOut, false, composite, under);
+    return finishFrame(first, splatCount, settings, targets, statsOut, false, composite, under, scanZeroed);
 }
 
 Status TileRasterizer::renderDiff(const Projection& projection, const kern::DiffCloud& cloud,
@@ -348,7 +358,7 @@ Status TileRasterizer::renderDiff(const Projection& projection, const kern::Diff
 
 Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount, const RenderSettings& settings,
                                    RenderTargets& targets, FrameStats* statsOut, bool training, bool composite,
-                                   const RenderTargets* under) {
+                                   const RenderTargets* under, bool scanZeroed) {
     const auto wallStart = std::chrono::steady_clock::now();
     const uint32_t tiles = first.tilesX * first.tilesY;
     uint32_t* counters = counters_.as<uint32_t>();
@@ -365,9 +375,11 @@ Status TileRasterizer::finishFrame(const FrameParams& first, uint32_t splatCount
 
     // How many pairs, how many visible and the drawn depth keys' range: one
     // scan, one readback.
+    // The counters are zeroed with the look-back words: by the first
+    // projection launch when it did so (scanZeroed), else by the scan.
     kern::scanExclusive(touched_.as<uint32_t>(), offsets_.as<uint32_t>(), splatCount, scanStatus_.as<uint64_t>(),
                         counters + 0, counters + 1, counters + 5, stream_, depthBlocks_.as<uint32_t>(), depthEntries_,
-                        counters + 8, counters, 12);   // the counters zeroed with the look-back words
+                        counters + 8, counters, 12, scanZeroed);
     mark(2);
     if (!(s = check(hipMemcpyAsync(hostCounters_, counters, 48, hipMemcpyDeviceToHost, stream_), "counts")).ok())
         return s;

@@ -291,9 +291,10 @@ hipError_t scanExclusiveStatic(const uint32_t* in, uint32_t* out, uint32_t n, ui
 
 void scanExclusive(const uint32_t* in, uint32_t* out, uint32_t n, uint64_t* status, uint32_t* total,
                    uint32_t* nonzero, uint32_t* error, hipStream_t stream, const uint32_t* depthBlocks,
-                   uint32_t depthEntries, uint32_t* depthRange, uint32_t* alsoZero, uint32_t alsoZeroWords) {
+                   uint32_t depthEntries, uint32_t* depthRange, uint32_t* alsoZero, uint32_t alsoZeroWords,
+                   bool zeroed) {
     const uint32_t parts = scanPartitions(n);
-    if (const uint64_t words = static_cast<uint64_t>(parts) * 2 + alsoZeroWords; words > 0) {
+    if (const uint64_t words = static_cast<uint64_t>(parts) * 2 + alsoZeroWords; words > 0 && !zeroed) {
         // (an empty scan with nothing else to clear launches nothing: a grid of 0 is an error)
         const uint32_t grid = static_cast<uint32_t>(std::min<uint64_t>((words + 255) / 256, 4096));
         zero2Kernel<<<grid, 256, 0, stream>>>(reinterpret_cast<uint32_t*>(status), static_cast<uint64_t>(parts) * 2,
