@@ -52,10 +52,13 @@ def test_discs_match_the_oracle_and_the_brute_force(rast, mode, size):
     assert p99 <= 2, (p99, mx, over2)
 
 
-def test_constant_colour_placed_instance_and_splats_mixed(rast):
-    print("mixed splats and discs", flush=True)
+@pytest.mark.parametrize("tile", [16, 8, 32])
+def test_constant_colour_placed_instance_and_splats_mixed(rast, tile):
+    """Splats and discs in one sort, at the reference's 16-pixel tiles and at
+    8 and 32 (the oracle drawing at the same size)."""
+    print(f"mixed splats and discs, tile {tile}", flush=True)
     cam = raster.Camera.looking_at((1.5, 1.0, 6.0), (0.0, 0.0, 0.0))
-    s = raster.RenderSettings(width=250, height=190, depth_mode=1)
+    s = raster.RenderSettings(width=250, height=190, depth_mode=1, tile_size=tile)
     scene = F.random_cloud(3000, 7).decode(3)
     pos, col = F.random_points(3000, 11)
     o2w = np.eye(4)
