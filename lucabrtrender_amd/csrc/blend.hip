@@ -563,27 +563,31 @@ __global__ __launch_bounds__(kOrderWaves * 64) void tileOrderKernel(const uint2*
     __syncthreads();
     // Bucket k's first place: the tiles of every heavier bucket; then wave w's
     // within it: the bucket's tiles in the waves before w.
-    __shared__ uint32_t sTotal[kBuckets];
-    if (tid < kBuckets) {
-        uint32_t sum = 0;
-        for (uint32_t ww = 0; ww < kOrderWaves; ++ww) sum += sCount[ww][tid];
-        sTotal[tid] = sum;
-    }
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t running = 0;
-        for (uint32_t k = kBuckets; k-- > 0;) {
-            const uint32_t n = sTotal[k];
-            sTotal[k] = running;
-            running += n;
+    // One wave: lane k < kBuckets holds bucket k's per-wave counts, scans them
+    // over the waves in registers, and takes its bucket's first place as the
+    // tiles of the heavier buckets (lanes above it: a suffix sum over lanes).
+    if (wave == 0) {
+        uint32_t c[kOrderWaves], sum = 0;
+#pragma unroll
+        for (uint32_t ww = 0; ww < kOrderWaves; ++ww) {
+            c[ww] = lane < kBuckets ? sCount[ww][lane] : 0u;
+            sum += c[ww];
         }
-    }
-    __syncthreads();
-    for (uint32_t e = tid; e < kOrderWaves * kBuckets; e += blockDim.x) {
-        const uint32_t w = e / kBuckets, k = e % kBuckets;
-        uint32_t start = sTotal[k];
-        for (uint32_t ww = 0; ww < w; ++ww) start += sCount[ww][k];
-        sStart[w][k] = start;
+        uint32_t suffix = sum;   // lanes lane .. 63 (zero past the buckets)
+#pragma unroll
+        for (uint32_t d = 1; d < 64; d <<= 1) {
+            const uint32_t o = static_cast<uint32_t>(__shfl_down(static_cast<int>(suffix), d, 64));
+            suffix += lane + d < 64 ? o : 0u;
+        }
+        const uint32_t above = suffix - sum;
+        if (lane < kBuckets) {
+            uint32_t start = above;
+#pragma unroll
+            for (uint32_t ww = 0; ww < kOrderWaves; ++ww) {
+                sStart[ww][lane] = start;
+                start += c[ww];
+            }
+        }
     }
     __syncthreads();
     // Each lane's rank among the lanes of its bucket (six ballots on the
