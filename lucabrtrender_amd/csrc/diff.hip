@@ -422,7 +422,10 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
         pairGrad.c[at] = 0.0f;
     }
 
-    float t = tFinal, behind[3] = {0.0f, 0.0f, 0.0f};
+    // t: T before the walk's current pair; behindDot: sum over the channels of
+    // dR times the colour accumulated behind it (the only use of that colour).
+    float t = tFinal, behindDot = 0.0f;
+    const float dTFinal = dT * tFinal;
     const uint64_t below = (1ull << lane) - 1ull;
     for (uint32_t top = range.x + maxLast; top > range.x;) {
         const uint32_t lo = top - range.x > kBatch ? top - kBatch : range.x;
@@ -498,26 +501,27 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
             if (ok) {
                 const float inv = __builtin_amdgcn_rcpf(1.0f - alpha);
                 const float tBefore = t * inv;
-                const float c[3] = {g2.x, g2.y, g2.z};
-                float dAlpha = -dT * tFinal * inv;
-#pragma unroll
-                for (int ch = 0; ch < 3; ++ch) dAlpha += dR[ch] * (tBefore * c[ch] - behind[ch] * inv);
+                // sum_ch dR (T_before c - behind / (1 - alpha)) - dT tFinal / (1 - alpha)
+                const float dotColour = __builtin_fmaf(dR[2], g2.z, __builtin_fmaf(dR[1], g2.y, dR[0] * g2.x));
+                const float dAlpha = __builtin_fmaf(tBefore, dotColour, -(behindDot + dTFinal) * inv);
                 const float weight = tBefore * alpha;
 #pragma unroll
-                for (int ch = 0; ch < 3; ++ch) {
-                    v[6 + ch] = dR[ch] * weight;
-                    behind[ch] += weight * c[ch];
-                }
+                for (int ch = 0; ch < 3; ++ch) v[6 + ch] = dR[ch] * weight;
+                behindDot = __builtin_fmaf(weight, dotColour, behindDot);
                 t = tBefore;
                 if (raw < 0.99f) {
-                    // alpha = opacity * exp(-power / 2)
+                    // alpha = opacity * exp(-power / 2). The power's gradient
+                    // is summed as moments over the pixels (dPower dx, dPower
+                    // dy, dPower dx^2, dPower dx dy, dPower dy^2); the pair's
+                    // conic turns them into its centre and conic terms once the
+                    // quadrants are added.
                     const float dPower = -0.5f * alpha * dAlpha;
                     v[5] = dAlpha * gauss;
-                    v[0] = -dPower * (2.0f * g1.x * dx + g1.y * dy);
-                    v[1] = -dPower * (g1.y * dx + 2.0f * g1.z * dy);
-                    v[2] = dPower * dx * dx;
-                    v[3] = dPower * 2.0f * dx * dy;
-                    v[4] = dPower * dy * dy;
+                    v[0] = dPower * dx;
+                    v[1] = dPower * dy;
+                    v[2] = v[0] * dx;
+                    v[3] = v[0] * dy;
+                    v[4] = v[1] * dy;
                 }
             }
             if (__ballot(ok) != 0) {
@@ -548,6 +552,13 @@ __device__ __forceinline__ void blendBackwardTile(const FrameParams& f, uint32_t
                     acc8 += sPart[q][tid][2].x;
                 }
             }
+            // The moments to d(centre) = -(2a Sx + 2b Sy, 2b Sx + 2c Sy) and
+            // d(conic) = (Sxx, 2 Sxy, Syy); sConic holds (a, 2b, c).
+            const float4 cn = sConic[tid];
+            const float sx = acc[0].x, sy = acc[0].y;
+            acc[0].x = -(2.0f * cn.x * sx + cn.y * sy);
+            acc[0].y = -(cn.y * sx + 2.0f * cn.z * sy);
+            acc[0].w = 2.0f * acc[0].w;
             const uint32_t at = pairIndex[top - 1u - tid];
             pairGrad.ab[2 * uint64_t{at}] = acc[0];
             pairGrad.ab[2 * uint64_t{at} + 1] = acc[1];
